@@ -1,0 +1,262 @@
+"""CPU oracle for the PADO GPU library -- TEST INFRASTRUCTURE ONLY.
+
+Wraps two checkers:
+  * liboracle.so: the C restatement in pado_oracle.c (sequential, exact
+    reference semantics; every function cites the reference file:line it
+    follows);
+  * _ref/pado_ref: the reference implementation itself, compiled from the
+    reference's headers by build_ref.sh (present only where it was built).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
+this package.  The product (parallel_distance_oracle_amd) never does.
+"""
+import ctypes
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_BIN = os.path.join(HERE, "_ref", "pado_ref")
+BP = 50
+
+_lib = None
+
+u8p = np.ctypeslib.ndpointer(np.uint8, flags="C")
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C")
+u64p = np.ctypeslib.ndpointer(np.uint64, flags="C")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        L.po_graph_num_v.restype = ctypes.c_uint32
+        L.po_graph_num_v.argtypes = [u32p, ctypes.c_uint64]
+        L.po_graph_csr.argtypes = [u32p, ctypes.c_uint64, ctypes.c_uint32, u64p, u32p]
+        L.po_make_rank.argtypes = [u64p, ctypes.c_uint32, u32p]
+        L.po_id_transfer.argtypes = [ctypes.c_uint32, u64p, u32p, u32p, u64p, u32p, u32p]
+        L.po_bfs.argtypes = [ctypes.c_uint32, u64p, u32p, ctypes.c_uint32, u32p]
+        L.po_bp_labeling.restype = ctypes.c_uint32
+        L.po_bp_labeling.argtypes = [ctypes.c_uint32, u64p, u32p, u8p, u64p, u8p, u32p]
+        L.po_labeling.restype = ctypes.c_void_p
+        L.po_labeling.argtypes = [ctypes.c_uint32, u64p, u32p, u8p, u64p, u8p, ctypes.c_uint32]
+        L.po_index_num_labels.restype = ctypes.c_uint64
+        L.po_index_num_labels.argtypes = [ctypes.c_void_p]
+        L.po_index_stats.argtypes = [ctypes.c_void_p, u64p]
+        L.po_index_export.argtypes = [ctypes.c_void_p, u64p, u32p, u8p]
+        L.po_index_free.argtypes = [ctypes.c_void_p]
+        L.po_query.restype = ctypes.c_uint32
+        L.po_query.argtypes = [ctypes.c_uint32, u8p, u64p, u64p, u32p, u8p,
+                               ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
+        L.po_query_batch.argtypes = [ctypes.c_uint32, u8p, u64p, u64p, u32p, u8p,
+                                     u32p, u32p, ctypes.c_uint64, u8p, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+# --- GR:120-206 ----------------------------------------------------------
+def graph_csr(edges):
+    """Graph(edge_list) (GR:120-153): returns (n, off[n+1] u64, adj[2m] u32)."""
+    e = _c(edges, np.uint32).reshape(-1)
+    m = e.size // 2
+    n = lib().po_graph_num_v(e, m) if m else 0
+    off = np.zeros(n + 1, np.uint64)
+    adj = np.zeros(max(2 * m, 1), np.uint32)
+    if n:
+        lib().po_graph_csr(e, m, n, off, adj)
+    return n, off, adj[: 2 * m]
+
+
+def make_rank(n, off):
+    """Graph::make_rank (GR:156-174) with glibc rand() default seed."""
+    rank = np.zeros(max(n, 1), np.uint32)
+    lib().po_make_rank(_c(off, np.uint64), n, rank)
+    return rank[:n]
+
+
+def id_transfer(n, off, adj, rank):
+    """Graph::id_transfer (GR:176-206): (off', adj', rank2id)."""
+    off2 = np.zeros(n + 1, np.uint64)
+    adj2 = np.zeros(max(adj.size, 1), np.uint32)
+    r2i = np.zeros(max(n, 1), np.uint32)
+    lib().po_id_transfer(n, _c(off, np.uint64), _c(adj, np.uint32) if adj.size else np.zeros(1, np.uint32),
+                         _c(rank, np.uint32), off2, adj2, r2i)
+    return off2, adj2[: adj.size], r2i[:n]
+
+
+def bfs(n, off, adj, src):
+    d = np.zeros(max(n, 1), np.uint32)
+    lib().po_bfs(n, _c(off, np.uint64), _c(adj, np.uint32) if adj.size else np.zeros(1, np.uint32), src, d)
+    return d[:n]
+
+
+# --- PARA:544-1836 -------------------------------------------------------
+def bp_labeling(n, off, adj):
+    """bit_parallel_labeling (PARA:544-840) -> (bp_d[n,50], bp_s[n,50,2], used[n], roots[50], max_level)."""
+    bp_d = np.zeros(max(n, 1) * BP, np.uint8)
+    bp_s = np.zeros(max(n, 1) * BP * 2, np.uint64)
+    used = np.zeros(max(n, 1), np.uint8)
+    roots = np.zeros(BP, np.uint32)
+    lvl = lib().po_bp_labeling(n, _c(off, np.uint64), _c(adj, np.uint32) if adj.size else np.zeros(1, np.uint32),
+                               bp_d, bp_s, used, roots)
+    return (bp_d[: n * BP].reshape(n, BP), bp_s[: n * BP * 2].reshape(n, BP, 2), used[:n], roots, lvl)
+
+
+def labeling(n, off, adj, bp_d, bp_s, used, batch=1024):
+    """construct (PARA:1740-1836) after BP: canonical labels (off, hub, dist) + stats."""
+    L = lib()
+    h = L.po_labeling(n, _c(off, np.uint64), _c(adj, np.uint32) if adj.size else np.zeros(1, np.uint32),
+                      _c(bp_d, np.uint8).reshape(-1), _c(bp_s, np.uint64).reshape(-1), _c(used, np.uint8), batch)
+    try:
+        tot = L.po_index_num_labels(h)
+        loff = np.zeros(n + 1, np.uint64)
+        hub = np.zeros(max(tot, 1), np.uint32)
+        dist = np.zeros(max(tot, 1), np.uint8)
+        L.po_index_export(h, loff, hub, dist)
+        st = np.zeros(7, np.uint64)
+        L.po_index_stats(h, st)
+    finally:
+        L.po_index_free(h)
+    keys = ["iters", "pushes", "ind_pass", "bp_pruned", "cands", "dq_pruned", "bytes_alg"]
+    return loff, hub[:tot], dist[:tot], dict(zip(keys, (int(x) for x in st)))
+
+
+class Index:
+    """A complete oracle index over a rank-space graph."""
+
+    def __init__(self, n, off, adj, batch=1024):
+        self.n = n
+        self.bp_d, self.bp_s, self.used, self.roots, self.max_level = bp_labeling(n, off, adj)
+        self.loff, self.hub, self.dist, self.stats = labeling(n, off, adj, self.bp_d, self.bp_s, self.used, batch)
+
+    def query(self, a, b, ref_compat=False):
+        """query_distance (PARA:2283-2416) on rank-space ids."""
+        a = _c(a, np.uint32).reshape(-1)
+        b = _c(b, np.uint32).reshape(-1)
+        out = np.zeros(max(a.size, 1), np.uint8)
+        hub = self.hub if self.hub.size else np.zeros(1, np.uint32)
+        dist = self.dist if self.dist.size else np.zeros(1, np.uint8)
+        lib().po_query_batch(self.n, self.bp_d.reshape(-1), self.bp_s.reshape(-1), self.loff, hub, dist,
+                             a, b, a.size, out, 1 if ref_compat else 0)
+        return out[: a.size]
+
+
+# --- reference harness (oracle/_ref/pado_ref) --------------------------------
+def have_ref():
+    return os.path.exists(REF_BIN)
+
+
+def _write_edges(path, edges):
+    e = _c(edges, np.uint32).reshape(-1)
+    with open(path, "wb") as f:
+        f.write(np.uint64(e.size // 2).tobytes())
+        f.write(e.tobytes())
+
+
+def _write_rank(path, rank):
+    r = _c(rank, np.uint32)
+    with open(path, "wb") as f:
+        f.write(np.uint32(r.size).tobytes())
+        f.write(r.tobytes())
+
+
+def read_labels(path):
+    with open(path, "rb") as f:
+        n = int(np.frombuffer(f.read(4), np.uint32)[0])
+        tot = int(np.frombuffer(f.read(8), np.uint64)[0])
+        off = np.frombuffer(f.read(8 * (n + 1)), np.uint64).copy()
+        hub = np.frombuffer(f.read(4 * tot), np.uint32).copy()
+        dist = np.frombuffer(f.read(tot), np.uint8).copy()
+    return off, hub, dist
+
+
+def read_bp(path):
+    with open(path, "rb") as f:
+        n = int(np.frombuffer(f.read(4), np.uint32)[0])
+        d = np.frombuffer(f.read(n * BP), np.uint8).reshape(n, BP).copy()
+        s = np.frombuffer(f.read(n * BP * 16), np.uint64).reshape(n, BP, 2).copy()
+    return d, s
+
+
+def run_ref(edges, rank=None, pairs=None, threads=None, repeat=1, dump=True):
+    """Run the reference build on `edges` (original ids) with `rank` (or its own
+    make_rank when None).  Returns dict with labels/bp (rank space), answers
+    (original-id pairs) and the timing lines it printed."""
+    if not have_ref():
+        raise FileNotFoundError(REF_BIN)
+    env = dict(os.environ)
+    if threads:
+        env["OMP_NUM_THREADS"] = str(threads)
+    with tempfile.TemporaryDirectory() as td:
+        ep = os.path.join(td, "e.bin")
+        _write_edges(ep, edges)
+        rp = "-"
+        if rank is not None:
+            rp = os.path.join(td, "r.bin")
+            _write_rank(rp, rank)
+        pp = "-"
+        if pairs is not None:
+            pp = os.path.join(td, "p.bin")
+            a = _c(pairs[0], np.uint32)
+            b = _c(pairs[1], np.uint32)
+            with open(pp, "wb") as f:
+                f.write(np.uint64(a.size).tobytes())
+                f.write(a.tobytes())
+                f.write(b.tobytes())
+        pre = os.path.join(td, "out") if dump else "-"
+        cp = subprocess.run([REF_BIN, "unw", ep, rp, pre, pp, str(repeat)], env=env,
+                            capture_output=True, text=True, check=True)
+        res = {"stdout": cp.stdout}
+        for line in cp.stdout.splitlines():
+            toks = line.split()
+            for i, t in enumerate(toks[:-1]):
+                if t.endswith(":") and t.startswith("ref_"):
+                    try:
+                        res[t[:-1]] = float(toks[i + 1])
+                    except ValueError:
+                        pass
+        if dump:
+            res["labels"] = read_labels(pre + ".labels.bin")
+            res["bp"] = read_bp(pre + ".bp.bin")
+            if pairs is not None:
+                res["answers"] = np.fromfile(pre + ".answers.bin", np.uint8)
+    return res
+
+
+def ref_rank(edges):
+    """The reference's own make_rank output for `edges` (fresh process)."""
+    with tempfile.TemporaryDirectory() as td:
+        ep = os.path.join(td, "e.bin")
+        _write_edges(ep, edges)
+        rp = os.path.join(td, "r.bin")
+        subprocess.run([REF_BIN, "rank", ep, rp], check=True, capture_output=True)
+        with open(rp, "rb") as f:
+            n = int(np.frombuffer(f.read(4), np.uint32)[0])
+            return np.frombuffer(f.read(4 * n), np.uint32).copy()
+
+
+def read_text_edges(path):
+    """Edge list text as the reference's Graph(const char*) reads it (GR:95-118):
+    lines starting with '#' or '%' skipped; first two integers per line."""
+    out = []
+    with open(path) as f:
+        for line in f:
+            if not line or line[0] in "#%":
+                continue
+            t = line.split()
+            if len(t) >= 2:
+                out.append((int(t[0]), int(t[1])))
+    return np.array(out, dtype=np.uint32).reshape(-1, 2)
