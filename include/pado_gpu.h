@@ -79,6 +79,103 @@ void pado_uniform_pairs(uint32_t n, uint64_t k, uint64_t seed, uint32_t *a, uint
 void pado_zipf_vertices(uint32_t n, double s_exp, uint64_t k, uint64_t seed, const uint32_t *rank2id,
                         uint32_t *out);
 
+/* ---------------------------------------------------------------------------
+ * Device: context, graph, index build.
+ * ------------------------------------------------------------------------ */
+
+/* Number of visible HIP devices (0 if none). */
+int pado_gpu_device_count(void);
+
+/* One context per device: a HIP stream at the default priority.  Replaces the
+ * reference's implicit OpenMP runtime (construct, PARA:1740-1895). */
+pado_gpu_status pado_gpu_ctx_create(int device, pado_gpu_ctx **out);
+void pado_gpu_ctx_destroy(pado_gpu_ctx *ctx);
+/* The context's hipStream_t, for callers that enqueue their own work. */
+void *pado_gpu_ctx_stream(pado_gpu_ctx *ctx);
+/* Wait for all work queued on the context's stream. */
+pado_gpu_status pado_gpu_ctx_sync(pado_gpu_ctx *ctx);
+
+/* Upload a rank-space CSR (the Graph after id_transfer: GR:31-92, 176-206).
+ * off: n+1 entries, adj: m_dir entries (each undirected edge twice).
+ * weights: NULL for unweighted graphs, else m_dir u8 weights (WeightedGraph,
+ * GR:223-424).  The library keeps its own copies. */
+pado_gpu_status pado_gpu_graph_upload(pado_gpu_ctx *ctx, uint32_t n, uint64_t m_dir, const uint64_t *off,
+                                      const uint32_t *adj, const uint8_t *weights, pado_gpu_graph **out);
+void pado_gpu_graph_free(pado_gpu_graph *g);
+
+typedef struct {
+    uint32_t batch_size;  /* roots per batch: 1..1024, 0 = 1024 (PARA:42) */
+    uint32_t flags;       /* reserved, 0 */
+    uint32_t reserved[6];
+} pado_gpu_build_cfg;
+
+/* ParaVertexCentricPLL<BATCH_SIZE>(const Graph&) -> construct (PARA:309-313,
+ * 1740-1895): bit-parallel labeling (PARA:544-840) then batch_process over all
+ * root batches (PARA:1413-1736).  cfg may be NULL.  Returns
+ * PADO_GPU_ERR_DOMAIN if a BFS level or batch iteration reaches 255 (the
+ * reference's u8 counters wrap there). */
+pado_gpu_status pado_gpu_build(pado_gpu_ctx *ctx, const pado_gpu_graph *graph, const pado_gpu_build_cfg *cfg,
+                               pado_gpu_index **out);
+
+typedef struct {
+    uint32_t num_v;
+    uint32_t weighted;
+    uint64_t num_labels;       /* normal labels (Label_size, PARA:2149) */
+    uint32_t ordered;          /* order_labels has run (query by original id) */
+    uint32_t bp_max_level;
+    uint64_t iterations;       /* batch iterations summed over batches */
+    double ms_bp;              /* bit-parallel labeling wall time */
+    double ms_batch;           /* batched labeling wall time (incl. final sort) */
+    uint32_t bp_roots[50];     /* slot roots (rank space), 0xFFFFFFFF if empty */
+    uint64_t pushes;           /* (tail, root) pairs offered by push_labels */
+    uint64_t ind_pass;         /* pairs that set a fresh indicator bit */
+    uint64_t bp_pruned;        /* pairs pruned by the bit-parallel check (bp_hit_count, PARA:1179) */
+    uint64_t cands;            /* candidates reaching distance_query */
+    uint64_t dq_pruned;        /* candidates pruned by distance_query */
+    uint64_t dq_rows;          /* root-distance rows scanned */
+} pado_gpu_index_info;
+
+pado_gpu_status pado_gpu_index_get_info(const pado_gpu_index *idx, pado_gpu_index_info *info);
+
+/* Canonical export in RANK space: bp_dist[n*50], bp_sets[n*50*2] (S^-1,
+ * S^0\S^-1 per slot), label_off[n+1], hubs[num_labels] ascending per vertex,
+ * dists[num_labels].  Any output may be NULL.  This is the (hub, dist) order
+ * order_labels produces (PARA:2221-2251), indexed by rank instead of id. */
+pado_gpu_status pado_gpu_index_export(pado_gpu_ctx *ctx, const pado_gpu_index *idx, uint8_t *bp_dist,
+                                      uint64_t *bp_sets, uint64_t *label_off, uint32_t *hubs, uint8_t *dists);
+/* Inverse of export (rank space); hubs must ascend per vertex. */
+pado_gpu_status pado_gpu_index_import(pado_gpu_ctx *ctx, uint32_t n, const uint8_t *bp_dist, const uint64_t *bp_sets,
+                                      const uint64_t *label_off, const uint32_t *hubs, const uint8_t *dists,
+                                      pado_gpu_index **out);
+
+/* order_labels(rank2id, rank) (PARA:2198-2281): builds the query copy indexed
+ * by ORIGINAL id; rank[x] = rank-space id of original vertex x. */
+pado_gpu_status pado_gpu_index_order(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uint32_t *rank, uint32_t n);
+
+#define PADO_GPU_QUERY_DEVICE_PTRS 1u  /* a, b, out are device pointers     */
+#define PADO_GPU_QUERY_REF_COMPAT  2u  /* reproduce the reference's unsigned
+                                          bit-parallel comparison            */
+#define PADO_GPU_QUERY_RANK_SPACE  4u  /* ids are rank-space ids            */
+
+/* query_distance(a, b) (PARA:2283-2416) for k pairs: out[i] = distance in hops,
+ * 255 if unreachable or >= 253; ids >= num_v give (a == b ? 0 : 255).  Exact
+ * (signed bit-parallel comparison) unless PADO_GPU_QUERY_REF_COMPAT.  With
+ * DEVICE_PTRS the call is asynchronous on `stream` (NULL = context stream);
+ * otherwise it stages through device buffers and returns when done. */
+pado_gpu_status pado_gpu_query(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uint32_t *a, const uint32_t *b, uint64_t k,
+                               uint8_t *out, uint32_t flags, void *stream);
+
+void pado_gpu_index_free(pado_gpu_index *idx);
+
+/* store_index_to_file / load_index_from_file in the reference's format
+ * (UNP:1120-1224; PARA:2082-2195 prints only): u32 num_v, u32 50, then per
+ * original id 50 x (u8, u64, u64), u32 size, size x (u32 hub, u8 dist). */
+pado_gpu_status pado_gpu_index_store(pado_gpu_ctx *ctx, const pado_gpu_index *idx, const char *filename,
+                                     const uint32_t *rank, uint32_t n);
+/* The loaded index is ordered (query by original id) and has no rank-space
+ * copy. */
+pado_gpu_status pado_gpu_index_load(pado_gpu_ctx *ctx, const char *filename, pado_gpu_index **out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -45,6 +45,7 @@ def lib():
         L.po_make_rank.argtypes = [u64p, ctypes.c_uint32, u32p]
         L.po_id_transfer.argtypes = [ctypes.c_uint32, u64p, u32p, u32p, u64p, u32p, u32p]
         L.po_bfs.argtypes = [ctypes.c_uint32, u64p, u32p, ctypes.c_uint32, u32p]
+        L.po_bfs_pairs.argtypes = [ctypes.c_uint32, u64p, u32p, u32p, u32p, ctypes.c_uint64, u32p]
         L.po_bp_labeling.restype = ctypes.c_uint32
         L.po_bp_labeling.argtypes = [ctypes.c_uint32, u64p, u32p, u8p, u64p, u8p, u32p]
         L.po_labeling.restype = ctypes.c_void_p
@@ -101,6 +102,24 @@ def bfs(n, off, adj, src):
     d = np.zeros(max(n, 1), np.uint32)
     lib().po_bfs(n, _c(off, np.uint64), _c(adj, np.uint32) if adj.size else np.zeros(1, np.uint32), src, d)
     return d[:n]
+
+
+def bfs_pairs(n, off, adj, a, b):
+    """Exact hop distances for pairs (uint32, 0xFFFFFFFF = unreachable)."""
+    a = _c(a, np.uint32).reshape(-1)
+    b = _c(b, np.uint32).reshape(-1)
+    out = np.zeros(max(a.size, 1), np.uint32)
+    lib().po_bfs_pairs(n, _c(off, np.uint64), _c(adj, np.uint32) if adj.size else np.zeros(1, np.uint32),
+                       a if a.size else np.zeros(1, np.uint32), b if b.size else np.zeros(1, np.uint32),
+                       a.size, out)
+    return out[: a.size]
+
+
+def bfs_to_u8(d):
+    """Map exact hop distances to the reference's u8 answer domain:
+    unreachable or >= 253 -> 255 (PARA:2412-2414)."""
+    d = np.asarray(d, np.uint64)
+    return np.where(d >= 253, 255, d).astype(np.uint8)
 
 
 # --- PARA:544-1836 -------------------------------------------------------
@@ -222,6 +241,9 @@ def run_ref(edges, rank=None, pairs=None, threads=None, repeat=1, dump=True):
         res = {"stdout": cp.stdout}
         for line in cp.stdout.splitlines():
             toks = line.split()
+            if "bp_hit_count:" in toks:
+                # printed by construct (PARA:1889); one line per build
+                res["bp_hit_count"] = int(toks[toks.index("bp_hit_count:") + 1].replace(",", ""))
             for i, t in enumerate(toks[:-1]):
                 if t.endswith(":") and t.startswith("ref_"):
                     try:
@@ -260,3 +282,8 @@ def read_text_edges(path):
             if len(t) >= 2:
                 out.append((int(t[0]), int(t[1])))
     return np.array(out, dtype=np.uint32).reshape(-1, 2)
+
+
+def set_mutation(m):
+    """Planted-mistake switch (tests only); 0 restores the faithful oracle."""
+    lib().po_set_mutation(int(m))

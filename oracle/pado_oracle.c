@@ -39,6 +39,17 @@
 #include <stdio.h>
 
 #define PO_BP 50          /* BITPARALLEL_SIZE, PARA:45 */
+
+/* Planted-mistake switch, set only by tests/test_planted.py to show that the
+ * pinning checks fail on plausible errors.  0 = faithful restatement.
+ *   1: bit-parallel neighbours scanned from the START of the adjacency
+ *   2: stored S^0 not masked with ~S^-1 (dropped term)
+ *   3: bit-parallel prune uses `<` instead of `<=`
+ *   4: indicator set AFTER the bit-parallel check instead of before
+ *   5: labels recorded with distance iter - 1 (off by one)
+ *   6: query intersects labels but skips the bit-parallel part          */
+static int po_mutation = 0;
+void po_set_mutation(int m) { po_mutation = m; }
 #define PO_INF8 255u      /* SMALLI_MAX / WEIGHTI_MAX, GLB:37-39 */
 
 /* ------------------------------------------------------------------------ */
@@ -166,6 +177,33 @@ void po_bfs(uint32_t n, const uint64_t *off, const uint32_t *adj, uint32_t src,
     free(q);
 }
 
+/* BFS answers for k (a, b) pairs: one BFS per distinct source.  out[i] = hop */
+/* distance, UINT32_MAX if unreachable.                                      */
+static const uint32_t *po_sort_key;
+static int po_idx_by_key(const void *x, const void *y)
+{
+    uint32_t a = po_sort_key[*(const uint64_t *)x], b = po_sort_key[*(const uint64_t *)y];
+    return a < b ? -1 : (a > b ? 1 : 0);
+}
+
+void po_bfs_pairs(uint32_t n, const uint64_t *off, const uint32_t *adj, const uint32_t *a,
+                  const uint32_t *b, uint64_t k, uint32_t *out)
+{
+    uint64_t *ord = (uint64_t *)malloc(sizeof(uint64_t) * (k ? k : 1));
+    for (uint64_t i = 0; i < k; ++i) ord[i] = i;
+    po_sort_key = a;
+    qsort(ord, k, sizeof(uint64_t), po_idx_by_key);
+    uint32_t *dist = (uint32_t *)malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t cur = UINT32_MAX;
+    for (uint64_t j = 0; j < k; ++j) {
+        uint64_t i = ord[j];
+        if (a[i] >= n || b[i] >= n) { out[i] = (a[i] == b[i]) ? 0 : UINT32_MAX; continue; }
+        if (a[i] != cur) { cur = a[i]; po_bfs(n, off, adj, cur, dist); }
+        out[i] = dist[b[i]];
+    }
+    free(ord); free(dist);
+}
+
 /* ------------------------------------------------------------------------ */
 /* Bit-parallel labeling: PARA:544-840 (semantics identical in its serial    */
 /* branch PARA:553-659 and parallel branch PARA:660-836).                    */
@@ -207,7 +245,7 @@ uint32_t po_bp_labeling(uint32_t n, const uint64_t *off, const uint32_t *adj,
         int ns = 0;
         uint64_t deg = off[r + 1] - off[r];
         for (uint64_t di = 0; di < deg; ++di) {
-            uint32_t v = adj[off[r] + deg - 1 - di];
+            uint32_t v = adj[off[r] + (po_mutation == 1 ? di : deg - 1 - di)];
             if (!used[v]) {
                 used[v] = 1;
                 que[qh++] = v;
@@ -247,7 +285,7 @@ uint32_t po_bp_labeling(uint32_t n, const uint64_t *off, const uint32_t *adj,
         for (uint32_t v = 0; v < n; ++v) {                     /* PARA:829-835 */
             bp_d[(uint64_t)v * PO_BP + i] = (uint8_t)tmp_d[v];
             bp_s[((uint64_t)v * PO_BP + i) * 2 + 0] = s1[v];
-            bp_s[((uint64_t)v * PO_BP + i) * 2 + 1] = s0[v] & ~s1[v];
+            bp_s[((uint64_t)v * PO_BP + i) * 2 + 1] = po_mutation == 2 ? s0[v] : (s0[v] & ~s1[v]);
         }
     }
     free(tmp_d); free(s1); free(s0); free(que); free(sib); free(chd);
@@ -295,7 +333,7 @@ static int po_bp_prunes(const uint8_t *bp_d, const uint64_t *bp_s,
         if (td - 2u <= iter) {
             if (sa[2 * i] & sb[2 * i]) td -= 2u;
             else if ((sa[2 * i] & sb[2 * i + 1]) | (sa[2 * i + 1] & sb[2 * i])) td -= 1u;
-            if (td <= iter) return 1;
+            if (po_mutation == 3 ? (td < iter) : (td <= iter)) return 1;
         }
     }
     return 0;
@@ -381,11 +419,12 @@ po_index *po_labeling(uint32_t n, const uint64_t *off, const uint32_t *adj,
                         uint64_t *w = ind + (uint64_t)vt * W + ro / 64;
                         uint64_t bit = 1ULL << (ro % 64);
                         if (*w & bit) continue;            /* PARA:1147-1153 */
+                        if (po_mutation == 4 && po_bp_prunes(bp_d, bp_s, rr, vt, iter)) { X->n_bp_pruned++; continue; }
                         *w |= bit;
                         X->n_ind_pass++;
                         if (!tail_seen[vt]) { tail_seen[vt] = 1; tail_q[n_tail++] = vt; }
                         if (!once[vt]) { once[vt] = 1; once_q[n_once++] = vt; }
-                        if (po_bp_prunes(bp_d, bp_s, rr, vt, iter)) { X->n_bp_pruned++; continue; }
+                        if (po_mutation != 4 && po_bp_prunes(bp_d, bp_s, rr, vt, iter)) { X->n_bp_pruned++; continue; }
                         /* is_candidate + candidates_que: a (tail, root) pair
                          * passes the indicator at most once per batch, so the
                          * queue never holds duplicates. PARA:1194-1198 */
@@ -428,7 +467,7 @@ po_index *po_labeling(uint32_t n, const uint64_t *off, const uint32_t *adj,
                     }
                     if (prune) { X->n_dq_pruned++; continue; }
                     if (!is_active[v]) { is_active[v] = 1; active[n_act++] = v; }
-                    po_lab_push(X, v, rr, (uint8_t)iter);  /* insert_label_only */
+                    po_lab_push(X, v, rr, (uint8_t)(po_mutation == 5 ? iter - 1 : iter));  /* insert_label_only */
                     if (v >= rs && v - rs < rsz)           /* PARA:1287-1290 */
                         dm[(uint64_t)(v - rs) * n + rr] = (uint8_t)iter;
                 }
@@ -515,7 +554,7 @@ uint32_t po_query(uint32_t n, const uint8_t *bp_d, const uint64_t *bp_s,
     uint32_t d = PO_INF8;
     const uint8_t *da = bp_d + (uint64_t)a * PO_BP, *db = bp_d + (uint64_t)b * PO_BP;
     const uint64_t *sa = bp_s + (uint64_t)a * PO_BP * 2, *sb = bp_s + (uint64_t)b * PO_BP * 2;
-    for (uint32_t i = 0; i < PO_BP; ++i) {
+    for (uint32_t i = 0; i < (po_mutation == 6 ? 0u : PO_BP); ++i) {
         int td = (int)da[i] + (int)db[i];
         int gate = ref_compat ? ((uint32_t)(td - 2) <= d) : (td - 2 <= (int)d);
         if (gate) {

@@ -1,0 +1,426 @@
+"""MI355X-native PADO: pruned landmark labeling + exact distance queries on one GPU.
+
+Python mirror of the reference's C++ API (johnpzh/parallel_distance_oracle):
+
+    G = Graph.from_edges(edges)          # Graph(vector<pair>)      includes/graph.h:120-153
+    rank = G.make_rank()                 # Graph::make_rank         includes/graph.h:156-174
+    rank2id = G.id_transfer(rank)        # Graph::id_transfer       includes/graph.h:176-206
+    pll = ParaVertexCentricPLL(G)        # construct                pado_unw_unv_para*.h:309-313
+    pll.order_labels(rank2id, rank)      # order_labels             pado_unw_unv_para*.h:2198-2281
+    d = pll.query_distance(a, b)         # query_distance           pado_unw_unv_para*.h:2283-2416
+
+Everything below the ctypes layer runs in libpado_gpu.so (HIP kernels for
+gfx950 behind a C ABI, include/pado_gpu.h).  There is no CPU fallback: if the
+library is missing or no GPU is visible, the device calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpado_gpu.so")
+
+BP_SIZE = 50
+BATCH_SIZE = 1024
+INF8 = 255
+
+QUERY_DEVICE_PTRS = 1
+QUERY_REF_COMPAT = 2
+QUERY_RANK_SPACE = 4
+
+_STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_NOMEM", 4: "ERR_DOMAIN", 5: "ERR_STATE",
+           6: "ERR_IO", 7: "ERR_BUSY"}
+
+
+class PadoError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"{_STATUS.get(status, status)}: {msg}")
+        self.status = status
+
+
+class BuildCfg(ctypes.Structure):
+    _fields_ = [("batch_size", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 6)]
+
+
+class IndexInfo(ctypes.Structure):
+    _fields_ = [
+        ("num_v", ctypes.c_uint32), ("weighted", ctypes.c_uint32), ("num_labels", ctypes.c_uint64),
+        ("ordered", ctypes.c_uint32), ("bp_max_level", ctypes.c_uint32), ("iterations", ctypes.c_uint64),
+        ("ms_bp", ctypes.c_double), ("ms_batch", ctypes.c_double), ("bp_roots", ctypes.c_uint32 * 50),
+        ("pushes", ctypes.c_uint64), ("ind_pass", ctypes.c_uint64), ("bp_pruned", ctypes.c_uint64),
+        ("cands", ctypes.c_uint64), ("dq_pruned", ctypes.c_uint64), ("dq_rows", ctypes.c_uint64),
+    ]
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+
+
+def lib():
+    """Load libpado_gpu.so (built by build.build()); raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} not built: run parallel_distance_oracle_amd.build.build()")
+    L = ctypes.CDLL(LIB_PATH)
+    sig = {
+        "pado_gpu_last_error": (ctypes.c_char_p, []),
+        "pado_graph_num_v": (_u32, [_vp, _u64]),
+        "pado_graph_csr": (None, [_vp, _u64, _u32, _vp, _vp]),
+        "pado_make_rank": (ctypes.c_int, [_vp, _u32, ctypes.c_int, _vp]),
+        "pado_id_transfer": (None, [_u32, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "pado_rmat_edges": (_u64, [_u32, _u32, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u64, _vp, _u64]),
+        "pado_grid_edges": (_u64, [_u32, ctypes.c_double, _u64, _vp, _u64]),
+        "pado_edge_weights": (None, [_u64, _u32, _u32, _u64, _vp]),
+        "pado_uniform_pairs": (None, [_u32, _u64, _u64, _vp, _vp]),
+        "pado_zipf_vertices": (None, [_u32, ctypes.c_double, _u64, _u64, _vp, _vp]),
+        "pado_gpu_device_count": (ctypes.c_int, []),
+        "pado_gpu_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_vp)]),
+        "pado_gpu_ctx_destroy": (None, [_vp]),
+        "pado_gpu_ctx_stream": (_vp, [_vp]),
+        "pado_gpu_ctx_sync": (ctypes.c_int, [_vp]),
+        "pado_gpu_graph_upload": (ctypes.c_int, [_vp, _u32, _u64, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
+        "pado_gpu_graph_free": (None, [_vp]),
+        "pado_gpu_build": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(BuildCfg), ctypes.POINTER(_vp)]),
+        "pado_gpu_index_get_info": (ctypes.c_int, [_vp, ctypes.POINTER(IndexInfo)]),
+        "pado_gpu_index_export": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "pado_gpu_index_import": (ctypes.c_int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
+        "pado_gpu_index_order": (ctypes.c_int, [_vp, _vp, _vp, _u32]),
+        "pado_gpu_query": (ctypes.c_int, [_vp, _vp, _vp, _vp, _u64, _vp, _u32, _vp]),
+        "pado_gpu_index_free": (None, [_vp]),
+        "pado_gpu_index_store": (ctypes.c_int, [_vp, _vp, ctypes.c_char_p, _vp, _u32]),
+        "pado_gpu_index_load": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_vp)]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    """Names the C ABI declares (include/pado_gpu.h) -- used by tests."""
+    import re
+    hdr = os.path.join(os.path.dirname(HERE), "include", "pado_gpu.h")
+    txt = open(hdr).read()
+    return sorted(set(re.findall(r"\b(pado_[a-z0-9_]+)\s*\(", txt)))
+
+
+def _check(status):
+    if status != 0:
+        raise PadoError(status, lib().pado_gpu_last_error().decode())
+
+
+def _p(a):
+    return a.ctypes.data_as(_vp) if a is not None and a.size else None
+
+
+def _c(a, dt):
+    return np.ascontiguousarray(a, dtype=dt)
+
+
+# ---------------------------------------------------------------------------
+# Workload generators (deterministic; seed-pinned splitmix64)
+# ---------------------------------------------------------------------------
+def rmat_edges(scale, edge_factor, a=0.57, b=0.19, c=0.19, seed=1):
+    """Undirected R-MAT edges (u < v), self-loops and duplicates removed."""
+    L = lib()
+    m = L.pado_rmat_edges(scale, edge_factor, a, b, c, seed, None, 0)
+    out = np.zeros(2 * max(m, 1), np.uint32)
+    L.pado_rmat_edges(scale, edge_factor, a, b, c, seed, _p(out), m)
+    return out[: 2 * m].reshape(-1, 2)
+
+
+def grid_edges(side, delete_frac=0.05, seed=1):
+    L = lib()
+    m = L.pado_grid_edges(side, delete_frac, seed, None, 0)
+    out = np.zeros(2 * max(m, 1), np.uint32)
+    L.pado_grid_edges(side, delete_frac, seed, _p(out), m)
+    return out[: 2 * m].reshape(-1, 2)
+
+
+def edge_weights(m, lo=1, hi=100, seed=1):
+    w = np.zeros(max(m, 1), np.uint32)
+    lib().pado_edge_weights(m, lo, hi, seed, _p(w))
+    return w[:m]
+
+
+def uniform_pairs(n, k, seed=1):
+    a = np.zeros(max(k, 1), np.uint32)
+    b = np.zeros(max(k, 1), np.uint32)
+    lib().pado_uniform_pairs(n, k, seed, _p(a), _p(b))
+    return a[:k], b[:k]
+
+
+def zipf_vertices(n, k, s=1.1, seed=1, rank2id=None):
+    out = np.zeros(max(k, 1), np.uint32)
+    r2i = _c(rank2id, np.uint32) if rank2id is not None else None
+    lib().pado_zipf_vertices(n, s, k, seed, _p(r2i), _p(out))
+    return out[:k]
+
+
+# ---------------------------------------------------------------------------
+# Graph: the reference's Graph class (includes/graph.h:31-92)
+# ---------------------------------------------------------------------------
+class Graph:
+    """Undirected CSR with the reference's semantics.
+
+    num_v = max id + 1, both directions per input edge, no de-duplication
+    (includes/graph.h:125-153).  make_rank / id_transfer renumber it into rank
+    space with descending adjacency lists (includes/graph.h:156-206)."""
+
+    def __init__(self, num_v, off, adj):
+        self.num_v = int(num_v)
+        self.off = _c(off, np.uint64)
+        self.adj = _c(adj, np.uint32)
+
+    @property
+    def num_e(self):
+        return int(self.adj.size)
+
+    @property
+    def out_degrees(self):
+        return np.diff(self.off).astype(np.uint32)
+
+    @classmethod
+    def from_edges(cls, edges):
+        e = _c(edges, np.uint32).reshape(-1)
+        m = e.size // 2
+        L = lib()
+        n = L.pado_graph_num_v(_p(e), m) if m else 0
+        off = np.zeros(n + 1, np.uint64)
+        adj = np.zeros(max(2 * m, 1), np.uint32)
+        if n:
+            L.pado_graph_csr(_p(e), m, n, _p(off), _p(adj))
+        return cls(n, off, adj[: 2 * m])
+
+    @classmethod
+    def from_file(cls, path):
+        """Graph(const char*) (includes/graph.h:95-118): '#'/'%' lines skipped."""
+        rows = []
+        with open(path) as f:
+            for line in f:
+                if not line or line[0] in "#%":
+                    continue
+                t = line.split()
+                if len(t) >= 2:
+                    rows.append((int(t[0]), int(t[1])))
+        return cls.from_edges(np.array(rows, dtype=np.uint32).reshape(-1, 2))
+
+    def make_rank(self, use_global_rand=False):
+        rank = np.zeros(max(self.num_v, 1), np.uint32)
+        _check(0 if lib().pado_make_rank(_p(self.off), self.num_v, 1 if use_global_rand else 0, _p(rank)) == 0
+               else 1)
+        return rank[: self.num_v]
+
+    def id_transfer(self, rank):
+        """Renumber in place into rank space; returns rank2id."""
+        rank = _c(rank, np.uint32)
+        n = self.num_v
+        off2 = np.zeros(n + 1, np.uint64)
+        adj2 = np.zeros(max(self.adj.size, 1), np.uint32)
+        r2i = np.zeros(max(n, 1), np.uint32)
+        lib().pado_id_transfer(n, _p(self.off), _p(self.adj) if self.adj.size else _p(adj2), _p(rank),
+                               _p(off2), _p(adj2), _p(r2i))
+        self.off, self.adj = off2, adj2[: self.adj.size]
+        return r2i[:n]
+
+
+# ---------------------------------------------------------------------------
+# Device objects
+# ---------------------------------------------------------------------------
+def device_count():
+    return lib().pado_gpu_device_count()
+
+
+class Context:
+    def __init__(self, device=0):
+        h = _vp()
+        _check(lib().pado_gpu_ctx_create(device, ctypes.byref(h)))
+        self.h = h
+        self.device = device
+
+    @property
+    def stream(self):
+        return lib().pado_gpu_ctx_stream(self.h)
+
+    def sync(self):
+        _check(lib().pado_gpu_ctx_sync(self.h))
+
+    def close(self):
+        if self.h:
+            lib().pado_gpu_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+class DeviceGraph:
+    def __init__(self, ctx, G, weights=None):
+        self.ctx = ctx
+        h = _vp()
+        w = _c(weights, np.uint8) if weights is not None else None
+        _check(lib().pado_gpu_graph_upload(ctx.h, G.num_v, G.num_e, _p(G.off), _p(G.adj) if G.adj.size else None,
+                                           _p(w), ctypes.byref(h)))
+        self.h = h
+        self.num_v = G.num_v
+
+    def close(self):
+        if self.h:
+            lib().pado_gpu_graph_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Index:
+    """A device-resident PADO index (rank space; plus an original-id copy
+    after order())."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.h = handle
+        self.info = self.get_info()
+        self.num_v = self.info.num_v
+
+    @classmethod
+    def build(cls, ctx, dgraph, batch_size=BATCH_SIZE):
+        cfg = BuildCfg()
+        cfg.batch_size = batch_size
+        h = _vp()
+        _check(lib().pado_gpu_build(ctx.h, dgraph.h, ctypes.byref(cfg), ctypes.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def import_(cls, ctx, n, bp_d, bp_s, loff, hub, dist):
+        h = _vp()
+        bp_d = _c(bp_d, np.uint8)
+        bp_s = _c(bp_s, np.uint64)
+        loff = _c(loff, np.uint64)
+        hub = _c(hub, np.uint32)
+        dist = _c(dist, np.uint8)
+        _check(lib().pado_gpu_index_import(ctx.h, n, _p(bp_d), _p(bp_s), _p(loff), _p(hub), _p(dist),
+                                           ctypes.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def load(cls, ctx, filename):
+        h = _vp()
+        _check(lib().pado_gpu_index_load(ctx.h, filename.encode(), ctypes.byref(h)))
+        return cls(ctx, h)
+
+    def get_info(self):
+        info = IndexInfo()
+        _check(lib().pado_gpu_index_get_info(self.h, ctypes.byref(info)))
+        return info
+
+    def stats(self):
+        i = self.get_info()
+        return {k: getattr(i, k) for k in ("num_labels", "iterations", "ms_bp", "ms_batch", "bp_max_level",
+                                            "pushes", "ind_pass", "bp_pruned", "cands", "dq_pruned", "dq_rows")}
+
+    def export(self):
+        """Canonical rank-space arrays: (bp_d[n,50], bp_s[n,50,2], loff[n+1], hub, dist)."""
+        n = self.num_v
+        tot = self.info.num_labels
+        bp_d = np.zeros(n * BP_SIZE, np.uint8)
+        bp_s = np.zeros(n * BP_SIZE * 2, np.uint64)
+        loff = np.zeros(n + 1, np.uint64)
+        hub = np.zeros(max(tot, 1), np.uint32)
+        dist = np.zeros(max(tot, 1), np.uint8)
+        _check(lib().pado_gpu_index_export(self.ctx.h, self.h, _p(bp_d), _p(bp_s), _p(loff), _p(hub), _p(dist)))
+        return bp_d.reshape(n, BP_SIZE), bp_s.reshape(n, BP_SIZE, 2), loff, hub[:tot], dist[:tot]
+
+    def order(self, rank):
+        rank = _c(rank, np.uint32)
+        _check(lib().pado_gpu_index_order(self.ctx.h, self.h, _p(rank), rank.size))
+        self.info = self.get_info()
+
+    def query(self, a, b, ref_compat=False, rank_space=False):
+        a = _c(a, np.uint32).reshape(-1)
+        b = _c(b, np.uint32).reshape(-1)
+        out = np.zeros(max(a.size, 1), np.uint8)
+        flags = (QUERY_REF_COMPAT if ref_compat else 0) | (QUERY_RANK_SPACE if rank_space else 0)
+        _check(lib().pado_gpu_query(self.ctx.h, self.h, _p(a), _p(b), a.size, _p(out), flags, None))
+        return out[: a.size]
+
+    def query_device(self, a_ptr, b_ptr, k, out_ptr, ref_compat=False, rank_space=False, stream=None):
+        """Asynchronous query on device buffers (e.g. torch tensors' data_ptr())."""
+        flags = QUERY_DEVICE_PTRS | (QUERY_REF_COMPAT if ref_compat else 0) | (QUERY_RANK_SPACE if rank_space else 0)
+        _check(lib().pado_gpu_query(self.ctx.h, self.h, _vp(a_ptr), _vp(b_ptr), k, _vp(out_ptr), flags,
+                                    _vp(stream) if stream else None))
+
+    def store(self, filename, rank):
+        rank = _c(rank, np.uint32)
+        _check(lib().pado_gpu_index_store(self.ctx.h, self.h, filename.encode(), _p(rank), rank.size))
+
+    def close(self):
+        if self.h:
+            lib().pado_gpu_index_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ParaVertexCentricPLL:
+    """ParaVertexCentricPLL<BATCH_SIZE> (pado_unw_unv_para*.h:42-304) on the GPU.
+
+    Constructing from a rank-space Graph builds the index (PARA:309-313);
+    order_labels / query_distance / store_index_to_file / load_index_from_file
+    keep the reference's meaning."""
+
+    def __init__(self, G=None, batch_size=BATCH_SIZE, ctx=None):
+        self.ctx = ctx or default_context()
+        self.index = None
+        self.BATCH_SIZE = batch_size
+        if G is not None:
+            dg = DeviceGraph(self.ctx, G)
+            try:
+                self.index = Index.build(self.ctx, dg, batch_size)
+            finally:
+                dg.close()
+
+    def order_labels(self, rank2id, rank):
+        self.index.order(rank)
+
+    def query_distance(self, a, b):
+        """Scalar (reference signature) or array arguments."""
+        scalar = np.isscalar(a) and np.isscalar(b)
+        out = self.index.query(np.atleast_1d(a), np.atleast_1d(b))
+        return int(out[0]) if scalar else out
+
+    def store_index_to_file(self, filename, rank):
+        self.index.store(filename, rank)
+
+    def load_index_from_file(self, filename):
+        self.index = Index.load(self.ctx, filename)
+
+
+VertexCentricPLL = ParaVertexCentricPLL

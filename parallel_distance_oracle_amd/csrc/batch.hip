@@ -1,0 +1,697 @@
+// batch.hip -- batched pruned labeling (the reference's batch_process) on gfx950.
+//
+// Reference (includes/pado_unw_unv_para.201912311607.parallel_bp_labeling.h):
+//   initialize            :848-1080   roots -> active, self labels, dist_matrix
+//   push_labels           :1084-1211  indicator test-and-set, BP check, candidates
+//   distance_query        :1218-1270  2-hop check against labels of earlier iterations
+//   insert_label_only     :1276-1291  append, root rows of dist_matrix
+//   update_label_indices  :1295-1322
+//   reset_at_end          :1328-1410
+//   batch_process         :1413-1736
+//   construct             :1740-1836  batches of BATCH_SIZE roots in rank order
+//
+// Semantics kept exactly: a (tail, root) pair is offered once per batch (the
+// indicator is set BEFORE the BP check, PARA:1147-1153); the BP check uses the
+// reference's unsigned `td - 2 <= iter` (slots with td < 2 never prune,
+// PARA:1169-1170); the distance query sees v's labels with dist < iter and
+// the batch roots' labels, hubs h < root only (PARA:1245-1260); pushes come
+// from the last distance group only and stop at tails < roots_start.  Every
+// decision is a set operation, so the label sets are independent of the
+// schedule (the reference's own OpenMP order is arbitrary too).
+//
+// GPU layout (HBM):
+//   ind[n][16] u64   indicator bitmap, 1024 roots per vertex row (rows of the
+//                    vertices touched in a batch are cleared at its end)
+//   cand[n][16] u64  candidate bitmap of the current iteration
+//   ag[A][16] u64    last distance group of each active vertex, as a bitmap
+//   chunks[]         16-edge push work descriptors (active index, edge start)
+//   pool[] u32       label store: per-vertex growable arrays of packed
+//                    (hub << 8 | dist) labels, doubled on overflow by a bump
+//                    allocator (one wave owns one vertex's array)
+//   hinfo[n] u32     per-hub (slot << 8 | min distance from any batch root)
+//   dc[slot][1024]   root-distance table: dc[slot(h)][r] = dist(root r, hub h),
+//                    the transposed, hub-compacted form of dist_matrix
+// Kernels per iteration: k_push (edge-parallel, quarter-wave per 16 edges) and
+// k_add (wave per candidate vertex: distance query + label append + next
+// active entry).  The host loop syncs once per iteration on the active count.
+#include "internal.hpp"
+
+namespace pado {
+namespace {
+
+constexpr uint32_t kClaim = 0xFFFFFFFEu;
+
+__device__ inline uint32_t lane_id() { return threadIdx.x & 63; }
+
+__device__ inline uint32_t wave_sum_u32(uint32_t x)
+{
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+__device__ inline unsigned long long wave_sum_u64(unsigned long long x)
+{
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+// Number of neighbours of v with id >= rs: the adjacency is sorted
+// descending, so they form a prefix (PARA:1121-1123 returns at the first tail
+// below roots_start).  Wave-cooperative 64-ary search.
+__device__ inline uint32_t push_degree(const uint64_t *off, const uint32_t *adj, uint32_t v, uint32_t rs)
+{
+    const uint32_t lane = lane_id();
+    const uint64_t base = off[v];
+    uint32_t lo = 0, hi = (uint32_t)(off[v + 1] - base);
+    while (hi - lo > 64) {
+        uint32_t step = (hi - lo + 63) / 64;
+        uint32_t pos = lo + lane * step;
+        bool p = pos < hi && adj[base + pos] >= rs;
+        uint32_t c = (uint32_t)__popcll(__ballot(p));
+        if (c == 0) {
+            hi = lo;
+        } else {
+            uint32_t nlo = lo + (c - 1) * step + 1;
+            uint32_t nhi = lo + c * step;
+            lo = nlo;
+            if (nhi < hi) hi = nhi;
+        }
+    }
+    uint32_t pos = lo + lane;
+    bool p = pos < hi && adj[base + pos] >= rs;
+    return lo + (uint32_t)__popcll(__ballot(p));
+}
+
+// Bit-parallel check of PARA:1164-1186 with the reference's unsigned
+// arithmetic; true = pruned.
+__device__ inline bool bp_prunes(const uint32_t (&dr)[13], const uint32_t (&dt)[13], const ulonglong2 *bps,
+                                 uint32_t r, uint32_t t, uint32_t iter)
+{
+#pragma unroll
+    for (int i = 0; i < (int)kBP; ++i) {
+        uint32_t td = ((dr[i >> 2] >> ((i & 3) * 8)) & 255u) + ((dt[i >> 2] >> ((i & 3) * 8)) & 255u);
+        if (td - 2u <= iter) {
+            ulonglong2 sr = bps[(uint64_t)r * kBP + i];
+            ulonglong2 st = bps[(uint64_t)t * kBP + i];
+            if (sr.x & st.x)
+                td -= 2u;
+            else if ((sr.x & st.y) | (sr.y & st.x))
+                td -= 1u;
+            if (td <= iter) return true;
+        }
+    }
+    return false;
+}
+
+__device__ inline void load_dline(const uint8_t *bpd, uint32_t v, uint32_t (&d)[13])
+{
+    const uint4 *p = reinterpret_cast<const uint4 *>(bpd + (uint64_t)v * kBpdStride);
+    uint4 a = p[0], b = p[1], c = p[2], e = p[3];
+    d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w;
+    d[4] = b.x; d[5] = b.y; d[6] = b.z; d[7] = b.w;
+    d[8] = c.x; d[9] = c.y; d[10] = c.z; d[11] = c.w;
+    d[12] = e.x;
+}
+
+struct LabelStore {
+    uint32_t *pool;
+    unsigned long long *base;
+    uint32_t *len;
+    uint32_t *cap;
+    unsigned long long pool_size;
+};
+
+// Reserve room for `add` more labels of v (wave-uniform call).  Returns the
+// (possibly new) base; on pool exhaustion sets the overflow flag and returns
+// ~0ull.
+__device__ inline unsigned long long store_reserve(LabelStore S, BuildCounters *C, uint32_t v, uint32_t len,
+                                                   uint32_t add)
+{
+    const uint32_t lane = lane_id();
+    unsigned long long base = S.base[v];
+    uint32_t cap = S.cap[v];
+    if (len + add <= cap) return base;
+    uint32_t ncap = cap * 2;
+    if (ncap < len + add) ncap = len + add;
+    if (ncap < 8) ncap = 8;
+    unsigned long long nb = 0;
+    if (lane == 0) nb = atomicAdd(&C->pool_top, (unsigned long long)ncap);
+    nb = __shfl(nb, 0);
+    if (nb + ncap > S.pool_size) {
+        if (lane == 0) atomicOr(&C->overflow, 1u);
+        return ~0ull;
+    }
+    for (uint32_t j = lane; j < len; j += 64) S.pool[nb + j] = S.pool[base + j];
+    if (lane == 0) {
+        S.base[v] = nb;
+        S.cap[v] = ncap;
+    }
+    return nb;
+}
+
+// Append the pushes of active index k: chunk descriptors over its first
+// `pdeg` neighbours (wave-uniform call).
+__device__ inline void emit_push_chunks(uint32_t k, uint32_t pdeg, BuildCounters *C, unsigned long long *chunks)
+{
+    const uint32_t lane = lane_id();
+    uint32_t nch = (pdeg + kChunk - 1) / kChunk;
+    if (nch == 0) return;
+    uint32_t c0 = 0;
+    if (lane == 0) c0 = atomicAdd(&C->n_chunk, nch);
+    c0 = __shfl(c0, 0);
+    for (uint32_t q = lane; q < nch; q += 64) chunks[c0 + q] = ((unsigned long long)(q * kChunk) << 32) | k;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint64_t count, uint32_t val)
+{
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = val;
+}
+
+__global__ void k_zero_u64(unsigned long long *p, uint64_t count)
+{
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = 0ull;
+}
+
+// initialize (PARA:848-1080): non-BP roots become active with their own bit as
+// last group, get the self label (r, 0), and set their own indicator bit.
+__global__ __launch_bounds__(256) void k_batch_init(uint32_t rs, uint32_t rsz, uint32_t gen_batch,
+                                                    const uint8_t *used, const uint64_t *off, const uint32_t *adj,
+                                                    LabelStore S, unsigned long long *ind, uint32_t *tmark,
+                                                    uint32_t *tlist, uint32_t *alist, unsigned long long *ag,
+                                                    uint32_t *amask, uint32_t *aend, unsigned long long *chunks,
+                                                    BuildCounters *C)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t ro = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (ro >= rsz) return;
+    const uint32_t r = rs + ro;
+    if (used[r]) return;
+    uint32_t len = S.len[r];
+    unsigned long long base = store_reserve(S, C, r, len, 1);
+    if (base == ~0ull) return;
+    if (lane == 0) {
+        S.pool[base + len] = r << 8;  // (hub r, dist 0), PARA:932-949
+        S.len[r] = len + 1;
+        ind[(uint64_t)r * kWords + (ro >> 6)] |= 1ull << (ro & 63);
+        if (atomicExch(&tmark[r], gen_batch) != gen_batch) tlist[atomicAdd(&C->n_touched, 1u)] = r;
+        atomicAdd(&C->labels, 1ull);
+    }
+    uint32_t k = 0;
+    if (lane == 0) k = atomicAdd(&C->n_act, 1u);
+    k = __shfl(k, 0);
+    if (lane < kWords) ag[(uint64_t)k * kWords + lane] = (lane == (ro >> 6)) ? (1ull << (ro & 63)) : 0ull;
+    uint32_t pdeg = push_degree(off, adj, r, rs);
+    if (lane == 0) {
+        alist[k] = r;
+        amask[k] = 1u << (ro >> 6);
+        aend[k] = pdeg;
+    }
+    emit_push_chunks(k, pdeg, C, chunks);
+}
+
+// Distinct hubs over the labels of the batch roots (for the root-distance table).
+__global__ __launch_bounds__(256) void k_hub_mark(uint32_t rs, uint32_t rsz, const uint8_t *used, LabelStore S,
+                                                  uint32_t *hinfo, uint32_t *hlist, BuildCounters *C)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t ro = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (ro >= rsz) return;
+    const uint32_t r = rs + ro;
+    if (used[r]) return;
+    const uint32_t len = S.len[r];
+    const unsigned long long base = S.base[r];
+    for (uint32_t j = lane; j < len; j += 64) {
+        uint32_t h = S.pool[base + j] >> 8;
+        if (atomicCAS(&hinfo[h], kEmpty, kClaim) == kEmpty) hlist[atomicAdd(&C->n_hub, 1u)] = h;
+    }
+}
+
+// Slot assignment: hinfo[h] = slot << 8 | 255, table row = all INF.
+__global__ __launch_bounds__(256) void k_hub_slots(const uint32_t *hlist, uint32_t *hinfo, uint8_t *dc,
+                                                   const BuildCounters *C)
+{
+    const uint32_t nh = C->n_hub;
+    const uint64_t total = (uint64_t)nh * (kBatch / 16);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        uint32_t slot = (uint32_t)(i / (kBatch / 16));
+        uint32_t part = (uint32_t)(i % (kBatch / 16));
+        reinterpret_cast<uint4 *>(dc + (uint64_t)slot * kBatch)[part] = make_uint4(~0u, ~0u, ~0u, ~0u);
+        if (part == 0) hinfo[hlist[slot]] = (slot << 8) | kInf8;
+    }
+}
+
+// dist_matrix scatter (PARA:986-1013), transposed: dc[slot(h)][r] = d.
+__global__ __launch_bounds__(256) void k_hub_fill(uint32_t rs, uint32_t rsz, const uint8_t *used, LabelStore S,
+                                                  uint32_t *hinfo, uint8_t *dc)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t ro = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (ro >= rsz) return;
+    const uint32_t r = rs + ro;
+    if (used[r]) return;
+    const uint32_t len = S.len[r];
+    const unsigned long long base = S.base[r];
+    for (uint32_t j = lane; j < len; j += 64) {
+        uint32_t lab = S.pool[base + j];
+        uint32_t h = lab >> 8, d = lab & 255u;
+        uint32_t slot = hinfo[h] >> 8;
+        dc[(uint64_t)slot * kBatch + ro] = (uint8_t)d;
+        atomicMin(&hinfo[h], (slot << 8) | d);
+    }
+}
+
+// push (PARA:1486-1559 / push_labels PARA:1084-1211), one quarter-wave per
+// 16-edge chunk of an active vertex's tails >= roots_start.
+__global__ __launch_bounds__(256) void k_push(uint32_t iter, uint32_t rs, uint32_t gen_batch, uint32_t gen_iter,
+                                              const uint64_t *off, const uint32_t *adj, const uint32_t *alist,
+                                              const unsigned long long *ag, const uint32_t *amask,
+                                              const uint32_t *aend, const unsigned long long *chunks,
+                                              unsigned long long *ind, unsigned long long *cand, uint32_t *tmark,
+                                              uint32_t *tlist, uint32_t *cmark, uint32_t *clist,
+                                              const uint8_t *bpd, const ulonglong2 *bps, BuildCounters *C)
+{
+    const uint32_t lane = threadIdx.x & (kChunk - 1);
+    const uint32_t ngroups = (gridDim.x * blockDim.x) / kChunk;
+    const uint32_t nc = C->n_chunk;
+    unsigned long long n_push = 0, n_pass = 0, n_bp = 0, n_cand = 0;
+    for (uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) / kChunk; c < nc; c += ngroups) {
+        const unsigned long long desc = chunks[c];
+        const uint32_t k = (uint32_t)desc;
+        const uint32_t e = (uint32_t)(desc >> 32) + lane;
+        if (e >= aend[k]) continue;
+        const uint32_t v = alist[k];
+        const uint32_t t = adj[off[v] + e];
+        const uint32_t lim = t - rs;  // roots with offset < lim satisfy root < tail (PARA:1134)
+        bool touched = false, got = false, tline = false;
+        uint32_t dt[13];
+        for (uint32_t wm = amask[k]; wm; wm &= wm - 1) {
+            const uint32_t w = __ffs(wm) - 1;
+            if (lim <= w * 64) break;
+            unsigned long long g = ag[(uint64_t)k * kWords + w];
+            if (lim < (w + 1) * 64) g &= (1ull << (lim - w * 64)) - 1ull;
+            if (!g) continue;
+            n_push += __popcll(g);
+            const unsigned long long old = atomicOr(&ind[(uint64_t)t * kWords + w], g);
+            unsigned long long nb = g & ~old;
+            if (!nb) continue;
+            touched = true;
+            n_pass += __popcll(nb);
+            if (!tline) {
+                load_dline(bpd, t, dt);
+                tline = true;
+            }
+            unsigned long long pass = 0;
+            while (nb) {
+                const uint32_t b = __ffsll(nb) - 1;
+                nb &= nb - 1;
+                const uint32_t r = rs + w * 64 + b;
+                uint32_t dr[13];
+                load_dline(bpd, r, dr);
+                if (bp_prunes(dr, dt, bps, r, t, iter))
+                    ++n_bp;
+                else
+                    pass |= 1ull << b;
+            }
+            if (pass) {
+                atomicOr(&cand[(uint64_t)t * kWords + w], pass);
+                n_cand += __popcll(pass);
+                got = true;
+            }
+        }
+        if (touched && tmark[t] != gen_batch) {
+            if (atomicExch(&tmark[t], gen_batch) != gen_batch) tlist[atomicAdd(&C->n_touched, 1u)] = t;
+        }
+        if (got && cmark[t] != gen_iter) {
+            if (atomicExch(&cmark[t], gen_iter) != gen_iter) clist[atomicAdd(&C->n_cand, 1u)] = t;
+        }
+    }
+    n_push = wave_sum_u64(n_push);
+    n_pass = wave_sum_u64(n_pass);
+    n_bp = wave_sum_u64(n_bp);
+    n_cand = wave_sum_u64(n_cand);
+    if ((threadIdx.x & 63) == 0) {
+        if (n_push) atomicAdd(&C->pushes, n_push);
+        if (n_pass) atomicAdd(&C->ind_pass, n_pass);
+        if (n_bp) atomicAdd(&C->bp_pruned, n_bp);
+        if (n_cand) atomicAdd(&C->cands, n_cand);
+    }
+}
+
+__global__ void k_mid(BuildCounters *C)
+{
+    C->n_act = 0;
+    C->n_chunk = 0;
+}
+
+__global__ void k_post(BuildCounters *C) { C->n_cand = 0; }
+
+__global__ void k_batch_counters_reset(BuildCounters *C)
+{
+    C->n_act = 0;
+    C->n_chunk = 0;
+    C->n_cand = 0;
+    C->n_touched = 0;
+    C->n_hub = 0;
+}
+
+// add (PARA:1568-1665): one wave per candidate vertex v.  Lane l owns roots
+// [16l, 16l+16).  distance_query (PARA:1218-1270) runs over v's labels with
+// dist < iter: a hub h can prune only if d_v(h) + min_r d_r(h) <= iter, so a
+// per-hub minimum filters most hubs before the 1 KB root-distance row of the
+// survivors is read (16 B per lane).
+__global__ __launch_bounds__(256) void k_add(uint32_t iter, uint32_t rs, uint32_t rsz, const uint8_t *used,
+                                             const uint64_t *off, const uint32_t *adj, const uint32_t *clist,
+                                             unsigned long long *cand, LabelStore S, uint32_t *hinfo, uint8_t *dc,
+                                             uint32_t *alist, unsigned long long *ag, uint32_t *amask,
+                                             uint32_t *aend, unsigned long long *chunks, BuildCounters *C)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t ncand = C->n_cand;
+    unsigned long long n_dq = 0, n_lab = 0, n_rows = 0;
+    for (uint32_t i = wave; i < ncand; i += nwaves) {
+        const uint32_t v = clist[i];
+        const unsigned long long word = cand[(uint64_t)v * kWords + (lane >> 2)];
+        const uint32_t cb = (uint32_t)(word >> (16 * (lane & 3))) & 0xFFFFu;
+        // The clear below is control-dependent on every lane's loaded word
+        // (through the ballot), so no lane can clear a word before all lanes
+        // have read it.
+        if (__ballot(cb != 0) != 0ull && lane < kWords) cand[(uint64_t)v * kWords + lane] = 0ull;
+        uint32_t prune = 0;
+        const uint32_t len = S.len[v];
+        const unsigned long long base = S.base[v];
+        const uint32_t my_lo = rs + 16 * lane;  // real id of this lane's first root
+        for (uint32_t j0 = 0; j0 < len; j0 += 64) {
+            const uint32_t j = j0 + lane;
+            uint32_t h = 0, d = 255, info = kEmpty;
+            if (j < len) {
+                const uint32_t lab = S.pool[base + j];
+                h = lab >> 8;
+                d = lab & 255u;
+                if (d < iter) info = hinfo[h];
+            }
+            const bool pass = info != kEmpty && d + (info & 255u) <= iter;
+            unsigned long long m = __ballot(pass);
+            while (m) {
+                const int src = __ffsll(m) - 1;
+                m &= m - 1;
+                const uint32_t hh = __shfl(h, src);
+                const uint32_t dd = __shfl(d, src);
+                const uint32_t slot = __shfl(info, src) >> 8;
+                ++n_rows;
+                const uint4 row = reinterpret_cast<const uint4 *>(dc + (uint64_t)slot * kBatch)[lane];
+                const uint32_t thr = iter - dd;  // prune root r if d_r(h) <= iter - d_v(h)
+                const uint32_t wds[4] = {row.x, row.y, row.z, row.w};
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const uint32_t x = (wds[q >> 2] >> ((q & 3) * 8)) & 255u;
+                    if (x <= thr && hh < my_lo + q) prune |= 1u << q;
+                }
+            }
+            if (__all((cb & ~prune) == 0)) break;
+        }
+        const uint32_t surv = cb & ~prune;
+        n_dq += __popc(cb & prune);
+        const uint32_t cnt = __popc(surv);
+        uint32_t incl = cnt;
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t y = __shfl_up(incl, o);
+            if (lane >= (uint32_t)o) incl += y;
+        }
+        const uint32_t total = __shfl(incl, 63);
+        if (total == 0) continue;
+        n_lab += total;
+        const unsigned long long nb = store_reserve(S, C, v, len, total);
+        if (nb == ~0ull) continue;
+        uint32_t pos = len + incl - cnt;
+        for (uint32_t sb = surv; sb; sb &= sb - 1) {
+            const uint32_t q = __ffs(sb) - 1;
+            S.pool[nb + pos++] = ((my_lo + q) << 8) | iter;  // insert_label_only, PARA:1285
+        }
+        if (lane == 0) S.len[v] = len + total;
+        // Roots of this batch also record the new label in the root table
+        // (dist_matrix[v_root][cand_real] = iter, PARA:1287-1290).
+        if (v >= rs && v < rs + rsz) {
+            for (uint32_t sb = surv; sb; sb &= sb - 1) {
+                const uint32_t h = my_lo + __ffs(sb) - 1;
+                const uint32_t info = hinfo[h];
+                if (info == kEmpty || info == kClaim) continue;
+                const uint32_t slot = info >> 8;
+                dc[(uint64_t)slot * kBatch + (v - rs)] = (uint8_t)iter;
+                atomicMin(&hinfo[h], (slot << 8) | iter);
+            }
+        }
+        // v becomes active with the new distance group as its last group.
+        uint32_t k = 0;
+        if (lane == 0) k = atomicAdd(&C->n_act, 1u);
+        k = __shfl(k, 0);
+        reinterpret_cast<uint16_t *>(ag + (uint64_t)k * kWords)[lane] = (uint16_t)surv;
+        const unsigned long long nzm = __ballot(surv != 0);
+        uint32_t wmask = 0;
+#pragma unroll
+        for (int w = 0; w < (int)kWords; ++w)
+            if ((nzm >> (4 * w)) & 0xFull) wmask |= 1u << w;
+        const uint32_t pdeg = used[v] ? 0u : push_degree(off, adj, v, rs);
+        if (lane == 0) {
+            alist[k] = v;
+            amask[k] = wmask;
+            aend[k] = pdeg;
+        }
+        emit_push_chunks(k, pdeg, C, chunks);
+    }
+    n_dq = wave_sum_u64(n_dq);
+    if (lane == 0) {
+        if (n_dq) atomicAdd(&C->dq_pruned, n_dq);
+        if (n_lab) atomicAdd(&C->labels, n_lab);
+        if (n_rows) atomicAdd(&C->dq_rows, (uint32_t)n_rows);
+    }
+}
+
+// reset_at_end (PARA:1328-1410) and the next batch's indicator reset
+// (PARA:877-901): clear indicator rows of touched vertices and the hub table.
+__global__ __launch_bounds__(256) void k_batch_end(unsigned long long *ind, const uint32_t *tlist,
+                                                   uint32_t *hinfo, const uint32_t *hlist, const BuildCounters *C)
+{
+    const uint32_t nt = C->n_touched, nh = C->n_hub;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t tot = (uint64_t)nt * kWords;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += stride)
+        ind[(uint64_t)tlist[i / kWords] * kWords + (i % kWords)] = 0ull;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh; i += stride) hinfo[hlist[i]] = kEmpty;
+}
+
+// Pool compaction: copy each vertex's labels into a fresh pool.
+__global__ __launch_bounds__(256) void k_pool_move(uint32_t n, const uint32_t *src, const unsigned long long *obase,
+                                                   const uint32_t *len, const uint64_t *nbase, uint32_t *dst,
+                                                   unsigned long long *base_out, uint32_t *cap_out,
+                                                   const uint32_t *cap_new)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t v = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; v < n; v += nwaves) {
+        const uint32_t L = len[v];
+        const unsigned long long ob = obase[v];
+        const uint64_t nb = nbase[v];
+        for (uint32_t j = lane; j < L; j += 64) dst[nb + j] = src[ob + j];
+        if (lane == 0) {
+            base_out[v] = nb;
+            cap_out[v] = cap_new[v];
+        }
+    }
+}
+
+__global__ void k_cap_for_len(uint32_t n, const uint32_t *len, uint32_t *cap, uint32_t exact)
+{
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        uint32_t L = len[v];
+        cap[v] = exact ? L : (L + (L >> 2) + 4);
+    }
+}
+
+// Final CSR gather (order is fixed by the segmented sort that follows).
+__global__ __launch_bounds__(256) void k_gather_labels(uint32_t n, const uint32_t *pool,
+                                                       const unsigned long long *base, const uint64_t *loff,
+                                                       uint32_t *lab)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t v = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; v < n; v += nwaves) {
+        const uint64_t o = loff[v];
+        const uint32_t L = (uint32_t)(loff[v + 1] - o);
+        const unsigned long long b = base[v];
+        for (uint32_t j = lane; j < L; j += 64) lab[o + j] = pool[b + j];
+    }
+}
+
+}  // namespace
+
+// Grows (or just compacts) the label pool between batches.
+static pado_gpu_status pool_repack(hipStream_t st, uint32_t n, DevBuf<uint32_t> &pool, DevBuf<unsigned long long> &base,
+                                   DevBuf<uint32_t> &len, DevBuf<uint32_t> &cap, unsigned long long new_size,
+                                   unsigned long long *new_top)
+{
+    DevBuf<uint32_t> ncap;
+    DevBuf<uint64_t> nbase;
+    PADO_TRY(ncap.alloc(n));
+    PADO_TRY(nbase.alloc((uint64_t)n + 1));
+    hipLaunchKernelGGL(k_cap_for_len, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, len.p, ncap.p, 0u);
+    PADO_TRY(scan_u32_to_u64(ncap.p, nbase.p, n, st));
+    uint64_t need = 0;
+    PADO_HIP(hipMemcpy(&need, nbase.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    if (new_size < need * 2) new_size = need * 2;
+    DevBuf<uint32_t> npool;
+    PADO_TRY(npool.alloc(new_size));
+    hipLaunchKernelGGL(k_pool_move, dim3(grid_for((uint64_t)n * 64, 256, 8192)), dim3(256), 0, st, n, pool.p, base.p,
+                       len.p, nbase.p, npool.p, base.p, cap.p, ncap.p);
+    PADO_HIP(hipStreamSynchronize(st));
+    std::swap(pool.p, npool.p);
+    std::swap(pool.n, npool.n);
+    *new_top = need;
+    return PADO_GPU_OK;
+}
+
+pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevBuf<uint8_t> &d_used,
+                              uint32_t batch)
+{
+    hipStream_t st = g->ctx->stream;
+    const uint32_t n = g->n;
+    if (batch == 0 || batch > kBatch) {
+        set_error("batch_size must be in [1, 1024]");
+        return PADO_GPU_ERR_ARG;
+    }
+
+    DevBuf<unsigned long long> ind, cand, ag, chunks, lbase;
+    DevBuf<uint32_t> tmark, tlist, cmark, clist, alist, amask, aend, hinfo, hlist, pool, llen, lcap;
+    DevBuf<uint8_t> dc;
+    DevBuf<BuildCounters> ctr;
+    PADO_TRY(ind.alloc((uint64_t)n * kWords));
+    PADO_TRY(cand.alloc((uint64_t)n * kWords));
+    PADO_TRY(ag.alloc((uint64_t)n * kWords));
+    PADO_TRY(chunks.alloc(g->m / kChunk + (uint64_t)n + 64));
+    PADO_TRY(tmark.alloc(n));
+    PADO_TRY(tlist.alloc(n));
+    PADO_TRY(cmark.alloc(n));
+    PADO_TRY(clist.alloc(n));
+    PADO_TRY(alist.alloc(n));
+    PADO_TRY(amask.alloc(n));
+    PADO_TRY(aend.alloc(n));
+    PADO_TRY(hinfo.alloc(n));
+    PADO_TRY(hlist.alloc(n));
+    PADO_TRY(lbase.alloc(n));
+    PADO_TRY(llen.alloc(n));
+    PADO_TRY(lcap.alloc(n));
+    PADO_TRY(ctr.alloc(1));
+    unsigned long long pool_size = std::max<unsigned long long>(1ull << 22, 16ull * n);
+    PADO_TRY(pool.alloc(pool_size));
+    uint32_t dc_slots = std::min<uint32_t>(n, 1u << 16);
+    PADO_TRY(dc.alloc((uint64_t)dc_slots * kBatch));
+
+    const uint32_t fg = 4096;
+    hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, ind.p, (uint64_t)n * kWords);
+    hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, cand.p, (uint64_t)n * kWords);
+    hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, tmark.p, (uint64_t)n, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, cmark.p, (uint64_t)n, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, hinfo.p, (uint64_t)n, kEmpty);
+    hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, llen.p, (uint64_t)n, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, lcap.p, (uint64_t)n, 0u);
+    hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, lbase.p, (uint64_t)n);
+    PADO_HIP(hipMemsetAsync(ctr.p, 0, sizeof(BuildCounters), st));
+
+    BuildCounters *h_ctr = nullptr;
+    PADO_HIP(hipHostMalloc((void **)&h_ctr, sizeof(BuildCounters), hipHostMallocDefault));
+    struct PinnedFree {
+        BuildCounters *p;
+        ~PinnedFree() { (void)hipHostFree(p); }
+    } pin{h_ctr};
+
+    LabelStore S{pool.p, lbase.p, llen.p, lcap.p, pool_size};
+    const uint32_t nb = (n + batch - 1) / batch;
+    uint32_t gen_iter = 0;
+    uint64_t iterations = 0;
+    const uint32_t push_grid = 4096, add_grid = 2048;
+    for (uint32_t b = 0; b < nb; ++b) {
+        const uint32_t rs = b * batch;
+        const uint32_t rsz = std::min(batch, n - rs);
+        const uint32_t gen_batch = b + 1;
+        const uint32_t root_blocks = (rsz * 64 + 255) / 256;
+        hipLaunchKernelGGL(k_batch_counters_reset, dim3(1), dim3(1), 0, st, ctr.p);
+        hipLaunchKernelGGL(k_batch_init, dim3(root_blocks), dim3(256), 0, st, rs, rsz, gen_batch, d_used.p, g->off.p,
+                           g->adj.p, S, ind.p, tmark.p, tlist.p, alist.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
+        hipLaunchKernelGGL(k_hub_mark, dim3(root_blocks), dim3(256), 0, st, rs, rsz, d_used.p, S, hinfo.p, hlist.p,
+                           ctr.p);
+        PADO_HIP(hipMemcpyAsync(h_ctr, ctr.p, sizeof(BuildCounters), hipMemcpyDeviceToHost, st));
+        PADO_HIP(hipStreamSynchronize(st));
+        if (h_ctr->overflow) {
+            set_error("label pool exhausted");
+            return PADO_GPU_ERR_NOMEM;
+        }
+        if (h_ctr->n_hub > dc_slots) {
+            dc_slots = std::min<uint32_t>(n, std::max<uint32_t>(h_ctr->n_hub, dc_slots * 2));
+            PADO_TRY(dc.alloc((uint64_t)dc_slots * kBatch));
+        }
+        hipLaunchKernelGGL(k_hub_slots, dim3(grid_for((uint64_t)h_ctr->n_hub * 64, 256, 8192)), dim3(256), 0, st,
+                           hlist.p, hinfo.p, dc.p, ctr.p);
+        hipLaunchKernelGGL(k_hub_fill, dim3(root_blocks), dim3(256), 0, st, rs, rsz, d_used.p, S, hinfo.p, dc.p);
+
+        uint32_t n_act = h_ctr->n_act;
+        uint32_t iter = 0;
+        while (n_act != 0) {
+            ++iter;
+            if (iter >= kInf8) {
+                set_error("batch iteration reached 255: distances outside the uint8 domain of the reference");
+                return PADO_GPU_ERR_DOMAIN;
+            }
+            ++gen_iter;
+            ++iterations;
+            hipLaunchKernelGGL(k_push, dim3(push_grid), dim3(256), 0, st, iter, rs, gen_batch, gen_iter, g->off.p,
+                               g->adj.p, alist.p, ag.p, amask.p, aend.p, chunks.p, ind.p, cand.p, tmark.p, tlist.p,
+                               cmark.p, clist.p, idx->bpd.p, (const ulonglong2 *)idx->bps.p, ctr.p);
+            hipLaunchKernelGGL(k_mid, dim3(1), dim3(1), 0, st, ctr.p);
+            hipLaunchKernelGGL(k_add, dim3(add_grid), dim3(256), 0, st, iter, rs, rsz, d_used.p, g->off.p, g->adj.p,
+                               clist.p, cand.p, S, hinfo.p, dc.p, alist.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
+            hipLaunchKernelGGL(k_post, dim3(1), dim3(1), 0, st, ctr.p);
+            PADO_HIP(hipMemcpyAsync(h_ctr, ctr.p, sizeof(BuildCounters), hipMemcpyDeviceToHost, st));
+            PADO_HIP(hipStreamSynchronize(st));
+            if (h_ctr->overflow) {
+                set_error("label pool exhausted");
+                return PADO_GPU_ERR_NOMEM;
+            }
+            n_act = h_ctr->n_act;
+        }
+        hipLaunchKernelGGL(k_batch_end, dim3(1024), dim3(256), 0, st, ind.p, tlist.p, hinfo.p, hlist.p, ctr.p);
+        // Keep at least half the pool free for the next batch.
+        if (h_ctr->pool_top * 2 > pool_size) {
+            unsigned long long top = 0;
+            PADO_TRY(pool_repack(st, n, pool, lbase, llen, lcap, pool_size * 2, &top));
+            pool_size = pool.n;
+            S.pool = pool.p;
+            S.pool_size = pool_size;
+            PADO_HIP(hipMemcpyAsync(&ctr.p->pool_top, &top, sizeof(top), hipMemcpyHostToDevice, st));
+            PADO_HIP(hipStreamSynchronize(st));
+        }
+    }
+    PADO_HIP(hipMemcpyAsync(h_ctr, ctr.p, sizeof(BuildCounters), hipMemcpyDeviceToHost, st));
+    PADO_HIP(hipStreamSynchronize(st));
+    idx->counters = *h_ctr;
+    idx->iterations = iterations;
+
+    // Canonical CSR: gather each vertex's labels and sort them by hub
+    // (order_labels' per-vertex std::sort, PARA:2240).
+    PADO_TRY(idx->loff.alloc((uint64_t)n + 1));
+    PADO_TRY(scan_u32_to_u64(llen.p, idx->loff.p, n, st));
+    uint64_t total = 0;
+    PADO_HIP(hipMemcpy(&total, idx->loff.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    idx->num_labels = total;
+    PADO_TRY(idx->lab.alloc(total));
+    hipLaunchKernelGGL(k_gather_labels, dim3(grid_for((uint64_t)n * 64, 256, 8192)), dim3(256), 0, st, n, pool.p,
+                       lbase.p, idx->loff.p, idx->lab.p);
+    PADO_TRY(segmented_sort_u32(idx->lab.p, total, idx->loff.p, n, st));
+    return PADO_GPU_OK;
+}
+
+}  // namespace pado

@@ -1,0 +1,148 @@
+// common.hpp -- shared definitions of the PADO GPU library (gfx950 / MI355X).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "pado_gpu.h"
+
+namespace pado {
+
+constexpr uint32_t kBP = 50;          // BITPARALLEL_SIZE (PARA:45)
+constexpr uint32_t kBpdStride = 64;   // bytes per vertex in the bp_dist array (50 used)
+constexpr uint32_t kBatch = 1024;     // BATCH_SIZE (PARA:42)
+constexpr uint32_t kWords = kBatch / 64;  // 64-bit words per 1024-root bitmap row
+constexpr uint8_t kInf8 = 255;        // SMALLI_MAX (GLB:40)
+constexpr uint32_t kHubBits = 24;     // packed label = (hub << 8) | dist
+constexpr uint32_t kMaxVertices = 1u << kHubBits;
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;
+constexpr uint32_t kChunk = 16;       // edges per quarter-wave work descriptor
+
+// Thread-local error text behind pado_gpu_last_error().
+void set_error(const std::string &msg);
+void clear_error();
+
+// Status-returning HIP check for the C ABI implementation.
+#define PADO_HIP(call)                                                              \
+    do {                                                                            \
+        hipError_t e_ = (call);                                                     \
+        if (e_ != hipSuccess) {                                                     \
+            ::pado::set_error(std::string(#call) + ": " + hipGetErrorString(e_));   \
+            return PADO_GPU_ERR_HIP;                                                \
+        }                                                                           \
+    } while (0)
+
+#define PADO_TRY(call)                                  \
+    do {                                                \
+        pado_gpu_status s_ = (call);                    \
+        if (s_ != PADO_GPU_OK) return s_;               \
+    } while (0)
+
+// Device buffer owned by a handle.
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    pado_gpu_status alloc(size_t count)
+    {
+        release();
+        if (count == 0) count = 1;
+        hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            set_error("hipMalloc(" + std::to_string(count * sizeof(T)) + " B): " + hipGetErrorString(e));
+            return PADO_GPU_ERR_NOMEM;
+        }
+        n = count;
+        return PADO_GPU_OK;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+};
+
+// Device-side counters shared by the labeling kernels (one struct per build).
+struct BuildCounters {
+    uint32_t n_act;        // active vertices produced for the next iteration
+    uint32_t n_chunk;      // push work descriptors for the next iteration
+    uint32_t n_cand;       // candidate vertices of the current iteration
+    uint32_t n_touched;    // vertices whose indicator row was touched this batch
+    uint32_t n_hub;        // distinct hubs in the batch's root-distance table
+    uint32_t overflow;     // label pool exhausted
+    uint32_t dq_rows;      // root-distance rows scanned (statistic)
+    uint32_t pad0;
+    unsigned long long pool_top;  // next free label slot in the pool
+    unsigned long long pushes;    // (tail, root) pairs offered by push
+    unsigned long long ind_pass;  // pairs that set a fresh indicator bit
+    unsigned long long bp_pruned; // pairs pruned by the bit-parallel check
+    unsigned long long cands;     // (vertex, root) candidates after the BP check
+    unsigned long long dq_pruned; // candidates pruned by the distance query
+    unsigned long long labels;    // labels appended
+    unsigned long long pad1;
+};
+
+}  // namespace pado
+
+// Opaque handle definitions (C ABI types).
+struct pado_gpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 0;
+};
+
+struct pado_gpu_graph {
+    pado_gpu_ctx *ctx = nullptr;
+    uint32_t n = 0;
+    uint64_t m = 0;  // directed edge count (2x undirected)
+    pado::DevBuf<uint64_t> off;
+    pado::DevBuf<uint32_t> adj;
+    pado::DevBuf<uint8_t> w;  // edge weights (weighted graphs), else empty
+    std::vector<uint64_t> h_off;  // host copy: root selection is a serial pass
+    std::vector<uint32_t> h_adj;
+    std::vector<uint8_t> h_w;
+};
+
+struct pado_gpu_index {
+    pado_gpu_ctx *ctx = nullptr;
+    uint32_t n = 0;
+    uint32_t weighted = 0;
+    // Bit-parallel labels, rank space: bpd[v*64 + i], bps[(v*50 + i)*2 + {0,1}].
+    pado::DevBuf<uint8_t> bpd;
+    pado::DevBuf<uint64_t> bps;
+    // Canonical labels, rank space: CSR of packed (hub << 8) | dist, hubs ascending.
+    pado::DevBuf<uint64_t> loff;
+    pado::DevBuf<uint32_t> lab;
+    uint64_t num_labels = 0;
+    // Weighted labels (hub u32, dist u32), rank space.
+    pado::DevBuf<uint32_t> whub;
+    pado::DevBuf<uint32_t> wdist;
+    // Query copy in original-id order (order_labels, PARA:2198-2281).
+    int ordered = 0;
+    pado::DevBuf<uint8_t> qbpd;
+    pado::DevBuf<uint64_t> qbps;
+    pado::DevBuf<uint64_t> qoff;
+    pado::DevBuf<uint32_t> qlab;
+    pado::DevBuf<uint32_t> qwhub;
+    pado::DevBuf<uint32_t> qwdist;
+    // Staging for host-pointer queries.
+    pado::DevBuf<uint32_t> qa, qb;
+    pado::DevBuf<uint8_t> qout;
+    pado::DevBuf<uint32_t> qout32;
+    // Build statistics.
+    double ms_bp = 0, ms_batch = 0, ms_final = 0;
+    uint32_t bp_roots[pado::kBP] = {0};
+    uint32_t bp_max_level = 0;
+    uint64_t iterations = 0;
+    pado::BuildCounters counters{};
+};

@@ -1,0 +1,37 @@
+// internal.hpp -- declarations shared between the translation units of the
+// PADO GPU library (not part of the C ABI).
+#pragma once
+
+#include "common.hpp"
+
+namespace pado {
+
+// Bit-parallel labeling (PARA:544-840).  Fills idx->bpd / idx->bps and the
+// used-root flags (device + host copies).
+pado_gpu_status build_bp(pado_gpu_graph *g, pado_gpu_index *idx, DevBuf<uint8_t> &d_used,
+                         std::vector<uint8_t> &h_used);
+
+// Batched pruned labeling (PARA:848-1836).  Produces the canonical rank-space
+// label CSR idx->loff / idx->lab.
+pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevBuf<uint8_t> &d_used,
+                              uint32_t batch_size);
+
+// Weighted labeling (WPARA).
+pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t batch_size);
+
+// Exclusive scan of n u32 values into n+1 u64 offsets (out[n] = total).
+pado_gpu_status scan_u32_to_u64(const uint32_t *in, uint64_t *out, uint32_t n, hipStream_t s);
+// Sort every CSR segment of u32 keys ascending in place.
+pado_gpu_status segmented_sort_u32(uint32_t *keys, uint64_t total, const uint64_t *off, uint32_t n,
+                                   hipStream_t s);
+
+// Launch geometry helpers.
+inline uint32_t grid_for(uint64_t threads, uint32_t block, uint32_t cap = 1u << 20)
+{
+    uint64_t g = (threads + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (uint32_t)g;
+}
+
+}  // namespace pado

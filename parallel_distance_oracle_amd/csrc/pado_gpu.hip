@@ -1,0 +1,595 @@
+// pado_gpu.hip -- C ABI of the PADO GPU library: contexts, graph upload, index
+// build / export / import / order, queries and index files.
+//
+// Each entry point's reference counterpart is named in include/pado_gpu.h.
+#include <chrono>
+#include <cstring>
+#include <fstream>
+
+#include "internal.hpp"
+
+namespace pado {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string &msg) { g_last_error = msg; }
+void clear_error() { g_last_error.clear(); }
+
+pado_gpu_status launch_query(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,
+                             uint64_t k, uint8_t *out, bool ref_compat, hipStream_t st);
+
+namespace {
+
+// order_labels (PARA:2198-2281): Index[rank2id[v]] = L[v].  Here new position
+// of original id x is x itself, its source is rank-space vertex rank[x].
+__global__ __launch_bounds__(256) void k_order_counts(uint32_t n, const uint32_t *rank, const uint64_t *loff,
+                                                      uint32_t *cnt)
+{
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t x = blockIdx.x * blockDim.x + threadIdx.x; x < n; x += stride) {
+        uint32_t v = rank[x];
+        cnt[x] = (uint32_t)(loff[v + 1] - loff[v]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_order_copy(uint32_t n, const uint32_t *rank, const uint8_t *bpd,
+                                                    const ulonglong2 *bps, const uint64_t *loff, const uint32_t *lab,
+                                                    const uint32_t *wdist, uint8_t *qbpd, ulonglong2 *qbps,
+                                                    const uint64_t *qoff, uint32_t *qlab, uint32_t *qwdist)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t x = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; x < n; x += nw) {
+        const uint32_t v = rank[x];
+        if (bpd) {
+            qbpd[(uint64_t)x * kBpdStride + lane] = bpd[(uint64_t)v * kBpdStride + lane];
+            if (lane < kBP) qbps[(uint64_t)x * kBP + lane] = bps[(uint64_t)v * kBP + lane];
+        }
+        const uint64_t s = loff[v], dsto = qoff[x];
+        const uint32_t L = (uint32_t)(loff[v + 1] - s);
+        for (uint32_t j = lane; j < L; j += 64) {
+            qlab[dsto + j] = lab[s + j];
+            if (wdist) qwdist[dsto + j] = wdist[s + j];
+        }
+    }
+}
+
+__global__ void k_pack_labels(uint64_t total, const uint32_t *hub, const uint8_t *dist, uint32_t *lab)
+{
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride)
+        lab[i] = (hub[i] << 8) | dist[i];
+}
+
+struct Timer {
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double ms() const
+    {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+pado_gpu_status check_ctx(pado_gpu_ctx *ctx)
+{
+    if (!ctx) {
+        set_error("null context");
+        return PADO_GPU_ERR_ARG;
+    }
+    PADO_HIP(hipSetDevice(ctx->device));
+    return PADO_GPU_OK;
+}
+
+// Builds the original-id query copy from the rank-space index.
+pado_gpu_status order_index(pado_gpu_index *idx, const uint32_t *h_rank)
+{
+    hipStream_t st = idx->ctx->stream;
+    const uint32_t n = idx->n;
+    // rank must be a permutation of [0, n).
+    std::vector<uint8_t> seen(n, 0);
+    for (uint32_t x = 0; x < n; ++x) {
+        if (h_rank[x] >= n || seen[h_rank[x]]) {
+            set_error("rank is not a permutation of [0, n)");
+            return PADO_GPU_ERR_ARG;
+        }
+        seen[h_rank[x]] = 1;
+    }
+    DevBuf<uint32_t> rank, cnt;
+    PADO_TRY(rank.alloc(n));
+    PADO_TRY(cnt.alloc(n));
+    PADO_HIP(hipMemcpyAsync(rank.p, h_rank, sizeof(uint32_t) * n, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_order_counts, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, rank.p, idx->loff.p, cnt.p);
+    PADO_TRY(idx->qoff.alloc((uint64_t)n + 1));
+    PADO_TRY(scan_u32_to_u64(cnt.p, idx->qoff.p, n, st));
+    const bool w = idx->weighted != 0;
+    if (!w) {
+        PADO_TRY(idx->qbpd.alloc((uint64_t)n * kBpdStride));
+        PADO_TRY(idx->qbps.alloc((uint64_t)n * kBP * 2));
+    } else {
+        PADO_TRY(idx->qwdist.alloc(idx->num_labels));
+    }
+    PADO_TRY(idx->qlab.alloc(idx->num_labels));
+    hipLaunchKernelGGL(k_order_copy, dim3(grid_for((uint64_t)n * 64, 256, 8192)), dim3(256), 0, st, n, rank.p,
+                       w ? nullptr : idx->bpd.p, w ? nullptr : (const ulonglong2 *)idx->bps.p, idx->loff.p,
+                       w ? idx->whub.p : idx->lab.p, w ? idx->wdist.p : nullptr, w ? nullptr : idx->qbpd.p,
+                       w ? nullptr : (ulonglong2 *)idx->qbps.p, idx->qoff.p, idx->qlab.p,
+                       w ? idx->qwdist.p : nullptr);
+    PADO_HIP(hipStreamSynchronize(st));
+    idx->ordered = 1;
+    return PADO_GPU_OK;
+}
+
+}  // namespace
+}  // namespace pado
+
+using namespace pado;
+
+extern "C" {
+
+const char *pado_gpu_last_error(void) { return g_last_error.c_str(); }
+
+int pado_gpu_device_count(void)
+{
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+pado_gpu_status pado_gpu_ctx_create(int device, pado_gpu_ctx **out)
+{
+    clear_error();
+    if (!out) {
+        set_error("null output pointer");
+        return PADO_GPU_ERR_ARG;
+    }
+    *out = nullptr;
+    int count = 0;
+    PADO_HIP(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) {
+        set_error("device " + std::to_string(device) + " out of range (" + std::to_string(count) + " devices)");
+        return PADO_GPU_ERR_ARG;
+    }
+    PADO_HIP(hipSetDevice(device));
+    auto *c = new pado_gpu_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        set_error("hipStreamCreateWithFlags failed");
+        return PADO_GPU_ERR_HIP;
+    }
+    *out = c;
+    return PADO_GPU_OK;
+}
+
+void pado_gpu_ctx_destroy(pado_gpu_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+void *pado_gpu_ctx_stream(pado_gpu_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+pado_gpu_status pado_gpu_ctx_sync(pado_gpu_ctx *ctx)
+{
+    clear_error();
+    PADO_TRY(check_ctx(ctx));
+    PADO_HIP(hipStreamSynchronize(ctx->stream));
+    return PADO_GPU_OK;
+}
+
+pado_gpu_status pado_gpu_graph_upload(pado_gpu_ctx *ctx, uint32_t n, uint64_t m_dir, const uint64_t *off,
+                                      const uint32_t *adj, const uint8_t *weights, pado_gpu_graph **out)
+{
+    clear_error();
+    PADO_TRY(check_ctx(ctx));
+    if (!out || !off || (m_dir && !adj)) {
+        set_error("null argument");
+        return PADO_GPU_ERR_ARG;
+    }
+    *out = nullptr;
+    if (n == 0 || n >= kMaxVertices) {
+        set_error("vertex count must be in [1, 2^24)");
+        return PADO_GPU_ERR_ARG;
+    }
+    if (m_dir >= (1ull << 32)) {
+        set_error("directed edge count must be < 2^32");
+        return PADO_GPU_ERR_ARG;
+    }
+    if (off[0] != 0 || off[n] != m_dir) {
+        set_error("offsets must start at 0 and end at m_dir");
+        return PADO_GPU_ERR_ARG;
+    }
+    for (uint32_t v = 0; v < n; ++v)
+        if (off[v + 1] < off[v]) {
+            set_error("offsets must be non-decreasing");
+            return PADO_GPU_ERR_ARG;
+        }
+    for (uint64_t e = 0; e < m_dir; ++e)
+        if (adj[e] >= n) {
+            set_error("adjacency id out of range");
+            return PADO_GPU_ERR_ARG;
+        }
+    // id_transfer sorts every list descending (GR:192); the push early exit
+    // and the bit-parallel neighbour scan rely on it.
+    for (uint32_t v = 0; v < n; ++v)
+        for (uint64_t e = off[v] + 1; e < off[v + 1]; ++e)
+            if (adj[e] > adj[e - 1]) {
+                set_error("adjacency lists must be sorted descending (rank space, after id_transfer)");
+                return PADO_GPU_ERR_ARG;
+            }
+    auto *g = new pado_gpu_graph();
+    g->ctx = ctx;
+    g->n = n;
+    g->m = m_dir;
+    g->h_off.assign(off, off + n + 1);
+    g->h_adj.assign(adj, adj + m_dir);
+    if (weights) g->h_w.assign(weights, weights + m_dir);
+    pado_gpu_status s = g->off.alloc((uint64_t)n + 1);
+    if (s == PADO_GPU_OK) s = g->adj.alloc(m_dir);
+    if (s == PADO_GPU_OK && weights) s = g->w.alloc(m_dir);
+    if (s != PADO_GPU_OK) {
+        delete g;
+        return s;
+    }
+    hipStream_t st = ctx->stream;
+    if (hipMemcpyAsync(g->off.p, off, sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, st) != hipSuccess ||
+        (m_dir && hipMemcpyAsync(g->adj.p, adj, sizeof(uint32_t) * m_dir, hipMemcpyHostToDevice, st) != hipSuccess) ||
+        (weights && hipMemcpyAsync(g->w.p, weights, m_dir, hipMemcpyHostToDevice, st) != hipSuccess) ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        delete g;
+        set_error("graph upload failed");
+        return PADO_GPU_ERR_HIP;
+    }
+    *out = g;
+    return PADO_GPU_OK;
+}
+
+void pado_gpu_graph_free(pado_gpu_graph *g) { delete g; }
+
+pado_gpu_status pado_gpu_build(pado_gpu_ctx *ctx, const pado_gpu_graph *graph, const pado_gpu_build_cfg *cfg,
+                               pado_gpu_index **out)
+{
+    clear_error();
+    PADO_TRY(check_ctx(ctx));
+    if (!graph || !out) {
+        set_error("null argument");
+        return PADO_GPU_ERR_ARG;
+    }
+    *out = nullptr;
+    pado_gpu_build_cfg c{};
+    c.batch_size = kBatch;
+    if (cfg) c = *cfg;
+    if (c.batch_size == 0) c.batch_size = kBatch;
+    auto *idx = new pado_gpu_index();
+    idx->ctx = ctx;
+    idx->n = graph->n;
+    pado_gpu_graph *g = const_cast<pado_gpu_graph *>(graph);
+    pado_gpu_status s = PADO_GPU_OK;
+    if (!g->h_w.empty()) {
+        delete idx;
+        set_error("weighted graphs are built by pado_gpu_wbuild");
+        return PADO_GPU_ERR_ARG;
+    } else {
+        DevBuf<uint8_t> used;
+        std::vector<uint8_t> h_used;
+        Timer t0;
+        s = build_bp(g, idx, used, h_used);
+        idx->ms_bp = t0.ms();
+        if (s == PADO_GPU_OK) {
+            Timer t1;
+            s = build_batches(g, idx, used, c.batch_size);
+            idx->ms_batch = t1.ms();
+        }
+    }
+    if (s != PADO_GPU_OK) {
+        delete idx;
+        return s;
+    }
+    *out = idx;
+    return PADO_GPU_OK;
+}
+
+pado_gpu_status pado_gpu_index_get_info(const pado_gpu_index *idx, pado_gpu_index_info *info)
+{
+    clear_error();
+    if (!idx || !info) {
+        set_error("null argument");
+        return PADO_GPU_ERR_ARG;
+    }
+    std::memset(info, 0, sizeof(*info));
+    info->num_v = idx->n;
+    info->weighted = idx->weighted;
+    info->num_labels = idx->num_labels;
+    info->ordered = idx->ordered;
+    info->iterations = idx->iterations;
+    info->ms_bp = idx->ms_bp;
+    info->ms_batch = idx->ms_batch;
+    info->bp_max_level = idx->bp_max_level;
+    for (uint32_t i = 0; i < kBP; ++i) info->bp_roots[i] = idx->bp_roots[i];
+    info->pushes = idx->counters.pushes;
+    info->ind_pass = idx->counters.ind_pass;
+    info->bp_pruned = idx->counters.bp_pruned;
+    info->cands = idx->counters.cands;
+    info->dq_pruned = idx->counters.dq_pruned;
+    info->dq_rows = idx->counters.dq_rows;
+    return PADO_GPU_OK;
+}
+
+pado_gpu_status pado_gpu_index_export(pado_gpu_ctx *ctx, const pado_gpu_index *idx, uint8_t *bp_dist,
+                                      uint64_t *bp_sets, uint64_t *label_off, uint32_t *hubs, uint8_t *dists)
+{
+    clear_error();
+    PADO_TRY(check_ctx(ctx));
+    if (!idx) {
+        set_error("null index");
+        return PADO_GPU_ERR_ARG;
+    }
+    if (idx->weighted) {
+        set_error("weighted index: use pado_gpu_windex_export");
+        return PADO_GPU_ERR_STATE;
+    }
+    if (!idx->loff.p) {
+        set_error("index has no rank-space labels (loaded from an ordered file)");
+        return PADO_GPU_ERR_STATE;
+    }
+    const uint32_t n = idx->n;
+    hipStream_t st = ctx->stream;
+    if (bp_dist) {
+        std::vector<uint8_t> tmp((uint64_t)n * kBpdStride);
+        PADO_HIP(hipMemcpyAsync(tmp.data(), idx->bpd.p, tmp.size(), hipMemcpyDeviceToHost, st));
+        PADO_HIP(hipStreamSynchronize(st));
+        for (uint32_t v = 0; v < n; ++v) std::memcpy(bp_dist + (uint64_t)v * kBP, &tmp[(uint64_t)v * kBpdStride], kBP);
+    }
+    if (bp_sets) PADO_HIP(hipMemcpyAsync(bp_sets, idx->bps.p, sizeof(uint64_t) * n * kBP * 2, hipMemcpyDeviceToHost, st));
+    if (label_off) PADO_HIP(hipMemcpyAsync(label_off, idx->loff.p, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost, st));
+    if (hubs || dists) {
+        std::vector<uint32_t> lab(idx->num_labels);
+        if (idx->num_labels)
+            PADO_HIP(hipMemcpyAsync(lab.data(), idx->lab.p, sizeof(uint32_t) * idx->num_labels, hipMemcpyDeviceToHost, st));
+        PADO_HIP(hipStreamSynchronize(st));
+        for (uint64_t i = 0; i < idx->num_labels; ++i) {
+            if (hubs) hubs[i] = lab[i] >> 8;
+            if (dists) dists[i] = (uint8_t)(lab[i] & 255u);
+        }
+    }
+    PADO_HIP(hipStreamSynchronize(st));
+    return PADO_GPU_OK;
+}
+
+pado_gpu_status pado_gpu_index_import(pado_gpu_ctx *ctx, uint32_t n, const uint8_t *bp_dist, const uint64_t *bp_sets,
+                                      const uint64_t *label_off, const uint32_t *hubs, const uint8_t *dists,
+                                      pado_gpu_index **out)
+{
+    clear_error();
+    PADO_TRY(check_ctx(ctx));
+    if (!out || !bp_dist || !bp_sets || !label_off || n == 0 || n >= kMaxVertices) {
+        set_error("bad argument");
+        return PADO_GPU_ERR_ARG;
+    }
+    *out = nullptr;
+    const uint64_t total = label_off[n];
+    if (total && (!hubs || !dists)) {
+        set_error("null label arrays");
+        return PADO_GPU_ERR_ARG;
+    }
+    for (uint32_t v = 0; v < n; ++v) {
+        if (label_off[v + 1] < label_off[v]) {
+            set_error("label offsets must be non-decreasing");
+            return PADO_GPU_ERR_ARG;
+        }
+        for (uint64_t i = label_off[v]; i < label_off[v + 1]; ++i) {
+            if (hubs[i] >= n || (i > label_off[v] && hubs[i] <= hubs[i - 1])) {
+                set_error("hubs must be < n and strictly ascending per vertex");
+                return PADO_GPU_ERR_ARG;
+            }
+        }
+    }
+    hipStream_t st = ctx->stream;
+    auto *idx = new pado_gpu_index();
+    idx->ctx = ctx;
+    idx->n = n;
+    idx->num_labels = total;
+    std::vector<uint8_t> tmp((uint64_t)n * kBpdStride, kInf8);
+    for (uint32_t v = 0; v < n; ++v) std::memcpy(&tmp[(uint64_t)v * kBpdStride], bp_dist + (uint64_t)v * kBP, kBP);
+    pado_gpu_status s = PADO_GPU_OK;
+    DevBuf<uint32_t> dh;
+    DevBuf<uint8_t> dd;
+    if ((s = idx->bpd.alloc(tmp.size())) != PADO_GPU_OK || (s = idx->bps.alloc((uint64_t)n * kBP * 2)) != PADO_GPU_OK ||
+        (s = idx->loff.alloc((uint64_t)n + 1)) != PADO_GPU_OK || (s = idx->lab.alloc(total)) != PADO_GPU_OK ||
+        (s = dh.alloc(total)) != PADO_GPU_OK || (s = dd.alloc(total)) != PADO_GPU_OK) {
+        delete idx;
+        return s;
+    }
+    if (hipMemcpyAsync(idx->bpd.p, tmp.data(), tmp.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(idx->bps.p, bp_sets, sizeof(uint64_t) * n * kBP * 2, hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(idx->loff.p, label_off, sizeof(uint64_t) * (n + 1), hipMemcpyHostToDevice, st) != hipSuccess ||
+        (total && hipMemcpyAsync(dh.p, hubs, sizeof(uint32_t) * total, hipMemcpyHostToDevice, st) != hipSuccess) ||
+        (total && hipMemcpyAsync(dd.p, dists, total, hipMemcpyHostToDevice, st) != hipSuccess)) {
+        delete idx;
+        set_error("index import copy failed");
+        return PADO_GPU_ERR_HIP;
+    }
+    if (total)
+        hipLaunchKernelGGL(k_pack_labels, dim3(grid_for(total, 256, 8192)), dim3(256), 0, st, total, dh.p, dd.p,
+                           idx->lab.p);
+    if (hipStreamSynchronize(st) != hipSuccess) {
+        delete idx;
+        set_error("index import failed");
+        return PADO_GPU_ERR_HIP;
+    }
+    *out = idx;
+    return PADO_GPU_OK;
+}
+
+pado_gpu_status pado_gpu_index_order(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uint32_t *rank, uint32_t n)
+{
+    clear_error();
+    PADO_TRY(check_ctx(ctx));
+    if (!idx || !rank || n != idx->n) {
+        set_error("rank must have num_v entries");
+        return PADO_GPU_ERR_ARG;
+    }
+    return order_index(idx, rank);
+}
+
+pado_gpu_status pado_gpu_query(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uint32_t *a, const uint32_t *b, uint64_t k,
+                               uint8_t *out, uint32_t flags, void *stream)
+{
+    clear_error();
+    PADO_TRY(check_ctx(ctx));
+    if (!idx || (k && (!a || !b || !out))) {
+        set_error("null argument");
+        return PADO_GPU_ERR_ARG;
+    }
+    if (idx->weighted) {
+        set_error("weighted index: use pado_gpu_wquery");
+        return PADO_GPU_ERR_STATE;
+    }
+    const bool rank_space = (flags & PADO_GPU_QUERY_RANK_SPACE) != 0;
+    if (!rank_space && !idx->ordered) {
+        set_error("query before order_labels: call pado_gpu_index_order or pass PADO_GPU_QUERY_RANK_SPACE");
+        return PADO_GPU_ERR_STATE;
+    }
+    if (rank_space && !idx->loff.p) {
+        set_error("index has no rank-space labels");
+        return PADO_GPU_ERR_STATE;
+    }
+    if (k == 0) return PADO_GPU_OK;
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    const bool ref = (flags & PADO_GPU_QUERY_REF_COMPAT) != 0;
+    if (flags & PADO_GPU_QUERY_DEVICE_PTRS) return launch_query(idx, rank_space, a, b, k, out, ref, st);
+    if (idx->qa.n < k) {
+        PADO_TRY(idx->qa.alloc(k));
+        PADO_TRY(idx->qb.alloc(k));
+        PADO_TRY(idx->qout.alloc(k));
+    }
+    PADO_HIP(hipMemcpyAsync(idx->qa.p, a, sizeof(uint32_t) * k, hipMemcpyHostToDevice, st));
+    PADO_HIP(hipMemcpyAsync(idx->qb.p, b, sizeof(uint32_t) * k, hipMemcpyHostToDevice, st));
+    PADO_TRY(launch_query(idx, rank_space, idx->qa.p, idx->qb.p, k, idx->qout.p, ref, st));
+    PADO_HIP(hipMemcpyAsync(out, idx->qout.p, k, hipMemcpyDeviceToHost, st));
+    PADO_HIP(hipStreamSynchronize(st));
+    return PADO_GPU_OK;
+}
+
+void pado_gpu_index_free(pado_gpu_index *idx) { delete idx; }
+
+// Index file in the reference's format (UNP:1121-1180 writer, 1183-1224
+// reader): u32 num_v, u32 BITPARALLEL_SIZE, then per ORIGINAL id: 50 x (u8 d,
+// u64 S^-1, u64 S^0) unpadded, u32 size, size x (u32 rank-space hub, u8 dist).
+pado_gpu_status pado_gpu_index_store(pado_gpu_ctx *ctx, const pado_gpu_index *idx, const char *filename,
+                                     const uint32_t *rank, uint32_t n)
+{
+    clear_error();
+    PADO_TRY(check_ctx(ctx));
+    if (!idx || !filename || !rank || n != idx->n || idx->weighted) {
+        set_error("bad argument");
+        return PADO_GPU_ERR_ARG;
+    }
+    std::vector<uint8_t> bd((uint64_t)n * kBP);
+    std::vector<uint64_t> bs((uint64_t)n * kBP * 2), off((uint64_t)n + 1);
+    std::vector<uint32_t> hub(idx->num_labels);
+    std::vector<uint8_t> dist(idx->num_labels);
+    PADO_TRY(pado_gpu_index_export(ctx, idx, bd.data(), bs.data(), off.data(), hub.data(), dist.data()));
+    std::ofstream f(filename, std::ios::binary);
+    if (!f.is_open()) {
+        set_error(std::string("cannot open file ") + filename);
+        return PADO_GPU_ERR_IO;
+    }
+    const uint32_t bp = kBP;
+    f.write((const char *)&n, 4);
+    f.write((const char *)&bp, 4);
+    for (uint32_t x = 0; x < n; ++x) {
+        const uint32_t v = rank[x];
+        if (v >= n) {
+            set_error("rank out of range");
+            return PADO_GPU_ERR_ARG;
+        }
+        for (uint32_t i = 0; i < kBP; ++i) {
+            f.write((const char *)&bd[(uint64_t)v * kBP + i], 1);
+            f.write((const char *)&bs[((uint64_t)v * kBP + i) * 2], 8);
+            f.write((const char *)&bs[((uint64_t)v * kBP + i) * 2 + 1], 8);
+        }
+        const uint32_t sz = (uint32_t)(off[v + 1] - off[v]);
+        f.write((const char *)&sz, 4);
+        for (uint64_t i = off[v]; i < off[v + 1]; ++i) {
+            f.write((const char *)&hub[i], 4);
+            f.write((const char *)&dist[i], 1);
+        }
+    }
+    if (!f.good()) {
+        set_error(std::string("write failed: ") + filename);
+        return PADO_GPU_ERR_IO;
+    }
+    return PADO_GPU_OK;
+}
+
+pado_gpu_status pado_gpu_index_load(pado_gpu_ctx *ctx, const char *filename, pado_gpu_index **out)
+{
+    clear_error();
+    PADO_TRY(check_ctx(ctx));
+    if (!filename || !out) {
+        set_error("bad argument");
+        return PADO_GPU_ERR_ARG;
+    }
+    *out = nullptr;
+    std::ifstream f(filename, std::ios::binary);
+    if (!f.is_open()) {
+        set_error(std::string("cannot open file ") + filename);
+        return PADO_GPU_ERR_IO;
+    }
+    uint32_t n = 0, bp = 0;
+    f.read((char *)&n, 4);
+    f.read((char *)&bp, 4);
+    if (!f.good() || bp != kBP || n == 0 || n >= kMaxVertices) {
+        set_error("bad index file header");
+        return PADO_GPU_ERR_IO;
+    }
+    std::vector<uint8_t> bd((uint64_t)n * kBP);
+    std::vector<uint64_t> bs((uint64_t)n * kBP * 2), off((uint64_t)n + 1, 0);
+    std::vector<uint32_t> hub;
+    std::vector<uint8_t> dist;
+    for (uint32_t x = 0; x < n; ++x) {
+        for (uint32_t i = 0; i < kBP; ++i) {
+            f.read((char *)&bd[(uint64_t)x * kBP + i], 1);
+            f.read((char *)&bs[((uint64_t)x * kBP + i) * 2], 8);
+            f.read((char *)&bs[((uint64_t)x * kBP + i) * 2 + 1], 8);
+        }
+        uint32_t sz = 0;
+        f.read((char *)&sz, 4);
+        if (!f.good()) {
+            set_error("truncated index file");
+            return PADO_GPU_ERR_IO;
+        }
+        for (uint32_t i = 0; i < sz; ++i) {
+            uint32_t h = 0;
+            uint8_t d = 0;
+            f.read((char *)&h, 4);
+            f.read((char *)&d, 1);
+            hub.push_back(h);
+            dist.push_back(d);
+        }
+        off[x + 1] = hub.size();
+    }
+    if (!f.good()) {
+        set_error("truncated index file");
+        return PADO_GPU_ERR_IO;
+    }
+    // The file is in original-id order: import it as-is and mark it ordered.
+    pado_gpu_index *idx = nullptr;
+    PADO_TRY(pado_gpu_index_import(ctx, n, bd.data(), bs.data(), off.data(), hub.data(), dist.data(), &idx));
+    std::swap(idx->qbpd.p, idx->bpd.p);
+    std::swap(idx->qbpd.n, idx->bpd.n);
+    std::swap(idx->qbps.p, idx->bps.p);
+    std::swap(idx->qbps.n, idx->bps.n);
+    std::swap(idx->qoff.p, idx->loff.p);
+    std::swap(idx->qoff.n, idx->loff.n);
+    std::swap(idx->qlab.p, idx->lab.p);
+    std::swap(idx->qlab.n, idx->lab.n);
+    idx->ordered = 1;
+    *out = idx;
+    return PADO_GPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,110 @@
+// query.hip -- exact distance queries over a PADO index on gfx950.
+//
+// Reference: query_distance, includes/pado_unw_unv_para.201912311607.
+// parallel_bp_labeling.h:2283-2416:
+//   d = min over the 50 bit-parallel slots of  da + db - {2 | 1 | 0}
+//       (2 if S^-1 sets intersect, 1 if S^-1 meets S^0, else 0),
+//   d = min(d, min over common hubs h of  La(h) + Lb(h)),
+//   d >= 253 -> 255;  ids out of range -> (a == b ? 0 : 255).
+// The reference gates each slot with `int td; inti d; if (td - 2 <= d)`,
+// an unsigned comparison that skips slots with td < 2 and so over-estimates
+// distances from a bit-parallel root to itself and to its neighbours.  The
+// default here is the exact (signed) comparison; PADO_GPU_QUERY_REF_COMPAT
+// reproduces the reference's answers bit for bit.  Either way the slot order
+// does not matter: the result is a minimum over slots.
+//
+// Mapping: one wave per pair.  Lanes 0..49 evaluate one bit-parallel slot each
+// (the 850-byte records of both vertices are read coalesced), then the wave
+// intersects the two hub-sorted label lists: lanes take 64 labels of the
+// shorter list at a time and binary-search them in the longer one.
+#include "internal.hpp"
+
+namespace pado {
+namespace {
+
+__device__ inline uint32_t wave_min_u32(uint32_t x)
+{
+    for (int o = 32; o > 0; o >>= 1) {
+        uint32_t y = __shfl_xor(x, o);
+        x = y < x ? y : x;
+    }
+    return x;
+}
+
+}  // namespace
+
+__device__ inline uint32_t pair_distance(uint32_t n, const uint8_t *bpd, const ulonglong2 *bps, const uint64_t *loff,
+                                         const uint32_t *lab, uint32_t a, uint32_t b, bool ref_compat)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    if (a >= n || b >= n) return a == b ? 0u : 255u;
+    uint32_t d = 255u;
+    if (lane < kBP) {
+        uint32_t da = bpd[(uint64_t)a * kBpdStride + lane];
+        uint32_t db = bpd[(uint64_t)b * kBpdStride + lane];
+        int td = (int)(da + db);
+        if (!(ref_compat && td < 2) && td - 2 <= 255) {
+            ulonglong2 sa = bps[(uint64_t)a * kBP + lane];
+            ulonglong2 sb = bps[(uint64_t)b * kBP + lane];
+            if (sa.x & sb.x)
+                td -= 2;
+            else if ((sa.x & sb.y) | (sa.y & sb.x))
+                td -= 1;
+            if ((uint32_t)td < d) d = (uint32_t)td;
+        }
+    }
+    const uint64_t oa = loff[a], ob = loff[b];
+    const uint32_t la = (uint32_t)(loff[a + 1] - oa), lb = (uint32_t)(loff[b + 1] - ob);
+    const uint32_t *S = lab + (la <= lb ? oa : ob);
+    const uint32_t *T = lab + (la <= lb ? ob : oa);
+    const uint32_t ls = la <= lb ? la : lb, lt = la <= lb ? lb : la;
+    for (uint32_t j0 = 0; j0 < ls; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        if (j < ls) {
+            const uint32_t s = S[j];
+            const uint32_t key = s >> 8;
+            uint32_t lo = 0, hi = lt;
+            while (lo < hi) {
+                uint32_t mid = (lo + hi) >> 1;
+                if ((T[mid] >> 8) < key)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            if (lo < lt && (T[lo] >> 8) == key) {
+                uint32_t td = (s & 255u) + (T[lo] & 255u);
+                if (td < d) d = td;
+            }
+        }
+    }
+    d = wave_min_u32(d);
+    return d >= 253u ? 255u : d;
+}
+
+__global__ __launch_bounds__(256) void k_query(uint32_t n, const uint8_t *bpd, const ulonglong2 *bps,
+                                               const uint64_t *loff, const uint32_t *lab, const uint32_t *qa,
+                                               const uint32_t *qb, uint64_t k, uint8_t *out, int ref_compat)
+{
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < k; i += nw) {
+        uint32_t d = pair_distance(n, bpd, bps, loff, lab, qa[i], qb[i], ref_compat != 0);
+        if ((threadIdx.x & 63) == 0) out[i] = (uint8_t)d;
+    }
+}
+
+pado_gpu_status launch_query(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,
+                             uint64_t k, uint8_t *out, bool ref_compat, hipStream_t st)
+{
+    if (k == 0) return PADO_GPU_OK;
+    const uint8_t *bpd = rank_space ? idx->bpd.p : idx->qbpd.p;
+    const ulonglong2 *bps = (const ulonglong2 *)(rank_space ? idx->bps.p : idx->qbps.p);
+    const uint64_t *loff = rank_space ? idx->loff.p : idx->qoff.p;
+    const uint32_t *lab = rank_space ? idx->lab.p : idx->qlab.p;
+    uint32_t grid = grid_for(k * 64, 256, 65536);
+    hipLaunchKernelGGL(k_query, dim3(grid), dim3(256), 0, st, idx->n, bpd, bps, loff, lab, qa, qb, k, out,
+                       ref_compat ? 1 : 0);
+    PADO_HIP(hipGetLastError());
+    return PADO_GPU_OK;
+}
+
+}  // namespace pado

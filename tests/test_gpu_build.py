@@ -1,0 +1,104 @@
+"""GPU index build through the C ABI against the reference's golden outputs and
+the CPU oracle: bit-parallel arrays and canonical label sets must be
+bit-exact, and the per-build counters must equal the oracle's."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import parallel_distance_oracle_amd as pado
+from helpers import (RMAT_SMALL, TINY, bp_digest, have_rmat, label_digest, load_rmat, load_tiny, rank_space,
+                     rmat_edges)
+
+pytestmark = pytest.mark.gpu
+
+
+def _build(ctx, G, batch=1024):
+    print(f"[gpu] build n={G.num_v} m={G.num_e} batch={batch}", flush=True)
+    sys.stdout.flush()
+    dg = pado.DeviceGraph(ctx, G)
+    try:
+        return pado.Index.build(ctx, dg, batch)
+    finally:
+        dg.close()
+
+
+@pytest.mark.parametrize("name", TINY)
+def test_tiny_graphs_bit_exact(gpu_ctx, name):
+    g = load_tiny(name)
+    G, rank, _ = rank_space(g["edges"], g["rank"])
+    idx = _build(gpu_ctx, G)
+    bp_d, bp_s, loff, hub, dist = idx.export()
+    assert np.array_equal(bp_d, g["bp_d"])
+    assert np.array_equal(bp_s, g["bp_s"])
+    assert np.array_equal(loff, g["loff"]) and np.array_equal(hub, g["hub"]) and np.array_equal(dist, g["dist"])
+    idx.close()
+
+
+@pytest.mark.parametrize("scale,ef", RMAT_SMALL)
+def test_rmat_bit_exact_and_counters(gpu_ctx, scale, ef):
+    meta, arr = load_rmat(scale, ef)
+    G, rank, _ = rank_space(rmat_edges(scale, ef))
+    idx = _build(gpu_ctx, G)
+    bp_d, bp_s, loff, hub, dist = idx.export()
+    assert bp_digest(bp_d, bp_s) == meta["bp_sha256"]
+    assert hub.size == meta["num_labels"]
+    assert np.array_equal(loff, arr["loff"])
+    assert np.array_equal(hub, arr["hub"]) and np.array_equal(dist, arr["dist"])
+    st = idx.stats()
+    X = oracle.Index(G.num_v, G.off, G.adj)
+    assert st["bp_pruned"] == meta["bp_hit_count"]
+    for k in ("pushes", "ind_pass", "bp_pruned", "cands", "dq_pruned"):
+        assert st[k] == X.stats[k], k
+    assert st["iterations"] == X.stats["iters"]
+    idx.close()
+
+
+@pytest.mark.parametrize("batch", [1, 64, 500])
+def test_other_batch_sizes_same_labels(gpu_ctx, batch):
+    meta, arr = load_rmat(12, 8)
+    G, _, _ = rank_space(rmat_edges(12, 8))
+    idx = _build(gpu_ctx, G, batch)
+    _, _, loff, hub, dist = idx.export()
+    assert np.array_equal(loff, arr["loff"]) and np.array_equal(hub, arr["hub"]) and np.array_equal(dist, arr["dist"])
+    idx.close()
+
+
+@pytest.mark.parametrize("scale,ef", [(16, 16), (18, 16)])
+def test_rmat_digests(gpu_ctx, scale, ef):
+    if not have_rmat(scale, ef):
+        pytest.skip("digest fixture not generated")
+    meta, _ = load_rmat(scale, ef)
+    G, _, _ = rank_space(rmat_edges(scale, ef))
+    idx = _build(gpu_ctx, G)
+    bp_d, bp_s, loff, hub, dist = idx.export()
+    assert hub.size == meta["num_labels"]
+    assert bp_digest(bp_d, bp_s) == meta["bp_sha256"]
+    assert label_digest(loff, hub, dist) == meta["labels_sha256"]
+    assert idx.stats()["bp_pruned"] == meta["bp_hit_count"]
+    idx.close()
+
+
+def test_graph_with_isolated_and_duplicate_edges(gpu_ctx):
+    """Edge lists as the reference's loader sees them: isolated ids below the
+    maximum, duplicated edges and a self-loop are all kept."""
+    rng = np.random.default_rng(5)
+    e = rng.integers(0, 300, size=(900, 2)).astype(np.uint32)
+    e = np.vstack([e, e[:50], [[7, 7]], [[299, 1000]]]).astype(np.uint32)
+    G, rank, _ = rank_space(e)
+    idx = _build(gpu_ctx, G)
+    bp_d, bp_s, loff, hub, dist = idx.export()
+    X = oracle.Index(G.num_v, G.off, G.adj)
+    assert np.array_equal(bp_d, X.bp_d) and np.array_equal(bp_s, X.bp_s)
+    assert np.array_equal(loff, X.loff) and np.array_equal(hub, X.hub) and np.array_equal(dist, X.dist)
+    idx.close()
+
+
+def test_bad_graphs_are_rejected(gpu_ctx):
+    G = pado.Graph(3, np.array([0, 1, 2, 2], np.uint64), np.array([1, 0], np.uint32))
+    with pytest.raises(pado.PadoError):
+        pado.DeviceGraph(gpu_ctx, G)  # off[n] != m
+    G = pado.Graph(3, np.array([0, 2, 3, 4], np.uint64), np.array([1, 2, 0, 0], np.uint32))
+    with pytest.raises(pado.PadoError):
+        pado.DeviceGraph(gpu_ctx, G)  # adjacency of 0 not descending

@@ -1,0 +1,126 @@
+"""GPU distance queries through the C ABI: exact against BFS, bit-exact to the
+reference's query_distance in ref-compat mode, original-id ordering, device
+pointers, file round trip."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+import parallel_distance_oracle_amd as pado
+from helpers import RMAT_SMALL, TINY, load_rmat, load_tiny, rank_space, rmat_edges
+
+pytestmark = pytest.mark.gpu
+
+
+def _pll(ctx, edges, rank=None):
+    G, rank, r2i = rank_space(edges, rank)
+    print(f"[gpu] build+order n={G.num_v}", flush=True)
+    sys.stdout.flush()
+    p = pado.ParaVertexCentricPLL(G, ctx=ctx)
+    p.order_labels(r2i, rank)
+    return p, G, rank, r2i
+
+
+@pytest.mark.parametrize("name", TINY)
+def test_tiny_all_pairs(gpu_ctx, name):
+    g = load_tiny(name)
+    p, G, rank, _ = _pll(gpu_ctx, g["edges"], g["rank"])
+    qa, qb = g["qa"], g["qb"]
+    assert np.array_equal(p.index.query(qa, qb, ref_compat=True), g["answers"])
+    exact = oracle.bfs_to_u8(oracle.bfs_pairs(G.num_v, G.off, G.adj, rank[qa], rank[qb]))
+    assert np.array_equal(p.query_distance(qa, qb), exact)
+
+
+@pytest.mark.parametrize("scale,ef", RMAT_SMALL)
+def test_rmat_uniform_pairs(gpu_ctx, scale, ef):
+    meta, arr = load_rmat(scale, ef)
+    p, G, rank, _ = _pll(gpu_ctx, rmat_edges(scale, ef))
+    qa, qb = pado.uniform_pairs(G.num_v, meta["n_pairs"], seed=1)
+    ref = p.index.query(qa, qb, ref_compat=True)
+    assert np.array_equal(ref, arr["answers"])
+    got = p.query_distance(qa, qb)
+    exact = oracle.bfs_to_u8(oracle.bfs_pairs(G.num_v, G.off, G.adj, rank[qa], rank[qb]))
+    assert np.array_equal(got, exact)
+    # The two modes differ exactly on the reference's defect set.
+    assert np.all((got == ref) | (ref != exact))
+    # Rank-space queries agree with original-id queries.
+    assert np.array_equal(p.index.query(rank[qa], rank[qb], rank_space=True), got)
+
+
+def test_zipf_pairs_hit_the_defect_set(gpu_ctx):
+    """Hub-heavy pairs (Zipf over degree rank) exercise the bit-parallel roots."""
+    p, G, rank, r2i = _pll(gpu_ctx, rmat_edges(14, 8))
+    a = pado.zipf_vertices(G.num_v, 50000, 1.1, seed=2, rank2id=r2i)
+    b = pado.zipf_vertices(G.num_v, 50000, 1.1, seed=3, rank2id=r2i)
+    exact = oracle.bfs_to_u8(oracle.bfs_pairs(G.num_v, G.off, G.adj, rank[a], rank[b]))
+    assert np.array_equal(p.query_distance(a, b), exact)
+    X = oracle.Index(G.num_v, G.off, G.adj)
+    assert np.array_equal(p.index.query(a, b, ref_compat=True), X.query(rank[a], rank[b], ref_compat=True))
+
+
+def test_out_of_range_and_self(gpu_ctx):
+    p, G, rank, _ = _pll(gpu_ctx, rmat_edges(10, 8))
+    n = G.num_v
+    a = np.array([0, n, n + 5, n, 3, 3], np.uint32)
+    b = np.array([0, n, n, 2, n + 1, 3], np.uint32)
+    out = p.query_distance(a, b)
+    assert list(out) == [0, 0, 255, 255, 255, 0]
+    assert p.query_distance(5, 5) == 0
+
+
+def test_query_before_order_is_an_error(gpu_ctx):
+    G, _, _ = rank_space(rmat_edges(10, 8))
+    p = pado.ParaVertexCentricPLL(G, ctx=gpu_ctx)
+    with pytest.raises(pado.PadoError):
+        p.query_distance(np.array([1], np.uint32), np.array([2], np.uint32))
+
+
+def test_device_pointer_queries(gpu_ctx):
+    torch = pytest.importorskip("torch")
+    p, G, rank, _ = _pll(gpu_ctx, rmat_edges(12, 8))
+    qa, qb = pado.uniform_pairs(G.num_v, 30000, seed=9)
+    da = torch.from_numpy(qa.astype(np.int32)).to("cuda")
+    db = torch.from_numpy(qb.astype(np.int32)).to("cuda")
+    out = torch.zeros(qa.size, dtype=torch.uint8, device="cuda")
+    torch.cuda.synchronize()
+    p.index.query_device(da.data_ptr(), db.data_ptr(), qa.size, out.data_ptr())
+    gpu_ctx.sync()
+    assert np.array_equal(out.cpu().numpy(), p.query_distance(qa, qb))
+
+
+def test_store_load_roundtrip_and_format(gpu_ctx, tmp_path):
+    g = load_tiny("edgelist_in")
+    p, G, rank, _ = _pll(gpu_ctx, g["edges"], g["rank"])
+    path = str(tmp_path / "idx.bin")
+    p.store_index_to_file(path, rank)
+    # Byte layout of the reference's writer (UNP:1121-1180).
+    n = G.num_v
+    exp = bytearray(np.uint32(n).tobytes() + np.uint32(50).tobytes())
+    for x in range(n):
+        v = rank[x]
+        for i in range(50):
+            exp += np.uint8(g["bp_d"][v, i]).tobytes() + np.uint64(g["bp_s"][v, i, 0]).tobytes() + \
+                np.uint64(g["bp_s"][v, i, 1]).tobytes()
+        lo, hi = int(g["loff"][v]), int(g["loff"][v + 1])
+        exp += np.uint32(hi - lo).tobytes()
+        for j in range(lo, hi):
+            exp += np.uint32(g["hub"][j]).tobytes() + np.uint8(g["dist"][j]).tobytes()
+    assert open(path, "rb").read() == bytes(exp)
+    q = pado.ParaVertexCentricPLL(ctx=gpu_ctx)
+    q.load_index_from_file(path)
+    assert np.array_equal(q.query_distance(g["qa"], g["qb"]), p.query_distance(g["qa"], g["qb"]))
+    os.remove(path)
+
+
+def test_export_import_roundtrip(gpu_ctx):
+    G, rank, r2i = rank_space(rmat_edges(12, 8))
+    idx = pado.Index.build(gpu_ctx, pado.DeviceGraph(gpu_ctx, G))
+    bp_d, bp_s, loff, hub, dist = idx.export()
+    idx2 = pado.Index.import_(gpu_ctx, G.num_v, bp_d, bp_s, loff, hub, dist)
+    e2 = idx2.export()
+    for x, y in zip((bp_d, bp_s, loff, hub, dist), e2):
+        assert np.array_equal(x, y)
+    qa, qb = pado.uniform_pairs(G.num_v, 5000, seed=4)
+    assert np.array_equal(idx.query(qa, qb, rank_space=True), idx2.query(qa, qb, rank_space=True))

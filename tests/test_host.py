@@ -1,0 +1,76 @@
+"""Host-side graph ingest of the product library (no GPU needed): the
+reference's Graph / make_rank / id_transfer semantics and the generators."""
+import numpy as np
+import pytest
+
+import oracle
+import parallel_distance_oracle_amd as pado
+from helpers import RMAT_SMALL, TINY, digest, load_rmat, load_tiny
+
+
+@pytest.mark.parametrize("name", TINY)
+def test_make_rank_matches_reference(name):
+    g = load_tiny(name)
+    G = pado.Graph.from_edges(g["edges"])
+    assert np.array_equal(G.make_rank(), g["rank"])
+
+
+@pytest.mark.parametrize("scale,ef", RMAT_SMALL)
+def test_generator_and_rank_digests(scale, ef):
+    meta, _ = load_rmat(scale, ef)
+    e = pado.rmat_edges(scale, ef, 0.57, 0.19, 0.19, 1)
+    assert e.shape[0] == meta["num_edges"]
+    assert digest(e.astype(np.uint32)) == meta["edges_sha256"]
+    assert np.all(e[:, 0] < e[:, 1])  # no self-loops, normalised
+    keys = e[:, 0].astype(np.uint64) << np.uint64(32) | e[:, 1]
+    assert np.all(np.diff(keys.astype(np.int64)) > 0)  # de-duplicated
+    G = pado.Graph.from_edges(e)
+    assert G.num_v == meta["num_v"]
+    assert digest(G.make_rank()) == meta["rank_sha256"]
+
+
+@pytest.mark.parametrize("scale,ef", [(10, 8), (12, 8)])
+def test_csr_and_id_transfer_match_oracle(scale, ef):
+    e = pado.rmat_edges(scale, ef)
+    G = pado.Graph.from_edges(e)
+    n, off, adj = oracle.graph_csr(e)
+    assert G.num_v == n and np.array_equal(G.off, off) and np.array_equal(G.adj, adj)
+    rank = G.make_rank()
+    r2i = G.id_transfer(rank)
+    off2, adj2, r2i2 = oracle.id_transfer(n, off, adj, rank)
+    assert np.array_equal(G.off, off2) and np.array_equal(G.adj, adj2) and np.array_equal(r2i, r2i2)
+    for v in range(0, n, 97):
+        row = G.adj[int(G.off[v]):int(G.off[v + 1])]
+        assert np.all(np.diff(row.astype(np.int64)) <= 0)
+
+
+def test_csr_multiset_invariant():
+    """Each input edge appears once in each direction; duplicates and
+    self-loops are kept as given (includes/graph.h:138-141)."""
+    e = np.array([[0, 1], [1, 2], [0, 1], [3, 3], [2, 5]], np.uint32)
+    G = pado.Graph.from_edges(e)
+    assert G.num_v == 6
+    pairs = sorted((int(v), int(w)) for v in range(G.num_v) for w in G.adj[int(G.off[v]):int(G.off[v + 1])])
+    exp = sorted([(0, 1), (1, 0), (1, 2), (2, 1), (0, 1), (1, 0), (3, 3), (3, 3), (2, 5), (5, 2)])
+    assert pairs == exp
+
+
+def test_text_loader_matches_reference_reader(tmp_path):
+    p = tmp_path / "g.txt"
+    p.write_text("# comment\n% other\n0 1\n1 2 7\n2 0\n")
+    G = pado.Graph.from_file(str(p))
+    assert G.num_v == 3 and G.num_e == 6
+
+
+def test_generators_are_deterministic():
+    a1, b1 = pado.uniform_pairs(1000, 5000, seed=7)
+    a2, b2 = pado.uniform_pairs(1000, 5000, seed=7)
+    assert np.array_equal(a1, a2) and np.array_equal(b1, b2)
+    assert a1.max() < 1000
+    z = pado.zipf_vertices(1000, 20000, 1.1, seed=3)
+    counts = np.bincount(z, minlength=1000)
+    assert counts[0] > counts[10] > counts[500]  # heavy head
+    g = pado.grid_edges(16, 0.05, seed=1)
+    assert 0.9 * 480 < g.shape[0] <= 480
+    w = pado.edge_weights(1000, 1, 100, seed=1)
+    assert w.min() >= 1 and w.max() <= 100
