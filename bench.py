@@ -1,0 +1,266 @@
+"""PADO on one MI355X: R-MAT-20 label build seconds, plus batch query throughput.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--scale 20] [--edge-factor 16]
+                    [--pairs 10000000] [--no-cpu] [--cpu-scale 18]
+
+A step is one full index build (pado_gpu_build: 50 bit-parallel BFSes + all
+1024-root batches + the canonical sort) of the R-MAT graph, whose rank-space
+CSR is already resident in HBM when the timed region starts.  value = mean
+build seconds over the K timed steps (lower is better).  The line also carries:
+  parity     -- the last build's label / bit-parallel digests against the
+                reference digests recorded in tests/golden (R-MAT 20: the
+                reference OpenMP build's own output),
+  query      -- plain single-batch GPU queries (10M uniform pairs by default),
+                HIP-event timed on the query stream, first 100K answers checked
+                against the reference's query_distance,
+  roofline   -- the dominant build kernel's algorithmic bytes per launch over
+                its HIP-event launch time,
+  cpu_baseline -- the reference OpenMP build (oracle/_ref) on a bounded sample
+                (R-MAT 18) on this host's cores, with the GPU time for the same
+                sample beside it.
+Progress goes to stderr; the JSON line to stdout.
+"""
+import argparse
+import ctypes
+import hashlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import parallel_distance_oracle_amd as pado  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+BP_RECORD_BYTES = 50 * (1 + 8 + 8)  # 850 B: bit-parallel record of one vertex
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def digest(*arrays):
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def golden_meta(scale, ef):
+    p = os.path.join(ROOT, "tests", "golden", f"rmat{scale}_ef{ef}.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    return None
+
+
+class HipEvents:
+    """hipEvent timing on an explicit stream (the library's query stream)."""
+
+    def __init__(self):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def event(self):
+        e = ctypes.c_void_p()
+        assert self.hip.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.hip.hipEventRecord(e, ctypes.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.hip.hipEventSynchronize(b) == 0
+        t = ctypes.c_float()
+        assert self.hip.hipEventElapsedTime(ctypes.byref(t), a, b) == 0
+        return float(t.value)
+
+
+def make_graph(scale, ef):
+    t0 = time.time()
+    edges = pado.rmat_edges(scale, ef, 0.57, 0.19, 0.19, 1)
+    G = pado.Graph.from_edges(edges)
+    rank = G.make_rank()
+    rank2id = G.id_transfer(rank)
+    log(f"R-MAT {scale} ef{ef}: n={G.num_v} edges={edges.shape[0]} host ingest {time.time() - t0:.1f}s")
+    return edges, G, rank, rank2id
+
+
+def cpu_baseline(args, ctx):
+    """Reference OpenMP build on a bounded sample, GPU build of the same sample."""
+    import oracle  # noqa: E402  (baseline leg only)
+
+    if not oracle.have_ref():
+        return None
+    threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16")))
+    edges, G, rank, _ = make_graph(args.cpu_scale, args.edge_factor)
+    dg = pado.DeviceGraph(ctx, G)
+    idx = pado.Index.build(ctx, dg)
+    idx.close()
+    ctx.sync()
+    t0 = time.time()
+    idx = pado.Index.build(ctx, dg)
+    ctx.sync()
+    gpu_s = time.time() - t0
+    idx.close()
+    dg.close()
+    log(f"reference build on R-MAT {args.cpu_scale} with {threads} threads ...")
+    t0 = time.time()
+    R = oracle.run_ref(edges, rank, threads=threads, dump=False)
+    wall = time.time() - t0
+    ref_s = R.get("ref_build_best_s", wall)
+    log(f"reference R-MAT {args.cpu_scale}: {ref_s:.2f}s (GPU {gpu_s:.3f}s)")
+    return {
+        "value": round(ref_s, 4), "unit": "s", "cores": threads, "kind": "reference",
+        "sample": (f"full ParaVertexCentricPLL<1024> build of R-MAT {args.cpu_scale} ef{args.edge_factor} "
+                   f"(n={G.num_v}), constructor wall time, OMP_NUM_THREADS={threads}; "
+                   f"GPU build of the same graph in this run: {gpu_s:.4f} s"),
+        "gpu_same_sample_s": round(gpu_s, 4),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=int, default=20)
+    ap.add_argument("--edge-factor", type=int, default=16)
+    ap.add_argument("--pairs", type=int, default=10_000_000)
+    ap.add_argument("--cpu-scale", type=int, default=18)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the index build and "
+              f"queries run on a single MI355X; see DESIGN.md for how they would shard)", file=sys.stderr)
+        sys.exit(2)
+    if pado.device_count() < 1:
+        sys.exit("no GPU visible")
+
+    edges, G, rank, rank2id = make_graph(args.scale, args.edge_factor)
+    ctx = pado.Context(0)
+    dg = pado.DeviceGraph(ctx, G)
+    ctx.sync()
+
+    for w in range(args.warmup):
+        t0 = time.time()
+        idx = pado.Index.build(ctx, dg)
+        ctx.sync()
+        log(f"warmup build {w}: {time.time() - t0:.3f}s labels={idx.info.num_labels}")
+        idx.close()
+
+    times = []
+    idx = None
+    ctx.sync()
+    for s in range(args.steps):
+        if idx is not None:
+            idx.close()
+        t0 = time.perf_counter()
+        idx = pado.Index.build(ctx, dg)
+        ctx.sync()
+        dt = time.perf_counter() - t0
+        times.append(dt)
+        log(f"build {s}: {dt:.3f}s")
+    build_s = float(np.mean(times))
+    st = idx.stats()
+    log(f"stats: {json.dumps({k: (round(v, 3) if isinstance(v, float) else v) for k, v in st.items()})}")
+
+    # Parity of the last build against the reference digests.
+    bp_d, bp_s, loff, hub, dist = idx.export()
+    meta = golden_meta(args.scale, args.edge_factor)
+    parity = {"labels": int(hub.size)}
+    if meta:
+        parity["labels_match_reference"] = digest(loff.astype(np.uint64), hub.astype(np.uint32),
+                                                  dist.astype(np.uint8)) == meta["labels_sha256"]
+        parity["bp_match_reference"] = digest(bp_d.astype(np.uint8), bp_s.astype(np.uint64)) == meta["bp_sha256"]
+        parity["bp_hit_count_match_reference"] = st["bp_pruned"] == meta["bp_hit_count"]
+        parity["reference_build_s_container"] = meta.get("ref_build_s")
+    del bp_d, bp_s, hub, dist
+
+    # Roofline of the dominant labeling kernel (push or add), per launch.
+    push_bytes = 8 * st["actives"] + 4 * st["edges"] + BP_RECORD_BYTES * st["tails"]
+    add_bytes = 5 * st["dq_labels"] + 5 * st["labels_appended"]
+    kern = {
+        "k_push": (st["ms_push"], st["n_push"], push_bytes),
+        "k_add": (st["ms_add"], st["n_add"], add_bytes),
+    }
+    dom = max(kern, key=lambda k: kern[k][0])
+    ms_tot, n_launch, bytes_tot = kern[dom]
+    avg_ms = ms_tot / max(n_launch, 1)
+    achieved = (bytes_tot / max(n_launch, 1)) / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "avg_launch_us": round(avg_ms * 1e3, 2), "launches": int(n_launch),
+                "alg_bytes_per_launch": round(bytes_tot / max(n_launch, 1), 1),
+                "kernel_ms": {k: round(v[0], 2) for k, v in kern.items()}}
+
+    # Plain batch query: order by original id, k pairs on device, one launch.
+    idx.order(rank)
+    qa, qb = pado.uniform_pairs(G.num_v, args.pairs, seed=1)
+    query = None
+    try:
+        import torch
+
+        da = torch.from_numpy(qa.view(np.int32)).to("cuda")
+        db = torch.from_numpy(qb.view(np.int32)).to("cuda")
+        out = torch.zeros(args.pairs, dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()
+        ev = HipEvents()
+        e0, e1 = ev.event(), ev.event()
+        stream = ctx.stream
+        idx.query_device(da.data_ptr(), db.data_ptr(), args.pairs, out.data_ptr(), stream=stream)
+        ctx.sync()
+        qt = []
+        for _ in range(max(args.steps, 1)):
+            ev.record(e0, stream)
+            idx.query_device(da.data_ptr(), db.data_ptr(), args.pairs, out.data_ptr(), stream=stream)
+            ev.record(e1, stream)
+            qt.append(ev.elapsed_ms(e0, e1))
+        q_ms = float(np.mean(qt))
+        res = out.cpu().numpy()
+        query = {"pairs": args.pairs, "ms": round(q_ms, 3), "queries_per_s": round(args.pairs / (q_ms * 1e-3), 1)}
+        # Signed (exact) answers must equal the reference's except on its defect set.
+        k = min(100_000, args.pairs)
+        ref_mode = idx.query(qa[:k], qb[:k], ref_compat=True)
+        query["ref_compat_first_100k_match_reference"] = (meta is not None and k == meta["n_pairs"] and
+                                                          digest(ref_mode) == meta["answers_sha256"])
+        query["exact_vs_ref_compat_differ"] = int(np.count_nonzero(ref_mode != res[:k]))
+        lab_per_v = np.diff(loff.astype(np.int64))
+        qbytes = 8 + 1 + 2 * BP_RECORD_BYTES + 5 * (lab_per_v[rank[qa]] + lab_per_v[rank[qb]])
+        query["alg_bytes"] = int(qbytes.sum())
+        query["achieved_GBps"] = round(query["alg_bytes"] / (q_ms * 1e-3) / 1e9, 1)
+        log(f"query: {query}")
+    except ImportError:
+        log("torch missing: query leg skipped")
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(args, ctx)
+        except Exception as e:  # the baseline is reported, never required
+            log(f"cpu baseline failed: {e}")
+
+    line = {
+        "metric": "R-MAT-20 label build s; served queries/s at p99 latency vs plain batch query",
+        "value": round(build_s, 4), "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(build_s * 1e3, 2), "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+        "config": {"workload": f"R-MAT scale {args.scale} ef {args.edge_factor} (0.57,0.19,0.19,0.05) seed 1, "
+                               f"BATCH_SIZE=1024, BITPARALLEL_SIZE=50; full index build",
+                   "num_v": G.num_v, "num_edges": int(edges.shape[0])},
+        "parity": parity, "query": query, "roofline": roofline, "cpu_baseline": cpu,
+        "build_breakdown_ms": {"bp": round(st["ms_bp"], 1), "batches_and_sort": round(st["ms_batch"], 1)},
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

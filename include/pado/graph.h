@@ -1,0 +1,131 @@
+// pado/graph.h -- the reference's Graph / WeightedGraph classes
+// (includes/graph.h:31-424) for the GPU drop-in.  Same public fields, same
+// constructors, same make_rank / id_transfer semantics; the work is done by
+// the host functions of libpado_gpu.so (include/pado_gpu.h).
+#ifndef PADO_GPU_DROPIN_GRAPH_H_
+#define PADO_GPU_DROPIN_GRAPH_H_
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <fstream>
+#include <sstream>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "pado/globals.h"
+#include "pado_gpu.h"
+
+namespace PADO {
+
+namespace detail {
+// CSR of the reference's shape (idi offsets) <-> the ABI's u64 offsets.
+inline void to_u64(const idi *vertices, idi num_v, idi num_e, std::vector<uint64_t> &off)
+{
+    off.resize((size_t)num_v + 1);
+    for (idi v = 0; v < num_v; ++v) off[v] = vertices[v];
+    off[num_v] = num_e;
+}
+}  // namespace detail
+
+class Graph final {
+public:
+    idi num_v = 0;
+    idi num_e = 0;  // directed edge count, 2 x input lines (includes/graph.h:127)
+    idi *vertices = nullptr;
+    idi *out_edges = nullptr;
+    idi *out_degrees = nullptr;
+
+    Graph() = default;
+    explicit Graph(const char *filename)
+    {
+        std::ifstream ifin(filename);
+        if (!ifin.is_open()) {
+            fprintf(stderr, "Error: cannot open file %s\n", filename);
+            exit(EXIT_FAILURE);
+        }
+        std::string line;
+        idi head = 0, tail = 0;
+        std::vector<std::pair<idi, idi>> edge_list;
+        while (getline(ifin, line)) {
+            if (line[0] == '#' || line[0] == '%') continue;
+            std::istringstream lin(line);
+            lin >> head >> tail;
+            edge_list.push_back(std::make_pair(head, tail));
+        }
+        construct(edge_list);
+    }
+    explicit Graph(std::vector<std::pair<idi, idi>> &edge_list) { construct(edge_list); }
+    Graph(const Graph &) = delete;
+    Graph &operator=(const Graph &) = delete;
+    ~Graph()
+    {
+        free(vertices);
+        free(out_edges);
+        free(out_degrees);
+    }
+    idi get_num_v() const { return num_v; }
+    idi get_num_e() const { return num_e; }
+    idi ith_get_edge(idi i, idi e) const { return out_edges[vertices[i] + e]; }
+    idi ith_get_out_degree(idi i) const { return out_degrees[i]; }
+
+    void construct(const std::vector<std::pair<idi, idi>> &edge_list)
+    {
+        std::vector<uint32_t> flat(2 * edge_list.size());
+        for (size_t i = 0; i < edge_list.size(); ++i) {
+            flat[2 * i] = edge_list[i].first;
+            flat[2 * i + 1] = edge_list[i].second;
+        }
+        num_e = (idi)(2 * edge_list.size());
+        num_v = edge_list.empty() ? 0 : pado_graph_num_v(flat.data(), edge_list.size());
+        std::vector<uint64_t> off((size_t)num_v + 1, 0);
+        vertices = (idi *)malloc(sizeof(idi) * (num_v ? num_v : 1));
+        out_edges = (idi *)malloc(sizeof(idi) * (num_e ? num_e : 1));
+        out_degrees = (idi *)malloc(sizeof(idi) * (num_v ? num_v : 1));
+        if (num_v) pado_graph_csr(flat.data(), edge_list.size(), num_v, off.data(), out_edges);
+        for (idi v = 0; v < num_v; ++v) {
+            vertices[v] = (idi)off[v];
+            out_degrees[v] = (idi)(off[v + 1] - off[v]);
+        }
+    }
+
+    // Degree order with rand() jitter, consuming the process-wide rand() as
+    // the reference does (includes/graph.h:156-174).
+    std::vector<idi> make_rank() const
+    {
+        std::vector<uint64_t> off;
+        detail::to_u64(vertices, num_v, num_e, off);
+        std::vector<idi> rank(num_v);
+        pado_make_rank(off.data(), num_v, 1, rank.data());
+        return rank;
+    }
+
+    // Renumber into rank space, adjacency sorted descending; returns rank2id
+    // (includes/graph.h:176-206).
+    std::vector<idi> id_transfer(const std::vector<idi> &rank)
+    {
+        std::vector<uint64_t> off, off2((size_t)num_v + 1);
+        detail::to_u64(vertices, num_v, num_e, off);
+        std::vector<idi> adj2(num_e ? num_e : 1), rank2id(num_v);
+        pado_id_transfer(num_v, off.data(), out_edges, rank.data(), off2.data(), adj2.data(), rank2id.data());
+        for (idi v = 0; v < num_v; ++v) {
+            vertices[v] = (idi)off2[v];
+            out_degrees[v] = (idi)(off2[v + 1] - off2[v]);
+        }
+        if (num_e) memcpy(out_edges, adj2.data(), sizeof(idi) * num_e);
+        return rank2id;
+    }
+
+    void print()
+    {
+        printf("num_v: %u, num_e: %u\n", num_v, num_e / 2);
+        for (idi h = 0; h < num_v; ++h)
+            for (idi e = vertices[h]; e < vertices[h] + out_degrees[h]; ++e) printf("%u %u\n", h, out_edges[e]);
+    }
+};
+
+}  // namespace PADO
+
+#endif  // PADO_GPU_DROPIN_GRAPH_H_

@@ -133,6 +133,15 @@ typedef struct {
     uint64_t cands;            /* candidates reaching distance_query */
     uint64_t dq_pruned;        /* candidates pruned by distance_query */
     uint64_t dq_rows;          /* root-distance rows scanned */
+    /* Units of the labeling kernels' algorithmic byte counts: */
+    uint64_t actives;          /* active vertices, summed over iterations */
+    uint64_t edges;            /* pushed edges (tails >= roots_start) */
+    uint64_t tails;            /* distinct tails with a fresh indicator bit, per iteration, summed */
+    uint64_t dq_labels;        /* labels with dist < iter read by distance queries */
+    uint64_t labels_appended;  /* labels written (== num_labels) */
+    /* HIP-event times of the labeling kernels on the build stream: */
+    double ms_push, ms_add, ms_bp_levels;
+    uint64_t n_push, n_add, n_bp_levels;  /* launches */
 } pado_gpu_index_info;
 
 pado_gpu_status pado_gpu_index_get_info(const pado_gpu_index *idx, pado_gpu_index_info *info);

@@ -50,6 +50,10 @@ class IndexInfo(ctypes.Structure):
         ("ms_bp", ctypes.c_double), ("ms_batch", ctypes.c_double), ("bp_roots", ctypes.c_uint32 * 50),
         ("pushes", ctypes.c_uint64), ("ind_pass", ctypes.c_uint64), ("bp_pruned", ctypes.c_uint64),
         ("cands", ctypes.c_uint64), ("dq_pruned", ctypes.c_uint64), ("dq_rows", ctypes.c_uint64),
+        ("actives", ctypes.c_uint64), ("edges", ctypes.c_uint64), ("tails", ctypes.c_uint64),
+        ("dq_labels", ctypes.c_uint64), ("labels_appended", ctypes.c_uint64),
+        ("ms_push", ctypes.c_double), ("ms_add", ctypes.c_double), ("ms_bp_levels", ctypes.c_double),
+        ("n_push", ctypes.c_uint64), ("n_add", ctypes.c_uint64), ("n_bp_levels", ctypes.c_uint64),
     ]
 
 
@@ -339,8 +343,7 @@ class Index:
 
     def stats(self):
         i = self.get_info()
-        return {k: getattr(i, k) for k in ("num_labels", "iterations", "ms_bp", "ms_batch", "bp_max_level",
-                                            "pushes", "ind_pass", "bp_pruned", "cands", "dq_pruned", "dq_rows")}
+        return {k: getattr(i, k) for k, _ in IndexInfo._fields_ if k != "bp_roots"}
 
     def export(self):
         """Canonical rank-space arrays: (bp_d[n,50], bp_s[n,50,2], loff[n+1], hub, dist)."""

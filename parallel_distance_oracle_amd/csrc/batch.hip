@@ -270,18 +270,19 @@ __global__ __launch_bounds__(256) void k_push(uint32_t iter, uint32_t rs, uint32
                                               const unsigned long long *ag, const uint32_t *amask,
                                               const uint32_t *aend, const unsigned long long *chunks,
                                               unsigned long long *ind, unsigned long long *cand, uint32_t *tmark,
-                                              uint32_t *tlist, uint32_t *cmark, uint32_t *clist,
+                                              uint32_t *tlist, uint32_t *cmark, uint32_t *clist, uint32_t *imark,
                                               const uint8_t *bpd, const ulonglong2 *bps, BuildCounters *C)
 {
     const uint32_t lane = threadIdx.x & (kChunk - 1);
     const uint32_t ngroups = (gridDim.x * blockDim.x) / kChunk;
     const uint32_t nc = C->n_chunk;
-    unsigned long long n_push = 0, n_pass = 0, n_bp = 0, n_cand = 0;
+    unsigned long long n_push = 0, n_pass = 0, n_bp = 0, n_cand = 0, n_edge = 0, n_tail = 0;
     for (uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) / kChunk; c < nc; c += ngroups) {
         const unsigned long long desc = chunks[c];
         const uint32_t k = (uint32_t)desc;
         const uint32_t e = (uint32_t)(desc >> 32) + lane;
         if (e >= aend[k]) continue;
+        ++n_edge;
         const uint32_t v = alist[k];
         const uint32_t t = adj[off[v] + e];
         const uint32_t lim = t - rs;  // roots with offset < lim satisfy root < tail (PARA:1134)
@@ -324,6 +325,9 @@ __global__ __launch_bounds__(256) void k_push(uint32_t iter, uint32_t rs, uint32
         if (touched && tmark[t] != gen_batch) {
             if (atomicExch(&tmark[t], gen_batch) != gen_batch) tlist[atomicAdd(&C->n_touched, 1u)] = t;
         }
+        if (touched && imark[t] != gen_iter) {
+            if (atomicExch(&imark[t], gen_iter) != gen_iter) ++n_tail;
+        }
         if (got && cmark[t] != gen_iter) {
             if (atomicExch(&cmark[t], gen_iter) != gen_iter) clist[atomicAdd(&C->n_cand, 1u)] = t;
         }
@@ -332,7 +336,11 @@ __global__ __launch_bounds__(256) void k_push(uint32_t iter, uint32_t rs, uint32
     n_pass = wave_sum_u64(n_pass);
     n_bp = wave_sum_u64(n_bp);
     n_cand = wave_sum_u64(n_cand);
+    n_edge = wave_sum_u64(n_edge);
+    n_tail = wave_sum_u64(n_tail);
     if ((threadIdx.x & 63) == 0) {
+        if (n_edge) atomicAdd(&C->edges, n_edge);
+        if (n_tail) atomicAdd(&C->tails, n_tail);
         if (n_push) atomicAdd(&C->pushes, n_push);
         if (n_pass) atomicAdd(&C->ind_pass, n_pass);
         if (n_bp) atomicAdd(&C->bp_pruned, n_bp);
@@ -372,7 +380,7 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, uint32_t rs, uint32_
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     const uint32_t ncand = C->n_cand;
-    unsigned long long n_dq = 0, n_lab = 0, n_rows = 0;
+    unsigned long long n_dq = 0, n_lab = 0, n_rows = 0, n_read = 0;
     for (uint32_t i = wave; i < ncand; i += nwaves) {
         const uint32_t v = clist[i];
         const unsigned long long word = cand[(uint64_t)v * kWords + (lane >> 2)];
@@ -392,7 +400,10 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, uint32_t rs, uint32_
                 const uint32_t lab = S.pool[base + j];
                 h = lab >> 8;
                 d = lab & 255u;
-                if (d < iter) info = hinfo[h];
+                if (d < iter) {
+                    info = hinfo[h];
+                    ++n_read;
+                }
             }
             const bool pass = info != kEmpty && d + (info & 255u) <= iter;
             unsigned long long m = __ballot(pass);
@@ -412,7 +423,6 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, uint32_t rs, uint32_
                     if (x <= thr && hh < my_lo + q) prune |= 1u << q;
                 }
             }
-            if (__all((cb & ~prune) == 0)) break;
         }
         const uint32_t surv = cb & ~prune;
         n_dq += __popc(cb & prune);
@@ -464,7 +474,9 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, uint32_t rs, uint32_
         emit_push_chunks(k, pdeg, C, chunks);
     }
     n_dq = wave_sum_u64(n_dq);
+    n_read = wave_sum_u64(n_read);
     if (lane == 0) {
+        if (n_read) atomicAdd(&C->dq_labels, n_read);
         if (n_dq) atomicAdd(&C->dq_pruned, n_dq);
         if (n_lab) atomicAdd(&C->labels, n_lab);
         if (n_rows) atomicAdd(&C->dq_rows, (uint32_t)n_rows);
@@ -566,9 +578,10 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     }
 
     DevBuf<unsigned long long> ind, cand, ag, chunks, lbase;
-    DevBuf<uint32_t> tmark, tlist, cmark, clist, alist, amask, aend, hinfo, hlist, pool, llen, lcap;
+    DevBuf<uint32_t> tmark, tlist, cmark, clist, imark, alist, amask, aend, hinfo, hlist, pool, llen, lcap;
     DevBuf<uint8_t> dc;
     DevBuf<BuildCounters> ctr;
+    PADO_TRY(imark.alloc(n));
     PADO_TRY(ind.alloc((uint64_t)n * kWords));
     PADO_TRY(cand.alloc((uint64_t)n * kWords));
     PADO_TRY(ag.alloc((uint64_t)n * kWords));
@@ -596,6 +609,7 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, cand.p, (uint64_t)n * kWords);
     hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, tmark.p, (uint64_t)n, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, cmark.p, (uint64_t)n, 0u);
+    hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, imark.p, (uint64_t)n, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, hinfo.p, (uint64_t)n, kEmpty);
     hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, llen.p, (uint64_t)n, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, lcap.p, (uint64_t)n, 0u);
@@ -608,6 +622,20 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
         BuildCounters *p;
         ~PinnedFree() { (void)hipHostFree(p); }
     } pin{h_ctr};
+
+    // HIP events around the two per-iteration kernels, read after the
+    // per-iteration synchronisation the loop needs anyway.
+    hipEvent_t ev[4];
+    for (auto &e : ev) PADO_HIP(hipEventCreate(&e));
+    struct EventFree {
+        hipEvent_t *e;
+        ~EventFree()
+        {
+            for (int i = 0; i < 4; ++i) (void)hipEventDestroy(e[i]);
+        }
+    } evf{ev};
+    KernelTimes kt{};
+    uint64_t actives = 0;
 
     LabelStore S{pool.p, lbase.p, llen.p, lcap.p, pool_size};
     const uint32_t nb = (n + batch - 1) / batch;
@@ -648,15 +676,28 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
             }
             ++gen_iter;
             ++iterations;
+            actives += n_act;
+            PADO_HIP(hipEventRecord(ev[0], st));
             hipLaunchKernelGGL(k_push, dim3(push_grid), dim3(256), 0, st, iter, rs, gen_batch, gen_iter, g->off.p,
                                g->adj.p, alist.p, ag.p, amask.p, aend.p, chunks.p, ind.p, cand.p, tmark.p, tlist.p,
-                               cmark.p, clist.p, idx->bpd.p, (const ulonglong2 *)idx->bps.p, ctr.p);
+                               cmark.p, clist.p, imark.p, idx->bpd.p, (const ulonglong2 *)idx->bps.p, ctr.p);
+            PADO_HIP(hipEventRecord(ev[1], st));
             hipLaunchKernelGGL(k_mid, dim3(1), dim3(1), 0, st, ctr.p);
+            PADO_HIP(hipEventRecord(ev[2], st));
             hipLaunchKernelGGL(k_add, dim3(add_grid), dim3(256), 0, st, iter, rs, rsz, d_used.p, g->off.p, g->adj.p,
                                clist.p, cand.p, S, hinfo.p, dc.p, alist.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
+            PADO_HIP(hipEventRecord(ev[3], st));
             hipLaunchKernelGGL(k_post, dim3(1), dim3(1), 0, st, ctr.p);
             PADO_HIP(hipMemcpyAsync(h_ctr, ctr.p, sizeof(BuildCounters), hipMemcpyDeviceToHost, st));
             PADO_HIP(hipStreamSynchronize(st));
+            PADO_HIP(hipGetLastError());
+            float t_push = 0.f, t_add = 0.f;
+            PADO_HIP(hipEventElapsedTime(&t_push, ev[0], ev[1]));
+            PADO_HIP(hipEventElapsedTime(&t_add, ev[2], ev[3]));
+            kt.ms_push += t_push;
+            kt.ms_add += t_add;
+            kt.n_push += 1;
+            kt.n_add += 1;
             if (h_ctr->overflow) {
                 set_error("label pool exhausted");
                 return PADO_GPU_ERR_NOMEM;
@@ -678,7 +719,12 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     PADO_HIP(hipMemcpyAsync(h_ctr, ctr.p, sizeof(BuildCounters), hipMemcpyDeviceToHost, st));
     PADO_HIP(hipStreamSynchronize(st));
     idx->counters = *h_ctr;
+    idx->counters.actives = actives;
     idx->iterations = iterations;
+    idx->ktimes.ms_push = kt.ms_push;
+    idx->ktimes.ms_add = kt.ms_add;
+    idx->ktimes.n_push = kt.n_push;
+    idx->ktimes.n_add = kt.n_add;
 
     // Canonical CSR: gather each vertex's labels and sort them by hub
     // (order_labels' per-vertex std::sort, PARA:2240).

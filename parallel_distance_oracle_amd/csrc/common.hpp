@@ -89,7 +89,16 @@ struct BuildCounters {
     unsigned long long cands;     // (vertex, root) candidates after the BP check
     unsigned long long dq_pruned; // candidates pruned by the distance query
     unsigned long long labels;    // labels appended
-    unsigned long long pad1;
+    unsigned long long edges;     // push edges scanned (tails >= roots_start)
+    unsigned long long tails;     // distinct tails with a fresh indicator bit, summed over iterations
+    unsigned long long dq_labels; // labels with dist < iter read by distance queries
+    unsigned long long actives;   // active vertices, summed over iterations
+};
+
+// Host-side timing of the labeling kernels (HIP events on the build stream).
+struct KernelTimes {
+    double ms_push = 0, ms_add = 0, ms_bp = 0;
+    uint64_t n_push = 0, n_add = 0, n_bp = 0;
 };
 
 }  // namespace pado
@@ -145,4 +154,5 @@ struct pado_gpu_index {
     uint32_t bp_max_level = 0;
     uint64_t iterations = 0;
     pado::BuildCounters counters{};
+    pado::KernelTimes ktimes{};
 };

@@ -314,6 +314,17 @@ pado_gpu_status pado_gpu_index_get_info(const pado_gpu_index *idx, pado_gpu_inde
     info->cands = idx->counters.cands;
     info->dq_pruned = idx->counters.dq_pruned;
     info->dq_rows = idx->counters.dq_rows;
+    info->actives = idx->counters.actives;
+    info->edges = idx->counters.edges;
+    info->tails = idx->counters.tails;
+    info->dq_labels = idx->counters.dq_labels;
+    info->labels_appended = idx->counters.labels;
+    info->ms_push = idx->ktimes.ms_push;
+    info->ms_add = idx->ktimes.ms_add;
+    info->ms_bp_levels = idx->ktimes.ms_bp;
+    info->n_push = idx->ktimes.n_push;
+    info->n_add = idx->ktimes.n_add;
+    info->n_bp_levels = idx->ktimes.n_bp;
     return PADO_GPU_OK;
 }
 
