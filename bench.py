@@ -127,6 +127,111 @@ def cpu_baseline(args, ctx):
     }
 
 
+def serving_leg(ctx, scale=14, ef=8, duration_s=4.0, seed=5):
+    """Config 3: the R-MAT 14 index resident; open-loop Poisson arrivals at
+    fixed rates; each request carries floor(exp(U(0, ln 65536))) pairs, half
+    uniform, half Zipf(1.1) over degree rank.  Latency = completion - arrival
+    (admission waits included).  Compared with the plain path: the same pairs
+    in one query launch."""
+    import threading
+
+    edges, G, rank, rank2id = make_graph(scale, ef)
+    idx = pado.Index.build(ctx, pado.DeviceGraph(ctx, G))
+    idx.order(rank)
+    n = G.num_v
+    rng = np.random.default_rng(seed)
+    pool_n = 4000
+    sizes = np.floor(np.exp(rng.uniform(0, np.log(65536), pool_n))).astype(np.int64)
+    sizes = np.maximum(sizes, 1)
+    tot = int(sizes.sum())
+    ua, ub = pado.uniform_pairs(n, tot, seed=seed)
+    za = pado.zipf_vertices(n, tot, 1.1, seed + 1, rank2id)
+    zb = pado.zipf_vertices(n, tot, 1.1, seed + 2, rank2id)
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    reqs = []
+    for i in range(pool_n):
+        lo, hi = offs[i], offs[i + 1]
+        h = (hi - lo) // 2
+        a = np.concatenate([ua[lo:lo + h], za[lo + h:hi]]).astype(np.uint32)
+        b = np.concatenate([ub[lo:lo + h], zb[lo + h:hi]]).astype(np.uint32)
+        reqs.append((a, b))
+    mean_pairs = tot / pool_n
+
+    # Plain single-batch path on the same pairs (host arrays, one launch).
+    allA = np.concatenate([r[0] for r in reqs])
+    allB = np.concatenate([r[1] for r in reqs])
+    idx.query(allA[:1000], allB[:1000])
+    t0 = time.perf_counter()
+    plain = idx.query(allA, allB)
+    plain_s = time.perf_counter() - t0
+    plain_qps = allA.size / plain_s
+
+    srv = pado.Server(ctx, idx, staging_pairs=8 << 20, max_launch_pairs=2 << 20, streams=3)
+    # warm
+    r = srv.submit(reqs[0][0], reqs[0][1])
+    srv.wait(r, reqs[0][0].size)
+    # Saturation probe: closed loop with everything submitted at once.
+    rates = []
+    results = []
+    ok_all = True
+    for frac in (0.2, 0.4, 0.6, 0.8):
+        rate_pairs = frac * plain_qps
+        rate_req = rate_pairs / mean_pairs
+        gaps = rng.exponential(1.0 / rate_req, size=pool_n)
+        arrivals = np.cumsum(gaps)
+        arrivals = arrivals[arrivals < duration_s]
+        m = arrivals.size
+        lat = np.zeros(m)
+        ids = [None] * m
+        sub_t = np.zeros(m)
+        done_evt = threading.Event()
+
+        def waiter():
+            for i in range(m):
+                while ids[i] is None:
+                    time.sleep(0.0001)
+                out, l_ms = srv.wait(ids[i], reqs[i % pool_n][0].size)
+                lat[i] = (sub_t[i] - arrivals[i]) * 1e3 + l_ms
+                if not np.array_equal(out, plain[offs[i % pool_n]:offs[i % pool_n + 1]]):
+                    results.append(i)
+            done_evt.set()
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t_start = time.perf_counter()
+        for i in range(m):
+            due = t_start + arrivals[i]
+            while time.perf_counter() < due:
+                pass
+            a, b = reqs[i % pool_n]
+            while True:
+                try:
+                    rid = srv.submit(a, b)
+                    break
+                except pado.Busy:
+                    time.sleep(0.00005)
+            sub_t[i] = time.perf_counter() - t_start
+            ids[i] = rid
+        th.join(timeout=120)
+        wall = time.perf_counter() - t_start
+        pairs = float(sum(reqs[i % pool_n][0].size for i in range(m)))
+        rates.append({"offered_frac_of_plain": frac, "offered_req_per_s": round(rate_req, 1),
+                      "requests": int(m), "served_queries_per_s": round(pairs / wall, 1),
+                      "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                      "p99_ms": round(float(np.percentile(lat, 99)), 3)})
+        log(f"serving {rates[-1]}")
+    st = srv.stats()
+    srv.stop()
+    ok_all = not results
+    best = max(rates, key=lambda r: r["served_queries_per_s"])
+    return {"workload": f"R-MAT {scale} ef{ef} index resident; open-loop Poisson; log-uniform 1..65536 pairs/request;"
+                        f" half uniform, half Zipf(1.1) over degree rank",
+            "plain_batch_queries_per_s": round(plain_qps, 1), "plain_batch_pairs": int(allA.size),
+            "rates": rates, "best_served_queries_per_s": best["served_queries_per_s"],
+            "p99_ms_at_best": best["p99_ms"], "results_equal_plain_batch": ok_all,
+            "launches": st["launches"], "requests": st["requests_done"], "rejected": st["rejected"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -137,6 +242,7 @@ def main():
     ap.add_argument("--pairs", type=int, default=10_000_000)
     ap.add_argument("--cpu-scale", type=int, default=18)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the index build and "
@@ -241,6 +347,13 @@ def main():
     except ImportError:
         log("torch missing: query leg skipped")
 
+    serving = None
+    if not args.no_serve:
+        try:
+            serving = serving_leg(ctx)
+        except Exception as e:
+            log(f"serving leg failed: {e}")
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -256,7 +369,7 @@ def main():
         "config": {"workload": f"R-MAT scale {args.scale} ef {args.edge_factor} (0.57,0.19,0.19,0.05) seed 1, "
                                f"BATCH_SIZE=1024, BITPARALLEL_SIZE=50; full index build",
                    "num_v": G.num_v, "num_edges": int(edges.shape[0])},
-        "parity": parity, "query": query, "roofline": roofline, "cpu_baseline": cpu,
+        "parity": parity, "query": query, "serving": serving, "roofline": roofline, "cpu_baseline": cpu,
         "build_breakdown_ms": {"bp": round(st["ms_bp"], 1), "batches_and_sort": round(st["ms_batch"], 1)},
     }
     print(json.dumps(line), flush=True)

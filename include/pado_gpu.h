@@ -185,6 +185,54 @@ pado_gpu_status pado_gpu_index_store(pado_gpu_ctx *ctx, const pado_gpu_index *id
  * copy. */
 pado_gpu_status pado_gpu_index_load(pado_gpu_ctx *ctx, const char *filename, pado_gpu_index **out);
 
+/* ---------------------------------------------------------------------------
+ * Serving: a request loop over a resident index (no reference counterpart;
+ * it turns query_distance, PARA:2283-2416, into a continuously batched
+ * service).  Requests carry any number of (a, b) pairs.  submit() admits a
+ * request only if a request slot and staging memory for its pairs are free
+ * (PADO_GPU_ERR_BUSY otherwise); a dispatcher thread coalesces admitted
+ * requests into query launches on several streams, so requests that arrive
+ * while others run are admitted as slots and memory free up and their work
+ * overlaps on the device.  Each answer is computed per pair, so it is the
+ * same alone and in any mix.
+ * ------------------------------------------------------------------------ */
+typedef struct pado_gpu_server pado_gpu_server;
+
+typedef struct {
+    uint64_t staging_pairs;    /* pair capacity of the admission ring; 0 = derived from free memory */
+    uint64_t max_launch_pairs; /* coalescing cap per launch (0 = 4M) */
+    uint32_t max_requests;     /* request slots (0 = 4096) */
+    uint32_t streams;          /* launches in flight (0 = 3) */
+    uint32_t flags;            /* PADO_GPU_QUERY_REF_COMPAT or 0 */
+    uint32_t reserved[5];
+} pado_gpu_serve_cfg;
+
+typedef struct {
+    uint64_t requests_done;
+    uint64_t pairs_done;
+    uint64_t launches;
+    uint64_t rejected;         /* submit() calls refused with PADO_GPU_ERR_BUSY */
+    uint64_t staging_pairs;    /* admission ring capacity in pairs */
+    double busy_ms;            /* sum of launch (copy + kernel) times */
+} pado_gpu_serve_stats;
+
+/* The index must be ordered (pado_gpu_index_order); it stays resident and
+ * must outlive the server. */
+pado_gpu_status pado_gpu_serve_start(pado_gpu_ctx *ctx, pado_gpu_index *idx, const pado_gpu_serve_cfg *cfg,
+                                     pado_gpu_server **out);
+/* Copies the k pairs into the admission ring and queues them; *req_id
+ * identifies the request.  k may be 0. */
+pado_gpu_status pado_gpu_serve_submit(pado_gpu_server *srv, const uint32_t *a, const uint32_t *b, uint64_t k,
+                                      uint64_t *req_id);
+/* *done = 1 once the request's answers are ready. */
+pado_gpu_status pado_gpu_serve_poll(pado_gpu_server *srv, uint64_t req_id, int *done);
+/* Blocks until done, copies the k answers to out, releases the slot.
+ * latency_ms (nullable) = completion time - submit time. */
+pado_gpu_status pado_gpu_serve_wait(pado_gpu_server *srv, uint64_t req_id, uint8_t *out, double *latency_ms);
+pado_gpu_status pado_gpu_serve_get_stats(pado_gpu_server *srv, pado_gpu_serve_stats *st);
+/* Drains in-flight launches and joins the dispatcher. */
+void pado_gpu_serve_stop(pado_gpu_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

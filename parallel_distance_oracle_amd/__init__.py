@@ -43,6 +43,17 @@ class BuildCfg(ctypes.Structure):
     _fields_ = [("batch_size", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 6)]
 
 
+class ServeCfg(ctypes.Structure):
+    _fields_ = [("staging_pairs", ctypes.c_uint64), ("max_launch_pairs", ctypes.c_uint64),
+                ("max_requests", ctypes.c_uint32), ("streams", ctypes.c_uint32), ("flags", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 5)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("requests_done", ctypes.c_uint64), ("pairs_done", ctypes.c_uint64), ("launches", ctypes.c_uint64),
+                ("rejected", ctypes.c_uint64), ("staging_pairs", ctypes.c_uint64), ("busy_ms", ctypes.c_double)]
+
+
 class IndexInfo(ctypes.Structure):
     _fields_ = [
         ("num_v", ctypes.c_uint32), ("weighted", ctypes.c_uint32), ("num_labels", ctypes.c_uint64),
@@ -98,6 +109,12 @@ def lib():
         "pado_gpu_index_free": (None, [_vp]),
         "pado_gpu_index_store": (ctypes.c_int, [_vp, _vp, ctypes.c_char_p, _vp, _u32]),
         "pado_gpu_index_load": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_vp)]),
+        "pado_gpu_serve_start": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ServeCfg), ctypes.POINTER(_vp)]),
+        "pado_gpu_serve_submit": (ctypes.c_int, [_vp, _vp, _vp, _u64, ctypes.POINTER(_u64)]),
+        "pado_gpu_serve_poll": (ctypes.c_int, [_vp, _u64, ctypes.POINTER(ctypes.c_int)]),
+        "pado_gpu_serve_wait": (ctypes.c_int, [_vp, _u64, _vp, ctypes.POINTER(ctypes.c_double)]),
+        "pado_gpu_serve_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ServeStats)]),
+        "pado_gpu_serve_stop": (None, [_vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -388,6 +405,66 @@ class Index:
     def __del__(self):
         try:
             self.close()
+        except Exception:
+            pass
+
+
+class Busy(PadoError):
+    pass
+
+
+class Server:
+    """Continuous-batching query service over an ordered, resident Index."""
+
+    def __init__(self, ctx, index, staging_pairs=0, max_launch_pairs=0, max_requests=0, streams=0,
+                 ref_compat=False):
+        cfg = ServeCfg()
+        cfg.staging_pairs = staging_pairs
+        cfg.max_launch_pairs = max_launch_pairs
+        cfg.max_requests = max_requests
+        cfg.streams = streams
+        cfg.flags = QUERY_REF_COMPAT if ref_compat else 0
+        h = _vp()
+        _check(lib().pado_gpu_serve_start(ctx.h, index.h, ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self.index = index  # keep the index alive while serving
+
+    def submit(self, a, b):
+        """Returns a request id; raises Busy when admission refuses."""
+        a = _c(a, np.uint32).reshape(-1)
+        b = _c(b, np.uint32).reshape(-1)
+        rid = _u64()
+        st = lib().pado_gpu_serve_submit(self.h, _p(a), _p(b), a.size, ctypes.byref(rid))
+        if st == 7:
+            raise Busy(st, lib().pado_gpu_last_error().decode())
+        _check(st)
+        return rid.value
+
+    def poll(self, rid):
+        d = ctypes.c_int()
+        _check(lib().pado_gpu_serve_poll(self.h, rid, ctypes.byref(d)))
+        return bool(d.value)
+
+    def wait(self, rid, k):
+        """Blocks; returns (answers u8[k], latency_ms from submit to completion)."""
+        out = np.zeros(max(k, 1), np.uint8)
+        lat = ctypes.c_double()
+        _check(lib().pado_gpu_serve_wait(self.h, rid, _p(out), ctypes.byref(lat)))
+        return out[:k], lat.value
+
+    def stats(self):
+        s = ServeStats()
+        _check(lib().pado_gpu_serve_get_stats(self.h, ctypes.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServeStats._fields_}
+
+    def stop(self):
+        if self.h:
+            lib().pado_gpu_serve_stop(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.stop()
         except Exception:
             pass
 

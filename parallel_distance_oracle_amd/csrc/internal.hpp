@@ -10,6 +10,9 @@ namespace pado {
 // used-root flags (device + host copies).
 pado_gpu_status build_bp(pado_gpu_graph *g, pado_gpu_index *idx, DevBuf<uint8_t> &d_used,
                          std::vector<uint8_t> &h_used);
+// Same result, all 50 slots in one multi-source sweep (bp_ms.hip).
+pado_gpu_status build_bp_multi(pado_gpu_graph *g, pado_gpu_index *idx, DevBuf<uint8_t> &d_used,
+                               std::vector<uint8_t> &h_used);
 
 // Batched pruned labeling (PARA:848-1836).  Produces the canonical rank-space
 // label CSR idx->loff / idx->lab.

@@ -1,0 +1,376 @@
+// serve.hip -- continuous-batching query service over a resident index.
+//
+// query_distance (includes/pado_unw_unv_para.201912311607.parallel_bp_labeling.h:
+// 2283-2416) answers one pair per call.  Here requests with any number of
+// pairs are admitted into a pinned staging ring (admission = a free request
+// slot + ring space, i.e. device staging memory), a dispatcher thread packs the
+// requests queued at that moment into one launch of the query kernel per
+// stream (up to `streams` launches in flight), and a completer thread marks
+// requests done in launch order.  A request that arrives while launches run
+// joins the next launch; its pairs overlap on the device with the running
+// ones.  Every answer depends only on its own pair, so a request's result is
+// the same alone and in any mix.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+#include "internal.hpp"
+
+namespace pado {
+pado_gpu_status launch_query(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,
+                             uint64_t k, uint8_t *out, bool ref_compat, hipStream_t st);
+}
+
+using namespace pado;
+
+namespace {
+
+double now_ms()
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+    uint64_t id = 0;
+    uint64_t start = 0;  // monotonic ring position of the first pair
+    uint64_t k = 0;
+    int state = 0;       // 0 queued, 1 launched, 2 done, 3 consumed
+    double t_submit = 0, t_done = 0;
+};
+
+struct Launch {
+    std::vector<Request *> reqs;
+    hipEvent_t ev_start = nullptr, ev_end = nullptr;
+    int stream = 0;
+};
+
+}  // namespace
+
+struct pado_gpu_server {
+    pado_gpu_ctx *ctx = nullptr;
+    pado_gpu_index *idx = nullptr;
+    pado_gpu_serve_cfg cfg{};
+    uint64_t cap = 0;               // ring capacity in pairs
+    uint64_t head = 0, tail = 0;    // monotonic ring positions (released / reserved)
+    uint32_t *h_a = nullptr, *h_b = nullptr;
+    uint8_t *h_out = nullptr;
+    DevBuf<uint32_t> d_a, d_b;
+    DevBuf<uint8_t> d_out;
+    std::vector<hipStream_t> streams;
+    std::vector<int> stream_busy;
+    std::deque<Request *> live;     // ring order, not yet released
+    std::deque<Request *> pending;  // admitted, not launched
+    std::unordered_map<uint64_t, Request *> by_id;
+    std::deque<Launch> inflight;    // issue order
+    uint64_t next_id = 1;
+    bool stopping = false;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done, cv_launch;
+    std::thread dispatcher, completer;
+    pado_gpu_serve_stats st{};
+    std::string error;
+};
+
+namespace {
+
+// Pairs of [start, start+k) must not wrap: a request that would cross the
+// physical end starts at the next lap.
+uint64_t place(uint64_t tail, uint64_t k, uint64_t cap)
+{
+    uint64_t phys = tail % cap;
+    if (phys + k > cap) return tail + (cap - phys);
+    return tail;
+}
+
+void release_consumed(pado_gpu_server *s)
+{
+    while (!s->live.empty() && s->live.front()->state == 3) {
+        Request *r = s->live.front();
+        s->live.pop_front();
+        s->by_id.erase(r->id);
+        s->head = r->start + r->k;
+        delete r;
+    }
+    if (s->live.empty()) s->head = s->tail;
+}
+
+void dispatcher_loop(pado_gpu_server *s)
+{
+    (void)hipSetDevice(s->ctx->device);
+    const uint64_t max_launch = s->cfg.max_launch_pairs;
+    for (;;) {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_work.wait(lk, [&] {
+            bool free_stream = std::find(s->stream_busy.begin(), s->stream_busy.end(), 0) != s->stream_busy.end();
+            return (s->stopping && s->pending.empty()) || (!s->pending.empty() && free_stream);
+        });
+        if (s->stopping && s->pending.empty()) break;
+        int si = (int)(std::find(s->stream_busy.begin(), s->stream_busy.end(), 0) - s->stream_busy.begin());
+        // Pack consecutive pending requests that are physically contiguous.
+        Launch L;
+        L.stream = si;
+        uint64_t p0 = s->pending.front()->start, p1 = p0;
+        while (!s->pending.empty()) {
+            Request *r = s->pending.front();
+            if (r->start != p1 && !L.reqs.empty()) break;              // wrap gap
+            if (!L.reqs.empty() && (r->start + r->k - p0) > max_launch) break;
+            if ((r->start % s->cap) + r->k > s->cap) break;            // cannot happen (place())
+            L.reqs.push_back(r);
+            r->state = 1;
+            p1 = r->start + r->k;
+            s->pending.pop_front();
+        }
+        s->stream_busy[si] = 1;
+        lk.unlock();
+
+        const uint64_t phys = p0 % s->cap, k = p1 - p0;
+        hipStream_t st = s->streams[si];
+        (void)hipEventCreate(&L.ev_start);
+        (void)hipEventCreate(&L.ev_end);
+        (void)hipEventRecord(L.ev_start, st);
+        if (k) {
+            (void)hipMemcpyAsync(s->d_a.p + phys, s->h_a + phys, k * 4, hipMemcpyHostToDevice, st);
+            (void)hipMemcpyAsync(s->d_b.p + phys, s->h_b + phys, k * 4, hipMemcpyHostToDevice, st);
+            pado_gpu_status qs = launch_query(s->idx, false, s->d_a.p + phys, s->d_b.p + phys, k, s->d_out.p + phys,
+                                              (s->cfg.flags & PADO_GPU_QUERY_REF_COMPAT) != 0, st);
+            if (qs != PADO_GPU_OK) {
+                std::lock_guard<std::mutex> g(s->mu);
+                s->error = pado_gpu_last_error();
+            }
+            (void)hipMemcpyAsync(s->h_out + phys, s->d_out.p + phys, k, hipMemcpyDeviceToHost, st);
+        }
+        (void)hipEventRecord(L.ev_end, st);
+        lk.lock();
+        s->inflight.push_back(std::move(L));
+        s->st.launches += 1;
+        lk.unlock();
+        s->cv_launch.notify_all();
+    }
+    s->cv_launch.notify_all();
+}
+
+void completer_loop(pado_gpu_server *s)
+{
+    (void)hipSetDevice(s->ctx->device);
+    for (;;) {
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_launch.wait(lk, [&] { return !s->inflight.empty() || (s->stopping && s->pending.empty()); });
+        if (s->inflight.empty()) {
+            if (s->stopping && s->pending.empty()) {
+                // The dispatcher may still be issuing its last launch.
+                bool any_busy = std::find(s->stream_busy.begin(), s->stream_busy.end(), 1) != s->stream_busy.end();
+                if (!any_busy) break;
+                s->cv_launch.wait_for(lk, std::chrono::milliseconds(1));
+            }
+            continue;
+        }
+        hipEvent_t ev0 = s->inflight.front().ev_start, ev1 = s->inflight.front().ev_end;
+        lk.unlock();
+        (void)hipEventSynchronize(ev1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, ev0, ev1);
+        const double t = now_ms();
+        lk.lock();
+        Launch L = std::move(s->inflight.front());
+        s->inflight.pop_front();
+        for (Request *r : L.reqs) {
+            r->state = 2;
+            r->t_done = t;
+            s->st.requests_done += 1;
+            s->st.pairs_done += r->k;
+        }
+        s->st.busy_ms += ms;
+        s->stream_busy[L.stream] = 0;
+        lk.unlock();
+        (void)hipEventDestroy(ev0);
+        (void)hipEventDestroy(ev1);
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+pado_gpu_status pado_gpu_serve_start(pado_gpu_ctx *ctx, pado_gpu_index *idx, const pado_gpu_serve_cfg *cfg,
+                                     pado_gpu_server **out)
+{
+    clear_error();
+    if (!ctx || !idx || !out) {
+        set_error("null argument");
+        return PADO_GPU_ERR_ARG;
+    }
+    *out = nullptr;
+    if (!idx->ordered || idx->weighted) {
+        set_error("serving needs an ordered unweighted index (call pado_gpu_index_order first)");
+        return PADO_GPU_ERR_STATE;
+    }
+    PADO_HIP(hipSetDevice(ctx->device));
+    auto *s = new pado_gpu_server();
+    s->ctx = ctx;
+    s->idx = idx;
+    if (cfg) s->cfg = *cfg;
+    if (!s->cfg.max_launch_pairs) s->cfg.max_launch_pairs = 4ull << 20;
+    if (!s->cfg.max_requests) s->cfg.max_requests = 4096;
+    if (!s->cfg.streams) s->cfg.streams = 3;
+    uint64_t cap = s->cfg.staging_pairs;
+    if (!cap) {
+        // Admission by free device memory: a quarter of what is free, 9 bytes
+        // per staged pair (two ids + answer), at most 16M pairs.
+        size_t fr = 0, tot = 0;
+        PADO_HIP(hipMemGetInfo(&fr, &tot));
+        cap = std::min<uint64_t>(16ull << 20, (uint64_t)(fr / 4) / 9);
+    }
+    if (cap < 1024) cap = 1024;
+    s->cap = cap;
+    s->st.staging_pairs = cap;
+    auto fail = [&](pado_gpu_status code) {
+        if (s->h_a) (void)hipHostFree(s->h_a);
+        if (s->h_b) (void)hipHostFree(s->h_b);
+        if (s->h_out) (void)hipHostFree(s->h_out);
+        for (auto q : s->streams) (void)hipStreamDestroy(q);
+        delete s;
+        return code;
+    };
+    if (hipHostMalloc((void **)&s->h_a, cap * 4, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&s->h_b, cap * 4, hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&s->h_out, cap, hipHostMallocDefault) != hipSuccess) {
+        set_error("pinned staging allocation failed");
+        return fail(PADO_GPU_ERR_NOMEM);
+    }
+    pado_gpu_status a1 = s->d_a.alloc(cap), a2 = s->d_b.alloc(cap), a3 = s->d_out.alloc(cap);
+    if (a1 != PADO_GPU_OK || a2 != PADO_GPU_OK || a3 != PADO_GPU_OK) return fail(PADO_GPU_ERR_NOMEM);
+    for (uint32_t i = 0; i < s->cfg.streams; ++i) {
+        hipStream_t q = nullptr;
+        if (hipStreamCreateWithFlags(&q, hipStreamNonBlocking) != hipSuccess) {
+            set_error("stream creation failed");
+            return fail(PADO_GPU_ERR_HIP);
+        }
+        s->streams.push_back(q);
+        s->stream_busy.push_back(0);
+    }
+    s->dispatcher = std::thread(dispatcher_loop, s);
+    s->completer = std::thread(completer_loop, s);
+    *out = s;
+    return PADO_GPU_OK;
+}
+
+pado_gpu_status pado_gpu_serve_submit(pado_gpu_server *s, const uint32_t *a, const uint32_t *b, uint64_t k,
+                                      uint64_t *req_id)
+{
+    clear_error();
+    if (!s || !req_id || (k && (!a || !b))) {
+        set_error("null argument");
+        return PADO_GPU_ERR_ARG;
+    }
+    if (k > s->cap) {
+        set_error("request larger than the staging ring");
+        return PADO_GPU_ERR_ARG;
+    }
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->stopping) {
+        set_error("server stopping");
+        return PADO_GPU_ERR_STATE;
+    }
+    const uint64_t start = place(s->tail, k, s->cap);
+    if (s->by_id.size() >= s->cfg.max_requests || start + k - s->head > s->cap) {
+        s->st.rejected += 1;
+        set_error("no free request slot or staging memory");
+        return PADO_GPU_ERR_BUSY;
+    }
+    auto *r = new Request();
+    r->id = s->next_id++;
+    r->start = start;
+    r->k = k;
+    r->t_submit = now_ms();
+    const uint64_t phys = start % s->cap;
+    if (k) {
+        std::memcpy(s->h_a + phys, a, k * 4);
+        std::memcpy(s->h_b + phys, b, k * 4);
+    }
+    s->tail = start + k;
+    s->live.push_back(r);
+    s->by_id[r->id] = r;
+    s->pending.push_back(r);
+    *req_id = r->id;
+    lk.unlock();
+    s->cv_work.notify_one();
+    return PADO_GPU_OK;
+}
+
+pado_gpu_status pado_gpu_serve_poll(pado_gpu_server *s, uint64_t req_id, int *done)
+{
+    if (!s || !done) return PADO_GPU_ERR_ARG;
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->by_id.find(req_id);
+    if (it == s->by_id.end()) {
+        set_error("unknown request id");
+        return PADO_GPU_ERR_ARG;
+    }
+    *done = it->second->state >= 2;
+    return PADO_GPU_OK;
+}
+
+pado_gpu_status pado_gpu_serve_wait(pado_gpu_server *s, uint64_t req_id, uint8_t *out, double *latency_ms)
+{
+    clear_error();
+    if (!s) return PADO_GPU_ERR_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->by_id.find(req_id);
+    if (it == s->by_id.end() || it->second->state == 3) {
+        set_error("unknown request id");
+        return PADO_GPU_ERR_ARG;
+    }
+    Request *r = it->second;
+    s->cv_done.wait(lk, [&] { return r->state >= 2; });
+    if (!s->error.empty()) {
+        set_error(s->error);
+        return PADO_GPU_ERR_HIP;
+    }
+    if (r->k && out) std::memcpy(out, s->h_out + (r->start % s->cap), r->k);
+    if (latency_ms) *latency_ms = r->t_done - r->t_submit;
+    r->state = 3;
+    release_consumed(s);
+    lk.unlock();
+    s->cv_work.notify_all();
+    return PADO_GPU_OK;
+}
+
+pado_gpu_status pado_gpu_serve_get_stats(pado_gpu_server *s, pado_gpu_serve_stats *st)
+{
+    if (!s || !st) return PADO_GPU_ERR_ARG;
+    std::lock_guard<std::mutex> g(s->mu);
+    *st = s->st;
+    return PADO_GPU_OK;
+}
+
+void pado_gpu_serve_stop(pado_gpu_server *s)
+{
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stopping = true;
+    }
+    s->cv_work.notify_all();
+    s->cv_launch.notify_all();
+    if (s->dispatcher.joinable()) s->dispatcher.join();
+    s->cv_launch.notify_all();
+    if (s->completer.joinable()) s->completer.join();
+    for (auto q : s->streams) {
+        (void)hipStreamSynchronize(q);
+        (void)hipStreamDestroy(q);
+    }
+    for (Request *r : s->live) delete r;
+    (void)hipHostFree(s->h_a);
+    (void)hipHostFree(s->h_b);
+    (void)hipHostFree(s->h_out);
+    delete s;
+}
+
+}  // extern "C"

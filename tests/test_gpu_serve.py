@@ -1,0 +1,100 @@
+"""The serving loop: each request's answers are the same alone and in a mix of
+concurrent uneven requests, equal to the plain batch query; admission refuses
+requests when staging memory or slots are exhausted."""
+import sys
+import threading
+
+import numpy as np
+import pytest
+
+import parallel_distance_oracle_amd as pado
+from helpers import rank_space, rmat_edges
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def served(gpu_ctx):
+    G, rank, r2i = rank_space(rmat_edges(14, 8))
+    print(f"[gpu] serve index n={G.num_v}", flush=True)
+    sys.stdout.flush()
+    idx = pado.Index.build(gpu_ctx, pado.DeviceGraph(gpu_ctx, G))
+    idx.order(rank)
+    return G, idx, r2i
+
+
+def _requests(n, r2i, count, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        k = int(np.floor(np.exp(rng.uniform(0, np.log(65536))))) if i % 7 else 0
+        half = k // 2
+        a = np.concatenate([rng.integers(0, n, half), pado.zipf_vertices(n, k - half, 1.1, seed + i, r2i)])
+        b = np.concatenate([rng.integers(0, n, half), pado.zipf_vertices(n, k - half, 1.1, seed + 7 * i, r2i)])
+        out.append((a.astype(np.uint32), b.astype(np.uint32)))
+    return out
+
+
+def test_alone_equals_mix_equals_batch(gpu_ctx, served):
+    G, idx, r2i = served
+    reqs = _requests(G.num_v, r2i, 60, seed=11)
+    srv = pado.Server(gpu_ctx, idx, staging_pairs=1 << 20, max_launch_pairs=1 << 18, streams=3)
+    try:
+        alone = []
+        for a, b in reqs:
+            rid = srv.submit(a, b)
+            alone.append(srv.wait(rid, a.size)[0])
+        # The same requests concurrently from two client threads.
+        mix = [None] * len(reqs)
+        lock = threading.Lock()
+
+        def client(indices):
+            for i in indices:
+                a, b = reqs[i]
+                while True:
+                    try:
+                        rid = srv.submit(a, b)
+                        break
+                    except pado.Busy:
+                        pass
+                with lock:
+                    pass
+                mix[i] = srv.wait(rid, a.size)[0]
+
+        t1 = threading.Thread(target=client, args=(range(0, len(reqs), 2),))
+        t2 = threading.Thread(target=client, args=(range(1, len(reqs), 2),))
+        t1.start()
+        t2.start()
+        t1.join(timeout=120)
+        t2.join(timeout=120)
+        st = srv.stats()
+    finally:
+        srv.stop()
+    for (a, b), x, y in zip(reqs, alone, mix):
+        assert y is not None
+        assert np.array_equal(x, y)
+        assert np.array_equal(x, idx.query(a, b))
+    assert st["requests_done"] == 2 * len(reqs)
+    assert st["launches"] < 2 * len(reqs)  # some requests were coalesced
+
+
+def test_admission_refuses_when_full(gpu_ctx, served):
+    G, idx, _ = served
+    srv = pado.Server(gpu_ctx, idx, staging_pairs=4096, max_requests=4)
+    try:
+        a = np.arange(3000, dtype=np.uint32) % G.num_v
+        r1 = srv.submit(a, a)
+        with pytest.raises(pado.Busy):
+            srv.submit(a, a)  # ring holds 4096 pairs
+        out, lat = srv.wait(r1, a.size)
+        assert np.all(out == 0) and lat >= 0
+        r2 = srv.submit(a, a)  # space released
+        srv.wait(r2, a.size)
+        ids = [srv.submit(a[:10], a[:10]) for _ in range(4)]
+        with pytest.raises(pado.Busy):
+            srv.submit(a[:10], a[:10])  # 4 slots
+        for r in ids:
+            srv.wait(r, 10)
+        assert srv.stats()["rejected"] == 2
+    finally:
+        srv.stop()
