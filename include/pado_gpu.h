@@ -105,9 +105,13 @@ void pado_gpu_graph_free(pado_gpu_graph *g);
 
 typedef struct {
     uint32_t batch_size;  /* roots per batch: 1..1024, 0 = 1024 (PARA:42) */
-    uint32_t flags;       /* reserved, 0 */
+    uint32_t flags;       /* PADO_GPU_BUILD_* */
     uint32_t reserved[6];
 } pado_gpu_build_cfg;
+
+/* Bit-parallel labeling one BFS per slot (as the reference runs it) instead of
+ * the default single multi-source sweep over all 50 slots; same output. */
+#define PADO_GPU_BUILD_BP_PER_SLOT 1u
 
 /* ParaVertexCentricPLL<BATCH_SIZE>(const Graph&) -> construct (PARA:309-313,
  * 1740-1895): bit-parallel labeling (PARA:544-840) then batch_process over all
@@ -184,6 +188,34 @@ pado_gpu_status pado_gpu_index_store(pado_gpu_ctx *ctx, const pado_gpu_index *id
 /* The loaded index is ordered (query by original id) and has no rank-space
  * copy. */
 pado_gpu_status pado_gpu_index_load(pado_gpu_ctx *ctx, const char *filename, pado_gpu_index **out);
+
+/* ---------------------------------------------------------------------------
+ * Many independent graphs in one job (no reference counterpart: the
+ * reference labels one graph per process).  Each graph g is a rank-space CSR
+ * of n[g] vertices; `off` concatenates the graphs' offset arrays (n[g] + 1
+ * entries each, starting at 0), `adj` their adjacency arrays (local ids,
+ * descending per list).  Graphs whose every distance is provably below 255
+ * (2 x the eccentricity of one vertex per component) are labeled together as
+ * one interleaved union -- label sets depend on neither BATCH_SIZE nor other
+ * components, so each graph gets exactly the labels of a build on its own --
+ * with every graph keeping its own 50 bit-parallel slots.  The others are
+ * built one by one and report PADO_GPU_ERR_DOMAIN if they leave the u8 range.
+ * ------------------------------------------------------------------------ */
+typedef struct pado_gpu_multi pado_gpu_multi;
+
+pado_gpu_status pado_gpu_build_multi(pado_gpu_ctx *ctx, uint32_t num_graphs, const uint32_t *n, const uint64_t *off,
+                                     const uint32_t *adj, const pado_gpu_build_cfg *cfg, pado_gpu_multi **out);
+/* Per-graph result: status (OK or ERR_DOMAIN), label count, and whether it
+ * was labeled inside the union (1) or on its own (0). */
+pado_gpu_status pado_gpu_multi_graph_info(const pado_gpu_multi *mi, uint32_t g, pado_gpu_status *status,
+                                          uint64_t *num_labels, int *in_union);
+/* Canonical rank-space export of graph g (same layout as pado_gpu_index_export). */
+pado_gpu_status pado_gpu_multi_export(pado_gpu_ctx *ctx, pado_gpu_multi *mi, uint32_t g, uint8_t *bp_dist,
+                                      uint64_t *bp_sets, uint64_t *label_off, uint32_t *hubs, uint8_t *dists);
+/* Whole-job figures: number of unions, union vertices, build ms. */
+pado_gpu_status pado_gpu_multi_info(const pado_gpu_multi *mi, uint32_t *num_unions, uint64_t *union_vertices,
+                                    double *ms_total);
+void pado_gpu_multi_free(pado_gpu_multi *mi);
 
 /* ---------------------------------------------------------------------------
  * Serving: a request loop over a resident index (no reference counterpart;

@@ -28,6 +28,7 @@ INF8 = 255
 QUERY_DEVICE_PTRS = 1
 QUERY_REF_COMPAT = 2
 QUERY_RANK_SPACE = 4
+BUILD_BP_PER_SLOT = 1
 
 _STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_NOMEM", 4: "ERR_DOMAIN", 5: "ERR_STATE",
            6: "ERR_IO", 7: "ERR_BUSY"}
@@ -109,6 +110,14 @@ def lib():
         "pado_gpu_index_free": (None, [_vp]),
         "pado_gpu_index_store": (ctypes.c_int, [_vp, _vp, ctypes.c_char_p, _vp, _u32]),
         "pado_gpu_index_load": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_vp)]),
+        "pado_gpu_build_multi": (ctypes.c_int, [_vp, _u32, _vp, _vp, _vp, ctypes.POINTER(BuildCfg),
+                                                ctypes.POINTER(_vp)]),
+        "pado_gpu_multi_graph_info": (ctypes.c_int, [_vp, _u32, ctypes.POINTER(ctypes.c_int),
+                                                     ctypes.POINTER(_u64), ctypes.POINTER(ctypes.c_int)]),
+        "pado_gpu_multi_export": (ctypes.c_int, [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp]),
+        "pado_gpu_multi_info": (ctypes.c_int, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u64),
+                                               ctypes.POINTER(ctypes.c_double)]),
+        "pado_gpu_multi_free": (None, [_vp]),
         "pado_gpu_serve_start": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(ServeCfg), ctypes.POINTER(_vp)]),
         "pado_gpu_serve_submit": (ctypes.c_int, [_vp, _vp, _vp, _u64, ctypes.POINTER(_u64)]),
         "pado_gpu_serve_poll": (ctypes.c_int, [_vp, _u64, ctypes.POINTER(ctypes.c_int)]),
@@ -328,9 +337,10 @@ class Index:
         self.num_v = self.info.num_v
 
     @classmethod
-    def build(cls, ctx, dgraph, batch_size=BATCH_SIZE):
+    def build(cls, ctx, dgraph, batch_size=BATCH_SIZE, bp_per_slot=False):
         cfg = BuildCfg()
         cfg.batch_size = batch_size
+        cfg.flags = BUILD_BP_PER_SLOT if bp_per_slot else 0
         h = _vp()
         _check(lib().pado_gpu_build(ctx.h, dgraph.h, ctypes.byref(cfg), ctypes.byref(h)))
         return cls(ctx, h)
@@ -400,6 +410,59 @@ class Index:
     def close(self):
         if self.h:
             lib().pado_gpu_index_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class MultiIndex:
+    """Many independent rank-space graphs labeled as one job."""
+
+    def __init__(self, ctx, graphs, batch_size=BATCH_SIZE):
+        self.ctx = ctx
+        ns = np.array([G.num_v for G in graphs], np.uint32)
+        off = np.concatenate([G.off for G in graphs]).astype(np.uint64)
+        adj = np.concatenate([G.adj for G in graphs]).astype(np.uint32) if graphs else np.zeros(0, np.uint32)
+        cfg = BuildCfg()
+        cfg.batch_size = batch_size
+        h = _vp()
+        _check(lib().pado_gpu_build_multi(ctx.h, len(graphs), _p(ns), _p(off), _p(adj) if adj.size else None,
+                                          ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self.sizes = ns
+
+    def graph_info(self, g):
+        st = ctypes.c_int()
+        nl = _u64()
+        iu = ctypes.c_int()
+        _check(lib().pado_gpu_multi_graph_info(self.h, g, ctypes.byref(st), ctypes.byref(nl), ctypes.byref(iu)))
+        return {"status": st.value, "num_labels": nl.value, "in_union": bool(iu.value)}
+
+    def export(self, g):
+        n = int(self.sizes[g])
+        tot = self.graph_info(g)["num_labels"]
+        bp_d = np.zeros(n * BP_SIZE, np.uint8)
+        bp_s = np.zeros(n * BP_SIZE * 2, np.uint64)
+        loff = np.zeros(n + 1, np.uint64)
+        hub = np.zeros(max(tot, 1), np.uint32)
+        dist = np.zeros(max(tot, 1), np.uint8)
+        _check(lib().pado_gpu_multi_export(self.ctx.h, self.h, g, _p(bp_d), _p(bp_s), _p(loff), _p(hub), _p(dist)))
+        return bp_d.reshape(n, BP_SIZE), bp_s.reshape(n, BP_SIZE, 2), loff, hub[:tot], dist[:tot]
+
+    def info(self):
+        nu = _u32()
+        uv = _u64()
+        ms = ctypes.c_double()
+        _check(lib().pado_gpu_multi_info(self.h, ctypes.byref(nu), ctypes.byref(uv), ctypes.byref(ms)))
+        return {"unions": nu.value, "union_vertices": uv.value, "ms_total": ms.value}
+
+    def close(self):
+        if self.h:
+            lib().pado_gpu_multi_free(self.h)
             self.h = None
 
     def __del__(self):

@@ -42,7 +42,7 @@ __global__ void k_ms_seed(uint32_t nseeds, const uint32_t *seed_v, const uint32_
                           uint32_t *cnt_cur_list, u64 *nxt_chunks, uint32_t *cnt_nxt_chunks, uint32_t *nxt_list,
                           uint32_t *cnt_nxt_list)
 {
-    for (uint32_t s = threadIdx.x; s < nseeds; s += blockDim.x) {
+    for (uint32_t s = blockIdx.x * blockDim.x + threadIdx.x; s < nseeds; s += gridDim.x * blockDim.x) {
         const uint32_t v = seed_v[s], i = seed_slot[s], j = seed_j[s];
         const u64 bit = 1ull << i;
         atomicOr(&vis[v], bit);
@@ -147,8 +147,48 @@ __global__ void k_ms_zero_counts(uint32_t *c) { c[threadIdx.x] = 0; }
 
 }  // namespace
 
+void select_bp_roots(uint32_t n, const uint64_t *off, const uint32_t *adj, std::vector<uint8_t> &used,
+                     uint32_t *roots, BpSeeds &seeds, const uint32_t *map)
+{
+    used.assign(n, 0);
+    uint32_t r = 0;
+    for (uint32_t i = 0; i < kBP; ++i) {
+        while (r < n && used[r]) ++r;
+        if (r == n) {
+            if (roots) roots[i] = kEmpty;
+            continue;
+        }
+        if (roots) roots[i] = r;
+        used[r] = 1;
+        seeds.v.push_back(map ? map[r] : r);
+        seeds.slot.push_back(i);
+        seeds.j.push_back(255u);
+        const uint64_t deg = off[r + 1] - off[r];
+        uint32_t ns = 0;
+        for (uint64_t di = 0; di < deg && ns < 64; ++di) {
+            const uint32_t v = adj[off[r] + deg - 1 - di];
+            if (!used[v]) {
+                used[v] = 1;
+                seeds.v.push_back(map ? map[v] : v);
+                seeds.slot.push_back(i);
+                seeds.j.push_back(ns);
+                ++ns;
+            }
+        }
+    }
+}
+
 pado_gpu_status build_bp_multi(pado_gpu_graph *g, pado_gpu_index *idx, DevBuf<uint8_t> &d_used,
                                std::vector<uint8_t> &h_used)
+{
+    // Serial root / neighbour selection (PARA:670-709).
+    BpSeeds seeds;
+    select_bp_roots(g->n, g->h_off.data(), g->h_adj.data(), h_used, idx->bp_roots, seeds, nullptr);
+    return build_bp_seeded(g, idx, d_used, h_used, seeds);
+}
+
+pado_gpu_status build_bp_seeded(pado_gpu_graph *g, pado_gpu_index *idx, DevBuf<uint8_t> &d_used,
+                                const std::vector<uint8_t> &h_used, const BpSeeds &seeds)
 {
     hipStream_t st = g->ctx->stream;
     const uint32_t n = g->n;
@@ -156,39 +196,11 @@ pado_gpu_status build_bp_multi(pado_gpu_graph *g, pado_gpu_index *idx, DevBuf<ui
     PADO_TRY(idx->bpd.alloc((uint64_t)n * kBpdStride));
     PADO_TRY(idx->bps.alloc((uint64_t)n * kBP * 2));
     PADO_TRY(d_used.alloc(n));
-
-    // Serial root / neighbour selection (PARA:670-709).
-    h_used.assign(n, 0);
-    std::vector<uint32_t> sv, ss, sj;
-    uint32_t r = 0;
-    for (uint32_t i = 0; i < kBP; ++i) {
-        while (r < n && h_used[r]) ++r;
-        if (r == n) {
-            idx->bp_roots[i] = kEmpty;
-            continue;
-        }
-        idx->bp_roots[i] = r;
-        h_used[r] = 1;
-        sv.push_back(r);
-        ss.push_back(i);
-        sj.push_back(255u);
-        uint64_t deg = g->h_off[r + 1] - g->h_off[r];
-        uint32_t ns = 0;
-        for (uint64_t di = 0; di < deg && ns < 64; ++di) {
-            uint32_t v = g->h_adj[g->h_off[r] + deg - 1 - di];
-            if (!h_used[v]) {
-                h_used[v] = 1;
-                sv.push_back(v);
-                ss.push_back(i);
-                sj.push_back(ns);
-                ++ns;
-            }
-        }
-    }
+    const std::vector<uint32_t> &sv = seeds.v, &ss = seeds.slot, &sj = seeds.j;
     PADO_HIP(hipMemcpyAsync(d_used.p, h_used.data(), n, hipMemcpyHostToDevice, st));
 
     DevBuf<u64> fma, fmb, vis, ca, cb;
-    DevBuf<uint32_t> la, lb, cnt, seeds;
+    DevBuf<uint32_t> la, lb, cnt, dseeds;
     PADO_TRY(fma.alloc(n));
     PADO_TRY(fmb.alloc(n));
     PADO_TRY(vis.alloc(n));
@@ -199,11 +211,11 @@ pado_gpu_status build_bp_multi(pado_gpu_graph *g, pado_gpu_index *idx, DevBuf<ui
     PADO_TRY(lb.alloc(n));
     PADO_TRY(cnt.alloc(4));  // [chunks a, list a, chunks b, list b]
     const uint32_t ns = (uint32_t)sv.size();
-    PADO_TRY(seeds.alloc(3ull * (ns ? ns : 1)));
+    PADO_TRY(dseeds.alloc(3ull * (ns ? ns : 1)));
     std::vector<uint32_t> packed(sv);
     packed.insert(packed.end(), ss.begin(), ss.end());
     packed.insert(packed.end(), sj.begin(), sj.end());
-    if (ns) PADO_HIP(hipMemcpyAsync(seeds.p, packed.data(), 4ull * packed.size(), hipMemcpyHostToDevice, st));
+    if (ns) PADO_HIP(hipMemcpyAsync(dseeds.p, packed.data(), 4ull * packed.size(), hipMemcpyHostToDevice, st));
 
     hipLaunchKernelGGL(k_ms_fill, dim3(8192), dim3(256), 0, st, idx->bpd.p, (uint64_t)n * kBpdStride,
                        (u64 *)idx->bps.p, (uint64_t)n * kBP * 2);
@@ -224,7 +236,7 @@ pado_gpu_status build_bp_multi(pado_gpu_graph *g, pado_gpu_index *idx, DevBuf<ui
     uint32_t *cur_l = la.p, *nxt_l = lb.p;
     uint32_t *cnt_cur_c = cnt.p, *cnt_cur_l = cnt.p + 1, *cnt_nxt_c = cnt.p + 2, *cnt_nxt_l = cnt.p + 3;
     if (ns)
-        hipLaunchKernelGGL(k_ms_seed, dim3(1), dim3(256), 0, st, ns, seeds.p, seeds.p + ns, seeds.p + 2 * ns, g->off.p,
+        hipLaunchKernelGGL(k_ms_seed, dim3(grid_for(ns, 256, 4096)), dim3(256), 0, st, ns, dseeds.p, dseeds.p + ns, dseeds.p + 2 * ns, g->off.p,
                            idx->bpd.p, bps, fm, nm, vis.p, cur_c, cnt_cur_c, cur_l, cnt_cur_l, nxt_c, cnt_nxt_c,
                            nxt_l, cnt_nxt_l);
     const uint32_t egrid = 4096;

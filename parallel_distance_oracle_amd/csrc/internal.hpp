@@ -14,6 +14,19 @@ pado_gpu_status build_bp(pado_gpu_graph *g, pado_gpu_index *idx, DevBuf<uint8_t>
 pado_gpu_status build_bp_multi(pado_gpu_graph *g, pado_gpu_index *idx, DevBuf<uint8_t> &d_used,
                                std::vector<uint8_t> &h_used);
 
+// Seeds of the multi-source sweep: slot roots (j = 255) and their selected
+// neighbours (j = neighbour index), as vertex ids of the graph being labeled.
+struct BpSeeds {
+    std::vector<uint32_t> v, slot, j;
+};
+// Root / neighbour selection of PARA:670-709 on one rank-space CSR; `map`
+// (nullable) renames the chosen vertices (e.g. into a union graph's ids).
+void select_bp_roots(uint32_t n, const uint64_t *off, const uint32_t *adj, std::vector<uint8_t> &used,
+                     uint32_t *roots, BpSeeds &seeds, const uint32_t *map);
+// Multi-source sweep from given seeds; h_used = the used flags (n entries).
+pado_gpu_status build_bp_seeded(pado_gpu_graph *g, pado_gpu_index *idx, DevBuf<uint8_t> &d_used,
+                                const std::vector<uint8_t> &h_used, const BpSeeds &seeds);
+
 // Batched pruned labeling (PARA:848-1836).  Produces the canonical rank-space
 // label CSR idx->loff / idx->lab.
 pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevBuf<uint8_t> &d_used,

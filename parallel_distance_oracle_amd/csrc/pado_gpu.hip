@@ -275,7 +275,8 @@ pado_gpu_status pado_gpu_build(pado_gpu_ctx *ctx, const pado_gpu_graph *graph, c
         DevBuf<uint8_t> used;
         std::vector<uint8_t> h_used;
         Timer t0;
-        s = build_bp(g, idx, used, h_used);
+        s = (c.flags & PADO_GPU_BUILD_BP_PER_SLOT) ? build_bp(g, idx, used, h_used)
+                                                   : build_bp_multi(g, idx, used, h_used);
         idx->ms_bp = t0.ms();
         if (s == PADO_GPU_OK) {
             Timer t1;

@@ -55,6 +55,33 @@ def test_rmat_bit_exact_and_counters(gpu_ctx, scale, ef):
     idx.close()
 
 
+@pytest.mark.parametrize("case", ["edgelist_in", "rmat12", "rmat14"])
+def test_bp_per_slot_and_multi_source_agree(gpu_ctx, case):
+    """The reference's one-BFS-per-slot order and the single multi-source
+    sweep produce the same bit-parallel records (and so the same labels)."""
+    if case.startswith("rmat"):
+        s = int(case[4:])
+        meta, arr = load_rmat(s, 8)
+        G, _, _ = rank_space(rmat_edges(s, 8))
+        exp_bp = meta["bp_sha256"]
+    else:
+        g = load_tiny(case)
+        G, _, _ = rank_space(g["edges"], g["rank"])
+        exp_bp = bp_digest(g["bp_d"], g["bp_s"])
+    print(f"[gpu] bp variants {case}", flush=True)
+    dg = pado.DeviceGraph(gpu_ctx, G)
+    a = pado.Index.build(gpu_ctx, dg, bp_per_slot=True)
+    b = pado.Index.build(gpu_ctx, dg, bp_per_slot=False)
+    ea, eb = a.export(), b.export()
+    assert bp_digest(ea[0], ea[1]) == exp_bp
+    assert bp_digest(eb[0], eb[1]) == exp_bp
+    for x, y in zip(ea, eb):
+        assert np.array_equal(x, y)
+    a.close()
+    b.close()
+    dg.close()
+
+
 @pytest.mark.parametrize("batch", [1, 64, 500])
 def test_other_batch_sizes_same_labels(gpu_ctx, batch):
     meta, arr = load_rmat(12, 8)
