@@ -104,9 +104,12 @@ pado_gpu_status pado_gpu_graph_upload(pado_gpu_ctx *ctx, uint32_t n, uint64_t m_
 void pado_gpu_graph_free(pado_gpu_graph *g);
 
 typedef struct {
-    uint32_t batch_size;  /* roots per batch: 1..1024, 0 = 1024 (PARA:42) */
+    uint32_t batch_size;  /* roots per batch: 1..1024, 0 = 1024 (PARA:42); any value
+                             gives the same labels, so it is a tuning knob */
     uint32_t flags;       /* PADO_GPU_BUILD_* */
-    uint32_t reserved[6];
+    uint32_t push_blocks; /* grid of the push kernel (256-thread blocks), 0 = 4096 */
+    uint32_t add_blocks;  /* grid of the add kernel, 0 = 2048 */
+    uint32_t reserved[4];
 } pado_gpu_build_cfg;
 
 /* Bit-parallel labeling one BFS per slot (as the reference runs it) instead of
@@ -188,6 +191,19 @@ pado_gpu_status pado_gpu_index_store(pado_gpu_ctx *ctx, const pado_gpu_index *id
 /* The loaded index is ordered (query by original id) and has no rank-space
  * copy. */
 pado_gpu_status pado_gpu_index_load(pado_gpu_ctx *ctx, const char *filename, pado_gpu_index **out);
+
+/* ---------------------------------------------------------------------------
+ * Autotuner (no reference counterpart).  Every knob of pado_gpu_build_cfg
+ * leaves the labels unchanged, so the tuner may pick any of them purely on
+ * speed: it times full trial builds of `graph` in a coordinate search over
+ * batch size {256, 512, 1024}, push grid {1024..8192}, add grid {1024..4096}
+ * and the bit-parallel variant, starting from `start` (NULL = defaults), and
+ * returns the fastest configuration.  trials_out (nullable) receives the
+ * number of builds run; best_ms their best time.
+ * ------------------------------------------------------------------------ */
+pado_gpu_status pado_gpu_autotune(pado_gpu_ctx *ctx, const pado_gpu_graph *graph, const pado_gpu_build_cfg *start,
+                                  uint32_t max_trials, pado_gpu_build_cfg *best, double *best_ms,
+                                  uint32_t *trials_out);
 
 /* ---------------------------------------------------------------------------
  * Many independent graphs in one job (no reference counterpart: the

@@ -41,7 +41,12 @@ class PadoError(RuntimeError):
 
 
 class BuildCfg(ctypes.Structure):
-    _fields_ = [("batch_size", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 6)]
+    _fields_ = [("batch_size", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("push_blocks", ctypes.c_uint32),
+                ("add_blocks", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4)]
+
+    def as_dict(self):
+        return {"batch_size": self.batch_size, "flags": self.flags, "push_blocks": self.push_blocks,
+                "add_blocks": self.add_blocks}
 
 
 class ServeCfg(ctypes.Structure):
@@ -110,6 +115,8 @@ def lib():
         "pado_gpu_index_free": (None, [_vp]),
         "pado_gpu_index_store": (ctypes.c_int, [_vp, _vp, ctypes.c_char_p, _vp, _u32]),
         "pado_gpu_index_load": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_vp)]),
+        "pado_gpu_autotune": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(BuildCfg), _u32, ctypes.POINTER(BuildCfg),
+                                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u32)]),
         "pado_gpu_build_multi": (ctypes.c_int, [_vp, _u32, _vp, _vp, _vp, ctypes.POINTER(BuildCfg),
                                                 ctypes.POINTER(_vp)]),
         "pado_gpu_multi_graph_info": (ctypes.c_int, [_vp, _u32, ctypes.POINTER(ctypes.c_int),
@@ -337,10 +344,11 @@ class Index:
         self.num_v = self.info.num_v
 
     @classmethod
-    def build(cls, ctx, dgraph, batch_size=BATCH_SIZE, bp_per_slot=False):
-        cfg = BuildCfg()
-        cfg.batch_size = batch_size
-        cfg.flags = BUILD_BP_PER_SLOT if bp_per_slot else 0
+    def build(cls, ctx, dgraph, batch_size=BATCH_SIZE, bp_per_slot=False, cfg=None):
+        if cfg is None:
+            cfg = BuildCfg()
+            cfg.batch_size = batch_size
+            cfg.flags = BUILD_BP_PER_SLOT if bp_per_slot else 0
         h = _vp()
         _check(lib().pado_gpu_build(ctx.h, dgraph.h, ctypes.byref(cfg), ctypes.byref(h)))
         return cls(ctx, h)
@@ -417,6 +425,16 @@ class Index:
             self.close()
         except Exception:
             pass
+
+
+def autotune(ctx, dgraph, max_trials=16, start=None):
+    """Times trial builds; returns (best BuildCfg, best ms, trials)."""
+    best = BuildCfg()
+    ms = ctypes.c_double()
+    tr = _u32()
+    _check(lib().pado_gpu_autotune(ctx.h, dgraph.h, ctypes.byref(start) if start is not None else None,
+                                   max_trials, ctypes.byref(best), ctypes.byref(ms), ctypes.byref(tr)))
+    return best, ms.value, tr.value
 
 
 class MultiIndex:

@@ -568,10 +568,11 @@ static pado_gpu_status pool_repack(hipStream_t st, uint32_t n, DevBuf<uint32_t> 
 }
 
 pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevBuf<uint8_t> &d_used,
-                              uint32_t batch)
+                              const pado_gpu_build_cfg &cfg)
 {
     hipStream_t st = g->ctx->stream;
     const uint32_t n = g->n;
+    const uint32_t batch = cfg.batch_size ? cfg.batch_size : kBatch;
     if (batch == 0 || batch > kBatch) {
         set_error("batch_size must be in [1, 1024]");
         return PADO_GPU_ERR_ARG;
@@ -641,7 +642,8 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     const uint32_t nb = (n + batch - 1) / batch;
     uint32_t gen_iter = 0;
     uint64_t iterations = 0;
-    const uint32_t push_grid = 4096, add_grid = 2048;
+    const uint32_t push_grid = cfg.push_blocks ? cfg.push_blocks : 4096;
+    const uint32_t add_grid = cfg.add_blocks ? cfg.add_blocks : 2048;
     for (uint32_t b = 0; b < nb; ++b) {
         const uint32_t rs = b * batch;
         const uint32_t rsz = std::min(batch, n - rs);

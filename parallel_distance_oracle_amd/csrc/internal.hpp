@@ -30,7 +30,7 @@ pado_gpu_status build_bp_seeded(pado_gpu_graph *g, pado_gpu_index *idx, DevBuf<u
 // Batched pruned labeling (PARA:848-1836).  Produces the canonical rank-space
 // label CSR idx->loff / idx->lab.
 pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevBuf<uint8_t> &d_used,
-                              uint32_t batch_size);
+                              const pado_gpu_build_cfg &cfg);
 
 // Weighted labeling (WPARA).
 pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t batch_size);

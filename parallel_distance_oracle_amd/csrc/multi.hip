@@ -220,7 +220,9 @@ pado_gpu_status pado_gpu_build_multi(pado_gpu_ctx *ctx, uint32_t num_graphs, con
         idx->ms_bp = ms_since(tb);
         if (s == PADO_GPU_OK) {
             const auto tc = std::chrono::steady_clock::now();
-            s = build_batches(ug, idx, d_used, cfg && cfg->batch_size ? cfg->batch_size : kBatch);
+            pado_gpu_build_cfg c{};
+            if (cfg) c = *cfg;
+            s = build_batches(ug, idx, d_used, c);
             idx->ms_batch = ms_since(tc);
         }
         pado_gpu_graph_free(ug);

@@ -280,7 +280,7 @@ pado_gpu_status pado_gpu_build(pado_gpu_ctx *ctx, const pado_gpu_graph *graph, c
         idx->ms_bp = t0.ms();
         if (s == PADO_GPU_OK) {
             Timer t1;
-            s = build_batches(g, idx, used, c.batch_size);
+            s = build_batches(g, idx, used, c);
             idx->ms_batch = t1.ms();
         }
     }
