@@ -287,3 +287,125 @@ def read_text_edges(path):
 def set_mutation(m):
     """Planted-mistake switch (tests only); 0 restores the faithful oracle."""
     lib().po_set_mutation(int(m))
+
+
+# --- weighted (WPARA) ----------------------------------------------------------
+i16p = np.ctypeslib.ndpointer(np.int16, flags="C")
+
+
+def _wlib():
+    L = lib()
+    if not getattr(L, "_w_ready", False):
+        L.wo_id_transfer.argtypes = [ctypes.c_uint32, u64p, u32p, u8p, u32p, u64p, u32p, u8p, u32p]
+        L.wo_labeling.restype = ctypes.c_void_p
+        L.wo_labeling.argtypes = [ctypes.c_uint32, u64p, u32p, u8p, ctypes.c_uint32]
+        L.wo_num_labels.restype = ctypes.c_uint64
+        L.wo_num_labels.argtypes = [ctypes.c_void_p]
+        L.wo_export.argtypes = [ctypes.c_void_p, u64p, u32p, i16p]
+        L.wo_free.argtypes = [ctypes.c_void_p]
+        L.wo_query_batch.argtypes = [ctypes.c_uint32, u64p, u32p, i16p, u32p, u32p, ctypes.c_uint64, i16p]
+        L.wo_dijkstra.argtypes = [ctypes.c_uint32, u64p, u32p, u8p, ctypes.c_uint32, u32p]
+        L._w_ready = True
+    return L
+
+
+def weighted_graph_csr(edges, weights):
+    """WeightedGraph(edge_list, weight_list) (GR:302-340): CSR + u8 weights."""
+    e = _c(edges, np.uint32).reshape(-1, 2)
+    w = _c(weights, np.uint32).astype(np.uint8)
+    n = int(e.max()) + 1 if e.size else 0
+    deg = np.bincount(e.reshape(-1), minlength=n)
+    off = np.zeros(n + 1, np.uint64)
+    off[1:] = np.cumsum(deg)
+    adj = np.zeros(2 * e.shape[0], np.uint32)
+    ww = np.zeros(2 * e.shape[0], np.uint8)
+    pos = off[:-1].copy()
+    for i in range(e.shape[0]):  # edge order as the reference appends
+        a, b = int(e[i, 0]), int(e[i, 1])
+        adj[pos[a]] = b; ww[pos[a]] = w[i]; pos[a] += 1
+        adj[pos[b]] = a; ww[pos[b]] = w[i]; pos[b] += 1
+    return n, off, adj, ww
+
+
+def weighted_id_transfer(n, off, adj, w, rank):
+    off2 = np.zeros(n + 1, np.uint64)
+    adj2 = np.zeros(max(adj.size, 1), np.uint32)
+    w2 = np.zeros(max(adj.size, 1), np.uint8)
+    r2i = np.zeros(max(n, 1), np.uint32)
+    _wlib().wo_id_transfer(n, _c(off, np.uint64), _c(adj, np.uint32), _c(w, np.uint8), _c(rank, np.uint32),
+                           off2, adj2, w2, r2i)
+    return off2, adj2[: adj.size], w2[: adj.size], r2i[:n]
+
+
+class WeightedIndex:
+    """WeightedParaVertexCentricPLL<B> restated (single-thread schedule)."""
+
+    def __init__(self, n, off, adj, w, batch=512):
+        L = _wlib()
+        h = L.wo_labeling(n, _c(off, np.uint64), _c(adj, np.uint32), _c(w, np.uint8), batch)
+        try:
+            tot = L.wo_num_labels(h)
+            self.loff = np.zeros(n + 1, np.uint64)
+            self.hub = np.zeros(max(tot, 1), np.uint32)
+            self.dist = np.zeros(max(tot, 1), np.int16)
+            L.wo_export(h, self.loff, self.hub, self.dist)
+            self.hub, self.dist = self.hub[:tot], self.dist[:tot]
+        finally:
+            L.wo_free(h)
+        self.n = n
+
+    def query(self, a, b):
+        a = _c(a, np.uint32).reshape(-1)
+        b = _c(b, np.uint32).reshape(-1)
+        out = np.zeros(max(a.size, 1), np.int16)
+        hub = self.hub if self.hub.size else np.zeros(1, np.uint32)
+        dist = self.dist if self.dist.size else np.zeros(1, np.int16)
+        _wlib().wo_query_batch(self.n, self.loff, hub, dist, a, b, a.size, out)
+        return out[: a.size]
+
+
+def dijkstra(n, off, adj, w, src):
+    d = np.zeros(max(n, 1), np.uint32)
+    _wlib().wo_dijkstra(n, _c(off, np.uint64), _c(adj, np.uint32), _c(w, np.uint8), src, d)
+    return d[:n]
+
+
+def read_wlabels(path):
+    with open(path, "rb") as f:
+        n = int(np.frombuffer(f.read(4), np.uint32)[0])
+        tot = int(np.frombuffer(f.read(8), np.uint64)[0])
+        off = np.frombuffer(f.read(8 * (n + 1)), np.uint64).copy()
+        hub = np.frombuffer(f.read(4 * tot), np.uint32).copy()
+        dist = np.frombuffer(f.read(2 * tot), np.int16).copy()
+    return off, hub, dist
+
+
+def run_ref_weighted(edges, weights, rank=None, pairs=None, threads=1):
+    """Reference WeightedParaVertexCentricPLL<512> build (rank-space labels) and
+    query answers for original-id pairs."""
+    env = dict(os.environ)
+    env["OMP_NUM_THREADS"] = str(threads)
+    e = _c(edges, np.uint32).reshape(-1, 2)
+    w = _c(weights, np.uint32)
+    with tempfile.TemporaryDirectory() as td:
+        ep = os.path.join(td, "w.bin")
+        with open(ep, "wb") as f:
+            f.write(np.uint64(e.shape[0]).tobytes())
+            f.write(np.column_stack([e, w]).astype(np.uint32).tobytes())
+        rp = "-"
+        if rank is not None:
+            rp = os.path.join(td, "r.bin")
+            _write_rank(rp, rank)
+        pp = "-"
+        if pairs is not None:
+            pp = os.path.join(td, "p.bin")
+            with open(pp, "wb") as f:
+                f.write(np.uint64(len(pairs[0])).tobytes())
+                f.write(_c(pairs[0], np.uint32).tobytes())
+                f.write(_c(pairs[1], np.uint32).tobytes())
+        pre = os.path.join(td, "out")
+        cp = subprocess.run([REF_BIN, "w", ep, rp, pre, pp], env=env, capture_output=True, text=True, check=True)
+        res = {"stdout": cp.stdout, "labels": read_wlabels(pre + ".wlabels.bin")}
+        if pairs is not None:
+            res["answers"] = np.fromfile(pre + ".answers.bin", np.int16)
+    return res

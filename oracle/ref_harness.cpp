@@ -254,6 +254,32 @@ int main(int argc, char **argv)
         auto *p = new WeightedParaVertexCentricPLL<512>(G);
         double t1 = now_s();
         printf("ref_build_wall_s: %.6f threads: %d\n", t1 - t0, omp_get_max_threads());
+        if (std::string(argv[4]) != "-") {
+            // Canonical weighted labels: rank-space vertex order, (hub, dist) sorted.
+            idi n = G.num_v;
+            std::vector<uint64_t> off(n + 1, 0);
+            std::vector<uint32_t> hub;
+            std::vector<int16_t> dist;
+            for (idi v = 0; v < n; ++v) {
+                std::vector<std::pair<idi, int16_t>> ol;
+                for (size_t i = 0; i < p->L[v].vertices.size(); ++i)
+                    ol.push_back(std::make_pair(p->L[v].vertices[i], p->L[v].distances[i]));
+                std::sort(ol.begin(), ol.end());
+                for (auto &x : ol) {
+                    hub.push_back(x.first);
+                    dist.push_back(x.second);
+                }
+                off[v + 1] = hub.size();
+            }
+            FILE *g = fopen((std::string(argv[4]) + ".wlabels.bin").c_str(), "wb");
+            uint64_t tot = hub.size();
+            fwrite(&n, 4, 1, g);
+            fwrite(&tot, 8, 1, g);
+            fwrite(off.data(), 8, n + 1, g);
+            fwrite(hub.data(), 4, tot, g);
+            fwrite(dist.data(), 2, tot, g);
+            fclose(g);
+        }
         if (argc > 5 && std::string(argv[5]) != "-") {
             p->order_labels(rank2id, rank);
             FILE *g = fopen(argv[5], "rb");

@@ -15,6 +15,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <thread>
 #include <vector>
 
 #include "pado_gpu.h"
@@ -31,6 +32,26 @@ inline uint64_t splitmix64(uint64_t &s)
 
 inline double unit53(uint64_t &s) { return (double)(splitmix64(s) >> 11) * (1.0 / 9007199254740992.0); }
 
+// Static partition of [0, count) over the host's cores.
+template <class F>
+void parallel_for(uint64_t count, F f)
+{
+    unsigned t = std::thread::hardware_concurrency();
+    if (t == 0) t = 1;
+    if (t > 32) t = 32;
+    if (count < (1u << 16)) t = 1;
+    std::vector<std::thread> th;
+    const uint64_t per = (count + t - 1) / t;
+    for (unsigned k = 0; k < t; ++k) {
+        const uint64_t lo = k * per, hi = std::min<uint64_t>(count, lo + per);
+        if (lo >= hi) break;
+        th.emplace_back([=, &f] {
+            for (uint64_t i = lo; i < hi; ++i) f(i);
+        });
+    }
+    for (auto &x : th) x.join();
+}
+
 }  // namespace
 
 extern "C" {
@@ -43,11 +64,13 @@ uint64_t pado_rmat_edges(uint32_t scale, uint32_t edge_factor, double a, double 
                          uint64_t seed, uint32_t *out, uint64_t cap)
 {
     const uint64_t samples = (uint64_t)edge_factor << scale;
-    std::vector<uint64_t> keys;
-    keys.reserve(samples);
-    uint64_t s = seed;
+    // Sample i uses draws i*scale .. i*scale+scale-1 of one splitmix64 stream;
+    // the stream's state after j draws is seed + j*gamma, so samples are
+    // generated independently (in parallel) with the same result.
+    std::vector<uint64_t> keys(samples);
     const double ab = a + b, abc = a + b + c;
-    for (uint64_t i = 0; i < samples; ++i) {
+    parallel_for(samples, [&](uint64_t i) {
+        uint64_t s = seed + (i * scale) * 0x9E3779B97F4A7C15ull;
         uint32_t u = 0, v = 0;
         for (uint32_t l = 0; l < scale; ++l) {
             double r = unit53(s);
@@ -64,11 +87,11 @@ uint64_t pado_rmat_edges(uint32_t scale, uint32_t edge_factor, double a, double 
             u = (u << 1) | bu;
             v = (v << 1) | bv;
         }
-        if (u == v) continue;
         if (u > v) std::swap(u, v);
-        keys.push_back(((uint64_t)u << 32) | v);
-    }
+        keys[i] = (u == v) ? ~0ull : (((uint64_t)u << 32) | v);  // self-loops sort last, dropped
+    });
     std::sort(keys.begin(), keys.end());
+    while (!keys.empty() && keys.back() == ~0ull) keys.pop_back();
     keys.erase(std::unique(keys.begin(), keys.end()), keys.end());
     if ((uint64_t)keys.size() > cap || out == nullptr) return keys.size();
     for (size_t i = 0; i < keys.size(); ++i) {

@@ -94,6 +94,36 @@ def rmat(scale, ef, n_pairs, full_labels, threads=8):
     print("wrote", tag, meta["num_labels"], "labels")
 
 
+def read_weighted(path):
+    rows = []
+    with open(path) as f:
+        for line in f:
+            if not line or line[0] in "#%":
+                continue
+            t = line.split()
+            if len(t) >= 3:
+                rows.append((int(t[0]), int(t[1]), int(t[2])))
+    a = np.array(rows, np.uint32)
+    return a[:, :2], a[:, 2]
+
+
+def weighted(tag, edges, weights, n_pairs=None):
+    """Reference WeightedParaVertexCentricPLL<512>, one OpenMP thread."""
+    n = int(edges.max()) + 1
+    rank = oracle.ref_rank(edges)
+    if n_pairs is None:
+        a, b = np.meshgrid(np.arange(n, dtype=np.uint32), np.arange(n, dtype=np.uint32), indexing="ij")
+        qa, qb = a.reshape(-1), b.reshape(-1)
+    else:
+        qa, qb = pado.uniform_pairs(n, n_pairs, seed=1)
+    R = oracle.run_ref_weighted(edges, weights, rank, pairs=(qa, qb), threads=1)
+    loff, hub, dist = R["labels"]
+    out = os.path.join(HERE, f"weighted_{tag}.npz")
+    np.savez_compressed(out, edges=edges, weights=weights, rank=rank, loff=loff, hub=hub, dist=dist, qa=qa, qb=qb,
+                        answers=R["answers"])
+    print("wrote", out, "n", n, "labels", hub.size)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--big", action="store_true", help="also R-MAT 16/18/20 digests (minutes)")
@@ -105,6 +135,13 @@ def main():
     rmat(10, 8, 20000, True)
     rmat(12, 8, 20000, True)
     rmat(14, 8, 100000, True)
+    for name in ("weighted_edgelist.in", "7_w_edgelist.in", "1_w_edgelist.in", "triangle.in", "rectangle.in"):
+        e, w = read_weighted(os.path.join(REF_TOOLS, name))
+        weighted(name.replace(".", "_"), e, w)
+    for scale in (10, 12):
+        e = rmat_config(scale, 8)
+        w = pado.edge_weights(e.shape[0], 1, 100, seed=1)
+        weighted(f"rmat{scale}_ef8", e, w, n_pairs=20000)
     if args.big:
         rmat(16, 16, 100000, False)
         rmat(18, 16, 100000, False)
