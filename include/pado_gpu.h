@@ -67,6 +67,17 @@ int pado_make_rank(const uint64_t *off, uint32_t n, int use_global_rand, uint32_
 /* Graph::id_transfer (GR:176-206); rank2id may be NULL. */
 void pado_id_transfer(uint32_t n, const uint64_t *off, const uint32_t *adj, const uint32_t *rank,
                       uint64_t *off_new, uint32_t *adj_new, uint32_t *rank2id);
+/* WeightedGraph::construct (GR:313-351): CSR plus u8 weights w[2m] (weights
+ * cast to weighti as the reference's file reader does, GR:277-304).
+ * WeightedGraph::make_rank (GR:353-372) is the same degree order as
+ * Graph::make_rank, so pado_make_rank applies. */
+void pado_wgraph_csr(const uint32_t *edges, const uint32_t *weights, uint64_t m, uint32_t n, uint64_t *off,
+                     uint32_t *adj, uint8_t *w);
+/* WeightedGraph::id_transfer (GR:374-408): lists of (id, weight) sorted
+ * descending; rank2id may be NULL. */
+void pado_wid_transfer(uint32_t n, const uint64_t *off, const uint32_t *adj, const uint8_t *w,
+                       const uint32_t *rank, uint64_t *off_new, uint32_t *adj_new, uint8_t *w_new,
+                       uint32_t *rank2id);
 
 /* Deterministic workload generators (no reference counterpart; they produce
  * the synthetic inputs the reference's scripts would download).  Return the
@@ -191,6 +202,25 @@ pado_gpu_status pado_gpu_index_store(pado_gpu_ctx *ctx, const pado_gpu_index *id
 /* The loaded index is ordered (query by original id) and has no rank-space
  * copy. */
 pado_gpu_status pado_gpu_index_load(pado_gpu_ctx *ctx, const char *filename, pado_gpu_index **out);
+
+/* ---------------------------------------------------------------------------
+ * Weighted graphs (WeightedParaVertexCentricPLL<512>, WPARA).  A graph
+ * uploaded with weights is built by pado_gpu_build into a weighted index:
+ * labels (hub, u32 distance), no bit-parallel part.  The label sets are the
+ * canonical pruned landmark labels -- (r, d(v,r)) is a label of v iff r ranks
+ * at or above v and no vertex ranked above r lies on a shortest v-r path --
+ * which is what the reference's weighted build produces.
+ * ------------------------------------------------------------------------ */
+#define PADO_GPU_WINF 0xFFFFFFFFu
+
+/* Canonical rank-space export: label_off[n+1], hubs ascending, u32 dists. */
+pado_gpu_status pado_gpu_windex_export(pado_gpu_ctx *ctx, const pado_gpu_index *idx, uint64_t *label_off,
+                                       uint32_t *hubs, uint32_t *dists);
+/* query_distance (WPARA:2125-2243) for k pairs: exact u32 distances,
+ * PADO_GPU_WINF if unreachable; with PADO_GPU_QUERY_REF_COMPAT the reference's
+ * int16 domain (>= 32765 -> 32767).  Flags as for pado_gpu_query. */
+pado_gpu_status pado_gpu_wquery(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uint32_t *a, const uint32_t *b,
+                                uint64_t k, uint32_t *out, uint32_t flags, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Autotuner (no reference counterpart).  Every knob of pado_gpu_build_cfg

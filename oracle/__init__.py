@@ -310,7 +310,7 @@ def _wlib():
 
 
 def weighted_graph_csr(edges, weights):
-    """WeightedGraph(edge_list, weight_list) (GR:302-340): CSR + u8 weights."""
+    """WeightedGraph(edge_list, weight_list) (GR:306-351): CSR + u8 weights."""
     e = _c(edges, np.uint32).reshape(-1, 2)
     w = _c(weights, np.uint32).astype(np.uint8)
     n = int(e.max()) + 1 if e.size else 0

@@ -29,7 +29,7 @@
 
 typedef int16_t wl;   /* weightiLarge */
 
-/* id_transfer for WeightedGraph (GR:352-386): adjacency of each vertex sorted
+/* id_transfer for WeightedGraph (GR:374-408): adjacency of each vertex sorted
  * descending as (id, weight) pairs. */
 typedef struct { uint32_t id; uint8_t w; } wo_pair;
 static int wo_pair_desc(const void *a, const void *b)

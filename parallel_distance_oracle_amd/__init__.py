@@ -29,6 +29,7 @@ QUERY_DEVICE_PTRS = 1
 QUERY_REF_COMPAT = 2
 QUERY_RANK_SPACE = 4
 BUILD_BP_PER_SLOT = 1
+WINF = 0xFFFFFFFF  # weighted query: no common hub (unreachable)
 
 _STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_NOMEM", 4: "ERR_DOMAIN", 5: "ERR_STATE",
            6: "ERR_IO", 7: "ERR_BUSY"}
@@ -94,6 +95,10 @@ def lib():
         "pado_graph_csr": (None, [_vp, _u64, _u32, _vp, _vp]),
         "pado_make_rank": (ctypes.c_int, [_vp, _u32, ctypes.c_int, _vp]),
         "pado_id_transfer": (None, [_u32, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "pado_wgraph_csr": (None, [_vp, _vp, _u64, _u32, _vp, _vp, _vp]),
+        "pado_wid_transfer": (None, [_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "pado_gpu_windex_export": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+        "pado_gpu_wquery": (ctypes.c_int, [_vp, _vp, _vp, _vp, _u64, _vp, _u32, _vp]),
         "pado_rmat_edges": (_u64, [_u32, _u32, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u64, _vp, _u64]),
         "pado_grid_edges": (_u64, [_u32, ctypes.c_double, _u64, _vp, _u64]),
         "pado_edge_weights": (None, [_u64, _u32, _u32, _u64, _vp]),
@@ -268,6 +273,63 @@ class Graph:
         return r2i[:n]
 
 
+class WeightedGraph(Graph):
+    """The reference's WeightedGraph (includes/graph.h:223-408): Graph's CSR
+    plus one u8 weight per directed edge (weights cast to weighti, as its file
+    reader does).  Its make_rank is Graph's degree order."""
+
+    def __init__(self, num_v, off, adj, w):
+        super().__init__(num_v, off, adj)
+        self.w = _c(w, np.uint8)
+
+    @classmethod
+    def from_edges(cls, edges, weights):
+        e = _c(edges, np.uint32).reshape(-1)
+        m = e.size // 2
+        wt = _c(weights, np.uint32).reshape(-1)
+        if wt.size != m:
+            raise ValueError(f"{wt.size} weights for {m} edges")
+        L = lib()
+        n = L.pado_graph_num_v(_p(e), m) if m else 0
+        off = np.zeros(n + 1, np.uint64)
+        adj = np.zeros(max(2 * m, 1), np.uint32)
+        w = np.zeros(max(2 * m, 1), np.uint8)
+        if n:
+            L.pado_wgraph_csr(_p(e), _p(wt), m, n, _p(off), _p(adj), _p(w))
+        return cls(n, off, adj[: 2 * m], w[: 2 * m])
+
+    @classmethod
+    def from_file(cls, path):
+        """WeightedGraph(const char*) (includes/graph.h:277-304): 'a b w' rows,
+        '#'/'%' lines skipped."""
+        rows = []
+        with open(path) as f:
+            for line in f:
+                if not line or line[0] in "#%":
+                    continue
+                t = line.split()
+                if len(t) >= 3:
+                    rows.append((int(t[0]), int(t[1]), int(t[2])))
+        a = np.array(rows, dtype=np.int64).reshape(-1, 3)
+        return cls.from_edges(a[:, :2].astype(np.uint32), (a[:, 2] & 0xFF).astype(np.uint32))
+
+    def id_transfer(self, rank):
+        """Renumber in place into rank space (includes/graph.h:374-408);
+        returns rank2id."""
+        rank = _c(rank, np.uint32)
+        n = self.num_v
+        off2 = np.zeros(n + 1, np.uint64)
+        adj2 = np.zeros(max(self.adj.size, 1), np.uint32)
+        w2 = np.zeros(max(self.adj.size, 1), np.uint8)
+        r2i = np.zeros(max(n, 1), np.uint32)
+        src_adj = self.adj if self.adj.size else adj2
+        src_w = self.w if self.w.size else w2
+        lib().pado_wid_transfer(n, _p(self.off), _p(src_adj), _p(src_w), _p(rank), _p(off2), _p(adj2), _p(w2),
+                                _p(r2i))
+        self.off, self.adj, self.w = off2, adj2[: self.adj.size], w2[: self.adj.size]
+        return r2i[:n]
+
+
 # ---------------------------------------------------------------------------
 # Device objects
 # ---------------------------------------------------------------------------
@@ -315,7 +377,11 @@ class DeviceGraph:
     def __init__(self, ctx, G, weights=None):
         self.ctx = ctx
         h = _vp()
+        if weights is None and isinstance(G, WeightedGraph):
+            weights = G.w
         w = _c(weights, np.uint8) if weights is not None else None
+        if w is not None and w.size == 0:
+            w = np.ones(1, np.uint8)  # a non-null pointer marks the graph weighted
         _check(lib().pado_gpu_graph_upload(ctx.h, G.num_v, G.num_e, _p(G.off), _p(G.adj) if G.adj.size else None,
                                            _p(w), ctypes.byref(h)))
         self.h = h
@@ -392,24 +458,48 @@ class Index:
         _check(lib().pado_gpu_index_export(self.ctx.h, self.h, _p(bp_d), _p(bp_s), _p(loff), _p(hub), _p(dist)))
         return bp_d.reshape(n, BP_SIZE), bp_s.reshape(n, BP_SIZE, 2), loff, hub[:tot], dist[:tot]
 
+    @property
+    def weighted(self):
+        return bool(self.info.weighted)
+
+    def wexport(self):
+        """Weighted index, canonical rank space: (loff[n+1], hub u32, dist u32)."""
+        n = self.num_v
+        tot = self.info.num_labels
+        loff = np.zeros(n + 1, np.uint64)
+        hub = np.zeros(max(tot, 1), np.uint32)
+        dist = np.zeros(max(tot, 1), np.uint32)
+        _check(lib().pado_gpu_windex_export(self.ctx.h, self.h, _p(loff), _p(hub), _p(dist)))
+        return loff, hub[:tot], dist[:tot]
+
     def order(self, rank):
         rank = _c(rank, np.uint32)
         _check(lib().pado_gpu_index_order(self.ctx.h, self.h, _p(rank), rank.size))
         self.info = self.get_info()
 
     def query(self, a, b, ref_compat=False, rank_space=False):
+        """Distances for pairs (a[i], b[i]): u8 for an unweighted index, u32
+        (WINF = unreachable) for a weighted one."""
         a = _c(a, np.uint32).reshape(-1)
         b = _c(b, np.uint32).reshape(-1)
-        out = np.zeros(max(a.size, 1), np.uint8)
+        if a.size != b.size:
+            raise ValueError("a and b differ in length")
         flags = (QUERY_REF_COMPAT if ref_compat else 0) | (QUERY_RANK_SPACE if rank_space else 0)
-        _check(lib().pado_gpu_query(self.ctx.h, self.h, _p(a), _p(b), a.size, _p(out), flags, None))
+        if self.weighted:
+            out = np.zeros(max(a.size, 1), np.uint32)
+            _check(lib().pado_gpu_wquery(self.ctx.h, self.h, _p(a), _p(b), a.size, _p(out), flags, None))
+        else:
+            out = np.zeros(max(a.size, 1), np.uint8)
+            _check(lib().pado_gpu_query(self.ctx.h, self.h, _p(a), _p(b), a.size, _p(out), flags, None))
         return out[: a.size]
 
     def query_device(self, a_ptr, b_ptr, k, out_ptr, ref_compat=False, rank_space=False, stream=None):
-        """Asynchronous query on device buffers (e.g. torch tensors' data_ptr())."""
+        """Asynchronous query on device buffers (e.g. torch tensors' data_ptr());
+        out holds u8 (unweighted) or u32 (weighted) distances."""
         flags = QUERY_DEVICE_PTRS | (QUERY_REF_COMPAT if ref_compat else 0) | (QUERY_RANK_SPACE if rank_space else 0)
-        _check(lib().pado_gpu_query(self.ctx.h, self.h, _vp(a_ptr), _vp(b_ptr), k, _vp(out_ptr), flags,
-                                    _vp(stream) if stream else None))
+        fn = lib().pado_gpu_wquery if self.weighted else lib().pado_gpu_query
+        _check(fn(self.ctx.h, self.h, _vp(a_ptr), _vp(b_ptr), k, _vp(out_ptr), flags,
+                  _vp(stream) if stream else None))
 
     def store(self, filename, rank):
         rank = _c(rank, np.uint32)

@@ -117,6 +117,7 @@ struct pado_gpu_graph {
     pado::DevBuf<uint64_t> off;
     pado::DevBuf<uint32_t> adj;
     pado::DevBuf<uint8_t> w;  // edge weights (weighted graphs), else empty
+    bool weighted = false;
     std::vector<uint64_t> h_off;  // host copy: root selection is a serial pass
     std::vector<uint32_t> h_adj;
     std::vector<uint8_t> h_w;

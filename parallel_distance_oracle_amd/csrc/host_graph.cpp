@@ -239,4 +239,47 @@ void pado_id_transfer(uint32_t n, const uint64_t *off, const uint32_t *adj, cons
     if (rank2id) std::copy(r2i.begin(), r2i.end(), rank2id);
 }
 
+// WeightedGraph::construct (includes/graph.h:313-351): edge i appended to both
+// endpoints' lists in input order, weight cast to the u8 edge weight type.
+void pado_wgraph_csr(const uint32_t *edges, const uint32_t *weights, uint64_t m, uint32_t n, uint64_t *off,
+                     uint32_t *adj, uint8_t *w)
+{
+    pado_graph_csr(edges, m, n, off, adj);
+    std::vector<uint64_t> pos(off, off + n);
+    for (uint64_t i = 0; i < m; ++i) {
+        const uint8_t wi = (uint8_t)weights[i];
+        w[pos[edges[2 * i]]++] = wi;
+        w[pos[edges[2 * i + 1]]++] = wi;
+    }
+}
+
+// WeightedGraph::id_transfer (includes/graph.h:374-408): each list of
+// (rank-space id, weight) pairs sorted descending, so parallel edges to one
+// neighbour keep the heavier weight first.
+void pado_wid_transfer(uint32_t n, const uint64_t *off, const uint32_t *adj, const uint8_t *w,
+                       const uint32_t *rank, uint64_t *off_new, uint32_t *adj_new, uint8_t *w_new,
+                       uint32_t *rank2id)
+{
+    std::vector<uint32_t> r2i(n);
+    for (uint32_t v = 0; v < n; ++v) r2i[rank[v]] = v;
+    off_new[0] = 0;
+    for (uint32_t nv = 0; nv < n; ++nv) {
+        uint32_t v = r2i[nv];
+        off_new[nv + 1] = off_new[nv] + (off[v + 1] - off[v]);
+    }
+    std::vector<uint64_t> key;
+    for (uint32_t nv = 0; nv < n; ++nv) {
+        uint32_t v = r2i[nv];
+        key.clear();
+        for (uint64_t e = off[v]; e < off[v + 1]; ++e) key.push_back(((uint64_t)rank[adj[e]] << 8) | w[e]);
+        std::sort(key.begin(), key.end(), std::greater<uint64_t>());
+        uint64_t o = off_new[nv];
+        for (uint64_t k : key) {
+            adj_new[o] = (uint32_t)(k >> 8);
+            w_new[o++] = (uint8_t)(k & 255u);
+        }
+    }
+    if (rank2id) std::copy(r2i.begin(), r2i.end(), rank2id);
+}
+
 }  // extern "C"

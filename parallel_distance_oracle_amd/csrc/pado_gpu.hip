@@ -16,6 +16,8 @@ void clear_error() { g_last_error.clear(); }
 
 pado_gpu_status launch_query(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,
                              uint64_t k, uint8_t *out, bool ref_compat, hipStream_t st);
+pado_gpu_status launch_wquery(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,
+                              uint64_t k, uint32_t *out, bool ref_compat, hipStream_t st);
 
 namespace {
 
@@ -225,7 +227,18 @@ pado_gpu_status pado_gpu_graph_upload(pado_gpu_ctx *ctx, uint32_t n, uint64_t m_
     g->m = m_dir;
     g->h_off.assign(off, off + n + 1);
     g->h_adj.assign(adj, adj + m_dir);
-    if (weights) g->h_w.assign(weights, weights + m_dir);
+    if (weights) {
+        // Pruning is exact only for positive lengths: a zero-length edge makes
+        // two vertices equidistant to every hub.
+        for (uint64_t e = 0; e < m_dir; ++e)
+            if (weights[e] == 0) {
+                delete g;
+                set_error("edge weights must be in [1, 255]");
+                return PADO_GPU_ERR_ARG;
+            }
+        g->h_w.assign(weights, weights + m_dir);
+        g->weighted = true;
+    }
     pado_gpu_status s = g->off.alloc((uint64_t)n + 1);
     if (s == PADO_GPU_OK) s = g->adj.alloc(m_dir);
     if (s == PADO_GPU_OK && weights) s = g->w.alloc(m_dir);
@@ -267,10 +280,12 @@ pado_gpu_status pado_gpu_build(pado_gpu_ctx *ctx, const pado_gpu_graph *graph, c
     idx->n = graph->n;
     pado_gpu_graph *g = const_cast<pado_gpu_graph *>(graph);
     pado_gpu_status s = PADO_GPU_OK;
-    if (!g->h_w.empty()) {
-        delete idx;
-        set_error("weighted graphs are built by pado_gpu_wbuild");
-        return PADO_GPU_ERR_ARG;
+    if (g->weighted) {
+        // WeightedParaVertexCentricPLL<512> (WPARA:1231-1566): no
+        // bit-parallel labels; batch_size above 512 is clamped to 512.
+        Timer t1;
+        s = build_weighted(g, idx, c.batch_size);
+        idx->ms_batch = t1.ms();
     } else {
         DevBuf<uint8_t> used;
         std::vector<uint8_t> h_used;
@@ -481,6 +496,64 @@ pado_gpu_status pado_gpu_query(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uin
     PADO_HIP(hipMemcpyAsync(idx->qb.p, b, sizeof(uint32_t) * k, hipMemcpyHostToDevice, st));
     PADO_TRY(launch_query(idx, rank_space, idx->qa.p, idx->qb.p, k, idx->qout.p, ref, st));
     PADO_HIP(hipMemcpyAsync(out, idx->qout.p, k, hipMemcpyDeviceToHost, st));
+    PADO_HIP(hipStreamSynchronize(st));
+    return PADO_GPU_OK;
+}
+
+pado_gpu_status pado_gpu_windex_export(pado_gpu_ctx *ctx, const pado_gpu_index *idx, uint64_t *label_off,
+                                       uint32_t *hubs, uint32_t *dists)
+{
+    clear_error();
+    PADO_TRY(check_ctx(ctx));
+    if (!idx) {
+        set_error("null index");
+        return PADO_GPU_ERR_ARG;
+    }
+    if (!idx->weighted) {
+        set_error("unweighted index: use pado_gpu_index_export");
+        return PADO_GPU_ERR_STATE;
+    }
+    hipStream_t st = ctx->stream;
+    const uint64_t L = idx->num_labels;
+    if (label_off)
+        PADO_HIP(hipMemcpyAsync(label_off, idx->loff.p, sizeof(uint64_t) * (idx->n + 1), hipMemcpyDeviceToHost, st));
+    if (hubs && L) PADO_HIP(hipMemcpyAsync(hubs, idx->whub.p, sizeof(uint32_t) * L, hipMemcpyDeviceToHost, st));
+    if (dists && L) PADO_HIP(hipMemcpyAsync(dists, idx->wdist.p, sizeof(uint32_t) * L, hipMemcpyDeviceToHost, st));
+    PADO_HIP(hipStreamSynchronize(st));
+    return PADO_GPU_OK;
+}
+
+pado_gpu_status pado_gpu_wquery(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uint32_t *a, const uint32_t *b,
+                                uint64_t k, uint32_t *out, uint32_t flags, void *stream)
+{
+    clear_error();
+    PADO_TRY(check_ctx(ctx));
+    if (!idx || (k && (!a || !b || !out))) {
+        set_error("null argument");
+        return PADO_GPU_ERR_ARG;
+    }
+    if (!idx->weighted) {
+        set_error("unweighted index: use pado_gpu_query");
+        return PADO_GPU_ERR_STATE;
+    }
+    const bool rank_space = (flags & PADO_GPU_QUERY_RANK_SPACE) != 0;
+    if (!rank_space && !idx->ordered) {
+        set_error("query before order_labels: call pado_gpu_index_order or pass PADO_GPU_QUERY_RANK_SPACE");
+        return PADO_GPU_ERR_STATE;
+    }
+    if (k == 0) return PADO_GPU_OK;
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    const bool ref = (flags & PADO_GPU_QUERY_REF_COMPAT) != 0;
+    if (flags & PADO_GPU_QUERY_DEVICE_PTRS) return launch_wquery(idx, rank_space, a, b, k, out, ref, st);
+    if (idx->qa.n < k) {
+        PADO_TRY(idx->qa.alloc(k));
+        PADO_TRY(idx->qb.alloc(k));
+    }
+    if (idx->qout32.n < k) PADO_TRY(idx->qout32.alloc(k));
+    PADO_HIP(hipMemcpyAsync(idx->qa.p, a, sizeof(uint32_t) * k, hipMemcpyHostToDevice, st));
+    PADO_HIP(hipMemcpyAsync(idx->qb.p, b, sizeof(uint32_t) * k, hipMemcpyHostToDevice, st));
+    PADO_TRY(launch_wquery(idx, rank_space, idx->qa.p, idx->qb.p, k, idx->qout32.p, ref, st));
+    PADO_HIP(hipMemcpyAsync(out, idx->qout32.p, sizeof(uint32_t) * k, hipMemcpyDeviceToHost, st));
     PADO_HIP(hipStreamSynchronize(st));
     return PADO_GPU_OK;
 }

@@ -92,6 +92,67 @@ __global__ __launch_bounds__(256) void k_query(uint32_t n, const uint8_t *bpd, c
     }
 }
 
+// Weighted query_distance (WPARA:2125-2243): min over common hubs of
+// La(h) + Lb(h).  Exact u32 answers, 0xFFFFFFFF when no hub is shared; with
+// ref_compat the reference's int16 domain (>= 32765 -> 32767, ids out of
+// range -> a == b ? 0 : 32767).  One wave per pair, the shorter list's hubs
+// binary-searched in the longer one as for unweighted labels.
+__global__ __launch_bounds__(256) void k_wquery(uint32_t n, const uint64_t *loff, const uint32_t *hub,
+                                                const uint32_t *dist, const uint32_t *qa, const uint32_t *qb,
+                                                uint64_t k, uint32_t *out, int ref_compat)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t inf = ref_compat ? 32767u : 0xFFFFFFFFu;
+    const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
+    for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < k; i += nw) {
+        const uint32_t a = qa[i], b = qb[i];
+        if (a >= n || b >= n) {
+            if (lane == 0) out[i] = a == b ? 0u : inf;
+            continue;
+        }
+        const uint64_t oa = loff[a], ob = loff[b];
+        const uint32_t la = (uint32_t)(loff[a + 1] - oa), lb = (uint32_t)(loff[b + 1] - ob);
+        const uint64_t os = la <= lb ? oa : ob, ot = la <= lb ? ob : oa;
+        const uint32_t ls = la <= lb ? la : lb, lt = la <= lb ? lb : la;
+        uint32_t d = 0xFFFFFFFFu;
+        for (uint32_t j0 = 0; j0 < ls; j0 += 64) {
+            const uint32_t j = j0 + lane;
+            if (j < ls) {
+                const uint32_t key = hub[os + j];
+                uint32_t lo = 0, hi = lt;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (hub[ot + mid] < key)
+                        lo = mid + 1;
+                    else
+                        hi = mid;
+                }
+                if (lo < lt && hub[ot + lo] == key) {
+                    const uint64_t td = (uint64_t)dist[os + j] + dist[ot + lo];
+                    if (td < d) d = (uint32_t)td;
+                }
+            }
+        }
+        d = wave_min_u32(d);
+        if (ref_compat && d >= 32765u) d = 32767u;
+        if (lane == 0) out[i] = d;
+    }
+}
+
+pado_gpu_status launch_wquery(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,
+                              uint64_t k, uint32_t *out, bool ref_compat, hipStream_t st)
+{
+    if (k == 0) return PADO_GPU_OK;
+    const uint64_t *loff = rank_space ? idx->loff.p : idx->qoff.p;
+    const uint32_t *hub = rank_space ? idx->whub.p : idx->qlab.p;
+    const uint32_t *dist = rank_space ? idx->wdist.p : idx->qwdist.p;
+    uint32_t grid = grid_for(k * 64, 256, 65536);
+    hipLaunchKernelGGL(k_wquery, dim3(grid), dim3(256), 0, st, idx->n, loff, hub, dist, qa, qb, k, out,
+                       ref_compat ? 1 : 0);
+    PADO_HIP(hipGetLastError());
+    return PADO_GPU_OK;
+}
+
 pado_gpu_status launch_query(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,
                              uint64_t k, uint8_t *out, bool ref_compat, hipStream_t st)
 {

@@ -1,0 +1,692 @@
+// wbatch.hip -- weighted pruned landmark labeling on gfx950.
+//
+// Reference: WeightedParaVertexCentricPLL<512>, includes/pado_weighted_unv_para.h
+// (initialize_tables :305-383, send_messages :435-522, distance_query
+// :527-914, filter_out_labels :1100-1228, reset_tables :918-956,
+// update_index :961-990, vertex_centric_labeling_in_batches :1231-1566).
+//
+// The reference's labels are the canonical pruned landmark labels: (r, d) is
+// a label of v iff r ranks at or above v, d = d(v, r), and no vertex ranked
+// above r lies on a shortest v-r path.  Its schedule (queue orders, first
+// witness returned by the distance query, in-place filtering) does not change
+// that set, and this build reaches the same set with a schedule-free scheme,
+// per batch of roots:
+//   1. Bellman-Ford rounds from all batch roots at once.  A root's distance
+//      travels only to lower-ranked tails (send_messages, WPARA:457-470); a
+//      (tail, root) value that improves is checked against the labels of
+//      earlier batches (distance_query's first loop, WPARA:569-580): if a
+//      2-hop path through an earlier hub is as short, the pair is marked
+//      pruned and does not propagate.  Rounds end when nothing improves.
+//   2. Filter (filter_out_labels): a surviving pair (v, r) is dropped if some
+//      batch root h < r has surviving pairs (v, h), (r, h) with
+//      S(v,h) + S(r,h) <= S(v,r).
+// Why this is exact: a canonical pair's shortest paths avoid every vertex
+// ranked above r, so the restricted search finds d(v,r) and no earlier hub can
+// prune it; a non-canonical pair has some higher vertex on a shortest path,
+// and the highest such vertex h* either belongs to an earlier batch (then the
+// earlier labels prune the pair at any value >= d) or to this batch (then
+// (v,h*) and (r,h*) are canonical, exact, and witness the filter).
+//
+// Layout per batch (B = 512 roots): T[n][B] u32 tentative distances, P and I
+// bitmaps (pruned at the current value / improved this round) [n][B/64] u64,
+// the earlier-batch hub table (hslot, hmin, dcw[slot][B] u32), and the
+// frontier as (vertex, root bitmap) entries with 16-edge push chunks.  Labels
+// are packed u64 (hub << 32 | dist) in per-vertex growable arrays.
+#include "internal.hpp"
+
+namespace pado {
+namespace {
+
+constexpr uint32_t kWB = 512;         // roots per batch (WPARA BATCH_SIZE)
+constexpr uint32_t kWW = kWB / 64;    // 64-bit words per root bitmap
+constexpr uint32_t kWInf = 0xFFFFFFFFu;
+using u64 = unsigned long long;
+
+struct WCounters {
+    uint32_t n_act, n_chunk, n_imp, n_touched, n_hub, overflow, pad0, pad1;
+    u64 pool_top, relax, improved, pruned, labels, filtered, rounds, need;
+};
+
+struct WStore {
+    u64 *pool;
+    u64 *base;
+    uint32_t *len, *cap;
+    u64 pool_size;
+};
+
+__device__ inline uint32_t lane_id() { return threadIdx.x & 63; }
+
+__device__ inline u64 wave_sum(u64 x)
+{
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+// Tails > lo form a prefix of a descending adjacency list (WPARA:457 breaks
+// at the first tail <= roots_start).
+__device__ inline uint32_t prefix_above(const uint64_t *off, const uint32_t *adj, uint32_t v, uint32_t lo)
+{
+    const uint32_t lane = lane_id();
+    const uint64_t base = off[v];
+    uint32_t a = 0, b = (uint32_t)(off[v + 1] - base);
+    while (b - a > 64) {
+        uint32_t step = (b - a + 63) / 64;
+        uint32_t pos = a + lane * step;
+        bool p = pos < b && adj[base + pos] > lo;
+        uint32_t c = (uint32_t)__popcll(__ballot(p));
+        if (c == 0) {
+            b = a;
+        } else {
+            uint32_t na = a + (c - 1) * step + 1, nb = a + c * step;
+            a = na;
+            if (nb < b) b = nb;
+        }
+    }
+    uint32_t pos = a + lane;
+    bool p = pos < b && adj[base + pos] > lo;
+    return a + (uint32_t)__popcll(__ballot(p));
+}
+
+__device__ inline u64 store_reserve(WStore S, WCounters *C, uint32_t v, uint32_t len, uint32_t add)
+{
+    const uint32_t lane = lane_id();
+    u64 base = S.base[v];
+    uint32_t cap = S.cap[v];
+    if (len + add <= cap) return base;
+    uint32_t ncap = cap * 2;
+    if (ncap < len + add) ncap = len + add;
+    if (ncap < 8) ncap = 8;
+    u64 nb = 0;
+    if (lane == 0) nb = atomicAdd(&C->pool_top, (u64)ncap);
+    nb = __shfl(nb, 0);
+    if (nb + ncap > S.pool_size) {
+        if (lane == 0) atomicOr(&C->overflow, 1u);
+        return ~0ull;
+    }
+    for (uint32_t j = lane; j < len; j += 64) S.pool[nb + j] = S.pool[base + j];
+    if (lane == 0) {
+        S.base[v] = nb;
+        S.cap[v] = ncap;
+    }
+    return nb;
+}
+
+__device__ inline void emit_chunks(uint32_t k, uint32_t pdeg, WCounters *C, u64 *chunks)
+{
+    const uint32_t lane = lane_id();
+    uint32_t nch = (pdeg + kChunk - 1) / kChunk;
+    if (!nch) return;
+    uint32_t c0 = 0;
+    if (lane == 0) c0 = atomicAdd(&C->n_chunk, nch);
+    c0 = __shfl(c0, 0);
+    for (uint32_t q = lane; q < nch; q += 64) chunks[c0 + q] = ((u64)(q * kChunk) << 32) | k;
+}
+
+__global__ void k_w_reset_counters(WCounters *C)
+{
+    C->n_act = C->n_chunk = C->n_imp = C->n_touched = C->n_hub = 0;
+    C->need = 0;
+}
+
+// initialize_tables (WPARA:305-383): every root at distance 0 from itself,
+// active with its own bit.
+__global__ __launch_bounds__(256) void k_w_init(uint32_t rs, uint32_t rsz, uint32_t gen_batch, const uint64_t *off,
+                                                const uint32_t *adj, uint32_t *T, uint32_t *tmark, uint32_t *tlist,
+                                                uint32_t *alist, u64 *ag, uint32_t *amask, uint32_t *aend, u64 *chunks,
+                                                WCounters *C)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t ro = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (ro >= rsz) return;
+    const uint32_t r = rs + ro;
+    uint32_t k = 0;
+    if (lane == 0) {
+        T[(uint64_t)r * kWB + ro] = 0;
+        if (atomicExch(&tmark[r], gen_batch) != gen_batch) tlist[atomicAdd(&C->n_touched, 1u)] = r;
+        k = atomicAdd(&C->n_act, 1u);
+    }
+    k = __shfl(k, 0);
+    if (lane < kWW) ag[(uint64_t)k * kWW + lane] = (lane == (ro >> 6)) ? (1ull << (ro & 63)) : 0ull;
+    const uint32_t pdeg = prefix_above(off, adj, r, rs);
+    if (lane == 0) {
+        alist[k] = r;
+        amask[k] = 1u << (ro >> 6);
+        aend[k] = pdeg;
+    }
+    emit_chunks(k, pdeg, C, chunks);
+}
+
+// Earlier-batch labels of the batch roots -> hub table (roots_labels_buffer,
+// WPARA:322-348, transposed and compacted to the hubs that occur).
+__global__ __launch_bounds__(256) void k_w_hub_mark(uint32_t rs, uint32_t rsz, WStore S, uint32_t *hslot,
+                                                    uint32_t *hlist, WCounters *C)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t ro = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (ro >= rsz) return;
+    const uint32_t r = rs + ro;
+    const uint32_t len = S.len[r];
+    const u64 base = S.base[r];
+    for (uint32_t j = lane; j < len; j += 64) {
+        const uint32_t h = (uint32_t)(S.pool[base + j] >> 32);
+        if (atomicCAS(&hslot[h], kEmpty, kEmpty - 1) == kEmpty) hlist[atomicAdd(&C->n_hub, 1u)] = h;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_w_hub_slots(const uint32_t *hlist, uint32_t *hslot, uint32_t *hmin,
+                                                     uint32_t *dcw, const WCounters *C)
+{
+    const uint32_t nh = C->n_hub;
+    const uint64_t total = (uint64_t)nh * (kWB / 4);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        const uint32_t slot = (uint32_t)(i / (kWB / 4)), part = (uint32_t)(i % (kWB / 4));
+        reinterpret_cast<uint4 *>(dcw + (uint64_t)slot * kWB)[part] = make_uint4(kWInf, kWInf, kWInf, kWInf);
+        if (part == 0) {
+            hslot[hlist[slot]] = slot;
+            hmin[hlist[slot]] = kWInf;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_w_hub_fill(uint32_t rs, uint32_t rsz, WStore S, const uint32_t *hslot,
+                                                    uint32_t *hmin, uint32_t *dcw)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t ro = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    if (ro >= rsz) return;
+    const uint32_t r = rs + ro;
+    const uint32_t len = S.len[r];
+    const u64 base = S.base[r];
+    for (uint32_t j = lane; j < len; j += 64) {
+        const u64 lab = S.pool[base + j];
+        const uint32_t h = (uint32_t)(lab >> 32), d = (uint32_t)lab;
+        dcw[(uint64_t)hslot[h] * kWB + ro] = d;
+        atomicMin(&hmin[h], d);
+    }
+}
+
+// send_messages (WPARA:435-522): relax the frontier's edges toward tails
+// ranked below each root.
+__global__ __launch_bounds__(256) void k_w_push(uint32_t rs, uint32_t gen_batch, uint32_t gen_round,
+                                                const uint64_t *off, const uint32_t *adj, const uint8_t *wgt,
+                                                const uint32_t *alist, const u64 *ag, const uint32_t *amask,
+                                                const uint32_t *aend, const u64 *chunks, uint32_t *T, u64 *I,
+                                                uint32_t *tmark, uint32_t *tlist, uint32_t *imark, uint32_t *ilist,
+                                                WCounters *C)
+{
+    const uint32_t lane = threadIdx.x & (kChunk - 1);
+    const uint32_t ngroups = (gridDim.x * blockDim.x) / kChunk;
+    const uint32_t nc = C->n_chunk;
+    u64 n_relax = 0, n_imp = 0;
+    for (uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) / kChunk; c < nc; c += ngroups) {
+        const u64 desc = chunks[c];
+        const uint32_t k = (uint32_t)desc;
+        const uint32_t e = (uint32_t)(desc >> 32) + lane;
+        if (e >= aend[k]) continue;
+        const uint32_t u = alist[k];
+        const uint64_t ei = off[u] + e;
+        const uint32_t t = adj[ei];
+        const uint32_t w = wgt[ei];
+        const uint32_t lim = t - rs;  // roots with offset < lim rank above t
+        bool got = false;
+        for (uint32_t wm = amask[k]; wm; wm &= wm - 1) {
+            const uint32_t wd = __ffs(wm) - 1;
+            if (lim <= wd * 64) break;
+            u64 g = ag[(uint64_t)k * kWW + wd];
+            if (lim < (wd + 1) * 64) g &= (1ull << (lim - wd * 64)) - 1ull;
+            u64 imp = 0;
+            while (g) {
+                const uint32_t b = __ffsll(g) - 1;
+                g &= g - 1;
+                const uint32_t ro = wd * 64 + b;
+                const uint32_t du = T[(uint64_t)u * kWB + ro];
+                if (du == kWInf) continue;
+                const uint32_t cand = du + w;
+                ++n_relax;
+                const uint32_t old = atomicMin(&T[(uint64_t)t * kWB + ro], cand);
+                if (cand < old) imp |= 1ull << b;
+            }
+            if (imp) {
+                atomicOr(&I[(uint64_t)t * kWW + wd], imp);
+                got = true;
+            }
+        }
+        if (got) {
+            if (tmark[t] != gen_batch && atomicExch(&tmark[t], gen_batch) != gen_batch)
+                tlist[atomicAdd(&C->n_touched, 1u)] = t;
+            if (imark[t] != gen_round && atomicExch(&imark[t], gen_round) != gen_round) {
+                ilist[atomicAdd(&C->n_imp, 1u)] = t;
+                ++n_imp;
+            }
+        }
+    }
+    n_relax = wave_sum(n_relax);
+    n_imp = wave_sum(n_imp);
+    if ((threadIdx.x & 63) == 0) {
+        if (n_relax) atomicAdd(&C->relax, n_relax);
+        if (n_imp) atomicAdd(&C->improved, n_imp);
+    }
+}
+
+__global__ void k_w_mid(WCounters *C)
+{
+    C->n_act = 0;
+    C->n_chunk = 0;
+}
+
+__global__ void k_w_post(WCounters *C)
+{
+    C->n_imp = 0;
+    C->rounds += 1;
+}
+
+// distance_query's first loop (WPARA:569-580) for every improved pair of an
+// improved tail: one wave per tail, lane l owns roots 8l..8l+7.  Pruned pairs
+// stop propagating; the others become the tail's next frontier bitmap.
+__global__ __launch_bounds__(256) void k_w_check(uint32_t rs, const uint32_t *ilist, const uint64_t *off,
+                                                 const uint32_t *adj, uint32_t *T, u64 *I, u64 *P, WStore S,
+                                                 const uint32_t *hslot, const uint32_t *hmin, const uint32_t *dcw,
+                                                 uint32_t *alist, u64 *ag, uint32_t *amask, uint32_t *aend,
+                                                 u64 *chunks, WCounters *C)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t nimp = C->n_imp;
+    u64 n_pruned = 0;
+    for (uint32_t i = wave; i < nimp; i += nwaves) {
+        const uint32_t t = ilist[i];
+        uint8_t *Ib = reinterpret_cast<uint8_t *>(I + (uint64_t)t * kWW);
+        uint8_t *Pb = reinterpret_cast<uint8_t *>(P + (uint64_t)t * kWW);
+        const uint32_t imp = Ib[lane];
+        // Clear after every lane has read (the store depends on the ballot).
+        if (__ballot(imp != 0) != 0ull) Ib[lane] = 0;
+        uint32_t dv[8];
+        uint32_t maxd = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            dv[q] = (imp >> q) & 1u ? T[(uint64_t)t * kWB + lane * 8 + q] : 0u;
+            if (((imp >> q) & 1u) && dv[q] > maxd) maxd = dv[q];
+        }
+        uint32_t wmax = maxd;
+        for (int o = 32; o > 0; o >>= 1) {
+            uint32_t y = __shfl_xor(wmax, o);
+            wmax = y > wmax ? y : wmax;
+        }
+        uint32_t prune = 0;
+        const uint32_t len = S.len[t];
+        const u64 base = S.base[t];
+        for (uint32_t j0 = 0; j0 < len; j0 += 64) {
+            const uint32_t j = j0 + lane;
+            uint32_t h = 0, d = kWInf, slot = kEmpty;
+            if (j < len) {
+                const u64 lab = S.pool[base + j];
+                h = (uint32_t)(lab >> 32);
+                d = (uint32_t)lab;
+                slot = hslot[h];
+            }
+            const bool pass = slot < kEmpty - 1 && (uint64_t)d + hmin[h] <= wmax;
+            u64 m = __ballot(pass);
+            while (m) {
+                const int src = __ffsll(m) - 1;
+                m &= m - 1;
+                const uint32_t dd = __shfl(d, src);
+                const uint32_t sl = __shfl(slot, src);
+                const uint4 *row = reinterpret_cast<const uint4 *>(dcw + (uint64_t)sl * kWB + lane * 8);
+                const uint4 r0 = row[0], r1 = row[1];
+                const uint32_t x[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if (((imp >> q) & 1u) && x[q] != kWInf && (uint64_t)dd + x[q] <= dv[q]) prune |= 1u << q;
+            }
+        }
+        const uint32_t live = imp & ~prune;
+        n_pruned += __popc(imp & prune);
+        // pruned-at-current-value bits: set for pruned, cleared for live
+        if (imp) Pb[lane] = (uint8_t)((Pb[lane] & ~imp) | (imp & prune));
+        const u64 any = __ballot(live != 0);
+        if (!any) continue;
+        uint32_t k = 0;
+        if (lane == 0) k = atomicAdd(&C->n_act, 1u);
+        k = __shfl(k, 0);
+        reinterpret_cast<uint8_t *>(ag + (uint64_t)k * kWW)[lane] = (uint8_t)live;
+        uint32_t wmask = 0;
+#pragma unroll
+        for (int w = 0; w < (int)kWW; ++w)
+            if ((any >> (8 * w)) & 0xFFull) wmask |= 1u << w;
+        const uint32_t pdeg = prefix_above(off, adj, t, rs);
+        if (lane == 0) {
+            alist[k] = t;
+            amask[k] = wmask;
+            aend[k] = pdeg;
+        }
+        emit_chunks(k, pdeg, C, chunks);
+    }
+    n_pruned = wave_sum(n_pruned);
+    if (lane == 0 && n_pruned) atomicAdd(&C->pruned, n_pruned);
+}
+
+// filter_out_labels (WPARA:1100-1228): one wave per touched vertex; surviving
+// pairs (finite, not pruned) are dropped if an earlier batch root h witnesses
+// S(v,h) + S(r,h) <= S(v,r).  Reads T/P only, so every wave sees the same
+// pre-filter state; the kept bitmap goes to I and the pool words the commit
+// may reserve are summed into C->need.
+__global__ __launch_bounds__(256) void k_w_filter(uint32_t rs, const uint32_t *tlist, const uint32_t *T,
+                                                  const u64 *P, u64 *I, WStore S, WCounters *C)
+{
+    __shared__ uint8_t sv[4][64];
+    const uint32_t lane = lane_id();
+    const uint32_t wib = threadIdx.x >> 6;
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t nt = C->n_touched;
+    u64 n_lab = 0, n_filt = 0, n_need = 0;
+    for (uint32_t i = wave; i < nt; i += nwaves) {
+        const uint32_t v = tlist[i];
+        const uint8_t pb = reinterpret_cast<const uint8_t *>(P + (uint64_t)v * kWW)[lane];
+        uint32_t dv[8];
+        uint32_t surv = 0;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            dv[q] = T[(uint64_t)v * kWB + lane * 8 + q];
+            if (dv[q] != kWInf && !((pb >> q) & 1u)) surv |= 1u << q;
+        }
+        sv[wib][lane] = (uint8_t)surv;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        uint32_t drop = 0;
+        for (uint32_t w = 0; w < kWW; ++w) {
+            u64 word = 0;
+#pragma unroll
+            for (int bb = 0; bb < 8; ++bb) word |= (u64)sv[wib][w * 8 + bb] << (8 * bb);
+            while (word) {
+                const uint32_t h = w * 64 + __ffsll(word) - 1;  // witness root offset
+                word &= word - 1;
+                if (h >= lane * 8 + 7) break;  // only roots r > h of this lane can use h
+                const uint32_t dvh = T[(uint64_t)v * kWB + h];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const uint32_t ro = lane * 8 + q;
+                    if (!((surv >> q) & 1u) || ro <= h) continue;
+                    const uint32_t r = rs + ro;
+                    const uint32_t drh = T[(uint64_t)r * kWB + h];
+                    const uint8_t prh = reinterpret_cast<const uint8_t *>(P + (uint64_t)r * kWW)[h >> 3];
+                    if (drh == kWInf || ((prh >> (h & 7)) & 1u)) continue;
+                    if ((uint64_t)dvh + drh <= dv[q]) drop |= 1u << q;
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        const uint32_t keep = surv & ~drop;
+        n_filt += __popc(drop);
+        // The improved bitmap is all zero once the rounds end; it carries the
+        // kept pairs to the commit kernel.
+        reinterpret_cast<uint8_t *>(I + (uint64_t)v * kWW)[lane] = (uint8_t)keep;
+        const uint32_t total = (uint32_t)wave_sum((u64)__popc(keep));
+        if (!total) continue;
+        n_lab += total;
+        if (lane == 0) {
+            // Pool words a doubling reservation may take (store_reserve).
+            const uint32_t len = S.len[v], cap = S.cap[v];
+            if (len + total > cap) {
+                uint32_t ncap = cap * 2;
+                if (ncap < len + total) ncap = len + total;
+                if (ncap < 8) ncap = 8;
+                n_need += ncap;
+            }
+        }
+    }
+    n_filt = wave_sum(n_filt);
+    if (lane == 0) {
+        if (n_lab) atomicAdd(&C->labels, n_lab);
+        if (n_filt) atomicAdd(&C->filtered, n_filt);
+        if (n_need) atomicAdd(&C->need, n_need);
+    }
+}
+
+// update_index (WPARA:961-990): append the kept pairs of each touched vertex,
+// hubs ascending (batches are committed in root order and each lane's roots
+// follow the lower lanes'), then clear the carried bitmap.
+__global__ __launch_bounds__(256) void k_w_commit(uint32_t rs, const uint32_t *tlist, const uint32_t *T, u64 *I,
+                                                  WStore S, WCounters *C)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t nt = C->n_touched;
+    for (uint32_t i = wave; i < nt; i += nwaves) {
+        const uint32_t v = tlist[i];
+        uint8_t *Kb = reinterpret_cast<uint8_t *>(I + (uint64_t)v * kWW);
+        const uint32_t keep = Kb[lane];
+        const uint32_t cnt = __popc(keep);
+        uint32_t incl = cnt;
+        for (int o = 1; o < 64; o <<= 1) {
+            uint32_t y = __shfl_up(incl, o);
+            if (lane >= (uint32_t)o) incl += y;
+        }
+        const uint32_t total = __shfl(incl, 63);
+        if (!total) continue;
+        if (keep) Kb[lane] = 0;
+        const uint32_t len = S.len[v];
+        const u64 nb = store_reserve(S, C, v, len, total);
+        if (nb == ~0ull) continue;
+        uint32_t pos = len + incl - cnt;
+        for (uint32_t kb = keep; kb; kb &= kb - 1) {
+            const uint32_t q = __ffs(kb) - 1;
+            S.pool[nb + pos++] = ((u64)(rs + lane * 8 + q) << 32) | T[(uint64_t)v * kWB + lane * 8 + q];
+        }
+        if (lane == 0) S.len[v] = len + total;
+    }
+}
+
+// reset_tables (WPARA:918-956) for touched rows and the hub table.
+__global__ __launch_bounds__(256) void k_w_batch_end(const uint32_t *tlist, uint32_t *T, u64 *P, uint32_t *hslot,
+                                                     const uint32_t *hlist, const WCounters *C)
+{
+    const uint32_t nt = C->n_touched, nh = C->n_hub;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t tot = (uint64_t)nt * (kWB / 4);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += stride) {
+        const uint32_t v = tlist[i / (kWB / 4)], part = (uint32_t)(i % (kWB / 4));
+        reinterpret_cast<uint4 *>(T + (uint64_t)v * kWB)[part] = make_uint4(kWInf, kWInf, kWInf, kWInf);
+        if (part < kWW) P[(uint64_t)v * kWW + part] = 0ull;
+    }
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh; i += stride) hslot[hlist[i]] = kEmpty;
+}
+
+__global__ void k_w_fill(uint32_t *p, uint64_t count, uint32_t val)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = val;
+}
+
+__global__ void k_w_zero64(u64 *p, uint64_t count)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = 0ull;
+}
+
+// Copies list v (len[v] words at pool[base[v]]) to dst[dst_off[v]].
+__global__ __launch_bounds__(256) void k_w_move(uint32_t n, const u64 *pool, const u64 *base, const uint32_t *len,
+                                                const uint64_t *dst_off, u64 *dst)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
+    for (uint32_t v = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; v < n; v += nw) {
+        const uint64_t o = dst_off[v], b = base[v];
+        const uint32_t L = len[v];
+        for (uint32_t j = lane; j < L; j += 64) dst[o + j] = pool[b + j];
+    }
+}
+
+__global__ void k_w_split(uint64_t total, const u64 *lab, uint32_t *hub, uint32_t *dist)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+        hub[i] = (uint32_t)(lab[i] >> 32);
+        dist[i] = (uint32_t)lab[i];
+    }
+}
+
+}  // namespace
+
+pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t batch_req)
+{
+    hipStream_t st = g->ctx->stream;
+    const uint32_t n = g->n;
+    const uint32_t batch = (batch_req == 0 || batch_req > kWB) ? kWB : batch_req;
+    DevBuf<uint32_t> T, tmark, tlist, imark, ilist, alist, amask, aend, hslot, hmin, hlist, dcw, len, cap;
+    DevBuf<u64> P, I, ag, chunks, pool, base;
+    DevBuf<WCounters> ctr;
+    PADO_TRY(T.alloc((uint64_t)n * kWB));
+    PADO_TRY(P.alloc((uint64_t)n * kWW));
+    PADO_TRY(I.alloc((uint64_t)n * kWW));
+    PADO_TRY(ag.alloc((uint64_t)n * kWW));
+    PADO_TRY(chunks.alloc(g->m / kChunk + (uint64_t)n + 64));
+    PADO_TRY(tmark.alloc(n));
+    PADO_TRY(tlist.alloc(n));
+    PADO_TRY(imark.alloc(n));
+    PADO_TRY(ilist.alloc(n));
+    PADO_TRY(alist.alloc(n));
+    PADO_TRY(amask.alloc(n));
+    PADO_TRY(aend.alloc(n));
+    PADO_TRY(hslot.alloc(n));
+    PADO_TRY(hmin.alloc(n));
+    PADO_TRY(hlist.alloc(n));
+    PADO_TRY(len.alloc(n));
+    PADO_TRY(cap.alloc(n));
+    PADO_TRY(base.alloc(n));
+    PADO_TRY(ctr.alloc(1));
+    u64 pool_size = std::max<u64>(1ull << 22, 32ull * n);
+    PADO_TRY(pool.alloc(pool_size));
+    uint32_t dc_slots = std::min<uint32_t>(n, 1u << 14);
+    PADO_TRY(dcw.alloc((uint64_t)dc_slots * kWB));
+    const uint32_t fg = 4096;
+    hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, T.p, (uint64_t)n * kWB, kWInf);
+    hipLaunchKernelGGL(k_w_zero64, dim3(fg), dim3(256), 0, st, P.p, (uint64_t)n * kWW);
+    hipLaunchKernelGGL(k_w_zero64, dim3(fg), dim3(256), 0, st, I.p, (uint64_t)n * kWW);
+    hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, tmark.p, (uint64_t)n, 0u);
+    hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, imark.p, (uint64_t)n, 0u);
+    hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, hslot.p, (uint64_t)n, kEmpty);
+    hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, len.p, (uint64_t)n, 0u);
+    hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, cap.p, (uint64_t)n, 0u);
+    hipLaunchKernelGGL(k_w_zero64, dim3(fg), dim3(256), 0, st, base.p, (uint64_t)n);
+    PADO_HIP(hipMemsetAsync(ctr.p, 0, sizeof(WCounters), st));
+    WCounters *h = nullptr;
+    PADO_HIP(hipHostMalloc((void **)&h, sizeof(WCounters), hipHostMallocDefault));
+    struct PinnedFree {
+        WCounters *p;
+        ~PinnedFree() { (void)hipHostFree(p); }
+    } pin{h};
+
+    WStore S{pool.p, base.p, len.p, cap.p, pool_size};
+    const uint32_t nb = (n + batch - 1) / batch;
+    uint32_t gen_round = 0;
+    uint64_t rounds = 0;
+    for (uint32_t b = 0; b < nb; ++b) {
+        const uint32_t rs = b * batch, rsz = std::min(batch, n - rs), gen_batch = b + 1;
+        const uint32_t rb = (rsz * 64 + 255) / 256;
+        hipLaunchKernelGGL(k_w_reset_counters, dim3(1), dim3(1), 0, st, ctr.p);
+        hipLaunchKernelGGL(k_w_init, dim3(rb), dim3(256), 0, st, rs, rsz, gen_batch, g->off.p, g->adj.p, T.p, tmark.p,
+                           tlist.p, alist.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
+        hipLaunchKernelGGL(k_w_hub_mark, dim3(rb), dim3(256), 0, st, rs, rsz, S, hslot.p, hlist.p, ctr.p);
+        PADO_HIP(hipMemcpyAsync(h, ctr.p, sizeof(WCounters), hipMemcpyDeviceToHost, st));
+        PADO_HIP(hipStreamSynchronize(st));
+        if (h->n_hub > dc_slots) {
+            dc_slots = std::min<uint32_t>(n, std::max<uint32_t>(h->n_hub, dc_slots * 2));
+            PADO_TRY(dcw.alloc((uint64_t)dc_slots * kWB));
+        }
+        hipLaunchKernelGGL(k_w_hub_slots, dim3(grid_for((uint64_t)h->n_hub * 128, 256, 8192)), dim3(256), 0, st,
+                           hlist.p, hslot.p, hmin.p, dcw.p, ctr.p);
+        hipLaunchKernelGGL(k_w_hub_fill, dim3(rb), dim3(256), 0, st, rs, rsz, S, hslot.p, hmin.p, dcw.p);
+        uint32_t n_act = h->n_act;
+        while (n_act) {
+            ++gen_round;
+            ++rounds;
+            hipLaunchKernelGGL(k_w_push, dim3(4096), dim3(256), 0, st, rs, gen_batch, gen_round, g->off.p, g->adj.p,
+                               g->w.p, alist.p, ag.p, amask.p, aend.p, chunks.p, T.p, I.p, tmark.p, tlist.p, imark.p,
+                               ilist.p, ctr.p);
+            hipLaunchKernelGGL(k_w_mid, dim3(1), dim3(1), 0, st, ctr.p);
+            hipLaunchKernelGGL(k_w_check, dim3(2048), dim3(256), 0, st, rs, ilist.p, g->off.p, g->adj.p, T.p, I.p,
+                               P.p, S, hslot.p, hmin.p, dcw.p, alist.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
+            hipLaunchKernelGGL(k_w_post, dim3(1), dim3(1), 0, st, ctr.p);
+            PADO_HIP(hipMemcpyAsync(h, ctr.p, sizeof(WCounters), hipMemcpyDeviceToHost, st));
+            PADO_HIP(hipStreamSynchronize(st));
+            n_act = h->n_act;
+        }
+        const u64 labels_before = h->labels;
+        hipLaunchKernelGGL(k_w_filter, dim3(2048), dim3(256), 0, st, rs, tlist.p, T.p, P.p, I.p, S, ctr.p);
+        PADO_HIP(hipMemcpyAsync(h, ctr.p, sizeof(WCounters), hipMemcpyDeviceToHost, st));
+        PADO_HIP(hipStreamSynchronize(st));
+        if (h->pool_top + h->need > pool_size) {
+            // Compact every list into a larger pool.  After compaction a list of
+            // length L has capacity L + L/4 + 4, so the commit can reserve at
+            // most 2.5 * live + new + 8 * touched words.
+            const u64 added = h->labels - labels_before;
+            std::vector<uint32_t> hl(n);
+            PADO_HIP(hipMemcpy(hl.data(), len.p, 4ull * n, hipMemcpyDeviceToHost));
+            std::vector<uint64_t> no(n + 1, 0);
+            std::vector<uint32_t> nc(n);
+            for (uint32_t v = 0; v < n; ++v) {
+                nc[v] = hl[v] + (hl[v] >> 2) + 4;
+                no[v + 1] = no[v] + nc[v];
+            }
+            const u64 nsize = std::max<u64>(pool_size * 2, 4 * no[n] + 2 * added + 8ull * h->n_touched + 1024);
+            DevBuf<u64> np;
+            DevBuf<uint64_t> dno;
+            PADO_TRY(np.alloc(nsize));
+            PADO_TRY(dno.alloc((uint64_t)n + 1));
+            PADO_HIP(hipMemcpy(dno.p, no.data(), 8ull * (n + 1), hipMemcpyHostToDevice));
+            hipLaunchKernelGGL(k_w_move, dim3(grid_for((uint64_t)n * 64, 256, 8192)), dim3(256), 0, st, n, pool.p,
+                               base.p, len.p, dno.p, np.p);
+            PADO_HIP(hipStreamSynchronize(st));
+            std::vector<u64> nb64(no.begin(), no.end() - 1);
+            PADO_HIP(hipMemcpy(base.p, nb64.data(), 8ull * n, hipMemcpyHostToDevice));
+            PADO_HIP(hipMemcpy(cap.p, nc.data(), 4ull * n, hipMemcpyHostToDevice));
+            std::swap(pool.p, np.p);
+            std::swap(pool.n, np.n);
+            pool_size = nsize;
+            S.pool = pool.p;
+            S.pool_size = pool_size;
+            const u64 top = no[n];
+            PADO_HIP(hipMemcpy(&ctr.p->pool_top, &top, 8, hipMemcpyHostToDevice));
+        }
+        hipLaunchKernelGGL(k_w_commit, dim3(2048), dim3(256), 0, st, rs, tlist.p, T.p, I.p, S, ctr.p);
+        hipLaunchKernelGGL(k_w_batch_end, dim3(2048), dim3(256), 0, st, tlist.p, T.p, P.p, hslot.p, hlist.p, ctr.p);
+    }
+    PADO_HIP(hipMemcpyAsync(h, ctr.p, sizeof(WCounters), hipMemcpyDeviceToHost, st));
+    PADO_HIP(hipStreamSynchronize(st));
+    if (h->overflow) {
+        set_error("weighted label pool exhausted");
+        return PADO_GPU_ERR_NOMEM;
+    }
+    idx->iterations = rounds;
+    idx->counters.labels = h->labels;
+    idx->counters.pushes = h->relax;
+    idx->counters.cands = h->improved;
+    idx->counters.dq_pruned = h->pruned;
+    idx->counters.bp_pruned = h->filtered;
+
+    // Canonical CSR.  Every list is already sorted by hub: batches commit in
+    // root order and k_w_commit writes a batch's roots in ascending order, as
+    // the reference's update_index appends them (WPARA:961-990).
+    PADO_TRY(idx->loff.alloc((uint64_t)n + 1));
+    PADO_TRY(scan_u32_to_u64(len.p, idx->loff.p, n, st));
+    uint64_t total = 0;
+    PADO_HIP(hipMemcpy(&total, idx->loff.p + n, 8, hipMemcpyDeviceToHost));
+    idx->num_labels = total;
+    DevBuf<u64> packed;
+    PADO_TRY(packed.alloc(total));
+    PADO_TRY(idx->whub.alloc(total));
+    PADO_TRY(idx->wdist.alloc(total));
+    hipLaunchKernelGGL(k_w_move, dim3(grid_for((uint64_t)n * 64, 256, 8192)), dim3(256), 0, st, n, pool.p, base.p,
+                       len.p, idx->loff.p, packed.p);
+    hipLaunchKernelGGL(k_w_split, dim3(grid_for(total, 256, 8192)), dim3(256), 0, st, total, packed.p, idx->whub.p,
+                       idx->wdist.p);
+    PADO_HIP(hipStreamSynchronize(st));
+    idx->weighted = 1;
+    return PADO_GPU_OK;
+}
+
+}  // namespace pado

@@ -74,3 +74,43 @@ def test_generators_are_deterministic():
     assert 0.9 * 480 < g.shape[0] <= 480
     w = pado.edge_weights(1000, 1, 100, seed=1)
     assert w.min() >= 1 and w.max() <= 100
+
+
+WEIGHTED_TAGS = ("weighted_edgelist_in", "7_w_edgelist_in", "1_w_edgelist_in", "triangle_in", "rectangle_in",
+                 "rmat10_ef8", "rmat12_ef8")
+
+
+def _weighted_fixture(tag):
+    import os
+    from helpers import GOLDEN
+    return dict(np.load(os.path.join(GOLDEN, f"weighted_{tag}.npz")))
+
+
+@pytest.mark.parametrize("tag", WEIGHTED_TAGS)
+def test_weighted_ingest_matches_oracle_and_reference_rank(tag):
+    """WeightedGraph construct / make_rank / id_transfer (includes/graph.h:
+    313-408) against the oracle's restatement and the reference's rank."""
+    g = _weighted_fixture(tag)
+    G = pado.WeightedGraph.from_edges(g["edges"], g["weights"])
+    n, off, adj, w = oracle.weighted_graph_csr(g["edges"], g["weights"])
+    assert G.num_v == n
+    assert np.array_equal(G.off, off) and np.array_equal(G.adj, adj) and np.array_equal(G.w, w)
+    rank = G.make_rank()
+    assert np.array_equal(rank, g["rank"])
+    r2i = G.id_transfer(rank)
+    off2, adj2, w2, r2i2 = oracle.weighted_id_transfer(n, off, adj, w, rank)
+    assert np.array_equal(G.off, off2) and np.array_equal(G.adj, adj2) and np.array_equal(G.w, w2)
+    assert np.array_equal(r2i, r2i2)
+    # (id, weight) pairs descending in every list
+    for v in range(n):
+        s, t = int(G.off[v]), int(G.off[v + 1])
+        key = G.adj[s:t].astype(np.int64) * 256 + G.w[s:t]
+        assert np.all(np.diff(key) <= 0)
+
+
+def test_weighted_text_loader(tmp_path):
+    p = tmp_path / "w.txt"
+    p.write_text("# c\n0 1 5\n1 2 300\n% x\n2 0 1\n")
+    G = pado.WeightedGraph.from_file(str(p))
+    H = pado.WeightedGraph.from_edges(np.array([[0, 1], [1, 2], [2, 0]], np.uint32), np.array([5, 44, 1]))
+    assert np.array_equal(G.off, H.off) and np.array_equal(G.adj, H.adj) and np.array_equal(G.w, H.w)
