@@ -120,12 +120,20 @@ typedef struct {
     uint32_t flags;       /* PADO_GPU_BUILD_* */
     uint32_t push_blocks; /* grid of the push kernel (256-thread blocks), 0 = 4096 */
     uint32_t add_blocks;  /* grid of the add kernel, 0 = 2048 */
-    uint32_t reserved[4];
+    uint32_t iters_per_sync; /* batch iterations enqueued per host check of the
+                                frontier (iterations past a batch's end are
+                                empty launches), 0 = 8 */
+    uint32_t reserved[3];
 } pado_gpu_build_cfg;
 
 /* Bit-parallel labeling one BFS per slot (as the reference runs it) instead of
  * the default single multi-source sweep over all 50 slots; same output. */
 #define PADO_GPU_BUILD_BP_PER_SLOT 1u
+/* Run the batch step as captured hipGraphs (batch prologue + iters_per_sync
+ * iterations; then iterations only until the frontier is empty) instead of
+ * per-kernel launches.  Same output; per-kernel event timings are not
+ * recorded in this mode. */
+#define PADO_GPU_BUILD_GRAPH 2u
 
 /* ParaVertexCentricPLL<BATCH_SIZE>(const Graph&) -> construct (PARA:309-313,
  * 1740-1895): bit-parallel labeling (PARA:544-840) then batch_process over all
@@ -226,10 +234,11 @@ pado_gpu_status pado_gpu_wquery(pado_gpu_ctx *ctx, pado_gpu_index *idx, const ui
  * Autotuner (no reference counterpart).  Every knob of pado_gpu_build_cfg
  * leaves the labels unchanged, so the tuner may pick any of them purely on
  * speed: it times full trial builds of `graph` in a coordinate search over
- * batch size {256, 512, 1024}, push grid {1024..8192}, add grid {1024..4096}
- * and the bit-parallel variant, starting from `start` (NULL = defaults), and
- * returns the fastest configuration.  trials_out (nullable) receives the
- * number of builds run; best_ms their best time.
+ * batch size {256, 512, 1024}, push grid {1024..8192}, add grid {1024..4096},
+ * iterations per host check {2..16} and the flags (hipGraph batch step,
+ * bit-parallel variant), starting from `start` (NULL = defaults), and returns
+ * the fastest configuration.  trials_out (nullable) receives the number of
+ * builds run (default cap 20); best_ms their best time.
  * ------------------------------------------------------------------------ */
 pado_gpu_status pado_gpu_autotune(pado_gpu_ctx *ctx, const pado_gpu_graph *graph, const pado_gpu_build_cfg *start,
                                   uint32_t max_trials, pado_gpu_build_cfg *best, double *best_ms,

@@ -29,6 +29,7 @@ QUERY_DEVICE_PTRS = 1
 QUERY_REF_COMPAT = 2
 QUERY_RANK_SPACE = 4
 BUILD_BP_PER_SLOT = 1
+BUILD_GRAPH = 2
 WINF = 0xFFFFFFFF  # weighted query: no common hub (unreachable)
 
 _STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_NOMEM", 4: "ERR_DOMAIN", 5: "ERR_STATE",
@@ -43,11 +44,12 @@ class PadoError(RuntimeError):
 
 class BuildCfg(ctypes.Structure):
     _fields_ = [("batch_size", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("push_blocks", ctypes.c_uint32),
-                ("add_blocks", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4)]
+                ("add_blocks", ctypes.c_uint32), ("iters_per_sync", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 3)]
 
     def as_dict(self):
         return {"batch_size": self.batch_size, "flags": self.flags, "push_blocks": self.push_blocks,
-                "add_blocks": self.add_blocks}
+                "add_blocks": self.add_blocks, "iters_per_sync": self.iters_per_sync}
 
 
 class ServeCfg(ctypes.Structure):

@@ -39,7 +39,8 @@ extern "C" pado_gpu_status pado_gpu_autotune(pado_gpu_ctx *ctx, const pado_gpu_g
     if (!cur.batch_size) cur.batch_size = pado::kBatch;
     if (!cur.push_blocks) cur.push_blocks = 4096;
     if (!cur.add_blocks) cur.add_blocks = 2048;
-    if (!max_trials) max_trials = 16;
+    if (!cur.iters_per_sync) cur.iters_per_sync = 8;
+    if (!max_trials) max_trials = 20;
     uint32_t trials = 0;
     pado_gpu_status st = PADO_GPU_OK;
     double cur_ms = time_build(ctx, graph, cur, &st);
@@ -49,7 +50,9 @@ extern "C" pado_gpu_status pado_gpu_autotune(pado_gpu_ctx *ctx, const pado_gpu_g
     const std::vector<uint32_t> batches = {256, 512, 1024};
     const std::vector<uint32_t> pushes = {1024, 2048, 4096, 8192};
     const std::vector<uint32_t> adds = {1024, 2048, 4096};
-    const std::vector<uint32_t> flags = {0u, PADO_GPU_BUILD_BP_PER_SLOT};
+    const std::vector<uint32_t> iters = {2, 4, 8, 16};
+    const std::vector<uint32_t> flags = {0u, PADO_GPU_BUILD_GRAPH, PADO_GPU_BUILD_BP_PER_SLOT,
+                                         PADO_GPU_BUILD_GRAPH | PADO_GPU_BUILD_BP_PER_SLOT};
     auto sweep = [&](uint32_t pado_gpu_build_cfg::*field, const std::vector<uint32_t> &vals) -> pado_gpu_status {
         for (uint32_t v : vals) {
             if (trials >= max_trials) return PADO_GPU_OK;
@@ -70,6 +73,7 @@ extern "C" pado_gpu_status pado_gpu_autotune(pado_gpu_ctx *ctx, const pado_gpu_g
     PADO_TRY(sweep(&pado_gpu_build_cfg::push_blocks, pushes));
     PADO_TRY(sweep(&pado_gpu_build_cfg::add_blocks, adds));
     PADO_TRY(sweep(&pado_gpu_build_cfg::batch_size, batches));
+    PADO_TRY(sweep(&pado_gpu_build_cfg::iters_per_sync, iters));
     PADO_TRY(sweep(&pado_gpu_build_cfg::flags, flags));
     *best = cur;
     if (best_ms) *best_ms = cur_ms;

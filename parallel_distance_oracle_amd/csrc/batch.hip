@@ -34,6 +34,8 @@
 // Kernels per iteration: k_push (edge-parallel, quarter-wave per 16 edges) and
 // k_add (wave per candidate vertex: distance query + label append + next
 // active entry).  The host loop syncs once per iteration on the active count.
+#include <functional>
+
 #include "internal.hpp"
 
 namespace pado {
@@ -176,8 +178,7 @@ __global__ void k_zero_u64(unsigned long long *p, uint64_t count)
 
 // initialize (PARA:848-1080): non-BP roots become active with their own bit as
 // last group, get the self label (r, 0), and set their own indicator bit.
-__global__ __launch_bounds__(256) void k_batch_init(uint32_t rs, uint32_t rsz, uint32_t gen_batch,
-                                                    const uint8_t *used, const uint64_t *off, const uint32_t *adj,
+__global__ __launch_bounds__(256) void k_batch_init(const uint8_t *used, const uint64_t *off, const uint32_t *adj,
                                                     LabelStore S, unsigned long long *ind, uint32_t *tmark,
                                                     uint32_t *tlist, uint32_t *alist, unsigned long long *ag,
                                                     uint32_t *amask, uint32_t *aend, unsigned long long *chunks,
@@ -185,6 +186,7 @@ __global__ __launch_bounds__(256) void k_batch_init(uint32_t rs, uint32_t rsz, u
 {
     const uint32_t lane = lane_id();
     const uint32_t ro = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t rs = C->rs, rsz = C->rsz, gen_batch = C->gen_batch;
     if (ro >= rsz) return;
     const uint32_t r = rs + ro;
     if (used[r]) return;
@@ -212,11 +214,12 @@ __global__ __launch_bounds__(256) void k_batch_init(uint32_t rs, uint32_t rsz, u
 }
 
 // Distinct hubs over the labels of the batch roots (for the root-distance table).
-__global__ __launch_bounds__(256) void k_hub_mark(uint32_t rs, uint32_t rsz, const uint8_t *used, LabelStore S,
-                                                  uint32_t *hinfo, uint32_t *hlist, BuildCounters *C)
+__global__ __launch_bounds__(256) void k_hub_mark(const uint8_t *used, LabelStore S, uint32_t *hinfo,
+                                                  uint32_t *hlist, BuildCounters *C)
 {
     const uint32_t lane = lane_id();
     const uint32_t ro = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t rs = C->rs, rsz = C->rsz;
     if (ro >= rsz) return;
     const uint32_t r = rs + ro;
     if (used[r]) return;
@@ -244,11 +247,12 @@ __global__ __launch_bounds__(256) void k_hub_slots(const uint32_t *hlist, uint32
 }
 
 // dist_matrix scatter (PARA:986-1013), transposed: dc[slot(h)][r] = d.
-__global__ __launch_bounds__(256) void k_hub_fill(uint32_t rs, uint32_t rsz, const uint8_t *used, LabelStore S,
-                                                  uint32_t *hinfo, uint8_t *dc)
+__global__ __launch_bounds__(256) void k_hub_fill(const uint8_t *used, LabelStore S, uint32_t *hinfo, uint8_t *dc,
+                                                  const BuildCounters *C)
 {
     const uint32_t lane = lane_id();
     const uint32_t ro = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t rs = C->rs, rsz = C->rsz;
     if (ro >= rsz) return;
     const uint32_t r = rs + ro;
     if (used[r]) return;
@@ -265,8 +269,7 @@ __global__ __launch_bounds__(256) void k_hub_fill(uint32_t rs, uint32_t rsz, con
 
 // push (PARA:1486-1559 / push_labels PARA:1084-1211), one quarter-wave per
 // 16-edge chunk of an active vertex's tails >= roots_start.
-__global__ __launch_bounds__(256) void k_push(uint32_t iter, uint32_t rs, uint32_t gen_batch, uint32_t gen_iter,
-                                              const uint64_t *off, const uint32_t *adj, const uint32_t *alist,
+__global__ __launch_bounds__(256) void k_push(const uint64_t *off, const uint32_t *adj, const uint32_t *alist,
                                               const unsigned long long *ag, const uint32_t *amask,
                                               const uint32_t *aend, const unsigned long long *chunks,
                                               unsigned long long *ind, unsigned long long *cand, uint32_t *tmark,
@@ -275,7 +278,8 @@ __global__ __launch_bounds__(256) void k_push(uint32_t iter, uint32_t rs, uint32
 {
     const uint32_t lane = threadIdx.x & (kChunk - 1);
     const uint32_t ngroups = (gridDim.x * blockDim.x) / kChunk;
-    const uint32_t nc = C->n_chunk;
+    const uint32_t nc = C->n_chunk;  // 0 once the batch's frontier is empty
+    const uint32_t iter = C->iter, rs = C->rs, gen_batch = C->gen_batch, gen_iter = C->gen_iter;
     unsigned long long n_push = 0, n_pass = 0, n_bp = 0, n_cand = 0, n_edge = 0, n_tail = 0;
     for (uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) / kChunk; c < nc; c += ngroups) {
         const unsigned long long desc = chunks[c];
@@ -356,8 +360,18 @@ __global__ void k_mid(BuildCounters *C)
 
 __global__ void k_post(BuildCounters *C) { C->n_cand = 0; }
 
-__global__ void k_batch_counters_reset(BuildCounters *C)
+// Start of a batch (construct's loop over batches, PARA:1791-1812): roots
+// [b * batch, ...) with b taken from the device-side batch counter.
+__global__ void k_batch_begin(BuildCounters *C, uint32_t batch, uint32_t n)
 {
+    const uint32_t b = C->next_batch;
+    const uint32_t rs = b * batch;
+    C->next_batch = b + 1;
+    C->rs = rs;
+    C->rsz = rs < n ? min(batch, n - rs) : 0u;
+    C->gen_batch = b + 1;
+    C->iter = 0;
+    C->live = 0;
     C->n_act = 0;
     C->n_chunk = 0;
     C->n_cand = 0;
@@ -365,13 +379,36 @@ __global__ void k_batch_counters_reset(BuildCounters *C)
     C->n_hub = 0;
 }
 
+// Start of an iteration of batch_process's loop (PARA:1478-1670): live while
+// the previous iteration produced active vertices.  Iteration 255 would leave
+// the reference's u8 domain: flag it and end the batch.
+__global__ void k_iter_begin(BuildCounters *C)
+{
+    if (C->n_act == 0) {
+        C->live = 0;
+        return;
+    }
+    const uint32_t it = C->iter + 1;
+    if (it >= kInf8) {
+        C->domain = 1;
+        C->live = 0;
+        C->n_act = 0;
+        C->n_chunk = 0;
+        return;
+    }
+    C->iter = it;
+    C->live = 1;
+    C->gen_iter += 1;
+    C->iterations += 1;
+    C->actives += C->n_act;
+}
+
 // add (PARA:1568-1665): one wave per candidate vertex v.  Lane l owns roots
 // [16l, 16l+16).  distance_query (PARA:1218-1270) runs over v's labels with
 // dist < iter: a hub h can prune only if d_v(h) + min_r d_r(h) <= iter, so a
 // per-hub minimum filters most hubs before the 1 KB root-distance row of the
 // survivors is read (16 B per lane).
-__global__ __launch_bounds__(256) void k_add(uint32_t iter, uint32_t rs, uint32_t rsz, const uint8_t *used,
-                                             const uint64_t *off, const uint32_t *adj, const uint32_t *clist,
+__global__ __launch_bounds__(256) void k_add(const uint8_t *used, const uint64_t *off, const uint32_t *adj, const uint32_t *clist,
                                              unsigned long long *cand, LabelStore S, uint32_t *hinfo, uint8_t *dc,
                                              uint32_t *alist, unsigned long long *ag, uint32_t *amask,
                                              uint32_t *aend, unsigned long long *chunks, BuildCounters *C)
@@ -379,7 +416,8 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, uint32_t rs, uint32_
     const uint32_t lane = lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    const uint32_t ncand = C->n_cand;
+    const uint32_t ncand = C->n_cand;  // 0 once the batch's frontier is empty
+    const uint32_t iter = C->iter, rs = C->rs, rsz = C->rsz;
     unsigned long long n_dq = 0, n_lab = 0, n_rows = 0, n_read = 0;
     for (uint32_t i = wave; i < ncand; i += nwaves) {
         const uint32_t v = clist[i];
@@ -600,8 +638,14 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     PADO_TRY(llen.alloc(n));
     PADO_TRY(lcap.alloc(n));
     PADO_TRY(ctr.alloc(1));
-    unsigned long long pool_size = std::max<unsigned long long>(1ull << 22, 16ull * n);
+    // The label pool starts at 64 words per vertex and doubles whenever a
+    // host check finds it over 40% full; the checks come every
+    // iters_per_sync iterations, so the headroom covers several iterations.
+    unsigned long long pool_size = std::max<unsigned long long>(1ull << 22, 64ull * n);
     PADO_TRY(pool.alloc(pool_size));
+    // Root-distance table rows: the hubs of batch b's roots are ids below the
+    // batch end, so rs + rsz slots always suffice; grown on the host before
+    // a batch that could exceed it (no device round trip).
     uint32_t dc_slots = std::min<uint32_t>(n, 1u << 16);
     PADO_TRY(dc.alloc((uint64_t)dc_slots * kBatch));
 
@@ -615,7 +659,8 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, llen.p, (uint64_t)n, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, lcap.p, (uint64_t)n, 0u);
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, lbase.p, (uint64_t)n);
-    PADO_HIP(hipMemsetAsync(ctr.p, 0, sizeof(BuildCounters), st));
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t *>(ctr.p),
+                       (uint64_t)(sizeof(BuildCounters) / 4), 0u);
 
     BuildCounters *h_ctr = nullptr;
     PADO_HIP(hipHostMalloc((void **)&h_ctr, sizeof(BuildCounters), hipHostMallocDefault));
@@ -624,105 +669,173 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
         ~PinnedFree() { (void)hipHostFree(p); }
     } pin{h_ctr};
 
-    // HIP events around the two per-iteration kernels, read after the
-    // per-iteration synchronisation the loop needs anyway.
-    hipEvent_t ev[4];
-    for (auto &e : ev) PADO_HIP(hipEventCreate(&e));
-    struct EventFree {
-        hipEvent_t *e;
-        ~EventFree()
-        {
-            for (int i = 0; i < 4; ++i) (void)hipEventDestroy(e[i]);
-        }
-    } evf{ev};
-    KernelTimes kt{};
-    uint64_t actives = 0;
-
-    LabelStore S{pool.p, lbase.p, llen.p, lcap.p, pool_size};
-    const uint32_t nb = (n + batch - 1) / batch;
-    uint32_t gen_iter = 0;
-    uint64_t iterations = 0;
+    const bool use_graph = (cfg.flags & PADO_GPU_BUILD_GRAPH) != 0;
+    const uint32_t U = cfg.iters_per_sync ? std::min<uint32_t>(cfg.iters_per_sync, 64u) : 8u;
     const uint32_t push_grid = cfg.push_blocks ? cfg.push_blocks : 4096;
     const uint32_t add_grid = cfg.add_blocks ? cfg.add_blocks : 2048;
-    for (uint32_t b = 0; b < nb; ++b) {
-        const uint32_t rs = b * batch;
-        const uint32_t rsz = std::min(batch, n - rs);
-        const uint32_t gen_batch = b + 1;
-        const uint32_t root_blocks = (rsz * 64 + 255) / 256;
-        hipLaunchKernelGGL(k_batch_counters_reset, dim3(1), dim3(1), 0, st, ctr.p);
-        hipLaunchKernelGGL(k_batch_init, dim3(root_blocks), dim3(256), 0, st, rs, rsz, gen_batch, d_used.p, g->off.p,
-                           g->adj.p, S, ind.p, tmark.p, tlist.p, alist.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
-        hipLaunchKernelGGL(k_hub_mark, dim3(root_blocks), dim3(256), 0, st, rs, rsz, d_used.p, S, hinfo.p, hlist.p,
-                           ctr.p);
-        PADO_HIP(hipMemcpyAsync(h_ctr, ctr.p, sizeof(BuildCounters), hipMemcpyDeviceToHost, st));
-        PADO_HIP(hipStreamSynchronize(st));
-        if (h_ctr->overflow) {
-            set_error("label pool exhausted");
-            return PADO_GPU_ERR_NOMEM;
-        }
-        if (h_ctr->n_hub > dc_slots) {
-            dc_slots = std::min<uint32_t>(n, std::max<uint32_t>(h_ctr->n_hub, dc_slots * 2));
-            PADO_TRY(dc.alloc((uint64_t)dc_slots * kBatch));
-        }
-        hipLaunchKernelGGL(k_hub_slots, dim3(grid_for((uint64_t)h_ctr->n_hub * 64, 256, 8192)), dim3(256), 0, st,
-                           hlist.p, hinfo.p, dc.p, ctr.p);
-        hipLaunchKernelGGL(k_hub_fill, dim3(root_blocks), dim3(256), 0, st, rs, rsz, d_used.p, S, hinfo.p, dc.p);
+    const uint32_t root_blocks = (batch * 64 + 255) / 256;
 
-        uint32_t n_act = h_ctr->n_act;
-        uint32_t iter = 0;
-        while (n_act != 0) {
-            ++iter;
-            if (iter >= kInf8) {
-                set_error("batch iteration reached 255: distances outside the uint8 domain of the reference");
-                return PADO_GPU_ERR_DOMAIN;
-            }
-            ++gen_iter;
-            ++iterations;
-            actives += n_act;
-            PADO_HIP(hipEventRecord(ev[0], st));
-            hipLaunchKernelGGL(k_push, dim3(push_grid), dim3(256), 0, st, iter, rs, gen_batch, gen_iter, g->off.p,
-                               g->adj.p, alist.p, ag.p, amask.p, aend.p, chunks.p, ind.p, cand.p, tmark.p, tlist.p,
-                               cmark.p, clist.p, imark.p, idx->bpd.p, (const ulonglong2 *)idx->bps.p, ctr.p);
-            PADO_HIP(hipEventRecord(ev[1], st));
+    // Stream mode brackets every push and add launch with HIP events (read
+    // after the host check that follows each block of U iterations).
+    std::vector<hipEvent_t> ev;
+    struct EventFree {
+        std::vector<hipEvent_t> *e;
+        ~EventFree()
+        {
+            for (auto x : *e) (void)hipEventDestroy(x);
+        }
+    } evf{&ev};
+    if (!use_graph) {
+        ev.resize(4 * U);
+        for (auto &e : ev) PADO_HIP(hipEventCreate(&e));
+    }
+    KernelTimes kt{};
+
+    LabelStore S{pool.p, lbase.p, llen.p, lcap.p, pool_size};
+    auto enqueue_prologue = [&]() {
+        // Previous batch's reset_at_end first (a no-op before the first batch).
+        hipLaunchKernelGGL(k_batch_end, dim3(1024), dim3(256), 0, st, ind.p, tlist.p, hinfo.p, hlist.p, ctr.p);
+        hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(1), 0, st, ctr.p, batch, n);
+        hipLaunchKernelGGL(k_batch_init, dim3(root_blocks), dim3(256), 0, st, d_used.p, g->off.p, g->adj.p, S, ind.p,
+                           tmark.p, tlist.p, alist.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
+        hipLaunchKernelGGL(k_hub_mark, dim3(root_blocks), dim3(256), 0, st, d_used.p, S, hinfo.p, hlist.p, ctr.p);
+        hipLaunchKernelGGL(k_hub_slots, dim3(2048), dim3(256), 0, st, hlist.p, hinfo.p, dc.p, ctr.p);
+        hipLaunchKernelGGL(k_hub_fill, dim3(root_blocks), dim3(256), 0, st, d_used.p, S, hinfo.p, dc.p, ctr.p);
+    };
+    auto enqueue_iterations = [&](uint32_t count, bool timed) -> pado_gpu_status {
+        for (uint32_t u = 0; u < count; ++u) {
+            hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(1), 0, st, ctr.p);
+            if (timed) PADO_HIP(hipEventRecord(ev[4 * u], st));
+            hipLaunchKernelGGL(k_push, dim3(push_grid), dim3(256), 0, st, g->off.p, g->adj.p, alist.p, ag.p, amask.p,
+                               aend.p, chunks.p, ind.p, cand.p, tmark.p, tlist.p, cmark.p, clist.p, imark.p,
+                               idx->bpd.p, (const ulonglong2 *)idx->bps.p, ctr.p);
+            if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 1], st));
             hipLaunchKernelGGL(k_mid, dim3(1), dim3(1), 0, st, ctr.p);
-            PADO_HIP(hipEventRecord(ev[2], st));
-            hipLaunchKernelGGL(k_add, dim3(add_grid), dim3(256), 0, st, iter, rs, rsz, d_used.p, g->off.p, g->adj.p,
-                               clist.p, cand.p, S, hinfo.p, dc.p, alist.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
-            PADO_HIP(hipEventRecord(ev[3], st));
+            if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 2], st));
+            hipLaunchKernelGGL(k_add, dim3(add_grid), dim3(256), 0, st, d_used.p, g->off.p, g->adj.p, clist.p, cand.p,
+                               S, hinfo.p, dc.p, alist.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
+            if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 3], st));
             hipLaunchKernelGGL(k_post, dim3(1), dim3(1), 0, st, ctr.p);
+        }
+        return PADO_GPU_OK;
+    };
+
+    // A block of k iterations (k a power of two <= U) is enqueued per host
+    // check.  k follows the previous batch's iteration count, so most batches
+    // end in one block with at most a few empty iterations.
+    // Graph mode: gx_batch[j] = prologue + 2^j iterations, gx_iter[j] = 2^j
+    // iterations, captured lazily.  Kernel arguments are fixed per capture,
+    // so all graphs are dropped when the pool or the root-distance table moves.
+    constexpr int kMaxLog = 7;  // U <= 64
+    hipGraphExec_t gx_batch[kMaxLog] = {}, gx_iter[kMaxLog] = {};
+    auto drop_graphs = [&]() {
+        for (int j = 0; j < kMaxLog; ++j) {
+            if (gx_batch[j]) (void)hipGraphExecDestroy(gx_batch[j]);
+            if (gx_iter[j]) (void)hipGraphExecDestroy(gx_iter[j]);
+            gx_batch[j] = gx_iter[j] = nullptr;
+        }
+    };
+    struct GraphFree {
+        std::function<void()> f;
+        ~GraphFree() { f(); }
+    } gxf{drop_graphs};
+    auto capture = [&](bool with_prologue, uint32_t count, hipGraphExec_t *out) -> pado_gpu_status {
+        hipGraph_t gr = nullptr;
+        PADO_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        if (with_prologue) enqueue_prologue();
+        pado_gpu_status s = enqueue_iterations(count, false);
+        hipError_t e = hipStreamEndCapture(st, &gr);
+        if (s != PADO_GPU_OK) return s;
+        if (e != hipSuccess) {
+            set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            return PADO_GPU_ERR_HIP;
+        }
+        e = hipGraphInstantiate(out, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        if (e != hipSuccess) {
+            set_error(std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            return PADO_GPU_ERR_HIP;
+        }
+        return PADO_GPU_OK;
+    };
+    auto log2_block = [&](uint32_t want) {
+        int j = 0;
+        while ((1u << j) < want && (1u << (j + 1)) <= U) ++j;
+        return j;
+    };
+
+    const uint32_t nb = (n + batch - 1) / batch;
+    uint64_t iterations_seen = 0;
+    uint32_t prev_batch_iters = 1;
+    for (uint32_t b = 0; b < nb; ++b) {
+        const uint32_t batch_end = std::min<uint64_t>((uint64_t)(b + 1) * batch, n);
+        if (batch_end > dc_slots) {
+            // Ordered after all earlier work: hipFree/hipMalloc synchronise.
+            PADO_HIP(hipStreamSynchronize(st));
+            dc_slots = std::min<uint32_t>(n, std::max<uint32_t>(batch_end, dc_slots * 2));
+            PADO_TRY(dc.alloc((uint64_t)dc_slots * kBatch));
+            drop_graphs();
+        }
+        const uint64_t batch_first_iter = iterations_seen;
+        bool first = true;
+        int j = log2_block(prev_batch_iters + 1);
+        for (;;) {
+            const uint32_t count = 1u << j;
+            if (use_graph) {
+                hipGraphExec_t *slot = first ? &gx_batch[j] : &gx_iter[j];
+                if (!*slot) PADO_TRY(capture(first, count, slot));
+                PADO_HIP(hipGraphLaunch(*slot, st));
+            } else {
+                if (first) enqueue_prologue();
+                PADO_TRY(enqueue_iterations(count, true));
+            }
+            first = false;
             PADO_HIP(hipMemcpyAsync(h_ctr, ctr.p, sizeof(BuildCounters), hipMemcpyDeviceToHost, st));
             PADO_HIP(hipStreamSynchronize(st));
             PADO_HIP(hipGetLastError());
-            float t_push = 0.f, t_add = 0.f;
-            PADO_HIP(hipEventElapsedTime(&t_push, ev[0], ev[1]));
-            PADO_HIP(hipEventElapsedTime(&t_add, ev[2], ev[3]));
-            kt.ms_push += t_push;
-            kt.ms_add += t_add;
-            kt.n_push += 1;
-            kt.n_add += 1;
+            if (!use_graph) {
+                const uint64_t live = h_ctr->iterations - iterations_seen;
+                for (uint64_t u = 0; u < live && u < count; ++u) {
+                    float t_push = 0.f, t_add = 0.f;
+                    PADO_HIP(hipEventElapsedTime(&t_push, ev[4 * u], ev[4 * u + 1]));
+                    PADO_HIP(hipEventElapsedTime(&t_add, ev[4 * u + 2], ev[4 * u + 3]));
+                    kt.ms_push += t_push;
+                    kt.ms_add += t_add;
+                    kt.n_push += 1;
+                    kt.n_add += 1;
+                }
+            }
+            iterations_seen = h_ctr->iterations;
             if (h_ctr->overflow) {
                 set_error("label pool exhausted");
                 return PADO_GPU_ERR_NOMEM;
             }
-            n_act = h_ctr->n_act;
+            if (h_ctr->domain) {
+                set_error("batch iteration reached 255: distances outside the uint8 domain of the reference");
+                return PADO_GPU_ERR_DOMAIN;
+            }
+            // Keep at least 60% of the pool free for the next block of
+            // iterations (the kernels are idle here).
+            if (h_ctr->pool_top * 5 > pool_size * 2) {
+                unsigned long long top = 0;
+                PADO_TRY(pool_repack(st, n, pool, lbase, llen, lcap, pool_size * 2, &top));
+                pool_size = pool.n;
+                S.pool = pool.p;
+                S.pool_size = pool_size;
+                PADO_HIP(hipMemcpyAsync(&ctr.p->pool_top, &top, sizeof(top), hipMemcpyHostToDevice, st));
+                PADO_HIP(hipStreamSynchronize(st));
+                drop_graphs();
+            }
+            if (h_ctr->n_act == 0) break;
+            if ((1u << (j + 1)) <= U) ++j;  // the batch runs long: double the block
         }
-        hipLaunchKernelGGL(k_batch_end, dim3(1024), dim3(256), 0, st, ind.p, tlist.p, hinfo.p, hlist.p, ctr.p);
-        // Keep at least half the pool free for the next batch.
-        if (h_ctr->pool_top * 2 > pool_size) {
-            unsigned long long top = 0;
-            PADO_TRY(pool_repack(st, n, pool, lbase, llen, lcap, pool_size * 2, &top));
-            pool_size = pool.n;
-            S.pool = pool.p;
-            S.pool_size = pool_size;
-            PADO_HIP(hipMemcpyAsync(&ctr.p->pool_top, &top, sizeof(top), hipMemcpyHostToDevice, st));
-            PADO_HIP(hipStreamSynchronize(st));
-        }
+        prev_batch_iters = (uint32_t)(iterations_seen - batch_first_iter);
     }
+    hipLaunchKernelGGL(k_batch_end, dim3(1024), dim3(256), 0, st, ind.p, tlist.p, hinfo.p, hlist.p, ctr.p);
     PADO_HIP(hipMemcpyAsync(h_ctr, ctr.p, sizeof(BuildCounters), hipMemcpyDeviceToHost, st));
     PADO_HIP(hipStreamSynchronize(st));
     idx->counters = *h_ctr;
-    idx->counters.actives = actives;
-    idx->iterations = iterations;
+    idx->iterations = h_ctr->iterations;
     idx->ktimes.ms_push = kt.ms_push;
     idx->ktimes.ms_add = kt.ms_add;
     idx->ktimes.n_push = kt.n_push;

@@ -93,6 +93,15 @@ struct BuildCounters {
     unsigned long long tails;     // distinct tails with a fresh indicator bit, summed over iterations
     unsigned long long dq_labels; // labels with dist < iter read by distance queries
     unsigned long long actives;   // active vertices, summed over iterations
+    // Device-side schedule: the batch step reads its parameters from here, so
+    // one launch sequence (or one captured hipGraph) serves every batch.
+    uint32_t next_batch;          // batch index the next k_batch_begin starts
+    uint32_t rs, rsz, gen_batch;  // current batch: roots [rs, rs + rsz)
+    uint32_t iter;                // current iteration of the batch (1-based)
+    uint32_t live;                // 1 while the current iteration has work
+    uint32_t gen_iter;            // global iteration generation (marks)
+    uint32_t domain;              // an iteration reached 255 (u8 domain)
+    unsigned long long iterations;  // live iterations, summed over batches
 };
 
 // Host-side timing of the labeling kernels (HIP events on the build stream).

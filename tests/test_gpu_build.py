@@ -82,6 +82,37 @@ def test_bp_per_slot_and_multi_source_agree(gpu_ctx, case):
     dg.close()
 
 
+@pytest.mark.parametrize("graph_mode,iters", [(False, 1), (False, 3), (False, 64), (True, 1), (True, 8),
+                                              (True, 16)])
+@pytest.mark.parametrize("scale", [12, 14])
+def test_schedule_variants_same_labels(gpu_ctx, scale, graph_mode, iters):
+    """Captured hipGraph batch steps and any number of iterations per host
+    check (empty iterations past a batch's end included) leave labels and
+    counters unchanged."""
+    meta, arr = load_rmat(scale, 8)
+    G, _, _ = rank_space(rmat_edges(scale, 8))
+    cfg = pado.BuildCfg()
+    cfg.batch_size = 1024
+    cfg.flags = pado.BUILD_GRAPH if graph_mode else 0
+    cfg.iters_per_sync = iters
+    print(f"[gpu] schedule rmat{scale} graph={graph_mode} iters={iters}", flush=True)
+    dg = pado.DeviceGraph(gpu_ctx, G)
+    idx = pado.Index.build(gpu_ctx, dg, cfg=cfg)
+    bp_d, bp_s, loff, hub, dist = idx.export()
+    assert bp_digest(bp_d, bp_s) == meta["bp_sha256"]
+    assert np.array_equal(loff, arr["loff"]) and np.array_equal(hub, arr["hub"]) and np.array_equal(dist, arr["dist"])
+    st = idx.stats()
+    assert st["bp_pruned"] == meta["bp_hit_count"]
+    X = oracle.Index(G.num_v, G.off, G.adj)
+    assert st["iterations"] == X.stats["iters"]
+    for k in ("pushes", "ind_pass", "cands", "dq_pruned"):
+        assert st[k] == X.stats[k], k
+    if not graph_mode:
+        assert st["n_push"] == st["iterations"] and st["ms_push"] > 0
+    idx.close()
+    dg.close()
+
+
 @pytest.mark.parametrize("batch", [1, 64, 500])
 def test_other_batch_sizes_same_labels(gpu_ctx, batch):
     meta, arr = load_rmat(12, 8)
