@@ -280,17 +280,23 @@ void pado_gpu_multi_free(pado_gpu_multi *mi);
  * (PADO_GPU_ERR_BUSY otherwise); a dispatcher thread coalesces admitted
  * requests into query launches on several streams, so requests that arrive
  * while others run are admitted as slots and memory free up and their work
- * overlaps on the device.  Each answer is computed per pair, so it is the
- * same alone and in any mix.
+ * overlaps on the device.  Each launch runs a persistent label-merge kernel (a
+ * grid of resident workgroups whose waves take pairs from a work counter
+ * until the launch is drained).  Each answer is computed per pair, so it is
+ * the same alone and in any mix.  Unweighted indexes answer u8 hops,
+ * weighted ones u32 distances (as pado_gpu_query / pado_gpu_wquery).
  * ------------------------------------------------------------------------ */
 typedef struct pado_gpu_server pado_gpu_server;
+
+/* Serve with the plain grid-stride query kernel instead of the persistent one. */
+#define PADO_GPU_SERVE_GRID_STRIDE 0x10u
 
 typedef struct {
     uint64_t staging_pairs;    /* pair capacity of the admission ring; 0 = derived from free memory */
     uint64_t max_launch_pairs; /* coalescing cap per launch (0 = 4M) */
     uint32_t max_requests;     /* request slots (0 = 4096) */
     uint32_t streams;          /* launches in flight (0 = 3) */
-    uint32_t flags;            /* PADO_GPU_QUERY_REF_COMPAT or 0 */
+    uint32_t flags;            /* PADO_GPU_QUERY_REF_COMPAT | PADO_GPU_SERVE_GRID_STRIDE */
     uint32_t reserved[5];
 } pado_gpu_serve_cfg;
 
@@ -313,9 +319,10 @@ pado_gpu_status pado_gpu_serve_submit(pado_gpu_server *srv, const uint32_t *a, c
                                       uint64_t *req_id);
 /* *done = 1 once the request's answers are ready. */
 pado_gpu_status pado_gpu_serve_poll(pado_gpu_server *srv, uint64_t req_id, int *done);
-/* Blocks until done, copies the k answers to out, releases the slot.
+/* Blocks until done, copies the k answers to out (k u8 values for an
+ * unweighted index, k u32 for a weighted one), releases the slot.
  * latency_ms (nullable) = completion time - submit time. */
-pado_gpu_status pado_gpu_serve_wait(pado_gpu_server *srv, uint64_t req_id, uint8_t *out, double *latency_ms);
+pado_gpu_status pado_gpu_serve_wait(pado_gpu_server *srv, uint64_t req_id, void *out, double *latency_ms);
 pado_gpu_status pado_gpu_serve_get_stats(pado_gpu_server *srv, pado_gpu_serve_stats *st);
 /* Drains in-flight launches and joins the dispatcher. */
 void pado_gpu_serve_stop(pado_gpu_server *srv);

@@ -30,6 +30,7 @@ QUERY_REF_COMPAT = 2
 QUERY_RANK_SPACE = 4
 BUILD_BP_PER_SLOT = 1
 BUILD_GRAPH = 2
+SERVE_GRID_STRIDE = 0x10
 WINF = 0xFFFFFFFF  # weighted query: no common hub (unreachable)
 
 _STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_NOMEM", 4: "ERR_DOMAIN", 5: "ERR_STATE",
@@ -590,13 +591,14 @@ class Server:
     """Continuous-batching query service over an ordered, resident Index."""
 
     def __init__(self, ctx, index, staging_pairs=0, max_launch_pairs=0, max_requests=0, streams=0,
-                 ref_compat=False):
+                 ref_compat=False, grid_stride=False):
         cfg = ServeCfg()
         cfg.staging_pairs = staging_pairs
         cfg.max_launch_pairs = max_launch_pairs
         cfg.max_requests = max_requests
         cfg.streams = streams
-        cfg.flags = QUERY_REF_COMPAT if ref_compat else 0
+        cfg.flags = (QUERY_REF_COMPAT if ref_compat else 0) | (SERVE_GRID_STRIDE if grid_stride else 0)
+        self.dtype = np.uint32 if index.weighted else np.uint8
         h = _vp()
         _check(lib().pado_gpu_serve_start(ctx.h, index.h, ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
@@ -619,8 +621,9 @@ class Server:
         return bool(d.value)
 
     def wait(self, rid, k):
-        """Blocks; returns (answers u8[k], latency_ms from submit to completion)."""
-        out = np.zeros(max(k, 1), np.uint8)
+        """Blocks; returns (answers[k] -- u8 hops, or u32 for a weighted index --,
+        latency_ms from submit to completion)."""
+        out = np.zeros(max(k, 1), self.dtype)
         lat = ctypes.c_double()
         _check(lib().pado_gpu_serve_wait(self.h, rid, _p(out), ctypes.byref(lat)))
         return out[:k], lat.value

@@ -97,46 +97,133 @@ __global__ __launch_bounds__(256) void k_query(uint32_t n, const uint8_t *bpd, c
 // ref_compat the reference's int16 domain (>= 32765 -> 32767, ids out of
 // range -> a == b ? 0 : 32767).  One wave per pair, the shorter list's hubs
 // binary-searched in the longer one as for unweighted labels.
+__device__ inline uint32_t wpair_distance(uint32_t n, const uint64_t *loff, const uint32_t *hub, const uint32_t *dist,
+                                          uint32_t a, uint32_t b, bool ref_compat)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    if (a >= n || b >= n) return a == b ? 0u : (ref_compat ? 32767u : 0xFFFFFFFFu);
+    const uint64_t oa = loff[a], ob = loff[b];
+    const uint32_t la = (uint32_t)(loff[a + 1] - oa), lb = (uint32_t)(loff[b + 1] - ob);
+    const uint64_t os = la <= lb ? oa : ob, ot = la <= lb ? ob : oa;
+    const uint32_t ls = la <= lb ? la : lb, lt = la <= lb ? lb : la;
+    uint32_t d = 0xFFFFFFFFu;
+    for (uint32_t j0 = 0; j0 < ls; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        if (j < ls) {
+            const uint32_t key = hub[os + j];
+            uint32_t lo = 0, hi = lt;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (hub[ot + mid] < key)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            if (lo < lt && hub[ot + lo] == key) {
+                const uint64_t td = (uint64_t)dist[os + j] + dist[ot + lo];
+                if (td < d) d = (uint32_t)td;
+            }
+        }
+    }
+    d = wave_min_u32(d);
+    if (ref_compat && d >= 32765u) d = 32767u;
+    return d;
+}
+
 __global__ __launch_bounds__(256) void k_wquery(uint32_t n, const uint64_t *loff, const uint32_t *hub,
                                                 const uint32_t *dist, const uint32_t *qa, const uint32_t *qb,
                                                 uint64_t k, uint32_t *out, int ref_compat)
 {
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t inf = ref_compat ? 32767u : 0xFFFFFFFFu;
     const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
     for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < k; i += nw) {
-        const uint32_t a = qa[i], b = qb[i];
-        if (a >= n || b >= n) {
-            if (lane == 0) out[i] = a == b ? 0u : inf;
-            continue;
-        }
-        const uint64_t oa = loff[a], ob = loff[b];
-        const uint32_t la = (uint32_t)(loff[a + 1] - oa), lb = (uint32_t)(loff[b + 1] - ob);
-        const uint64_t os = la <= lb ? oa : ob, ot = la <= lb ? ob : oa;
-        const uint32_t ls = la <= lb ? la : lb, lt = la <= lb ? lb : la;
-        uint32_t d = 0xFFFFFFFFu;
-        for (uint32_t j0 = 0; j0 < ls; j0 += 64) {
-            const uint32_t j = j0 + lane;
-            if (j < ls) {
-                const uint32_t key = hub[os + j];
-                uint32_t lo = 0, hi = lt;
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (hub[ot + mid] < key)
-                        lo = mid + 1;
-                    else
-                        hi = mid;
-                }
-                if (lo < lt && hub[ot + lo] == key) {
-                    const uint64_t td = (uint64_t)dist[os + j] + dist[ot + lo];
-                    if (td < d) d = (uint32_t)td;
-                }
-            }
-        }
-        d = wave_min_u32(d);
-        if (ref_compat && d >= 32765u) d = 32767u;
-        if (lane == 0) out[i] = d;
+        const uint32_t d = wpair_distance(n, loff, hub, dist, qa[i], qb[i], ref_compat != 0);
+        if ((threadIdx.x & 63) == 0) out[i] = d;
     }
+}
+
+// Persistent label-merge kernels for the serving loop: a grid sized to the
+// chip's resident waves; each wave takes kGrab pairs at a time from a
+// per-launch work counter until the launch's k pairs are exhausted, so waves
+// that draw long label lists do not hold back the launch.  Every wave leaves
+// once the counter passes k.
+constexpr uint32_t kGrab = 16;
+
+__global__ __launch_bounds__(256) void k_query_ws(uint32_t n, const uint8_t *bpd, const ulonglong2 *bps,
+                                                  const uint64_t *loff, const uint32_t *lab, const uint32_t *qa,
+                                                  const uint32_t *qb, uint64_t k, uint8_t *out, int ref_compat,
+                                                  unsigned long long *work)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    for (;;) {
+        unsigned long long c0 = 0;
+        if (lane == 0) c0 = atomicAdd(work, (unsigned long long)kGrab);
+        c0 = __shfl(c0, 0);
+        if (c0 >= k) break;
+        const uint64_t c1 = c0 + kGrab < k ? c0 + kGrab : k;
+        for (uint64_t i = c0; i < c1; ++i) {
+            const uint32_t d = pair_distance(n, bpd, bps, loff, lab, qa[i], qb[i], ref_compat != 0);
+            if (lane == 0) out[i] = (uint8_t)d;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_wquery_ws(uint32_t n, const uint64_t *loff, const uint32_t *hub,
+                                                   const uint32_t *dist, const uint32_t *qa, const uint32_t *qb,
+                                                   uint64_t k, uint32_t *out, int ref_compat,
+                                                   unsigned long long *work)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    for (;;) {
+        unsigned long long c0 = 0;
+        if (lane == 0) c0 = atomicAdd(work, (unsigned long long)kGrab);
+        c0 = __shfl(c0, 0);
+        if (c0 >= k) break;
+        const uint64_t c1 = c0 + kGrab < k ? c0 + kGrab : k;
+        for (uint64_t i = c0; i < c1; ++i) {
+            const uint32_t d = wpair_distance(n, loff, hub, dist, qa[i], qb[i], ref_compat != 0);
+            if (lane == 0) out[i] = d;
+        }
+    }
+}
+
+__global__ void k_zero_work(unsigned long long *work) { *work = 0ull; }
+
+pado_gpu_status launch_wquery(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,
+                              uint64_t k, uint32_t *out, bool ref_compat, hipStream_t st);
+pado_gpu_status launch_query(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,
+                             uint64_t k, uint8_t *out, bool ref_compat, hipStream_t st);
+
+// Query launch on device pairs for either index type: out holds u8
+// (unweighted) or u32 (weighted) answers.  work == nullptr selects the plain
+// grid-stride kernel; otherwise the persistent kernel with `blocks`
+// workgroups, its counter cleared on the same stream first.
+pado_gpu_status launch_query_any(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,
+                                 uint64_t k, void *out, bool ref_compat, hipStream_t st,
+                                 unsigned long long *work, uint32_t blocks)
+{
+    if (k == 0) return PADO_GPU_OK;
+    const uint64_t *loff = rank_space ? idx->loff.p : idx->qoff.p;
+    if (!work) {
+        if (idx->weighted) return launch_wquery(idx, rank_space, qa, qb, k, (uint32_t *)out, ref_compat, st);
+        return launch_query(idx, rank_space, qa, qb, k, (uint8_t *)out, ref_compat, st);
+    }
+    const uint64_t need = (k + kGrab - 1) / kGrab;  // grabs; one wave per grab at most
+    uint32_t grid = blocks;
+    if ((uint64_t)grid * 4 > need) grid = (uint32_t)((need + 3) / 4);
+    if (grid == 0) grid = 1;
+    hipLaunchKernelGGL(k_zero_work, dim3(1), dim3(1), 0, st, work);
+    if (idx->weighted) {
+        hipLaunchKernelGGL(k_wquery_ws, dim3(grid), dim3(256), 0, st, idx->n, loff,
+                           rank_space ? idx->whub.p : idx->qlab.p, rank_space ? idx->wdist.p : idx->qwdist.p, qa, qb,
+                           k, (uint32_t *)out, ref_compat ? 1 : 0, work);
+    } else {
+        hipLaunchKernelGGL(k_query_ws, dim3(grid), dim3(256), 0, st, idx->n, rank_space ? idx->bpd.p : idx->qbpd.p,
+                           (const ulonglong2 *)(rank_space ? idx->bps.p : idx->qbps.p), loff,
+                           rank_space ? idx->lab.p : idx->qlab.p, qa, qb, k, (uint8_t *)out, ref_compat ? 1 : 0,
+                           work);
+    }
+    PADO_HIP(hipGetLastError());
+    return PADO_GPU_OK;
 }
 
 pado_gpu_status launch_wquery(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,

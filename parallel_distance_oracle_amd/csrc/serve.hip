@@ -4,9 +4,11 @@
 // 2283-2416) answers one pair per call.  Here requests with any number of
 // pairs are admitted into a pinned staging ring (admission = a free request
 // slot + ring space, i.e. device staging memory), a dispatcher thread packs the
-// requests queued at that moment into one launch of the query kernel per
-// stream (up to `streams` launches in flight), and a completer thread marks
-// requests done in launch order.  A request that arrives while launches run
+// requests queued at that moment into one launch per stream (up to `streams`
+// launches in flight) of the persistent label-merge kernel (resident
+// workgroups draining the launch's pairs through a work counter), and a
+// completer thread marks requests done in launch order.  Weighted indexes are
+// served the same way with u32 answers.  A request that arrives while launches run
 // joins the next launch; its pairs overlap on the device with the running
 // ones.  Every answer depends only on its own pair, so a request's result is
 // the same alone and in any mix.
@@ -22,8 +24,9 @@
 #include "internal.hpp"
 
 namespace pado {
-pado_gpu_status launch_query(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,
-                             uint64_t k, uint8_t *out, bool ref_compat, hipStream_t st);
+pado_gpu_status launch_query_any(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,
+                                 uint64_t k, void *out, bool ref_compat, hipStream_t st,
+                                 unsigned long long *work, uint32_t blocks);
 }
 
 using namespace pado;
@@ -57,10 +60,13 @@ struct pado_gpu_server {
     pado_gpu_serve_cfg cfg{};
     uint64_t cap = 0;               // ring capacity in pairs
     uint64_t head = 0, tail = 0;    // monotonic ring positions (released / reserved)
+    uint32_t es = 1;                // answer bytes: 1 (u8 hops) or 4 (u32 weighted)
+    uint32_t ws_blocks = 0;         // persistent kernel grid (workgroups)
     uint32_t *h_a = nullptr, *h_b = nullptr;
     uint8_t *h_out = nullptr;
     DevBuf<uint32_t> d_a, d_b;
     DevBuf<uint8_t> d_out;
+    DevBuf<unsigned long long> work;  // one work counter per stream
     std::vector<hipStream_t> streams;
     std::vector<int> stream_busy;
     std::deque<Request *> live;     // ring order, not yet released
@@ -136,13 +142,17 @@ void dispatcher_loop(pado_gpu_server *s)
         if (k) {
             (void)hipMemcpyAsync(s->d_a.p + phys, s->h_a + phys, k * 4, hipMemcpyHostToDevice, st);
             (void)hipMemcpyAsync(s->d_b.p + phys, s->h_b + phys, k * 4, hipMemcpyHostToDevice, st);
-            pado_gpu_status qs = launch_query(s->idx, false, s->d_a.p + phys, s->d_b.p + phys, k, s->d_out.p + phys,
-                                              (s->cfg.flags & PADO_GPU_QUERY_REF_COMPAT) != 0, st);
+            const bool grid_stride = (s->cfg.flags & PADO_GPU_SERVE_GRID_STRIDE) != 0;
+            pado_gpu_status qs = launch_query_any(s->idx, false, s->d_a.p + phys, s->d_b.p + phys, k,
+                                                  s->d_out.p + phys * s->es,
+                                                  (s->cfg.flags & PADO_GPU_QUERY_REF_COMPAT) != 0, st,
+                                                  grid_stride ? nullptr : s->work.p + si, s->ws_blocks);
             if (qs != PADO_GPU_OK) {
                 std::lock_guard<std::mutex> g(s->mu);
                 s->error = pado_gpu_last_error();
             }
-            (void)hipMemcpyAsync(s->h_out + phys, s->d_out.p + phys, k, hipMemcpyDeviceToHost, st);
+            (void)hipMemcpyAsync(s->h_out + phys * s->es, s->d_out.p + phys * s->es, k * s->es,
+                                 hipMemcpyDeviceToHost, st);
         }
         (void)hipEventRecord(L.ev_end, st);
         lk.lock();
@@ -207,8 +217,8 @@ pado_gpu_status pado_gpu_serve_start(pado_gpu_ctx *ctx, pado_gpu_index *idx, con
         return PADO_GPU_ERR_ARG;
     }
     *out = nullptr;
-    if (!idx->ordered || idx->weighted) {
-        set_error("serving needs an ordered unweighted index (call pado_gpu_index_order first)");
+    if (!idx->ordered) {
+        set_error("serving needs an ordered index (call pado_gpu_index_order first)");
         return PADO_GPU_ERR_STATE;
     }
     PADO_HIP(hipSetDevice(ctx->device));
@@ -219,13 +229,16 @@ pado_gpu_status pado_gpu_serve_start(pado_gpu_ctx *ctx, pado_gpu_index *idx, con
     if (!s->cfg.max_launch_pairs) s->cfg.max_launch_pairs = 4ull << 20;
     if (!s->cfg.max_requests) s->cfg.max_requests = 4096;
     if (!s->cfg.streams) s->cfg.streams = 3;
+    s->es = idx->weighted ? 4u : 1u;
+    // Persistent kernel: 4 workgroups of 256 threads per compute unit.
+    s->ws_blocks = 4u * (ctx->num_cu > 0 ? (uint32_t)ctx->num_cu : 256u);
     uint64_t cap = s->cfg.staging_pairs;
     if (!cap) {
-        // Admission by free device memory: a quarter of what is free, 9 bytes
-        // per staged pair (two ids + answer), at most 16M pairs.
+        // Admission by free device memory: a quarter of what is free, 8 bytes
+        // of ids plus the answer per staged pair, at most 16M pairs.
         size_t fr = 0, tot = 0;
         PADO_HIP(hipMemGetInfo(&fr, &tot));
-        cap = std::min<uint64_t>(16ull << 20, (uint64_t)(fr / 4) / 9);
+        cap = std::min<uint64_t>(16ull << 20, (uint64_t)(fr / 4) / (8 + s->es));
     }
     if (cap < 1024) cap = 1024;
     s->cap = cap;
@@ -240,12 +253,14 @@ pado_gpu_status pado_gpu_serve_start(pado_gpu_ctx *ctx, pado_gpu_index *idx, con
     };
     if (hipHostMalloc((void **)&s->h_a, cap * 4, hipHostMallocDefault) != hipSuccess ||
         hipHostMalloc((void **)&s->h_b, cap * 4, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&s->h_out, cap, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&s->h_out, cap * s->es, hipHostMallocDefault) != hipSuccess) {
         set_error("pinned staging allocation failed");
         return fail(PADO_GPU_ERR_NOMEM);
     }
-    pado_gpu_status a1 = s->d_a.alloc(cap), a2 = s->d_b.alloc(cap), a3 = s->d_out.alloc(cap);
-    if (a1 != PADO_GPU_OK || a2 != PADO_GPU_OK || a3 != PADO_GPU_OK) return fail(PADO_GPU_ERR_NOMEM);
+    pado_gpu_status a1 = s->d_a.alloc(cap), a2 = s->d_b.alloc(cap), a3 = s->d_out.alloc(cap * s->es);
+    pado_gpu_status a4 = s->work.alloc(s->cfg.streams);
+    if (a1 != PADO_GPU_OK || a2 != PADO_GPU_OK || a3 != PADO_GPU_OK || a4 != PADO_GPU_OK)
+        return fail(PADO_GPU_ERR_NOMEM);
     for (uint32_t i = 0; i < s->cfg.streams; ++i) {
         hipStream_t q = nullptr;
         if (hipStreamCreateWithFlags(&q, hipStreamNonBlocking) != hipSuccess) {
@@ -317,7 +332,7 @@ pado_gpu_status pado_gpu_serve_poll(pado_gpu_server *s, uint64_t req_id, int *do
     return PADO_GPU_OK;
 }
 
-pado_gpu_status pado_gpu_serve_wait(pado_gpu_server *s, uint64_t req_id, uint8_t *out, double *latency_ms)
+pado_gpu_status pado_gpu_serve_wait(pado_gpu_server *s, uint64_t req_id, void *out, double *latency_ms)
 {
     clear_error();
     if (!s) return PADO_GPU_ERR_ARG;
@@ -333,7 +348,7 @@ pado_gpu_status pado_gpu_serve_wait(pado_gpu_server *s, uint64_t req_id, uint8_t
         set_error(s->error);
         return PADO_GPU_ERR_HIP;
     }
-    if (r->k && out) std::memcpy(out, s->h_out + (r->start % s->cap), r->k);
+    if (r->k && out) std::memcpy(out, s->h_out + (r->start % s->cap) * s->es, r->k * s->es);
     if (latency_ms) *latency_ms = r->t_done - r->t_submit;
     r->state = 3;
     release_consumed(s);

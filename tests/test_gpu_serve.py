@@ -98,3 +98,52 @@ def test_admission_refuses_when_full(gpu_ctx, served):
         assert srv.stats()["rejected"] == 2
     finally:
         srv.stop()
+
+
+def test_persistent_and_grid_stride_kernels_agree(gpu_ctx, served):
+    """The persistent label-merge kernel (work-counter waves) and the plain
+    grid-stride kernel give the same answers, exact and reference-compatible."""
+    G, idx, r2i = served
+    reqs = _requests(G.num_v, r2i, 12, seed=23)
+    for ref in (False, True):
+        outs = []
+        for gs in (False, True):
+            srv = pado.Server(gpu_ctx, idx, staging_pairs=1 << 20, streams=2, ref_compat=ref, grid_stride=gs)
+            try:
+                ids = [srv.submit(a, b) for a, b in reqs]
+                outs.append([srv.wait(r, a.size)[0] for r, (a, _) in zip(ids, reqs)])
+            finally:
+                srv.stop()
+        for (a, b), x, y in zip(reqs, *outs):
+            assert np.array_equal(x, y)
+            assert np.array_equal(x, idx.query(a, b, ref_compat=ref))
+
+
+def test_weighted_index_served(gpu_ctx):
+    """A weighted index is served with u32 answers equal to the batch query and
+    to Dijkstra."""
+    import oracle
+
+    e = rmat_edges(12, 8)
+    w = pado.edge_weights(e.shape[0], 1, 100, seed=1)
+    G = pado.WeightedGraph.from_edges(e, w)
+    rank = G.make_rank()
+    r2i = G.id_transfer(rank)
+    print(f"[gpu] weighted serve n={G.num_v}", flush=True)
+    X = pado.Index.build(gpu_ctx, pado.DeviceGraph(gpu_ctx, G))
+    X.order(rank)
+    reqs = _requests(G.num_v, r2i, 20, seed=5)
+    srv = pado.Server(gpu_ctx, X, staging_pairs=1 << 20, streams=3)
+    try:
+        ids = [srv.submit(a, b) for a, b in reqs]
+        outs = [srv.wait(r, a.size)[0] for r, (a, _) in zip(ids, reqs)]
+    finally:
+        srv.stop()
+    for (a, b), o in zip(reqs, outs):
+        assert o.dtype == np.uint32
+        assert np.array_equal(o, X.query(a, b))
+    a, b = reqs[3]
+    src = int(rank[a[0]])
+    d = oracle.dijkstra(G.num_v, G.off, G.adj, G.w, src)
+    sel = a == a[0]
+    assert np.array_equal(outs[3][sel], d[rank[b[sel]]])
