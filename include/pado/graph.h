@@ -126,6 +126,119 @@ public:
     }
 };
 
+// WeightedGraph (includes/graph.h:223-408): Graph's CSR plus out_weights.
+class WeightedGraph final {
+public:
+    idi num_v = 0;
+    idi num_e = 0;
+    idi *vertices = nullptr;
+    idi *out_edges = nullptr;
+    idi *out_degrees = nullptr;
+    weighti *out_weights = nullptr;
+
+    WeightedGraph() = default;
+    // "head tail weight" lines; '#' / '%' lines skipped (includes/graph.h:277-304).
+    explicit WeightedGraph(const char *filename)
+    {
+        std::ifstream ifin(filename);
+        if (!ifin.is_open()) {
+            fprintf(stderr, "Error: cannot open file %s\n", filename);
+            exit(EXIT_FAILURE);
+        }
+        std::string line;
+        idi head = 0, tail = 0;
+        inti weight = 0;
+        std::vector<std::pair<idi, idi>> edge_list;
+        std::vector<weighti> weight_list;
+        while (getline(ifin, line)) {
+            if (line[0] == '#' || line[0] == '%') continue;
+            std::istringstream lin(line);
+            lin >> head >> tail >> weight;
+            edge_list.push_back(std::make_pair(head, tail));
+            weight_list.push_back(static_cast<weighti>(weight));
+        }
+        construct(edge_list, weight_list);
+    }
+    WeightedGraph(const std::vector<std::pair<idi, idi>> &edge_list, const std::vector<weighti> &weight_list)
+    {
+        construct(edge_list, weight_list);
+    }
+    WeightedGraph(const WeightedGraph &) = delete;
+    WeightedGraph &operator=(const WeightedGraph &) = delete;
+    ~WeightedGraph()
+    {
+        free(vertices);
+        free(out_edges);
+        free(out_degrees);
+        free(out_weights);
+    }
+    idi get_num_v() const { return num_v; }
+    idi get_num_e() const { return num_e; }
+    idi ith_get_edge(idi i, idi e) const { return out_edges[vertices[i] + e]; }
+    idi ith_get_out_degree(idi i) const { return out_degrees[i]; }
+
+    void construct(const std::vector<std::pair<idi, idi>> &edge_list, const std::vector<weighti> &weight_list)
+    {
+        const size_t m = edge_list.size();
+        std::vector<uint32_t> flat(2 * m), w32(m);
+        for (size_t i = 0; i < m; ++i) {
+            flat[2 * i] = edge_list[i].first;
+            flat[2 * i + 1] = edge_list[i].second;
+            w32[i] = i < weight_list.size() ? weight_list[i] : 0;
+        }
+        num_e = (idi)(2 * m);
+        num_v = m ? pado_graph_num_v(flat.data(), m) : 0;
+        std::vector<uint64_t> off((size_t)num_v + 1, 0);
+        vertices = (idi *)malloc(sizeof(idi) * (num_v ? num_v : 1));
+        out_edges = (idi *)malloc(sizeof(idi) * (num_e ? num_e : 1));
+        out_degrees = (idi *)malloc(sizeof(idi) * (num_v ? num_v : 1));
+        out_weights = (weighti *)malloc(sizeof(weighti) * (num_e ? num_e : 1));
+        if (num_v) pado_wgraph_csr(flat.data(), w32.data(), m, num_v, off.data(), out_edges, out_weights);
+        for (idi v = 0; v < num_v; ++v) {
+            vertices[v] = (idi)off[v];
+            out_degrees[v] = (idi)(off[v + 1] - off[v]);
+        }
+    }
+
+    // Same degree order as Graph::make_rank (includes/graph.h:353-372).
+    std::vector<idi> make_rank()
+    {
+        std::vector<uint64_t> off;
+        detail::to_u64(vertices, num_v, num_e, off);
+        std::vector<idi> rank(num_v);
+        pado_make_rank(off.data(), num_v, 1, rank.data());
+        return rank;
+    }
+
+    // (id, weight) lists sorted descending (includes/graph.h:374-408).
+    std::vector<idi> id_transfer(const std::vector<idi> &rank)
+    {
+        std::vector<uint64_t> off, off2((size_t)num_v + 1);
+        detail::to_u64(vertices, num_v, num_e, off);
+        std::vector<idi> adj2(num_e ? num_e : 1), rank2id(num_v);
+        std::vector<weighti> w2(num_e ? num_e : 1);
+        pado_wid_transfer(num_v, off.data(), out_edges, out_weights, rank.data(), off2.data(), adj2.data(),
+                          w2.data(), rank2id.data());
+        for (idi v = 0; v < num_v; ++v) {
+            vertices[v] = (idi)off2[v];
+            out_degrees[v] = (idi)(off2[v + 1] - off2[v]);
+        }
+        if (num_e) {
+            memcpy(out_edges, adj2.data(), sizeof(idi) * num_e);
+            memcpy(out_weights, w2.data(), sizeof(weighti) * num_e);
+        }
+        return rank2id;
+    }
+
+    void print()
+    {
+        printf("num_v: %u, num_e: %u\n", num_v, num_e / 2);
+        for (idi h = 0; h < num_v; ++h)
+            for (idi e = vertices[h]; e < vertices[h] + out_degrees[h]; ++e)
+                printf("%u %u %u\n", h, out_edges[e], (unsigned)out_weights[e]);
+    }
+};
+
 }  // namespace PADO
 
 #endif  // PADO_GPU_DROPIN_GRAPH_H_

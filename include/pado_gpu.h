@@ -230,6 +230,16 @@ pado_gpu_status pado_gpu_windex_export(pado_gpu_ctx *ctx, const pado_gpu_index *
 pado_gpu_status pado_gpu_wquery(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uint32_t *a, const uint32_t *b,
                                 uint64_t k, uint32_t *out, uint32_t flags, void *stream);
 
+/* store_index_to_file / load_index_from_file of the weighted class
+ * (WPARA:1982-2018, 2021-2055): u32 num_v, then per original id u32 size and
+ * size x (u32 rank-space hub, i16 dist), hubs ascending.  store needs the
+ * rank array (num_v entries) and returns PADO_GPU_ERR_DOMAIN if a distance
+ * does not fit int16; load returns an ordered weighted index (queries by
+ * original id through pado_gpu_wquery). */
+pado_gpu_status pado_gpu_windex_store(pado_gpu_ctx *ctx, const pado_gpu_index *idx, const char *filename,
+                                      const uint32_t *rank, uint32_t n);
+pado_gpu_status pado_gpu_windex_load(pado_gpu_ctx *ctx, const char *filename, pado_gpu_index **out);
+
 /* ---------------------------------------------------------------------------
  * Autotuner (no reference counterpart).  Every knob of pado_gpu_build_cfg
  * leaves the labels unchanged, so the tuner may pick any of them purely on

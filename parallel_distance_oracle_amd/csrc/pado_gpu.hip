@@ -677,4 +677,125 @@ pado_gpu_status pado_gpu_index_load(pado_gpu_ctx *ctx, const char *filename, pad
     return PADO_GPU_OK;
 }
 
+// Weighted index file (WPARA:1982-2018 writer, 2021-2055 reader): u32 num_v,
+// then per ORIGINAL id: u32 size, size x (u32 rank-space hub, i16 dist),
+// hubs ascending.
+pado_gpu_status pado_gpu_windex_store(pado_gpu_ctx *ctx, const pado_gpu_index *idx, const char *filename,
+                                      const uint32_t *rank, uint32_t n)
+{
+    clear_error();
+    PADO_TRY(check_ctx(ctx));
+    if (!idx || !filename || !rank || n != idx->n || !idx->weighted || !idx->loff.p) {
+        set_error("bad argument (a weighted rank-space index and num_v rank entries are required)");
+        return PADO_GPU_ERR_ARG;
+    }
+    std::vector<uint64_t> off((uint64_t)n + 1);
+    std::vector<uint32_t> hub(idx->num_labels ? idx->num_labels : 1), dist(idx->num_labels ? idx->num_labels : 1);
+    PADO_TRY(pado_gpu_windex_export(ctx, idx, off.data(), hub.data(), dist.data()));
+    for (uint64_t i = 0; i < idx->num_labels; ++i)
+        if (dist[i] >= 32767u) {
+            set_error("a label distance does not fit the file's int16 field");
+            return PADO_GPU_ERR_DOMAIN;
+        }
+    std::ofstream f(filename, std::ios::binary);
+    if (!f.is_open()) {
+        set_error(std::string("cannot open file ") + filename);
+        return PADO_GPU_ERR_IO;
+    }
+    f.write((const char *)&n, 4);
+    for (uint32_t x = 0; x < n; ++x) {
+        const uint32_t v = rank[x];
+        if (v >= n) {
+            set_error("rank out of range");
+            return PADO_GPU_ERR_ARG;
+        }
+        const uint32_t sz = (uint32_t)(off[v + 1] - off[v]);
+        f.write((const char *)&sz, 4);
+        for (uint64_t i = off[v]; i < off[v + 1]; ++i) {
+            const int16_t d = (int16_t)dist[i];
+            f.write((const char *)&hub[i], 4);
+            f.write((const char *)&d, 2);
+        }
+    }
+    if (!f.good()) {
+        set_error(std::string("write failed: ") + filename);
+        return PADO_GPU_ERR_IO;
+    }
+    return PADO_GPU_OK;
+}
+
+pado_gpu_status pado_gpu_windex_load(pado_gpu_ctx *ctx, const char *filename, pado_gpu_index **out)
+{
+    clear_error();
+    PADO_TRY(check_ctx(ctx));
+    if (!filename || !out) {
+        set_error("bad argument");
+        return PADO_GPU_ERR_ARG;
+    }
+    *out = nullptr;
+    std::ifstream f(filename, std::ios::binary);
+    if (!f.is_open()) {
+        set_error(std::string("cannot open file ") + filename);
+        return PADO_GPU_ERR_IO;
+    }
+    uint32_t n = 0;
+    f.read((char *)&n, 4);
+    if (!f.good() || n == 0 || n >= kMaxVertices) {
+        set_error("bad index file header");
+        return PADO_GPU_ERR_IO;
+    }
+    std::vector<uint64_t> off((uint64_t)n + 1, 0);
+    std::vector<uint32_t> hub, dist;
+    for (uint32_t x = 0; x < n; ++x) {
+        uint32_t sz = 0;
+        f.read((char *)&sz, 4);
+        if (!f.good()) {
+            set_error("truncated index file");
+            return PADO_GPU_ERR_IO;
+        }
+        for (uint32_t i = 0; i < sz; ++i) {
+            uint32_t h = 0;
+            int16_t d = 0;
+            f.read((char *)&h, 4);
+            f.read((char *)&d, 2);
+            if (h >= n || d < 0) {
+                set_error("bad label in index file");
+                return PADO_GPU_ERR_IO;
+            }
+            hub.push_back(h);
+            dist.push_back((uint32_t)d);
+        }
+        off[x + 1] = hub.size();
+    }
+    if (!f.good()) {
+        set_error("truncated index file");
+        return PADO_GPU_ERR_IO;
+    }
+    auto *idx = new pado_gpu_index();
+    idx->ctx = ctx;
+    idx->n = n;
+    idx->weighted = 1;
+    idx->num_labels = hub.size();
+    const uint64_t L = hub.size();
+    hipStream_t st = ctx->stream;
+    pado_gpu_status s = idx->qoff.alloc((uint64_t)n + 1);
+    if (s == PADO_GPU_OK) s = idx->qlab.alloc(L);
+    if (s == PADO_GPU_OK) s = idx->qwdist.alloc(L);
+    if (s != PADO_GPU_OK) {
+        delete idx;
+        return s;
+    }
+    if (hipMemcpyAsync(idx->qoff.p, off.data(), 8ull * (n + 1), hipMemcpyHostToDevice, st) != hipSuccess ||
+        (L && hipMemcpyAsync(idx->qlab.p, hub.data(), 4ull * L, hipMemcpyHostToDevice, st) != hipSuccess) ||
+        (L && hipMemcpyAsync(idx->qwdist.p, dist.data(), 4ull * L, hipMemcpyHostToDevice, st) != hipSuccess) ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        delete idx;
+        set_error("index upload failed");
+        return PADO_GPU_ERR_HIP;
+    }
+    idx->ordered = 1;
+    *out = idx;
+    return PADO_GPU_OK;
+}
+
 }  // extern "C"

@@ -35,8 +35,9 @@ def test_null_arguments_are_rejected():
 
 
 @pytest.mark.skipif(shutil.which("g++") is None, reason="no host C++ compiler")
-def test_cpp_drop_in_compiles(tmp_path):
-    src = os.path.join(ROOT, "tests", "cpp", "dropin_example.cpp")
+@pytest.mark.parametrize("example", ["dropin_example.cpp", "wdropin_example.cpp"])
+def test_cpp_drop_in_compiles(tmp_path, example):
+    src = os.path.join(ROOT, "tests", "cpp", example)
     if not os.path.exists(src):
         pytest.skip("C++ drop-in example not present")
     out = tmp_path / "dropin"
