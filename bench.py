@@ -232,6 +232,31 @@ def serving_leg(ctx, scale=14, ef=8, duration_s=4.0, seed=5):
             "launches": st["launches"], "requests": st["requests_done"], "rejected": st["rejected"]}
 
 
+def copy_calibrate(mib):
+    """A plain copy of known size: under rocprofv3 --pmc its FETCH_SIZE and
+    WRITE_SIZE per dispatch give the counters' scale (expected mib MiB each)."""
+    import torch
+
+    n = mib << 20
+    src = torch.ones(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"copied {mib} MiB x 5")
+
+
+def load_traffic():
+    """HBM bytes per launch of the dominant kernel from the committed PMC
+    summary (two rocprofv3 --pmc runs, FETCH_SIZE then WRITE_SIZE, scaled by
+    the copy-kernel calibration), or None."""
+    p = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        return json.load(f)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -243,7 +268,15 @@ def main():
     ap.add_argument("--cpu-scale", type=int, default=18)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--build-only", action="store_true",
+                    help="timed builds and the roofline only (for profiler runs)")
+    ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (PMC byte-counter calibration) and exit")
+    ap.add_argument("--graph", action="store_true", help="build with the hipGraph-captured batch step")
     args = ap.parse_args()
+    if args.copy_calibrate:
+        copy_calibrate(args.copy_calibrate)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the index build and "
               f"queries run on a single MI355X; see DESIGN.md for how they would shard)", file=sys.stderr)
@@ -255,10 +288,13 @@ def main():
     ctx = pado.Context(0)
     dg = pado.DeviceGraph(ctx, G)
     ctx.sync()
+    cfg = pado.BuildCfg()
+    cfg.batch_size = pado.BATCH_SIZE
+    cfg.flags = pado.BUILD_GRAPH if args.graph else 0
 
     for w in range(args.warmup):
         t0 = time.time()
-        idx = pado.Index.build(ctx, dg)
+        idx = pado.Index.build(ctx, dg, cfg=cfg)
         ctx.sync()
         log(f"warmup build {w}: {time.time() - t0:.3f}s labels={idx.info.num_labels}")
         idx.close()
@@ -270,7 +306,7 @@ def main():
         if idx is not None:
             idx.close()
         t0 = time.perf_counter()
-        idx = pado.Index.build(ctx, dg)
+        idx = pado.Index.build(ctx, dg, cfg=cfg)
         ctx.sync()
         dt = time.perf_counter() - t0
         times.append(dt)
@@ -291,7 +327,19 @@ def main():
         parity["reference_build_s_container"] = meta.get("ref_build_s")
     del bp_d, bp_s, hub, dist
 
-    # Roofline of the dominant labeling kernel (push or add), per launch.
+    # Roofline of the dominant labeling kernel (push or add), per launch.  The
+    # stream-mode build brackets each push / add launch with HIP events on the
+    # build stream; a hipGraph build records none, so then one extra
+    # stream-mode build of the same graph (same kernels and grids) times them.
+    timing_src = "timed builds (HIP events on the build stream)"
+    if args.graph:
+        tb = pado.Index.build(ctx, dg)
+        ctx.sync()
+        st_t = tb.stats()
+        tb.close()
+        for k in ("ms_push", "ms_add", "n_push", "n_add"):
+            st[k] = st_t[k]
+        timing_src = "one stream-mode build after the timed hipGraph builds (HIP events on the build stream)"
     push_bytes = 8 * st["actives"] + 4 * st["edges"] + BP_RECORD_BYTES * st["tails"]
     add_bytes = 5 * st["dq_labels"] + 5 * st["labels_appended"]
     kern = {
@@ -306,7 +354,17 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
                 "avg_launch_us": round(avg_ms * 1e3, 2), "launches": int(n_launch),
                 "alg_bytes_per_launch": round(bytes_tot / max(n_launch, 1), 1),
-                "kernel_ms": {k: round(v[0], 2) for k, v in kern.items()}}
+                "kernel_ms": {k: round(v[0], 2) for k, v in kern.items()}, "timing": timing_src}
+    tr = load_traffic()
+    wl = f"rmat{args.scale}_ef{args.edge_factor}"
+    if tr and tr.get("workload") == wl and dom in tr.get("kernels", {}):
+        t = tr["kernels"][dom]
+        roofline["traffic"] = t["bytes_per_launch"]
+        roofline["traffic_source"] = tr.get("source")
+    if args.build_only:
+        print(json.dumps({"metric": "R-MAT build only (profiling run)", "value": round(build_s, 4), "unit": "s",
+                          "roofline": roofline, "parity": parity, "stats": st}), flush=True)
+        return
 
     # Plain batch query: order by original id, k pairs on device, one launch.
     idx.order(rank)
@@ -368,7 +426,8 @@ def main():
         "vs_baseline": None, "dtype": "u8", "data": "synthetic",
         "config": {"workload": f"R-MAT scale {args.scale} ef {args.edge_factor} (0.57,0.19,0.19,0.05) seed 1, "
                                f"BATCH_SIZE=1024, BITPARALLEL_SIZE=50; full index build",
-                   "num_v": G.num_v, "num_edges": int(edges.shape[0])},
+                   "num_v": G.num_v, "num_edges": int(edges.shape[0]),
+                   "batch_step": "hipGraph" if args.graph else "stream (launch-ahead blocks)"},
         "parity": parity, "query": query, "serving": serving, "roofline": roofline, "cpu_baseline": cpu,
         "build_breakdown_ms": {"bp": round(st["ms_bp"], 1), "batches_and_sort": round(st["ms_batch"], 1)},
     }
