@@ -133,12 +133,69 @@ def serving_leg(ctx, scale=14, ef=8, duration_s=4.0, seed=5):
     uniform, half Zipf(1.1) over degree rank.  Latency = completion - arrival
     (admission waits included).  Compared with the plain path: the same pairs
     in one query launch."""
-    import threading
-
     edges, G, rank, rank2id = make_graph(scale, ef)
     idx = pado.Index.build(ctx, pado.DeviceGraph(ctx, G))
     idx.order(rank)
+    res = serve_measure(ctx, idx, G.num_v, rank2id, duration_s, seed)
+    res["workload"] = (f"R-MAT {scale} ef{ef} index resident; open-loop Poisson; log-uniform 1..65536 "
+                       f"pairs/request; half uniform, half Zipf(1.1) over degree rank")
+    return res
+
+
+def weighted_leg(ctx, scale, ef=16, duration_s=4.0, seed=7, dijkstra_sources=4):
+    """Weighted variant: R-MAT `scale` with integer weights U[1,100] from the
+    seed, weighted index built and kept resident, served with the same request
+    mix; sampled distances checked exact against Dijkstra (scipy)."""
+    import scipy.sparse as sp
+    from scipy.sparse.csgraph import dijkstra
+
+    t0 = time.time()
+    edges = pado.rmat_edges(scale, ef, 0.57, 0.19, 0.19, 1)
+    w = pado.edge_weights(edges.shape[0], 1, 100, seed=1)
+    G = pado.WeightedGraph.from_edges(edges, w)
+    rank = G.make_rank()
+    rank2id = G.id_transfer(rank)
+    log(f"weighted R-MAT {scale} ef{ef}: n={G.num_v} edges={edges.shape[0]} host ingest {time.time() - t0:.1f}s")
+    dg = pado.DeviceGraph(ctx, G)
+    ctx.sync()
+    t0 = time.perf_counter()
+    idx = pado.Index.build(ctx, dg)
+    ctx.sync()
+    build_s = time.perf_counter() - t0
+    st = idx.stats()
+    log(f"weighted build {build_s:.3f}s labels={st['num_labels']} rounds={st['iterations']}")
+    idx.order(rank)
+    # Dijkstra on the original-id graph from a few sources.
     n = G.num_v
+    src_rank_space = G.off, G.adj, G.w
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(G.off.astype(np.int64)))
+    A = sp.csr_matrix((G.w.astype(np.float64), (rows, G.adj.astype(np.int64))), shape=(n, n))
+    rng = np.random.default_rng(seed)
+    ok = True
+    checked = 0
+    for s_orig in rng.integers(0, n, dijkstra_sources):
+        d = dijkstra(A, directed=False, indices=int(rank[s_orig]), min_only=True)
+        tgt = rng.integers(0, n, 20000).astype(np.uint32)
+        got = idx.query(np.full(tgt.size, s_orig, np.uint32), tgt).astype(np.float64)
+        exp = d[rank[tgt]]
+        exp = np.where(np.isinf(exp), float(pado.WINF), exp)
+        ok &= bool(np.array_equal(got, exp))
+        checked += tgt.size
+    del src_rank_space
+    res = serve_measure(ctx, idx, n, rank2id, duration_s, seed)
+    res.update({"workload": f"weighted R-MAT {scale} ef{ef}, weights U[1,100] seed 1; same request mix",
+                "num_v": n, "num_edges": int(edges.shape[0]), "build_s": round(build_s, 4),
+                "labels": int(st["num_labels"]), "rounds": int(st["iterations"]),
+                "dijkstra_pairs_checked": checked, "dijkstra_exact": ok})
+    idx.close()
+    dg.close()
+    return res
+
+
+def serve_measure(ctx, idx, n, rank2id, duration_s, seed):
+    """Open-loop Poisson request stream against a resident ordered index."""
+    import threading
+
     rng = np.random.default_rng(seed)
     pool_n = 4000
     sizes = np.floor(np.exp(rng.uniform(0, np.log(65536), pool_n))).astype(np.int64)
@@ -224,9 +281,7 @@ def serving_leg(ctx, scale=14, ef=8, duration_s=4.0, seed=5):
     srv.stop()
     ok_all = not results
     best = max(rates, key=lambda r: r["served_queries_per_s"])
-    return {"workload": f"R-MAT {scale} ef{ef} index resident; open-loop Poisson; log-uniform 1..65536 pairs/request;"
-                        f" half uniform, half Zipf(1.1) over degree rank",
-            "plain_batch_queries_per_s": round(plain_qps, 1), "plain_batch_pairs": int(allA.size),
+    return {"plain_batch_queries_per_s": round(plain_qps, 1), "plain_batch_pairs": int(allA.size),
             "rates": rates, "best_served_queries_per_s": best["served_queries_per_s"],
             "p99_ms_at_best": best["p99_ms"], "results_equal_plain_batch": ok_all,
             "launches": st["launches"], "requests": st["requests_done"], "rejected": st["rejected"]}
@@ -273,6 +328,8 @@ def main():
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (PMC byte-counter calibration) and exit")
     ap.add_argument("--graph", action="store_true", help="build with the hipGraph-captured batch step")
+    ap.add_argument("--weighted-scale", type=int, default=0,
+                    help="also build and serve a weighted R-MAT of this scale (0 = skip)")
     args = ap.parse_args()
     if args.copy_calibrate:
         copy_calibrate(args.copy_calibrate)
@@ -412,6 +469,14 @@ def main():
         except Exception as e:
             log(f"serving leg failed: {e}")
 
+    weighted = None
+    if args.weighted_scale:
+        try:
+            weighted = weighted_leg(ctx, args.weighted_scale)
+            log(f"weighted: {weighted}")
+        except Exception as e:
+            log(f"weighted leg failed: {e}")
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -428,7 +493,8 @@ def main():
                                f"BATCH_SIZE=1024, BITPARALLEL_SIZE=50; full index build",
                    "num_v": G.num_v, "num_edges": int(edges.shape[0]),
                    "batch_step": "hipGraph" if args.graph else "stream (launch-ahead blocks)"},
-        "parity": parity, "query": query, "serving": serving, "roofline": roofline, "cpu_baseline": cpu,
+        "parity": parity, "query": query, "serving": serving, "weighted": weighted, "roofline": roofline,
+        "cpu_baseline": cpu,
         "build_breakdown_ms": {"bp": round(st["ms_bp"], 1), "batches_and_sort": round(st["ms_batch"], 1)},
     }
     print(json.dumps(line), flush=True)
