@@ -43,7 +43,7 @@ constexpr uint32_t kWInf = 0xFFFFFFFFu;
 using u64 = unsigned long long;
 
 struct WCounters {
-    uint32_t n_act, n_chunk, n_imp, n_touched, n_hub, overflow, pad0, pad1;
+    uint32_t n_act, n_chunk, n_imp, n_touched, n_hub, overflow, gen_round, pad1;
     u64 pool_top, relax, improved, pruned, labels, filtered, rounds, need;
 };
 
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(256) void k_w_hub_fill(uint32_t rs, uint32_t rsz, W
 
 // send_messages (WPARA:435-522): relax the frontier's edges toward tails
 // ranked below each root.
-__global__ __launch_bounds__(256) void k_w_push(uint32_t rs, uint32_t gen_batch, uint32_t gen_round,
+__global__ __launch_bounds__(256) void k_w_push(uint32_t rs, uint32_t gen_batch,
                                                 const uint64_t *off, const uint32_t *adj, const uint8_t *wgt,
                                                 const uint32_t *alist, const u64 *ag, const uint32_t *amask,
                                                 const uint32_t *aend, const u64 *chunks, uint32_t *T, u64 *I,
@@ -217,7 +217,8 @@ __global__ __launch_bounds__(256) void k_w_push(uint32_t rs, uint32_t gen_batch,
 {
     const uint32_t lane = threadIdx.x & (kChunk - 1);
     const uint32_t ngroups = (gridDim.x * blockDim.x) / kChunk;
-    const uint32_t nc = C->n_chunk;
+    const uint32_t nc = C->n_chunk;  // 0 once the batch's frontier is empty
+    const uint32_t gen_round = C->gen_round;
     u64 n_relax = 0, n_imp = 0;
     for (uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) / kChunk; c < nc; c += ngroups) {
         const u64 desc = chunks[c];
@@ -275,9 +276,15 @@ __global__ void k_w_mid(WCounters *C)
     C->n_chunk = 0;
 }
 
-__global__ void k_w_post(WCounters *C)
+__global__ void k_w_post(WCounters *C) { C->n_imp = 0; }
+
+// Start of a Bellman-Ford round: a round with an empty frontier is an empty
+// launch sequence (push sees no chunks, check no improved tails), so the host
+// can enqueue several rounds per check of the frontier.
+__global__ void k_w_round_begin(WCounters *C)
 {
-    C->n_imp = 0;
+    if (C->n_act == 0) return;
+    C->gen_round += 1;
     C->rounds += 1;
 }
 
@@ -582,8 +589,7 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
 
     WStore S{pool.p, base.p, len.p, cap.p, pool_size};
     const uint32_t nb = (n + batch - 1) / batch;
-    uint32_t gen_round = 0;
-    uint64_t rounds = 0;
+    uint32_t prev_rounds = 1;
     for (uint32_t b = 0; b < nb; ++b) {
         const uint32_t rs = b * batch, rsz = std::min(batch, n - rs), gen_batch = b + 1;
         const uint32_t rb = (rsz * 64 + 255) / 256;
@@ -600,21 +606,30 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
         hipLaunchKernelGGL(k_w_hub_slots, dim3(grid_for((uint64_t)h->n_hub * 128, 256, 8192)), dim3(256), 0, st,
                            hlist.p, hslot.p, hmin.p, dcw.p, ctr.p);
         hipLaunchKernelGGL(k_w_hub_fill, dim3(rb), dim3(256), 0, st, rs, rsz, S, hslot.p, hmin.p, dcw.p);
+        // Blocks of 2^j rounds per host check; j follows the previous batch's
+        // round count and doubles while a batch runs long.
         uint32_t n_act = h->n_act;
+        const uint64_t rounds_before = h->rounds;
+        int j = 0;
+        while ((1u << j) < prev_rounds + 1 && j < 6) ++j;
         while (n_act) {
-            ++gen_round;
-            ++rounds;
-            hipLaunchKernelGGL(k_w_push, dim3(4096), dim3(256), 0, st, rs, gen_batch, gen_round, g->off.p, g->adj.p,
-                               g->w.p, alist.p, ag.p, amask.p, aend.p, chunks.p, T.p, I.p, tmark.p, tlist.p, imark.p,
-                               ilist.p, ctr.p);
-            hipLaunchKernelGGL(k_w_mid, dim3(1), dim3(1), 0, st, ctr.p);
-            hipLaunchKernelGGL(k_w_check, dim3(2048), dim3(256), 0, st, rs, ilist.p, g->off.p, g->adj.p, T.p, I.p,
-                               P.p, S, hslot.p, hmin.p, dcw.p, alist.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
-            hipLaunchKernelGGL(k_w_post, dim3(1), dim3(1), 0, st, ctr.p);
+            for (uint32_t u = 0; u < (1u << j); ++u) {
+                hipLaunchKernelGGL(k_w_round_begin, dim3(1), dim3(1), 0, st, ctr.p);
+                hipLaunchKernelGGL(k_w_push, dim3(4096), dim3(256), 0, st, rs, gen_batch, g->off.p, g->adj.p, g->w.p,
+                                   alist.p, ag.p, amask.p, aend.p, chunks.p, T.p, I.p, tmark.p, tlist.p, imark.p,
+                                   ilist.p, ctr.p);
+                hipLaunchKernelGGL(k_w_mid, dim3(1), dim3(1), 0, st, ctr.p);
+                hipLaunchKernelGGL(k_w_check, dim3(2048), dim3(256), 0, st, rs, ilist.p, g->off.p, g->adj.p, T.p,
+                                   I.p, P.p, S, hslot.p, hmin.p, dcw.p, alist.p, ag.p, amask.p, aend.p, chunks.p,
+                                   ctr.p);
+                hipLaunchKernelGGL(k_w_post, dim3(1), dim3(1), 0, st, ctr.p);
+            }
             PADO_HIP(hipMemcpyAsync(h, ctr.p, sizeof(WCounters), hipMemcpyDeviceToHost, st));
             PADO_HIP(hipStreamSynchronize(st));
             n_act = h->n_act;
+            if (j < 6) ++j;
         }
+        prev_rounds = (uint32_t)(h->rounds - rounds_before);
         const u64 labels_before = h->labels;
         hipLaunchKernelGGL(k_w_filter, dim3(2048), dim3(256), 0, st, rs, tlist.p, T.p, P.p, I.p, S, ctr.p);
         PADO_HIP(hipMemcpyAsync(h, ctr.p, sizeof(WCounters), hipMemcpyDeviceToHost, st));
@@ -661,7 +676,7 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
         set_error("weighted label pool exhausted");
         return PADO_GPU_ERR_NOMEM;
     }
-    idx->iterations = rounds;
+    idx->iterations = h->rounds;
     idx->counters.labels = h->labels;
     idx->counters.pushes = h->relax;
     idx->counters.cands = h->improved;
