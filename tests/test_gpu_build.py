@@ -154,9 +154,9 @@ def test_graph_with_isolated_and_duplicate_edges(gpu_ctx):
 
 
 def test_bad_graphs_are_rejected(gpu_ctx):
-    G = pado.Graph(3, np.array([0, 1, 2, 2], np.uint64), np.array([1, 0], np.uint32))
+    G = pado.Graph(3, np.array([0, 1, 2, 3], np.uint64), np.array([1, 0], np.uint32))
     with pytest.raises(pado.PadoError):
-        pado.DeviceGraph(gpu_ctx, G)  # off[n] != m
+        pado.DeviceGraph(gpu_ctx, G)  # off[n] = 3 != m = 2
     G = pado.Graph(3, np.array([0, 2, 3, 4], np.uint64), np.array([1, 2, 0, 0], np.uint32))
     with pytest.raises(pado.PadoError):
         pado.DeviceGraph(gpu_ctx, G)  # adjacency of 0 not descending
