@@ -114,6 +114,24 @@ pado_gpu_status pado_gpu_graph_upload(pado_gpu_ctx *ctx, uint32_t n, uint64_t m_
                                       const uint32_t *adj, const uint8_t *weights, pado_gpu_graph **out);
 void pado_gpu_graph_free(pado_gpu_graph *g);
 
+/* Graph(edge_list) / WeightedGraph(edge_list, weights) + make_rank +
+ * id_transfer (GR:120-206, 313-408) on the device: m undirected edges
+ * (edges[2i], edges[2i+1]) with optional weights[i] (cast to u8; 0 refused),
+ * both directions, duplicates and self-loops kept.  rank_in (nullable, num_v
+ * entries) fixes the order; otherwise make_rank runs on the host from the
+ * degrees (use_global_rand as for pado_make_rank).  The result is the same
+ * rank-space CSR the host path produces.  num_v = max id + 1 (as
+ * pado_graph_num_v); rank_out / rank2id_out (nullable) receive num_v entries.
+ * Fewer than 2^30 edges. */
+pado_gpu_status pado_gpu_graph_from_edges(pado_gpu_ctx *ctx, const uint32_t *edges, uint64_t m,
+                                          const uint32_t *weights, const uint32_t *rank_in, int use_global_rand,
+                                          pado_gpu_graph **out, uint32_t *num_v, uint32_t *rank_out,
+                                          uint32_t *rank2id_out);
+/* Copies of a device graph: off[n+1], adj[m_dir], w[m_dir] (each nullable). */
+pado_gpu_status pado_gpu_graph_export(pado_gpu_ctx *ctx, const pado_gpu_graph *g, uint64_t *off, uint32_t *adj,
+                                      uint8_t *w);
+pado_gpu_status pado_gpu_graph_get_info(const pado_gpu_graph *g, uint32_t *num_v, uint64_t *m_dir, int *weighted);
+
 typedef struct {
     uint32_t batch_size;  /* roots per batch: 1..1024, 0 = 1024 (PARA:42); any value
                              gives the same labels, so it is a tuning knob */

@@ -114,6 +114,11 @@ def lib():
         "pado_gpu_ctx_sync": (ctypes.c_int, [_vp]),
         "pado_gpu_graph_upload": (ctypes.c_int, [_vp, _u32, _u64, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
         "pado_gpu_graph_free": (None, [_vp]),
+        "pado_gpu_graph_from_edges": (ctypes.c_int, [_vp, _vp, _u64, _vp, _vp, ctypes.c_int, ctypes.POINTER(_vp),
+                                                     ctypes.POINTER(_u32), _vp, _vp]),
+        "pado_gpu_graph_export": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
+        "pado_gpu_graph_get_info": (ctypes.c_int, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u64),
+                                                   ctypes.POINTER(ctypes.c_int)]),
         "pado_gpu_build": (ctypes.c_int, [_vp, _vp, ctypes.POINTER(BuildCfg), ctypes.POINTER(_vp)]),
         "pado_gpu_index_get_info": (ctypes.c_int, [_vp, ctypes.POINTER(IndexInfo)]),
         "pado_gpu_index_export": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
@@ -389,6 +394,44 @@ class DeviceGraph:
                                            _p(w), ctypes.byref(h)))
         self.h = h
         self.num_v = G.num_v
+
+    @classmethod
+    def from_edges(cls, ctx, edges, weights=None, rank=None):
+        """Device ingest: the reference's Graph / WeightedGraph construct,
+        make_rank and id_transfer (includes/graph.h:120-206, 313-408) run
+        on the GPU.  Returns (device graph, rank, rank2id)."""
+        e = _c(edges, np.uint32).reshape(-1)
+        m = e.size // 2
+        n = lib().pado_graph_num_v(_p(e), m) if m else 0
+        w = _c(weights, np.uint32).reshape(-1) if weights is not None else None
+        if w is not None and w.size != m:
+            raise ValueError(f"{w.size} weights for {m} edges")
+        r_in = _c(rank, np.uint32) if rank is not None else None
+        rank_out = np.zeros(max(n, 1), np.uint32)
+        r2i = np.zeros(max(n, 1), np.uint32)
+        h = _vp()
+        nv = _u32()
+        _check(lib().pado_gpu_graph_from_edges(ctx.h, _p(e), m, _p(w), _p(r_in), 0, ctypes.byref(h),
+                                               ctypes.byref(nv), _p(rank_out), _p(r2i)))
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.h = h
+        self.num_v = nv.value
+        return self, rank_out[:n], r2i[:n]
+
+    def info(self):
+        n, m, w = _u32(), _u64(), ctypes.c_int()
+        _check(lib().pado_gpu_graph_get_info(self.h, ctypes.byref(n), ctypes.byref(m), ctypes.byref(w)))
+        return n.value, m.value, bool(w.value)
+
+    def export(self):
+        """(off[n+1], adj[m_dir], w[m_dir] or None) of the device CSR."""
+        n, m, weighted = self.info()
+        off = np.zeros(n + 1, np.uint64)
+        adj = np.zeros(max(m, 1), np.uint32)
+        w = np.zeros(max(m, 1), np.uint8) if weighted else None
+        _check(lib().pado_gpu_graph_export(self.ctx.h, self.h, _p(off), _p(adj), _p(w)))
+        return off, adj[:m], (w[:m] if weighted else None)
 
     def close(self):
         if self.h:
