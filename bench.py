@@ -21,7 +21,6 @@ build seconds over the K timed steps (lower is better).  The line also carries:
 Progress goes to stderr; the JSON line to stdout.
 """
 import argparse
-import ctypes
 import hashlib
 import json
 import os
@@ -56,32 +55,6 @@ def golden_meta(scale, ef):
         with open(p) as f:
             return json.load(f)
     return None
-
-
-class HipEvents:
-    """hipEvent timing on an explicit stream (the library's query stream)."""
-
-    def __init__(self):
-        self.hip = ctypes.CDLL("libamdhip64.so")
-        self.hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
-        self.hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
-        self.hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
-        self.hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
-        self.hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
-
-    def event(self):
-        e = ctypes.c_void_p()
-        assert self.hip.hipEventCreate(ctypes.byref(e)) == 0
-        return e
-
-    def record(self, e, stream):
-        assert self.hip.hipEventRecord(e, ctypes.c_void_p(stream)) == 0
-
-    def elapsed_ms(self, a, b):
-        assert self.hip.hipEventSynchronize(b) == 0
-        t = ctypes.c_float()
-        assert self.hip.hipEventElapsedTime(ctypes.byref(t), a, b) == 0
-        return float(t.value)
 
 
 def make_graph(scale, ef):
@@ -290,15 +263,17 @@ def serve_measure(ctx, idx, n, rank2id, duration_s, seed):
 def copy_calibrate(mib):
     """A plain copy of known size: under rocprofv3 --pmc its FETCH_SIZE and
     WRITE_SIZE per dispatch give the counters' scale (expected mib MiB each)."""
-    import torch
+    from parallel_distance_oracle_amd.hip import DeviceArray
 
     n = mib << 20
-    src = torch.ones(n, dtype=torch.uint8, device="cuda")
-    dst = torch.empty_like(src)
+    ctx = pado.Context(0)
+    src = DeviceArray.from_numpy(np.ones(n, np.uint8))
+    dst = DeviceArray(n, np.uint8)
     for _ in range(5):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    log(f"copied {mib} MiB x 5")
+        pado.copy_bytes(ctx, dst.ptr, src.ptr, n)
+    ctx.sync()
+    assert np.array_equal(dst.numpy()[:4096], np.ones(4096, np.uint8))
+    log(f"copied {mib} MiB x 5 with k_copy_bytes")
 
 
 def load_traffic():
@@ -427,40 +402,41 @@ def main():
     idx.order(rank)
     qa, qb = pado.uniform_pairs(G.num_v, args.pairs, seed=1)
     query = None
-    try:
-        import torch
+    # Pairs in HBM as buffers of the library's own HIP runtime; one launch of
+    # the query kernel on the context's stream, bracketed by HIP events there.
+    from parallel_distance_oracle_amd.hip import DeviceArray, Events
 
-        da = torch.from_numpy(qa.view(np.int32)).to("cuda")
-        db = torch.from_numpy(qb.view(np.int32)).to("cuda")
-        out = torch.zeros(args.pairs, dtype=torch.uint8, device="cuda")
-        torch.cuda.synchronize()
-        ev = HipEvents()
-        e0, e1 = ev.event(), ev.event()
-        stream = ctx.stream
-        idx.query_device(da.data_ptr(), db.data_ptr(), args.pairs, out.data_ptr(), stream=stream)
-        ctx.sync()
-        qt = []
-        for _ in range(max(args.steps, 1)):
-            ev.record(e0, stream)
-            idx.query_device(da.data_ptr(), db.data_ptr(), args.pairs, out.data_ptr(), stream=stream)
-            ev.record(e1, stream)
-            qt.append(ev.elapsed_ms(e0, e1))
-        q_ms = float(np.mean(qt))
-        res = out.cpu().numpy()
-        query = {"pairs": args.pairs, "ms": round(q_ms, 3), "queries_per_s": round(args.pairs / (q_ms * 1e-3), 1)}
-        # Signed (exact) answers must equal the reference's except on its defect set.
-        k = min(100_000, args.pairs)
-        ref_mode = idx.query(qa[:k], qb[:k], ref_compat=True)
-        query["ref_compat_first_100k_match_reference"] = (meta is not None and k == meta["n_pairs"] and
-                                                          digest(ref_mode) == meta["answers_sha256"])
-        query["exact_vs_ref_compat_differ"] = int(np.count_nonzero(ref_mode != res[:k]))
-        lab_per_v = np.diff(loff.astype(np.int64))
-        qbytes = 8 + 1 + 2 * BP_RECORD_BYTES + 5 * (lab_per_v[rank[qa]] + lab_per_v[rank[qb]])
-        query["alg_bytes"] = int(qbytes.sum())
-        query["achieved_GBps"] = round(query["alg_bytes"] / (q_ms * 1e-3) / 1e9, 1)
-        log(f"query: {query}")
-    except ImportError:
-        log("torch missing: query leg skipped")
+    da, db = DeviceArray.from_numpy(qa), DeviceArray.from_numpy(qb)
+    out = DeviceArray(args.pairs, np.uint8)
+    ev = Events()
+    e0, e1 = ev.event(), ev.event()
+    stream = ctx.stream
+    idx.query_device(da.ptr, db.ptr, args.pairs, out.ptr, stream=stream)
+    ctx.sync()
+    qt = []
+    for _ in range(max(args.steps, 1)):
+        ev.record(e0, stream)
+        idx.query_device(da.ptr, db.ptr, args.pairs, out.ptr, stream=stream)
+        ev.record(e1, stream)
+        qt.append(ev.elapsed_ms(e0, e1))
+    ev.close()
+    q_ms = float(np.mean(qt))
+    res = out.numpy()
+    da.free()
+    db.free()
+    out.free()
+    query = {"pairs": args.pairs, "ms": round(q_ms, 3), "queries_per_s": round(args.pairs / (q_ms * 1e-3), 1)}
+    # Signed (exact) answers must equal the reference's except on its defect set.
+    k = min(100_000, args.pairs)
+    ref_mode = idx.query(qa[:k], qb[:k], ref_compat=True)
+    query["ref_compat_first_100k_match_reference"] = (meta is not None and k == meta["n_pairs"] and
+                                                      digest(ref_mode) == meta["answers_sha256"])
+    query["exact_vs_ref_compat_differ"] = int(np.count_nonzero(ref_mode != res[:k]))
+    lab_per_v = np.diff(loff.astype(np.int64))
+    qbytes = 8 + 1 + 2 * BP_RECORD_BYTES + 5 * (lab_per_v[rank[qa]] + lab_per_v[rank[qb]])
+    query["alg_bytes"] = int(qbytes.sum())
+    query["achieved_GBps"] = round(query["alg_bytes"] / (q_ms * 1e-3) / 1e9, 1)
+    log(f"query: {query}")
 
     serving = None
     if not args.no_serve:

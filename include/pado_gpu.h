@@ -105,6 +105,10 @@ void pado_gpu_ctx_destroy(pado_gpu_ctx *ctx);
 void *pado_gpu_ctx_stream(pado_gpu_ctx *ctx);
 /* Wait for all work queued on the context's stream. */
 pado_gpu_status pado_gpu_ctx_sync(pado_gpu_ctx *ctx);
+/* Device-to-device copy of `bytes` bytes with a plain copy kernel on the
+ * context's stream (16-byte aligned buffers; no reference counterpart: it is
+ * the known-size kernel the HBM byte counters are calibrated with). */
+pado_gpu_status pado_gpu_copy_bytes(pado_gpu_ctx *ctx, void *dst, const void *src, uint64_t bytes);
 
 /* Upload a rank-space CSR (the Graph after id_transfer: GR:31-92, 176-206).
  * off: n+1 entries, adj: m_dir entries (each undirected edge twice).

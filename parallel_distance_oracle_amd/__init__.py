@@ -112,6 +112,7 @@ def lib():
         "pado_gpu_ctx_destroy": (None, [_vp]),
         "pado_gpu_ctx_stream": (_vp, [_vp]),
         "pado_gpu_ctx_sync": (ctypes.c_int, [_vp]),
+        "pado_gpu_copy_bytes": (ctypes.c_int, [_vp, _vp, _vp, _u64]),
         "pado_gpu_graph_upload": (ctypes.c_int, [_vp, _u32, _u64, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
         "pado_gpu_graph_free": (None, [_vp]),
         "pado_gpu_graph_from_edges": (ctypes.c_int, [_vp, _vp, _u64, _vp, _vp, ctypes.c_int, ctypes.POINTER(_vp),
@@ -371,6 +372,11 @@ class Context:
             pass
 
 
+def copy_bytes(ctx, dst_ptr, src_ptr, nbytes):
+    """Plain device copy kernel on the context's stream (asynchronous)."""
+    _check(lib().pado_gpu_copy_bytes(ctx.h, _vp(dst_ptr), _vp(src_ptr), nbytes))
+
+
 _default_ctx = None
 
 
@@ -540,7 +546,8 @@ class Index:
         return out[: a.size]
 
     def query_device(self, a_ptr, b_ptr, k, out_ptr, ref_compat=False, rank_space=False, stream=None):
-        """Asynchronous query on device buffers (e.g. torch tensors' data_ptr());
+        """Asynchronous query on device buffers (device pointers of the library's
+        HIP runtime, e.g. hip.DeviceArray.ptr);
         out holds u8 (unweighted) or u32 (weighted) distances."""
         flags = QUERY_DEVICE_PTRS | (QUERY_REF_COMPAT if ref_compat else 0) | (QUERY_RANK_SPACE if rank_space else 0)
         fn = lib().pado_gpu_wquery if self.weighted else lib().pado_gpu_query
