@@ -165,10 +165,9 @@ def weighted_leg(ctx, scale, ef=16, duration_s=4.0, seed=7, dijkstra_sources=4):
     return res
 
 
-def serve_measure(ctx, idx, n, rank2id, duration_s, seed):
-    """Open-loop Poisson request stream against a resident ordered index."""
-    import threading
-
+def serve_measure(ctx, idx, n, rank2id, duration_s, seed, max_pairs_per_rate=120_000_000):
+    """Open-loop Poisson request stream against a resident ordered index,
+    driven by the server's native load generator (pado_gpu_serve_replay)."""
     rng = np.random.default_rng(seed)
     pool_n = 4000
     sizes = np.floor(np.exp(rng.uniform(0, np.log(65536), pool_n))).astype(np.int64)
@@ -178,86 +177,55 @@ def serve_measure(ctx, idx, n, rank2id, duration_s, seed):
     za = pado.zipf_vertices(n, tot, 1.1, seed + 1, rank2id)
     zb = pado.zipf_vertices(n, tot, 1.1, seed + 2, rank2id)
     offs = np.concatenate([[0], np.cumsum(sizes)])
-    reqs = []
-    for i in range(pool_n):
-        lo, hi = offs[i], offs[i + 1]
-        h = (hi - lo) // 2
-        a = np.concatenate([ua[lo:lo + h], za[lo + h:hi]]).astype(np.uint32)
-        b = np.concatenate([ub[lo:lo + h], zb[lo + h:hi]]).astype(np.uint32)
-        reqs.append((a, b))
+    # Request t: first half of its pairs uniform, second half Zipf over degree rank.
+    pos = np.arange(tot)
+    req_of = np.repeat(np.arange(pool_n), sizes)
+    zipf_half = (pos - offs[req_of]) >= (sizes[req_of] // 2)
+    allA = np.where(zipf_half, za, ua).astype(np.uint32)
+    allB = np.where(zipf_half, zb, ub).astype(np.uint32)
     mean_pairs = tot / pool_n
 
-    # Plain single-batch path on the same pairs (host arrays, one launch).
-    allA = np.concatenate([r[0] for r in reqs])
-    allB = np.concatenate([r[1] for r in reqs])
+    # Plain single-batch path on the same pairs (host arrays in and out, one launch).
     idx.query(allA[:1000], allB[:1000])
     t0 = time.perf_counter()
     plain = idx.query(allA, allB)
     plain_s = time.perf_counter() - t0
     plain_qps = allA.size / plain_s
 
-    srv = pado.Server(ctx, idx, staging_pairs=8 << 20, max_launch_pairs=2 << 20, streams=3)
-    # warm
-    r = srv.submit(reqs[0][0], reqs[0][1])
-    srv.wait(r, reqs[0][0].size)
-    # Saturation probe: closed loop with everything submitted at once.
+    srv = pado.Server(ctx, idx, staging_pairs=16 << 20, max_launch_pairs=4 << 20, streams=3)
+    srv.replay(allA[:offs[8]], allB[:offs[8]], offs[:9], np.zeros(8))  # warm
     rates = []
-    results = []
     ok_all = True
     for frac in (0.2, 0.4, 0.6, 0.8):
         rate_pairs = frac * plain_qps
         rate_req = rate_pairs / mean_pairs
-        gaps = rng.exponential(1.0 / rate_req, size=pool_n)
-        arrivals = np.cumsum(gaps)
-        arrivals = arrivals[arrivals < duration_s]
-        m = arrivals.size
-        lat = np.zeros(m)
-        ids = [None] * m
-        sub_t = np.zeros(m)
-        done_evt = threading.Event()
-
-        def waiter():
-            for i in range(m):
-                while ids[i] is None:
-                    time.sleep(0.0001)
-                out, l_ms = srv.wait(ids[i], reqs[i % pool_n][0].size)
-                lat[i] = (sub_t[i] - arrivals[i]) * 1e3 + l_ms
-                if not np.array_equal(out, plain[offs[i % pool_n]:offs[i % pool_n + 1]]):
-                    results.append(i)
-            done_evt.set()
-
-        th = threading.Thread(target=waiter)
-        th.start()
-        t_start = time.perf_counter()
-        for i in range(m):
-            due = t_start + arrivals[i]
-            while time.perf_counter() < due:
-                pass
-            a, b = reqs[i % pool_n]
-            while True:
-                try:
-                    rid = srv.submit(a, b)
-                    break
-                except pado.Busy:
-                    time.sleep(0.00005)
-            sub_t[i] = time.perf_counter() - t_start
-            ids[i] = rid
-        th.join(timeout=120)
-        wall = time.perf_counter() - t_start
-        pairs = float(sum(reqs[i % pool_n][0].size for i in range(m)))
+        dur = min(duration_s, max_pairs_per_rate / rate_pairs)
+        m = max(int(rate_req * dur), 16)
+        arrivals = np.cumsum(rng.exponential(1.0 / rate_req, size=m))
+        tmpl = np.arange(m) % pool_n
+        ksz = sizes[tmpl]
+        roff = np.concatenate([[0], np.cumsum(ksz)]).astype(np.uint64)
+        # request i replays template i % pool_n: its pairs follow in order
+        gather = np.tile(np.arange(tot), -(-m // pool_n))[: int(roff[-1])]
+        A, B = allA[gather], allB[gather]
+        lat, ans, wall = srv.replay(A, B, roff, arrivals)
+        ok = bool(np.array_equal(ans, plain[gather]))
+        ok_all &= ok
         rates.append({"offered_frac_of_plain": frac, "offered_req_per_s": round(rate_req, 1),
-                      "requests": int(m), "served_queries_per_s": round(pairs / wall, 1),
+                      "offered_queries_per_s": round(rate_pairs, 1), "requests": int(m),
+                      "pairs": int(roff[-1]), "window_s": round(float(arrivals[-1]), 4),
+                      "served_queries_per_s": round(float(roff[-1]) / wall, 1),
                       "p50_ms": round(float(np.percentile(lat, 50)), 3),
-                      "p99_ms": round(float(np.percentile(lat, 99)), 3)})
+                      "p99_ms": round(float(np.percentile(lat, 99)), 3), "answers_equal_plain": ok})
         log(f"serving {rates[-1]}")
     st = srv.stats()
     srv.stop()
-    ok_all = not results
     best = max(rates, key=lambda r: r["served_queries_per_s"])
     return {"plain_batch_queries_per_s": round(plain_qps, 1), "plain_batch_pairs": int(allA.size),
             "rates": rates, "best_served_queries_per_s": best["served_queries_per_s"],
             "p99_ms_at_best": best["p99_ms"], "results_equal_plain_batch": ok_all,
-            "launches": st["launches"], "requests": st["requests_done"], "rejected": st["rejected"]}
+            "launches": st["launches"], "requests": st["requests_done"], "rejected": st["rejected"],
+            "busy_ms": round(st["busy_ms"], 2)}
 
 
 def copy_calibrate(mib):

@@ -329,7 +329,10 @@ typedef struct {
     uint32_t max_requests;     /* request slots (0 = 4096) */
     uint32_t streams;          /* launches in flight (0 = 3) */
     uint32_t flags;            /* PADO_GPU_QUERY_REF_COMPAT | PADO_GPU_SERVE_GRID_STRIDE */
-    uint32_t reserved[5];
+    uint32_t min_launch_pairs; /* while launches are in flight, pending pairs that start a
+                                  launch at once (0 = 256Ki) */
+    uint32_t max_wait_us;      /* ... or the oldest pending request's wait that does (0 = 200) */
+    uint32_t reserved[3];
 } pado_gpu_serve_cfg;
 
 typedef struct {
@@ -356,6 +359,16 @@ pado_gpu_status pado_gpu_serve_poll(pado_gpu_server *srv, uint64_t req_id, int *
  * latency_ms (nullable) = completion time - submit time. */
 pado_gpu_status pado_gpu_serve_wait(pado_gpu_server *srv, uint64_t req_id, void *out, double *latency_ms);
 pado_gpu_status pado_gpu_serve_get_stats(pado_gpu_server *srv, pado_gpu_serve_stats *st);
+/* Open-loop load generator: request i (pairs [req_off[i], req_off[i+1]) of a
+ * and b) is submitted at arrival_s[i] seconds after the start, whatever
+ * happened to earlier requests (refused admissions are retried), and waited
+ * for in order.  latency_ms[i] = completion - arrival (admission waits
+ * included); answers (nullable) receive every request's answers at its
+ * offset (u8 or u32 per index type); wall_s = last completion - start.
+ * arrival_s must be non-decreasing. */
+pado_gpu_status pado_gpu_serve_replay(pado_gpu_server *srv, const uint32_t *a, const uint32_t *b,
+                                      const uint64_t *req_off, uint32_t num_req, const double *arrival_s,
+                                      double *latency_ms, void *answers, double *wall_s);
 /* Drains in-flight launches and joins the dispatcher. */
 void pado_gpu_serve_stop(pado_gpu_server *srv);
 

@@ -56,7 +56,8 @@ class BuildCfg(ctypes.Structure):
 class ServeCfg(ctypes.Structure):
     _fields_ = [("staging_pairs", ctypes.c_uint64), ("max_launch_pairs", ctypes.c_uint64),
                 ("max_requests", ctypes.c_uint32), ("streams", ctypes.c_uint32), ("flags", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 5)]
+                ("min_launch_pairs", ctypes.c_uint32), ("max_wait_us", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 3)]
 
 
 class ServeStats(ctypes.Structure):
@@ -144,6 +145,8 @@ def lib():
         "pado_gpu_serve_poll": (ctypes.c_int, [_vp, _u64, ctypes.POINTER(ctypes.c_int)]),
         "pado_gpu_serve_wait": (ctypes.c_int, [_vp, _u64, _vp, ctypes.POINTER(ctypes.c_double)]),
         "pado_gpu_serve_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ServeStats)]),
+        "pado_gpu_serve_replay": (ctypes.c_int, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp,
+                                                 ctypes.POINTER(ctypes.c_double)]),
         "pado_gpu_serve_stop": (None, [_vp]),
     }
     for name, (res, args) in sig.items():
@@ -641,8 +644,10 @@ class Server:
     """Continuous-batching query service over an ordered, resident Index."""
 
     def __init__(self, ctx, index, staging_pairs=0, max_launch_pairs=0, max_requests=0, streams=0,
-                 ref_compat=False, grid_stride=False):
+                 ref_compat=False, grid_stride=False, min_launch_pairs=0, max_wait_us=0):
         cfg = ServeCfg()
+        cfg.min_launch_pairs = min_launch_pairs
+        cfg.max_wait_us = max_wait_us
         cfg.staging_pairs = staging_pairs
         cfg.max_launch_pairs = max_launch_pairs
         cfg.max_requests = max_requests
@@ -677,6 +682,22 @@ class Server:
         lat = ctypes.c_double()
         _check(lib().pado_gpu_serve_wait(self.h, rid, _p(out), ctypes.byref(lat)))
         return out[:k], lat.value
+
+    def replay(self, a, b, req_off, arrival_s):
+        """Native open-loop load generator: request i = pairs req_off[i]:req_off[i+1]
+        arriving at arrival_s[i] s.  Returns (latency_ms per request from
+        arrival to completion, answers for all pairs, wall seconds)."""
+        a = _c(a, np.uint32).reshape(-1)
+        b = _c(b, np.uint32).reshape(-1)
+        off = _c(req_off, np.uint64).reshape(-1)
+        arr = _c(arrival_s, np.float64).reshape(-1)
+        nreq = off.size - 1
+        lat = np.zeros(max(nreq, 1), np.float64)
+        ans = np.zeros(max(int(off[-1]), 1), self.dtype)
+        wall = ctypes.c_double()
+        _check(lib().pado_gpu_serve_replay(self.h, _p(a), _p(b), _p(off), nreq, _p(arr), _p(lat), _p(ans),
+                                           ctypes.byref(wall)))
+        return lat[:nreq], ans[:int(off[-1])], wall.value
 
     def stats(self):
         s = ServeStats()

@@ -445,22 +445,42 @@ __global__ __launch_bounds__(256) void k_add(const uint8_t *used, const uint64_t
             }
             const bool pass = info != kEmpty && d + (info & 255u) <= iter;
             unsigned long long m = __ballot(pass);
+            // Up to four passing hubs per step: their row loads are issued
+            // together, and only lanes that still hold an unpruned candidate
+            // load their 16-byte slice of a row.
             while (m) {
-                const int src = __ffsll(m) - 1;
-                m &= m - 1;
-                const uint32_t hh = __shfl(h, src);
-                const uint32_t dd = __shfl(d, src);
-                const uint32_t slot = __shfl(info, src) >> 8;
-                ++n_rows;
-                const uint4 row = reinterpret_cast<const uint4 *>(dc + (uint64_t)slot * kBatch)[lane];
-                const uint32_t thr = iter - dd;  // prune root r if d_r(h) <= iter - d_v(h)
-                const uint32_t wds[4] = {row.x, row.y, row.z, row.w};
+                int src[4];
+                bool ok[4];
 #pragma unroll
-                for (int q = 0; q < 16; ++q) {
-                    const uint32_t x = (wds[q >> 2] >> ((q & 3) * 8)) & 255u;
-                    if (x <= thr && hh < my_lo + q) prune |= 1u << q;
+                for (int u = 0; u < 4; ++u) {
+                    ok[u] = m != 0ull;
+                    src[u] = ok[u] ? __ffsll(m) - 1 : 0;
+                    if (ok[u]) m &= m - 1;
+                }
+                const bool need = (cb & ~prune) != 0u;
+                uint32_t hh[4], thr[4];
+                uint4 row[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    hh[u] = __shfl(h, src[u]);
+                    thr[u] = iter - __shfl(d, src[u]);  // prune root r if d_r(h) <= iter - d_v(h)
+                    const uint32_t slot = __shfl(info, src[u]) >> 8;
+                    row[u] = make_uint4(~0u, ~0u, ~0u, ~0u);
+                    if (ok[u] && need) row[u] = reinterpret_cast<const uint4 *>(dc + (uint64_t)slot * kBatch)[lane];
+                    n_rows += ok[u] ? 1 : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t wds[4] = {row[u].x, row[u].y, row[u].z, row[u].w};
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) {
+                        const uint32_t x = (wds[q >> 2] >> ((q & 3) * 8)) & 255u;
+                        if (x <= thr[u] && hh[u] < my_lo + q) prune |= 1u << q;
+                    }
                 }
             }
+            // Every candidate of v pruned: the rest of its labels cannot matter.
+            if (__ballot((cb & ~prune) != 0u) == 0ull) break;
         }
         const uint32_t surv = cb & ~prune;
         n_dq += __popc(cb & prune);

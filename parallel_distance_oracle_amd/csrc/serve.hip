@@ -13,6 +13,7 @@
 // ones.  Every answer depends only on its own pair, so a request's result is
 // the same alone and in any mix.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -48,7 +49,6 @@ struct Request {
 
 struct Launch {
     std::vector<Request *> reqs;
-    hipEvent_t ev_start = nullptr, ev_end = nullptr;
     int stream = 0;
 };
 
@@ -69,6 +69,10 @@ struct pado_gpu_server {
     DevBuf<unsigned long long> work;  // one work counter per stream
     std::vector<hipStream_t> streams;
     std::vector<int> stream_busy;
+    std::vector<hipEvent_t> ev_start, ev_end;  // per stream, reused launch after launch
+    uint64_t pending_pairs = 0;     // pairs admitted but not launched
+    double max_wait_ms = 0.2;       // coalescing window while launches are in flight
+    uint64_t min_launch = 0;        // pairs that start a launch at once
     std::deque<Request *> live;     // ring order, not yet released
     std::deque<Request *> pending;  // admitted, not launched
     std::unordered_map<uint64_t, Request *> by_id;
@@ -105,16 +109,32 @@ void release_consumed(pado_gpu_server *s)
     if (s->live.empty()) s->head = s->tail;
 }
 
+// Continuous batching policy: with the device idle a launch starts at once
+// (lowest latency); while launches are in flight, pending requests are held
+// until they fill min_launch pairs or the oldest has waited max_wait_ms, so
+// that launches stay large enough to amortise their fixed cost.
 void dispatcher_loop(pado_gpu_server *s)
 {
     (void)hipSetDevice(s->ctx->device);
     const uint64_t max_launch = s->cfg.max_launch_pairs;
     for (;;) {
         std::unique_lock<std::mutex> lk(s->mu);
-        s->cv_work.wait(lk, [&] {
-            bool free_stream = std::find(s->stream_busy.begin(), s->stream_busy.end(), 0) != s->stream_busy.end();
-            return (s->stopping && s->pending.empty()) || (!s->pending.empty() && free_stream);
-        });
+        for (;;) {
+            if (s->pending.empty()) {
+                if (s->stopping) break;
+                s->cv_work.wait(lk);
+                continue;
+            }
+            const size_t busy = (size_t)std::count(s->stream_busy.begin(), s->stream_busy.end(), 1);
+            if (busy == s->stream_busy.size()) {
+                s->cv_work.wait(lk);
+                continue;
+            }
+            if (busy == 0 || s->stopping || s->pending_pairs >= s->min_launch) break;
+            const double age = now_ms() - s->pending.front()->t_submit;
+            if (age >= s->max_wait_ms) break;
+            s->cv_work.wait_for(lk, std::chrono::microseconds((int64_t)((s->max_wait_ms - age) * 1000.0) + 1));
+        }
         if (s->stopping && s->pending.empty()) break;
         int si = (int)(std::find(s->stream_busy.begin(), s->stream_busy.end(), 0) - s->stream_busy.begin());
         // Pack consecutive pending requests that are physically contiguous.
@@ -129,6 +149,7 @@ void dispatcher_loop(pado_gpu_server *s)
             L.reqs.push_back(r);
             r->state = 1;
             p1 = r->start + r->k;
+            s->pending_pairs -= r->k;
             s->pending.pop_front();
         }
         s->stream_busy[si] = 1;
@@ -136,9 +157,7 @@ void dispatcher_loop(pado_gpu_server *s)
 
         const uint64_t phys = p0 % s->cap, k = p1 - p0;
         hipStream_t st = s->streams[si];
-        (void)hipEventCreate(&L.ev_start);
-        (void)hipEventCreate(&L.ev_end);
-        (void)hipEventRecord(L.ev_start, st);
+        (void)hipEventRecord(s->ev_start[si], st);
         if (k) {
             (void)hipMemcpyAsync(s->d_a.p + phys, s->h_a + phys, k * 4, hipMemcpyHostToDevice, st);
             (void)hipMemcpyAsync(s->d_b.p + phys, s->h_b + phys, k * 4, hipMemcpyHostToDevice, st);
@@ -154,7 +173,7 @@ void dispatcher_loop(pado_gpu_server *s)
             (void)hipMemcpyAsync(s->h_out + phys * s->es, s->d_out.p + phys * s->es, k * s->es,
                                  hipMemcpyDeviceToHost, st);
         }
-        (void)hipEventRecord(L.ev_end, st);
+        (void)hipEventRecord(s->ev_end[si], st);
         lk.lock();
         s->inflight.push_back(std::move(L));
         s->st.launches += 1;
@@ -179,7 +198,8 @@ void completer_loop(pado_gpu_server *s)
             }
             continue;
         }
-        hipEvent_t ev0 = s->inflight.front().ev_start, ev1 = s->inflight.front().ev_end;
+        const int si = s->inflight.front().stream;
+        hipEvent_t ev0 = s->ev_start[si], ev1 = s->ev_end[si];
         lk.unlock();
         (void)hipEventSynchronize(ev1);
         float ms = 0.f;
@@ -197,11 +217,34 @@ void completer_loop(pado_gpu_server *s)
         s->st.busy_ms += ms;
         s->stream_busy[L.stream] = 0;
         lk.unlock();
-        (void)hipEventDestroy(ev0);
-        (void)hipEventDestroy(ev1);
         s->cv_done.notify_all();
         s->cv_work.notify_all();
     }
+}
+
+// Blocks until request `id` is done, copies its answers, releases its slot.
+pado_gpu_status wait_request(pado_gpu_server *s, uint64_t id, void *out, double *t_submit, double *t_done)
+{
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->by_id.find(id);
+    if (it == s->by_id.end() || it->second->state == 3) {
+        set_error("unknown request id");
+        return PADO_GPU_ERR_ARG;
+    }
+    Request *r = it->second;
+    s->cv_done.wait(lk, [&] { return r->state >= 2; });
+    if (!s->error.empty()) {
+        set_error(s->error);
+        return PADO_GPU_ERR_HIP;
+    }
+    if (r->k && out) std::memcpy(out, s->h_out + (r->start % s->cap) * s->es, r->k * s->es);
+    *t_submit = r->t_submit;
+    *t_done = r->t_done;
+    r->state = 3;
+    release_consumed(s);
+    lk.unlock();
+    s->cv_work.notify_all();
+    return PADO_GPU_OK;
 }
 
 }  // namespace
@@ -248,6 +291,8 @@ pado_gpu_status pado_gpu_serve_start(pado_gpu_ctx *ctx, pado_gpu_index *idx, con
         if (s->h_b) (void)hipHostFree(s->h_b);
         if (s->h_out) (void)hipHostFree(s->h_out);
         for (auto q : s->streams) (void)hipStreamDestroy(q);
+        for (auto e : s->ev_start) (void)hipEventDestroy(e);
+        for (auto e : s->ev_end) (void)hipEventDestroy(e);
         delete s;
         return code;
     };
@@ -269,7 +314,16 @@ pado_gpu_status pado_gpu_serve_start(pado_gpu_ctx *ctx, pado_gpu_index *idx, con
         }
         s->streams.push_back(q);
         s->stream_busy.push_back(0);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+            set_error("event creation failed");
+            return fail(PADO_GPU_ERR_HIP);
+        }
+        s->ev_start.push_back(e0);
+        s->ev_end.push_back(e1);
     }
+    s->min_launch = s->cfg.min_launch_pairs ? s->cfg.min_launch_pairs : (256u << 10);
+    s->max_wait_ms = (s->cfg.max_wait_us ? s->cfg.max_wait_us : 200u) * 1e-3;
     s->dispatcher = std::thread(dispatcher_loop, s);
     s->completer = std::thread(completer_loop, s);
     *out = s;
@@ -313,6 +367,7 @@ pado_gpu_status pado_gpu_serve_submit(pado_gpu_server *s, const uint32_t *a, con
     s->live.push_back(r);
     s->by_id[r->id] = r;
     s->pending.push_back(r);
+    s->pending_pairs += k;
     *req_id = r->id;
     lk.unlock();
     s->cv_work.notify_one();
@@ -336,25 +391,78 @@ pado_gpu_status pado_gpu_serve_wait(pado_gpu_server *s, uint64_t req_id, void *o
 {
     clear_error();
     if (!s) return PADO_GPU_ERR_ARG;
-    std::unique_lock<std::mutex> lk(s->mu);
-    auto it = s->by_id.find(req_id);
-    if (it == s->by_id.end() || it->second->state == 3) {
-        set_error("unknown request id");
+    double t_submit = 0, t_done = 0;
+    PADO_TRY(wait_request(s, req_id, out, &t_submit, &t_done));
+    if (latency_ms) *latency_ms = t_done - t_submit;
+    return PADO_GPU_OK;
+}
+
+pado_gpu_status pado_gpu_serve_replay(pado_gpu_server *s, const uint32_t *a, const uint32_t *b,
+                                      const uint64_t *req_off, uint32_t num_req, const double *arrival_s,
+                                      double *latency_ms, void *answers, double *wall_s)
+{
+    clear_error();
+    if (!s || !req_off || !arrival_s || !latency_ms || (req_off[num_req] && (!a || !b))) {
+        set_error("null argument");
         return PADO_GPU_ERR_ARG;
     }
-    Request *r = it->second;
-    s->cv_done.wait(lk, [&] { return r->state >= 2; });
-    if (!s->error.empty()) {
-        set_error(s->error);
-        return PADO_GPU_ERR_HIP;
+    for (uint32_t i = 0; i < num_req; ++i)
+        if (req_off[i + 1] < req_off[i] || (i && arrival_s[i] < arrival_s[i - 1])) {
+            set_error("request offsets and arrival times must be non-decreasing");
+            return PADO_GPU_ERR_ARG;
+        }
+    std::vector<std::atomic<uint64_t>> ids(num_req);
+    for (auto &x : ids) x.store(0);
+    std::atomic<int> failed{0};
+    std::string sub_error;
+    const double t0 = now_ms() + 1.0;
+    // Open-loop submitter: request i enters at t0 + arrival_s[i] whatever
+    // happened to earlier requests; a refused admission is retried.
+    std::thread submitter([&] {
+        for (uint32_t i = 0; i < num_req; ++i) {
+            const double due = t0 + arrival_s[i] * 1e3;
+            while (now_ms() < due) {
+                if (due - now_ms() > 0.2) std::this_thread::yield();
+            }
+            const uint64_t k = req_off[i + 1] - req_off[i];
+            uint64_t id = 0;
+            for (;;) {
+                pado_gpu_status st = pado_gpu_serve_submit(s, a + req_off[i], b + req_off[i], k, &id);
+                if (st == PADO_GPU_OK) break;
+                if (st != PADO_GPU_ERR_BUSY) {
+                    sub_error = pado_gpu_last_error();
+                    failed.store(1);
+                    return;
+                }
+                std::this_thread::yield();
+            }
+            ids[i].store(id);
+        }
+    });
+    pado_gpu_status result = PADO_GPU_OK;
+    double last_done = t0;
+    for (uint32_t i = 0; i < num_req; ++i) {
+        uint64_t id = 0;
+        while ((id = ids[i].load()) == 0 && !failed.load()) std::this_thread::yield();
+        if (id == 0) break;
+        double t_submit = 0, t_done = 0;
+        uint8_t *dst = answers ? (uint8_t *)answers + req_off[i] * s->es : nullptr;
+        pado_gpu_status st = wait_request(s, id, dst, &t_submit, &t_done);
+        if (st != PADO_GPU_OK) {
+            result = st;
+            failed.store(1);
+            break;
+        }
+        latency_ms[i] = t_done - (t0 + arrival_s[i] * 1e3);
+        if (t_done > last_done) last_done = t_done;
     }
-    if (r->k && out) std::memcpy(out, s->h_out + (r->start % s->cap) * s->es, r->k * s->es);
-    if (latency_ms) *latency_ms = r->t_done - r->t_submit;
-    r->state = 3;
-    release_consumed(s);
-    lk.unlock();
-    s->cv_work.notify_all();
-    return PADO_GPU_OK;
+    submitter.join();
+    if (failed.load() && result == PADO_GPU_OK) {
+        set_error(sub_error.empty() ? std::string("replay failed") : sub_error);
+        result = PADO_GPU_ERR_HIP;
+    }
+    if (wall_s) *wall_s = (last_done - t0) * 1e-3;
+    return result;
 }
 
 pado_gpu_status pado_gpu_serve_get_stats(pado_gpu_server *s, pado_gpu_serve_stats *st)
@@ -381,6 +489,8 @@ void pado_gpu_serve_stop(pado_gpu_server *s)
         (void)hipStreamSynchronize(q);
         (void)hipStreamDestroy(q);
     }
+    for (auto e : s->ev_start) (void)hipEventDestroy(e);
+    for (auto e : s->ev_end) (void)hipEventDestroy(e);
     for (Request *r : s->live) delete r;
     (void)hipHostFree(s->h_a);
     (void)hipHostFree(s->h_b);

@@ -147,3 +147,23 @@ def test_weighted_index_served(gpu_ctx):
     d = oracle.dijkstra(G.num_v, G.off, G.adj, G.w, src)
     sel = a == a[0]
     assert np.array_equal(outs[3][sel], d[rank[b[sel]]])
+
+
+def test_replay_load_generator(gpu_ctx, served):
+    """The native open-loop generator: every request's answers equal the plain
+    batch query; latencies are non-negative; arrivals spread over time."""
+    G, idx, r2i = served
+    reqs = _requests(G.num_v, r2i, 300, seed=31)
+    A = np.concatenate([a for a, _ in reqs]).astype(np.uint32)
+    B = np.concatenate([b for _, b in reqs]).astype(np.uint32)
+    off = np.concatenate([[0], np.cumsum([a.size for a, _ in reqs])]).astype(np.uint64)
+    arrivals = np.cumsum(np.random.default_rng(3).exponential(2e-4, size=len(reqs)))
+    srv = pado.Server(gpu_ctx, idx, staging_pairs=1 << 21, streams=3, min_launch_pairs=1 << 16, max_wait_us=100)
+    try:
+        lat, ans, wall = srv.replay(A, B, off, arrivals)
+        st = srv.stats()
+    finally:
+        srv.stop()
+    assert np.array_equal(ans, idx.query(A, B))
+    assert np.all(lat >= 0) and wall >= arrivals[-1] * 0.5
+    assert st["requests_done"] == len(reqs)
