@@ -2,8 +2,10 @@
  * pado_gpu.h -- C ABI of the MI355X-native PADO library (libpado_gpu.so).
  *
  * The library builds PADO's unweighted pruned landmark labeling (50
- * bit-parallel roots + 1024-root batched pruned BFS) on one AMD Instinct
- * MI355X and answers exact hop-distance queries from it.  Plain C types only;
+ * bit-parallel roots + 1024-root batched pruned BFS) and its weighted variant
+ * (512-root batches) on one AMD Instinct MI355X, answers exact distance
+ * queries from them (u8 hops / u32 weighted), and serves continuous query
+ * streams over a resident index.  Plain C types only;
  * no exceptions or exit() cross this boundary.  Device memory is owned by the
  * opaque handles; host buffers are owned by the caller.
  *

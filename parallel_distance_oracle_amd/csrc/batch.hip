@@ -463,7 +463,11 @@ __global__ __launch_bounds__(256) void k_add(const uint8_t *used, const uint64_t
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     hh[u] = __shfl(h, src[u]);
-                    thr[u] = iter - __shfl(d, src[u]);  // prune root r if d_r(h) <= iter - d_v(h)
+                    // prune root r if d_r(h) <= iter - d_v(h); an unused slot u
+                    // (fewer than four hubs left) gets threshold 0, which the
+                    // all-255 row never meets
+                    const uint32_t du = __shfl(d, src[u]);
+                    thr[u] = ok[u] ? iter - du : 0u;
                     const uint32_t slot = __shfl(info, src[u]) >> 8;
                     row[u] = make_uint4(~0u, ~0u, ~0u, ~0u);
                     if (ok[u] && need) row[u] = reinterpret_cast<const uint4 *>(dc + (uint64_t)slot * kBatch)[lane];
