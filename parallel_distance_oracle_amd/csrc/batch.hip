@@ -85,22 +85,32 @@ __device__ inline uint32_t push_degree(const uint64_t *off, const uint32_t *adj,
 }
 
 // Bit-parallel check of PARA:1164-1186 with the reference's unsigned
-// arithmetic; true = pruned.
+// arithmetic; true = pruned.  Per slot the reference tests
+// `td - 2 <= iter` (unsigned, so td < 2 never passes), subtracts 2 or 1 by
+// the S^-1 / S^0 intersections and prunes if the result is <= iter.  As the
+// sets can only lower td, that is: prune if 2 <= td <= iter; prune if
+// td == iter + 1 and S^-1_r meets S^-1_t or S^0_t (or S^0_r meets S^-1_t);
+// prune if td == iter + 2 and S^-1_r meets S^-1_t.  The slots are an OR, so
+// the distance-only test runs first over all 50 slots (no memory traffic)
+// and the 16-byte set pairs are loaded only for slots at iter + 1 / iter + 2.
 __device__ inline bool bp_prunes(const uint32_t (&dr)[13], const uint32_t (&dt)[13], const ulonglong2 *bps,
                                  uint32_t r, uint32_t t, uint32_t iter)
 {
+    unsigned long long sets = 0;  // slots whose verdict needs the sets
 #pragma unroll
     for (int i = 0; i < (int)kBP; ++i) {
-        uint32_t td = ((dr[i >> 2] >> ((i & 3) * 8)) & 255u) + ((dt[i >> 2] >> ((i & 3) * 8)) & 255u);
-        if (td - 2u <= iter) {
-            ulonglong2 sr = bps[(uint64_t)r * kBP + i];
-            ulonglong2 st = bps[(uint64_t)t * kBP + i];
-            if (sr.x & st.x)
-                td -= 2u;
-            else if ((sr.x & st.y) | (sr.y & st.x))
-                td -= 1u;
-            if (td <= iter) return true;
-        }
+        const uint32_t td = ((dr[i >> 2] >> ((i & 3) * 8)) & 255u) + ((dt[i >> 2] >> ((i & 3) * 8)) & 255u);
+        if (td >= 2u && td <= iter) return true;
+        if (td - iter - 1u <= 1u) sets |= 1ull << i;  // td == iter + 1 or iter + 2
+    }
+    while (sets) {
+        const int i = __ffsll(sets) - 1;
+        sets &= sets - 1;
+        const uint32_t td = ((dr[i >> 2] >> ((i & 3) * 8)) & 255u) + ((dt[i >> 2] >> ((i & 3) * 8)) & 255u);
+        const ulonglong2 sr = bps[(uint64_t)r * kBP + i];
+        const ulonglong2 st = bps[(uint64_t)t * kBP + i];
+        if (sr.x & st.x) return true;
+        if (td == iter + 1u && ((sr.x & st.y) | (sr.y & st.x))) return true;
     }
     return false;
 }
