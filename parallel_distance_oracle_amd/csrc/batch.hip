@@ -96,20 +96,36 @@ __device__ inline uint32_t push_degree(const uint64_t *off, const uint32_t *adj,
 __device__ inline bool bp_prunes(const uint32_t (&dr)[13], const uint32_t (&dt)[13], const ulonglong2 *bps,
                                  uint32_t r, uint32_t t, uint32_t iter)
 {
+    // Four slots per 32-bit word, as 16-bit lanes: lo = slots 4w and 4w+2,
+    // hi = slots 4w+1 and 4w+3 (td <= 510).  (0x8000 | c) - td keeps bit 15
+    // of a lane set iff td <= c, and no borrow crosses lanes.  Bytes 50..63
+    // of the distance line are padding: slots 50 and 51 are masked off.
+    const uint32_t c_it = 0x80008000u | (iter * 0x00010001u);
+    const uint32_t c_it2 = 0x80008000u | ((iter + 2u) * 0x00010001u);
+    const uint32_t c_1 = 0x80018001u;
     unsigned long long sets = 0;  // slots whose verdict needs the sets
 #pragma unroll
-    for (int i = 0; i < (int)kBP; ++i) {
-        const uint32_t td = ((dr[i >> 2] >> ((i & 3) * 8)) & 255u) + ((dt[i >> 2] >> ((i & 3) * 8)) & 255u);
-        if (td >= 2u && td <= iter) return true;
-        if (td - iter - 1u <= 1u) sets |= 1ull << i;  // td == iter + 1 or iter + 2
+    for (int w = 0; w < 13; ++w) {
+        const uint32_t valid = w == 12 ? 0x00008000u : 0x80008000u;
+        const uint32_t lo = (dr[w] & 0x00FF00FFu) + (dt[w] & 0x00FF00FFu);
+        const uint32_t hi = ((dr[w] >> 8) & 0x00FF00FFu) + ((dt[w] >> 8) & 0x00FF00FFu);
+        const uint32_t lo_it = c_it - lo, hi_it = c_it - hi;
+        if (((lo_it & ~(c_1 - lo)) | (hi_it & ~(c_1 - hi))) & valid) return true;  // 2 <= td <= iter
+        const uint32_t lo_n = (c_it2 - lo) & ~lo_it & valid, hi_n = (c_it2 - hi) & ~hi_it & valid;
+        const uint32_t nib = ((lo_n >> 15) & 1u) | ((hi_n >> 14) & 2u) | ((lo_n >> 29) & 4u) | ((hi_n >> 28) & 8u);
+        sets |= (unsigned long long)nib << (4 * w);  // td == iter + 1 or iter + 2
     }
     while (sets) {
         const int i = __ffsll(sets) - 1;
         sets &= sets - 1;
-        const uint32_t td = ((dr[i >> 2] >> ((i & 3) * 8)) & 255u) + ((dt[i >> 2] >> ((i & 3) * 8)) & 255u);
         const ulonglong2 sr = bps[(uint64_t)r * kBP + i];
         const ulonglong2 st = bps[(uint64_t)t * kBP + i];
         if (sr.x & st.x) return true;
+        const uint32_t wi = i >> 2, sh = (i & 3) * 8;
+        uint32_t td = 0;
+#pragma unroll
+        for (int w = 0; w < 13; ++w)
+            if (w == (int)wi) td = ((dr[w] >> sh) & 255u) + ((dt[w] >> sh) & 255u);
         if (td == iter + 1u && ((sr.x & st.y) | (sr.y & st.x))) return true;
     }
     return false;
@@ -288,8 +304,14 @@ __global__ __launch_bounds__(256) void k_push(const uint64_t *off, const uint32_
 {
     const uint32_t lane = threadIdx.x & (kChunk - 1);
     const uint32_t ngroups = (gridDim.x * blockDim.x) / kChunk;
-    const uint32_t nc = C->n_chunk;  // 0 once the batch's frontier is empty
-    const uint32_t iter = C->iter, rs = C->rs, gen_batch = C->gen_batch, gen_iter = C->gen_iter;
+    // This launch is iteration C->iter + 1 of the batch if the previous one
+    // left active vertices (batch_process's loop, PARA:1478-1670); iteration
+    // 255 would leave the reference's u8 domain and is not run.
+    const uint32_t n_act0 = C->n_act;
+    const uint32_t iter = C->iter + 1, gen_iter = C->gen_iter + 1;
+    const bool live = n_act0 != 0 && iter < kInf8;
+    const uint32_t nc = live ? C->n_chunk : 0u;
+    const uint32_t rs = C->rs, gen_batch = C->gen_batch;
     unsigned long long n_push = 0, n_pass = 0, n_bp = 0, n_cand = 0, n_edge = 0, n_tail = 0;
     for (uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) / kChunk; c < nc; c += ngroups) {
         const unsigned long long desc = chunks[c];
@@ -360,15 +382,29 @@ __global__ __launch_bounds__(256) void k_push(const uint64_t *off, const uint32_
         if (n_bp) atomicAdd(&C->bp_pruned, n_bp);
         if (n_cand) atomicAdd(&C->cands, n_cand);
     }
+    // The last block to finish closes the iteration: it advances the
+    // iteration counters and empties the frontier for k_add to refill.  Every
+    // block has read the counters it needs before it counts itself done.
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&C->blocks_done, 1u) == gridDim.x - 1) {
+            C->blocks_done = 0;
+            if (n_act0 != 0) {
+                if (iter >= kInf8) {
+                    C->domain = 1;
+                } else {
+                    C->iter = iter;
+                    C->gen_iter = gen_iter;
+                    C->iterations += 1;
+                    C->actives += n_act0;
+                }
+            }
+            C->n_act = 0;
+            C->n_chunk = 0;
+        }
+    }
 }
-
-__global__ void k_mid(BuildCounters *C)
-{
-    C->n_act = 0;
-    C->n_chunk = 0;
-}
-
-__global__ void k_post(BuildCounters *C) { C->n_cand = 0; }
 
 // Start of a batch (construct's loop over batches, PARA:1791-1812): roots
 // [b * batch, ...) with b taken from the device-side batch counter.
@@ -381,36 +417,11 @@ __global__ void k_batch_begin(BuildCounters *C, uint32_t batch, uint32_t n)
     C->rsz = rs < n ? min(batch, n - rs) : 0u;
     C->gen_batch = b + 1;
     C->iter = 0;
-    C->live = 0;
     C->n_act = 0;
     C->n_chunk = 0;
     C->n_cand = 0;
     C->n_touched = 0;
     C->n_hub = 0;
-}
-
-// Start of an iteration of batch_process's loop (PARA:1478-1670): live while
-// the previous iteration produced active vertices.  Iteration 255 would leave
-// the reference's u8 domain: flag it and end the batch.
-__global__ void k_iter_begin(BuildCounters *C)
-{
-    if (C->n_act == 0) {
-        C->live = 0;
-        return;
-    }
-    const uint32_t it = C->iter + 1;
-    if (it >= kInf8) {
-        C->domain = 1;
-        C->live = 0;
-        C->n_act = 0;
-        C->n_chunk = 0;
-        return;
-    }
-    C->iter = it;
-    C->live = 1;
-    C->gen_iter += 1;
-    C->iterations += 1;
-    C->actives += C->n_act;
 }
 
 // add (PARA:1568-1665): one wave per candidate vertex v.  Lane l owns roots
@@ -552,6 +563,15 @@ __global__ __launch_bounds__(256) void k_add(const uint8_t *used, const uint64_t
         if (n_dq) atomicAdd(&C->dq_pruned, n_dq);
         if (n_lab) atomicAdd(&C->labels, n_lab);
         if (n_rows) atomicAdd(&C->dq_rows, (uint32_t)n_rows);
+    }
+    // The last block to finish empties the candidate list of this iteration.
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&C->blocks_done, 1u) == gridDim.x - 1) {
+            C->blocks_done = 0;
+            C->n_cand = 0;
+        }
     }
 }
 
@@ -738,18 +758,15 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     };
     auto enqueue_iterations = [&](uint32_t count, bool timed) -> pado_gpu_status {
         for (uint32_t u = 0; u < count; ++u) {
-            hipLaunchKernelGGL(k_iter_begin, dim3(1), dim3(1), 0, st, ctr.p);
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u], st));
             hipLaunchKernelGGL(k_push, dim3(push_grid), dim3(256), 0, st, g->off.p, g->adj.p, alist.p, ag.p, amask.p,
                                aend.p, chunks.p, ind.p, cand.p, tmark.p, tlist.p, cmark.p, clist.p, imark.p,
                                idx->bpd.p, (const ulonglong2 *)idx->bps.p, ctr.p);
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 1], st));
-            hipLaunchKernelGGL(k_mid, dim3(1), dim3(1), 0, st, ctr.p);
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 2], st));
             hipLaunchKernelGGL(k_add, dim3(add_grid), dim3(256), 0, st, d_used.p, g->off.p, g->adj.p, clist.p, cand.p,
                                S, hinfo.p, dc.p, alist.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 3], st));
-            hipLaunchKernelGGL(k_post, dim3(1), dim3(1), 0, st, ctr.p);
         }
         return PADO_GPU_OK;
     };

@@ -98,7 +98,7 @@ struct BuildCounters {
     uint32_t next_batch;          // batch index the next k_batch_begin starts
     uint32_t rs, rsz, gen_batch;  // current batch: roots [rs, rs + rsz)
     uint32_t iter;                // current iteration of the batch (1-based)
-    uint32_t live;                // 1 while the current iteration has work
+    uint32_t blocks_done;         // blocks of the running launch that finished
     uint32_t gen_iter;            // global iteration generation (marks)
     uint32_t domain;              // an iteration reached 255 (u8 domain)
     unsigned long long iterations;  // live iterations, summed over batches
