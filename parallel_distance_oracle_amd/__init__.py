@@ -19,7 +19,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpado_gpu.so")
+# $PADO_GPU_LIB selects another build of the library (e.g. a variant being
+# timed); by default the in-tree build is loaded.
+LIB_PATH = os.environ.get("PADO_GPU_LIB") or os.path.join(HERE, "libpado_gpu.so")
 
 BP_SIZE = 50
 BATCH_SIZE = 1024

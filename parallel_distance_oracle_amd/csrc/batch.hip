@@ -35,6 +35,7 @@
 // k_add (wave per candidate vertex: distance query + label append + next
 // active entry).  The host loop syncs once per iteration on the active count.
 #include <functional>
+#include <map>
 
 #include "internal.hpp"
 
@@ -103,7 +104,6 @@ __device__ inline bool bp_prunes(const uint32_t (&dr)[13], const uint32_t (&dt)[
     const uint32_t c_it = 0x80008000u | (iter * 0x00010001u);
     const uint32_t c_it2 = 0x80008000u | ((iter + 2u) * 0x00010001u);
     const uint32_t c_1 = 0x80018001u;
-    unsigned long long sets = 0;  // slots whose verdict needs the sets
 #pragma unroll
     for (int w = 0; w < 13; ++w) {
         const uint32_t valid = w == 12 ? 0x00008000u : 0x80008000u;
@@ -112,21 +112,19 @@ __device__ inline bool bp_prunes(const uint32_t (&dr)[13], const uint32_t (&dt)[
         const uint32_t lo_it = c_it - lo, hi_it = c_it - hi;
         if (((lo_it & ~(c_1 - lo)) | (hi_it & ~(c_1 - hi))) & valid) return true;  // 2 <= td <= iter
         const uint32_t lo_n = (c_it2 - lo) & ~lo_it & valid, hi_n = (c_it2 - hi) & ~hi_it & valid;
-        const uint32_t nib = ((lo_n >> 15) & 1u) | ((hi_n >> 14) & 2u) | ((lo_n >> 29) & 4u) | ((hi_n >> 28) & 8u);
-        sets |= (unsigned long long)nib << (4 * w);  // td == iter + 1 or iter + 2
-    }
-    while (sets) {
-        const int i = __ffsll(sets) - 1;
-        sets &= sets - 1;
-        const ulonglong2 sr = bps[(uint64_t)r * kBP + i];
-        const ulonglong2 st = bps[(uint64_t)t * kBP + i];
-        if (sr.x & st.x) return true;
-        const uint32_t wi = i >> 2, sh = (i & 3) * 8;
-        uint32_t td = 0;
-#pragma unroll
-        for (int w = 0; w < 13; ++w)
-            if (w == (int)wi) td = ((dr[w] >> sh) & 255u) + ((dt[w] >> sh) & 255u);
-        if (td == iter + 1u && ((sr.x & st.y) | (sr.y & st.x))) return true;
+        const uint32_t lo_1 = (c_it + 0x00010001u - lo) & ~lo_it, hi_1 = (c_it + 0x00010001u - hi) & ~hi_it;
+        uint32_t nib = ((lo_n >> 15) & 1u) | ((hi_n >> 14) & 2u) | ((lo_n >> 29) & 4u) | ((hi_n >> 28) & 8u);
+        const uint32_t one = ((lo_1 >> 15) & 1u) | ((hi_1 >> 14) & 2u) | ((lo_1 >> 29) & 4u) | ((hi_1 >> 28) & 8u);
+        // td == iter + 1 (bit in `one`) or iter + 2: the sets decide (rare)
+        while (nib) {
+            const uint32_t q = __ffs(nib) - 1;
+            nib &= nib - 1;
+            const uint32_t i = 4 * w + q;
+            const ulonglong2 sr = bps[(uint64_t)r * kBP + i];
+            const ulonglong2 st = bps[(uint64_t)t * kBP + i];
+            if (sr.x & st.x) return true;
+            if (((one >> q) & 1u) && ((sr.x & st.y) | (sr.y & st.x))) return true;
+        }
     }
     return false;
 }
@@ -179,13 +177,13 @@ __device__ inline unsigned long long store_reserve(LabelStore S, BuildCounters *
 
 // Append the pushes of active index k: chunk descriptors over its first
 // `pdeg` neighbours (wave-uniform call).
-__device__ inline void emit_push_chunks(uint32_t k, uint32_t pdeg, BuildCounters *C, unsigned long long *chunks)
+__device__ inline void emit_push_chunks(uint32_t k, uint32_t pdeg, uint32_t *n_chunk, unsigned long long *chunks)
 {
     const uint32_t lane = lane_id();
     uint32_t nch = (pdeg + kChunk - 1) / kChunk;
     if (nch == 0) return;
     uint32_t c0 = 0;
-    if (lane == 0) c0 = atomicAdd(&C->n_chunk, nch);
+    if (lane == 0) c0 = atomicAdd(n_chunk, nch);
     c0 = __shfl(c0, 0);
     for (uint32_t q = lane; q < nch; q += 64) chunks[c0 + q] = ((unsigned long long)(q * kChunk) << 32) | k;
 }
@@ -226,8 +224,9 @@ __global__ __launch_bounds__(256) void k_batch_init(const uint8_t *used, const u
         if (atomicExch(&tmark[r], gen_batch) != gen_batch) tlist[atomicAdd(&C->n_touched, 1u)] = r;
         atomicAdd(&C->labels, 1ull);
     }
+    // The frontier of iteration i is counted in slot i & 1: iteration 1 -> 1.
     uint32_t k = 0;
-    if (lane == 0) k = atomicAdd(&C->n_act, 1u);
+    if (lane == 0) k = atomicAdd(&C->n_act[1], 1u);
     k = __shfl(k, 0);
     if (lane < kWords) ag[(uint64_t)k * kWords + lane] = (lane == (ro >> 6)) ? (1ull << (ro & 63)) : 0ull;
     uint32_t pdeg = push_degree(off, adj, r, rs);
@@ -236,7 +235,7 @@ __global__ __launch_bounds__(256) void k_batch_init(const uint8_t *used, const u
         amask[k] = 1u << (ro >> 6);
         aend[k] = pdeg;
     }
-    emit_push_chunks(k, pdeg, C, chunks);
+    emit_push_chunks(k, pdeg, &C->n_chunk[1], chunks);
 }
 
 // Distinct hubs over the labels of the batch roots (for the root-distance table).
@@ -295,23 +294,39 @@ __global__ __launch_bounds__(256) void k_hub_fill(const uint8_t *used, LabelStor
 
 // push (PARA:1486-1559 / push_labels PARA:1084-1211), one quarter-wave per
 // 16-edge chunk of an active vertex's tails >= roots_start.
-__global__ __launch_bounds__(256) void k_push(const uint64_t *off, const uint32_t *adj, const uint32_t *alist,
-                                              const unsigned long long *ag, const uint32_t *amask,
-                                              const uint32_t *aend, const unsigned long long *chunks,
-                                              unsigned long long *ind, unsigned long long *cand, uint32_t *tmark,
-                                              uint32_t *tlist, uint32_t *cmark, uint32_t *clist, uint32_t *imark,
+__global__ __launch_bounds__(256) void k_push(uint32_t iter, const uint64_t *off, const uint32_t *adj,
+                                              const uint32_t *alist, const unsigned long long *ag,
+                                              const uint32_t *amask, const uint32_t *aend,
+                                              const unsigned long long *chunks, unsigned long long *ind,
+                                              unsigned long long *cand, uint32_t *tmark, uint32_t *tlist,
+                                              unsigned long long *cmark, uint32_t *clist, unsigned long long *imark,
                                               const uint8_t *bpd, const ulonglong2 *bps, BuildCounters *C)
 {
     const uint32_t lane = threadIdx.x & (kChunk - 1);
     const uint32_t ngroups = (gridDim.x * blockDim.x) / kChunk;
-    // This launch is iteration C->iter + 1 of the batch if the previous one
-    // left active vertices (batch_process's loop, PARA:1478-1670); iteration
-    // 255 would leave the reference's u8 domain and is not run.
-    const uint32_t n_act0 = C->n_act;
-    const uint32_t iter = C->iter + 1, gen_iter = C->gen_iter + 1;
+    // Iteration `iter` of batch_process's loop (PARA:1478-1670) runs if the
+    // previous one left active vertices (frontier slot iter & 1); past the
+    // batch's end the launch is empty.  Iteration 255 would leave the
+    // reference's u8 domain: it is flagged, not run.  Block 0 also empties the
+    // other slot, which k_add fills with the next frontier.
+    const uint32_t p = iter & 1u;
+    const uint32_t n_act0 = C->n_act[p];
     const bool live = n_act0 != 0 && iter < kInf8;
-    const uint32_t nc = live ? C->n_chunk : 0u;
+    const uint32_t nc = live ? C->n_chunk[p] : 0u;
     const uint32_t rs = C->rs, gen_batch = C->gen_batch;
+    const unsigned long long gen_iter = ((unsigned long long)gen_batch << 16) | iter;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        C->n_act[p ^ 1u] = 0;
+        C->n_chunk[p ^ 1u] = 0;
+        if (n_act0 != 0) {
+            if (iter >= kInf8) {
+                C->domain = 1;
+            } else {
+                C->iterations += 1;
+                C->actives += n_act0;
+            }
+        }
+    }
     unsigned long long n_push = 0, n_pass = 0, n_bp = 0, n_cand = 0, n_edge = 0, n_tail = 0;
     for (uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) / kChunk; c < nc; c += ngroups) {
         const unsigned long long desc = chunks[c];
@@ -365,7 +380,7 @@ __global__ __launch_bounds__(256) void k_push(const uint64_t *off, const uint32_
             if (atomicExch(&imark[t], gen_iter) != gen_iter) ++n_tail;
         }
         if (got && cmark[t] != gen_iter) {
-            if (atomicExch(&cmark[t], gen_iter) != gen_iter) clist[atomicAdd(&C->n_cand, 1u)] = t;
+            if (atomicExch(&cmark[t], gen_iter) != gen_iter) clist[atomicAdd(&C->n_cand[p], 1u)] = t;
         }
     }
     n_push = wave_sum_u64(n_push);
@@ -382,28 +397,6 @@ __global__ __launch_bounds__(256) void k_push(const uint64_t *off, const uint32_
         if (n_bp) atomicAdd(&C->bp_pruned, n_bp);
         if (n_cand) atomicAdd(&C->cands, n_cand);
     }
-    // The last block to finish closes the iteration: it advances the
-    // iteration counters and empties the frontier for k_add to refill.  Every
-    // block has read the counters it needs before it counts itself done.
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        if (atomicAdd(&C->blocks_done, 1u) == gridDim.x - 1) {
-            C->blocks_done = 0;
-            if (n_act0 != 0) {
-                if (iter >= kInf8) {
-                    C->domain = 1;
-                } else {
-                    C->iter = iter;
-                    C->gen_iter = gen_iter;
-                    C->iterations += 1;
-                    C->actives += n_act0;
-                }
-            }
-            C->n_act = 0;
-            C->n_chunk = 0;
-        }
-    }
 }
 
 // Start of a batch (construct's loop over batches, PARA:1791-1812): roots
@@ -416,10 +409,9 @@ __global__ void k_batch_begin(BuildCounters *C, uint32_t batch, uint32_t n)
     C->rs = rs;
     C->rsz = rs < n ? min(batch, n - rs) : 0u;
     C->gen_batch = b + 1;
-    C->iter = 0;
-    C->n_act = 0;
-    C->n_chunk = 0;
-    C->n_cand = 0;
+    C->n_act[0] = C->n_act[1] = 0;
+    C->n_chunk[0] = C->n_chunk[1] = 0;
+    C->n_cand[0] = C->n_cand[1] = 0;
     C->n_touched = 0;
     C->n_hub = 0;
 }
@@ -429,7 +421,7 @@ __global__ void k_batch_begin(BuildCounters *C, uint32_t batch, uint32_t n)
 // dist < iter: a hub h can prune only if d_v(h) + min_r d_r(h) <= iter, so a
 // per-hub minimum filters most hubs before the 1 KB root-distance row of the
 // survivors is read (16 B per lane).
-__global__ __launch_bounds__(256) void k_add(const uint8_t *used, const uint64_t *off, const uint32_t *adj, const uint32_t *clist,
+__global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used, const uint64_t *off, const uint32_t *adj, const uint32_t *clist,
                                              unsigned long long *cand, LabelStore S, uint32_t *hinfo, uint8_t *dc,
                                              uint32_t *alist, unsigned long long *ag, uint32_t *amask,
                                              uint32_t *aend, unsigned long long *chunks, BuildCounters *C)
@@ -437,8 +429,13 @@ __global__ __launch_bounds__(256) void k_add(const uint8_t *used, const uint64_t
     const uint32_t lane = lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    const uint32_t ncand = C->n_cand;  // 0 once the batch's frontier is empty
-    const uint32_t iter = C->iter, rs = C->rs, rsz = C->rsz;
+    // Candidates of iteration `iter` (slot iter & 1); the next frontier goes
+    // to the other slot, and block 0 empties the other candidate slot for
+    // the next iteration's push.
+    const uint32_t p = iter & 1u;
+    const uint32_t ncand = iter < kInf8 ? C->n_cand[p] : 0u;  // 0 past the batch's end
+    const uint32_t rs = C->rs, rsz = C->rsz;
+    if (blockIdx.x == 0 && threadIdx.x == 0) C->n_cand[p ^ 1u] = 0;
     unsigned long long n_dq = 0, n_lab = 0, n_rows = 0, n_read = 0;
     for (uint32_t i = wave; i < ncand; i += nwaves) {
         const uint32_t v = clist[i];
@@ -540,7 +537,7 @@ __global__ __launch_bounds__(256) void k_add(const uint8_t *used, const uint64_t
         }
         // v becomes active with the new distance group as its last group.
         uint32_t k = 0;
-        if (lane == 0) k = atomicAdd(&C->n_act, 1u);
+        if (lane == 0) k = atomicAdd(&C->n_act[p ^ 1u], 1u);
         k = __shfl(k, 0);
         reinterpret_cast<uint16_t *>(ag + (uint64_t)k * kWords)[lane] = (uint16_t)surv;
         const unsigned long long nzm = __ballot(surv != 0);
@@ -554,7 +551,7 @@ __global__ __launch_bounds__(256) void k_add(const uint8_t *used, const uint64_t
             amask[k] = wmask;
             aend[k] = pdeg;
         }
-        emit_push_chunks(k, pdeg, C, chunks);
+        emit_push_chunks(k, pdeg, &C->n_chunk[p ^ 1u], chunks);
     }
     n_dq = wave_sum_u64(n_dq);
     n_read = wave_sum_u64(n_read);
@@ -563,15 +560,6 @@ __global__ __launch_bounds__(256) void k_add(const uint8_t *used, const uint64_t
         if (n_dq) atomicAdd(&C->dq_pruned, n_dq);
         if (n_lab) atomicAdd(&C->labels, n_lab);
         if (n_rows) atomicAdd(&C->dq_rows, (uint32_t)n_rows);
-    }
-    // The last block to finish empties the candidate list of this iteration.
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        if (atomicAdd(&C->blocks_done, 1u) == gridDim.x - 1) {
-            C->blocks_done = 0;
-            C->n_cand = 0;
-        }
     }
 }
 
@@ -671,7 +659,8 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     }
 
     DevBuf<unsigned long long> ind, cand, ag, chunks, lbase;
-    DevBuf<uint32_t> tmark, tlist, cmark, clist, imark, alist, amask, aend, hinfo, hlist, pool, llen, lcap;
+    DevBuf<uint32_t> tmark, tlist, clist, alist, amask, aend, hinfo, hlist, pool, llen, lcap;
+    DevBuf<unsigned long long> cmark, imark;  // (batch << 16 | iteration) of the last mark
     DevBuf<uint8_t> dc;
     DevBuf<BuildCounters> ctr;
     PADO_TRY(imark.alloc(n));
@@ -707,8 +696,8 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, ind.p, (uint64_t)n * kWords);
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, cand.p, (uint64_t)n * kWords);
     hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, tmark.p, (uint64_t)n, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, cmark.p, (uint64_t)n, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, imark.p, (uint64_t)n, 0u);
+    hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, cmark.p, (uint64_t)n);
+    hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, imark.p, (uint64_t)n);
     hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, hinfo.p, (uint64_t)n, kEmpty);
     hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, llen.p, (uint64_t)n, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, lcap.p, (uint64_t)n, 0u);
@@ -756,16 +745,18 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
         hipLaunchKernelGGL(k_hub_slots, dim3(2048), dim3(256), 0, st, hlist.p, hinfo.p, dc.p, ctr.p);
         hipLaunchKernelGGL(k_hub_fill, dim3(root_blocks), dim3(256), 0, st, d_used.p, S, hinfo.p, dc.p, ctr.p);
     };
-    auto enqueue_iterations = [&](uint32_t count, bool timed) -> pado_gpu_status {
+    // Iterations first + 1 .. first + count of the current batch.
+    auto enqueue_iterations = [&](uint32_t first, uint32_t count, bool timed) -> pado_gpu_status {
         for (uint32_t u = 0; u < count; ++u) {
+            const uint32_t iter = first + 1 + u;
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u], st));
-            hipLaunchKernelGGL(k_push, dim3(push_grid), dim3(256), 0, st, g->off.p, g->adj.p, alist.p, ag.p, amask.p,
-                               aend.p, chunks.p, ind.p, cand.p, tmark.p, tlist.p, cmark.p, clist.p, imark.p,
+            hipLaunchKernelGGL(k_push, dim3(push_grid), dim3(256), 0, st, iter, g->off.p, g->adj.p, alist.p, ag.p,
+                               amask.p, aend.p, chunks.p, ind.p, cand.p, tmark.p, tlist.p, cmark.p, clist.p, imark.p,
                                idx->bpd.p, (const ulonglong2 *)idx->bps.p, ctr.p);
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 1], st));
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 2], st));
-            hipLaunchKernelGGL(k_add, dim3(add_grid), dim3(256), 0, st, d_used.p, g->off.p, g->adj.p, clist.p, cand.p,
-                               S, hinfo.p, dc.p, alist.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
+            hipLaunchKernelGGL(k_add, dim3(add_grid), dim3(256), 0, st, iter, d_used.p, g->off.p, g->adj.p, clist.p,
+                               cand.p, S, hinfo.p, dc.p, alist.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 3], st));
         }
         return PADO_GPU_OK;
@@ -774,27 +765,24 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     // A block of k iterations (k a power of two <= U) is enqueued per host
     // check.  k follows the previous batch's iteration count, so most batches
     // end in one block with at most a few empty iterations.
-    // Graph mode: gx_batch[j] = prologue + 2^j iterations, gx_iter[j] = 2^j
-    // iterations, captured lazily.  Kernel arguments are fixed per capture,
-    // so all graphs are dropped when the pool or the root-distance table moves.
-    constexpr int kMaxLog = 7;  // U <= 64
-    hipGraphExec_t gx_batch[kMaxLog] = {}, gx_iter[kMaxLog] = {};
+    // Graph mode: one captured graph per (first iteration, count), the batch
+    // prologue included when first == 0; captured lazily and replayed for
+    // every batch.  Kernel arguments are fixed per capture, so all graphs are
+    // dropped when the pool or the root-distance table moves.
+    std::map<std::pair<uint32_t, uint32_t>, hipGraphExec_t> graphs;
     auto drop_graphs = [&]() {
-        for (int j = 0; j < kMaxLog; ++j) {
-            if (gx_batch[j]) (void)hipGraphExecDestroy(gx_batch[j]);
-            if (gx_iter[j]) (void)hipGraphExecDestroy(gx_iter[j]);
-            gx_batch[j] = gx_iter[j] = nullptr;
-        }
+        for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second);
+        graphs.clear();
     };
     struct GraphFree {
         std::function<void()> f;
         ~GraphFree() { f(); }
     } gxf{drop_graphs};
-    auto capture = [&](bool with_prologue, uint32_t count, hipGraphExec_t *out) -> pado_gpu_status {
+    auto capture = [&](uint32_t first, uint32_t count, hipGraphExec_t *out) -> pado_gpu_status {
         hipGraph_t gr = nullptr;
         PADO_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        if (with_prologue) enqueue_prologue();
-        pado_gpu_status s = enqueue_iterations(count, false);
+        if (first == 0) enqueue_prologue();
+        pado_gpu_status s = enqueue_iterations(first, count, false);
         hipError_t e = hipStreamEndCapture(st, &gr);
         if (s != PADO_GPU_OK) return s;
         if (e != hipSuccess) {
@@ -828,19 +816,21 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
             drop_graphs();
         }
         const uint64_t batch_first_iter = iterations_seen;
-        bool first = true;
+        uint32_t launched = 0;  // iterations of this batch enqueued so far
         int j = log2_block(prev_batch_iters + 1);
         for (;;) {
-            const uint32_t count = 1u << j;
+            // Iteration 255 is the last one ever launched: k_push flags it if
+            // the frontier is still live there.
+            const uint32_t count = std::min<uint32_t>(1u << j, (uint32_t)kInf8 - launched);
             if (use_graph) {
-                hipGraphExec_t *slot = first ? &gx_batch[j] : &gx_iter[j];
-                if (!*slot) PADO_TRY(capture(first, count, slot));
-                PADO_HIP(hipGraphLaunch(*slot, st));
+                hipGraphExec_t &gx = graphs[std::make_pair(launched, count)];
+                if (!gx) PADO_TRY(capture(launched, count, &gx));
+                PADO_HIP(hipGraphLaunch(gx, st));
             } else {
-                if (first) enqueue_prologue();
-                PADO_TRY(enqueue_iterations(count, true));
+                if (launched == 0) enqueue_prologue();
+                PADO_TRY(enqueue_iterations(launched, count, true));
             }
-            first = false;
+            launched += count;
             PADO_HIP(hipMemcpyAsync(h_ctr, ctr.p, sizeof(BuildCounters), hipMemcpyDeviceToHost, st));
             PADO_HIP(hipStreamSynchronize(st));
             PADO_HIP(hipGetLastError());
@@ -877,7 +867,12 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
                 PADO_HIP(hipStreamSynchronize(st));
                 drop_graphs();
             }
-            if (h_ctr->n_act == 0) break;
+            // The frontier of the next iteration is in slot (launched + 1) & 1.
+            if (h_ctr->n_act[(launched + 1) & 1u] == 0) break;
+            if (launched >= kInf8) {
+                set_error("batch iteration reached 255: distances outside the uint8 domain of the reference");
+                return PADO_GPU_ERR_DOMAIN;
+            }
             if ((1u << (j + 1)) <= U) ++j;  // the batch runs long: double the block
         }
         prev_batch_iters = (uint32_t)(iterations_seen - batch_first_iter);

@@ -74,14 +74,16 @@ struct DevBuf {
 
 // Device-side counters shared by the labeling kernels (one struct per build).
 struct BuildCounters {
-    uint32_t n_act;        // active vertices produced for the next iteration
-    uint32_t n_chunk;      // push work descriptors for the next iteration
-    uint32_t n_cand;       // candidate vertices of the current iteration
+    // Frontier and candidate counts of batch iteration i live in slot i & 1:
+    // each kernel reads its own slot and empties the other one for the next
+    // kernel to fill, so no control launch runs between iterations.
+    uint32_t n_act[2];     // active vertices of the iteration
+    uint32_t n_chunk[2];   // push work descriptors of the iteration
+    uint32_t n_cand[2];    // candidate vertices of the iteration
     uint32_t n_touched;    // vertices whose indicator row was touched this batch
     uint32_t n_hub;        // distinct hubs in the batch's root-distance table
     uint32_t overflow;     // label pool exhausted
     uint32_t dq_rows;      // root-distance rows scanned (statistic)
-    uint32_t pad0;
     unsigned long long pool_top;  // next free label slot in the pool
     unsigned long long pushes;    // (tail, root) pairs offered by push
     unsigned long long ind_pass;  // pairs that set a fresh indicator bit
@@ -97,9 +99,6 @@ struct BuildCounters {
     // one launch sequence (or one captured hipGraph) serves every batch.
     uint32_t next_batch;          // batch index the next k_batch_begin starts
     uint32_t rs, rsz, gen_batch;  // current batch: roots [rs, rs + rsz)
-    uint32_t iter;                // current iteration of the batch (1-based)
-    uint32_t blocks_done;         // blocks of the running launch that finished
-    uint32_t gen_iter;            // global iteration generation (marks)
     uint32_t domain;              // an iteration reached 255 (u8 domain)
     unsigned long long iterations;  // live iterations, summed over batches
 };
