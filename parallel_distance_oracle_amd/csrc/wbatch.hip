@@ -42,8 +42,10 @@ constexpr uint32_t kWW = kWB / 64;    // 64-bit words per root bitmap
 constexpr uint32_t kWInf = 0xFFFFFFFFu;
 using u64 = unsigned long long;
 
+// Frontier / improved-tail counts of round i live in slot i & 1: each round's
+// kernels read their slot and empty the other one for the next kernel.
 struct WCounters {
-    uint32_t n_act, n_chunk, n_imp, n_touched, n_hub, overflow, gen_round, pad1;
+    uint32_t n_act[2], n_chunk[2], n_imp[2], n_touched, n_hub, overflow, pad1;
     u64 pool_top, relax, improved, pruned, labels, filtered, rounds, need;
 };
 
@@ -111,20 +113,21 @@ __device__ inline u64 store_reserve(WStore S, WCounters *C, uint32_t v, uint32_t
     return nb;
 }
 
-__device__ inline void emit_chunks(uint32_t k, uint32_t pdeg, WCounters *C, u64 *chunks)
+__device__ inline void emit_chunks(uint32_t k, uint32_t pdeg, uint32_t *n_chunk, u64 *chunks)
 {
     const uint32_t lane = lane_id();
     uint32_t nch = (pdeg + kChunk - 1) / kChunk;
     if (!nch) return;
     uint32_t c0 = 0;
-    if (lane == 0) c0 = atomicAdd(&C->n_chunk, nch);
+    if (lane == 0) c0 = atomicAdd(n_chunk, nch);
     c0 = __shfl(c0, 0);
     for (uint32_t q = lane; q < nch; q += 64) chunks[c0 + q] = ((u64)(q * kChunk) << 32) | k;
 }
 
 __global__ void k_w_reset_counters(WCounters *C)
 {
-    C->n_act = C->n_chunk = C->n_imp = C->n_touched = C->n_hub = 0;
+    C->n_act[0] = C->n_act[1] = C->n_chunk[0] = C->n_chunk[1] = C->n_imp[0] = C->n_imp[1] = 0;
+    C->n_touched = C->n_hub = 0;
     C->need = 0;
 }
 
@@ -143,7 +146,7 @@ __global__ __launch_bounds__(256) void k_w_init(uint32_t rs, uint32_t rsz, uint3
     if (lane == 0) {
         T[(uint64_t)r * kWB + ro] = 0;
         if (atomicExch(&tmark[r], gen_batch) != gen_batch) tlist[atomicAdd(&C->n_touched, 1u)] = r;
-        k = atomicAdd(&C->n_act, 1u);
+        k = atomicAdd(&C->n_act[1], 1u);  // round 1 -> slot 1
     }
     k = __shfl(k, 0);
     if (lane < kWW) ag[(uint64_t)k * kWW + lane] = (lane == (ro >> 6)) ? (1ull << (ro & 63)) : 0ull;
@@ -153,7 +156,7 @@ __global__ __launch_bounds__(256) void k_w_init(uint32_t rs, uint32_t rsz, uint3
         amask[k] = 1u << (ro >> 6);
         aend[k] = pdeg;
     }
-    emit_chunks(k, pdeg, C, chunks);
+    emit_chunks(k, pdeg, &C->n_chunk[1], chunks);
 }
 
 // Earlier-batch labels of the batch roots -> hub table (roots_labels_buffer,
@@ -208,17 +211,25 @@ __global__ __launch_bounds__(256) void k_w_hub_fill(uint32_t rs, uint32_t rsz, W
 
 // send_messages (WPARA:435-522): relax the frontier's edges toward tails
 // ranked below each root.
-__global__ __launch_bounds__(256) void k_w_push(uint32_t rs, uint32_t gen_batch,
+__global__ __launch_bounds__(256) void k_w_push(uint32_t round, uint32_t rs, uint32_t gen_batch,
                                                 const uint64_t *off, const uint32_t *adj, const uint8_t *wgt,
                                                 const uint32_t *alist, const u64 *ag, const uint32_t *amask,
                                                 const uint32_t *aend, const u64 *chunks, uint32_t *T, u64 *I,
-                                                uint32_t *tmark, uint32_t *tlist, uint32_t *imark, uint32_t *ilist,
+                                                uint32_t *tmark, uint32_t *tlist, u64 *imark, uint32_t *ilist,
                                                 WCounters *C)
 {
     const uint32_t lane = threadIdx.x & (kChunk - 1);
     const uint32_t ngroups = (gridDim.x * blockDim.x) / kChunk;
-    const uint32_t nc = C->n_chunk;  // 0 once the batch's frontier is empty
-    const uint32_t gen_round = C->gen_round;
+    // Round `round` of the batch (slot round & 1; empty past the batch's
+    // end).  Block 0 empties the other slot, which k_w_check fills.
+    const uint32_t p = round & 1u;
+    const uint32_t nc = C->n_chunk[p];
+    const u64 gen_round = ((u64)gen_batch << 24) | round;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (C->n_act[p]) C->rounds += 1;
+        C->n_act[p ^ 1u] = 0;
+        C->n_chunk[p ^ 1u] = 0;
+    }
     u64 n_relax = 0, n_imp = 0;
     for (uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) / kChunk; c < nc; c += ngroups) {
         const u64 desc = chunks[c];
@@ -257,7 +268,7 @@ __global__ __launch_bounds__(256) void k_w_push(uint32_t rs, uint32_t gen_batch,
             if (tmark[t] != gen_batch && atomicExch(&tmark[t], gen_batch) != gen_batch)
                 tlist[atomicAdd(&C->n_touched, 1u)] = t;
             if (imark[t] != gen_round && atomicExch(&imark[t], gen_round) != gen_round) {
-                ilist[atomicAdd(&C->n_imp, 1u)] = t;
+                ilist[atomicAdd(&C->n_imp[p], 1u)] = t;
                 ++n_imp;
             }
         }
@@ -270,28 +281,12 @@ __global__ __launch_bounds__(256) void k_w_push(uint32_t rs, uint32_t gen_batch,
     }
 }
 
-__global__ void k_w_mid(WCounters *C)
-{
-    C->n_act = 0;
-    C->n_chunk = 0;
-}
 
-__global__ void k_w_post(WCounters *C) { C->n_imp = 0; }
-
-// Start of a Bellman-Ford round: a round with an empty frontier is an empty
-// launch sequence (push sees no chunks, check no improved tails), so the host
-// can enqueue several rounds per check of the frontier.
-__global__ void k_w_round_begin(WCounters *C)
-{
-    if (C->n_act == 0) return;
-    C->gen_round += 1;
-    C->rounds += 1;
-}
 
 // distance_query's first loop (WPARA:569-580) for every improved pair of an
 // improved tail: one wave per tail, lane l owns roots 8l..8l+7.  Pruned pairs
 // stop propagating; the others become the tail's next frontier bitmap.
-__global__ __launch_bounds__(256) void k_w_check(uint32_t rs, const uint32_t *ilist, const uint64_t *off,
+__global__ __launch_bounds__(256) void k_w_check(uint32_t round, uint32_t rs, const uint32_t *ilist, const uint64_t *off,
                                                  const uint32_t *adj, uint32_t *T, u64 *I, u64 *P, WStore S,
                                                  const uint32_t *hslot, const uint32_t *hmin, const uint32_t *dcw,
                                                  uint32_t *alist, u64 *ag, uint32_t *amask, uint32_t *aend,
@@ -300,7 +295,9 @@ __global__ __launch_bounds__(256) void k_w_check(uint32_t rs, const uint32_t *il
     const uint32_t lane = lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
-    const uint32_t nimp = C->n_imp;
+    const uint32_t p = round & 1u;
+    const uint32_t nimp = C->n_imp[p];
+    if (blockIdx.x == 0 && threadIdx.x == 0) C->n_imp[p ^ 1u] = 0;
     u64 n_pruned = 0;
     for (uint32_t i = wave; i < nimp; i += nwaves) {
         const uint32_t t = ilist[i];
@@ -355,7 +352,7 @@ __global__ __launch_bounds__(256) void k_w_check(uint32_t rs, const uint32_t *il
         const u64 any = __ballot(live != 0);
         if (!any) continue;
         uint32_t k = 0;
-        if (lane == 0) k = atomicAdd(&C->n_act, 1u);
+        if (lane == 0) k = atomicAdd(&C->n_act[p ^ 1u], 1u);
         k = __shfl(k, 0);
         reinterpret_cast<uint8_t *>(ag + (uint64_t)k * kWW)[lane] = (uint8_t)live;
         uint32_t wmask = 0;
@@ -368,7 +365,7 @@ __global__ __launch_bounds__(256) void k_w_check(uint32_t rs, const uint32_t *il
             amask[k] = wmask;
             aend[k] = pdeg;
         }
-        emit_chunks(k, pdeg, C, chunks);
+        emit_chunks(k, pdeg, &C->n_chunk[p ^ 1u], chunks);
     }
     n_pruned = wave_sum(n_pruned);
     if (lane == 0 && n_pruned) atomicAdd(&C->pruned, n_pruned);
@@ -543,8 +540,8 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
     hipStream_t st = g->ctx->stream;
     const uint32_t n = g->n;
     const uint32_t batch = (batch_req == 0 || batch_req > kWB) ? kWB : batch_req;
-    DevBuf<uint32_t> T, tmark, tlist, imark, ilist, alist, amask, aend, hslot, hmin, hlist, dcw, len, cap;
-    DevBuf<u64> P, I, ag, chunks, pool, base;
+    DevBuf<uint32_t> T, tmark, tlist, ilist, alist, amask, aend, hslot, hmin, hlist, dcw, len, cap;
+    DevBuf<u64> P, I, ag, chunks, pool, base, imark;
     DevBuf<WCounters> ctr;
     PADO_TRY(T.alloc((uint64_t)n * kWB));
     PADO_TRY(P.alloc((uint64_t)n * kWW));
@@ -574,7 +571,7 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
     hipLaunchKernelGGL(k_w_zero64, dim3(fg), dim3(256), 0, st, P.p, (uint64_t)n * kWW);
     hipLaunchKernelGGL(k_w_zero64, dim3(fg), dim3(256), 0, st, I.p, (uint64_t)n * kWW);
     hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, tmark.p, (uint64_t)n, 0u);
-    hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, imark.p, (uint64_t)n, 0u);
+    hipLaunchKernelGGL(k_w_zero64, dim3(fg), dim3(256), 0, st, imark.p, (uint64_t)n);
     hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, hslot.p, (uint64_t)n, kEmpty);
     hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, len.p, (uint64_t)n, 0u);
     hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, cap.p, (uint64_t)n, 0u);
@@ -608,25 +605,29 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
         hipLaunchKernelGGL(k_w_hub_fill, dim3(rb), dim3(256), 0, st, rs, rsz, S, hslot.p, hmin.p, dcw.p);
         // Blocks of 2^j rounds per host check; j follows the previous batch's
         // round count and doubles while a batch runs long.
-        uint32_t n_act = h->n_act;
+        uint32_t live = h->n_act[1];
         const uint64_t rounds_before = h->rounds;
+        uint32_t launched = 0;
         int j = 0;
         while ((1u << j) < prev_rounds + 1 && j < 6) ++j;
-        while (n_act) {
-            for (uint32_t u = 0; u < (1u << j); ++u) {
-                hipLaunchKernelGGL(k_w_round_begin, dim3(1), dim3(1), 0, st, ctr.p);
-                hipLaunchKernelGGL(k_w_push, dim3(4096), dim3(256), 0, st, rs, gen_batch, g->off.p, g->adj.p, g->w.p,
-                                   alist.p, ag.p, amask.p, aend.p, chunks.p, T.p, I.p, tmark.p, tlist.p, imark.p,
-                                   ilist.p, ctr.p);
-                hipLaunchKernelGGL(k_w_mid, dim3(1), dim3(1), 0, st, ctr.p);
-                hipLaunchKernelGGL(k_w_check, dim3(2048), dim3(256), 0, st, rs, ilist.p, g->off.p, g->adj.p, T.p,
-                                   I.p, P.p, S, hslot.p, hmin.p, dcw.p, alist.p, ag.p, amask.p, aend.p, chunks.p,
-                                   ctr.p);
-                hipLaunchKernelGGL(k_w_post, dim3(1), dim3(1), 0, st, ctr.p);
+        while (live) {
+            if (launched + (1u << j) >= (1u << 24)) {
+                set_error("a weighted batch needs more than 2^24 Bellman-Ford rounds");
+                return PADO_GPU_ERR_DOMAIN;
             }
+            for (uint32_t u = 0; u < (1u << j); ++u) {
+                const uint32_t round = launched + 1 + u;
+                hipLaunchKernelGGL(k_w_push, dim3(4096), dim3(256), 0, st, round, rs, gen_batch, g->off.p, g->adj.p,
+                                   g->w.p, alist.p, ag.p, amask.p, aend.p, chunks.p, T.p, I.p, tmark.p, tlist.p,
+                                   imark.p, ilist.p, ctr.p);
+                hipLaunchKernelGGL(k_w_check, dim3(2048), dim3(256), 0, st, round, rs, ilist.p, g->off.p, g->adj.p,
+                                   T.p, I.p, P.p, S, hslot.p, hmin.p, dcw.p, alist.p, ag.p, amask.p, aend.p,
+                                   chunks.p, ctr.p);
+            }
+            launched += 1u << j;
             PADO_HIP(hipMemcpyAsync(h, ctr.p, sizeof(WCounters), hipMemcpyDeviceToHost, st));
             PADO_HIP(hipStreamSynchronize(st));
-            n_act = h->n_act;
+            live = h->n_act[(launched + 1) & 1u];  // frontier of the next round
             if (j < 6) ++j;
         }
         prev_rounds = (uint32_t)(h->rounds - rounds_before);
