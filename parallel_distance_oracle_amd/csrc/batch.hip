@@ -58,6 +58,26 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long x)
     return x;
 }
 
+// Append one item per lane with `pred` to a list counted by *counter: the
+// active lanes of the wave take consecutive slots with a single atomic, so a
+// shared counter sees one update per wave instead of one per item.  Returns
+// the lane's slot (meaningless where !pred).
+__device__ inline uint32_t wave_append(uint32_t *counter, bool pred)
+{
+    const unsigned long long m = __ballot(pred);
+    if (m == 0ull) return 0u;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t leader = (uint32_t)__ffsll(m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
+// Push descriptor written into reserved chunk slots that carry no work: its
+// edge offset is past every adjacency list, so k_push skips it.
+constexpr unsigned long long kChunkPad = 0xFFFFFFF0ull << 32;
+
 // Number of neighbours of v with id >= rs: the adjacency is sorted
 // descending, so they form a prefix (PARA:1121-1123 returns at the first tail
 // below roots_start).  Wave-cooperative 64-ary search.
@@ -226,7 +246,10 @@ __global__ __launch_bounds__(256) void k_batch_init(const uint8_t *used, const u
     }
     // The frontier of iteration i is counted in slot i & 1: iteration 1 -> 1.
     uint32_t k = 0;
-    if (lane == 0) k = atomicAdd(&C->n_act[1], 1u);
+    if (lane == 0) {
+        k = atomicAdd(&C->a_top[1], 1u);
+        atomicAdd(&C->n_act[1], 1u);
+    }
     k = __shfl(k, 0);
     if (lane < kWords) ag[(uint64_t)k * kWords + lane] = (lane == (ro >> 6)) ? (1ull << (ro & 63)) : 0ull;
     uint32_t pdeg = push_degree(off, adj, r, rs);
@@ -318,6 +341,7 @@ __global__ __launch_bounds__(256) void k_push(uint32_t iter, const uint64_t *off
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         C->n_act[p ^ 1u] = 0;
         C->n_chunk[p ^ 1u] = 0;
+        C->a_top[p ^ 1u] = 0;
         if (n_act0 != 0) {
             if (iter >= kInf8) {
                 C->domain = 1;
@@ -330,6 +354,7 @@ __global__ __launch_bounds__(256) void k_push(uint32_t iter, const uint64_t *off
     unsigned long long n_push = 0, n_pass = 0, n_bp = 0, n_cand = 0, n_edge = 0, n_tail = 0;
     for (uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) / kChunk; c < nc; c += ngroups) {
         const unsigned long long desc = chunks[c];
+        if (desc == kChunkPad) continue;  // reserved slot without work
         const uint32_t k = (uint32_t)desc;
         const uint32_t e = (uint32_t)(desc >> 32) + lane;
         if (e >= aend[k]) continue;
@@ -373,15 +398,13 @@ __global__ __launch_bounds__(256) void k_push(uint32_t iter, const uint64_t *off
                 got = true;
             }
         }
-        if (touched && tmark[t] != gen_batch) {
-            if (atomicExch(&tmark[t], gen_batch) != gen_batch) tlist[atomicAdd(&C->n_touched, 1u)] = t;
-        }
-        if (touched && imark[t] != gen_iter) {
-            if (atomicExch(&imark[t], gen_iter) != gen_iter) ++n_tail;
-        }
-        if (got && cmark[t] != gen_iter) {
-            if (atomicExch(&cmark[t], gen_iter) != gen_iter) clist[atomicAdd(&C->n_cand[p], 1u)] = t;
-        }
+        const bool new_t = touched && tmark[t] != gen_batch && atomicExch(&tmark[t], gen_batch) != gen_batch;
+        if (touched && imark[t] != gen_iter && atomicExch(&imark[t], gen_iter) != gen_iter) ++n_tail;
+        const bool new_c = got && cmark[t] != gen_iter && atomicExch(&cmark[t], gen_iter) != gen_iter;
+        const uint32_t ti = wave_append(&C->n_touched, new_t);
+        if (new_t) tlist[ti] = t;
+        const uint32_t ci = wave_append(&C->n_cand[p], new_c);
+        if (new_c) clist[ci] = t;
     }
     n_push = wave_sum_u64(n_push);
     n_pass = wave_sum_u64(n_pass);
@@ -410,6 +433,7 @@ __global__ void k_batch_begin(BuildCounters *C, uint32_t batch, uint32_t n)
     C->rsz = rs < n ? min(batch, n - rs) : 0u;
     C->gen_batch = b + 1;
     C->n_act[0] = C->n_act[1] = 0;
+    C->a_top[0] = C->a_top[1] = 0;
     C->n_chunk[0] = C->n_chunk[1] = 0;
     C->n_cand[0] = C->n_cand[1] = 0;
     C->n_touched = 0;
@@ -437,6 +461,12 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
     const uint32_t rs = C->rs, rsz = C->rsz;
     if (blockIdx.x == 0 && threadIdx.x == 0) C->n_cand[p ^ 1u] = 0;
     unsigned long long n_dq = 0, n_lab = 0, n_rows = 0, n_read = 0;
+    // Next-frontier slots (alist entries and push descriptors) are reserved
+    // kAlistGrab / kChunkGrab at a time per wave, so the shared counters see
+    // one atomic per reservation, not per vertex; unused descriptor slots are
+    // padded.  The exact active count is added once per wave at the end.
+    constexpr uint32_t kAlistGrab = 16, kChunkGrab = 64;
+    uint32_t a_next = 0, a_left = 0, c_next = 0, c_left = 0, my_act = 0;
     for (uint32_t i = wave; i < ncand; i += nwaves) {
         const uint32_t v = clist[i];
         const unsigned long long word = cand[(uint64_t)v * kWords + (lane >> 2)];
@@ -536,9 +566,15 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
             }
         }
         // v becomes active with the new distance group as its last group.
-        uint32_t k = 0;
-        if (lane == 0) k = atomicAdd(&C->n_act[p ^ 1u], 1u);
-        k = __shfl(k, 0);
+        if (a_left == 0) {
+            uint32_t b0 = 0;
+            if (lane == 0) b0 = atomicAdd(&C->a_top[p ^ 1u], kAlistGrab);
+            a_next = __shfl(b0, 0);
+            a_left = kAlistGrab;
+        }
+        const uint32_t k = a_next++;
+        --a_left;
+        ++my_act;
         reinterpret_cast<uint16_t *>(ag + (uint64_t)k * kWords)[lane] = (uint16_t)surv;
         const unsigned long long nzm = __ballot(surv != 0);
         uint32_t wmask = 0;
@@ -551,8 +587,21 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
             amask[k] = wmask;
             aend[k] = pdeg;
         }
-        emit_push_chunks(k, pdeg, &C->n_chunk[p ^ 1u], chunks);
+        const uint32_t nch = (pdeg + kChunk - 1) / kChunk;
+        if (nch > c_left) {
+            for (uint32_t q = lane; q < c_left; q += 64) chunks[c_next + q] = kChunkPad;
+            const uint32_t want = nch > kChunkGrab ? nch : kChunkGrab;
+            uint32_t b0 = 0;
+            if (lane == 0) b0 = atomicAdd(&C->n_chunk[p ^ 1u], want);
+            c_next = __shfl(b0, 0);
+            c_left = want;
+        }
+        for (uint32_t q = lane; q < nch; q += 64) chunks[c_next + q] = ((unsigned long long)(q * kChunk) << 32) | k;
+        c_next += nch;
+        c_left -= nch;
     }
+    for (uint32_t q = lane; q < c_left; q += 64) chunks[c_next + q] = kChunkPad;
+    if (lane == 0 && my_act) atomicAdd(&C->n_act[p ^ 1u], my_act);
     n_dq = wave_sum_u64(n_dq);
     n_read = wave_sum_u64(n_read);
     if (lane == 0) {
@@ -666,15 +715,20 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     PADO_TRY(imark.alloc(n));
     PADO_TRY(ind.alloc((uint64_t)n * kWords));
     PADO_TRY(cand.alloc((uint64_t)n * kWords));
-    PADO_TRY(ag.alloc((uint64_t)n * kWords));
-    PADO_TRY(chunks.alloc(g->m / kChunk + (uint64_t)n + 64));
+    // Per-wave reservations in k_add leave holes: the active list can exceed
+    // the active count by kAlistGrab slots per wave, and padded descriptors
+    // are fewer than the real ones plus kChunkGrab per wave.
+    const uint64_t add_waves = 4ull * (cfg.add_blocks ? cfg.add_blocks : 2048);
+    const uint64_t alist_cap = (uint64_t)n + 16 * add_waves;
+    PADO_TRY(ag.alloc(alist_cap * kWords));
+    PADO_TRY(chunks.alloc(2 * (g->m / kChunk + (uint64_t)n) + 64 * add_waves + 64));
     PADO_TRY(tmark.alloc(n));
     PADO_TRY(tlist.alloc(n));
     PADO_TRY(cmark.alloc(n));
     PADO_TRY(clist.alloc(n));
-    PADO_TRY(alist.alloc(n));
-    PADO_TRY(amask.alloc(n));
-    PADO_TRY(aend.alloc(n));
+    PADO_TRY(alist.alloc(alist_cap));
+    PADO_TRY(amask.alloc(alist_cap));
+    PADO_TRY(aend.alloc(alist_cap));
     PADO_TRY(hinfo.alloc(n));
     PADO_TRY(hlist.alloc(n));
     PADO_TRY(lbase.alloc(n));

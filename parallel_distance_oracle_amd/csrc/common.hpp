@@ -78,6 +78,7 @@ struct BuildCounters {
     // each kernel reads its own slot and empties the other one for the next
     // kernel to fill, so no control launch runs between iterations.
     uint32_t n_act[2];     // active vertices of the iteration
+    uint32_t a_top[2];     // active-list slots handed out (>= n_act: per-wave reservations)
     uint32_t n_chunk[2];   // push work descriptors of the iteration
     uint32_t n_cand[2];    // candidate vertices of the iteration
     uint32_t n_touched;    // vertices whose indicator row was touched this batch
