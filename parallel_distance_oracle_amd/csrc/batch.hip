@@ -58,22 +58,6 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long x)
     return x;
 }
 
-// Append one item per lane with `pred` to a list counted by *counter: the
-// active lanes of the wave take consecutive slots with a single atomic, so a
-// shared counter sees one update per wave instead of one per item.  Returns
-// the lane's slot (meaningless where !pred).
-__device__ inline uint32_t wave_append(uint32_t *counter, bool pred)
-{
-    const unsigned long long m = __ballot(pred);
-    if (m == 0ull) return 0u;
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t leader = (uint32_t)__ffsll(m) - 1;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-    base = __shfl(base, leader);
-    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-}
-
 // Push descriptor written into reserved chunk slots that carry no work: its
 // edge offset is past every adjacency list, so k_push skips it.
 constexpr unsigned long long kChunkPad = 0xFFFFFFF0ull << 32;
@@ -275,7 +259,9 @@ __global__ __launch_bounds__(256) void k_hub_mark(const uint8_t *used, LabelStor
     const unsigned long long base = S.base[r];
     for (uint32_t j = lane; j < len; j += 64) {
         uint32_t h = S.pool[base + j] >> 8;
-        if (atomicCAS(&hinfo[h], kEmpty, kClaim) == kEmpty) hlist[atomicAdd(&C->n_hub, 1u)] = h;
+        const bool fresh = atomicCAS(&hinfo[h], kEmpty, kClaim) == kEmpty;
+        const uint32_t hi = wave_append(&C->n_hub, fresh);
+        if (fresh) hlist[hi] = h;
     }
 }
 

@@ -72,6 +72,22 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
+// Append one item per lane with `pred` to a list counted by *counter: the
+// active lanes of the wave take consecutive slots with a single atomic, so a
+// shared counter sees one update per wave instead of one per item.  Returns
+// the lane's slot (meaningless where !pred).
+__device__ inline uint32_t wave_append(uint32_t *counter, bool pred)
+{
+    const unsigned long long m = __ballot(pred);
+    if (m == 0ull) return 0u;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t leader = (uint32_t)__ffsll(m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+}
+
 // Device-side counters shared by the labeling kernels (one struct per build).
 struct BuildCounters {
     // Frontier and candidate counts of batch iteration i live in slot i & 1:

@@ -172,7 +172,9 @@ __global__ __launch_bounds__(256) void k_w_hub_mark(uint32_t rs, uint32_t rsz, W
     const u64 base = S.base[r];
     for (uint32_t j = lane; j < len; j += 64) {
         const uint32_t h = (uint32_t)(S.pool[base + j] >> 32);
-        if (atomicCAS(&hslot[h], kEmpty, kEmpty - 1) == kEmpty) hlist[atomicAdd(&C->n_hub, 1u)] = h;
+        const bool fresh = atomicCAS(&hslot[h], kEmpty, kEmpty - 1) == kEmpty;
+        const uint32_t hi = wave_append(&C->n_hub, fresh);
+        if (fresh) hlist[hi] = h;
     }
 }
 
@@ -264,13 +266,14 @@ __global__ __launch_bounds__(256) void k_w_push(uint32_t round, uint32_t rs, uin
                 got = true;
             }
         }
-        if (got) {
-            if (tmark[t] != gen_batch && atomicExch(&tmark[t], gen_batch) != gen_batch)
-                tlist[atomicAdd(&C->n_touched, 1u)] = t;
-            if (imark[t] != gen_round && atomicExch(&imark[t], gen_round) != gen_round) {
-                ilist[atomicAdd(&C->n_imp[p], 1u)] = t;
-                ++n_imp;
-            }
+        const bool new_t = got && tmark[t] != gen_batch && atomicExch(&tmark[t], gen_batch) != gen_batch;
+        const bool new_i = got && imark[t] != gen_round && atomicExch(&imark[t], gen_round) != gen_round;
+        const uint32_t ti = wave_append(&C->n_touched, new_t);
+        if (new_t) tlist[ti] = t;
+        const uint32_t ii = wave_append(&C->n_imp[p], new_i);
+        if (new_i) {
+            ilist[ii] = t;
+            ++n_imp;
         }
     }
     n_relax = wave_sum(n_relax);
