@@ -552,11 +552,15 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
             }
         }
         // v becomes active with the new distance group as its last group.
+        // Reservations are capped by the vertices this wave has left in the
+        // launch (grid-stride), so light launches reserve exactly.
+        const uint32_t rem = (ncand - 1 - i) / nwaves + 1;
         if (a_left == 0) {
+            const uint32_t grab = rem < kAlistGrab ? rem : kAlistGrab;
             uint32_t b0 = 0;
-            if (lane == 0) b0 = atomicAdd(&C->a_top[p ^ 1u], kAlistGrab);
+            if (lane == 0) b0 = atomicAdd(&C->a_top[p ^ 1u], grab);
             a_next = __shfl(b0, 0);
-            a_left = kAlistGrab;
+            a_left = grab;
         }
         const uint32_t k = a_next++;
         --a_left;
@@ -576,7 +580,9 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
         const uint32_t nch = (pdeg + kChunk - 1) / kChunk;
         if (nch > c_left) {
             for (uint32_t q = lane; q < c_left; q += 64) chunks[c_next + q] = kChunkPad;
-            const uint32_t want = nch > kChunkGrab ? nch : kChunkGrab;
+            const uint64_t guess = (uint64_t)nch * rem;  // chunks left if v is typical
+            const uint32_t cap = guess < kChunkGrab ? (uint32_t)guess : kChunkGrab;
+            const uint32_t want = nch > cap ? nch : cap;
             uint32_t b0 = 0;
             if (lane == 0) b0 = atomicAdd(&C->n_chunk[p ^ 1u], want);
             c_next = __shfl(b0, 0);
