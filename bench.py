@@ -270,7 +270,8 @@ def main():
                     help="timed builds and the roofline only (for profiler runs)")
     ap.add_argument("--copy-calibrate", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (PMC byte-counter calibration) and exit")
-    ap.add_argument("--graph", action="store_true", help="build with the hipGraph-captured batch step")
+    ap.add_argument("--stream", action="store_true",
+                    help="run the batch step as per-kernel launches with HIP-event timing instead of the default hipGraphs")
     ap.add_argument("--weighted-scale", type=int, default=0,
                     help="also build and serve a weighted R-MAT of this scale (0 = skip)")
     args = ap.parse_args()
@@ -290,7 +291,7 @@ def main():
     ctx.sync()
     cfg = pado.BuildCfg()
     cfg.batch_size = pado.BATCH_SIZE
-    cfg.flags = pado.BUILD_GRAPH if args.graph else 0
+    cfg.flags = pado.BUILD_STREAM if args.stream else 0
 
     for w in range(args.warmup):
         t0 = time.time()
@@ -332,8 +333,11 @@ def main():
     # build stream; a hipGraph build records none, so then one extra
     # stream-mode build of the same graph (same kernels and grids) times them.
     timing_src = "timed builds (HIP events on the build stream)"
-    if args.graph:
-        tb = pado.Index.build(ctx, dg)
+    if not args.stream:
+        tcfg = pado.BuildCfg()
+        tcfg.batch_size = pado.BATCH_SIZE
+        tcfg.flags = pado.BUILD_STREAM
+        tb = pado.Index.build(ctx, dg, cfg=tcfg)
         ctx.sync()
         st_t = tb.stats()
         tb.close()
@@ -436,7 +440,7 @@ def main():
         "config": {"workload": f"R-MAT scale {args.scale} ef {args.edge_factor} (0.57,0.19,0.19,0.05) seed 1, "
                                f"BATCH_SIZE=1024, BITPARALLEL_SIZE=50; full index build",
                    "num_v": G.num_v, "num_edges": int(edges.shape[0]),
-                   "batch_step": "hipGraph" if args.graph else "stream (launch-ahead blocks)"},
+                   "batch_step": "stream (launch-ahead blocks)" if args.stream else "hipGraph"},
         "parity": parity, "query": query, "serving": serving, "weighted": weighted, "roofline": roofline,
         "cpu_baseline": cpu,
         "build_breakdown_ms": {"bp": round(st["ms_bp"], 1), "batches_and_sort": round(st["ms_batch"], 1)},

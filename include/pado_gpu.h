@@ -153,11 +153,12 @@ typedef struct {
 /* Bit-parallel labeling one BFS per slot (as the reference runs it) instead of
  * the default single multi-source sweep over all 50 slots; same output. */
 #define PADO_GPU_BUILD_BP_PER_SLOT 1u
-/* Run the batch step as captured hipGraphs (batch prologue + iters_per_sync
- * iterations; then iterations only until the frontier is empty) instead of
- * per-kernel launches.  Same output; per-kernel event timings are not
- * recorded in this mode. */
-#define PADO_GPU_BUILD_GRAPH 2u
+/* The batch step runs by default as captured hipGraphs (batch prologue plus a
+ * block of iterations, then iteration blocks until the frontier is empty).
+ * With this flag it runs as per-kernel launches instead, each push / add
+ * launch bracketed by HIP events (pado_gpu_index_info's ms_push / ms_add).
+ * Same output either way. */
+#define PADO_GPU_BUILD_STREAM 2u
 
 /* ParaVertexCentricPLL<BATCH_SIZE>(const Graph&) -> construct (PARA:309-313,
  * 1740-1895): bit-parallel labeling (PARA:544-840) then batch_process over all

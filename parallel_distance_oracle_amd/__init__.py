@@ -31,7 +31,7 @@ QUERY_DEVICE_PTRS = 1
 QUERY_REF_COMPAT = 2
 QUERY_RANK_SPACE = 4
 BUILD_BP_PER_SLOT = 1
-BUILD_GRAPH = 2
+BUILD_STREAM = 2  # per-kernel launches with HIP-event timing (default: hipGraph batch step)
 SERVE_GRID_STRIDE = 0x10
 WINF = 0xFFFFFFFF  # weighted query: no common hub (unreachable)
 

@@ -51,8 +51,8 @@ extern "C" pado_gpu_status pado_gpu_autotune(pado_gpu_ctx *ctx, const pado_gpu_g
     const std::vector<uint32_t> pushes = {1024, 2048, 4096, 8192};
     const std::vector<uint32_t> adds = {1024, 2048, 4096};
     const std::vector<uint32_t> iters = {2, 4, 8, 16};
-    const std::vector<uint32_t> flags = {0u, PADO_GPU_BUILD_GRAPH, PADO_GPU_BUILD_BP_PER_SLOT,
-                                         PADO_GPU_BUILD_GRAPH | PADO_GPU_BUILD_BP_PER_SLOT};
+    const std::vector<uint32_t> flags = {0u, PADO_GPU_BUILD_STREAM, PADO_GPU_BUILD_BP_PER_SLOT,
+                                         PADO_GPU_BUILD_STREAM | PADO_GPU_BUILD_BP_PER_SLOT};
     auto sweep = [&](uint32_t pado_gpu_build_cfg::*field, const std::vector<uint32_t> &vals) -> pado_gpu_status {
         for (uint32_t v : vals) {
             if (trials >= max_trials) return PADO_GPU_OK;

@@ -758,7 +758,7 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
         ~PinnedFree() { (void)hipHostFree(p); }
     } pin{h_ctr};
 
-    const bool use_graph = (cfg.flags & PADO_GPU_BUILD_GRAPH) != 0;
+    const bool use_graph = (cfg.flags & PADO_GPU_BUILD_STREAM) == 0;
     const uint32_t U = cfg.iters_per_sync ? std::min<uint32_t>(cfg.iters_per_sync, 64u) : 8u;
     const uint32_t push_grid = cfg.push_blocks ? cfg.push_blocks : 4096;
     const uint32_t add_grid = cfg.add_blocks ? cfg.add_blocks : 2048;

@@ -93,7 +93,7 @@ def test_schedule_variants_same_labels(gpu_ctx, scale, graph_mode, iters):
     G, _, _ = rank_space(rmat_edges(scale, 8))
     cfg = pado.BuildCfg()
     cfg.batch_size = 1024
-    cfg.flags = pado.BUILD_GRAPH if graph_mode else 0
+    cfg.flags = 0 if graph_mode else pado.BUILD_STREAM
     cfg.iters_per_sync = iters
     print(f"[gpu] schedule rmat{scale} graph={graph_mode} iters={iters}", flush=True)
     dg = pado.DeviceGraph(gpu_ctx, G)
