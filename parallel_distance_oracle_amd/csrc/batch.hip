@@ -206,48 +206,67 @@ __global__ void k_zero_u64(unsigned long long *p, uint64_t count)
 
 // initialize (PARA:848-1080): non-BP roots become active with their own bit as
 // last group, get the self label (r, 0), and set their own indicator bit.
-__global__ __launch_bounds__(256) void k_batch_init(const uint8_t *used, const uint64_t *off, const uint32_t *adj,
-                                                    LabelStore S, unsigned long long *ind, uint32_t *tmark,
-                                                    uint32_t *tlist, uint32_t *alist, unsigned long long *ag,
-                                                    uint32_t *amask, uint32_t *aend, unsigned long long *chunks,
-                                                    BuildCounters *C)
+// One wave per root, sixteen roots per block.  Root offset ro owns active
+// slot ro and touched-list slot ro (k_batch_begin sets both counts and the
+// frontier size), so the only shared counter left is the push-descriptor
+// count, updated once per block.
+__global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const uint64_t *off, const uint32_t *adj,
+                                                     LabelStore S, unsigned long long *ind, uint32_t *tmark,
+                                                     uint32_t *tlist, uint32_t *alist, unsigned long long *ag,
+                                                     uint32_t *amask, uint32_t *aend, unsigned long long *chunks,
+                                                     BuildCounters *C)
 {
-    const uint32_t lane = lane_id();
+    __shared__ uint32_t s_nch[16];
+    __shared__ uint32_t s_base;
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
     const uint32_t ro = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t rs = C->rs, rsz = C->rsz, gen_batch = C->gen_batch;
-    if (ro >= rsz) return;
     const uint32_t r = rs + ro;
-    if (used[r]) return;
-    uint32_t len = S.len[r];
-    unsigned long long base = store_reserve(S, C, r, len, 1);
-    if (base == ~0ull) return;
-    if (lane == 0) {
-        S.pool[base + len] = r << 8;  // (hub r, dist 0), PARA:932-949
-        S.len[r] = len + 1;
-        ind[(uint64_t)r * kWords + (ro >> 6)] |= 1ull << (ro & 63);
-        if (atomicExch(&tmark[r], gen_batch) != gen_batch) tlist[atomicAdd(&C->n_touched, 1u)] = r;
-        atomicAdd(&C->labels, 1ull);
+    // Every root of the batch (bit-parallel roots included) is on the
+    // touched list; a row that stays empty is cleared at no harm.
+    if (ro < rsz && lane == 0) {
+        tmark[r] = gen_batch;
+        tlist[ro] = r;
     }
-    // The frontier of iteration i is counted in slot i & 1: iteration 1 -> 1.
-    uint32_t k = 0;
-    if (lane == 0) {
-        k = atomicAdd(&C->a_top[1], 1u);
-        atomicAdd(&C->n_act[1], 1u);
+    uint32_t pdeg = 0;
+    if (ro < rsz && !used[r]) {
+        const uint32_t len = S.len[r];
+        const unsigned long long base = store_reserve(S, C, r, len, 1);
+        if (base != ~0ull) {
+            if (lane == 0) {
+                S.pool[base + len] = r << 8;  // (hub r, dist 0), PARA:932-949
+                S.len[r] = len + 1;
+                ind[(uint64_t)r * kWords + (ro >> 6)] |= 1ull << (ro & 63);
+                alist[ro] = r;
+                amask[ro] = 1u << (ro >> 6);
+            }
+            if (lane < kWords) ag[(uint64_t)ro * kWords + lane] = (lane == (ro >> 6)) ? (1ull << (ro & 63)) : 0ull;
+            pdeg = push_degree(off, adj, r, rs);
+            if (lane == 0) aend[ro] = pdeg;
+        }
     }
-    k = __shfl(k, 0);
-    if (lane < kWords) ag[(uint64_t)k * kWords + lane] = (lane == (ro >> 6)) ? (1ull << (ro & 63)) : 0ull;
-    uint32_t pdeg = push_degree(off, adj, r, rs);
-    if (lane == 0) {
-        alist[k] = r;
-        amask[k] = 1u << (ro >> 6);
-        aend[k] = pdeg;
+    const uint32_t nch = (pdeg + kChunk - 1) / kChunk;
+    if (lane == 0) s_nch[wv] = nch;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (uint32_t w = 0; w < blockDim.x / 64; ++w) tot += s_nch[w];
+        // The frontier of iteration i is counted in slot i & 1: iteration 1 -> 1.
+        s_base = tot ? atomicAdd(&C->n_chunk[1], tot) : 0u;
     }
-    emit_push_chunks(k, pdeg, &C->n_chunk[1], chunks);
+    __syncthreads();
+    if (nch == 0) return;
+    uint32_t c0 = s_base;
+    for (uint32_t w = 0; w < wv; ++w) c0 += s_nch[w];
+    for (uint32_t q = lane; q < nch; q += 64) chunks[c0 + q] = ((unsigned long long)(q * kChunk) << 32) | ro;
 }
 
-// Distinct hubs over the labels of the batch roots (for the root-distance table).
+// Distinct hubs over the labels of the batch roots, with their rows of the
+// root-distance table: a hub claimed here gets the next slot, hinfo[h] =
+// slot << 8 | 255 and an all-INF row.  One wave per root; each lane handles
+// four labels per step so the dependent loads and atomics of a step overlap.
 __global__ __launch_bounds__(256) void k_hub_mark(const uint8_t *used, LabelStore S, uint32_t *hinfo,
-                                                  uint32_t *hlist, BuildCounters *C)
+                                                  uint32_t *hlist, uint8_t *dc, BuildCounters *C)
 {
     const uint32_t lane = lane_id();
     const uint32_t ro = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -257,30 +276,52 @@ __global__ __launch_bounds__(256) void k_hub_mark(const uint8_t *used, LabelStor
     if (used[r]) return;
     const uint32_t len = S.len[r];
     const unsigned long long base = S.base[r];
-    for (uint32_t j = lane; j < len; j += 64) {
-        uint32_t h = S.pool[base + j] >> 8;
-        const bool fresh = atomicCAS(&hinfo[h], kEmpty, kClaim) == kEmpty;
-        const uint32_t hi = wave_append(&C->n_hub, fresh);
-        if (fresh) hlist[hi] = h;
+    for (uint32_t j0 = 0; j0 < len; j0 += 256) {
+        uint32_t h[4];
+        bool fresh[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t j = j0 + 64 * u + lane;
+            h[u] = j < len ? S.pool[base + j] >> 8 : kEmpty;
+        }
+        // A hub that reads as taken is taken (no entry returns to empty in
+        // this kernel), so the CAS runs only for hubs not yet seen taken:
+        // the common hubs of all roots would otherwise take one CAS per root.
+#pragma unroll
+        for (int u = 0; u < 4; ++u) fresh[u] = h[u] != kEmpty && hinfo[h[u]] == kEmpty;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) fresh[u] = fresh[u] && atomicCAS(&hinfo[h[u]], kEmpty, kClaim) == kEmpty;
+        // One slot reservation per step for the claims of all four sets.
+        unsigned long long m[4];
+        uint32_t tot = 0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            m[u] = __ballot(fresh[u]);
+            tot += (uint32_t)__popcll(m[u]);
+        }
+        if (tot == 0) continue;
+        uint32_t s0 = 0;
+        if (lane == 0) s0 = atomicAdd(&C->n_hub, tot);
+        s0 = __shfl(s0, 0);
+        const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t slot = s0 + (uint32_t)__popcll(m[u] & below);
+            if (fresh[u]) {
+                hlist[slot] = h[u];
+                hinfo[h[u]] = (slot << 8) | kInf8;
+            }
+            for (unsigned long long mm = m[u]; mm; mm &= mm - 1) {
+                const uint32_t sf = s0 + (uint32_t)__popcll(m[u] & ((1ull << (__ffsll(mm) - 1)) - 1ull));
+                reinterpret_cast<uint4 *>(dc + (uint64_t)sf * kBatch)[lane] = make_uint4(~0u, ~0u, ~0u, ~0u);
+            }
+            s0 += (uint32_t)__popcll(m[u]);
+        }
     }
 }
 
-// Slot assignment: hinfo[h] = slot << 8 | 255, table row = all INF.
-__global__ __launch_bounds__(256) void k_hub_slots(const uint32_t *hlist, uint32_t *hinfo, uint8_t *dc,
-                                                   const BuildCounters *C)
-{
-    const uint32_t nh = C->n_hub;
-    const uint64_t total = (uint64_t)nh * (kBatch / 16);
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-        uint32_t slot = (uint32_t)(i / (kBatch / 16));
-        uint32_t part = (uint32_t)(i % (kBatch / 16));
-        reinterpret_cast<uint4 *>(dc + (uint64_t)slot * kBatch)[part] = make_uint4(~0u, ~0u, ~0u, ~0u);
-        if (part == 0) hinfo[hlist[slot]] = (slot << 8) | kInf8;
-    }
-}
-
-// dist_matrix scatter (PARA:986-1013), transposed: dc[slot(h)][r] = d.
+// dist_matrix scatter (PARA:986-1013), transposed: dc[slot(h)][r] = d, and
+// the per-hub minimum over the batch roots in the low byte of hinfo[h].
 __global__ __launch_bounds__(256) void k_hub_fill(const uint8_t *used, LabelStore S, uint32_t *hinfo, uint8_t *dc,
                                                   const BuildCounters *C)
 {
@@ -292,12 +333,25 @@ __global__ __launch_bounds__(256) void k_hub_fill(const uint8_t *used, LabelStor
     if (used[r]) return;
     const uint32_t len = S.len[r];
     const unsigned long long base = S.base[r];
-    for (uint32_t j = lane; j < len; j += 64) {
-        uint32_t lab = S.pool[base + j];
-        uint32_t h = lab >> 8, d = lab & 255u;
-        uint32_t slot = hinfo[h] >> 8;
-        dc[(uint64_t)slot * kBatch + ro] = (uint8_t)d;
-        atomicMin(&hinfo[h], (slot << 8) | d);
+    for (uint32_t j0 = 0; j0 < len; j0 += 256) {
+        uint32_t lab[4], info[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const uint32_t j = j0 + 64 * u + lane;
+            lab[u] = j < len ? S.pool[base + j] : kEmpty;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) info[u] = lab[u] != kEmpty ? hinfo[lab[u] >> 8] : 0u;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (lab[u] == kEmpty) continue;
+            const uint32_t h = lab[u] >> 8, d = lab[u] & 255u;
+            const uint32_t slot = info[u] >> 8;
+            dc[(uint64_t)slot * kBatch + ro] = (uint8_t)d;
+            // The minimum only falls during the kernel, so a value read as
+            // <= d (even a stale one) makes the atomic a no-op.
+            if (d < (info[u] & 255u)) atomicMin(&hinfo[h], (slot << 8) | d);
+        }
     }
 }
 
@@ -409,21 +463,33 @@ __global__ __launch_bounds__(256) void k_push(uint32_t iter, const uint64_t *off
 }
 
 // Start of a batch (construct's loop over batches, PARA:1791-1812): roots
-// [b * batch, ...) with b taken from the device-side batch counter.
-__global__ void k_batch_begin(BuildCounters *C, uint32_t batch, uint32_t n)
+// [b * batch, ...) with b taken from the device-side batch counter.  One
+// thread per root offset: the block also counts the batch's
+// non-bit-parallel roots, which are the first frontier (active slots and
+// touched-list slots [0, rsz) belong to the root offsets, filled by
+// k_batch_init) and get one self label each.
+__global__ __launch_bounds__(1024) void k_batch_begin(const uint8_t *used, BuildCounters *C, uint32_t batch,
+                                                      uint32_t n)
 {
     const uint32_t b = C->next_batch;
     const uint32_t rs = b * batch;
+    const uint32_t rsz = rs < n ? min(batch, n - rs) : 0u;
+    const uint32_t o = threadIdx.x;
+    const uint32_t cnt = (uint32_t)__syncthreads_count(o < rsz && !used[rs + o]);
+    if (o != 0) return;
     C->next_batch = b + 1;
     C->rs = rs;
-    C->rsz = rs < n ? min(batch, n - rs) : 0u;
+    C->rsz = rsz;
     C->gen_batch = b + 1;
-    C->n_act[0] = C->n_act[1] = 0;
-    C->a_top[0] = C->a_top[1] = 0;
+    C->n_act[0] = 0;
+    C->n_act[1] = cnt;
+    C->a_top[0] = 0;
+    C->a_top[1] = rsz;
     C->n_chunk[0] = C->n_chunk[1] = 0;
     C->n_cand[0] = C->n_cand[1] = 0;
-    C->n_touched = 0;
+    C->n_touched = rsz;
     C->n_hub = 0;
+    C->labels += cnt;
 }
 
 // add (PARA:1568-1665): one wave per candidate vertex v.  Lane l owns roots
@@ -784,11 +850,11 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     auto enqueue_prologue = [&]() {
         // Previous batch's reset_at_end first (a no-op before the first batch).
         hipLaunchKernelGGL(k_batch_end, dim3(1024), dim3(256), 0, st, ind.p, tlist.p, hinfo.p, hlist.p, ctr.p);
-        hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(1), 0, st, ctr.p, batch, n);
-        hipLaunchKernelGGL(k_batch_init, dim3(root_blocks), dim3(256), 0, st, d_used.p, g->off.p, g->adj.p, S, ind.p,
+        hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(batch), 0, st, d_used.p, ctr.p, batch, n);
+        hipLaunchKernelGGL(k_batch_init, dim3((batch + 15) / 16), dim3(1024), 0, st, d_used.p, g->off.p, g->adj.p, S, ind.p,
                            tmark.p, tlist.p, alist.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
-        hipLaunchKernelGGL(k_hub_mark, dim3(root_blocks), dim3(256), 0, st, d_used.p, S, hinfo.p, hlist.p, ctr.p);
-        hipLaunchKernelGGL(k_hub_slots, dim3(2048), dim3(256), 0, st, hlist.p, hinfo.p, dc.p, ctr.p);
+        hipLaunchKernelGGL(k_hub_mark, dim3(root_blocks), dim3(256), 0, st, d_used.p, S, hinfo.p, hlist.p, dc.p,
+                           ctr.p);
         hipLaunchKernelGGL(k_hub_fill, dim3(root_blocks), dim3(256), 0, st, d_used.p, S, hinfo.p, dc.p, ctr.p);
     };
     // Iterations first + 1 .. first + count of the current batch.
