@@ -90,16 +90,17 @@ __device__ inline uint32_t push_degree(const uint64_t *off, const uint32_t *adj,
 }
 
 // Bit-parallel check of PARA:1164-1186 with the reference's unsigned
-// arithmetic; true = pruned.  Per slot the reference tests
-// `td - 2 <= iter` (unsigned, so td < 2 never passes), subtracts 2 or 1 by
-// the S^-1 / S^0 intersections and prunes if the result is <= iter.  As the
-// sets can only lower td, that is: prune if 2 <= td <= iter; prune if
-// td == iter + 1 and S^-1_r meets S^-1_t or S^0_t (or S^0_r meets S^-1_t);
-// prune if td == iter + 2 and S^-1_r meets S^-1_t.  The slots are an OR, so
-// the distance-only test runs first over all 50 slots (no memory traffic)
-// and the 16-byte set pairs are loaded only for slots at iter + 1 / iter + 2.
-__device__ inline bool bp_prunes(const uint32_t (&dr)[13], const uint32_t (&dt)[13], const ulonglong2 *bps,
-                                 uint32_t r, uint32_t t, uint32_t iter)
+// arithmetic.  Per slot the reference tests `td - 2 <= iter` (unsigned, so
+// td < 2 never passes), subtracts 2 or 1 by the S^-1 / S^0 intersections and
+// prunes if the result is <= iter.  As the sets can only lower td, that is:
+// prune if 2 <= td <= iter; prune if td == iter + 1 and S^-1_r meets S^-1_t
+// or S^0_t (or S^0_r meets S^-1_t); prune if td == iter + 2 and S^-1_r meets
+// S^-1_t.  The slots are an OR, so the check runs in two phases: this scan
+// of the distance lines (no memory traffic) settles the first case and
+// returns the slots at iter + 1 / iter + 2 (`cand`, with `one` marking
+// iter + 1) whose set pairs the second phase loads.
+__device__ inline bool bp_dist_scan(const uint32_t (&dr)[13], const uint32_t (&dt)[13], uint32_t iter,
+                                    unsigned long long &cand, unsigned long long &one)
 {
     // Four slots per 32-bit word, as 16-bit lanes: lo = slots 4w and 4w+2,
     // hi = slots 4w+1 and 4w+3 (td <= 510).  (0x8000 | c) - td keeps bit 15
@@ -108,29 +109,165 @@ __device__ inline bool bp_prunes(const uint32_t (&dr)[13], const uint32_t (&dt)[
     const uint32_t c_it = 0x80008000u | (iter * 0x00010001u);
     const uint32_t c_it2 = 0x80008000u | ((iter + 2u) * 0x00010001u);
     const uint32_t c_1 = 0x80018001u;
+    uint32_t hit = 0;
+    cand = 0;
+    one = 0;
 #pragma unroll
     for (int w = 0; w < 13; ++w) {
         const uint32_t valid = w == 12 ? 0x00008000u : 0x80008000u;
         const uint32_t lo = (dr[w] & 0x00FF00FFu) + (dt[w] & 0x00FF00FFu);
         const uint32_t hi = ((dr[w] >> 8) & 0x00FF00FFu) + ((dt[w] >> 8) & 0x00FF00FFu);
         const uint32_t lo_it = c_it - lo, hi_it = c_it - hi;
-        if (((lo_it & ~(c_1 - lo)) | (hi_it & ~(c_1 - hi))) & valid) return true;  // 2 <= td <= iter
+        hit |= ((lo_it & ~(c_1 - lo)) | (hi_it & ~(c_1 - hi))) & valid;  // 2 <= td <= iter
         const uint32_t lo_n = (c_it2 - lo) & ~lo_it & valid, hi_n = (c_it2 - hi) & ~hi_it & valid;
         const uint32_t lo_1 = (c_it + 0x00010001u - lo) & ~lo_it, hi_1 = (c_it + 0x00010001u - hi) & ~hi_it;
-        uint32_t nib = ((lo_n >> 15) & 1u) | ((hi_n >> 14) & 2u) | ((lo_n >> 29) & 4u) | ((hi_n >> 28) & 8u);
-        const uint32_t one = ((lo_1 >> 15) & 1u) | ((hi_1 >> 14) & 2u) | ((lo_1 >> 29) & 4u) | ((hi_1 >> 28) & 8u);
-        // td == iter + 1 (bit in `one`) or iter + 2: the sets decide (rare)
-        while (nib) {
-            const uint32_t q = __ffs(nib) - 1;
-            nib &= nib - 1;
-            const uint32_t i = 4 * w + q;
-            const ulonglong2 sr = bps[(uint64_t)r * kBP + i];
-            const ulonglong2 st = bps[(uint64_t)t * kBP + i];
-            if (sr.x & st.x) return true;
-            if (((one >> q) & 1u) && ((sr.x & st.y) | (sr.y & st.x))) return true;
+        const uint32_t nib = ((lo_n >> 15) & 1u) | ((hi_n >> 14) & 2u) | ((lo_n >> 29) & 4u) | ((hi_n >> 28) & 8u);
+        const uint32_t o = ((lo_1 >> 15) & 1u) | ((hi_1 >> 14) & 2u) | ((lo_1 >> 29) & 4u) | ((hi_1 >> 28) & 8u);
+        cand |= (unsigned long long)nib << (4 * w);
+        one |= (unsigned long long)(o & nib) << (4 * w);
+    }
+    return hit != 0;
+}
+
+// Inclusive prefix sum over the 64 lanes of a wave.
+__device__ inline uint32_t wave_incl_scan(uint32_t x)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    return x;
+}
+
+// Orders LDS writes of some lanes of a wave before reads by other lanes of
+// the same wave.
+__device__ inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Position of the rank-th set bit of m (rank < popcount(m)).
+__device__ inline uint32_t nth_set_bit(unsigned long long m, uint32_t rank)
+{
+    uint32_t pos = 0;
+    uint32_t c = __popc((uint32_t)m);
+    if (rank >= c) {
+        rank -= c;
+        m >>= 32;
+        pos = 32;
+    }
+    uint32_t x = (uint32_t)m;
+    c = __popc(x & 0xFFFFu);
+    if (rank >= c) {
+        rank -= c;
+        x >>= 16;
+        pos += 16;
+    }
+    c = __popc(x & 0xFFu);
+    if (rank >= c) {
+        rank -= c;
+        x >>= 8;
+        pos += 8;
+    }
+    c = __popc(x & 0xFu);
+    if (rank >= c) {
+        rank -= c;
+        x >>= 4;
+        pos += 4;
+    }
+    c = __popc(x & 0x3u);
+    if (rank >= c) {
+        rank -= c;
+        x >>= 2;
+        pos += 2;
+    }
+    if (rank >= (x & 1u)) pos += 1;
+    return pos;
+}
+
+// Last lane o whose exclusive prefix ex[o] <= x (x below the total): the
+// lane that owns flattened item x (it has at least one item).
+__device__ inline uint32_t item_owner(const uint32_t *ex, uint32_t x)
+{
+    uint32_t o = 0;
+#pragma unroll
+    for (uint32_t sft = 32; sft; sft >>= 1)
+        if (ex[o + sft] <= x) o += sft;
+    return o;
+}
+
+// One set-pair test: slot i, td == iter + 2 (S^-1_r meets S^-1_t) or, with
+// `one`, td == iter + 1 (also S^-1 against S^0 either way).
+__device__ inline bool bp_set_hit(ulonglong2 r, ulonglong2 t, bool one)
+{
+    return (r.x & t.x) != 0ull || (one && ((r.x & t.y) | (r.y & t.x)) != 0ull);
+}
+
+// LDS scratch of one wave for the set phase.
+struct SetScratch {
+    unsigned long long *rem, *one;
+    uint32_t *ro, *t, *ex, *hit;
+};
+
+// Second phase of the bit-parallel check, across the 64 lanes of a wave
+// (every lane calls it; rem = 0 for a lane with nothing to test).  Each lane
+// first tests its two lowest candidate slots, which settles most pruned
+// pairs; the (pair, slot) tests left over, mostly those of pairs that pass,
+// are then spread over the wave 64 at a time, so a pair with dozens of
+// candidate slots costs a few wave steps instead of dozens.  Root set pairs
+// come from the batch's slot-major table (consecutive roots are adjacent),
+// tail set pairs from the vertex-major table.  Returns whether the lane's
+// pair is pruned.
+__device__ inline bool bp_set_phase(unsigned long long rem, unsigned long long one, const ulonglong2 *rset,
+                                    uint32_t ro, const ulonglong2 *bps, uint32_t t, const SetScratch &sc)
+{
+    const uint32_t lane = lane_id();
+    bool pr = false;
+    if (rem) {
+        const uint32_t i0 = (uint32_t)__ffsll(rem) - 1;
+        rem &= rem - 1;
+        const bool two = rem != 0ull;
+        const uint32_t i1 = two ? (uint32_t)__ffsll(rem) - 1 : i0;
+        if (two) rem &= rem - 1;
+        const ulonglong2 r0 = rset[i0 * kBatch + ro], t0 = bps[(uint64_t)t * kBP + i0];
+        ulonglong2 r1 = make_ulonglong2(0ull, 0ull), t1 = r1;
+        if (two) {
+            r1 = rset[i1 * kBatch + ro];
+            t1 = bps[(uint64_t)t * kBP + i1];
+        }
+        if (bp_set_hit(r0, t0, (one >> i0) & 1ull) || (two && bp_set_hit(r1, t1, (one >> i1) & 1ull))) {
+            pr = true;
+            rem = 0ull;
         }
     }
-    return false;
+    const uint32_t cnt = (uint32_t)__popcll(rem);
+    const uint32_t incl = wave_incl_scan(cnt);
+    const uint32_t total = __shfl(incl, 63);
+    if (total == 0) return pr;
+    sc.rem[lane] = rem;
+    sc.one[lane] = one;
+    sc.ro[lane] = ro;
+    sc.t[lane] = t;
+    sc.ex[lane] = incl - cnt;
+    sc.hit[lane] = 0u;
+    wave_lds_sync();
+    for (uint32_t x0 = 0; x0 < total; x0 += 64) {
+        const uint32_t x = x0 + lane;
+        if (x >= total) continue;
+        const uint32_t o = item_owner(sc.ex, x);
+        const unsigned long long ro_rem = sc.rem[o];
+        const uint32_t i = nth_set_bit(ro_rem, x - sc.ex[o]);
+        const ulonglong2 rr = rset[i * kBatch + sc.ro[o]], tt = bps[(uint64_t)sc.t[o] * kBP + i];
+        if (bp_set_hit(rr, tt, (sc.one[o] >> i) & 1ull)) atomicOr(&sc.hit[o], 1u);
+    }
+    wave_lds_sync();
+    pr = pr || sc.hit[lane] != 0u;
+    // The scratch is rewritten by the next call.
+    wave_lds_sync();
+    return pr;
 }
 
 __device__ inline void load_dline(const uint8_t *bpd, uint32_t v, uint32_t (&d)[13])
@@ -179,19 +316,6 @@ __device__ inline unsigned long long store_reserve(LabelStore S, BuildCounters *
     return nb;
 }
 
-// Append the pushes of active index k: chunk descriptors over its first
-// `pdeg` neighbours (wave-uniform call).
-__device__ inline void emit_push_chunks(uint32_t k, uint32_t pdeg, uint32_t *n_chunk, unsigned long long *chunks)
-{
-    const uint32_t lane = lane_id();
-    uint32_t nch = (pdeg + kChunk - 1) / kChunk;
-    if (nch == 0) return;
-    uint32_t c0 = 0;
-    if (lane == 0) c0 = atomicAdd(n_chunk, nch);
-    c0 = __shfl(c0, 0);
-    for (uint32_t q = lane; q < nch; q += 64) chunks[c0 + q] = ((unsigned long long)(q * kChunk) << 32) | k;
-}
-
 __global__ void k_fill_u32(uint32_t *p, uint64_t count, uint32_t val)
 {
     uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -211,10 +335,10 @@ __global__ void k_zero_u64(unsigned long long *p, uint64_t count)
 // frontier size), so the only shared counter left is the push-descriptor
 // count, updated once per block.
 __global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const uint64_t *off, const uint32_t *adj,
-                                                     LabelStore S, unsigned long long *ind, uint32_t *tmark,
+                                                     LabelStore S, unsigned long long *ind, unsigned long long *vmark,
                                                      uint32_t *tlist, uint32_t *alist, unsigned long long *ag,
                                                      uint32_t *amask, uint32_t *aend, unsigned long long *chunks,
-                                                     BuildCounters *C)
+                                                     const ulonglong2 *bps, ulonglong2 *rset, BuildCounters *C)
 {
     __shared__ uint32_t s_nch[16];
     __shared__ uint32_t s_base;
@@ -225,9 +349,11 @@ __global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const 
     // Every root of the batch (bit-parallel roots included) is on the
     // touched list; a row that stays empty is cleared at no harm.
     if (ro < rsz && lane == 0) {
-        tmark[r] = gen_batch;
+        vmark[r] = (unsigned long long)gen_batch << 17;
         tlist[ro] = r;
     }
+    // Slot-major copy of the batch roots' bit-parallel sets (push's set phase).
+    if (ro < rsz && lane < kBP) rset[lane * kBatch + ro] = bps[(uint64_t)r * kBP + lane];
     uint32_t pdeg = 0;
     if (ro < rsz && !used[r]) {
         const uint32_t len = S.len[r];
@@ -355,18 +481,36 @@ __global__ __launch_bounds__(256) void k_hub_fill(const uint8_t *used, LabelStor
     }
 }
 
-// push (PARA:1486-1559 / push_labels PARA:1084-1211), one quarter-wave per
-// 16-edge chunk of an active vertex's tails >= roots_start.
-__global__ __launch_bounds__(256) void k_push(uint32_t iter, const uint64_t *off, const uint32_t *adj,
+constexpr uint32_t kPushWaves = 4;  // waves per k_push block
+
+// push (PARA:1486-1559 / push_labels PARA:1084-1211).  A wave takes four
+// 16-edge chunks (one edge per lane) and walks the root words of its edges'
+// last distance groups in step: per word, each lane test-and-sets its
+// indicator word, and the fresh (tail, root) pairs of all 64 lanes are then
+// checked against the bit-parallel labels 64 at a time, one pair per lane,
+// so a tail with many fresh roots does not hold up the wave.  The tail's
+// distance line is staged in LDS by the lane that owns the edge.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_push(uint32_t iter, const uint64_t *off, const uint32_t *adj,
                                               const uint32_t *alist, const unsigned long long *ag,
                                               const uint32_t *amask, const uint32_t *aend,
                                               const unsigned long long *chunks, unsigned long long *ind,
-                                              unsigned long long *cand, uint32_t *tmark, uint32_t *tlist,
-                                              unsigned long long *cmark, uint32_t *clist, unsigned long long *imark,
-                                              const uint8_t *bpd, const ulonglong2 *bps, BuildCounters *C)
+                                              unsigned long long *cand, unsigned long long *vmark, uint32_t *tlist,
+                                              uint32_t *clist,
+                                              const uint8_t *bpd, const ulonglong2 *bps, const ulonglong2 *rset,
+                                              BuildCounters *C)
 {
-    const uint32_t lane = threadIdx.x & (kChunk - 1);
-    const uint32_t ngroups = (gridDim.x * blockDim.x) / kChunk;
+    __shared__ uint4 s_dt[kPushWaves][64][4];              // tail distance lines (52 of 64 bytes used)
+    __shared__ unsigned long long s_nb[kPushWaves][64];    // fresh roots of each lane's edge, this word
+    __shared__ unsigned long long s_pass[kPushWaves][64];  // of those, the ones the BP check keeps
+    __shared__ uint32_t s_ex[kPushWaves][64];              // exclusive prefix of the fresh counts
+    __shared__ uint32_t s_t[kPushWaves][64];               // each lane's tail
+    // set-phase scratch (bp_set_phase)
+    __shared__ unsigned long long s_srem[kPushWaves][64], s_sone[kPushWaves][64];
+    __shared__ uint32_t s_sro[kPushWaves][64], s_st[kPushWaves][64], s_sex[kPushWaves][64], s_shit[kPushWaves][64];
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+    const uint32_t wave = blockIdx.x * kPushWaves + wv;
+    const uint32_t nwaves = gridDim.x * kPushWaves;
+    const SetScratch sc{s_srem[wv], s_sone[wv], s_sro[wv], s_st[wv], s_sex[wv], s_shit[wv]};
     // Iteration `iter` of batch_process's loop (PARA:1478-1670) runs if the
     // previous one left active vertices (frontier slot iter & 1); past the
     // batch's end the launch is empty.  Iteration 255 would leave the
@@ -391,74 +535,137 @@ __global__ __launch_bounds__(256) void k_push(uint32_t iter, const uint64_t *off
             }
         }
     }
-    unsigned long long n_push = 0, n_pass = 0, n_bp = 0, n_cand = 0, n_edge = 0, n_tail = 0;
-    for (uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) / kChunk; c < nc; c += ngroups) {
-        const unsigned long long desc = chunks[c];
-        if (desc == kChunkPad) continue;  // reserved slot without work
-        const uint32_t k = (uint32_t)desc;
-        const uint32_t e = (uint32_t)(desc >> 32) + lane;
-        if (e >= aend[k]) continue;
-        ++n_edge;
-        const uint32_t v = alist[k];
-        const uint32_t t = adj[off[v] + e];
-        const uint32_t lim = t - rs;  // roots with offset < lim satisfy root < tail (PARA:1134)
+    // Per-lane statistics of one launch fit 32 bits; summed in 64 at the end.
+    uint32_t n_push = 0, n_pass = 0, n_bp = 0, n_cand = 0, n_edge = 0, n_tail = 0;
+    for (uint32_t c0 = wave * 4; c0 < nc; c0 += nwaves * 4) {
+        const uint32_t c = c0 + (lane >> 4);
+        const unsigned long long desc = c < nc ? chunks[c] : kChunkPad;
+        uint32_t k = 0, t = 0, lim = 0, words = 0;
+        if (desc != kChunkPad) {
+            k = (uint32_t)desc;
+            const uint32_t e = (uint32_t)(desc >> 32) + (lane & (kChunk - 1));
+            if (e < aend[k]) {
+                ++n_edge;
+                const uint32_t v = alist[k];
+                t = adj[off[v] + e];
+                lim = t - rs;  // roots with offset < lim satisfy root < tail (PARA:1134)
+                // words w with w * 64 < lim
+                words = amask[k] & (lim >= kBatch ? 0xFFFFu : ((1u << ((lim + 63) / 64)) - 1u));
+            }
+        }
+        s_t[wv][lane] = t;
+        uint32_t wall = words;
+        for (int o = 32; o > 0; o >>= 1) wall |= __shfl_xor(wall, o);
         bool touched = false, got = false, tline = false;
-        uint32_t dt[13];
-        for (uint32_t wm = amask[k]; wm; wm &= wm - 1) {
-            const uint32_t w = __ffs(wm) - 1;
-            if (lim <= w * 64) break;
-            unsigned long long g = ag[(uint64_t)k * kWords + w];
-            if (lim < (w + 1) * 64) g &= (1ull << (lim - w * 64)) - 1ull;
-            if (!g) continue;
-            n_push += __popcll(g);
-            const unsigned long long old = atomicOr(&ind[(uint64_t)t * kWords + w], g);
-            unsigned long long nb = g & ~old;
-            if (!nb) continue;
-            touched = true;
-            n_pass += __popcll(nb);
-            if (!tline) {
-                load_dline(bpd, t, dt);
-                tline = true;
+        while (wall) {
+            const uint32_t w = __ffs(wall) - 1;
+            wall &= wall - 1;
+            unsigned long long nb = 0;
+            if ((words >> w) & 1u) {
+                unsigned long long g = ag[(uint64_t)k * kWords + w];
+                if (lim < (w + 1) * 64) g &= (1ull << (lim - w * 64)) - 1ull;
+                if (g) {
+                    n_push += __popcll(g);
+                    // Bits are only set during the kernel, so bits read as set
+                    // are set; the test-and-set runs only if some look fresh.
+                    unsigned long long *iw = &ind[(uint64_t)t * kWords + w];
+                    if (g & ~*iw) nb = g & ~atomicOr(iw, g);
+                }
             }
-            unsigned long long pass = 0;
-            while (nb) {
-                const uint32_t b = __ffsll(nb) - 1;
-                nb &= nb - 1;
-                const uint32_t r = rs + w * 64 + b;
-                uint32_t dr[13];
-                load_dline(bpd, r, dr);
-                if (bp_prunes(dr, dt, bps, r, t, iter))
-                    ++n_bp;
-                else
-                    pass |= 1ull << b;
+            if (nb) {
+                touched = true;
+                n_pass += __popcll(nb);
+                if (!tline) {
+                    const uint4 *src = reinterpret_cast<const uint4 *>(bpd + (uint64_t)t * kBpdStride);
+                    s_dt[wv][lane][0] = src[0];
+                    s_dt[wv][lane][1] = src[1];
+                    s_dt[wv][lane][2] = src[2];
+                    s_dt[wv][lane][3] = src[3];
+                    tline = true;
+                }
             }
+            const uint32_t cnt = (uint32_t)__popcll(nb);
+            const uint32_t incl = wave_incl_scan(cnt);
+            const uint32_t total = __shfl(incl, 63);
+            if (total == 0) continue;
+            s_nb[wv][lane] = nb;
+            s_ex[wv][lane] = incl - cnt;
+            s_pass[wv][lane] = 0ull;
+            wave_lds_sync();
+            // Fresh pair x of the wave: owner lane o (the last lane whose
+            // prefix is <= x; it has a fresh root), root = its (x - prefix)-th.
+            // All 64 lanes run every pass (the set phase works across the
+            // wave); lanes past the last pair carry nothing.
+            for (uint32_t x0 = 0; x0 < total; x0 += 64) {
+                const uint32_t x = x0 + lane;
+                const bool act = x < total;
+                uint32_t o = 0, b = 0, to = 0;
+                unsigned long long cs = 0ull, one = 0ull;
+                bool dist_pr = true;
+                if (act) {
+                    o = item_owner(s_ex[wv], x);
+                    b = nth_set_bit(s_nb[wv][o], x - s_ex[wv][o]);
+                    to = s_t[wv][o];
+                    uint32_t dr[13], dt[13];
+                    load_dline(bpd, rs + w * 64 + b, dr);
+                    const uint4 q0 = s_dt[wv][o][0], q1 = s_dt[wv][o][1], q2 = s_dt[wv][o][2], q3 = s_dt[wv][o][3];
+                    dt[0] = q0.x; dt[1] = q0.y; dt[2] = q0.z; dt[3] = q0.w;
+                    dt[4] = q1.x; dt[5] = q1.y; dt[6] = q1.z; dt[7] = q1.w;
+                    dt[8] = q2.x; dt[9] = q2.y; dt[10] = q2.z; dt[11] = q2.w;
+                    dt[12] = q3.x;
+                    dist_pr = bp_dist_scan(dr, dt, iter, cs, one);
+                }
+                const bool set_pr = bp_set_phase(dist_pr ? 0ull : cs, one, rset, w * 64 + b, bps, to, sc);
+                if (act) {
+                    if (dist_pr || set_pr)
+                        ++n_bp;
+                    else
+                        atomicOr(&s_pass[wv][o], 1ull << b);
+                }
+            }
+            wave_lds_sync();
+            const unsigned long long pass = s_pass[wv][lane];
             if (pass) {
                 atomicOr(&cand[(uint64_t)t * kWords + w], pass);
                 n_cand += __popcll(pass);
                 got = true;
             }
+            // The next word's writes must not overtake this word's reads.
+            wave_lds_sync();
         }
-        const bool new_t = touched && tmark[t] != gen_batch && atomicExch(&tmark[t], gen_batch) != gen_batch;
-        if (touched && imark[t] != gen_iter && atomicExch(&imark[t], gen_iter) != gen_iter) ++n_tail;
-        const bool new_c = got && cmark[t] != gen_iter && atomicExch(&cmark[t], gen_iter) != gen_iter;
+        // vmark[t] = (batch << 16 | iteration) << 1 | candidate, raised with
+        // atomicMax: one atomic tells whether t is new to the batch's touched
+        // list, new to this iteration, and new to its candidate list.  The
+        // plain read (never above the true value) skips the atomic when
+        // nothing would change.
+        bool new_t = false, new_c = false;
+        if (touched) {
+            const unsigned long long want = (gen_iter << 1) | (got ? 1ull : 0ull);
+            unsigned long long prev = vmark[t];
+            if (prev < want) {
+                prev = atomicMax(&vmark[t], want);
+                new_t = (prev >> 17) != gen_batch;
+                if ((prev >> 1) != gen_iter) ++n_tail;
+                new_c = got && prev < want;
+            }
+        }
         const uint32_t ti = wave_append(&C->n_touched, new_t);
         if (new_t) tlist[ti] = t;
         const uint32_t ci = wave_append(&C->n_cand[p], new_c);
         if (new_c) clist[ci] = t;
+        // s_t is rewritten by the next step.
+        wave_lds_sync();
     }
-    n_push = wave_sum_u64(n_push);
-    n_pass = wave_sum_u64(n_pass);
-    n_bp = wave_sum_u64(n_bp);
-    n_cand = wave_sum_u64(n_cand);
-    n_edge = wave_sum_u64(n_edge);
-    n_tail = wave_sum_u64(n_tail);
-    if ((threadIdx.x & 63) == 0) {
-        if (n_edge) atomicAdd(&C->edges, n_edge);
-        if (n_tail) atomicAdd(&C->tails, n_tail);
-        if (n_push) atomicAdd(&C->pushes, n_push);
-        if (n_pass) atomicAdd(&C->ind_pass, n_pass);
-        if (n_bp) atomicAdd(&C->bp_pruned, n_bp);
-        if (n_cand) atomicAdd(&C->cands, n_cand);
+    const unsigned long long t_push = wave_sum_u64(n_push), t_pass = wave_sum_u64(n_pass);
+    const unsigned long long t_bp = wave_sum_u64(n_bp), t_cand = wave_sum_u64(n_cand);
+    const unsigned long long t_edge = wave_sum_u64(n_edge), t_tail = wave_sum_u64(n_tail);
+    if (lane == 0) {
+        if (t_edge) atomicAdd(&C->edges, t_edge);
+        if (t_tail) atomicAdd(&C->tails, t_tail);
+        if (t_push) atomicAdd(&C->pushes, t_push);
+        if (t_pass) atomicAdd(&C->ind_pass, t_pass);
+        if (t_bp) atomicAdd(&C->bp_pruned, t_bp);
+        if (t_cand) atomicAdd(&C->cands, t_cand);
     }
 }
 
@@ -766,11 +973,12 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     }
 
     DevBuf<unsigned long long> ind, cand, ag, chunks, lbase;
-    DevBuf<uint32_t> tmark, tlist, clist, alist, amask, aend, hinfo, hlist, pool, llen, lcap;
-    DevBuf<unsigned long long> cmark, imark;  // (batch << 16 | iteration) of the last mark
+    DevBuf<uint32_t> tlist, clist, alist, amask, aend, hinfo, hlist, pool, llen, lcap;
+    DevBuf<unsigned long long> vmark;  // ((batch << 16 | iteration) << 1 | candidate) of the last mark
     DevBuf<uint8_t> dc;
+    DevBuf<ulonglong2> rset;  // batch roots' bit-parallel sets, slot-major [kBP][kBatch]
     DevBuf<BuildCounters> ctr;
-    PADO_TRY(imark.alloc(n));
+    PADO_TRY(vmark.alloc(n));
     PADO_TRY(ind.alloc((uint64_t)n * kWords));
     PADO_TRY(cand.alloc((uint64_t)n * kWords));
     // Per-wave reservations in k_add leave holes: the active list can exceed
@@ -780,9 +988,7 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     const uint64_t alist_cap = (uint64_t)n + 16 * add_waves;
     PADO_TRY(ag.alloc(alist_cap * kWords));
     PADO_TRY(chunks.alloc(2 * (g->m / kChunk + (uint64_t)n) + 64 * add_waves + 64));
-    PADO_TRY(tmark.alloc(n));
     PADO_TRY(tlist.alloc(n));
-    PADO_TRY(cmark.alloc(n));
     PADO_TRY(clist.alloc(n));
     PADO_TRY(alist.alloc(alist_cap));
     PADO_TRY(amask.alloc(alist_cap));
@@ -792,6 +998,7 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     PADO_TRY(lbase.alloc(n));
     PADO_TRY(llen.alloc(n));
     PADO_TRY(lcap.alloc(n));
+    PADO_TRY(rset.alloc((uint64_t)kBP * kBatch));
     PADO_TRY(ctr.alloc(1));
     // The label pool starts at 64 words per vertex and doubles whenever a
     // host check finds it over 40% full; the checks come every
@@ -807,9 +1014,7 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     const uint32_t fg = 4096;
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, ind.p, (uint64_t)n * kWords);
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, cand.p, (uint64_t)n * kWords);
-    hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, tmark.p, (uint64_t)n, 0u);
-    hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, cmark.p, (uint64_t)n);
-    hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, imark.p, (uint64_t)n);
+    hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, vmark.p, (uint64_t)n);
     hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, hinfo.p, (uint64_t)n, kEmpty);
     hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, llen.p, (uint64_t)n, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, lcap.p, (uint64_t)n, 0u);
@@ -852,7 +1057,8 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
         hipLaunchKernelGGL(k_batch_end, dim3(1024), dim3(256), 0, st, ind.p, tlist.p, hinfo.p, hlist.p, ctr.p);
         hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(batch), 0, st, d_used.p, ctr.p, batch, n);
         hipLaunchKernelGGL(k_batch_init, dim3((batch + 15) / 16), dim3(1024), 0, st, d_used.p, g->off.p, g->adj.p, S, ind.p,
-                           tmark.p, tlist.p, alist.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
+                           vmark.p, tlist.p, alist.p, ag.p, amask.p, aend.p, chunks.p,
+                           (const ulonglong2 *)idx->bps.p, rset.p, ctr.p);
         hipLaunchKernelGGL(k_hub_mark, dim3(root_blocks), dim3(256), 0, st, d_used.p, S, hinfo.p, hlist.p, dc.p,
                            ctr.p);
         hipLaunchKernelGGL(k_hub_fill, dim3(root_blocks), dim3(256), 0, st, d_used.p, S, hinfo.p, dc.p, ctr.p);
@@ -863,8 +1069,8 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
             const uint32_t iter = first + 1 + u;
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u], st));
             hipLaunchKernelGGL(k_push, dim3(push_grid), dim3(256), 0, st, iter, g->off.p, g->adj.p, alist.p, ag.p,
-                               amask.p, aend.p, chunks.p, ind.p, cand.p, tmark.p, tlist.p, cmark.p, clist.p, imark.p,
-                               idx->bpd.p, (const ulonglong2 *)idx->bps.p, ctr.p);
+                               amask.p, aend.p, chunks.p, ind.p, cand.p, vmark.p, tlist.p, clist.p,
+                               idx->bpd.p, (const ulonglong2 *)idx->bps.p, rset.p, ctr.p);
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 1], st));
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 2], st));
             hipLaunchKernelGGL(k_add, dim3(add_grid), dim3(256), 0, st, iter, d_used.p, g->off.p, g->adj.p, clist.p,
