@@ -129,6 +129,61 @@ __device__ inline bool bp_dist_scan(const uint32_t (&dr)[13], const uint32_t (&d
     return hit != 0;
 }
 
+// Level masks of a vertex's bit-parallel distances: bit i of word k is set
+// iff slot i is at distance k (k < 8; larger distances and INF are in no
+// word).  For iter <= kMaskIters every td the check can act on is at most
+// iter + 2 <= 7, so the scan above becomes a few 64-bit ANDs: td == a + b for
+// the slots in L_r[a] & L_t[b].
+constexpr uint32_t kMaskIters = 5;
+
+template <int IT>
+__device__ inline bool bp_mask_scan_it(const unsigned long long *rm, uint32_t ro, const unsigned long long *lt,
+                                       unsigned long long &cand, unsigned long long &one)
+{
+    // Levels above IT + 2 never matter: only those below are loaded.
+    constexpr int L = IT + 3 < 8 ? IT + 3 : 8;
+    unsigned long long lr[L], tl[L];
+#pragma unroll
+    for (int a = 0; a < L; ++a) {
+        lr[a] = rm[a * kBatch + ro];
+        tl[a] = lt[a];
+    }
+    unsigned long long hit = 0, e1 = 0, e2 = 0;
+#pragma unroll
+    for (int a = 0; a < L; ++a) {
+#pragma unroll
+        for (int b = 0; b < L; ++b) {
+            const int td = a + b;
+            if (td >= 2 && td <= IT)
+                hit |= lr[a] & tl[b];
+            else if (td == IT + 1)
+                e1 |= lr[a] & tl[b];
+            else if (td == IT + 2)
+                e2 |= lr[a] & tl[b];
+        }
+    }
+    cand = e1 | e2;
+    one = e1;
+    return hit != 0ull;
+}
+
+// Same contract as bp_dist_scan for 1 <= iter <= kMaskIters (wave-uniform):
+// root offset ro's masks come from the batch's level-major root table rm
+// (rm[k * kBatch + ro]: lanes holding consecutive roots read consecutive
+// words), the tail's from its 8-word line lt.
+__device__ inline bool bp_mask_scan(uint32_t iter, const unsigned long long *rm, uint32_t ro,
+                                    const unsigned long long *lt, unsigned long long &cand,
+                                    unsigned long long &one)
+{
+    switch (iter) {
+    case 1: return bp_mask_scan_it<1>(rm, ro, lt, cand, one);
+    case 2: return bp_mask_scan_it<2>(rm, ro, lt, cand, one);
+    case 3: return bp_mask_scan_it<3>(rm, ro, lt, cand, one);
+    case 4: return bp_mask_scan_it<4>(rm, ro, lt, cand, one);
+    default: return bp_mask_scan_it<5>(rm, ro, lt, cand, one);
+    }
+}
+
 // Inclusive prefix sum over the 64 lanes of a wave.
 __device__ inline uint32_t wave_incl_scan(uint32_t x)
 {
@@ -209,7 +264,7 @@ __device__ inline bool bp_set_hit(ulonglong2 r, ulonglong2 t, bool one)
 // LDS scratch of one wave for the set phase.
 struct SetScratch {
     unsigned long long *rem, *one;
-    uint32_t *ro, *t, *ex, *hit;
+    uint32_t *r, *t, *ex, *hit;
 };
 
 // Second phase of the bit-parallel check, across the 64 lanes of a wave
@@ -217,12 +272,13 @@ struct SetScratch {
 // first tests its two lowest candidate slots, which settles most pruned
 // pairs; the (pair, slot) tests left over, mostly those of pairs that pass,
 // are then spread over the wave 64 at a time, so a pair with dozens of
-// candidate slots costs a few wave steps instead of dozens.  Root set pairs
-// come from the batch's slot-major table (consecutive roots are adjacent),
-// tail set pairs from the vertex-major table.  Returns whether the lane's
-// pair is pruned.
+// candidate slots costs a few wave steps instead of dozens.  In the first
+// step lanes hold different roots, so root set pairs come from the batch's
+// slot-major table (consecutive roots are adjacent); in the spread steps
+// lanes hold slots of the same few pairs, so both sides come from the
+// vertex-major table.  Returns whether the lane's pair is pruned.
 __device__ inline bool bp_set_phase(unsigned long long rem, unsigned long long one, const ulonglong2 *rset,
-                                    uint32_t ro, const ulonglong2 *bps, uint32_t t, const SetScratch &sc)
+                                    uint32_t ro, uint32_t rs, const ulonglong2 *bps, uint32_t t, const SetScratch &sc)
 {
     const uint32_t lane = lane_id();
     bool pr = false;
@@ -249,7 +305,7 @@ __device__ inline bool bp_set_phase(unsigned long long rem, unsigned long long o
     if (total == 0) return pr;
     sc.rem[lane] = rem;
     sc.one[lane] = one;
-    sc.ro[lane] = ro;
+    sc.r[lane] = rs + ro;
     sc.t[lane] = t;
     sc.ex[lane] = incl - cnt;
     sc.hit[lane] = 0u;
@@ -260,7 +316,9 @@ __device__ inline bool bp_set_phase(unsigned long long rem, unsigned long long o
         const uint32_t o = item_owner(sc.ex, x);
         const unsigned long long ro_rem = sc.rem[o];
         const uint32_t i = nth_set_bit(ro_rem, x - sc.ex[o]);
-        const ulonglong2 rr = rset[i * kBatch + sc.ro[o]], tt = bps[(uint64_t)sc.t[o] * kBP + i];
+        // Items of one pair are slots of one root: its vertex-major record
+        // keeps their loads together.
+        const ulonglong2 rr = bps[(uint64_t)sc.r[o] * kBP + i], tt = bps[(uint64_t)sc.t[o] * kBP + i];
         if (bp_set_hit(rr, tt, (sc.one[o] >> i) & 1ull)) atomicOr(&sc.hit[o], 1u);
     }
     wave_lds_sync();
@@ -328,6 +386,29 @@ __global__ void k_zero_u64(unsigned long long *p, uint64_t count)
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = 0ull;
 }
 
+// Level masks (see bp_mask_scan) of every vertex, 64 bytes per vertex.
+__global__ __launch_bounds__(256) void k_bp_masks(uint32_t n, const uint8_t *bpd, unsigned long long *lmask)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
+        uint32_t d[13];
+        load_dline(bpd, (uint32_t)v, d);
+        unsigned long long m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < (int)kBP; ++i) {
+            const uint32_t x = (d[i >> 2] >> (8 * (i & 3))) & 255u;
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                if (x == (uint32_t)k) m[k] |= 1ull << i;
+        }
+        uint4 *dst = reinterpret_cast<uint4 *>(lmask + v * 8);
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            dst[q] = make_uint4((uint32_t)m[2 * q], (uint32_t)(m[2 * q] >> 32), (uint32_t)m[2 * q + 1],
+                                (uint32_t)(m[2 * q + 1] >> 32));
+    }
+}
+
 // initialize (PARA:848-1080): non-BP roots become active with their own bit as
 // last group, get the self label (r, 0), and set their own indicator bit.
 // One wave per root, sixteen roots per block.  Root offset ro owns active
@@ -338,7 +419,9 @@ __global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const 
                                                      LabelStore S, unsigned long long *ind, unsigned long long *vmark,
                                                      uint32_t *tlist, uint32_t *alist, unsigned long long *ag,
                                                      uint32_t *amask, uint32_t *aend, unsigned long long *chunks,
-                                                     const ulonglong2 *bps, ulonglong2 *rset, BuildCounters *C)
+                                                     const ulonglong2 *bps, ulonglong2 *rset,
+                                                     const unsigned long long *lmask, unsigned long long *rmask,
+                                                     BuildCounters *C)
 {
     __shared__ uint32_t s_nch[16];
     __shared__ uint32_t s_base;
@@ -352,8 +435,10 @@ __global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const 
         vmark[r] = (unsigned long long)gen_batch << 17;
         tlist[ro] = r;
     }
-    // Slot-major copy of the batch roots' bit-parallel sets (push's set phase).
+    // Slot-major copy of the batch roots' bit-parallel sets and level-major
+    // copy of their level masks (push's bit-parallel check).
     if (ro < rsz && lane < kBP) rset[lane * kBatch + ro] = bps[(uint64_t)r * kBP + lane];
+    if (ro < rsz && lane < 8) rmask[lane * kBatch + ro] = lmask[(uint64_t)r * 8 + lane];
     uint32_t pdeg = 0;
     if (ro < rsz && !used[r]) {
         const uint32_t len = S.len[r];
@@ -485,24 +570,25 @@ constexpr uint32_t kPushWaves = 4;  // waves per k_push block
 
 // push (PARA:1486-1559 / push_labels PARA:1084-1211).  A wave takes four
 // 16-edge chunks (one edge per lane) and walks the root words of its edges'
-// last distance groups in step: per word, each lane test-and-sets its
-// indicator word, and the fresh (tail, root) pairs of all 64 lanes are then
-// checked against the bit-parallel labels 64 at a time, one pair per lane,
-// so a tail with many fresh roots does not hold up the wave.  The tail's
-// distance line is staged in LDS by the lane that owns the edge.
+// last distance groups in step, four words at a time: each lane
+// test-and-sets its indicator words, and the fresh (tail, root) pairs of all
+// 64 lanes are then checked against the bit-parallel labels 64 at a time,
+// one pair per lane, so a tail with many fresh roots does not hold up the
+// wave.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_push(uint32_t iter, const uint64_t *off, const uint32_t *adj,
                                               const uint32_t *alist, const unsigned long long *ag,
                                               const uint32_t *amask, const uint32_t *aend,
                                               const unsigned long long *chunks, unsigned long long *ind,
                                               unsigned long long *cand, unsigned long long *vmark, uint32_t *tlist,
                                               uint32_t *clist,
-                                              const uint8_t *bpd, const ulonglong2 *bps, const ulonglong2 *rset,
-                                              BuildCounters *C)
+                                              const uint8_t *bpd, const unsigned long long *lmask,
+                                              const unsigned long long *rmask, const ulonglong2 *bps,
+                                              const ulonglong2 *rset, BuildCounters *C)
 {
-    __shared__ uint4 s_dt[kPushWaves][64][4];              // tail distance lines (52 of 64 bytes used)
-    __shared__ unsigned long long s_nb[kPushWaves][64];    // fresh roots of each lane's edge, this word
-    __shared__ unsigned long long s_pass[kPushWaves][64];  // of those, the ones the BP check keeps
-    __shared__ uint32_t s_ex[kPushWaves][64];              // exclusive prefix of the fresh counts
+    __shared__ unsigned long long s_nb[kPushWaves][4][64]; // fresh roots of each lane's edge, per word of the step
+    __shared__ unsigned long long s_keep[kPushWaves][4][64];  // of those, the ones the BP check keeps
+    __shared__ uint32_t s_ex[kPushWaves][64];              // exclusive prefix of the lanes' fresh counts
+    __shared__ uint32_t s_cu[kPushWaves][64];              // a lane's running counts over its four words
     __shared__ uint32_t s_t[kPushWaves][64];               // each lane's tail
     // set-phase scratch (bp_set_phase)
     __shared__ unsigned long long s_srem[kPushWaves][64], s_sone[kPushWaves][64];
@@ -511,6 +597,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const uint32_t wave = blockIdx.x * kPushWaves + wv;
     const uint32_t nwaves = gridDim.x * kPushWaves;
     const SetScratch sc{s_srem[wv], s_sone[wv], s_sro[wv], s_st[wv], s_sex[wv], s_shit[wv]};
+    // The check reads level masks while they cover every td it acts on, the
+    // distance bytes after that.
+    const bool use_masks = iter <= kMaskIters;
     // Iteration `iter` of batch_process's loop (PARA:1478-1670) runs if the
     // previous one left active vertices (frontier slot iter & 1); past the
     // batch's end the launch is empty.  Iteration 255 would leave the
@@ -556,81 +645,115 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         s_t[wv][lane] = t;
         uint32_t wall = words;
         for (int o = 32; o > 0; o >>= 1) wall |= __shfl_xor(wall, o);
-        bool touched = false, got = false, tline = false;
+        bool touched = false, got = false;
         while (wall) {
-            const uint32_t w = __ffs(wall) - 1;
-            wall &= wall - 1;
-            unsigned long long nb = 0;
-            if ((words >> w) & 1u) {
-                unsigned long long g = ag[(uint64_t)k * kWords + w];
-                if (lim < (w + 1) * 64) g &= (1ull << (lim - w * 64)) - 1ull;
-                if (g) {
-                    n_push += __popcll(g);
-                    // Bits are only set during the kernel, so bits read as set
-                    // are set; the test-and-set runs only if some look fresh.
-                    unsigned long long *iw = &ind[(uint64_t)t * kWords + w];
-                    if (g & ~*iw) nb = g & ~atomicOr(iw, g);
+            // Up to four root words per step (wave-uniform): their indicator
+            // test-and-sets are issued together, and their fresh pairs share
+            // the bit-parallel passes.
+            uint32_t wsel[4] = {0u, 0u, 0u, 0u}, gsel = 0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (wall) {
+                    wsel[u] = __ffs(wall) - 1;
+                    wall &= wall - 1;
+                    gsel |= 1u << u;
                 }
             }
-            if (nb) {
+            unsigned long long g[4], seen[4], nb[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                g[u] = 0ull;
+                seen[u] = 0ull;
+                const uint32_t w = wsel[u];
+                if (((gsel >> u) & 1u) && ((words >> w) & 1u)) {
+                    g[u] = ag[(uint64_t)k * kWords + w];
+                    if (lim < (w + 1) * 64) g[u] &= (1ull << (lim - w * 64)) - 1ull;
+                    seen[u] = ind[(uint64_t)t * kWords + w];
+                }
+            }
+            // Bits are only set during the kernel, so bits read as set are
+            // set; the test-and-set runs only where some look fresh.
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                nb[u] = 0ull;
+                if (g[u]) {
+                    n_push += __popcll(g[u]);
+                    if (g[u] & ~seen[u]) nb[u] = g[u] & ~atomicOr(&ind[(uint64_t)t * kWords + wsel[u]], g[u]);
+                }
+            }
+            const uint32_t cu1 = (uint32_t)__popcll(nb[0]);
+            const uint32_t cu2 = cu1 + (uint32_t)__popcll(nb[1]);
+            const uint32_t cu3 = cu2 + (uint32_t)__popcll(nb[2]);
+            const uint32_t cnt = cu3 + (uint32_t)__popcll(nb[3]);
+            if (cnt) {
                 touched = true;
-                n_pass += __popcll(nb);
-                if (!tline) {
-                    const uint4 *src = reinterpret_cast<const uint4 *>(bpd + (uint64_t)t * kBpdStride);
-                    s_dt[wv][lane][0] = src[0];
-                    s_dt[wv][lane][1] = src[1];
-                    s_dt[wv][lane][2] = src[2];
-                    s_dt[wv][lane][3] = src[3];
-                    tline = true;
-                }
+                n_pass += cnt;
             }
-            const uint32_t cnt = (uint32_t)__popcll(nb);
             const uint32_t incl = wave_incl_scan(cnt);
             const uint32_t total = __shfl(incl, 63);
             if (total == 0) continue;
-            s_nb[wv][lane] = nb;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                s_nb[wv][u][lane] = nb[u];
+                s_keep[wv][u][lane] = 0ull;
+            }
             s_ex[wv][lane] = incl - cnt;
-            s_pass[wv][lane] = 0ull;
+            s_cu[wv][lane] = cu1 | (cu2 << 10) | (cu3 << 20);
             wave_lds_sync();
             // Fresh pair x of the wave: owner lane o (the last lane whose
-            // prefix is <= x; it has a fresh root), root = its (x - prefix)-th.
+            // prefix is <= x; it has a fresh root), then its word u and root
+            // bit b from the lane's per-word running counts.  Lanes of one
+            // owner share its tail, so their loads of the tail's line merge.
             // All 64 lanes run every pass (the set phase works across the
             // wave); lanes past the last pair carry nothing.
             for (uint32_t x0 = 0; x0 < total; x0 += 64) {
                 const uint32_t x = x0 + lane;
                 const bool act = x < total;
-                uint32_t o = 0, b = 0, to = 0;
+                uint32_t o = 0, u = 0, b = 0, w = 0, to = 0;
                 unsigned long long cs = 0ull, one = 0ull;
                 bool dist_pr = true;
                 if (act) {
                     o = item_owner(s_ex[wv], x);
-                    b = nth_set_bit(s_nb[wv][o], x - s_ex[wv][o]);
+                    const uint32_t rank = x - s_ex[wv][o], cu = s_cu[wv][o];
+                    const uint32_t c1 = cu & 1023u, c2 = (cu >> 10) & 1023u, c3 = cu >> 20;
+                    u = (rank >= c1 ? 1u : 0u) + (rank >= c2 ? 1u : 0u) + (rank >= c3 ? 1u : 0u);
+                    const uint32_t base = u == 0 ? 0u : (u == 1 ? c1 : (u == 2 ? c2 : c3));
+                    b = nth_set_bit(s_nb[wv][u][o], rank - base);
+                    w = u == 0 ? wsel[0] : (u == 1 ? wsel[1] : (u == 2 ? wsel[2] : wsel[3]));
                     to = s_t[wv][o];
-                    uint32_t dr[13], dt[13];
-                    load_dline(bpd, rs + w * 64 + b, dr);
-                    const uint4 q0 = s_dt[wv][o][0], q1 = s_dt[wv][o][1], q2 = s_dt[wv][o][2], q3 = s_dt[wv][o][3];
-                    dt[0] = q0.x; dt[1] = q0.y; dt[2] = q0.z; dt[3] = q0.w;
-                    dt[4] = q1.x; dt[5] = q1.y; dt[6] = q1.z; dt[7] = q1.w;
-                    dt[8] = q2.x; dt[9] = q2.y; dt[10] = q2.z; dt[11] = q2.w;
-                    dt[12] = q3.x;
-                    dist_pr = bp_dist_scan(dr, dt, iter, cs, one);
+                    if (use_masks) {
+                        dist_pr = bp_mask_scan(iter, rmask, w * 64 + b, lmask + (uint64_t)to * 8, cs, one);
+                    } else {
+                        const uint4 *rl = reinterpret_cast<const uint4 *>(bpd + (uint64_t)(rs + w * 64 + b) * kBpdStride);
+                        const uint4 *tl = reinterpret_cast<const uint4 *>(bpd + (uint64_t)to * kBpdStride);
+                        const uint4 a0 = rl[0], a1 = rl[1], a2 = rl[2], a3 = rl[3];
+                        const uint4 q0 = tl[0], q1 = tl[1], q2 = tl[2], q3 = tl[3];
+                        const uint32_t dr[13] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w,
+                                                 a2.x, a2.y, a2.z, a2.w, a3.x};
+                        const uint32_t dt[13] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
+                                                 q2.x, q2.y, q2.z, q2.w, q3.x};
+                        dist_pr = bp_dist_scan(dr, dt, iter, cs, one);
+                    }
                 }
-                const bool set_pr = bp_set_phase(dist_pr ? 0ull : cs, one, rset, w * 64 + b, bps, to, sc);
+                const bool set_pr = bp_set_phase(dist_pr ? 0ull : cs, one, rset, w * 64 + b, rs, bps, to, sc);
                 if (act) {
                     if (dist_pr || set_pr)
                         ++n_bp;
                     else
-                        atomicOr(&s_pass[wv][o], 1ull << b);
+                        atomicOr(&s_keep[wv][u][o], 1ull << b);
                 }
             }
             wave_lds_sync();
-            const unsigned long long pass = s_pass[wv][lane];
-            if (pass) {
-                atomicOr(&cand[(uint64_t)t * kWords + w], pass);
-                n_cand += __popcll(pass);
-                got = true;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const unsigned long long pass = s_keep[wv][u][lane];
+                if (pass) {
+                    atomicOr(&cand[(uint64_t)t * kWords + wsel[u]], pass);
+                    n_cand += __popcll(pass);
+                    got = true;
+                }
             }
-            // The next word's writes must not overtake this word's reads.
+            // The next step's writes must not overtake this step's reads.
             wave_lds_sync();
         }
         // vmark[t] = (batch << 16 | iteration) << 1 | candidate, raised with
@@ -977,6 +1100,8 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     DevBuf<unsigned long long> vmark;  // ((batch << 16 | iteration) << 1 | candidate) of the last mark
     DevBuf<uint8_t> dc;
     DevBuf<ulonglong2> rset;  // batch roots' bit-parallel sets, slot-major [kBP][kBatch]
+    DevBuf<unsigned long long> lmask;  // bit-parallel level masks, 8 words per vertex
+    DevBuf<unsigned long long> rmask;  // batch roots' level masks, level-major [8][kBatch]
     DevBuf<BuildCounters> ctr;
     PADO_TRY(vmark.alloc(n));
     PADO_TRY(ind.alloc((uint64_t)n * kWords));
@@ -999,6 +1124,8 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     PADO_TRY(llen.alloc(n));
     PADO_TRY(lcap.alloc(n));
     PADO_TRY(rset.alloc((uint64_t)kBP * kBatch));
+    PADO_TRY(lmask.alloc((uint64_t)n * 8));
+    PADO_TRY(rmask.alloc(8ull * kBatch));
     PADO_TRY(ctr.alloc(1));
     // The label pool starts at 64 words per vertex and doubles whenever a
     // host check finds it over 40% full; the checks come every
@@ -1015,6 +1142,7 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, ind.p, (uint64_t)n * kWords);
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, cand.p, (uint64_t)n * kWords);
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, vmark.p, (uint64_t)n);
+    hipLaunchKernelGGL(k_bp_masks, dim3(fg), dim3(256), 0, st, n, idx->bpd.p, lmask.p);
     hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, hinfo.p, (uint64_t)n, kEmpty);
     hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, llen.p, (uint64_t)n, 0u);
     hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, lcap.p, (uint64_t)n, 0u);
@@ -1058,7 +1186,7 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
         hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(batch), 0, st, d_used.p, ctr.p, batch, n);
         hipLaunchKernelGGL(k_batch_init, dim3((batch + 15) / 16), dim3(1024), 0, st, d_used.p, g->off.p, g->adj.p, S, ind.p,
                            vmark.p, tlist.p, alist.p, ag.p, amask.p, aend.p, chunks.p,
-                           (const ulonglong2 *)idx->bps.p, rset.p, ctr.p);
+                           (const ulonglong2 *)idx->bps.p, rset.p, lmask.p, rmask.p, ctr.p);
         hipLaunchKernelGGL(k_hub_mark, dim3(root_blocks), dim3(256), 0, st, d_used.p, S, hinfo.p, hlist.p, dc.p,
                            ctr.p);
         hipLaunchKernelGGL(k_hub_fill, dim3(root_blocks), dim3(256), 0, st, d_used.p, S, hinfo.p, dc.p, ctr.p);
@@ -1070,7 +1198,7 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u], st));
             hipLaunchKernelGGL(k_push, dim3(push_grid), dim3(256), 0, st, iter, g->off.p, g->adj.p, alist.p, ag.p,
                                amask.p, aend.p, chunks.p, ind.p, cand.p, vmark.p, tlist.p, clist.p,
-                               idx->bpd.p, (const ulonglong2 *)idx->bps.p, rset.p, ctr.p);
+                               idx->bpd.p, lmask.p, rmask.p, (const ulonglong2 *)idx->bps.p, rset.p, ctr.p);
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 1], st));
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 2], st));
             hipLaunchKernelGGL(k_add, dim3(add_grid), dim3(256), 0, st, iter, d_used.p, g->off.p, g->adj.p, clist.p,
