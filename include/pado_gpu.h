@@ -142,8 +142,8 @@ typedef struct {
     uint32_t batch_size;  /* roots per batch: 1..1024, 0 = 1024 (PARA:42); any value
                              gives the same labels, so it is a tuning knob */
     uint32_t flags;       /* PADO_GPU_BUILD_* */
-    uint32_t push_blocks; /* grid of the push kernel (256-thread blocks), 0 = 4096 */
-    uint32_t add_blocks;  /* grid of the add kernel, 0 = 2048 */
+    uint32_t push_blocks; /* grid of the push kernel (256-thread blocks), 0 = 1024 */
+    uint32_t add_blocks;  /* grid of the add kernel, 0 = 1024 */
     uint32_t iters_per_sync; /* batch iterations enqueued per host check of the
                                 frontier (iterations past a batch's end are
                                 empty launches), 0 = 8 */

@@ -37,8 +37,8 @@ extern "C" pado_gpu_status pado_gpu_autotune(pado_gpu_ctx *ctx, const pado_gpu_g
     pado_gpu_build_cfg cur{};
     if (start) cur = *start;
     if (!cur.batch_size) cur.batch_size = pado::kBatch;
-    if (!cur.push_blocks) cur.push_blocks = 4096;
-    if (!cur.add_blocks) cur.add_blocks = 2048;
+    if (!cur.push_blocks) cur.push_blocks = pado::kPushGrid;
+    if (!cur.add_blocks) cur.add_blocks = pado::kAddGrid;
     if (!cur.iters_per_sync) cur.iters_per_sync = 8;
     if (!max_trials) max_trials = 20;
     uint32_t trials = 0;
@@ -48,8 +48,8 @@ extern "C" pado_gpu_status pado_gpu_autotune(pado_gpu_ctx *ctx, const pado_gpu_g
     if (st != PADO_GPU_OK) return st;
     // Coordinate search: one knob at a time, keep the best value.
     const std::vector<uint32_t> batches = {256, 512, 1024};
-    const std::vector<uint32_t> pushes = {1024, 2048, 4096, 8192};
-    const std::vector<uint32_t> adds = {1024, 2048, 4096};
+    const std::vector<uint32_t> pushes = {512, 1024, 2048, 4096};
+    const std::vector<uint32_t> adds = {512, 1024, 2048};
     const std::vector<uint32_t> iters = {2, 4, 8, 16};
     const std::vector<uint32_t> flags = {0u, PADO_GPU_BUILD_STREAM, PADO_GPU_BUILD_BP_PER_SLOT,
                                          PADO_GPU_BUILD_STREAM | PADO_GPU_BUILD_BP_PER_SLOT};

@@ -26,14 +26,16 @@
 //   ag[A][16] u64    last distance group of each active vertex, as a bitmap
 //   chunks[]         16-edge push work descriptors (active index, edge start)
 //   pool[] u32       label store: per-vertex growable arrays of packed
-//                    (hub << 8 | dist) labels, doubled on overflow by a bump
-//                    allocator (one wave owns one vertex's array)
+//                    (hub << 8 | dist) labels, grown fourfold (at least 32)
+//                    on overflow by a bump allocator (one wave owns one
+//                    vertex's array)
 //   hinfo[n] u32     per-hub (slot << 8 | min distance from any batch root)
 //   dc[slot][1024]   root-distance table: dc[slot(h)][r] = dist(root r, hub h),
 //                    the transposed, hub-compacted form of dist_matrix
-// Kernels per iteration: k_push (edge-parallel, quarter-wave per 16 edges) and
-// k_add (wave per candidate vertex: distance query + label append + next
-// active entry).  The host loop syncs once per iteration on the active count.
+// Kernels per iteration: k_push (edge-parallel, one edge per lane, 16-edge
+// chunks) and k_add (wave per candidate vertex: distance query + label
+// append + next active entry).  The host enqueues blocks of iterations and
+// checks the frontier count once per block.
 #include <functional>
 #include <map>
 
@@ -65,11 +67,10 @@ constexpr unsigned long long kChunkPad = 0xFFFFFFF0ull << 32;
 // Number of neighbours of v with id >= rs: the adjacency is sorted
 // descending, so they form a prefix (PARA:1121-1123 returns at the first tail
 // below roots_start).  Wave-cooperative 64-ary search.
-__device__ inline uint32_t push_degree(const uint64_t *off, const uint32_t *adj, uint32_t v, uint32_t rs)
+__device__ inline uint32_t push_degree_range(const uint32_t *adj, uint64_t base, uint64_t end, uint32_t rs)
 {
     const uint32_t lane = lane_id();
-    const uint64_t base = off[v];
-    uint32_t lo = 0, hi = (uint32_t)(off[v + 1] - base);
+    uint32_t lo = 0, hi = (uint32_t)(end - base);
     while (hi - lo > 64) {
         uint32_t step = (hi - lo + 63) / 64;
         uint32_t pos = lo + lane * step;
@@ -87,6 +88,11 @@ __device__ inline uint32_t push_degree(const uint64_t *off, const uint32_t *adj,
     uint32_t pos = lo + lane;
     bool p = pos < hi && adj[base + pos] >= rs;
     return lo + (uint32_t)__popcll(__ballot(p));
+}
+
+__device__ inline uint32_t push_degree(const uint64_t *off, const uint32_t *adj, uint32_t v, uint32_t rs)
+{
+    return push_degree_range(adj, off[v], off[v + 1], rs);
 }
 
 // Bit-parallel check of PARA:1164-1186 with the reference's unsigned
@@ -349,16 +355,14 @@ struct LabelStore {
 // Reserve room for `add` more labels of v (wave-uniform call).  Returns the
 // (possibly new) base; on pool exhaustion sets the overflow flag and returns
 // ~0ull.
-__device__ inline unsigned long long store_reserve(LabelStore S, BuildCounters *C, uint32_t v, uint32_t len,
-                                                   uint32_t add)
+__device__ inline unsigned long long store_reserve_at(LabelStore S, BuildCounters *C, uint32_t v, uint32_t len,
+                                                      unsigned long long base, uint32_t cap, uint32_t add)
 {
     const uint32_t lane = lane_id();
-    unsigned long long base = S.base[v];
-    uint32_t cap = S.cap[v];
     if (len + add <= cap) return base;
-    uint32_t ncap = cap * 2;
+    uint32_t ncap = cap * 4;  // few reallocations: each is an atomic on one shared counter
     if (ncap < len + add) ncap = len + add;
-    if (ncap < 8) ncap = 8;
+    if (ncap < 32) ncap = 32;
     unsigned long long nb = 0;
     if (lane == 0) nb = atomicAdd(&C->pool_top, (unsigned long long)ncap);
     nb = __shfl(nb, 0);
@@ -372,6 +376,12 @@ __device__ inline unsigned long long store_reserve(LabelStore S, BuildCounters *
         S.cap[v] = ncap;
     }
     return nb;
+}
+
+__device__ inline unsigned long long store_reserve(LabelStore S, BuildCounters *C, uint32_t v, uint32_t len,
+                                                   uint32_t add)
+{
+    return store_reserve_at(S, C, v, len, S.base[v], S.cap[v], add);
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint64_t count, uint32_t val)
@@ -847,25 +857,36 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
     // kAlistGrab / kChunkGrab at a time per wave, so the shared counters see
     // one atomic per reservation, not per vertex; unused descriptor slots are
     // padded.  The exact active count is added once per wave at the end.
-    constexpr uint32_t kAlistGrab = 16, kChunkGrab = 64;
+    constexpr uint32_t kAlistGrab = 64, kChunkGrab = 64;
     uint32_t a_next = 0, a_left = 0, c_next = 0, c_left = 0, my_act = 0;
+    // The candidate list is read one vertex ahead, and everything that
+    // depends only on the vertex (candidate word, label-store entry, the
+    // first 64 labels, the push degree of its next-frontier entry) is loaded
+    // before the first store of the step, so those loads overlap.
+    uint32_t v_next = wave < ncand ? clist[wave] : 0u;
     for (uint32_t i = wave; i < ncand; i += nwaves) {
-        const uint32_t v = clist[i];
+        const uint32_t v = v_next;
         const unsigned long long word = cand[(uint64_t)v * kWords + (lane >> 2)];
+        const uint32_t len = S.len[v];
+        const unsigned long long base = S.base[v];
+        const uint32_t cap = S.cap[v];
+        const bool v_used = used[v] != 0;
+        const uint64_t e0 = off[v], e1 = off[v + 1];
+        if (i + nwaves < ncand) v_next = clist[i + nwaves];
+        const uint32_t lab0 = lane < len ? S.pool[base + lane] : 0u;
+        const uint32_t pdeg = v_used ? 0u : push_degree_range(adj, e0, e1, rs);
         const uint32_t cb = (uint32_t)(word >> (16 * (lane & 3))) & 0xFFFFu;
         // The clear below is control-dependent on every lane's loaded word
         // (through the ballot), so no lane can clear a word before all lanes
         // have read it.
         if (__ballot(cb != 0) != 0ull && lane < kWords) cand[(uint64_t)v * kWords + lane] = 0ull;
         uint32_t prune = 0;
-        const uint32_t len = S.len[v];
-        const unsigned long long base = S.base[v];
         const uint32_t my_lo = rs + 16 * lane;  // real id of this lane's first root
         for (uint32_t j0 = 0; j0 < len; j0 += 64) {
             const uint32_t j = j0 + lane;
             uint32_t h = 0, d = 255, info = kEmpty;
             if (j < len) {
-                const uint32_t lab = S.pool[base + j];
+                const uint32_t lab = j0 == 0 ? lab0 : S.pool[base + j];
                 h = lab >> 8;
                 d = lab & 255u;
                 if (d < iter) {
@@ -927,7 +948,7 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
         const uint32_t total = __shfl(incl, 63);
         if (total == 0) continue;
         n_lab += total;
-        const unsigned long long nb = store_reserve(S, C, v, len, total);
+        const unsigned long long nb = store_reserve_at(S, C, v, len, base, cap, total);
         if (nb == ~0ull) continue;
         uint32_t pos = len + incl - cnt;
         for (uint32_t sb = surv; sb; sb &= sb - 1) {
@@ -967,7 +988,6 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
 #pragma unroll
         for (int w = 0; w < (int)kWords; ++w)
             if ((nzm >> (4 * w)) & 0xFull) wmask |= 1u << w;
-        const uint32_t pdeg = used[v] ? 0u : push_degree(off, adj, v, rs);
         if (lane == 0) {
             alist[k] = v;
             amask[k] = wmask;
@@ -1109,8 +1129,8 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     // Per-wave reservations in k_add leave holes: the active list can exceed
     // the active count by kAlistGrab slots per wave, and padded descriptors
     // are fewer than the real ones plus kChunkGrab per wave.
-    const uint64_t add_waves = 4ull * (cfg.add_blocks ? cfg.add_blocks : 2048);
-    const uint64_t alist_cap = (uint64_t)n + 16 * add_waves;
+    const uint64_t add_waves = 4ull * (cfg.add_blocks ? cfg.add_blocks : kAddGrid);
+    const uint64_t alist_cap = (uint64_t)n + 64 * add_waves;
     PADO_TRY(ag.alloc(alist_cap * kWords));
     PADO_TRY(chunks.alloc(2 * (g->m / kChunk + (uint64_t)n) + 64 * add_waves + 64));
     PADO_TRY(tlist.alloc(n));
@@ -1159,8 +1179,8 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
 
     const bool use_graph = (cfg.flags & PADO_GPU_BUILD_STREAM) == 0;
     const uint32_t U = cfg.iters_per_sync ? std::min<uint32_t>(cfg.iters_per_sync, 64u) : 8u;
-    const uint32_t push_grid = cfg.push_blocks ? cfg.push_blocks : 4096;
-    const uint32_t add_grid = cfg.add_blocks ? cfg.add_blocks : 2048;
+    const uint32_t push_grid = cfg.push_blocks ? cfg.push_blocks : kPushGrid;
+    const uint32_t add_grid = cfg.add_blocks ? cfg.add_blocks : kAddGrid;
     const uint32_t root_blocks = (batch * 64 + 255) / 256;
 
     // Stream mode brackets every push and add launch with HIP events (read

@@ -20,7 +20,11 @@ constexpr uint8_t kInf8 = 255;        // SMALLI_MAX (GLB:40)
 constexpr uint32_t kHubBits = 24;     // packed label = (hub << 8) | dist
 constexpr uint32_t kMaxVertices = 1u << kHubBits;
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
-constexpr uint32_t kChunk = 16;       // edges per quarter-wave work descriptor
+constexpr uint32_t kChunk = 16;       // edges per push work descriptor
+// Default grids of the labeling kernels (256-thread blocks): one resident
+// round of waves at their occupancy, so a launch with little work ends fast.
+constexpr uint32_t kPushGrid = 1024;
+constexpr uint32_t kAddGrid = 1024;
 
 // Thread-local error text behind pado_gpu_last_error().
 void set_error(const std::string &msg);
