@@ -59,25 +59,35 @@ __global__ void k_ms_seed(uint32_t nseeds, const uint32_t *seed_v, const uint32_
     }
 }
 
+// The 16 lanes of a chunk share v: per slot their S^-1 sets are OR-reduced
+// across the 16 lanes and one lane applies the union, so v's record sees one
+// atomic per slot and chunk instead of one per edge.
 __global__ __launch_bounds__(256) void k_ms_sib(const uint64_t *off, const uint32_t *adj, const u64 *fm,
                                                 ulonglong2 *bps, const u64 *chunks, const uint32_t *cnt_chunks)
 {
     const uint32_t lane = threadIdx.x & (kChunk - 1);
     const uint32_t ngroups = (gridDim.x * blockDim.x) / kChunk;
     const uint32_t nc = *cnt_chunks;
+    // c is uniform over each 16-lane group, so the group runs the loop body
+    // together (the shuffles below stay inside the group).
     for (uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) / kChunk; c < nc; c += ngroups) {
         const u64 desc = chunks[c];
         const uint32_t v = (uint32_t)desc;
         const uint64_t e = off[v] + (desc >> 32) + lane;
-        if (e >= off[v + 1]) continue;
-        const uint32_t u = adj[e];
-        if (u == v) continue;
-        u64 m = fm[v] & fm[u];
-        while (m) {
-            const uint32_t i = __ffsll(m) - 1;
-            m &= m - 1;
-            const u64 x = bps[(uint64_t)u * kBP + i].x;
-            if (x) atomicOr(&bps[(uint64_t)v * kBP + i].y, x);
+        u64 m = 0ull;
+        uint32_t u = v;
+        if (e < off[v + 1]) {
+            u = adj[e];
+            if (u != v) m = fm[v] & fm[u];
+        }
+        u64 un = m;
+        for (int o = 1; o < (int)kChunk; o <<= 1) un |= __shfl_xor(un, o);
+        while (un) {
+            const uint32_t i = __ffsll(un) - 1;
+            un &= un - 1;
+            u64 x = ((m >> i) & 1ull) ? bps[(uint64_t)u * kBP + i].x : 0ull;
+            for (int o = 1; o < (int)kChunk; o <<= 1) x |= __shfl_xor(x, o);
+            if (lane == 0 && x) atomicOr(&bps[(uint64_t)v * kBP + i].y, x);
         }
     }
 }
