@@ -29,9 +29,9 @@
 //                    (hub << 8 | dist) labels, grown fourfold (at least 32)
 //                    on overflow by a bump allocator (one wave owns one
 //                    vertex's array)
-//   hinfo[n] u32     per-hub (slot << 8 | min distance from any batch root)
-//   dc[slot][1024]   root-distance table: dc[slot(h)][r] = dist(root r, hub h),
-//                    the transposed, hub-compacted form of dist_matrix
+//   hinfo[n] u32     per-hub 0xFFFFFF00 | min distance from any batch root
+//   dc[h][1024] u8   root-distance table: dc[h][r - rs] = dist(root r, hub h),
+//                    the transposed form of dist_matrix, rows by hub id
 // Kernels per iteration: k_push (edge-parallel, one edge per lane, 16-edge
 // chunks) and k_add (wave per candidate vertex: distance query + label
 // append + next active entry).  The host enqueues blocks of iterations and
@@ -43,8 +43,6 @@
 
 namespace pado {
 namespace {
-
-constexpr uint32_t kClaim = 0xFFFFFFFEu;
 
 __device__ inline uint32_t lane_id() { return threadIdx.x & 63; }
 
@@ -482,69 +480,16 @@ __global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const 
     for (uint32_t q = lane; q < nch; q += 64) chunks[c0 + q] = ((unsigned long long)(q * kChunk) << 32) | ro;
 }
 
-// Distinct hubs over the labels of the batch roots, with their rows of the
-// root-distance table: a hub claimed here gets the next slot, hinfo[h] =
-// slot << 8 | 255 and an all-INF row.  One wave per root; each lane handles
-// four labels per step so the dependent loads and atomics of a step overlap.
-__global__ __launch_bounds__(256) void k_hub_mark(const uint8_t *used, LabelStore S, uint32_t *hinfo,
-                                                  uint32_t *hlist, uint8_t *dc, BuildCounters *C)
-{
-    const uint32_t lane = lane_id();
-    const uint32_t ro = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t rs = C->rs, rsz = C->rsz;
-    if (ro >= rsz) return;
-    const uint32_t r = rs + ro;
-    if (used[r]) return;
-    const uint32_t len = S.len[r];
-    const unsigned long long base = S.base[r];
-    for (uint32_t j0 = 0; j0 < len; j0 += 256) {
-        uint32_t h[4];
-        bool fresh[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const uint32_t j = j0 + 64 * u + lane;
-            h[u] = j < len ? S.pool[base + j] >> 8 : kEmpty;
-        }
-        // A hub that reads as taken is taken (no entry returns to empty in
-        // this kernel), so the CAS runs only for hubs not yet seen taken:
-        // the common hubs of all roots would otherwise take one CAS per root.
-#pragma unroll
-        for (int u = 0; u < 4; ++u) fresh[u] = h[u] != kEmpty && hinfo[h[u]] == kEmpty;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) fresh[u] = fresh[u] && atomicCAS(&hinfo[h[u]], kEmpty, kClaim) == kEmpty;
-        // One slot reservation per step for the claims of all four sets.
-        unsigned long long m[4];
-        uint32_t tot = 0;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            m[u] = __ballot(fresh[u]);
-            tot += (uint32_t)__popcll(m[u]);
-        }
-        if (tot == 0) continue;
-        uint32_t s0 = 0;
-        if (lane == 0) s0 = atomicAdd(&C->n_hub, tot);
-        s0 = __shfl(s0, 0);
-        const unsigned long long below = (1ull << lane) - 1ull;
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const uint32_t slot = s0 + (uint32_t)__popcll(m[u] & below);
-            if (fresh[u]) {
-                hlist[slot] = h[u];
-                hinfo[h[u]] = (slot << 8) | kInf8;
-            }
-            for (unsigned long long mm = m[u]; mm; mm &= mm - 1) {
-                const uint32_t sf = s0 + (uint32_t)__popcll(m[u] & ((1ull << (__ffsll(mm) - 1)) - 1ull));
-                reinterpret_cast<uint4 *>(dc + (uint64_t)sf * kBatch)[lane] = make_uint4(~0u, ~0u, ~0u, ~0u);
-            }
-            s0 += (uint32_t)__popcll(m[u]);
-        }
-    }
-}
-
-// dist_matrix scatter (PARA:986-1013), transposed: dc[slot(h)][r] = d, and
-// the per-hub minimum over the batch roots in the low byte of hinfo[h].
-__global__ __launch_bounds__(256) void k_hub_fill(const uint8_t *used, LabelStore S, uint32_t *hinfo, uint8_t *dc,
-                                                  const BuildCounters *C)
+// Root-distance table (dist_matrix, PARA:986-1013, transposed): row h holds
+// dist(root r, hub h) at column r - rs for the hubs of the batch roots'
+// labels, 255 elsewhere; hinfo[h] holds 0xFFFFFF00 | min over the batch
+// roots (0xFFFFFFFF when h is no hub of any of them).  One wave per root; each
+// lane handles four labels per step so the dependent loads of a step overlap.
+// With `fill` the root's labels are entered; without, the entries of the
+// previous batch's roots (the labels they held at its end, which include
+// every entry written during it) are reset.
+__global__ __launch_bounds__(256) void k_hub_table(uint32_t fill, const uint8_t *used, LabelStore S, uint32_t *hinfo,
+                                                   uint8_t *dc, const BuildCounters *C)
 {
     const uint32_t lane = lane_id();
     const uint32_t ro = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -561,17 +506,26 @@ __global__ __launch_bounds__(256) void k_hub_fill(const uint8_t *used, LabelStor
             const uint32_t j = j0 + 64 * u + lane;
             lab[u] = j < len ? S.pool[base + j] : kEmpty;
         }
+        if (!fill) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (lab[u] == kEmpty) continue;
+                const uint32_t h = lab[u] >> 8;
+                dc[(uint64_t)h * kBatch + ro] = kInf8;
+                hinfo[h] = kEmpty;
+            }
+            continue;
+        }
 #pragma unroll
         for (int u = 0; u < 4; ++u) info[u] = lab[u] != kEmpty ? hinfo[lab[u] >> 8] : 0u;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (lab[u] == kEmpty) continue;
             const uint32_t h = lab[u] >> 8, d = lab[u] & 255u;
-            const uint32_t slot = info[u] >> 8;
-            dc[(uint64_t)slot * kBatch + ro] = (uint8_t)d;
+            dc[(uint64_t)h * kBatch + ro] = (uint8_t)d;
             // The minimum only falls during the kernel, so a value read as
             // <= d (even a stale one) makes the atomic a no-op.
-            if (d < (info[u] & 255u)) atomicMin(&hinfo[h], (slot << 8) | d);
+            if (d < (info[u] & 255u)) atomicMin(&hinfo[h], 0xFFFFFF00u | d);
         }
     }
 }
@@ -828,7 +782,6 @@ __global__ __launch_bounds__(1024) void k_batch_begin(const uint8_t *used, Build
     C->n_chunk[0] = C->n_chunk[1] = 0;
     C->n_cand[0] = C->n_cand[1] = 0;
     C->n_touched = rsz;
-    C->n_hub = 0;
     C->labels += cnt;
 }
 
@@ -894,7 +847,7 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
                     ++n_read;
                 }
             }
-            const bool pass = info != kEmpty && d + (info & 255u) <= iter;
+            const bool pass = d + (info & 255u) <= iter;
             unsigned long long m = __ballot(pass);
             // Up to four passing hubs per step: their row loads are issued
             // together, and only lanes that still hold an unpruned candidate
@@ -919,9 +872,8 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
                     // all-255 row never meets
                     const uint32_t du = __shfl(d, src[u]);
                     thr[u] = ok[u] ? iter - du : 0u;
-                    const uint32_t slot = __shfl(info, src[u]) >> 8;
                     row[u] = make_uint4(~0u, ~0u, ~0u, ~0u);
-                    if (ok[u] && need) row[u] = reinterpret_cast<const uint4 *>(dc + (uint64_t)slot * kBatch)[lane];
+                    if (ok[u] && need) row[u] = reinterpret_cast<const uint4 *>(dc + (uint64_t)hh[u] * kBatch)[lane];
                     n_rows += ok[u] ? 1 : 0;
                 }
 #pragma unroll
@@ -961,11 +913,8 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
         if (v >= rs && v < rs + rsz) {
             for (uint32_t sb = surv; sb; sb &= sb - 1) {
                 const uint32_t h = my_lo + __ffs(sb) - 1;
-                const uint32_t info = hinfo[h];
-                if (info == kEmpty || info == kClaim) continue;
-                const uint32_t slot = info >> 8;
-                dc[(uint64_t)slot * kBatch + (v - rs)] = (uint8_t)iter;
-                atomicMin(&hinfo[h], (slot << 8) | iter);
+                dc[(uint64_t)h * kBatch + (v - rs)] = (uint8_t)iter;
+                if (iter < (hinfo[h] & 255u)) atomicMin(&hinfo[h], 0xFFFFFF00u | iter);
             }
         }
         // v becomes active with the new distance group as its last group.
@@ -1021,16 +970,16 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
 }
 
 // reset_at_end (PARA:1328-1410) and the next batch's indicator reset
-// (PARA:877-901): clear indicator rows of touched vertices and the hub table.
+// (PARA:877-901): clear the indicator rows of the touched vertices (the
+// root-distance table is reset by k_hub_table).
 __global__ __launch_bounds__(256) void k_batch_end(unsigned long long *ind, const uint32_t *tlist,
-                                                   uint32_t *hinfo, const uint32_t *hlist, const BuildCounters *C)
+                                                   const BuildCounters *C)
 {
-    const uint32_t nt = C->n_touched, nh = C->n_hub;
+    const uint32_t nt = C->n_touched;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t tot = (uint64_t)nt * kWords;
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += stride)
         ind[(uint64_t)tlist[i / kWords] * kWords + (i % kWords)] = 0ull;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh; i += stride) hinfo[hlist[i]] = kEmpty;
 }
 
 // Pool compaction: copy each vertex's labels into a fresh pool.
@@ -1116,7 +1065,7 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     }
 
     DevBuf<unsigned long long> ind, cand, ag, chunks, lbase;
-    DevBuf<uint32_t> tlist, clist, alist, amask, aend, hinfo, hlist, pool, llen, lcap;
+    DevBuf<uint32_t> tlist, clist, alist, amask, aend, hinfo, pool, llen, lcap;
     DevBuf<unsigned long long> vmark;  // ((batch << 16 | iteration) << 1 | candidate) of the last mark
     DevBuf<uint8_t> dc;
     DevBuf<ulonglong2> rset;  // batch roots' bit-parallel sets, slot-major [kBP][kBatch]
@@ -1139,7 +1088,6 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     PADO_TRY(amask.alloc(alist_cap));
     PADO_TRY(aend.alloc(alist_cap));
     PADO_TRY(hinfo.alloc(n));
-    PADO_TRY(hlist.alloc(n));
     PADO_TRY(lbase.alloc(n));
     PADO_TRY(llen.alloc(n));
     PADO_TRY(lcap.alloc(n));
@@ -1152,13 +1100,15 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     // iters_per_sync iterations, so the headroom covers several iterations.
     unsigned long long pool_size = std::max<unsigned long long>(1ull << 22, 64ull * n);
     PADO_TRY(pool.alloc(pool_size));
-    // Root-distance table rows: the hubs of batch b's roots are ids below the
-    // batch end, so rs + rsz slots always suffice; grown on the host before
-    // a batch that could exceed it (no device round trip).
-    uint32_t dc_slots = std::min<uint32_t>(n, 1u << 16);
-    PADO_TRY(dc.alloc((uint64_t)dc_slots * kBatch));
+    // Root-distance table, one 1 KB row per hub id: the hubs of batch b's
+    // roots are ids below the batch end, so that many rows suffice; grown
+    // (all 255) on the host before a batch that could exceed it.
+    uint32_t dc_rows = std::min<uint32_t>(n, 1u << 16);
+    PADO_TRY(dc.alloc((uint64_t)dc_rows * kBatch));
 
     const uint32_t fg = 4096;
+    hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, reinterpret_cast<uint32_t *>(dc.p),
+                       (uint64_t)dc_rows * kBatch / 4, ~0u);
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, ind.p, (uint64_t)n * kWords);
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, cand.p, (uint64_t)n * kWords);
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, vmark.p, (uint64_t)n);
@@ -1202,14 +1152,13 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     LabelStore S{pool.p, lbase.p, llen.p, lcap.p, pool_size};
     auto enqueue_prologue = [&]() {
         // Previous batch's reset_at_end first (a no-op before the first batch).
-        hipLaunchKernelGGL(k_batch_end, dim3(1024), dim3(256), 0, st, ind.p, tlist.p, hinfo.p, hlist.p, ctr.p);
+        hipLaunchKernelGGL(k_hub_table, dim3(root_blocks), dim3(256), 0, st, 0u, d_used.p, S, hinfo.p, dc.p, ctr.p);
+        hipLaunchKernelGGL(k_batch_end, dim3(1024), dim3(256), 0, st, ind.p, tlist.p, ctr.p);
         hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(batch), 0, st, d_used.p, ctr.p, batch, n);
         hipLaunchKernelGGL(k_batch_init, dim3((batch + 15) / 16), dim3(1024), 0, st, d_used.p, g->off.p, g->adj.p, S, ind.p,
                            vmark.p, tlist.p, alist.p, ag.p, amask.p, aend.p, chunks.p,
                            (const ulonglong2 *)idx->bps.p, rset.p, lmask.p, rmask.p, ctr.p);
-        hipLaunchKernelGGL(k_hub_mark, dim3(root_blocks), dim3(256), 0, st, d_used.p, S, hinfo.p, hlist.p, dc.p,
-                           ctr.p);
-        hipLaunchKernelGGL(k_hub_fill, dim3(root_blocks), dim3(256), 0, st, d_used.p, S, hinfo.p, dc.p, ctr.p);
+        hipLaunchKernelGGL(k_hub_table, dim3(root_blocks), dim3(256), 0, st, 1u, d_used.p, S, hinfo.p, dc.p, ctr.p);
     };
     // Iterations first + 1 .. first + count of the current batch.
     auto enqueue_iterations = [&](uint32_t first, uint32_t count, bool timed) -> pado_gpu_status {
@@ -1274,11 +1223,15 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     uint32_t prev_batch_iters = 1;
     for (uint32_t b = 0; b < nb; ++b) {
         const uint32_t batch_end = std::min<uint64_t>((uint64_t)(b + 1) * batch, n);
-        if (batch_end > dc_slots) {
+        if (batch_end > dc_rows) {
             // Ordered after all earlier work: hipFree/hipMalloc synchronise.
+            // The new table starts all 255, so the previous batch's reset is
+            // a no-op on it.
             PADO_HIP(hipStreamSynchronize(st));
-            dc_slots = std::min<uint32_t>(n, std::max<uint32_t>(batch_end, dc_slots * 2));
-            PADO_TRY(dc.alloc((uint64_t)dc_slots * kBatch));
+            dc_rows = std::min<uint32_t>(n, std::max<uint32_t>(batch_end, dc_rows * 2));
+            PADO_TRY(dc.alloc((uint64_t)dc_rows * kBatch));
+            hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, reinterpret_cast<uint32_t *>(dc.p),
+                               (uint64_t)dc_rows * kBatch / 4, ~0u);
             drop_graphs();
         }
         const uint64_t batch_first_iter = iterations_seen;
@@ -1343,7 +1296,7 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
         }
         prev_batch_iters = (uint32_t)(iterations_seen - batch_first_iter);
     }
-    hipLaunchKernelGGL(k_batch_end, dim3(1024), dim3(256), 0, st, ind.p, tlist.p, hinfo.p, hlist.p, ctr.p);
+    hipLaunchKernelGGL(k_batch_end, dim3(1024), dim3(256), 0, st, ind.p, tlist.p, ctr.p);
     PADO_HIP(hipMemcpyAsync(h_ctr, ctr.p, sizeof(BuildCounters), hipMemcpyDeviceToHost, st));
     PADO_HIP(hipStreamSynchronize(st));
     idx->counters = *h_ctr;

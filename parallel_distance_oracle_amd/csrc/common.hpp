@@ -102,7 +102,7 @@ struct BuildCounters {
     uint32_t n_chunk[2];   // push work descriptors of the iteration
     uint32_t n_cand[2];    // candidate vertices of the iteration
     uint32_t n_touched;    // vertices whose indicator row was touched this batch
-    uint32_t n_hub;        // distinct hubs in the batch's root-distance table
+    uint32_t pad0;
     uint32_t overflow;     // label pool exhausted
     uint32_t dq_rows;      // root-distance rows scanned (statistic)
     unsigned long long pool_top;  // next free label slot in the pool
