@@ -433,6 +433,7 @@ __global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const 
 {
     __shared__ uint32_t s_nch[16];
     __shared__ uint32_t s_base;
+    if (C->hold) return;
     const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
     const uint32_t ro = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t rs = C->rs, rsz = C->rsz, gen_batch = C->gen_batch;
@@ -491,6 +492,7 @@ __global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const 
 __global__ __launch_bounds__(256) void k_hub_table(uint32_t fill, const uint8_t *used, LabelStore S, uint32_t *hinfo,
                                                    uint8_t *dc, const BuildCounters *C)
 {
+    if (C->hold) return;
     const uint32_t lane = lane_id();
     const uint32_t ro = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const uint32_t rs = C->rs, rsz = C->rsz;
@@ -569,6 +571,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     // batch's end the launch is empty.  Iteration 255 would leave the
     // reference's u8 domain: it is flagged, not run.  Block 0 also empties the
     // other slot, which k_add fills with the next frontier.
+    if (C->hold) return;
     const uint32_t p = iter & 1u;
     const uint32_t n_act0 = C->n_act[p];
     const bool live = n_act0 != 0 && iter < kInf8;
@@ -576,6 +579,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const uint32_t rs = C->rs, gen_batch = C->gen_batch;
     const unsigned long long gen_iter = ((unsigned long long)gen_batch << 16) | iter;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
+        C->iter_last = iter;
         C->n_act[p ^ 1u] = 0;
         C->n_chunk[p ^ 1u] = 0;
         C->a_top[p ^ 1u] = 0;
@@ -765,6 +769,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 __global__ __launch_bounds__(1024) void k_batch_begin(const uint8_t *used, BuildCounters *C, uint32_t batch,
                                                       uint32_t n)
 {
+    if (C->hold) return;
     const uint32_t b = C->next_batch;
     const uint32_t rs = b * batch;
     const uint32_t rsz = rs < n ? min(batch, n - rs) : 0u;
@@ -801,6 +806,7 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
     // Candidates of iteration `iter` (slot iter & 1); the next frontier goes
     // to the other slot, and block 0 empties the other candidate slot for
     // the next iteration's push.
+    if (C->hold) return;
     const uint32_t p = iter & 1u;
     const uint32_t ncand = iter < kInf8 ? C->n_cand[p] : 0u;  // 0 past the batch's end
     const uint32_t rs = C->rs, rsz = C->rsz;
@@ -969,12 +975,24 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
     }
 }
 
+// First kernel of a batch's prologue: if the previous batch still has a
+// frontier (its next iteration's count in slot (iter_last + 1) & 1), this
+// batch was enqueued too early and everything up to the host's k_unhold is
+// a no-op.  A no-op before the first batch (all counts zero).
+__global__ void k_batch_gate(BuildCounters *C)
+{
+    if (!C->hold && C->n_act[(C->iter_last + 1u) & 1u] != 0u) C->hold = 1u;
+}
+
+__global__ void k_unhold(BuildCounters *C) { C->hold = 0u; }
+
 // reset_at_end (PARA:1328-1410) and the next batch's indicator reset
 // (PARA:877-901): clear the indicator rows of the touched vertices (the
 // root-distance table is reset by k_hub_table).
 __global__ __launch_bounds__(256) void k_batch_end(unsigned long long *ind, const uint32_t *tlist,
                                                    const BuildCounters *C)
 {
+    if (C->hold) return;
     const uint32_t nt = C->n_touched;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     const uint64_t tot = (uint64_t)nt * kWords;
@@ -1121,7 +1139,7 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
                        (uint64_t)(sizeof(BuildCounters) / 4), 0u);
 
     BuildCounters *h_ctr = nullptr;
-    PADO_HIP(hipHostMalloc((void **)&h_ctr, sizeof(BuildCounters), hipHostMallocDefault));
+    PADO_HIP(hipHostMalloc((void **)&h_ctr, 4 * sizeof(BuildCounters), hipHostMallocDefault));
     struct PinnedFree {
         BuildCounters *p;
         ~PinnedFree() { (void)hipHostFree(p); }
@@ -1152,6 +1170,7 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     LabelStore S{pool.p, lbase.p, llen.p, lcap.p, pool_size};
     auto enqueue_prologue = [&]() {
         // Previous batch's reset_at_end first (a no-op before the first batch).
+        hipLaunchKernelGGL(k_batch_gate, dim3(1), dim3(1), 0, st, ctr.p);
         hipLaunchKernelGGL(k_hub_table, dim3(root_blocks), dim3(256), 0, st, 0u, d_used.p, S, hinfo.p, dc.p, ctr.p);
         hipLaunchKernelGGL(k_batch_end, dim3(1024), dim3(256), 0, st, ind.p, tlist.p, ctr.p);
         hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(batch), 0, st, d_used.p, ctr.p, batch, n);
@@ -1177,9 +1196,11 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
         return PADO_GPU_OK;
     };
 
-    // A block of k iterations (k a power of two <= U) is enqueued per host
-    // check.  k follows the previous batch's iteration count, so most batches
-    // end in one block with at most a few empty iterations.
+    // A block of k <= U iterations is enqueued per host check.  A batch's
+    // first block has as many iterations as the previous batch needed (the
+    // next batch usually needs the same or fewer, and every iteration
+    // launched past a batch's end is two empty kernels); a batch that needs
+    // more continues in blocks of 2, 4, 8, ... iterations.
     // Graph mode: one captured graph per (first iteration, count), the batch
     // prologue included when first == 0; captured lazily and replayed for
     // every batch.  Kernel arguments are fixed per capture, so all graphs are
@@ -1212,21 +1233,59 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
         }
         return PADO_GPU_OK;
     };
-    auto log2_block = [&](uint32_t want) {
-        int j = 0;
-        while ((1u << j) < want && (1u << (j + 1)) <= U) ++j;
-        return j;
+    auto first_block = [&](uint32_t prev) { return std::max<uint32_t>(1u, std::min<uint32_t>(prev, U)); };
+
+    // Counter snapshots: each enqueued block of iterations is followed by a
+    // copy of the counters into its own pinned slot and an event, so the host
+    // reads a block's state while later blocks run.
+    constexpr uint32_t kSnaps = 4;  // pinned counter slots in h_ctr
+    std::vector<hipEvent_t> sev(kSnaps, nullptr);
+    struct SnapEventFree {
+        std::vector<hipEvent_t> *e;
+        ~SnapEventFree()
+        {
+            for (auto x : *e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } sevf{&sev};
+    for (auto &e : sev) PADO_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    uint32_t snap_next = 0;
+    auto snapshot = [&](uint32_t *slot) -> pado_gpu_status {
+        *slot = snap_next;
+        snap_next = (snap_next + 1) % kSnaps;
+        PADO_HIP(hipMemcpyAsync(&h_ctr[*slot], ctr.p, sizeof(BuildCounters), hipMemcpyDeviceToHost, st));
+        PADO_HIP(hipEventRecord(sev[*slot], st));
+        return PADO_GPU_OK;
+    };
+    auto launch_block = [&](uint32_t first, uint32_t count, bool timed) -> pado_gpu_status {
+        if (use_graph) {
+            hipGraphExec_t &gx = graphs[std::make_pair(first, count)];
+            if (!gx) PADO_TRY(capture(first, count, &gx));
+            PADO_HIP(hipGraphLaunch(gx, st));
+        } else {
+            if (first == 0) enqueue_prologue();
+            PADO_TRY(enqueue_iterations(first, count, timed));
+        }
+        return PADO_GPU_OK;
     };
 
+    // Graph mode runs one batch ahead when the label pool has ample room: the
+    // next batch's first block is enqueued before the host has checked the
+    // current one, and its gate makes it a no-op if the current batch turns
+    // out to need more iterations (the host then clears the gate, runs them,
+    // and enqueues the next batch again).
     const uint32_t nb = (n + batch - 1) / batch;
     uint64_t iterations_seen = 0;
     uint32_t prev_batch_iters = 1;
+    bool ahead = false;               // batch b's first block is already enqueued
+    uint32_t ahead_count = 0, ahead_slot = 0;
+    unsigned long long top_prev = 0, growth = ~0ull;  // pool use after the last batch, and its growth
     for (uint32_t b = 0; b < nb; ++b) {
         const uint32_t batch_end = std::min<uint64_t>((uint64_t)(b + 1) * batch, n);
         if (batch_end > dc_rows) {
-            // Ordered after all earlier work: hipFree/hipMalloc synchronise.
-            // The new table starts all 255, so the previous batch's reset is
-            // a no-op on it.
+            // Never with a block ahead: a batch is only enqueued early when the
+            // table already covers it.  hipFree/hipMalloc synchronise.  The new
+            // table starts all 255, so the previous batch's reset is a no-op on it.
             PADO_HIP(hipStreamSynchronize(st));
             dc_rows = std::min<uint32_t>(n, std::max<uint32_t>(batch_end, dc_rows * 2));
             PADO_TRY(dc.alloc((uint64_t)dc_rows * kBatch));
@@ -1235,27 +1294,37 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
             drop_graphs();
         }
         const uint64_t batch_first_iter = iterations_seen;
-        uint32_t launched = 0;  // iterations of this batch enqueued so far
-        int j = log2_block(prev_batch_iters + 1);
-        for (;;) {
+        uint32_t next_count = std::min<uint32_t>(2u, U);  // a continuation block's size
+        bool repacked = false;
+        unsigned long long last_top = 0;
+        uint32_t launched, cur_slot, cur_count;  // cur_count: iterations of the block being checked
+        if (ahead) {
+            launched = cur_count = ahead_count;
+            cur_slot = ahead_slot;
+            ahead = false;
+        } else {
             // Iteration 255 is the last one ever launched: k_push flags it if
             // the frontier is still live there.
-            const uint32_t count = std::min<uint32_t>(1u << j, (uint32_t)kInf8 - launched);
-            if (use_graph) {
-                hipGraphExec_t &gx = graphs[std::make_pair(launched, count)];
-                if (!gx) PADO_TRY(capture(launched, count, &gx));
-                PADO_HIP(hipGraphLaunch(gx, st));
-            } else {
-                if (launched == 0) enqueue_prologue();
-                PADO_TRY(enqueue_iterations(launched, count, true));
-            }
-            launched += count;
-            PADO_HIP(hipMemcpyAsync(h_ctr, ctr.p, sizeof(BuildCounters), hipMemcpyDeviceToHost, st));
-            PADO_HIP(hipStreamSynchronize(st));
+            launched = cur_count = std::min<uint32_t>(first_block(prev_batch_iters), (uint32_t)kInf8);
+            PADO_TRY(launch_block(0, launched, true));
+            PADO_TRY(snapshot(&cur_slot));
+        }
+        const uint32_t next_end = std::min<uint64_t>((uint64_t)(b + 2) * batch, n);
+        bool spec = use_graph && b + 1 < nb && next_end <= dc_rows && growth != ~0ull &&
+                    (top_prev + 4 * growth) * 5 < pool_size * 2;
+        if (spec) {
+            ahead_count = std::min<uint32_t>(first_block(prev_batch_iters), (uint32_t)kInf8);
+            PADO_TRY(launch_block(0, ahead_count, false));
+            PADO_TRY(snapshot(&ahead_slot));
+        }
+        for (;;) {
+            PADO_HIP(hipEventSynchronize(sev[cur_slot]));
             PADO_HIP(hipGetLastError());
+            const BuildCounters snap = h_ctr[cur_slot];
+            last_top = snap.pool_top;
             if (!use_graph) {
-                const uint64_t live = h_ctr->iterations - iterations_seen;
-                for (uint64_t u = 0; u < live && u < count; ++u) {
+                const uint64_t live = snap.iterations - iterations_seen;
+                for (uint64_t u = 0; u < live && u < cur_count; ++u) {
                     float t_push = 0.f, t_add = 0.f;
                     PADO_HIP(hipEventElapsedTime(&t_push, ev[4 * u], ev[4 * u + 1]));
                     PADO_HIP(hipEventElapsedTime(&t_add, ev[4 * u + 2], ev[4 * u + 3]));
@@ -1265,18 +1334,19 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
                     kt.n_add += 1;
                 }
             }
-            iterations_seen = h_ctr->iterations;
-            if (h_ctr->overflow) {
+            iterations_seen = snap.iterations;
+            if (snap.overflow) {
                 set_error("label pool exhausted");
                 return PADO_GPU_ERR_NOMEM;
             }
-            if (h_ctr->domain) {
+            if (snap.domain) {
                 set_error("batch iteration reached 255: distances outside the uint8 domain of the reference");
                 return PADO_GPU_ERR_DOMAIN;
             }
             // Keep at least 60% of the pool free for the next block of
-            // iterations (the kernels are idle here).
-            if (h_ctr->pool_top * 5 > pool_size * 2) {
+            // iterations; with a block ahead, it runs to completion first.
+            if (snap.pool_top * 5 > pool_size * 2) {
+                PADO_HIP(hipStreamSynchronize(st));
                 unsigned long long top = 0;
                 PADO_TRY(pool_repack(st, n, pool, lbase, llen, lcap, pool_size * 2, &top));
                 pool_size = pool.n;
@@ -1285,16 +1355,39 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
                 PADO_HIP(hipMemcpyAsync(&ctr.p->pool_top, &top, sizeof(top), hipMemcpyHostToDevice, st));
                 PADO_HIP(hipStreamSynchronize(st));
                 drop_graphs();
+                repacked = true;
+                // The block ahead's snapshot predates the repack: take it again.
+                if (spec) PADO_TRY(snapshot(&ahead_slot));
             }
             // The frontier of the next iteration is in slot (launched + 1) & 1.
-            if (h_ctr->n_act[(launched + 1) & 1u] == 0) break;
+            if (snap.n_act[(launched + 1) & 1u] == 0) break;
             if (launched >= kInf8) {
                 set_error("batch iteration reached 255: distances outside the uint8 domain of the reference");
                 return PADO_GPU_ERR_DOMAIN;
             }
-            if ((1u << (j + 1)) <= U) ++j;  // the batch runs long: double the block
+            // The batch continues: a block enqueued ahead was gated off.
+            if (spec) {
+                hipLaunchKernelGGL(k_unhold, dim3(1), dim3(1), 0, st, ctr.p);
+                spec = false;
+            }
+            cur_count = std::min<uint32_t>(next_count, (uint32_t)kInf8 - launched);
+            next_count = std::min<uint32_t>(2 * next_count, U);  // the batch runs long: double the block
+            PADO_TRY(launch_block(launched, cur_count, true));
+            PADO_TRY(snapshot(&cur_slot));
+            launched += cur_count;
         }
         prev_batch_iters = (uint32_t)(iterations_seen - batch_first_iter);
+        // Pool growth of this batch (unknown across a repack, which blocks
+        // running ahead until a batch has been measured again).
+        if (repacked) {
+            growth = ~0ull;
+            top_prev = 0;
+        } else {
+            const unsigned long long top_now = last_top;
+            growth = top_now > top_prev ? top_now - top_prev : 0ull;
+            top_prev = top_now;
+        }
+        ahead = spec;
     }
     hipLaunchKernelGGL(k_batch_end, dim3(1024), dim3(256), 0, st, ind.p, tlist.p, ctr.p);
     PADO_HIP(hipMemcpyAsync(h_ctr, ctr.p, sizeof(BuildCounters), hipMemcpyDeviceToHost, st));

@@ -102,7 +102,10 @@ struct BuildCounters {
     uint32_t n_chunk[2];   // push work descriptors of the iteration
     uint32_t n_cand[2];    // candidate vertices of the iteration
     uint32_t n_touched;    // vertices whose indicator row was touched this batch
-    uint32_t pad0;
+    // Batch gate: a batch enqueued before the host has seen the previous one
+    // finish runs as no-ops if the previous batch still has a frontier.
+    uint32_t hold;         // set: every batch kernel returns at once
+    uint32_t iter_last;    // last iteration k_push ran (not held)
     uint32_t overflow;     // label pool exhausted
     uint32_t dq_rows;      // root-distance rows scanned (statistic)
     unsigned long long pool_top;  // next free label slot in the pool
