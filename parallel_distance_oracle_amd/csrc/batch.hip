@@ -62,6 +62,11 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long x)
 // edge offset is past every adjacency list, so k_push skips it.
 constexpr unsigned long long kChunkPad = 0xFFFFFFF0ull << 32;
 
+// k_add reserves next-frontier slots per wave: kAlistGrab active-list
+// entries and up to kChunkGrab push descriptors per atomic on the shared
+// counters (the unused rest of a reservation is a hole / padding).
+constexpr uint32_t kAlistGrab = 64, kChunkGrab = 64;
+
 // Number of neighbours of v with id >= rs: the adjacency is sorted
 // descending, so they form a prefix (PARA:1121-1123 returns at the first tail
 // below roots_start).  Wave-cooperative 64-ary search.
@@ -342,11 +347,17 @@ __device__ inline void load_dline(const uint8_t *bpd, uint32_t v, uint32_t (&d)[
     d[12] = e.x;
 }
 
+// A vertex's label array in the pool: one 16-byte record, so the three
+// fields come in with one access.
+struct alignas(16) LRec {
+    unsigned long long base;  // first pool slot
+    uint32_t len;             // labels held
+    uint32_t cap;             // slots owned
+};
+
 struct LabelStore {
     uint32_t *pool;
-    unsigned long long *base;
-    uint32_t *len;
-    uint32_t *cap;
+    LRec *rec;
     unsigned long long pool_size;
 };
 
@@ -370,8 +381,8 @@ __device__ inline unsigned long long store_reserve_at(LabelStore S, BuildCounter
     }
     for (uint32_t j = lane; j < len; j += 64) S.pool[nb + j] = S.pool[base + j];
     if (lane == 0) {
-        S.base[v] = nb;
-        S.cap[v] = ncap;
+        S.rec[v].base = nb;
+        S.rec[v].cap = ncap;
     }
     return nb;
 }
@@ -379,7 +390,8 @@ __device__ inline unsigned long long store_reserve_at(LabelStore S, BuildCounter
 __device__ inline unsigned long long store_reserve(LabelStore S, BuildCounters *C, uint32_t v, uint32_t len,
                                                    uint32_t add)
 {
-    return store_reserve_at(S, C, v, len, S.base[v], S.cap[v], add);
+    const LRec r = S.rec[v];
+    return store_reserve_at(S, C, v, len, r.base, r.cap, add);
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint64_t count, uint32_t val)
@@ -450,12 +462,12 @@ __global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const 
     if (ro < rsz && lane < 8) rmask[lane * kBatch + ro] = lmask[(uint64_t)r * 8 + lane];
     uint32_t pdeg = 0;
     if (ro < rsz && !used[r]) {
-        const uint32_t len = S.len[r];
+        const uint32_t len = S.rec[r].len;
         const unsigned long long base = store_reserve(S, C, r, len, 1);
         if (base != ~0ull) {
             if (lane == 0) {
                 S.pool[base + len] = r << 8;  // (hub r, dist 0), PARA:932-949
-                S.len[r] = len + 1;
+                S.rec[r].len = len + 1;
                 ind[(uint64_t)r * kWords + (ro >> 6)] |= 1ull << (ro & 63);
                 alist[ro] = r;
                 amask[ro] = 1u << (ro >> 6);
@@ -499,8 +511,9 @@ __global__ __launch_bounds__(256) void k_hub_table(uint32_t fill, const uint8_t 
     if (ro >= rsz) return;
     const uint32_t r = rs + ro;
     if (used[r]) return;
-    const uint32_t len = S.len[r];
-    const unsigned long long base = S.base[r];
+    const LRec rec = S.rec[r];
+    const uint32_t len = rec.len;
+    const unsigned long long base = rec.base;
     for (uint32_t j0 = 0; j0 < len; j0 += 256) {
         uint32_t lab[4], info[4];
 #pragma unroll
@@ -816,7 +829,6 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
     // kAlistGrab / kChunkGrab at a time per wave, so the shared counters see
     // one atomic per reservation, not per vertex; unused descriptor slots are
     // padded.  The exact active count is added once per wave at the end.
-    constexpr uint32_t kAlistGrab = 64, kChunkGrab = 64;
     uint32_t a_next = 0, a_left = 0, c_next = 0, c_left = 0, my_act = 0;
     // The candidate list is read one vertex ahead, and everything that
     // depends only on the vertex (candidate word, label-store entry, the
@@ -826,9 +838,10 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
     for (uint32_t i = wave; i < ncand; i += nwaves) {
         const uint32_t v = v_next;
         const unsigned long long word = cand[(uint64_t)v * kWords + (lane >> 2)];
-        const uint32_t len = S.len[v];
-        const unsigned long long base = S.base[v];
-        const uint32_t cap = S.cap[v];
+        const LRec rec = S.rec[v];
+        const uint32_t len = rec.len;
+        const unsigned long long base = rec.base;
+        const uint32_t cap = rec.cap;
         const bool v_used = used[v] != 0;
         const uint64_t e0 = off[v], e1 = off[v + 1];
         if (i + nwaves < ncand) v_next = clist[i + nwaves];
@@ -913,7 +926,7 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
             const uint32_t q = __ffs(sb) - 1;
             S.pool[nb + pos++] = ((my_lo + q) << 8) | iter;  // insert_label_only, PARA:1285
         }
-        if (lane == 0) S.len[v] = len + total;
+        if (lane == 0) S.rec[v].len = len + total;
         // Roots of this batch also record the new label in the root table
         // (dist_matrix[v_root][cand_real] = iter, PARA:1287-1290).
         if (v >= rs && v < rs + rsz) {
@@ -1001,45 +1014,42 @@ __global__ __launch_bounds__(256) void k_batch_end(unsigned long long *ind, cons
 }
 
 // Pool compaction: copy each vertex's labels into a fresh pool.
-__global__ __launch_bounds__(256) void k_pool_move(uint32_t n, const uint32_t *src, const unsigned long long *obase,
-                                                   const uint32_t *len, const uint64_t *nbase, uint32_t *dst,
-                                                   unsigned long long *base_out, uint32_t *cap_out,
-                                                   const uint32_t *cap_new)
+__global__ __launch_bounds__(256) void k_pool_move(uint32_t n, const uint32_t *src, LRec *rec,
+                                                   const uint64_t *nbase, uint32_t *dst, const uint32_t *cap_new)
 {
     const uint32_t lane = lane_id();
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     for (uint32_t v = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; v < n; v += nwaves) {
-        const uint32_t L = len[v];
-        const unsigned long long ob = obase[v];
+        const LRec r = rec[v];
         const uint64_t nb = nbase[v];
-        for (uint32_t j = lane; j < L; j += 64) dst[nb + j] = src[ob + j];
+        for (uint32_t j = lane; j < r.len; j += 64) dst[nb + j] = src[r.base + j];
         if (lane == 0) {
-            base_out[v] = nb;
-            cap_out[v] = cap_new[v];
+            rec[v].base = nb;
+            rec[v].cap = cap_new[v];
         }
     }
 }
 
-__global__ void k_cap_for_len(uint32_t n, const uint32_t *len, uint32_t *cap, uint32_t exact)
+// Per-vertex pool slots for a repack (exact = 0) or the label counts (1).
+__global__ void k_cap_for_len(uint32_t n, const LRec *rec, uint32_t *cap, uint32_t exact)
 {
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
-        uint32_t L = len[v];
+        uint32_t L = rec[v].len;
         cap[v] = exact ? L : (L + (L >> 2) + 4);
     }
 }
 
 // Final CSR gather (order is fixed by the segmented sort that follows).
-__global__ __launch_bounds__(256) void k_gather_labels(uint32_t n, const uint32_t *pool,
-                                                       const unsigned long long *base, const uint64_t *loff,
-                                                       uint32_t *lab)
+__global__ __launch_bounds__(256) void k_gather_labels(uint32_t n, const uint32_t *pool, const LRec *rec,
+                                                       const uint64_t *loff, uint32_t *lab)
 {
     const uint32_t lane = lane_id();
     const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
     for (uint32_t v = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; v < n; v += nwaves) {
         const uint64_t o = loff[v];
         const uint32_t L = (uint32_t)(loff[v + 1] - o);
-        const unsigned long long b = base[v];
+        const unsigned long long b = rec[v].base;
         for (uint32_t j = lane; j < L; j += 64) lab[o + j] = pool[b + j];
     }
 }
@@ -1047,23 +1057,22 @@ __global__ __launch_bounds__(256) void k_gather_labels(uint32_t n, const uint32_
 }  // namespace
 
 // Grows (or just compacts) the label pool between batches.
-static pado_gpu_status pool_repack(hipStream_t st, uint32_t n, DevBuf<uint32_t> &pool, DevBuf<unsigned long long> &base,
-                                   DevBuf<uint32_t> &len, DevBuf<uint32_t> &cap, unsigned long long new_size,
-                                   unsigned long long *new_top)
+static pado_gpu_status pool_repack(hipStream_t st, uint32_t n, DevBuf<uint32_t> &pool, DevBuf<LRec> &rec,
+                                   unsigned long long new_size, unsigned long long *new_top)
 {
     DevBuf<uint32_t> ncap;
     DevBuf<uint64_t> nbase;
     PADO_TRY(ncap.alloc(n));
     PADO_TRY(nbase.alloc((uint64_t)n + 1));
-    hipLaunchKernelGGL(k_cap_for_len, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, len.p, ncap.p, 0u);
+    hipLaunchKernelGGL(k_cap_for_len, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, rec.p, ncap.p, 0u);
     PADO_TRY(scan_u32_to_u64(ncap.p, nbase.p, n, st));
     uint64_t need = 0;
     PADO_HIP(hipMemcpy(&need, nbase.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost));
     if (new_size < need * 2) new_size = need * 2;
     DevBuf<uint32_t> npool;
     PADO_TRY(npool.alloc(new_size));
-    hipLaunchKernelGGL(k_pool_move, dim3(grid_for((uint64_t)n * 64, 256, 8192)), dim3(256), 0, st, n, pool.p, base.p,
-                       len.p, nbase.p, npool.p, base.p, cap.p, ncap.p);
+    hipLaunchKernelGGL(k_pool_move, dim3(grid_for((uint64_t)n * 64, 256, 8192)), dim3(256), 0, st, n, pool.p, rec.p,
+                       nbase.p, npool.p, ncap.p);
     PADO_HIP(hipStreamSynchronize(st));
     std::swap(pool.p, npool.p);
     std::swap(pool.n, npool.n);
@@ -1082,8 +1091,9 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
         return PADO_GPU_ERR_ARG;
     }
 
-    DevBuf<unsigned long long> ind, cand, ag, chunks, lbase;
-    DevBuf<uint32_t> tlist, clist, alist, amask, aend, hinfo, pool, llen, lcap;
+    DevBuf<unsigned long long> ind, cand, ag, chunks;
+    DevBuf<uint32_t> tlist, clist, alist, amask, aend, hinfo, pool;
+    DevBuf<LRec> lrec;
     DevBuf<unsigned long long> vmark;  // ((batch << 16 | iteration) << 1 | candidate) of the last mark
     DevBuf<uint8_t> dc;
     DevBuf<ulonglong2> rset;  // batch roots' bit-parallel sets, slot-major [kBP][kBatch]
@@ -1097,18 +1107,16 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     // the active count by kAlistGrab slots per wave, and padded descriptors
     // are fewer than the real ones plus kChunkGrab per wave.
     const uint64_t add_waves = 4ull * (cfg.add_blocks ? cfg.add_blocks : kAddGrid);
-    const uint64_t alist_cap = (uint64_t)n + 64 * add_waves;
+    const uint64_t alist_cap = (uint64_t)n + kAlistGrab * add_waves;
     PADO_TRY(ag.alloc(alist_cap * kWords));
-    PADO_TRY(chunks.alloc(2 * (g->m / kChunk + (uint64_t)n) + 64 * add_waves + 64));
+    PADO_TRY(chunks.alloc(2 * (g->m / kChunk + (uint64_t)n) + kChunkGrab * add_waves + 64));
     PADO_TRY(tlist.alloc(n));
     PADO_TRY(clist.alloc(n));
     PADO_TRY(alist.alloc(alist_cap));
     PADO_TRY(amask.alloc(alist_cap));
     PADO_TRY(aend.alloc(alist_cap));
     PADO_TRY(hinfo.alloc(n));
-    PADO_TRY(lbase.alloc(n));
-    PADO_TRY(llen.alloc(n));
-    PADO_TRY(lcap.alloc(n));
+    PADO_TRY(lrec.alloc(n));
     PADO_TRY(rset.alloc((uint64_t)kBP * kBatch));
     PADO_TRY(lmask.alloc((uint64_t)n * 8));
     PADO_TRY(rmask.alloc(8ull * kBatch));
@@ -1132,9 +1140,8 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, vmark.p, (uint64_t)n);
     hipLaunchKernelGGL(k_bp_masks, dim3(fg), dim3(256), 0, st, n, idx->bpd.p, lmask.p);
     hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, hinfo.p, (uint64_t)n, kEmpty);
-    hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, llen.p, (uint64_t)n, 0u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, lcap.p, (uint64_t)n, 0u);
-    hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, lbase.p, (uint64_t)n);
+    hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, reinterpret_cast<unsigned long long *>(lrec.p),
+                       2ull * n);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t *>(ctr.p),
                        (uint64_t)(sizeof(BuildCounters) / 4), 0u);
 
@@ -1167,7 +1174,7 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     }
     KernelTimes kt{};
 
-    LabelStore S{pool.p, lbase.p, llen.p, lcap.p, pool_size};
+    LabelStore S{pool.p, lrec.p, pool_size};
     auto enqueue_prologue = [&]() {
         // Previous batch's reset_at_end first (a no-op before the first batch).
         hipLaunchKernelGGL(k_batch_gate, dim3(1), dim3(1), 0, st, ctr.p);
@@ -1348,7 +1355,7 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
             if (snap.pool_top * 5 > pool_size * 2) {
                 PADO_HIP(hipStreamSynchronize(st));
                 unsigned long long top = 0;
-                PADO_TRY(pool_repack(st, n, pool, lbase, llen, lcap, pool_size * 2, &top));
+                PADO_TRY(pool_repack(st, n, pool, lrec, pool_size * 2, &top));
                 pool_size = pool.n;
                 S.pool = pool.p;
                 S.pool_size = pool_size;
@@ -1402,13 +1409,16 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     // Canonical CSR: gather each vertex's labels and sort them by hub
     // (order_labels' per-vertex std::sort, PARA:2240).
     PADO_TRY(idx->loff.alloc((uint64_t)n + 1));
-    PADO_TRY(scan_u32_to_u64(llen.p, idx->loff.p, n, st));
+    DevBuf<uint32_t> lens;  // label counts (lives until the end: the scan is asynchronous)
+    PADO_TRY(lens.alloc(n));
+    hipLaunchKernelGGL(k_cap_for_len, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, lrec.p, lens.p, 1u);
+    PADO_TRY(scan_u32_to_u64(lens.p, idx->loff.p, n, st));
     uint64_t total = 0;
     PADO_HIP(hipMemcpy(&total, idx->loff.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost));
     idx->num_labels = total;
     PADO_TRY(idx->lab.alloc(total));
     hipLaunchKernelGGL(k_gather_labels, dim3(grid_for((uint64_t)n * 64, 256, 8192)), dim3(256), 0, st, n, pool.p,
-                       lbase.p, idx->loff.p, idx->lab.p);
+                       lrec.p, idx->loff.p, idx->lab.p);
     PADO_TRY(segmented_sort_u32(idx->lab.p, total, idx->loff.p, n, st));
     return PADO_GPU_OK;
 }
