@@ -115,6 +115,27 @@ def serving_leg(ctx, scale=14, ef=8, duration_s=4.0, seed=5):
     return res
 
 
+def weighted_build_only(scale, ef=16):
+    """Weighted R-MAT build alone (warmup + one timed build), for profiler runs."""
+    ctx = pado.Context(0)
+    edges = pado.rmat_edges(scale, ef, 0.57, 0.19, 0.19, 1)
+    w = pado.edge_weights(edges.shape[0], 1, 100, seed=1)
+    G = pado.WeightedGraph.from_edges(edges, w)
+    G.id_transfer(G.make_rank())
+    dg = pado.DeviceGraph(ctx, G)
+    ctx.sync()
+    times = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        idx = pado.Index.build(ctx, dg)
+        ctx.sync()
+        times.append(time.perf_counter() - t0)
+        st = idx.stats()
+        idx.close()
+    print(json.dumps({"metric": "weighted R-MAT build only (profiling run)", "scale": scale, "value": round(times[-1], 4),
+                      "unit": "s", "labels": st["num_labels"], "rounds": st["iterations"]}), flush=True)
+
+
 def weighted_leg(ctx, scale, ef=16, duration_s=4.0, seed=7, dijkstra_sources=4):
     """Weighted variant: R-MAT `scale` with integer weights U[1,100] from the
     seed, weighted index built and kept resident, served with the same request
@@ -274,9 +295,14 @@ def main():
                     help="run the batch step as per-kernel launches with HIP-event timing instead of the default hipGraphs")
     ap.add_argument("--weighted-scale", type=int, default=0,
                     help="also build and serve a weighted R-MAT of this scale (0 = skip)")
+    ap.add_argument("--weighted-build-only", type=int, default=0, metavar="SCALE",
+                    help="only build the weighted R-MAT of this scale (twice: warmup + timed) and exit (profiler runs)")
     args = ap.parse_args()
     if args.copy_calibrate:
         copy_calibrate(args.copy_calibrate)
+        return
+    if args.weighted_build_only:
+        weighted_build_only(args.weighted_build_only)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the index build and "

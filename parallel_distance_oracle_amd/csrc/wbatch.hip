@@ -32,6 +32,8 @@
 // the earlier-batch hub table (hslot, hmin, dcw[slot][B] u32), and the
 // frontier as (vertex, root bitmap) entries with 16-edge push chunks.  Labels
 // are packed u64 (hub << 32 | dist) in per-vertex growable arrays.
+#include <cstring>
+
 #include "internal.hpp"
 
 namespace pado {
@@ -211,8 +213,44 @@ __global__ __launch_bounds__(256) void k_w_hub_fill(uint32_t rs, uint32_t rsz, W
     }
 }
 
+// Position of the rank-th set bit of m (rank < popcount(m)).
+__device__ inline uint32_t w_nth_set_bit(u64 m, uint32_t rank)
+{
+    uint32_t pos = 0;
+    uint32_t c = __popc((uint32_t)m);
+    if (rank >= c) {
+        rank -= c;
+        m >>= 32;
+        pos = 32;
+    }
+    uint32_t x = (uint32_t)m;
+#pragma unroll
+    for (uint32_t width = 16; width; width >>= 1) {
+        c = __popc(x & ((1u << width) - 1u));
+        if (rank >= c) {
+            rank -= c;
+            x >>= width;
+            pos += width;
+        }
+    }
+    return pos;
+}
+
+// Orders LDS writes of some lanes of a wave before reads by other lanes.
+__device__ inline void w_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+constexpr uint32_t kWPushWaves = 4;  // waves per k_w_push block
+
 // send_messages (WPARA:435-522): relax the frontier's edges toward tails
-// ranked below each root.
+// ranked below each root.  A wave takes four 16-edge chunks (one edge per
+// lane) and walks the root words of its edges' groups in step; per word the
+// (edge, root) relaxations of all 64 lanes are spread over the wave, 64 per
+// step, so an edge with many roots does not hold up the others.
 __global__ __launch_bounds__(256) void k_w_push(uint32_t round, uint32_t rs, uint32_t gen_batch,
                                                 const uint64_t *off, const uint32_t *adj, const uint8_t *wgt,
                                                 const uint32_t *alist, const u64 *ag, const uint32_t *amask,
@@ -220,8 +258,11 @@ __global__ __launch_bounds__(256) void k_w_push(uint32_t round, uint32_t rs, uin
                                                 uint32_t *tmark, uint32_t *tlist, u64 *imark, uint32_t *ilist,
                                                 WCounters *C)
 {
-    const uint32_t lane = threadIdx.x & (kChunk - 1);
-    const uint32_t ngroups = (gridDim.x * blockDim.x) / kChunk;
+    __shared__ u64 s_g[kWPushWaves][64], s_imp[kWPushWaves][64];
+    __shared__ uint32_t s_ex[kWPushWaves][64], s_u[kWPushWaves][64], s_t[kWPushWaves][64], s_w[kWPushWaves][64];
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+    const uint32_t wave = blockIdx.x * kWPushWaves + wv;
+    const uint32_t nwaves = gridDim.x * kWPushWaves;
     // Round `round` of the batch (slot round & 1; empty past the batch's
     // end).  Block 0 empties the other slot, which k_w_check fills.
     const uint32_t p = round & 1u;
@@ -232,39 +273,73 @@ __global__ __launch_bounds__(256) void k_w_push(uint32_t round, uint32_t rs, uin
         C->n_act[p ^ 1u] = 0;
         C->n_chunk[p ^ 1u] = 0;
     }
-    u64 n_relax = 0, n_imp = 0;
-    for (uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) / kChunk; c < nc; c += ngroups) {
-        const u64 desc = chunks[c];
-        const uint32_t k = (uint32_t)desc;
-        const uint32_t e = (uint32_t)(desc >> 32) + lane;
-        if (e >= aend[k]) continue;
-        const uint32_t u = alist[k];
-        const uint64_t ei = off[u] + e;
-        const uint32_t t = adj[ei];
-        const uint32_t w = wgt[ei];
-        const uint32_t lim = t - rs;  // roots with offset < lim rank above t
-        bool got = false;
-        for (uint32_t wm = amask[k]; wm; wm &= wm - 1) {
-            const uint32_t wd = __ffs(wm) - 1;
-            if (lim <= wd * 64) break;
-            u64 g = ag[(uint64_t)k * kWW + wd];
-            if (lim < (wd + 1) * 64) g &= (1ull << (lim - wd * 64)) - 1ull;
-            u64 imp = 0;
-            while (g) {
-                const uint32_t b = __ffsll(g) - 1;
-                g &= g - 1;
-                const uint32_t ro = wd * 64 + b;
-                const uint32_t du = T[(uint64_t)u * kWB + ro];
-                if (du == kWInf) continue;
-                const uint32_t cand = du + w;
-                ++n_relax;
-                const uint32_t old = atomicMin(&T[(uint64_t)t * kWB + ro], cand);
-                if (cand < old) imp |= 1ull << b;
+    uint32_t n_relax = 0, n_imp = 0;
+    for (uint32_t c0 = wave * 4; c0 < nc; c0 += nwaves * 4) {
+        const uint32_t c = c0 + (lane >> 4);
+        uint32_t k = 0, u = 0, t = 0, w = 0, lim = 0, words = 0;
+        if (c < nc) {
+            const u64 desc = chunks[c];
+            k = (uint32_t)desc;
+            const uint32_t e = (uint32_t)(desc >> 32) + (lane & (kChunk - 1));
+            if (e < aend[k]) {
+                u = alist[k];
+                const uint64_t ei = off[u] + e;
+                t = adj[ei];
+                w = wgt[ei];
+                lim = t - rs;  // roots with offset < lim rank above t
+                words = amask[k] & (lim >= kWB ? ((1u << kWW) - 1u) : ((1u << ((lim + 63) / 64)) - 1u));
             }
+        }
+        s_u[wv][lane] = u;
+        s_t[wv][lane] = t;
+        s_w[wv][lane] = w;
+        uint32_t wall = words;
+        for (int o = 32; o > 0; o >>= 1) wall |= __shfl_xor(wall, o);
+        bool got = false;
+        while (wall) {
+            const uint32_t wd = __ffs(wall) - 1;
+            wall &= wall - 1;
+            u64 g = 0;
+            if ((words >> wd) & 1u) {
+                g = ag[(uint64_t)k * kWW + wd];
+                if (lim < (wd + 1) * 64) g &= (1ull << (lim - wd * 64)) - 1ull;
+            }
+            const uint32_t cnt = (uint32_t)__popcll(g);
+            uint32_t incl = cnt;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(incl, o);
+                if (lane >= (uint32_t)o) incl += y;
+            }
+            const uint32_t total = __shfl(incl, 63);
+            if (total == 0) continue;
+            s_g[wv][lane] = g;
+            s_ex[wv][lane] = incl - cnt;
+            s_imp[wv][lane] = 0ull;
+            w_lds_sync();
+            for (uint32_t x0 = 0; x0 < total; x0 += 64) {
+                const uint32_t x = x0 + lane;
+                if (x >= total) continue;
+                uint32_t o = 0;
+#pragma unroll
+                for (uint32_t sft = 32; sft; sft >>= 1)
+                    if (s_ex[wv][o + sft] <= x) o += sft;
+                const uint32_t b = w_nth_set_bit(s_g[wv][o], x - s_ex[wv][o]);
+                const uint32_t ro = wd * 64 + b;
+                const uint32_t du = T[(uint64_t)s_u[wv][o] * kWB + ro];
+                if (du == kWInf) continue;
+                const uint32_t cand = du + s_w[wv][o];
+                ++n_relax;
+                const uint32_t old = atomicMin(&T[(uint64_t)s_t[wv][o] * kWB + ro], cand);
+                if (cand < old) atomicOr(&s_imp[wv][o], 1ull << b);
+            }
+            w_lds_sync();
+            const u64 imp = s_imp[wv][lane];
             if (imp) {
                 atomicOr(&I[(uint64_t)t * kWW + wd], imp);
                 got = true;
             }
+            // The next word's writes must not overtake this word's reads.
+            w_lds_sync();
         }
         const bool new_t = got && tmark[t] != gen_batch && atomicExch(&tmark[t], gen_batch) != gen_batch;
         const bool new_i = got && imark[t] != gen_round && atomicExch(&imark[t], gen_round) != gen_round;
@@ -275,16 +350,15 @@ __global__ __launch_bounds__(256) void k_w_push(uint32_t round, uint32_t rs, uin
             ilist[ii] = t;
             ++n_imp;
         }
+        // s_u / s_t / s_w are rewritten by the next step.
+        w_lds_sync();
     }
-    n_relax = wave_sum(n_relax);
-    n_imp = wave_sum(n_imp);
-    if ((threadIdx.x & 63) == 0) {
-        if (n_relax) atomicAdd(&C->relax, n_relax);
-        if (n_imp) atomicAdd(&C->improved, n_imp);
+    const u64 t_relax = wave_sum(n_relax), t_imp = wave_sum(n_imp);
+    if (lane == 0) {
+        if (t_relax) atomicAdd(&C->relax, t_relax);
+        if (t_imp) atomicAdd(&C->improved, t_imp);
     }
 }
-
-
 
 // distance_query's first loop (WPARA:569-580) for every improved pair of an
 // improved tail: one wave per tail, lane l owns roots 8l..8l+7.  Pruned pairs
@@ -582,6 +656,7 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
     PADO_HIP(hipMemsetAsync(ctr.p, 0, sizeof(WCounters), st));
     WCounters *h = nullptr;
     PADO_HIP(hipHostMalloc((void **)&h, sizeof(WCounters), hipHostMallocDefault));
+    std::memset(h, 0, sizeof(WCounters));
     struct PinnedFree {
         WCounters *p;
         ~PinnedFree() { (void)hipHostFree(p); }
@@ -597,18 +672,19 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
         hipLaunchKernelGGL(k_w_init, dim3(rb), dim3(256), 0, st, rs, rsz, gen_batch, g->off.p, g->adj.p, T.p, tmark.p,
                            tlist.p, alist.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
         hipLaunchKernelGGL(k_w_hub_mark, dim3(rb), dim3(256), 0, st, rs, rsz, S, hslot.p, hlist.p, ctr.p);
-        PADO_HIP(hipMemcpyAsync(h, ctr.p, sizeof(WCounters), hipMemcpyDeviceToHost, st));
-        PADO_HIP(hipStreamSynchronize(st));
-        if (h->n_hub > dc_slots) {
-            dc_slots = std::min<uint32_t>(n, std::max<uint32_t>(h->n_hub, dc_slots * 2));
+        // The hubs of the batch roots' labels are ids below the batch end, so
+        // that many table rows always suffice (no device round trip).
+        if (rs + rsz > dc_slots) {
+            dc_slots = std::min<uint32_t>(n, std::max<uint32_t>(rs + rsz, dc_slots * 2));
             PADO_TRY(dcw.alloc((uint64_t)dc_slots * kWB));
         }
-        hipLaunchKernelGGL(k_w_hub_slots, dim3(grid_for((uint64_t)h->n_hub * 128, 256, 8192)), dim3(256), 0, st,
-                           hlist.p, hslot.p, hmin.p, dcw.p, ctr.p);
+        hipLaunchKernelGGL(k_w_hub_slots, dim3(2048), dim3(256), 0, st, hlist.p, hslot.p, hmin.p, dcw.p, ctr.p);
         hipLaunchKernelGGL(k_w_hub_fill, dim3(rb), dim3(256), 0, st, rs, rsz, S, hslot.p, hmin.p, dcw.p);
         // Blocks of 2^j rounds per host check; j follows the previous batch's
         // round count and doubles while a batch runs long.
-        uint32_t live = h->n_act[1];
+        // Round 1's frontier is the batch's roots (never empty); the counters
+        // seen last (after the previous batch's filter) hold every round so far.
+        uint32_t live = rsz;
         const uint64_t rounds_before = h->rounds;
         uint32_t launched = 0;
         int j = 0;
@@ -620,10 +696,10 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
             }
             for (uint32_t u = 0; u < (1u << j); ++u) {
                 const uint32_t round = launched + 1 + u;
-                hipLaunchKernelGGL(k_w_push, dim3(4096), dim3(256), 0, st, round, rs, gen_batch, g->off.p, g->adj.p,
+                hipLaunchKernelGGL(k_w_push, dim3(kPushGrid), dim3(256), 0, st, round, rs, gen_batch, g->off.p, g->adj.p,
                                    g->w.p, alist.p, ag.p, amask.p, aend.p, chunks.p, T.p, I.p, tmark.p, tlist.p,
                                    imark.p, ilist.p, ctr.p);
-                hipLaunchKernelGGL(k_w_check, dim3(2048), dim3(256), 0, st, round, rs, ilist.p, g->off.p, g->adj.p,
+                hipLaunchKernelGGL(k_w_check, dim3(kAddGrid), dim3(256), 0, st, round, rs, ilist.p, g->off.p, g->adj.p,
                                    T.p, I.p, P.p, S, hslot.p, hmin.p, dcw.p, alist.p, ag.p, amask.p, aend.p,
                                    chunks.p, ctr.p);
             }
