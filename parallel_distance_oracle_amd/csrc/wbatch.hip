@@ -409,18 +409,42 @@ __global__ __launch_bounds__(256) void k_w_check(uint32_t round, uint32_t rs, co
             }
             const bool pass = slot < kEmpty - 1 && (uint64_t)d + hmin[h] <= wmax;
             u64 m = __ballot(pass);
+            // Up to four passing hubs per step: their row loads go out
+            // together (32 B per lane each), and only lanes that still hold an
+            // unpruned improved pair load.
             while (m) {
-                const int src = __ffsll(m) - 1;
-                m &= m - 1;
-                const uint32_t dd = __shfl(d, src);
-                const uint32_t sl = __shfl(slot, src);
-                const uint4 *row = reinterpret_cast<const uint4 *>(dcw + (uint64_t)sl * kWB + lane * 8);
-                const uint4 r0 = row[0], r1 = row[1];
-                const uint32_t x[8] = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+                int src[4];
+                bool ok[4];
 #pragma unroll
-                for (int q = 0; q < 8; ++q)
-                    if (((imp >> q) & 1u) && x[q] != kWInf && (uint64_t)dd + x[q] <= dv[q]) prune |= 1u << q;
+                for (int u = 0; u < 4; ++u) {
+                    ok[u] = m != 0ull;
+                    src[u] = ok[u] ? __ffsll(m) - 1 : 0;
+                    if (ok[u]) m &= m - 1;
+                }
+                const bool need = (imp & ~prune) != 0u;
+                uint32_t dd[4];
+                uint4 r0[4], r1[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    dd[u] = __shfl(d, src[u]);
+                    const uint32_t sl = __shfl(slot, src[u]);
+                    r0[u] = r1[u] = make_uint4(kWInf, kWInf, kWInf, kWInf);
+                    if (ok[u] && need) {
+                        const uint4 *row = reinterpret_cast<const uint4 *>(dcw + (uint64_t)sl * kWB + lane * 8);
+                        r0[u] = row[0];
+                        r1[u] = row[1];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t x[8] = {r0[u].x, r0[u].y, r0[u].z, r0[u].w, r1[u].x, r1[u].y, r1[u].z, r1[u].w};
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (((imp >> q) & 1u) && x[q] != kWInf && (uint64_t)dd[u] + x[q] <= dv[q]) prune |= 1u << q;
+                }
             }
+            // Every improved pair pruned: the rest of the labels cannot matter.
+            if (__ballot((imp & ~prune) != 0u) == 0ull) break;
         }
         const uint32_t live = imp & ~prune;
         n_pruned += __popc(imp & prune);
