@@ -472,13 +472,43 @@ __global__ __launch_bounds__(256) void k_w_check(uint32_t round, uint32_t rs, co
     if (lane == 0 && n_pruned) atomicAdd(&C->pruned, n_pruned);
 }
 
+// The batch roots' mutual entries, transposed for the filter: RT[h][r] =
+// T[rs + r][h] and bit r of PT[h] = pruned bit h of root rs + r, so a witness
+// h's column reads as one contiguous row.  One block per 64 x 64 tile (four
+// waves, sixteen rows each); roots past the batch read as INF / unpruned.
+__global__ __launch_bounds__(256) void k_w_roots_t(uint32_t rs, uint32_t rsz, const uint32_t *T, const u64 *P,
+                                                   uint32_t *RT, u64 *PT)
+{
+    __shared__ uint32_t tile[64][65];
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+    const uint32_t r0 = (blockIdx.x % (kWB / 64)) * 64, h0 = (blockIdx.x / (kWB / 64)) * 64;
+    for (uint32_t i = wv; i < 64; i += 4) {
+        const uint32_t r = r0 + i;
+        tile[i][lane] = r < rsz ? T[(uint64_t)(rs + r) * kWB + h0 + lane] : kWInf;
+    }
+    __syncthreads();
+    for (uint32_t j = wv; j < 64; j += 4) RT[(uint64_t)(h0 + j) * kWB + r0 + lane] = tile[lane][j];
+    // Pruned bits: lane = root r0 + lane, ballot over the roots per witness.
+    if (wv == 0) {
+        const uint32_t r = r0 + lane;
+        const u64 pw = r < rsz ? P[(uint64_t)(rs + r) * kWW + h0 / 64] : 0ull;
+        u64 mine = 0;
+        for (uint32_t h = 0; h < 64; ++h) {
+            const u64 bits = __ballot((pw >> h) & 1ull);
+            if (lane == h) mine = bits;
+        }
+        PT[(uint64_t)(h0 + lane) * kWW + r0 / 64] = mine;
+    }
+}
+
 // filter_out_labels (WPARA:1100-1228): one wave per touched vertex; surviving
 // pairs (finite, not pruned) are dropped if an earlier batch root h witnesses
 // S(v,h) + S(r,h) <= S(v,r).  Reads T/P only, so every wave sees the same
 // pre-filter state; the kept bitmap goes to I and the pool words the commit
 // may reserve are summed into C->need.
 __global__ __launch_bounds__(256) void k_w_filter(uint32_t rs, const uint32_t *tlist, const uint32_t *T,
-                                                  const u64 *P, u64 *I, WStore S, WCounters *C)
+                                                  const u64 *P, const uint32_t *RT, const u64 *PT, u64 *I, WStore S,
+                                                  WCounters *C)
 {
     __shared__ uint8_t sv[4][64];
     const uint32_t lane = lane_id();
@@ -510,15 +540,17 @@ __global__ __launch_bounds__(256) void k_w_filter(uint32_t rs, const uint32_t *t
                 word &= word - 1;
                 if (h >= lane * 8 + 7) break;  // only roots r > h of this lane can use h
                 const uint32_t dvh = T[(uint64_t)v * kWB + h];
+                // Column h of the roots' entries, 8 consecutive roots per lane.
+                const uint4 *rt = reinterpret_cast<const uint4 *>(RT + (uint64_t)h * kWB + lane * 8);
+                const uint4 ra = rt[0], rb = rt[1];
+                const uint32_t drh[8] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y, rb.z, rb.w};
+                const uint32_t prh = reinterpret_cast<const uint8_t *>(PT + (uint64_t)h * kWW)[lane];
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
                     const uint32_t ro = lane * 8 + q;
                     if (!((surv >> q) & 1u) || ro <= h) continue;
-                    const uint32_t r = rs + ro;
-                    const uint32_t drh = T[(uint64_t)r * kWB + h];
-                    const uint8_t prh = reinterpret_cast<const uint8_t *>(P + (uint64_t)r * kWW)[h >> 3];
-                    if (drh == kWInf || ((prh >> (h & 7)) & 1u)) continue;
-                    if ((uint64_t)dvh + drh <= dv[q]) drop |= 1u << q;
+                    if (drh[q] == kWInf || ((prh >> q) & 1u)) continue;
+                    if ((uint64_t)dvh + drh[q] <= dv[q]) drop |= 1u << q;
                 }
             }
         }
@@ -643,8 +675,12 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
     const uint32_t batch = (batch_req == 0 || batch_req > kWB) ? kWB : batch_req;
     DevBuf<uint32_t> T, tmark, tlist, ilist, alist, amask, aend, hslot, hmin, hlist, dcw, len, cap;
     DevBuf<u64> P, I, ag, chunks, pool, base, imark;
+    DevBuf<uint32_t> RT;  // batch roots' entries, transposed for the filter
+    DevBuf<u64> PT;
     DevBuf<WCounters> ctr;
     PADO_TRY(T.alloc((uint64_t)n * kWB));
+    PADO_TRY(RT.alloc((uint64_t)kWB * kWB));
+    PADO_TRY(PT.alloc((uint64_t)kWB * kWW));
     PADO_TRY(P.alloc((uint64_t)n * kWW));
     PADO_TRY(I.alloc((uint64_t)n * kWW));
     PADO_TRY(ag.alloc((uint64_t)n * kWW));
@@ -735,7 +771,9 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
         }
         prev_rounds = (uint32_t)(h->rounds - rounds_before);
         const u64 labels_before = h->labels;
-        hipLaunchKernelGGL(k_w_filter, dim3(2048), dim3(256), 0, st, rs, tlist.p, T.p, P.p, I.p, S, ctr.p);
+        hipLaunchKernelGGL(k_w_roots_t, dim3((kWB / 64) * (kWB / 64)), dim3(256), 0, st, rs, rsz, T.p, P.p, RT.p, PT.p);
+        hipLaunchKernelGGL(k_w_filter, dim3(2048), dim3(256), 0, st, rs, tlist.p, T.p, P.p, RT.p, PT.p, I.p, S,
+                           ctr.p);
         PADO_HIP(hipMemcpyAsync(h, ctr.p, sizeof(WCounters), hipMemcpyDeviceToHost, st));
         PADO_HIP(hipStreamSynchronize(st));
         if (h->pool_top + h->need > pool_size) {
