@@ -48,6 +48,7 @@ using u64 = unsigned long long;
 // kernels read their slot and empty the other one for the next kernel.
 struct WCounters {
     uint32_t n_act[2], n_chunk[2], n_imp[2], n_touched, n_hub, overflow, pad1;
+    uint32_t a_top[2];  // active-list slots handed out (>= n_act: per-wave reservations)
     u64 pool_top, relax, improved, pruned, labels, filtered, rounds, need;
 };
 
@@ -129,6 +130,7 @@ __device__ inline void emit_chunks(uint32_t k, uint32_t pdeg, uint32_t *n_chunk,
 __global__ void k_w_reset_counters(WCounters *C)
 {
     C->n_act[0] = C->n_act[1] = C->n_chunk[0] = C->n_chunk[1] = C->n_imp[0] = C->n_imp[1] = 0;
+    C->a_top[0] = C->a_top[1] = 0;
     C->n_touched = C->n_hub = 0;
     C->need = 0;
 }
@@ -148,7 +150,8 @@ __global__ __launch_bounds__(256) void k_w_init(uint32_t rs, uint32_t rsz, uint3
     if (lane == 0) {
         T[(uint64_t)r * kWB + ro] = 0;
         if (atomicExch(&tmark[r], gen_batch) != gen_batch) tlist[atomicAdd(&C->n_touched, 1u)] = r;
-        k = atomicAdd(&C->n_act[1], 1u);  // round 1 -> slot 1
+        k = atomicAdd(&C->a_top[1], 1u);  // round 1 -> slot 1
+        atomicAdd(&C->n_act[1], 1u);
     }
     k = __shfl(k, 0);
     if (lane < kWW) ag[(uint64_t)k * kWW + lane] = (lane == (ro >> 6)) ? (1ull << (ro & 63)) : 0ull;
@@ -246,6 +249,13 @@ __device__ inline void w_lds_sync()
 
 constexpr uint32_t kWPushWaves = 4;  // waves per k_w_push block
 
+// k_w_check reserves next-frontier slots per wave: kWAlistGrab active-list
+// entries and up to kWChunkGrab push descriptors per atomic on the shared
+// counters; a descriptor slot reserved but not used holds kWChunkPad (edge
+// offset past every list), which k_w_push skips.
+constexpr uint32_t kWAlistGrab = 64, kWChunkGrab = 64;
+constexpr u64 kWChunkPad = 0xFFFFFFF0ull << 32;
+
 // send_messages (WPARA:435-522): relax the frontier's edges toward tails
 // ranked below each root.  A wave takes four 16-edge chunks (one edge per
 // lane) and walks the root words of its edges' groups in step; per word the
@@ -271,14 +281,15 @@ __global__ __launch_bounds__(256) void k_w_push(uint32_t round, uint32_t rs, uin
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         if (C->n_act[p]) C->rounds += 1;
         C->n_act[p ^ 1u] = 0;
+        C->a_top[p ^ 1u] = 0;
         C->n_chunk[p ^ 1u] = 0;
     }
     uint32_t n_relax = 0, n_imp = 0;
     for (uint32_t c0 = wave * 4; c0 < nc; c0 += nwaves * 4) {
         const uint32_t c = c0 + (lane >> 4);
         uint32_t k = 0, u = 0, t = 0, w = 0, lim = 0, words = 0;
-        if (c < nc) {
-            const u64 desc = chunks[c];
+        const u64 desc = c < nc ? chunks[c] : kWChunkPad;
+        if (desc != kWChunkPad) {
             k = (uint32_t)desc;
             const uint32_t e = (uint32_t)(desc >> 32) + (lane & (kChunk - 1));
             if (e < aend[k]) {
@@ -376,6 +387,7 @@ __global__ __launch_bounds__(256) void k_w_check(uint32_t round, uint32_t rs, co
     const uint32_t nimp = C->n_imp[p];
     if (blockIdx.x == 0 && threadIdx.x == 0) C->n_imp[p ^ 1u] = 0;
     u64 n_pruned = 0;
+    uint32_t a_next = 0, a_left = 0, c_next = 0, c_left = 0, my_act = 0;
     for (uint32_t i = wave; i < nimp; i += nwaves) {
         const uint32_t t = ilist[i];
         uint8_t *Ib = reinterpret_cast<uint8_t *>(I + (uint64_t)t * kWW);
@@ -452,9 +464,19 @@ __global__ __launch_bounds__(256) void k_w_check(uint32_t round, uint32_t rs, co
         if (imp) Pb[lane] = (uint8_t)((Pb[lane] & ~imp) | (imp & prune));
         const u64 any = __ballot(live != 0);
         if (!any) continue;
-        uint32_t k = 0;
-        if (lane == 0) k = atomicAdd(&C->n_act[p ^ 1u], 1u);
-        k = __shfl(k, 0);
+        // Next-frontier slots come from per-wave reservations, capped by the
+        // tails this wave has left in the launch (grid-stride).
+        const uint32_t rem = (nimp - 1 - i) / nwaves + 1;
+        if (a_left == 0) {
+            const uint32_t grab = rem < kWAlistGrab ? rem : kWAlistGrab;
+            uint32_t b0 = 0;
+            if (lane == 0) b0 = atomicAdd(&C->a_top[p ^ 1u], grab);
+            a_next = __shfl(b0, 0);
+            a_left = grab;
+        }
+        const uint32_t k = a_next++;
+        --a_left;
+        ++my_act;
         reinterpret_cast<uint8_t *>(ag + (uint64_t)k * kWW)[lane] = (uint8_t)live;
         uint32_t wmask = 0;
 #pragma unroll
@@ -466,8 +488,23 @@ __global__ __launch_bounds__(256) void k_w_check(uint32_t round, uint32_t rs, co
             amask[k] = wmask;
             aend[k] = pdeg;
         }
-        emit_chunks(k, pdeg, &C->n_chunk[p ^ 1u], chunks);
+        const uint32_t nch = (pdeg + kChunk - 1) / kChunk;
+        if (nch > c_left) {
+            for (uint32_t q = lane; q < c_left; q += 64) chunks[c_next + q] = kWChunkPad;
+            const uint64_t guess = (uint64_t)nch * rem;  // descriptors left if t is typical
+            const uint32_t cap = guess < kWChunkGrab ? (uint32_t)guess : kWChunkGrab;
+            const uint32_t want = nch > cap ? nch : cap;
+            uint32_t b0 = 0;
+            if (lane == 0) b0 = atomicAdd(&C->n_chunk[p ^ 1u], want);
+            c_next = __shfl(b0, 0);
+            c_left = want;
+        }
+        for (uint32_t q = lane; q < nch; q += 64) chunks[c_next + q] = ((u64)(q * kChunk) << 32) | k;
+        c_next += nch;
+        c_left -= nch;
     }
+    for (uint32_t q = lane; q < c_left; q += 64) chunks[c_next + q] = kWChunkPad;
+    if (lane == 0 && my_act) atomicAdd(&C->n_act[p ^ 1u], my_act);
     n_pruned = wave_sum(n_pruned);
     if (lane == 0 && n_pruned) atomicAdd(&C->pruned, n_pruned);
 }
@@ -683,15 +720,20 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
     PADO_TRY(PT.alloc((uint64_t)kWB * kWW));
     PADO_TRY(P.alloc((uint64_t)n * kWW));
     PADO_TRY(I.alloc((uint64_t)n * kWW));
-    PADO_TRY(ag.alloc((uint64_t)n * kWW));
-    PADO_TRY(chunks.alloc(g->m / kChunk + (uint64_t)n + 64));
+    // Per-wave reservations in k_w_check leave holes: up to kWAlistGrab
+    // active-list slots per wave beyond the real ones, and padded
+    // descriptors fewer than the real ones plus kWChunkGrab per wave.
+    const uint64_t check_waves = 4ull * kAddGrid;
+    const uint64_t alist_cap = (uint64_t)n + kWAlistGrab * check_waves;
+    PADO_TRY(ag.alloc(alist_cap * kWW));
+    PADO_TRY(chunks.alloc(2 * (g->m / kChunk + (uint64_t)n) + kWChunkGrab * check_waves + 64));
     PADO_TRY(tmark.alloc(n));
     PADO_TRY(tlist.alloc(n));
     PADO_TRY(imark.alloc(n));
     PADO_TRY(ilist.alloc(n));
-    PADO_TRY(alist.alloc(n));
-    PADO_TRY(amask.alloc(n));
-    PADO_TRY(aend.alloc(n));
+    PADO_TRY(alist.alloc(alist_cap));
+    PADO_TRY(amask.alloc(alist_cap));
+    PADO_TRY(aend.alloc(alist_cap));
     PADO_TRY(hslot.alloc(n));
     PADO_TRY(hmin.alloc(n));
     PADO_TRY(hlist.alloc(n));
