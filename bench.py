@@ -115,6 +115,45 @@ def serving_leg(ctx, scale=14, ef=8, duration_s=4.0, seed=5):
     return res
 
 
+def multi_leg(count, seed=4):
+    """Many graphs in one job: `count` graphs with vertex counts log-uniform in
+    [2^10, 2^16], half R-MAT (ef 8), half 2D grids with 5% of edges deleted
+    (large diameter), labeled by one MultiIndex build.  Graphs whose distances
+    leave the reference's u8 domain are reported, not labeled (their labels
+    cannot be bit-exact with a u8 build); the labeled ones are checked
+    bit-exact against the oracle in tests/test_gpu_multi.py."""
+    ctx = pado.Context(0)
+    rng = np.random.default_rng(seed)
+    graphs = []
+    t0 = time.time()
+    for i in range(count):
+        nv = 2.0 ** rng.uniform(10, 16)
+        if i % 2 == 0:
+            e = pado.rmat_edges(max(1, int(round(np.log2(nv)))), 8, 0.57, 0.19, 0.19, seed + i)
+        else:
+            e = pado.grid_edges(max(2, int(round(np.sqrt(nv)))), 0.05, seed + i)
+        G = pado.Graph.from_edges(e)
+        G.id_transfer(G.make_rank())
+        graphs.append(G)
+    log(f"multi: {count} graphs, {sum(G.num_v for G in graphs)} vertices, host ingest {time.time() - t0:.1f}s")
+    times = []
+    for _ in range(2):
+        t0 = time.perf_counter()
+        mi = pado.MultiIndex(ctx, graphs)
+        ctx.sync()
+        times.append(time.perf_counter() - t0)
+        info = mi.info()
+        st = [mi.graph_info(g) for g in range(count)]
+        mi.close()
+    ok = [x for x in st if x["status"] == 0]
+    return {"graphs": count, "vertices": int(sum(G.num_v for G in graphs)),
+            "edges": int(sum(int(G.off[-1]) for G in graphs) // 2), "build_s": round(times[-1], 4),
+            "graphs_labeled": len(ok), "graphs_outside_u8_domain": sum(1 for x in st if x["status"] == 4),
+            "labels": int(sum(x["num_labels"] for x in ok)), "union_graphs": info["unions"],
+            "workload": "vertex counts log-uniform in [2^10, 2^16]; half R-MAT ef8, half 2D grids with 5% of "
+                        "edges deleted; one MultiIndex build (second of two timed)"}
+
+
 def weighted_build_only(scale, ef=16):
     """Weighted R-MAT build alone (warmup + one timed build), for profiler runs."""
     ctx = pado.Context(0)
@@ -295,6 +334,8 @@ def main():
                     help="run the batch step as per-kernel launches with HIP-event timing instead of the default hipGraphs")
     ap.add_argument("--weighted-scale", type=int, default=0,
                     help="also build and serve a weighted R-MAT of this scale (0 = skip)")
+    ap.add_argument("--multi-graphs", type=int, default=0, metavar="N",
+                    help="only label N independent graphs in one job (multi-graph workload) and exit")
     ap.add_argument("--weighted-build-only", type=int, default=0, metavar="SCALE",
                     help="only build the weighted R-MAT of this scale (twice: warmup + timed) and exit (profiler runs)")
     args = ap.parse_args()
@@ -303,6 +344,9 @@ def main():
         return
     if args.weighted_build_only:
         weighted_build_only(args.weighted_build_only)
+        return
+    if args.multi_graphs:
+        print(json.dumps({"metric": "multi-graph labeling job", "multi": multi_leg(args.multi_graphs)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the index build and "
