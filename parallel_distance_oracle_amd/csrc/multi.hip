@@ -46,29 +46,84 @@ struct Union {
 };
 
 // Upper bound on the graph's largest distance: per component, 2 x the BFS
-// eccentricity of its first vertex.  Returns UINT32_MAX if any bound >= 255.
+// eccentricity of some vertex bounds every distance.  The vertex is the
+// middle of a double-sweep path (first vertex -> farthest a -> farthest b,
+// then the vertex halfway from b back to a), whose eccentricity is close to
+// the radius on path- and grid-like components, so the bound is near the
+// diameter instead of up to twice it.  Returns UINT32_MAX if any bound >= 255.
 uint32_t distance_bound(uint32_t n, const uint64_t *off, const uint32_t *adj)
 {
-    std::vector<uint32_t> dist(n, UINT32_MAX), q(n);
-    uint32_t bound = 0;
-    for (uint32_t s = 0; s < n; ++s) {
-        if (dist[s] != UINT32_MAX) continue;
-        uint32_t h = 0, t = 0, ecc = 0;
+    std::vector<uint32_t> comp(n, UINT32_MAX), dist(n, UINT32_MAX), par(n), q(n);
+    // BFS from s over its component; returns (eccentricity, farthest vertex).
+    auto bfs = [&](uint32_t s) {
+        uint32_t h = 0, t = 0, far = s;
         dist[s] = 0;
+        par[s] = s;
         q[t++] = s;
         while (h < t) {
-            uint32_t v = q[h++];
-            ecc = std::max(ecc, dist[v]);
+            const uint32_t v = q[h++];
+            if (dist[v] > dist[far]) far = v;
+            for (uint64_t e = off[v]; e < off[v + 1]; ++e) {
+                const uint32_t u = adj[e];
+                if (dist[u] == UINT32_MAX) {
+                    dist[u] = dist[v] + 1;
+                    par[u] = v;
+                    q[t++] = u;
+                }
+            }
+        }
+        const uint32_t ecc = dist[far];
+        for (uint32_t i = 0; i < t; ++i) dist[q[i]] = UINT32_MAX;  // ready for the next sweep
+        return std::make_pair(ecc, far);
+    };
+    uint32_t bound = 0;
+    for (uint32_t s = 0; s < n; ++s) {
+        if (comp[s] != UINT32_MAX) continue;
+        // Mark the component (BFS from s), then the double sweep.
+        uint32_t h = 0, t = 0;
+        comp[s] = s;
+        q[t++] = s;
+        while (h < t) {
+            const uint32_t v = q[h++];
             for (uint64_t e = off[v]; e < off[v + 1]; ++e)
-                if (dist[adj[e]] == UINT32_MAX) {
-                    dist[adj[e]] = dist[v] + 1;
+                if (comp[adj[e]] == UINT32_MAX) {
+                    comp[adj[e]] = s;
                     q[t++] = adj[e];
                 }
         }
-        bound = std::max(bound, 2 * ecc);
+        const auto e0 = bfs(s);
+        const auto ea = bfs(e0.second);
+        // Walk from b (farthest from a) back toward a by half the a-b distance.
+        uint32_t mid = ea.second;
+        {
+            // par[] still holds the BFS tree rooted at a.
+            for (uint32_t k = 0; k < ea.first / 2; ++k) mid = par[mid];
+        }
+        const auto em = bfs(mid);
+        const uint32_t b = 2 * std::min(std::min(e0.first, ea.first), em.first);
+        bound = std::max(bound, b);
         if (bound >= 255) return UINT32_MAX;
     }
     return bound;
+}
+
+// BFS eccentricity of vertex s (within its component).
+uint32_t eccentricity(uint32_t n, const uint64_t *off, const uint32_t *adj, uint32_t s)
+{
+    std::vector<uint32_t> dist(n, UINT32_MAX), q(n);
+    uint32_t h = 0, t = 0, ecc = 0;
+    dist[s] = 0;
+    q[t++] = s;
+    while (h < t) {
+        const uint32_t v = q[h++];
+        ecc = std::max(ecc, dist[v]);
+        for (uint64_t e = off[v]; e < off[v + 1]; ++e)
+            if (dist[adj[e]] == UINT32_MAX) {
+                dist[adj[e]] = dist[v] + 1;
+                q[t++] = adj[e];
+            }
+    }
+    return ecc;
 }
 
 double ms_since(std::chrono::steady_clock::time_point t0)
@@ -248,9 +303,16 @@ pado_gpu_status pado_gpu_build_multi(pado_gpu_ctx *ctx, uint32_t num_graphs, con
         return s;
     }
     // Graphs outside the proven range: on their own (exact domain detection).
+    // Vertex 0 is the first bit-parallel root, and its BFS runs to its
+    // eccentricity, so an eccentricity >= 255 is the domain error that
+    // build would stop with, known without building.
     for (uint32_t g = 0; g < num_graphs; ++g) {
         GraphSlot &G = mi->graphs[g];
         if (G.union_id >= 0) continue;
+        if (eccentricity(G.n, off + G.voff, adj + G.eoff, 0) >= kInf8) {
+            G.status = PADO_GPU_ERR_DOMAIN;
+            continue;
+        }
         pado_gpu_graph *gg = nullptr;
         s = pado_gpu_graph_upload(ctx, G.n, off[G.voff + G.n], off + G.voff, adj + G.eoff, nullptr, &gg);
         if (s != PADO_GPU_OK) {
