@@ -28,14 +28,20 @@
 //   pool[] u32       label store: per-vertex growable arrays of packed
 //                    (hub << 8 | dist) labels, grown fourfold (at least 32)
 //                    on overflow by a bump allocator (one wave owns one
-//                    vertex's array)
+//                    vertex's array); rec[n] = (base, len, cap), 16 B each
+//   vmark[n] u64     ((batch << 16 | iteration) << 1 | candidate) of the last
+//                    push into the vertex, raised by atomicMax
+//   lmask[n][8] u64  bit-parallel level masks (bit i of word k: slot i at
+//                    distance k); rmask / rset: the batch roots' masks and
+//                    set pairs, level- / slot-major
 //   hinfo[n] u32     per-hub 0xFFFFFF00 | min distance from any batch root
 //   dc[h][1024] u8   root-distance table: dc[h][r - rs] = dist(root r, hub h),
 //                    the transposed form of dist_matrix, rows by hub id
 // Kernels per iteration: k_push (edge-parallel, one edge per lane, 16-edge
 // chunks) and k_add (wave per candidate vertex: distance query + label
 // append + next active entry).  The host enqueues blocks of iterations and
-// checks the frontier count once per block.
+// checks the frontier count once per block, running one batch ahead behind a
+// device-side gate.
 #include <functional>
 #include <map>
 
