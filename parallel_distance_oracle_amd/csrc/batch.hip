@@ -443,7 +443,7 @@ __global__ __launch_bounds__(256) void k_bp_masks(uint32_t n, const uint8_t *bpd
 // count, updated once per block.
 __global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const uint64_t *off, const uint32_t *adj,
                                                      LabelStore S, unsigned long long *ind, unsigned long long *vmark,
-                                                     uint32_t *tlist, uint32_t *alist, unsigned long long *ag,
+                                                     uint32_t *tlist, unsigned long long *aoff, unsigned long long *ag,
                                                      uint32_t *amask, uint32_t *aend, unsigned long long *chunks,
                                                      const ulonglong2 *bps, ulonglong2 *rset,
                                                      const unsigned long long *lmask, unsigned long long *rmask,
@@ -475,7 +475,7 @@ __global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const 
                 S.pool[base + len] = r << 8;  // (hub r, dist 0), PARA:932-949
                 S.rec[r].len = len + 1;
                 ind[(uint64_t)r * kWords + (ro >> 6)] |= 1ull << (ro & 63);
-                alist[ro] = r;
+                aoff[ro] = off[r];
                 amask[ro] = 1u << (ro >> 6);
             }
             if (lane < kWords) ag[(uint64_t)ro * kWords + lane] = (lane == (ro >> 6)) ? (1ull << (ro & 63)) : 0ull;
@@ -560,8 +560,8 @@ constexpr uint32_t kPushWaves = 4;  // waves per k_push block
 // 64 lanes are then checked against the bit-parallel labels 64 at a time,
 // one pair per lane, so a tail with many fresh roots does not hold up the
 // wave.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_push(uint32_t iter, const uint64_t *off, const uint32_t *adj,
-                                              const uint32_t *alist, const unsigned long long *ag,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_push(uint32_t iter, const uint32_t *adj,
+                                              const unsigned long long *aoff, const unsigned long long *ag,
                                               const uint32_t *amask, const uint32_t *aend,
                                               const unsigned long long *chunks, unsigned long long *ind,
                                               unsigned long long *cand, unsigned long long *vmark, uint32_t *tlist,
@@ -622,8 +622,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
             const uint32_t e = (uint32_t)(desc >> 32) + (lane & (kChunk - 1));
             if (e < aend[k]) {
                 ++n_edge;
-                const uint32_t v = alist[k];
-                t = adj[off[v] + e];
+                t = adj[aoff[k] + e];
                 lim = t - rs;  // roots with offset < lim satisfy root < tail (PARA:1134)
                 // words w with w * 64 < lim
                 words = amask[k] & (lim >= kBatch ? 0xFFFFu : ((1u << ((lim + 63) / 64)) - 1u));
@@ -816,7 +815,7 @@ __global__ __launch_bounds__(1024) void k_batch_begin(const uint8_t *used, Build
 // survivors is read (16 B per lane).
 __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used, const uint64_t *off, const uint32_t *adj, const uint32_t *clist,
                                              unsigned long long *cand, LabelStore S, uint32_t *hinfo, uint8_t *dc,
-                                             uint32_t *alist, unsigned long long *ag, uint32_t *amask,
+                                             unsigned long long *aoff, unsigned long long *ag, uint32_t *amask,
                                              uint32_t *aend, unsigned long long *chunks, BuildCounters *C)
 {
     const uint32_t lane = lane_id();
@@ -831,7 +830,7 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
     const uint32_t rs = C->rs, rsz = C->rsz;
     if (blockIdx.x == 0 && threadIdx.x == 0) C->n_cand[p ^ 1u] = 0;
     unsigned long long n_dq = 0, n_lab = 0, n_rows = 0, n_read = 0;
-    // Next-frontier slots (alist entries and push descriptors) are reserved
+    // Next-frontier slots (active entries and push descriptors) are reserved
     // kAlistGrab / kChunkGrab at a time per wave, so the shared counters see
     // one atomic per reservation, not per vertex; unused descriptor slots are
     // padded.  The exact active count is added once per wave at the end.
@@ -963,7 +962,7 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
         for (int w = 0; w < (int)kWords; ++w)
             if ((nzm >> (4 * w)) & 0xFull) wmask |= 1u << w;
         if (lane == 0) {
-            alist[k] = v;
+            aoff[k] = e0;
             amask[k] = wmask;
             aend[k] = pdeg;
         }
@@ -1098,7 +1097,8 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     }
 
     DevBuf<unsigned long long> ind, cand, ag, chunks;
-    DevBuf<uint32_t> tlist, clist, alist, amask, aend, hinfo, pool;
+    DevBuf<uint32_t> tlist, clist, amask, aend, hinfo, pool;
+    DevBuf<unsigned long long> aoff;  // per active slot: the vertex's first edge
     DevBuf<LRec> lrec;
     DevBuf<unsigned long long> vmark;  // ((batch << 16 | iteration) << 1 | candidate) of the last mark
     DevBuf<uint8_t> dc;
@@ -1118,7 +1118,7 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     PADO_TRY(chunks.alloc(2 * (g->m / kChunk + (uint64_t)n) + kChunkGrab * add_waves + 64));
     PADO_TRY(tlist.alloc(n));
     PADO_TRY(clist.alloc(n));
-    PADO_TRY(alist.alloc(alist_cap));
+    PADO_TRY(aoff.alloc(alist_cap));
     PADO_TRY(amask.alloc(alist_cap));
     PADO_TRY(aend.alloc(alist_cap));
     PADO_TRY(hinfo.alloc(n));
@@ -1188,7 +1188,7 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
         hipLaunchKernelGGL(k_batch_end, dim3(1024), dim3(256), 0, st, ind.p, tlist.p, ctr.p);
         hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(batch), 0, st, d_used.p, ctr.p, batch, n);
         hipLaunchKernelGGL(k_batch_init, dim3((batch + 15) / 16), dim3(1024), 0, st, d_used.p, g->off.p, g->adj.p, S, ind.p,
-                           vmark.p, tlist.p, alist.p, ag.p, amask.p, aend.p, chunks.p,
+                           vmark.p, tlist.p, aoff.p, ag.p, amask.p, aend.p, chunks.p,
                            (const ulonglong2 *)idx->bps.p, rset.p, lmask.p, rmask.p, ctr.p);
         hipLaunchKernelGGL(k_hub_table, dim3(root_blocks), dim3(256), 0, st, 1u, d_used.p, S, hinfo.p, dc.p, ctr.p);
     };
@@ -1197,13 +1197,13 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
         for (uint32_t u = 0; u < count; ++u) {
             const uint32_t iter = first + 1 + u;
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u], st));
-            hipLaunchKernelGGL(k_push, dim3(push_grid), dim3(256), 0, st, iter, g->off.p, g->adj.p, alist.p, ag.p,
+            hipLaunchKernelGGL(k_push, dim3(push_grid), dim3(256), 0, st, iter, g->adj.p, aoff.p, ag.p,
                                amask.p, aend.p, chunks.p, ind.p, cand.p, vmark.p, tlist.p, clist.p,
                                idx->bpd.p, lmask.p, rmask.p, (const ulonglong2 *)idx->bps.p, rset.p, ctr.p);
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 1], st));
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 2], st));
             hipLaunchKernelGGL(k_add, dim3(add_grid), dim3(256), 0, st, iter, d_used.p, g->off.p, g->adj.p, clist.p,
-                               cand.p, S, hinfo.p, dc.p, alist.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
+                               cand.p, S, hinfo.p, dc.p, aoff.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 3], st));
         }
         return PADO_GPU_OK;
