@@ -225,7 +225,7 @@ def weighted_leg(ctx, scale, ef=16, duration_s=4.0, seed=7, dijkstra_sources=4):
     return res
 
 
-def serve_measure(ctx, idx, n, rank2id, duration_s, seed, max_pairs_per_rate=120_000_000):
+def serve_measure(ctx, idx, n, rank2id, duration_s, seed, max_pairs_per_rate=600_000_000):
     """Open-loop Poisson request stream against a resident ordered index,
     driven by the server's native load generator (pado_gpu_serve_replay)."""
     rng = np.random.default_rng(seed)
@@ -253,7 +253,8 @@ def serve_measure(ctx, idx, n, rank2id, duration_s, seed, max_pairs_per_rate=120
     plain_qps = allA.size / plain_s
 
     srv = pado.Server(ctx, idx, staging_pairs=16 << 20, max_launch_pairs=4 << 20, streams=3)
-    srv.replay(allA[:offs[8]], allB[:offs[8]], offs[:9], np.zeros(8))  # warm
+    # Warm every stream and launch size with a burst of 256 requests.
+    srv.replay(allA[:offs[256]], allB[:offs[256]], offs[:257], np.zeros(256))
     rates = []
     ok_all = True
     for frac in (0.2, 0.4, 0.6, 0.8):

@@ -201,7 +201,9 @@ void completer_loop(pado_gpu_server *s)
         const int si = s->inflight.front().stream;
         hipEvent_t ev0 = s->ev_start[si], ev1 = s->ev_end[si];
         lk.unlock();
-        (void)hipEventSynchronize(ev1);
+        // Poll rather than block: a blocking event wait can resume
+        // milliseconds late, which would land in the latency tail.
+        while (hipEventQuery(ev1) == hipErrorNotReady) std::this_thread::yield();
         float ms = 0.f;
         (void)hipEventElapsedTime(&ms, ev0, ev1);
         const double t = now_ms();
