@@ -13,10 +13,13 @@
 // reproduces the reference's answers bit for bit.  Either way the slot order
 // does not matter: the result is a minimum over slots.
 //
-// Mapping: one wave per pair.  Lanes 0..49 evaluate one bit-parallel slot each
-// (the 850-byte records of both vertices are read coalesced), then the wave
-// intersects the two hub-sorted label lists: lanes take 64 labels of the
-// shorter list at a time and binary-search them in the longer one.
+// Mapping: one wave per pair.  Lanes 0..49 take one bit-parallel slot each:
+// first the distance bytes only, then, after a wave minimum m of the slot
+// sums, the set pairs of just the slots that could still go below m (sum
+// <= m + 1).  The wave intersects the two hub-sorted label lists: the longer
+// list is copied into the wave's LDS (up to kQLds labels) and lanes take 64
+// labels of the shorter list at a time and binary-search them there, so the
+// search's dependent steps are LDS reads instead of global ones.
 #include "internal.hpp"
 
 namespace pado {
@@ -31,52 +34,89 @@ __device__ inline uint32_t wave_min_u32(uint32_t x)
     return x;
 }
 
+// Orders LDS writes of some lanes of a wave before reads by other lanes.
+__device__ inline void q_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+constexpr uint32_t kQLds = 1024;   // labels of the longer list staged in LDS per wave
+constexpr uint32_t kQWaves = 4;    // waves per query block
+
 }  // namespace
 
+// Index of the first entry of the sorted list L[0..len) whose hub is >= key
+// (labels compared by hub, the upper 24 bits).
+__device__ inline uint32_t lower_hub(const uint32_t *L, uint32_t len, uint32_t key)
+{
+    uint32_t lo = 0, hi = len;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((L[mid] >> 8) < key)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// sbuf: the wave's kQLds-entry LDS buffer.
 __device__ inline uint32_t pair_distance(uint32_t n, const uint8_t *bpd, const ulonglong2 *bps, const uint64_t *loff,
-                                         const uint32_t *lab, uint32_t a, uint32_t b, bool ref_compat)
+                                         const uint32_t *lab, uint32_t a, uint32_t b, bool ref_compat,
+                                         uint32_t *sbuf)
 {
     const uint32_t lane = threadIdx.x & 63;
     if (a >= n || b >= n) return a == b ? 0u : 255u;
-    uint32_t d = 255u;
-    if (lane < kBP) {
-        uint32_t da = bpd[(uint64_t)a * kBpdStride + lane];
-        uint32_t db = bpd[(uint64_t)b * kBpdStride + lane];
-        int td = (int)(da + db);
-        if (!(ref_compat && td < 2) && td - 2 <= 255) {
-            ulonglong2 sa = bps[(uint64_t)a * kBP + lane];
-            ulonglong2 sb = bps[(uint64_t)b * kBP + lane];
-            if (sa.x & sb.x)
-                td -= 2;
-            else if ((sa.x & sb.y) | (sa.y & sb.x))
-                td -= 1;
-            if ((uint32_t)td < d) d = (uint32_t)td;
-        }
-    }
     const uint64_t oa = loff[a], ob = loff[b];
     const uint32_t la = (uint32_t)(loff[a + 1] - oa), lb = (uint32_t)(loff[b + 1] - ob);
+    // Bit-parallel slots, distance bytes first.  A slot counts if the
+    // reference's gate passes; its result is td - {2, 1, 0}, so only slots
+    // with td <= m + 1 (m = the smallest counted td) can beat m.
+    int td = 0x7FFF;
+    bool gate = false;
+    if (lane < kBP) {
+        const uint32_t da = bpd[(uint64_t)a * kBpdStride + lane];
+        const uint32_t db = bpd[(uint64_t)b * kBpdStride + lane];
+        td = (int)(da + db);
+        gate = !(ref_compat && td < 2) && td - 2 <= 255;
+    }
+    const uint32_t m = wave_min_u32(gate ? (uint32_t)td : 0xFFFFu);
+    uint32_t d = 255u;
+    if (gate && (uint32_t)td <= m + 1u) {
+        const ulonglong2 sa = bps[(uint64_t)a * kBP + lane];
+        const ulonglong2 sb = bps[(uint64_t)b * kBP + lane];
+        int t = td;
+        if (sa.x & sb.x)
+            t -= 2;
+        else if ((sa.x & sb.y) | (sa.y & sb.x))
+            t -= 1;
+        if ((uint32_t)t < d) d = (uint32_t)t;
+    }
     const uint32_t *S = lab + (la <= lb ? oa : ob);
     const uint32_t *T = lab + (la <= lb ? ob : oa);
     const uint32_t ls = la <= lb ? la : lb, lt = la <= lb ? lb : la;
+    const bool staged = lt <= kQLds;
+    if (staged) {
+        for (uint32_t j = lane; j < lt; j += 64) sbuf[j] = T[j];
+        q_lds_sync();
+    }
     for (uint32_t j0 = 0; j0 < ls; j0 += 64) {
         const uint32_t j = j0 + lane;
         if (j < ls) {
             const uint32_t s = S[j];
             const uint32_t key = s >> 8;
-            uint32_t lo = 0, hi = lt;
-            while (lo < hi) {
-                uint32_t mid = (lo + hi) >> 1;
-                if ((T[mid] >> 8) < key)
-                    lo = mid + 1;
-                else
-                    hi = mid;
-            }
-            if (lo < lt && (T[lo] >> 8) == key) {
-                uint32_t td = (s & 255u) + (T[lo] & 255u);
-                if (td < d) d = td;
+            const uint32_t *L = staged ? sbuf : T;
+            const uint32_t lo = lower_hub(L, lt, key);
+            if (lo < lt && (L[lo] >> 8) == key) {
+                const uint32_t t = (s & 255u) + (L[lo] & 255u);
+                if (t < d) d = t;
             }
         }
     }
+    // The next pair's copy must not overtake this pair's reads.
+    if (staged) q_lds_sync();
     d = wave_min_u32(d);
     return d >= 253u ? 255u : d;
 }
@@ -85,9 +125,11 @@ __global__ __launch_bounds__(256) void k_query(uint32_t n, const uint8_t *bpd, c
                                                const uint64_t *loff, const uint32_t *lab, const uint32_t *qa,
                                                const uint32_t *qb, uint64_t k, uint8_t *out, int ref_compat)
 {
+    __shared__ uint32_t s_q[kQWaves][kQLds];
+    uint32_t *sbuf = s_q[threadIdx.x >> 6];
     const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
     for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < k; i += nw) {
-        uint32_t d = pair_distance(n, bpd, bps, loff, lab, qa[i], qb[i], ref_compat != 0);
+        uint32_t d = pair_distance(n, bpd, bps, loff, lab, qa[i], qb[i], ref_compat != 0, sbuf);
         if ((threadIdx.x & 63) == 0) out[i] = (uint8_t)d;
     }
 }
@@ -153,6 +195,8 @@ __global__ __launch_bounds__(256) void k_query_ws(uint32_t n, const uint8_t *bpd
                                                   const uint32_t *qb, uint64_t k, uint8_t *out, int ref_compat,
                                                   unsigned long long *work)
 {
+    __shared__ uint32_t s_q[kQWaves][kQLds];
+    uint32_t *sbuf = s_q[threadIdx.x >> 6];
     const uint32_t lane = threadIdx.x & 63;
     for (;;) {
         unsigned long long c0 = 0;
@@ -161,7 +205,7 @@ __global__ __launch_bounds__(256) void k_query_ws(uint32_t n, const uint8_t *bpd
         if (c0 >= k) break;
         const uint64_t c1 = c0 + kGrab < k ? c0 + kGrab : k;
         for (uint64_t i = c0; i < c1; ++i) {
-            const uint32_t d = pair_distance(n, bpd, bps, loff, lab, qa[i], qb[i], ref_compat != 0);
+            const uint32_t d = pair_distance(n, bpd, bps, loff, lab, qa[i], qb[i], ref_compat != 0, sbuf);
             if (lane == 0) out[i] = (uint8_t)d;
         }
     }
