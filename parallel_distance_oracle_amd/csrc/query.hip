@@ -140,7 +140,7 @@ __global__ __launch_bounds__(256) void k_query(uint32_t n, const uint8_t *bpd, c
 // range -> a == b ? 0 : 32767).  One wave per pair, the shorter list's hubs
 // binary-searched in the longer one as for unweighted labels.
 __device__ inline uint32_t wpair_distance(uint32_t n, const uint64_t *loff, const uint32_t *hub, const uint32_t *dist,
-                                          uint32_t a, uint32_t b, bool ref_compat)
+                                          uint32_t a, uint32_t b, bool ref_compat, uint32_t *sbuf)
 {
     const uint32_t lane = threadIdx.x & 63;
     if (a >= n || b >= n) return a == b ? 0u : (ref_compat ? 32767u : 0xFFFFFFFFu);
@@ -148,6 +148,14 @@ __device__ inline uint32_t wpair_distance(uint32_t n, const uint64_t *loff, cons
     const uint32_t la = (uint32_t)(loff[a + 1] - oa), lb = (uint32_t)(loff[b + 1] - ob);
     const uint64_t os = la <= lb ? oa : ob, ot = la <= lb ? ob : oa;
     const uint32_t ls = la <= lb ? la : lb, lt = la <= lb ? lb : la;
+    // The longer list's hubs go to the wave's LDS (distances are read only
+    // for matches).
+    const bool staged = lt <= kQLds;
+    if (staged) {
+        for (uint32_t j = lane; j < lt; j += 64) sbuf[j] = hub[ot + j];
+        q_lds_sync();
+    }
+    const uint32_t *H = staged ? sbuf : hub + ot;
     uint32_t d = 0xFFFFFFFFu;
     for (uint32_t j0 = 0; j0 < ls; j0 += 64) {
         const uint32_t j = j0 + lane;
@@ -156,17 +164,19 @@ __device__ inline uint32_t wpair_distance(uint32_t n, const uint64_t *loff, cons
             uint32_t lo = 0, hi = lt;
             while (lo < hi) {
                 const uint32_t mid = (lo + hi) >> 1;
-                if (hub[ot + mid] < key)
+                if (H[mid] < key)
                     lo = mid + 1;
                 else
                     hi = mid;
             }
-            if (lo < lt && hub[ot + lo] == key) {
+            if (lo < lt && H[lo] == key) {
                 const uint64_t td = (uint64_t)dist[os + j] + dist[ot + lo];
                 if (td < d) d = (uint32_t)td;
             }
         }
     }
+    // The next pair's copy must not overtake this pair's reads.
+    if (staged) q_lds_sync();
     d = wave_min_u32(d);
     if (ref_compat && d >= 32765u) d = 32767u;
     return d;
@@ -176,9 +186,11 @@ __global__ __launch_bounds__(256) void k_wquery(uint32_t n, const uint64_t *loff
                                                 const uint32_t *dist, const uint32_t *qa, const uint32_t *qb,
                                                 uint64_t k, uint32_t *out, int ref_compat)
 {
+    __shared__ uint32_t s_q[kQWaves][kQLds];
+    uint32_t *sbuf = s_q[threadIdx.x >> 6];
     const uint64_t nw = ((uint64_t)gridDim.x * blockDim.x) >> 6;
     for (uint64_t i = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6; i < k; i += nw) {
-        const uint32_t d = wpair_distance(n, loff, hub, dist, qa[i], qb[i], ref_compat != 0);
+        const uint32_t d = wpair_distance(n, loff, hub, dist, qa[i], qb[i], ref_compat != 0, sbuf);
         if ((threadIdx.x & 63) == 0) out[i] = d;
     }
 }
@@ -216,6 +228,8 @@ __global__ __launch_bounds__(256) void k_wquery_ws(uint32_t n, const uint64_t *l
                                                    uint64_t k, uint32_t *out, int ref_compat,
                                                    unsigned long long *work)
 {
+    __shared__ uint32_t s_q[kQWaves][kQLds];
+    uint32_t *sbuf = s_q[threadIdx.x >> 6];
     const uint32_t lane = threadIdx.x & 63;
     for (;;) {
         unsigned long long c0 = 0;
@@ -224,7 +238,7 @@ __global__ __launch_bounds__(256) void k_wquery_ws(uint32_t n, const uint64_t *l
         if (c0 >= k) break;
         const uint64_t c1 = c0 + kGrab < k ? c0 + kGrab : k;
         for (uint64_t i = c0; i < c1; ++i) {
-            const uint32_t d = wpair_distance(n, loff, hub, dist, qa[i], qb[i], ref_compat != 0);
+            const uint32_t d = wpair_distance(n, loff, hub, dist, qa[i], qb[i], ref_compat != 0, sbuf);
             if (lane == 0) out[i] = d;
         }
     }
