@@ -245,11 +245,15 @@ def serve_measure(ctx, idx, n, rank2id, duration_s, seed, max_pairs_per_rate=600
     allB = np.where(zipf_half, zb, ub).astype(np.uint32)
     mean_pairs = tot / pool_n
 
-    # Plain single-batch path on the same pairs (host arrays in and out, one launch).
+    # Plain single-batch path on the same pairs (host arrays in and out, one
+    # launch); its pageable host copies vary from call to call, so the offered
+    # rates are scaled to the best of three calls.
     idx.query(allA[:1000], allB[:1000])
-    t0 = time.perf_counter()
-    plain = idx.query(allA, allB)
-    plain_s = time.perf_counter() - t0
+    plain_s = float("inf")
+    for _ in range(3):
+        t0 = time.perf_counter()
+        plain = idx.query(allA, allB)
+        plain_s = min(plain_s, time.perf_counter() - t0)
     plain_qps = allA.size / plain_s
 
     srv = pado.Server(ctx, idx, staging_pairs=16 << 20, max_launch_pairs=4 << 20, streams=3)

@@ -359,7 +359,9 @@ pado_gpu_status pado_gpu_serve_submit(pado_gpu_server *srv, const uint32_t *a, c
 pado_gpu_status pado_gpu_serve_poll(pado_gpu_server *srv, uint64_t req_id, int *done);
 /* Blocks until done, copies the k answers to out (k u8 values for an
  * unweighted index, k u32 for a weighted one), releases the slot.
- * latency_ms (nullable) = completion time - submit time. */
+ * latency_ms (nullable) = completion time - submit time.  A request is
+ * waited for once: another wait on the same id, concurrent or later,
+ * returns PADO_GPU_ERR_ARG. */
 pado_gpu_status pado_gpu_serve_wait(pado_gpu_server *srv, uint64_t req_id, void *out, double *latency_ms);
 pado_gpu_status pado_gpu_serve_get_stats(pado_gpu_server *srv, pado_gpu_serve_stats *st);
 /* Open-loop load generator: request i (pairs [req_off[i], req_off[i+1]) of a

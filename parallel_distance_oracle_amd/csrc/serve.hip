@@ -5,8 +5,9 @@
 // pairs are admitted into a pinned staging ring (admission = a free request
 // slot + ring space, i.e. device staging memory), a dispatcher thread packs the
 // requests queued at that moment into one launch per stream (up to `streams`
-// launches in flight) of the persistent label-merge kernel (resident
-// workgroups draining the launch's pairs through a work counter), and a
+// launches in flight) of a staging kernel (the launch's pairs read from the
+// mapped ring into device memory) and the persistent label-merge kernel
+// (resident workgroups draining the pairs through a work counter), and a
 // completer thread marks requests done in launch order.  Weighted indexes are
 // served the same way with u32 answers.  A request that arrives while launches run
 // joins the next launch; its pairs overlap on the device with the running
@@ -34,6 +35,40 @@ using namespace pado;
 
 namespace {
 
+// Copies a launch's k pairs from the mapped host ring into device memory.  The
+// runtime's host-to-device copy occasionally spent several milliseconds of
+// host time inside the call (about once in 10^4 launches), holding back the
+// launch behind it; a kernel goes through the stream's queue like every other
+// launch.  Each thread reads four ids of each array before it writes, so
+// that a wave keeps eight reads over the host link in flight.
+constexpr uint32_t kStageUnroll = 4;
+
+__global__ __launch_bounds__(256) void k_stage_pairs(const uint32_t *__restrict__ ha,
+                                                     const uint32_t *__restrict__ hb, uint32_t *__restrict__ da,
+                                                     uint32_t *__restrict__ db, uint64_t k)
+{
+    const uint64_t step = (uint64_t)gridDim.x * blockDim.x * kStageUnroll;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x * kStageUnroll + threadIdx.x; base < k; base += step) {
+        uint32_t va[kStageUnroll], vb[kStageUnroll];
+#pragma unroll
+        for (uint32_t j = 0; j < kStageUnroll; ++j) {
+            const uint64_t i = base + (uint64_t)j * blockDim.x;
+            if (i < k) {
+                va[j] = ha[i];
+                vb[j] = hb[i];
+            }
+        }
+#pragma unroll
+        for (uint32_t j = 0; j < kStageUnroll; ++j) {
+            const uint64_t i = base + (uint64_t)j * blockDim.x;
+            if (i < k) {
+                da[i] = va[j];
+                db[i] = vb[j];
+            }
+        }
+    }
+}
+
 double now_ms()
 {
     return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
@@ -44,6 +79,7 @@ struct Request {
     uint64_t start = 0;  // monotonic ring position of the first pair
     uint64_t k = 0;
     int state = 0;       // 0 queued, 1 launched, 2 done, 3 consumed
+    bool claimed = false;  // a waiter is copying the answers out
     double t_submit = 0, t_done = 0;
 };
 
@@ -63,6 +99,7 @@ struct pado_gpu_server {
     uint32_t es = 1;                // answer bytes: 1 (u8 hops) or 4 (u32 weighted)
     uint32_t ws_blocks = 0;         // persistent kernel grid (workgroups)
     uint32_t *h_a = nullptr, *h_b = nullptr;
+    const uint32_t *m_a = nullptr, *m_b = nullptr;  // the same ring as mapped for the device
     uint8_t *h_out = nullptr;
     DevBuf<uint32_t> d_a, d_b;
     DevBuf<uint8_t> d_out;
@@ -137,15 +174,18 @@ void dispatcher_loop(pado_gpu_server *s)
         }
         if (s->stopping && s->pending.empty()) break;
         int si = (int)(std::find(s->stream_busy.begin(), s->stream_busy.end(), 0) - s->stream_busy.begin());
-        // Pack consecutive pending requests that are physically contiguous.
+        // Pack consecutive pending requests that are physically contiguous.  A
+        // request that ends exactly at the ring's end is followed, at the same
+        // monotonic position, by one at physical offset 0 of the next lap:
+        // the launch stops at the physical end, not only at a placement gap.
         Launch L;
         L.stream = si;
         uint64_t p0 = s->pending.front()->start, p1 = p0;
         while (!s->pending.empty()) {
             Request *r = s->pending.front();
-            if (r->start != p1 && !L.reqs.empty()) break;              // wrap gap
+            if (r->start != p1 && !L.reqs.empty()) break;              // placement gap
             if (!L.reqs.empty() && (r->start + r->k - p0) > max_launch) break;
-            if ((r->start % s->cap) + r->k > s->cap) break;            // cannot happen (place())
+            if ((p0 % s->cap) + (r->start + r->k - p0) > s->cap) break;  // crosses the physical end
             L.reqs.push_back(r);
             r->state = 1;
             p1 = r->start + r->k;
@@ -157,10 +197,16 @@ void dispatcher_loop(pado_gpu_server *s)
 
         const uint64_t phys = p0 % s->cap, k = p1 - p0;
         hipStream_t st = s->streams[si];
+        const bool in_ring = phys + k <= s->cap;  // holds by the packing above; never launch past the ring
+        if (!in_ring) {
+            std::lock_guard<std::mutex> g(s->mu);
+            s->error = "serving launch crosses the staging ring's end";
+        }
         (void)hipEventRecord(s->ev_start[si], st);
-        if (k) {
-            (void)hipMemcpyAsync(s->d_a.p + phys, s->h_a + phys, k * 4, hipMemcpyHostToDevice, st);
-            (void)hipMemcpyAsync(s->d_b.p + phys, s->h_b + phys, k * 4, hipMemcpyHostToDevice, st);
+        if (k && in_ring) {
+            const uint32_t sgrid = (uint32_t)std::min<uint64_t>((k + 1023) / 1024, 1024);
+            hipLaunchKernelGGL(k_stage_pairs, dim3(sgrid), dim3(256), 0, st, s->m_a + phys, s->m_b + phys,
+                               s->d_a.p + phys, s->d_b.p + phys, k);
             const bool grid_stride = (s->cfg.flags & PADO_GPU_SERVE_GRID_STRIDE) != 0;
             pado_gpu_status qs = launch_query_any(s->idx, false, s->d_a.p + phys, s->d_b.p + phys, k,
                                                   s->d_out.p + phys * s->es,
@@ -225,21 +271,28 @@ void completer_loop(pado_gpu_server *s)
 }
 
 // Blocks until request `id` is done, copies its answers, releases its slot.
+// The copy runs outside the lock (the slot is not reused before release): a
+// caller buffer that page-faults on first touch holds back only this caller,
+// not admission, dispatch or completion.
 pado_gpu_status wait_request(pado_gpu_server *s, uint64_t id, void *out, double *t_submit, double *t_done)
 {
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->by_id.find(id);
-    if (it == s->by_id.end() || it->second->state == 3) {
+    if (it == s->by_id.end() || it->second->state == 3 || it->second->claimed) {
         set_error("unknown request id");
         return PADO_GPU_ERR_ARG;
     }
     Request *r = it->second;
+    r->claimed = true;
     s->cv_done.wait(lk, [&] { return r->state >= 2; });
     if (!s->error.empty()) {
+        r->claimed = false;
         set_error(s->error);
         return PADO_GPU_ERR_HIP;
     }
+    lk.unlock();
     if (r->k && out) std::memcpy(out, s->h_out + (r->start % s->cap) * s->es, r->k * s->es);
+    lk.lock();
     *t_submit = r->t_submit;
     *t_done = r->t_done;
     r->state = 3;
@@ -298,11 +351,19 @@ pado_gpu_status pado_gpu_serve_start(pado_gpu_ctx *ctx, pado_gpu_index *idx, con
         delete s;
         return code;
     };
-    if (hipHostMalloc((void **)&s->h_a, cap * 4, hipHostMallocDefault) != hipSuccess ||
-        hipHostMalloc((void **)&s->h_b, cap * 4, hipHostMallocDefault) != hipSuccess ||
+    // The id rings are read by k_stage_pairs over the host link: coherent
+    // (not cached on the device), since every lap rewrites the same addresses.
+    const unsigned ring_flags = hipHostMallocMapped | hipHostMallocCoherent;
+    if (hipHostMalloc((void **)&s->h_a, cap * 4, ring_flags) != hipSuccess ||
+        hipHostMalloc((void **)&s->h_b, cap * 4, ring_flags) != hipSuccess ||
         hipHostMalloc((void **)&s->h_out, cap * s->es, hipHostMallocDefault) != hipSuccess) {
         set_error("pinned staging allocation failed");
         return fail(PADO_GPU_ERR_NOMEM);
+    }
+    if (hipHostGetDevicePointer((void **)&s->m_a, s->h_a, 0) != hipSuccess ||
+        hipHostGetDevicePointer((void **)&s->m_b, s->h_b, 0) != hipSuccess) {
+        set_error("staging ring is not mapped for the device");
+        return fail(PADO_GPU_ERR_HIP);
     }
     pado_gpu_status a1 = s->d_a.alloc(cap), a2 = s->d_b.alloc(cap), a3 = s->d_out.alloc(cap * s->es);
     pado_gpu_status a4 = s->work.alloc(s->cfg.streams);
@@ -360,17 +421,21 @@ pado_gpu_status pado_gpu_serve_submit(pado_gpu_server *s, const uint32_t *a, con
     r->start = start;
     r->k = k;
     r->t_submit = now_ms();
+    s->tail = start + k;
+    s->live.push_back(r);
+    s->by_id[r->id] = r;
+    *req_id = r->id;
+    // The slot is reserved; the pairs are copied in without the lock and the
+    // request becomes visible to the dispatcher only after the copy.
+    lk.unlock();
     const uint64_t phys = start % s->cap;
     if (k) {
         std::memcpy(s->h_a + phys, a, k * 4);
         std::memcpy(s->h_b + phys, b, k * 4);
     }
-    s->tail = start + k;
-    s->live.push_back(r);
-    s->by_id[r->id] = r;
+    lk.lock();
     s->pending.push_back(r);
     s->pending_pairs += k;
-    *req_id = r->id;
     lk.unlock();
     s->cv_work.notify_one();
     return PADO_GPU_OK;

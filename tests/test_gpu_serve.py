@@ -75,7 +75,9 @@ def test_alone_equals_mix_equals_batch(gpu_ctx, served):
         assert np.array_equal(x, y)
         assert np.array_equal(x, idx.query(a, b))
     assert st["requests_done"] == 2 * len(reqs)
-    assert st["launches"] < 2 * len(reqs)  # some requests were coalesced
+    # Two closed-loop clients coalesce only when their timing lines up; packing
+    # is asserted with back-to-back arrivals in the ring-end test.
+    assert st["launches"] <= 2 * len(reqs)
 
 
 def test_admission_refuses_when_full(gpu_ctx, served):
@@ -167,3 +169,27 @@ def test_replay_load_generator(gpu_ctx, served):
     assert np.array_equal(ans, idx.query(A, B))
     assert np.all(lat >= 0) and wall >= arrivals[-1] * 0.5
     assert st["requests_done"] == len(reqs)
+
+
+def test_requests_ending_at_the_ring_end(gpu_ctx, served):
+    """Requests whose sizes divide the staging ring end exactly at its physical
+    end, and the next one starts at offset 0 of the next lap at the same
+    monotonic position; back-to-back arrivals held behind running launches
+    pack such neighbours, and every launch must stop at the ring's end."""
+    G, idx, r2i = served
+    rng = np.random.default_rng(41)
+    cap, k, count = 4096, 256, 400
+    A = rng.integers(0, G.num_v, count * k).astype(np.uint32)
+    B = pado.zipf_vertices(G.num_v, count * k, 1.1, 41, r2i).astype(np.uint32)
+    off = (np.arange(count + 1) * k).astype(np.uint64)
+    for streams in (2, 3):
+        srv = pado.Server(gpu_ctx, idx, staging_pairs=cap, max_launch_pairs=cap, streams=streams,
+                          min_launch_pairs=cap, max_wait_us=2000)
+        try:
+            lat, ans, _ = srv.replay(A, B, off, np.zeros(count))
+            st = srv.stats()
+        finally:
+            srv.stop()
+        assert np.array_equal(ans, idx.query(A, B))
+        assert st["requests_done"] == count and st["launches"] < count  # neighbours were packed
+        assert np.all(lat >= 0)
