@@ -261,7 +261,9 @@ def serve_measure(ctx, idx, n, rank2id, duration_s, seed, max_pairs_per_rate=600
     srv.replay(allA[:offs[256]], allB[:offs[256]], offs[:257], np.zeros(256))
     rates = []
     ok_all = True
-    for frac in (0.2, 0.4, 0.6, 0.8):
+    # Offered rates as fractions of the plain batch path; the server overlaps
+    # its copies with the label merge, so rates above 1 can still be served.
+    for frac in (0.2, 0.4, 0.6, 0.8, 1.0, 1.2):
         rate_pairs = frac * plain_qps
         rate_req = rate_pairs / mean_pairs
         dur = min(duration_s, max_pairs_per_rate / rate_pairs)

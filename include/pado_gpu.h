@@ -315,16 +315,22 @@ void pado_gpu_multi_free(pado_gpu_multi *mi);
  * (PADO_GPU_ERR_BUSY otherwise); a dispatcher thread coalesces admitted
  * requests into query launches on several streams, so requests that arrive
  * while others run are admitted as slots and memory free up and their work
- * overlaps on the device.  Each launch runs a persistent label-merge kernel (a
- * grid of resident workgroups whose waves take pairs from a work counter
- * until the launch is drained).  Each answer is computed per pair, so it is
+ * overlaps on the device.  Each launch runs the grid-stride label-merge
+ * kernel (one wave per pair; the hardware's workgroup dispatch balances long
+ * and short label lists), or on request a persistent one (resident
+ * workgroups whose waves take pairs from a work counter).  Each answer is computed per pair, so it is
  * the same alone and in any mix.  Unweighted indexes answer u8 hops,
  * weighted ones u32 distances (as pado_gpu_query / pado_gpu_wquery).
  * ------------------------------------------------------------------------ */
 typedef struct pado_gpu_server pado_gpu_server;
 
-/* Serve with the plain grid-stride query kernel instead of the persistent one. */
+/* Serve with the grid-stride query kernel: the default; the flag is kept for
+ * callers that name it. */
 #define PADO_GPU_SERVE_GRID_STRIDE 0x10u
+/* Serve with the persistent work-counter kernel instead (one counter per
+ * launch, taken 16 pairs at a time; its contended atomics make it slower on
+ * a multi-XCD device, kept for comparison). */
+#define PADO_GPU_SERVE_PERSISTENT 0x20u
 
 typedef struct {
     uint64_t staging_pairs;    /* pair capacity of the admission ring; 0 = derived from free memory */

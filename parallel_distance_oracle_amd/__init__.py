@@ -33,6 +33,7 @@ QUERY_RANK_SPACE = 4
 BUILD_BP_PER_SLOT = 1
 BUILD_STREAM = 2  # per-kernel launches with HIP-event timing (default: hipGraph batch step)
 SERVE_GRID_STRIDE = 0x10
+SERVE_PERSISTENT = 0x20
 WINF = 0xFFFFFFFF  # weighted query: no common hub (unreachable)
 
 _STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_NOMEM", 4: "ERR_DOMAIN", 5: "ERR_STATE",
@@ -646,7 +647,7 @@ class Server:
     """Continuous-batching query service over an ordered, resident Index."""
 
     def __init__(self, ctx, index, staging_pairs=0, max_launch_pairs=0, max_requests=0, streams=0,
-                 ref_compat=False, grid_stride=False, min_launch_pairs=0, max_wait_us=0):
+                 ref_compat=False, persistent=False, min_launch_pairs=0, max_wait_us=0):
         cfg = ServeCfg()
         cfg.min_launch_pairs = min_launch_pairs
         cfg.max_wait_us = max_wait_us
@@ -654,7 +655,7 @@ class Server:
         cfg.max_launch_pairs = max_launch_pairs
         cfg.max_requests = max_requests
         cfg.streams = streams
-        cfg.flags = (QUERY_REF_COMPAT if ref_compat else 0) | (SERVE_GRID_STRIDE if grid_stride else 0)
+        cfg.flags = (QUERY_REF_COMPAT if ref_compat else 0) | (SERVE_PERSISTENT if persistent else 0)
         self.dtype = np.uint32 if index.weighted else np.uint8
         h = _vp()
         _check(lib().pado_gpu_serve_start(ctx.h, index.h, ctypes.byref(cfg), ctypes.byref(h)))

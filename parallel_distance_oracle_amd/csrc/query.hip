@@ -246,6 +246,17 @@ __global__ __launch_bounds__(256) void k_wquery_ws(uint32_t n, const uint64_t *l
 
 __global__ void k_zero_work(unsigned long long *work) { *work = 0ull; }
 
+// Resident workgroups per CU of the persistent kernel: the label merge waits
+// on dependent loads, so the grid fills every slot the kernel's registers and
+// LDS leave (a fixed 4 per CU left serving at 70% of the grid-stride kernel).
+uint32_t query_ws_blocks_per_cu(bool weighted)
+{
+    int nb = 0;
+    const hipError_t e = weighted ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_wquery_ws, 256, 0)
+                                  : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_query_ws, 256, 0);
+    return (e == hipSuccess && nb > 0) ? (uint32_t)nb : 4u;
+}
+
 pado_gpu_status launch_wquery(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,
                               uint64_t k, uint32_t *out, bool ref_compat, hipStream_t st);
 pado_gpu_status launch_query(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,

@@ -6,8 +6,8 @@
 // slot + ring space, i.e. device staging memory), a dispatcher thread packs the
 // requests queued at that moment into one launch per stream (up to `streams`
 // launches in flight) of a staging kernel (the launch's pairs read from the
-// mapped ring into device memory) and the persistent label-merge kernel
-// (resident workgroups draining the pairs through a work counter), and a
+// mapped ring into device memory) and the label-merge kernel (one wave per
+// pair, grid-stride; a persistent work-counter variant on request), and a
 // completer thread marks requests done in launch order.  Weighted indexes are
 // served the same way with u32 answers.  A request that arrives while launches run
 // joins the next launch; its pairs overlap on the device with the running
@@ -29,6 +29,7 @@ namespace pado {
 pado_gpu_status launch_query_any(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,
                                  uint64_t k, void *out, bool ref_compat, hipStream_t st,
                                  unsigned long long *work, uint32_t blocks);
+uint32_t query_ws_blocks_per_cu(bool weighted);
 }
 
 using namespace pado;
@@ -207,7 +208,7 @@ void dispatcher_loop(pado_gpu_server *s)
             const uint32_t sgrid = (uint32_t)std::min<uint64_t>((k + 1023) / 1024, 1024);
             hipLaunchKernelGGL(k_stage_pairs, dim3(sgrid), dim3(256), 0, st, s->m_a + phys, s->m_b + phys,
                                s->d_a.p + phys, s->d_b.p + phys, k);
-            const bool grid_stride = (s->cfg.flags & PADO_GPU_SERVE_GRID_STRIDE) != 0;
+            const bool grid_stride = (s->cfg.flags & PADO_GPU_SERVE_PERSISTENT) == 0;
             pado_gpu_status qs = launch_query_any(s->idx, false, s->d_a.p + phys, s->d_b.p + phys, k,
                                                   s->d_out.p + phys * s->es,
                                                   (s->cfg.flags & PADO_GPU_QUERY_REF_COMPAT) != 0, st,
@@ -328,8 +329,9 @@ pado_gpu_status pado_gpu_serve_start(pado_gpu_ctx *ctx, pado_gpu_index *idx, con
     if (!s->cfg.max_requests) s->cfg.max_requests = 4096;
     if (!s->cfg.streams) s->cfg.streams = 3;
     s->es = idx->weighted ? 4u : 1u;
-    // Persistent kernel: 4 workgroups of 256 threads per compute unit.
-    s->ws_blocks = 4u * (ctx->num_cu > 0 ? (uint32_t)ctx->num_cu : 256u);
+    // Persistent kernel: as many resident workgroups of 256 threads per
+    // compute unit as the kernel's occupancy allows.
+    s->ws_blocks = query_ws_blocks_per_cu(idx->weighted) * (ctx->num_cu > 0 ? (uint32_t)ctx->num_cu : 256u);
     uint64_t cap = s->cfg.staging_pairs;
     if (!cap) {
         // Admission by free device memory: a quarter of what is free, 8 bytes

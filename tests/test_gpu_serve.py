@@ -109,8 +109,8 @@ def test_persistent_and_grid_stride_kernels_agree(gpu_ctx, served):
     reqs = _requests(G.num_v, r2i, 12, seed=23)
     for ref in (False, True):
         outs = []
-        for gs in (False, True):
-            srv = pado.Server(gpu_ctx, idx, staging_pairs=1 << 20, streams=2, ref_compat=ref, grid_stride=gs)
+        for persistent in (True, False):
+            srv = pado.Server(gpu_ctx, idx, staging_pairs=1 << 20, streams=2, ref_compat=ref, persistent=persistent)
             try:
                 ids = [srv.submit(a, b) for a, b in reqs]
                 outs.append([srv.wait(r, a.size)[0] for r, (a, _) in zip(ids, reqs)])
