@@ -100,14 +100,18 @@ def cpu_baseline(args, ctx):
     }
 
 
-def serving_leg(ctx, scale=14, ef=8, duration_s=4.0, seed=5):
-    """Config 3: the R-MAT 14 index resident; open-loop Poisson arrivals at
-    fixed rates; each request carries floor(exp(U(0, ln 65536))) pairs, half
+def serving_leg(ctx, scale=14, ef=8, duration_s=4.0, seed=5, stream=False):
+    """The R-MAT 14 index resident; open-loop Poisson arrivals at fixed
+    rates; each request carries floor(exp(U(0, ln 65536))) pairs, half
     uniform, half Zipf(1.1) over degree rank.  Latency = completion - arrival
     (admission waits included).  Compared with the plain path: the same pairs
-    in one query launch."""
+    in one query launch.  stream=True builds the index with per-kernel
+    launches instead of hipGraphs (for profiler runs)."""
     edges, G, rank, rank2id = make_graph(scale, ef)
-    idx = pado.Index.build(ctx, pado.DeviceGraph(ctx, G))
+    cfg = pado.BuildCfg()
+    cfg.batch_size = pado.BATCH_SIZE
+    cfg.flags = pado.BUILD_STREAM if stream else 0
+    idx = pado.Index.build(ctx, pado.DeviceGraph(ctx, G), cfg=cfg)
     idx.order(rank)
     res = serve_measure(ctx, idx, G.num_v, rank2id, duration_s, seed)
     res["workload"] = (f"R-MAT {scale} ef{ef} index resident; open-loop Poisson; log-uniform 1..65536 "
@@ -345,6 +349,9 @@ def main():
                     help="only label N independent graphs in one job (multi-graph workload) and exit")
     ap.add_argument("--weighted-build-only", type=int, default=0, metavar="SCALE",
                     help="only build the weighted R-MAT of this scale (twice: warmup + timed) and exit (profiler runs)")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="only run the serving leg and exit (profiler runs; add --stream to build its index "
+                         "with per-kernel launches)")
     args = ap.parse_args()
     if args.copy_calibrate:
         copy_calibrate(args.copy_calibrate)
@@ -352,12 +359,16 @@ def main():
     if args.weighted_build_only:
         weighted_build_only(args.weighted_build_only)
         return
+    if args.serve_only:
+        res = serving_leg(pado.Context(0), stream=args.stream)
+        print(json.dumps({"metric": "serving leg only (profiling run)", "serving": res}), flush=True)
+        return
     if args.multi_graphs:
         print(json.dumps({"metric": "multi-graph labeling job", "multi": multi_leg(args.multi_graphs)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the index build and "
-              f"queries run on a single MI355X; see DESIGN.md for how they would shard)", file=sys.stderr)
+              f"queries run on a single MI355X)", file=sys.stderr)
         sys.exit(2)
     if pado.device_count() < 1:
         sys.exit("no GPU visible")
