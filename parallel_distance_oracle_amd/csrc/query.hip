@@ -26,9 +26,10 @@
 // queries/s on R-MAT 20 uniform pairs; loading the next pair's bit-parallel
 // bytes early as well measured slower).  Within a pair, every read that needs
 // only the label offsets is issued first (distance bytes, the shorter list's
-// first 64 labels, the longer list's first 128), the set pairs are requested
-// next and the label search runs while they are in flight (1.45 -> 1.78 G
-// queries/s; 256 or 512 early labels per wave measured slower).
+// first 64 labels, the longer list's first 256), the set pairs are requested
+// next and the label search runs while they are in flight.  k_query is held
+// to 64 VGPRs (8 waves per SIMD instead of 7): 1.45 -> 1.98 G queries/s for
+// both changes (without the cap, 128 early labels per wave measured best).
 #include "internal.hpp"
 
 namespace pado {
@@ -53,8 +54,8 @@ __device__ inline void q_lds_sync()
 
 constexpr uint32_t kQLds = 1024;   // labels of the longer list staged in LDS per wave
 constexpr uint32_t kQWaves = 4;    // waves per query block
-constexpr uint32_t kQPre = 2;      // longer-list loads per lane issued with the pair's first reads
-constexpr uint32_t kWPre = 4;      // the same for weighted pairs (their merge holds fewer values, so 4 pay)
+constexpr uint32_t kQPre = 4;      // longer-list loads per lane issued with the pair's first reads
+constexpr uint32_t kWPre = 4;      // the same for weighted pairs (8 measured slower)
 
 }  // namespace
 
@@ -194,7 +195,7 @@ __device__ inline void q_load_offsets(QPair &p, uint32_t n, const uint64_t *loff
     }
 }
 
-__global__ __launch_bounds__(256) void k_query(uint32_t n, const uint8_t *bpd, const ulonglong2 *bps,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_query(uint32_t n, const uint8_t *bpd, const ulonglong2 *bps,
                                                const uint64_t *loff, const uint32_t *lab, const uint32_t *qa,
                                                const uint32_t *qb, uint64_t k, uint8_t *out, int ref_compat)
 {
