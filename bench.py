@@ -261,8 +261,17 @@ def serve_measure(ctx, idx, n, rank2id, duration_s, seed, max_pairs_per_rate=600
     plain_qps = allA.size / plain_s
 
     srv = pado.Server(ctx, idx, staging_pairs=16 << 20, max_launch_pairs=4 << 20, streams=3)
-    # Warm every stream and launch size with a burst of 256 requests.
+    # Warm every stream and launch size with a burst of 256 requests, then with
+    # a discarded open-loop window (0.25 s at the lowest offered rate): a
+    # one-off host stall of several ms has landed in the first open-loop window
+    # after a burst.
     srv.replay(allA[:offs[256]], allB[:offs[256]], offs[:257], np.zeros(256))
+    w_rate = 0.2 * plain_qps / mean_pairs
+    w_m = max(int(w_rate * 0.25), 16)
+    w_arr = np.cumsum(rng.exponential(1.0 / w_rate, size=w_m))
+    w_off = np.concatenate([[0], np.cumsum(sizes[np.arange(w_m) % pool_n])]).astype(np.uint64)
+    w_gather = np.tile(np.arange(tot), -(-w_m // pool_n))[: int(w_off[-1])]
+    srv.replay(allA[w_gather], allB[w_gather], w_off, w_arr)
     rates = []
     ok_all = True
     # Offered rates as fractions of the plain batch path; the server overlaps

@@ -139,6 +139,8 @@ struct KernelTimes {
 struct pado_gpu_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;       // second stream of chunked host-array queries
+    hipEvent_t side_ev = nullptr;     // orders the side stream after work queued before a query
     int num_cu = 0;
 };
 

@@ -154,9 +154,13 @@ pado_gpu_status pado_gpu_ctx_create(int device, pado_gpu_ctx **out)
     c->device = device;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->num_cu = prop.multiProcessorCount;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->side_ev, hipEventDisableTiming) != hipSuccess) {
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        if (c->side) (void)hipStreamDestroy(c->side);
         delete c;
-        set_error("hipStreamCreateWithFlags failed");
+        set_error("stream or event creation failed");
         return PADO_GPU_ERR_HIP;
     }
     *out = c;
@@ -168,6 +172,8 @@ void pado_gpu_ctx_destroy(pado_gpu_ctx *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->side) (void)hipStreamDestroy(ctx->side);
+    if (ctx->side_ev) (void)hipEventDestroy(ctx->side_ev);
     delete ctx;
 }
 
@@ -461,6 +467,52 @@ pado_gpu_status pado_gpu_index_order(pado_gpu_ctx *ctx, pado_gpu_index *idx, con
     return order_index(idx, rank);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Host-array queries move 8 bytes of ids in and 1 (or 4) bytes of answers out
+// per pair.  In chunks of kHostChunk pairs alternating between the query
+// stream and the context's side stream, chunk i + 1's copy in runs while
+// chunk i's kernel does, and chunk i's answers are copied out after chunk
+// i + 1 is queued.  `launch(da, db, n, dout, stream)` queues one chunk's
+// kernel; es = answer bytes.
+constexpr uint64_t kHostChunk = 4ull << 20;
+
+template <typename Launch>
+pado_gpu_status host_query_chunked(pado_gpu_ctx *ctx, hipStream_t st, const uint32_t *a, const uint32_t *b,
+                                   uint64_t k, void *out, uint32_t es, uint32_t *da, uint32_t *db, uint8_t *dout,
+                                   Launch launch)
+{
+    hipStream_t ss[2] = {st, ctx->side};
+    // The side stream starts after whatever was queued on st before the call.
+    PADO_HIP(hipEventRecord(ctx->side_ev, st));
+    PADO_HIP(hipStreamWaitEvent(ctx->side, ctx->side_ev, 0));
+    uint8_t *hout = static_cast<uint8_t *>(out);
+    uint64_t prev0 = 0, prevn = 0;
+    hipStream_t prevs = st;
+    for (uint64_t c0 = 0, i = 0; c0 < k; c0 += kHostChunk, ++i) {
+        const uint64_t n = k - c0 < kHostChunk ? k - c0 : kHostChunk;
+        hipStream_t s = ss[i & 1];
+        PADO_HIP(hipMemcpyAsync(da + c0, a + c0, sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+        PADO_HIP(hipMemcpyAsync(db + c0, b + c0, sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+        PADO_TRY(launch(da + c0, db + c0, n, dout + c0 * es, s));
+        if (prevn)
+            PADO_HIP(hipMemcpyAsync(hout + prev0 * es, dout + prev0 * es, prevn * es, hipMemcpyDeviceToHost, prevs));
+        prev0 = c0;
+        prevn = n;
+        prevs = s;
+    }
+    PADO_HIP(hipMemcpyAsync(hout + prev0 * es, dout + prev0 * es, prevn * es, hipMemcpyDeviceToHost, prevs));
+    PADO_HIP(hipStreamSynchronize(ctx->side));
+    PADO_HIP(hipStreamSynchronize(st));
+    return PADO_GPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 pado_gpu_status pado_gpu_query(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uint32_t *a, const uint32_t *b, uint64_t k,
                                uint8_t *out, uint32_t flags, void *stream)
 {
@@ -492,12 +544,10 @@ pado_gpu_status pado_gpu_query(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uin
         PADO_TRY(idx->qb.alloc(k));
         PADO_TRY(idx->qout.alloc(k));
     }
-    PADO_HIP(hipMemcpyAsync(idx->qa.p, a, sizeof(uint32_t) * k, hipMemcpyHostToDevice, st));
-    PADO_HIP(hipMemcpyAsync(idx->qb.p, b, sizeof(uint32_t) * k, hipMemcpyHostToDevice, st));
-    PADO_TRY(launch_query(idx, rank_space, idx->qa.p, idx->qb.p, k, idx->qout.p, ref, st));
-    PADO_HIP(hipMemcpyAsync(out, idx->qout.p, k, hipMemcpyDeviceToHost, st));
-    PADO_HIP(hipStreamSynchronize(st));
-    return PADO_GPU_OK;
+    return host_query_chunked(ctx, st, a, b, k, out, 1u, idx->qa.p, idx->qb.p, idx->qout.p,
+                              [&](const uint32_t *da, const uint32_t *db, uint64_t n, uint8_t *dout, hipStream_t s) {
+                                  return launch_query(idx, rank_space, da, db, n, dout, ref, s);
+                              });
 }
 
 pado_gpu_status pado_gpu_windex_export(pado_gpu_ctx *ctx, const pado_gpu_index *idx, uint64_t *label_off,
@@ -550,12 +600,12 @@ pado_gpu_status pado_gpu_wquery(pado_gpu_ctx *ctx, pado_gpu_index *idx, const ui
         PADO_TRY(idx->qb.alloc(k));
     }
     if (idx->qout32.n < k) PADO_TRY(idx->qout32.alloc(k));
-    PADO_HIP(hipMemcpyAsync(idx->qa.p, a, sizeof(uint32_t) * k, hipMemcpyHostToDevice, st));
-    PADO_HIP(hipMemcpyAsync(idx->qb.p, b, sizeof(uint32_t) * k, hipMemcpyHostToDevice, st));
-    PADO_TRY(launch_wquery(idx, rank_space, idx->qa.p, idx->qb.p, k, idx->qout32.p, ref, st));
-    PADO_HIP(hipMemcpyAsync(out, idx->qout32.p, sizeof(uint32_t) * k, hipMemcpyDeviceToHost, st));
-    PADO_HIP(hipStreamSynchronize(st));
-    return PADO_GPU_OK;
+    return host_query_chunked(ctx, st, a, b, k, out, 4u, idx->qa.p, idx->qb.p,
+                              reinterpret_cast<uint8_t *>(idx->qout32.p),
+                              [&](const uint32_t *da, const uint32_t *db, uint64_t n, uint8_t *dout, hipStream_t s) {
+                                  return launch_wquery(idx, rank_space, da, db, n, reinterpret_cast<uint32_t *>(dout),
+                                                       ref, s);
+                              });
 }
 
 void pado_gpu_index_free(pado_gpu_index *idx) { delete idx; }
