@@ -125,3 +125,42 @@ def test_export_import_roundtrip(gpu_ctx):
         assert np.array_equal(x, y)
     qa, qb = pado.uniform_pairs(G.num_v, 5000, seed=4)
     assert np.array_equal(idx.query(qa, qb, rank_space=True), idx2.query(qa, qb, rank_space=True))
+
+
+@pytest.mark.parametrize("weighted", [False, True])
+def test_host_array_queries_in_chunks(gpu_ctx, weighted):
+    """Host-array queries longer than one 4M-pair chunk (copies and kernels of
+    consecutive chunks on two streams, a ragged last chunk) give the answers of
+    one device-pointer launch over the same pairs, exact and
+    reference-compatible; out-of-range ids and self pairs sit in several chunks."""
+    from parallel_distance_oracle_amd.hip import DeviceArray
+
+    e = rmat_edges(12, 8)
+    if weighted:
+        G = pado.WeightedGraph.from_edges(e, pado.edge_weights(e.shape[0], 1, 100, seed=1))
+    else:
+        G = pado.Graph.from_edges(e)
+    rank = G.make_rank()
+    G.id_transfer(rank)
+    print(f"[gpu] chunked host queries weighted={weighted} n={G.num_v}", flush=True)
+    idx = pado.Index.build(gpu_ctx, pado.DeviceGraph(gpu_ctx, G))
+    idx.order(rank)
+    k = 9_000_001  # chunks of 4M, 4M and 1M + 1 pairs
+    qa, qb = pado.uniform_pairs(G.num_v, k, seed=13)
+    for pos in (5, 4_194_300, 4_194_310, 8_388_700, k - 1):
+        qa[pos] = G.num_v + 3
+        qb[pos + 1 if pos + 1 < k else pos - 1] = qa[pos + 1 if pos + 1 < k else pos - 1]
+    dt = np.uint32 if weighted else np.uint8
+    da, db = DeviceArray.from_numpy(qa), DeviceArray.from_numpy(qb)
+    out = DeviceArray(k, dt)
+    for ref in (False, True):
+        host = idx.query(qa, qb, ref_compat=ref)
+        idx.query_device(da.ptr, db.ptr, k, out.ptr, ref_compat=ref, stream=gpu_ctx.stream)
+        gpu_ctx.sync()
+        dev = out.numpy()
+        assert host.dtype == dt and host.size == k
+        assert np.array_equal(host, dev)
+    da.free()
+    db.free()
+    out.free()
+    idx.close()
