@@ -119,6 +119,7 @@ struct pado_gpu_server {
     bool stopping = false;
     std::mutex mu;
     std::condition_variable cv_work, cv_done, cv_launch;
+    std::condition_variable cv_space;  // a request released its slot and ring space
     std::thread dispatcher, completer;
     pado_gpu_serve_stats st{};
     std::string error;
@@ -300,6 +301,7 @@ pado_gpu_status wait_request(pado_gpu_server *s, uint64_t id, void *out, double 
     release_consumed(s);
     lk.unlock();
     s->cv_work.notify_all();
+    s->cv_space.notify_all();
     return PADO_GPU_OK;
 }
 
@@ -503,7 +505,11 @@ pado_gpu_status pado_gpu_serve_replay(pado_gpu_server *s, const uint32_t *a, con
                     failed.store(1);
                     return;
                 }
-                std::this_thread::yield();
+                // Refused: wait for a release instead of retrying at once, so
+                // that retries do not contend for the lock the dispatcher and
+                // completer need (under overload that cut the served rate).
+                std::unique_lock<std::mutex> lk(s->mu);
+                s->cv_space.wait_for(lk, std::chrono::microseconds(200));
             }
             ids[i].store(id);
         }
