@@ -40,7 +40,8 @@ typedef enum {
     PADO_GPU_ERR_DOMAIN = 4,  /* graph outside the uint8 distance domain       */
     PADO_GPU_ERR_STATE = 5,   /* call out of order (e.g. query before order)  */
     PADO_GPU_ERR_IO = 6,      /* file could not be opened / read / written     */
-    PADO_GPU_ERR_BUSY = 7     /* serving admission refused (no slot / memory)  */
+    PADO_GPU_ERR_BUSY = 7,    /* serving admission refused (no slot / memory)  */
+    PADO_GPU_ERR_TIMEOUT = 8  /* serving wait ended before the request was done */
 } pado_gpu_status;
 
 typedef struct pado_gpu_ctx pado_gpu_ctx;
@@ -80,6 +81,13 @@ void pado_wgraph_csr(const uint32_t *edges, const uint32_t *weights, uint64_t m,
 void pado_wid_transfer(uint32_t n, const uint64_t *off, const uint32_t *adj, const uint8_t *w,
                        const uint32_t *rank, uint64_t *off_new, uint32_t *adj_new, uint8_t *w_new,
                        uint32_t *rank2id);
+
+/* Vertex ranges of the final canonical label sort (order_labels' per-vertex
+ * sort, PARA:2240, done on the device in calls of bounded size): cuts[0] = 0
+ * < ... < cuts[r] = n, each range [cuts[i], cuts[i+1]) holding at most
+ * max_items labels of the CSR off[n+1].  Returns r, or 0 if one vertex alone
+ * exceeds max_items or max_cuts (>= r + 1 entries) is too small. */
+uint32_t pado_sort_plan(const uint64_t *off, uint32_t n, uint64_t max_items, uint32_t *cuts, uint32_t max_cuts);
 
 /* Deterministic workload generators (no reference counterpart; they produce
  * the synthetic inputs the reference's scripts would download).  Return the
@@ -147,7 +155,10 @@ typedef struct {
     uint32_t iters_per_sync; /* batch iterations enqueued per host check of the
                                 frontier (iterations past a batch's end are
                                 empty launches), 0 = 8 */
-    uint32_t reserved[3];
+    uint32_t sort_max_items; /* labels per call of the final canonical sort
+                                (0 = 2^31 - 1025, hipCUB's int item count);
+                                larger indexes are sorted in vertex ranges */
+    uint32_t reserved[2];
 } pado_gpu_build_cfg;
 
 /* Bit-parallel labeling one BFS per slot (as the reference runs it) instead of
@@ -159,12 +170,29 @@ typedef struct {
  * launch bracketed by HIP events (pado_gpu_index_info's ms_push / ms_add).
  * Same output either way. */
 #define PADO_GPU_BUILD_STREAM 2u
+/* Wide hop domain (no reference counterpart: the reference's u8 level and
+ * iteration counters wrap at 255, GLB:29-40, PARA:711/1470).  Labels carry
+ * 12-bit hop distances (up to 4094; hubs below 2^20, so at most 2^20
+ * vertices), bit-parallel distances stay u8 with the bit-parallel BFS stopped
+ * at level 254 (farther vertices keep 255 in that slot, which pruning and
+ * queries then skip).  When that cut happens, the bit-parallel roots and
+ * their selected neighbours are labeled as ordinary hubs and the batch phase
+ * prunes by labels alone (the reference never pushes from those vertices,
+ * PARA:1117-1119, and prunes by the slots, both exact only while every slot
+ * reaches every vertex); queries still use the slots wherever both ends were
+ * reached.  For graphs whose BFS depth reaches 255 -- large
+ * grids, road-like graphs.  Every distance stays exact.  On a graph inside
+ * the u8 domain (every batch iteration below 253) the label sets equal the
+ * default build's.  Query with pado_gpu_query_wide; export with
+ * pado_gpu_index_export_wide.  Ignored for weighted graphs. */
+#define PADO_GPU_BUILD_WIDE 4u
 
 /* ParaVertexCentricPLL<BATCH_SIZE>(const Graph&) -> construct (PARA:309-313,
  * 1740-1895): bit-parallel labeling (PARA:544-840) then batch_process over all
  * root batches (PARA:1413-1736).  cfg may be NULL.  Returns
  * PADO_GPU_ERR_DOMAIN if a BFS level or batch iteration reaches 255 (the
- * reference's u8 counters wrap there). */
+ * reference's u8 counters wrap there), or with PADO_GPU_BUILD_WIDE if an
+ * iteration reaches 4095 or the graph has more than 2^20 vertices. */
 pado_gpu_status pado_gpu_build(pado_gpu_ctx *ctx, const pado_gpu_graph *graph, const pado_gpu_build_cfg *cfg,
                                pado_gpu_index **out);
 
@@ -193,6 +221,9 @@ typedef struct {
     /* HIP-event times of the labeling kernels on the build stream: */
     double ms_push, ms_add, ms_bp_levels;
     uint64_t n_push, n_add, n_bp_levels;  /* launches */
+    uint32_t dist_bits;        /* label distance bits: 8 (the reference's u8
+                                  domain) or 12 (PADO_GPU_BUILD_WIDE) */
+    uint32_t reserved_info;
 } pado_gpu_index_info;
 
 pado_gpu_status pado_gpu_index_get_info(const pado_gpu_index *idx, pado_gpu_index_info *info);
@@ -203,6 +234,10 @@ pado_gpu_status pado_gpu_index_get_info(const pado_gpu_index *idx, pado_gpu_inde
  * order_labels produces (PARA:2221-2251), indexed by rank instead of id. */
 pado_gpu_status pado_gpu_index_export(pado_gpu_ctx *ctx, const pado_gpu_index *idx, uint8_t *bp_dist,
                                       uint64_t *bp_sets, uint64_t *label_off, uint32_t *hubs, uint8_t *dists);
+/* The same export for either hop domain, with u16 label distances (a wide
+ * index's labels do not fit export's u8 dists). */
+pado_gpu_status pado_gpu_index_export_wide(pado_gpu_ctx *ctx, const pado_gpu_index *idx, uint8_t *bp_dist,
+                                           uint64_t *bp_sets, uint64_t *label_off, uint32_t *hubs, uint16_t *dists);
 /* Inverse of export (rank space); hubs must ascend per vertex. */
 pado_gpu_status pado_gpu_index_import(pado_gpu_ctx *ctx, uint32_t n, const uint8_t *bp_dist, const uint64_t *bp_sets,
                                       const uint64_t *label_off, const uint32_t *hubs, const uint8_t *dists,
@@ -224,6 +259,17 @@ pado_gpu_status pado_gpu_index_order(pado_gpu_ctx *ctx, pado_gpu_index *idx, con
  * otherwise it stages through device buffers and returns when done. */
 pado_gpu_status pado_gpu_query(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uint32_t *a, const uint32_t *b, uint64_t k,
                                uint8_t *out, uint32_t flags, void *stream);
+
+/* query_distance for an index of either hop domain with u16 answers: a
+ * wide-domain index answers exact hop counts, 0xFFFF if unreachable (its
+ * bit-parallel part uses only slots reached on both sides; REF_COMPAT is
+ * refused, the reference has no such domain); an index in the reference's
+ * domain answers what pado_gpu_query answers, widened.  Other flags as for
+ * pado_gpu_query. */
+#define PADO_GPU_WIDE_INF 0xFFFFu
+pado_gpu_status pado_gpu_query_wide(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uint32_t *a, const uint32_t *b,
+                                    uint64_t k, uint16_t *out, uint32_t flags, void *stream);
+
 
 void pado_gpu_index_free(pado_gpu_index *idx);
 
@@ -289,19 +335,33 @@ pado_gpu_status pado_gpu_autotune(pado_gpu_ctx *ctx, const pado_gpu_graph *graph
  * one interleaved union -- label sets depend on neither BATCH_SIZE nor other
  * components, so each graph gets exactly the labels of a build on its own --
  * with every graph keeping its own 50 bit-parallel slots.  The others are
- * built one by one and report PADO_GPU_ERR_DOMAIN if they leave the u8 range.
+ * built one by one: in the u8 domain if they stay inside it, else in the
+ * wide hop domain (PADO_GPU_BUILD_WIDE: a graph whose first bit-parallel
+ * root's eccentricity reaches 255 goes there directly); only a graph outside
+ * the wide domain too reports PADO_GPU_ERR_DOMAIN.
  * ------------------------------------------------------------------------ */
 typedef struct pado_gpu_multi pado_gpu_multi;
 
 pado_gpu_status pado_gpu_build_multi(pado_gpu_ctx *ctx, uint32_t num_graphs, const uint32_t *n, const uint64_t *off,
                                      const uint32_t *adj, const pado_gpu_build_cfg *cfg, pado_gpu_multi **out);
-/* Per-graph result: status (OK or ERR_DOMAIN), label count, and whether it
- * was labeled inside the union (1) or on its own (0). */
+/* Per-graph result: status (OK or ERR_DOMAIN), label count, whether it was
+ * labeled inside the union (1) or on its own (0), and its label distance
+ * bits (8: the reference's u8 domain, 12: wide).  Outputs are nullable. */
 pado_gpu_status pado_gpu_multi_graph_info(const pado_gpu_multi *mi, uint32_t g, pado_gpu_status *status,
-                                          uint64_t *num_labels, int *in_union);
-/* Canonical rank-space export of graph g (same layout as pado_gpu_index_export). */
+                                          uint64_t *num_labels, int *in_union, uint32_t *dist_bits);
+/* Canonical rank-space export of graph g (same layout as pado_gpu_index_export;
+ * PADO_GPU_ERR_STATE for a wide-domain graph's u8 distances). */
 pado_gpu_status pado_gpu_multi_export(pado_gpu_ctx *ctx, pado_gpu_multi *mi, uint32_t g, uint8_t *bp_dist,
                                       uint64_t *bp_sets, uint64_t *label_off, uint32_t *hubs, uint8_t *dists);
+/* The same with u16 distances, for a graph of either domain. */
+pado_gpu_status pado_gpu_multi_export_wide(pado_gpu_ctx *ctx, pado_gpu_multi *mi, uint32_t g, uint8_t *bp_dist,
+                                           uint64_t *bp_sets, uint64_t *label_off, uint32_t *hubs, uint16_t *dists);
+/* query_distance (PARA:2283-2416) on graph g for k pairs of its rank-space
+ * ids (host arrays): u16 answers -- for a graph in the u8 domain the
+ * reference's answers (255 = unreachable or >= 253), for a wide-domain graph
+ * exact hops (0xFFFF = unreachable). */
+pado_gpu_status pado_gpu_multi_query(pado_gpu_ctx *ctx, pado_gpu_multi *mi, uint32_t g, const uint32_t *a,
+                                     const uint32_t *b, uint64_t k, uint16_t *out);
 /* Whole-job figures: number of unions, union vertices, build ms. */
 pado_gpu_status pado_gpu_multi_info(const pado_gpu_multi *mi, uint32_t *num_unions, uint64_t *union_vertices,
                                     double *ms_total);
@@ -364,11 +424,23 @@ pado_gpu_status pado_gpu_serve_submit(pado_gpu_server *srv, const uint32_t *a, c
 /* *done = 1 once the request's answers are ready. */
 pado_gpu_status pado_gpu_serve_poll(pado_gpu_server *srv, uint64_t req_id, int *done);
 /* Blocks until done, copies the k answers to out (k u8 values for an
- * unweighted index, k u32 for a weighted one), releases the slot.
- * latency_ms (nullable) = completion time - submit time.  A request is
- * waited for once: another wait on the same id, concurrent or later,
- * returns PADO_GPU_ERR_ARG. */
+ * unweighted index, k u16 for a wide-domain one, k u32 for a weighted one),
+ * releases the slot.  latency_ms (nullable) = completion time - submit time.
+ * A request is waited for once: another wait on the same id, concurrent or
+ * later, returns PADO_GPU_ERR_ARG.  If its launch failed, the request is
+ * released and PADO_GPU_ERR_HIP returned; after a failure the server admits
+ * nothing more (submit returns PADO_GPU_ERR_STATE). */
 pado_gpu_status pado_gpu_serve_wait(pado_gpu_server *srv, uint64_t req_id, void *out, double *latency_ms);
+/* The same, waiting at most timeout_us microseconds (0 = a poll): on
+ * PADO_GPU_ERR_TIMEOUT the request stays live, to be waited for again or
+ * cancelled. */
+pado_gpu_status pado_gpu_serve_wait_for(pado_gpu_server *srv, uint64_t req_id, void *out, double *latency_ms,
+                                        uint64_t timeout_us);
+/* Withdraws a request: queued, it is never launched; launched, its answers
+ * are dropped when the launch ends; done, they are discarded.  Its slot and
+ * ring space are released (a launched one's when the launch ends), and its
+ * id becomes unknown.  PADO_GPU_ERR_STATE while a wait on it is in progress. */
+pado_gpu_status pado_gpu_serve_cancel(pado_gpu_server *srv, uint64_t req_id);
 pado_gpu_status pado_gpu_serve_get_stats(pado_gpu_server *srv, pado_gpu_serve_stats *st);
 /* Open-loop load generator: request i (pairs [req_off[i], req_off[i+1]) of a
  * and b) is submitted at arrival_s[i] seconds after the start, whatever

@@ -210,15 +210,21 @@ def read_bp(path):
     return d, s
 
 
-def run_ref(edges, rank=None, pairs=None, threads=None, repeat=1, dump=True):
+def run_ref(edges, rank=None, pairs=None, threads=None, repeat=1, dump=True, env_extra=None):
     """Run the reference build on `edges` (original ids) with `rank` (or its own
     make_rank when None).  Returns dict with labels/bp (rank space), answers
-    (original-id pairs) and the timing lines it printed."""
+    (original-id pairs) and the timing lines it printed: ref_build_wall_s
+    (constructor wall time, last build), ref_build_best_s, build_walls (every
+    build), total_labeling_s (the reference's own Total_labeling_time per
+    build, PARA:1770-1837) and, with pairs, ref_query_serial_s /
+    ref_query_omp_s."""
     if not have_ref():
         raise FileNotFoundError(REF_BIN)
     env = dict(os.environ)
     if threads:
         env["OMP_NUM_THREADS"] = str(threads)
+    if env_extra:
+        env.update(env_extra)
     with tempfile.TemporaryDirectory() as td:
         ep = os.path.join(td, "e.bin")
         _write_edges(ep, edges)
@@ -238,9 +244,13 @@ def run_ref(edges, rank=None, pairs=None, threads=None, repeat=1, dump=True):
         pre = os.path.join(td, "out") if dump else "-"
         cp = subprocess.run([REF_BIN, "unw", ep, rp, pre, pp, str(repeat)], env=env,
                             capture_output=True, text=True, check=True)
-        res = {"stdout": cp.stdout}
+        res = {"stdout": cp.stdout, "build_walls": [], "total_labeling_s": []}
         for line in cp.stdout.splitlines():
             toks = line.split()
+            if toks[:1] == ["ref_build_wall_s:"]:
+                res["build_walls"].append(float(toks[1]))
+            if "Total_labeling_time:" in toks:
+                res["total_labeling_s"].append(float(toks[toks.index("Total_labeling_time:") + 1]))
             if "bp_hit_count:" in toks:
                 # printed by construct (PARA:1889); one line per build
                 res["bp_hit_count"] = int(toks[toks.index("bp_hit_count:") + 1].replace(",", ""))
@@ -256,6 +266,22 @@ def run_ref(edges, rank=None, pairs=None, threads=None, repeat=1, dump=True):
             if pairs is not None:
                 res["answers"] = np.fromfile(pre + ".answers.bin", np.uint8)
     return res
+
+
+def ref_unp_file(edges, rank):
+    """The index file the reference's own writer produces
+    (VertexCentricPLL<1024>::store_index_to_file, UNP:1120-1180) for `edges`
+    in the vertex order `rank`, as bytes."""
+    unp = os.path.join(HERE, "_ref", "pado_ref_unp")
+    if not os.path.exists(unp):
+        raise FileNotFoundError(unp)
+    with tempfile.TemporaryDirectory() as td:
+        ep, rp, ip = (os.path.join(td, x) for x in ("e.bin", "r.bin", "i.idx"))
+        _write_edges(ep, edges)
+        _write_rank(rp, rank)
+        subprocess.run([unp, ep, rp, ip], check=True, capture_output=True)
+        with open(ip, "rb") as f:
+            return f.read()
 
 
 def ref_rank(edges):
@@ -305,6 +331,9 @@ def _wlib():
         L.wo_free.argtypes = [ctypes.c_void_p]
         L.wo_query_batch.argtypes = [ctypes.c_uint32, u64p, u32p, i16p, u32p, u32p, ctypes.c_uint64, i16p]
         L.wo_dijkstra.argtypes = [ctypes.c_uint32, u64p, u32p, u8p, ctypes.c_uint32, u32p]
+        L.wo_canonical_of.restype = ctypes.c_uint64
+        L.wo_canonical_of.argtypes = [ctypes.c_uint32, u64p, u32p, u8p, ctypes.c_uint32, u32p, u32p, u32p,
+                                      ctypes.c_uint64]
         L._w_ready = True
     return L
 
@@ -368,6 +397,22 @@ def dijkstra(n, off, adj, w, src):
     d = np.zeros(max(n, 1), np.uint32)
     _wlib().wo_dijkstra(n, _c(off, np.uint64), _c(adj, np.uint32), _c(w, np.uint8), src, d)
     return d[:n]
+
+
+def canonical_of(n, off, adj, w, v):
+    """Dijkstra distances from rank-space vertex v and v's canonical label
+    set (hubs ascending, distances) from the same run: (dist[n], hub, hd)."""
+    L = _wlib()
+    d = np.zeros(max(n, 1), np.uint32)
+    off, adj, w = _c(off, np.uint64), _c(adj, np.uint32), _c(w, np.uint8)
+    cap = 4096
+    while True:
+        hub = np.zeros(cap, np.uint32)
+        hd = np.zeros(cap, np.uint32)
+        cnt = int(L.wo_canonical_of(n, off, adj, w, v, d, hub, hd, cap))
+        if cnt <= cap:
+            return d[:n], hub[:cnt], hd[:cnt]
+        cap = cnt
 
 
 def read_wlabels(path):

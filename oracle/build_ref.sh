@@ -22,7 +22,8 @@ fi
 mkdir -p "$OUT"
 TMP=$(mktemp -d /tmp/pado_ref_build.XXXXXX)
 trap 'rm -rf "$TMP"' EXIT
-for h in globals.h graph.h pado_unw_unv_para.201912311607.parallel_bp_labeling.h pado_weighted_unv_para.h; do
+for h in globals.h graph.h pado_unw_unv_para.201912311607.parallel_bp_labeling.h pado_weighted_unv_para.h \
+         pado_unw_unv_unp.h; do
     sed -e 's/push_back(IndexType::/push_back(typename IndexType::/g' "$REF/includes/$h" > "$TMP/$h"
 done
 CXX=${CXX:-g++}
@@ -30,3 +31,7 @@ $CXX -std=c++14 -O3 -march=x86-64-v3 -fopenmp -fpermissive -w -DPADO_REF_WEIGHTE
     -I"$TMP" "$HERE/ref_harness.cpp" -o "$OUT/pado_ref.tmp"
 mv "$OUT/pado_ref.tmp" "$OUT/pado_ref"
 echo "built $OUT/pado_ref"
+# The sequential class, whose index-file writer works (UNP:1120-1180).
+$CXX -std=c++14 -O2 -march=x86-64-v3 -fopenmp -fpermissive -w -I"$TMP" "$HERE/ref_unp.cpp" -o "$OUT/pado_ref_unp.tmp"
+mv "$OUT/pado_ref_unp.tmp" "$OUT/pado_ref_unp"
+echo "built $OUT/pado_ref_unp"

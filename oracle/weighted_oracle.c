@@ -351,3 +351,74 @@ void wo_dijkstra(uint32_t n, const uint64_t *off, const uint32_t *adj, const uin
     }
     free(h);
 }
+
+/* The canonical label set of ONE vertex v from one Dijkstra, for checks at
+ * sizes where all-pairs Dijkstra is out of reach.  (h, d(v, h)) is a label of
+ * v iff h ranks at or above v (h <= v in rank space), h is reachable, and no
+ * vertex ranked above h lies on any shortest v-h path -- the set
+ * WeightedParaVertexCentricPLL builds (WPARA:1100-1228 filters exactly the
+ * pairs with such a vertex).  With worst[x] = the smallest id over all
+ * shortest v-x paths (endpoints included), h qualifies iff worst[h] == h.
+ * Edge weights are positive, so every shortest-path predecessor of x settles
+ * before x and one pass in settle order computes worst[].  Writes dist[n] and
+ * up to cap labels (hub ascending); returns the label count (which may
+ * exceed cap). */
+uint64_t wo_canonical_of(uint32_t n, const uint64_t *off, const uint32_t *adj, const uint8_t *w, uint32_t v,
+                         uint32_t *dist, uint32_t *hub, uint32_t *hd, uint64_t cap)
+{
+    typedef struct { uint32_t d, v; } it;
+    it *h = malloc(sizeof(it) * (off[n] + 2));
+    uint32_t *order = malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint32_t *worst = malloc(sizeof(uint32_t) * (n ? n : 1));
+    uint64_t hn = 0, no = 0;
+    for (uint32_t x = 0; x < n; ++x) dist[x] = UINT32_MAX;
+    dist[v] = 0;
+    h[hn++] = (it){0, v};
+    while (hn) {
+        it top = h[0];
+        h[0] = h[--hn];
+        for (uint64_t i = 0;;) {
+            uint64_t l = 2 * i + 1, r = l + 1, s = i;
+            if (l < hn && h[l].d < h[s].d) s = l;
+            if (r < hn && h[r].d < h[s].d) s = r;
+            if (s == i) break;
+            it t = h[i]; h[i] = h[s]; h[s] = t; i = s;
+        }
+        if (top.d != dist[top.v]) continue;
+        order[no++] = top.v;
+        for (uint64_t e = off[top.v]; e < off[top.v + 1]; ++e) {
+            uint32_t u = adj[e], nd = top.d + w[e];
+            if (nd < dist[u]) {
+                dist[u] = nd;
+                uint64_t i = hn++;
+                h[i] = (it){nd, u};
+                while (i && h[(i - 1) / 2].d > h[i].d) {
+                    it t = h[i]; h[i] = h[(i - 1) / 2]; h[(i - 1) / 2] = t; i = (i - 1) / 2;
+                }
+            }
+        }
+    }
+    for (uint64_t k = 0; k < no; ++k) {
+        const uint32_t x = order[k];
+        uint32_t m = x;
+        if (x != v)
+            for (uint64_t e = off[x]; e < off[x + 1]; ++e) {
+                const uint32_t p = adj[e];
+                if (dist[p] != UINT32_MAX && (uint64_t)dist[p] + w[e] == dist[x] && worst[p] < m) m = worst[p];
+            }
+        worst[x] = m;
+    }
+    uint64_t cnt = 0;
+    for (uint32_t x = 0; x <= v && x < n; ++x) {
+        if (dist[x] == UINT32_MAX || worst[x] != x) continue;
+        if (cnt < cap) {
+            hub[cnt] = x;
+            hd[cnt] = dist[x];
+        }
+        ++cnt;
+    }
+    free(h);
+    free(order);
+    free(worst);
+    return cnt;
+}

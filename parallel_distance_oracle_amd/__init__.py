@@ -32,12 +32,14 @@ QUERY_REF_COMPAT = 2
 QUERY_RANK_SPACE = 4
 BUILD_BP_PER_SLOT = 1
 BUILD_STREAM = 2  # per-kernel launches with HIP-event timing (default: hipGraph batch step)
+BUILD_WIDE = 4  # wide hop domain: 12-bit label distances, u16 answers (graphs deeper than 254)
+WIDE_INF = 0xFFFF  # wide-domain query: unreachable
 SERVE_GRID_STRIDE = 0x10
 SERVE_PERSISTENT = 0x20
 WINF = 0xFFFFFFFF  # weighted query: no common hub (unreachable)
 
 _STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_NOMEM", 4: "ERR_DOMAIN", 5: "ERR_STATE",
-           6: "ERR_IO", 7: "ERR_BUSY"}
+           6: "ERR_IO", 7: "ERR_BUSY", 8: "ERR_TIMEOUT"}
 
 
 class PadoError(RuntimeError):
@@ -49,7 +51,7 @@ class PadoError(RuntimeError):
 class BuildCfg(ctypes.Structure):
     _fields_ = [("batch_size", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("push_blocks", ctypes.c_uint32),
                 ("add_blocks", ctypes.c_uint32), ("iters_per_sync", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 3)]
+                ("sort_max_items", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
 
     def as_dict(self):
         return {"batch_size": self.batch_size, "flags": self.flags, "push_blocks": self.push_blocks,
@@ -79,6 +81,7 @@ class IndexInfo(ctypes.Structure):
         ("dq_labels", ctypes.c_uint64), ("labels_appended", ctypes.c_uint64),
         ("ms_push", ctypes.c_double), ("ms_add", ctypes.c_double), ("ms_bp_levels", ctypes.c_double),
         ("n_push", ctypes.c_uint64), ("n_add", ctypes.c_uint64), ("n_bp_levels", ctypes.c_uint64),
+        ("dist_bits", ctypes.c_uint32), ("reserved_info", ctypes.c_uint32),
     ]
 
 
@@ -104,6 +107,7 @@ def lib():
         "pado_id_transfer": (None, [_u32, _vp, _vp, _vp, _vp, _vp, _vp]),
         "pado_wgraph_csr": (None, [_vp, _vp, _u64, _u32, _vp, _vp, _vp]),
         "pado_wid_transfer": (None, [_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+        "pado_sort_plan": (_u32, [_vp, _u32, _u64, _vp, _u32]),
         "pado_gpu_windex_export": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
         "pado_gpu_wquery": (ctypes.c_int, [_vp, _vp, _vp, _vp, _u64, _vp, _u32, _vp]),
         "pado_rmat_edges": (_u64, [_u32, _u32, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u64, _vp, _u64]),
@@ -130,6 +134,8 @@ def lib():
         "pado_gpu_index_import": (ctypes.c_int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
         "pado_gpu_index_order": (ctypes.c_int, [_vp, _vp, _vp, _u32]),
         "pado_gpu_query": (ctypes.c_int, [_vp, _vp, _vp, _vp, _u64, _vp, _u32, _vp]),
+        "pado_gpu_query_wide": (ctypes.c_int, [_vp, _vp, _vp, _vp, _u64, _vp, _u32, _vp]),
+        "pado_gpu_index_export_wide": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "pado_gpu_index_free": (None, [_vp]),
         "pado_gpu_index_store": (ctypes.c_int, [_vp, _vp, ctypes.c_char_p, _vp, _u32]),
         "pado_gpu_index_load": (ctypes.c_int, [_vp, ctypes.c_char_p, ctypes.POINTER(_vp)]),
@@ -138,8 +144,11 @@ def lib():
         "pado_gpu_build_multi": (ctypes.c_int, [_vp, _u32, _vp, _vp, _vp, ctypes.POINTER(BuildCfg),
                                                 ctypes.POINTER(_vp)]),
         "pado_gpu_multi_graph_info": (ctypes.c_int, [_vp, _u32, ctypes.POINTER(ctypes.c_int),
-                                                     ctypes.POINTER(_u64), ctypes.POINTER(ctypes.c_int)]),
+                                                     ctypes.POINTER(_u64), ctypes.POINTER(ctypes.c_int),
+                                                     ctypes.POINTER(_u32)]),
         "pado_gpu_multi_export": (ctypes.c_int, [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp]),
+        "pado_gpu_multi_export_wide": (ctypes.c_int, [_vp, _vp, _u32, _vp, _vp, _vp, _vp, _vp]),
+        "pado_gpu_multi_query": (ctypes.c_int, [_vp, _vp, _u32, _vp, _vp, _u64, _vp]),
         "pado_gpu_multi_info": (ctypes.c_int, [_vp, ctypes.POINTER(_u32), ctypes.POINTER(_u64),
                                                ctypes.POINTER(ctypes.c_double)]),
         "pado_gpu_multi_free": (None, [_vp]),
@@ -147,6 +156,8 @@ def lib():
         "pado_gpu_serve_submit": (ctypes.c_int, [_vp, _vp, _vp, _u64, ctypes.POINTER(_u64)]),
         "pado_gpu_serve_poll": (ctypes.c_int, [_vp, _u64, ctypes.POINTER(ctypes.c_int)]),
         "pado_gpu_serve_wait": (ctypes.c_int, [_vp, _u64, _vp, ctypes.POINTER(ctypes.c_double)]),
+        "pado_gpu_serve_wait_for": (ctypes.c_int, [_vp, _u64, _vp, ctypes.POINTER(ctypes.c_double), _u64]),
+        "pado_gpu_serve_cancel": (ctypes.c_int, [_vp, _u64]),
         "pado_gpu_serve_get_stats": (ctypes.c_int, [_vp, ctypes.POINTER(ServeStats)]),
         "pado_gpu_serve_replay": (ctypes.c_int, [_vp, _vp, _vp, _vp, _u32, _vp, _vp, _vp,
                                                  ctypes.POINTER(ctypes.c_double)]),
@@ -212,6 +223,35 @@ def uniform_pairs(n, k, seed=1):
     b = np.zeros(max(k, 1), np.uint32)
     lib().pado_uniform_pairs(n, k, seed, _p(a), _p(b))
     return a[:k], b[:k]
+
+
+def mixed_graphs(count, seed=4):
+    """The many-graph workload: `count` rank-space graphs with vertex counts
+    log-uniform in [2^10, 2^16]; even-numbered ones R-MAT (edge factor 8,
+    (0.57, 0.19, 0.19, 0.05)), odd-numbered ones 2D grids with 5% of the edges
+    deleted (road-like, large diameter)."""
+    rng = np.random.default_rng(seed)
+    graphs = []
+    for i in range(count):
+        nv = 2.0 ** rng.uniform(10, 16)
+        if i % 2 == 0:
+            e = rmat_edges(max(1, int(round(np.log2(nv)))), 8, 0.57, 0.19, 0.19, seed + i)
+        else:
+            e = grid_edges(max(2, int(round(np.sqrt(nv)))), 0.05, seed + i)
+        G = Graph.from_edges(e)
+        G.id_transfer(G.make_rank())
+        graphs.append(G)
+    return graphs
+
+
+def sort_plan(off, max_items):
+    """Vertex ranges of the final label sort (pado_sort_plan): the cut list,
+    or None if one vertex alone exceeds max_items."""
+    off = _c(off, np.uint64)
+    n = off.size - 1
+    cuts = np.zeros(n + 2, np.uint32)
+    r = lib().pado_sort_plan(_p(off), n, max_items, _p(cuts), n + 2)
+    return None if r == 0 else cuts[: r + 1]
 
 
 def zipf_vertices(n, k, s=1.1, seed=1, rank2id=None):
@@ -468,11 +508,11 @@ class Index:
         self.num_v = self.info.num_v
 
     @classmethod
-    def build(cls, ctx, dgraph, batch_size=BATCH_SIZE, bp_per_slot=False, cfg=None):
+    def build(cls, ctx, dgraph, batch_size=BATCH_SIZE, bp_per_slot=False, cfg=None, wide=False):
         if cfg is None:
             cfg = BuildCfg()
             cfg.batch_size = batch_size
-            cfg.flags = BUILD_BP_PER_SLOT if bp_per_slot else 0
+            cfg.flags = (BUILD_BP_PER_SLOT if bp_per_slot else 0) | (BUILD_WIDE if wide else 0)
         h = _vp()
         _check(lib().pado_gpu_build(ctx.h, dgraph.h, ctypes.byref(cfg), ctypes.byref(h)))
         return cls(ctx, h)
@@ -505,20 +545,31 @@ class Index:
         return {k: getattr(i, k) for k, _ in IndexInfo._fields_ if k != "bp_roots"}
 
     def export(self):
-        """Canonical rank-space arrays: (bp_d[n,50], bp_s[n,50,2], loff[n+1], hub, dist)."""
+        """Canonical rank-space arrays: (bp_d[n,50], bp_s[n,50,2], loff[n+1], hub, dist); dist is u8, or
+        u16 for a wide-domain index."""
         n = self.num_v
         tot = self.info.num_labels
         bp_d = np.zeros(n * BP_SIZE, np.uint8)
         bp_s = np.zeros(n * BP_SIZE * 2, np.uint64)
         loff = np.zeros(n + 1, np.uint64)
         hub = np.zeros(max(tot, 1), np.uint32)
-        dist = np.zeros(max(tot, 1), np.uint8)
-        _check(lib().pado_gpu_index_export(self.ctx.h, self.h, _p(bp_d), _p(bp_s), _p(loff), _p(hub), _p(dist)))
+        if self.wide:
+            dist = np.zeros(max(tot, 1), np.uint16)
+            fn = lib().pado_gpu_index_export_wide
+        else:
+            dist = np.zeros(max(tot, 1), np.uint8)
+            fn = lib().pado_gpu_index_export
+        _check(fn(self.ctx.h, self.h, _p(bp_d), _p(bp_s), _p(loff), _p(hub), _p(dist)))
         return bp_d.reshape(n, BP_SIZE), bp_s.reshape(n, BP_SIZE, 2), loff, hub[:tot], dist[:tot]
 
     @property
     def weighted(self):
         return bool(self.info.weighted)
+
+    @property
+    def wide(self):
+        """Labels in the wide hop domain (12-bit distances, u16 answers)."""
+        return self.info.dist_bits == 12
 
     def wexport(self):
         """Weighted index, canonical rank space: (loff[n+1], hub u32, dist u32)."""
@@ -536,8 +587,9 @@ class Index:
         self.info = self.get_info()
 
     def query(self, a, b, ref_compat=False, rank_space=False):
-        """Distances for pairs (a[i], b[i]): u8 for an unweighted index, u32
-        (WINF = unreachable) for a weighted one."""
+        """Distances for pairs (a[i], b[i]): u8 for an unweighted index, u16
+        (WIDE_INF = unreachable) for a wide-domain one, u32 (WINF =
+        unreachable) for a weighted one."""
         a = _c(a, np.uint32).reshape(-1)
         b = _c(b, np.uint32).reshape(-1)
         if a.size != b.size:
@@ -546,6 +598,9 @@ class Index:
         if self.weighted:
             out = np.zeros(max(a.size, 1), np.uint32)
             _check(lib().pado_gpu_wquery(self.ctx.h, self.h, _p(a), _p(b), a.size, _p(out), flags, None))
+        elif self.wide:
+            out = np.zeros(max(a.size, 1), np.uint16)
+            _check(lib().pado_gpu_query_wide(self.ctx.h, self.h, _p(a), _p(b), a.size, _p(out), flags, None))
         else:
             out = np.zeros(max(a.size, 1), np.uint8)
             _check(lib().pado_gpu_query(self.ctx.h, self.h, _p(a), _p(b), a.size, _p(out), flags, None))
@@ -554,9 +609,10 @@ class Index:
     def query_device(self, a_ptr, b_ptr, k, out_ptr, ref_compat=False, rank_space=False, stream=None):
         """Asynchronous query on device buffers (device pointers of the library's
         HIP runtime, e.g. hip.DeviceArray.ptr);
-        out holds u8 (unweighted) or u32 (weighted) distances."""
+        out holds u8 (unweighted), u16 (wide domain) or u32 (weighted) distances."""
         flags = QUERY_DEVICE_PTRS | (QUERY_REF_COMPAT if ref_compat else 0) | (QUERY_RANK_SPACE if rank_space else 0)
-        fn = lib().pado_gpu_wquery if self.weighted else lib().pado_gpu_query
+        fn = (lib().pado_gpu_wquery if self.weighted else
+              lib().pado_gpu_query_wide if self.wide else lib().pado_gpu_query)
         _check(fn(self.ctx.h, self.h, _vp(a_ptr), _vp(b_ptr), k, _vp(out_ptr), flags,
                   _vp(stream) if stream else None))
 
@@ -606,19 +662,39 @@ class MultiIndex:
         st = ctypes.c_int()
         nl = _u64()
         iu = ctypes.c_int()
-        _check(lib().pado_gpu_multi_graph_info(self.h, g, ctypes.byref(st), ctypes.byref(nl), ctypes.byref(iu)))
-        return {"status": st.value, "num_labels": nl.value, "in_union": bool(iu.value)}
+        db = _u32()
+        _check(lib().pado_gpu_multi_graph_info(self.h, g, ctypes.byref(st), ctypes.byref(nl), ctypes.byref(iu),
+                                               ctypes.byref(db)))
+        return {"status": st.value, "num_labels": nl.value, "in_union": bool(iu.value), "wide": db.value == 12}
 
     def export(self, g):
+        """Graph g's canonical rank-space arrays; dist is u8, or u16 for a
+        graph labeled in the wide hop domain."""
         n = int(self.sizes[g])
-        tot = self.graph_info(g)["num_labels"]
+        gi = self.graph_info(g)
+        tot = gi["num_labels"]
         bp_d = np.zeros(n * BP_SIZE, np.uint8)
         bp_s = np.zeros(n * BP_SIZE * 2, np.uint64)
         loff = np.zeros(n + 1, np.uint64)
         hub = np.zeros(max(tot, 1), np.uint32)
-        dist = np.zeros(max(tot, 1), np.uint8)
-        _check(lib().pado_gpu_multi_export(self.ctx.h, self.h, g, _p(bp_d), _p(bp_s), _p(loff), _p(hub), _p(dist)))
+        if gi["wide"]:
+            dist = np.zeros(max(tot, 1), np.uint16)
+            fn = lib().pado_gpu_multi_export_wide
+        else:
+            dist = np.zeros(max(tot, 1), np.uint8)
+            fn = lib().pado_gpu_multi_export
+        _check(fn(self.ctx.h, self.h, g, _p(bp_d), _p(bp_s), _p(loff), _p(hub), _p(dist)))
         return bp_d.reshape(n, BP_SIZE), bp_s.reshape(n, BP_SIZE, 2), loff, hub[:tot], dist[:tot]
+
+    def query(self, g, a, b):
+        """Distances on graph g between its rank-space vertices a[i], b[i] (u16:
+        the reference's u8 answers for a graph in its domain, exact hops with
+        WIDE_INF = unreachable for a wide-domain graph)."""
+        a = _c(a, np.uint32).reshape(-1)
+        b = _c(b, np.uint32).reshape(-1)
+        out = np.zeros(max(a.size, 1), np.uint16)
+        _check(lib().pado_gpu_multi_query(self.ctx.h, self.h, g, _p(a), _p(b), a.size, _p(out)))
+        return out[: a.size]
 
     def info(self):
         nu = _u32()
@@ -643,6 +719,10 @@ class Busy(PadoError):
     pass
 
 
+class Timeout(PadoError):
+    pass
+
+
 class Server:
     """Continuous-batching query service over an ordered, resident Index."""
 
@@ -656,7 +736,7 @@ class Server:
         cfg.max_requests = max_requests
         cfg.streams = streams
         cfg.flags = (QUERY_REF_COMPAT if ref_compat else 0) | (SERVE_PERSISTENT if persistent else 0)
-        self.dtype = np.uint32 if index.weighted else np.uint8
+        self.dtype = np.uint32 if index.weighted else (np.uint16 if index.wide else np.uint8)
         h = _vp()
         _check(lib().pado_gpu_serve_start(ctx.h, index.h, ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
@@ -678,13 +758,24 @@ class Server:
         _check(lib().pado_gpu_serve_poll(self.h, rid, ctypes.byref(d)))
         return bool(d.value)
 
-    def wait(self, rid, k):
-        """Blocks; returns (answers[k] -- u8 hops, or u32 for a weighted index --,
-        latency_ms from submit to completion)."""
+    def wait(self, rid, k, timeout_us=None):
+        """Blocks (at most timeout_us, then raises Timeout); returns
+        (answers[k] -- u8 hops, u16 for a wide-domain index, u32 for a
+        weighted one --, latency_ms from submit to completion)."""
         out = np.zeros(max(k, 1), self.dtype)
         lat = ctypes.c_double()
-        _check(lib().pado_gpu_serve_wait(self.h, rid, _p(out), ctypes.byref(lat)))
+        if timeout_us is None:
+            st = lib().pado_gpu_serve_wait(self.h, rid, _p(out), ctypes.byref(lat))
+        else:
+            st = lib().pado_gpu_serve_wait_for(self.h, rid, _p(out), ctypes.byref(lat), int(timeout_us))
+        if st == 8:
+            raise Timeout(st, lib().pado_gpu_last_error().decode())
+        _check(st)
         return out[:k], lat.value
+
+    def cancel(self, rid):
+        """Withdraws a request (its slot and ring space are released)."""
+        _check(lib().pado_gpu_serve_cancel(self.h, rid))
 
     def replay(self, a, b, req_off, arrival_s):
         """Native open-loop load generator: request i = pairs req_off[i]:req_off[i+1]

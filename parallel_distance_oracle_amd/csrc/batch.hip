@@ -114,13 +114,24 @@ __device__ inline uint32_t push_degree(const uint64_t *off, const uint32_t *adj,
 // of the distance lines (no memory traffic) settles the first case and
 // returns the slots at iter + 1 / iter + 2 (`cand`, with `one` marking
 // iter + 1) whose set pairs the second phase loads.
+// Bytes of x that are not 0xFF: bit 7 of each such byte set.
+__device__ inline uint32_t bytes_not_ff(uint32_t x)
+{
+    const uint32_t y = ~x;  // a byte of y is nonzero iff x's byte is not 0xFF
+    return (((y & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | y) & 0x80808080u;
+}
+
+template <bool W>
 __device__ inline bool bp_dist_scan(const uint32_t (&dr)[13], const uint32_t (&dt)[13], uint32_t iter,
                                     unsigned long long &cand, unsigned long long &one)
 {
     // Four slots per 32-bit word, as 16-bit lanes: lo = slots 4w and 4w+2,
     // hi = slots 4w+1 and 4w+3 (td <= 510).  (0x8000 | c) - td keeps bit 15
-    // of a lane set iff td <= c, and no borrow crosses lanes.  Bytes 50..63
-    // of the distance line are padding: slots 50 and 51 are masked off.
+    // of a lane set iff td <= c (c < 0x8000), and no borrow crosses lanes.
+    // Bytes 50..63 of the distance line are padding: slots 50 and 51 are
+    // masked off.  In the wide domain iter can pass 255, so a slot with INF
+    // (255) on either side is masked off explicitly; in the reference's domain
+    // such a slot takes part as the reference's arithmetic has it.
     const uint32_t c_it = 0x80008000u | (iter * 0x00010001u);
     const uint32_t c_it2 = 0x80008000u | ((iter + 2u) * 0x00010001u);
     const uint32_t c_1 = 0x80018001u;
@@ -129,12 +140,20 @@ __device__ inline bool bp_dist_scan(const uint32_t (&dr)[13], const uint32_t (&d
     one = 0;
 #pragma unroll
     for (int w = 0; w < 13; ++w) {
-        const uint32_t valid = w == 12 ? 0x00008000u : 0x80008000u;
+        uint32_t valid = w == 12 ? 0x00008000u : 0x80008000u;
+        uint32_t valid_hi = valid;
+        if (W) {
+            // Bit 7 / 23 (bytes 0 / 2) move to lane bits 15 / 31 for lo; bytes
+            // 1 / 3 already sit at bits 15 / 31 for hi.
+            const uint32_t ok = bytes_not_ff(dr[w]) & bytes_not_ff(dt[w]);
+            valid_hi = valid & ok;
+            valid &= (ok & 0x00800080u) << 8;
+        }
         const uint32_t lo = (dr[w] & 0x00FF00FFu) + (dt[w] & 0x00FF00FFu);
         const uint32_t hi = ((dr[w] >> 8) & 0x00FF00FFu) + ((dt[w] >> 8) & 0x00FF00FFu);
         const uint32_t lo_it = c_it - lo, hi_it = c_it - hi;
-        hit |= ((lo_it & ~(c_1 - lo)) | (hi_it & ~(c_1 - hi))) & valid;  // 2 <= td <= iter
-        const uint32_t lo_n = (c_it2 - lo) & ~lo_it & valid, hi_n = (c_it2 - hi) & ~hi_it & valid;
+        hit |= ((lo_it & ~(c_1 - lo)) & valid) | ((hi_it & ~(c_1 - hi)) & valid_hi);  // 2 <= td <= iter
+        const uint32_t lo_n = (c_it2 - lo) & ~lo_it & valid, hi_n = (c_it2 - hi) & ~hi_it & valid_hi;
         const uint32_t lo_1 = (c_it + 0x00010001u - lo) & ~lo_it, hi_1 = (c_it + 0x00010001u - hi) & ~hi_it;
         const uint32_t nib = ((lo_n >> 15) & 1u) | ((hi_n >> 14) & 2u) | ((lo_n >> 29) & 4u) | ((hi_n >> 28) & 8u);
         const uint32_t o = ((lo_1 >> 15) & 1u) | ((hi_1 >> 14) & 2u) | ((lo_1 >> 29) & 4u) | ((hi_1 >> 28) & 8u);
@@ -441,6 +460,7 @@ __global__ __launch_bounds__(256) void k_bp_masks(uint32_t n, const uint8_t *bpd
 // slot ro and touched-list slot ro (k_batch_begin sets both counts and the
 // frontier size), so the only shared counter left is the push-descriptor
 // count, updated once per block.
+template <bool W>
 __global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const uint64_t *off, const uint32_t *adj,
                                                      LabelStore S, unsigned long long *ind, unsigned long long *vmark,
                                                      uint32_t *tlist, unsigned long long *aoff, unsigned long long *ag,
@@ -472,7 +492,7 @@ __global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const 
         const unsigned long long base = store_reserve(S, C, r, len, 1);
         if (base != ~0ull) {
             if (lane == 0) {
-                S.pool[base + len] = r << 8;  // (hub r, dist 0), PARA:932-949
+                S.pool[base + len] = r << Hop<W>::kBits;  // (hub r, dist 0), PARA:932-949
                 S.rec[r].len = len + 1;
                 ind[(uint64_t)r * kWords + (ro >> 6)] |= 1ull << (ro & 63);
                 aoff[ro] = off[r];
@@ -507,9 +527,11 @@ __global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const 
 // With `fill` the root's labels are entered; without, the entries of the
 // previous batch's roots (the labels they held at its end, which include
 // every entry written during it) are reset.
+template <bool W>
 __global__ __launch_bounds__(256) void k_hub_table(uint32_t fill, const uint8_t *used, LabelStore S, uint32_t *hinfo,
-                                                   uint8_t *dc, const BuildCounters *C)
+                                                   typename Hop<W>::Dist *dc, const BuildCounters *C)
 {
+    constexpr uint32_t B = Hop<W>::kBits, INF = Hop<W>::kInf;
     if (C->hold) return;
     const uint32_t lane = lane_id();
     const uint32_t ro = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -531,22 +553,22 @@ __global__ __launch_bounds__(256) void k_hub_table(uint32_t fill, const uint8_t 
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 if (lab[u] == kEmpty) continue;
-                const uint32_t h = lab[u] >> 8;
-                dc[(uint64_t)h * kBatch + ro] = kInf8;
+                const uint32_t h = lab[u] >> B;
+                dc[(uint64_t)h * kBatch + ro] = (typename Hop<W>::Dist)~0u;
                 hinfo[h] = kEmpty;
             }
             continue;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) info[u] = lab[u] != kEmpty ? hinfo[lab[u] >> 8] : 0u;
+        for (int u = 0; u < 4; ++u) info[u] = lab[u] != kEmpty ? hinfo[lab[u] >> B] : 0u;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (lab[u] == kEmpty) continue;
-            const uint32_t h = lab[u] >> 8, d = lab[u] & 255u;
-            dc[(uint64_t)h * kBatch + ro] = (uint8_t)d;
+            const uint32_t h = lab[u] >> B, d = lab[u] & INF;
+            dc[(uint64_t)h * kBatch + ro] = (typename Hop<W>::Dist)d;
             // The minimum only falls during the kernel, so a value read as
             // <= d (even a stale one) makes the atomic a no-op.
-            if (d < (info[u] & 255u)) atomicMin(&hinfo[h], 0xFFFFFF00u | d);
+            if (d < (info[u] & INF)) atomicMin(&hinfo[h], ~INF | d);
         }
     }
 }
@@ -560,6 +582,7 @@ constexpr uint32_t kPushWaves = 4;  // waves per k_push block
 // 64 lanes are then checked against the bit-parallel labels 64 at a time,
 // one pair per lane, so a tail with many fresh roots does not hold up the
 // wave.
+template <bool W>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_push(uint32_t iter, const uint32_t *adj,
                                               const unsigned long long *aoff, const unsigned long long *ag,
                                               const uint32_t *amask, const uint32_t *aend,
@@ -568,7 +591,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                               uint32_t *clist,
                                               const uint8_t *bpd, const unsigned long long *lmask,
                                               const unsigned long long *rmask, const ulonglong2 *bps,
-                                              const ulonglong2 *rset, BuildCounters *C)
+                                              const ulonglong2 *rset, BuildCounters *C, uint32_t bp_prune)
 {
     __shared__ unsigned long long s_nb[kPushWaves][4][64]; // fresh roots of each lane's edge, per word of the step
     __shared__ unsigned long long s_keep[kPushWaves][4][64];  // of those, the ones the BP check keeps
@@ -593,7 +616,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     if (C->hold) return;
     const uint32_t p = iter & 1u;
     const uint32_t n_act0 = C->n_act[p];
-    const bool live = n_act0 != 0 && iter < kInf8;
+    const bool live = n_act0 != 0 && iter < Hop<W>::kInf;
     const uint32_t nc = live ? C->n_chunk[p] : 0u;
     const uint32_t rs = C->rs, gen_batch = C->gen_batch;
     const unsigned long long gen_iter = ((unsigned long long)gen_batch << 16) | iter;
@@ -603,7 +626,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
         C->n_chunk[p ^ 1u] = 0;
         C->a_top[p ^ 1u] = 0;
         if (n_act0 != 0) {
-            if (iter >= kInf8) {
+            if (iter >= Hop<W>::kInf) {
                 C->domain = 1;
             } else {
                 C->iterations += 1;
@@ -718,7 +741,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                                  a2.x, a2.y, a2.z, a2.w, a3.x};
                         const uint32_t dt[13] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w,
                                                  q2.x, q2.y, q2.z, q2.w, q3.x};
-                        dist_pr = bp_dist_scan(dr, dt, iter, cs, one);
+                        dist_pr = bp_dist_scan<W>(dr, dt, iter, cs, one);
+                    }
+                    if (!bp_prune) {
+                        // Labels-only pruning (a wide build whose bit-parallel
+                        // BFS was cut at level 254: a slot that stops short
+                        // cannot vouch for the vertices behind a pruned one).
+                        dist_pr = false;
+                        cs = 0ull;
                     }
                 }
                 const bool set_pr = bp_set_phase(dist_pr ? 0ull : cs, one, rset, w * 64 + b, rs, bps, to, sc);
@@ -813,8 +843,10 @@ __global__ __launch_bounds__(1024) void k_batch_begin(const uint8_t *used, Build
 // dist < iter: a hub h can prune only if d_v(h) + min_r d_r(h) <= iter, so a
 // per-hub minimum filters most hubs before the 1 KB root-distance row of the
 // survivors is read (16 B per lane).
+template <bool W>
 __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used, const uint64_t *off, const uint32_t *adj, const uint32_t *clist,
-                                             unsigned long long *cand, LabelStore S, uint32_t *hinfo, uint8_t *dc,
+                                             unsigned long long *cand, LabelStore S, uint32_t *hinfo,
+                                             typename Hop<W>::Dist *dc,
                                              unsigned long long *aoff, unsigned long long *ag, uint32_t *amask,
                                              uint32_t *aend, unsigned long long *chunks, BuildCounters *C)
 {
@@ -826,7 +858,8 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
     // the next iteration's push.
     if (C->hold) return;
     const uint32_t p = iter & 1u;
-    const uint32_t ncand = iter < kInf8 ? C->n_cand[p] : 0u;  // 0 past the batch's end
+    constexpr uint32_t B = Hop<W>::kBits, INF = Hop<W>::kInf;
+    const uint32_t ncand = iter < INF ? C->n_cand[p] : 0u;  // 0 past the batch's end
     const uint32_t rs = C->rs, rsz = C->rsz;
     if (blockIdx.x == 0 && threadIdx.x == 0) C->n_cand[p ^ 1u] = 0;
     unsigned long long n_dq = 0, n_lab = 0, n_rows = 0, n_read = 0;
@@ -861,21 +894,21 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
         const uint32_t my_lo = rs + 16 * lane;  // real id of this lane's first root
         for (uint32_t j0 = 0; j0 < len; j0 += 64) {
             const uint32_t j = j0 + lane;
-            uint32_t h = 0, d = 255, info = kEmpty;
+            uint32_t h = 0, d = INF, info = kEmpty;
             if (j < len) {
                 const uint32_t lab = j0 == 0 ? lab0 : S.pool[base + j];
-                h = lab >> 8;
-                d = lab & 255u;
+                h = lab >> B;
+                d = lab & INF;
                 if (d < iter) {
                     info = hinfo[h];
                     ++n_read;
                 }
             }
-            const bool pass = d + (info & 255u) <= iter;
+            const bool pass = d + (info & INF) <= iter;
             unsigned long long m = __ballot(pass);
             // Up to four passing hubs per step: their row loads are issued
             // together, and only lanes that still hold an unpruned candidate
-            // load their 16-byte slice of a row.
+            // load their slice of a row (16 entries: 16 bytes, 32 wide).
             while (m) {
                 int src[4];
                 bool ok[4];
@@ -886,26 +919,40 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
                     if (ok[u]) m &= m - 1;
                 }
                 const bool need = (cb & ~prune) != 0u;
+                constexpr uint32_t RV = W ? 2 : 1;  // 16-byte vectors of a lane's slice
                 uint32_t hh[4], thr[4];
-                uint4 row[4];
+                uint4 row[4][RV];
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
                     hh[u] = __shfl(h, src[u]);
                     // prune root r if d_r(h) <= iter - d_v(h); an unused slot u
                     // (fewer than four hubs left) gets threshold 0, which the
-                    // all-255 row never meets
+                    // all-INF row never meets
                     const uint32_t du = __shfl(d, src[u]);
                     thr[u] = ok[u] ? iter - du : 0u;
-                    row[u] = make_uint4(~0u, ~0u, ~0u, ~0u);
-                    if (ok[u] && need) row[u] = reinterpret_cast<const uint4 *>(dc + (uint64_t)hh[u] * kBatch)[lane];
+#pragma unroll
+                    for (uint32_t q = 0; q < RV; ++q) row[u][q] = make_uint4(~0u, ~0u, ~0u, ~0u);
+                    if (ok[u] && need) {
+                        const uint4 *rp = reinterpret_cast<const uint4 *>(dc + (uint64_t)hh[u] * kBatch) + RV * lane;
+#pragma unroll
+                        for (uint32_t q = 0; q < RV; ++q) row[u][q] = rp[q];
+                    }
                     n_rows += ok[u] ? 1 : 0;
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const uint32_t wds[4] = {row[u].x, row[u].y, row[u].z, row[u].w};
+                    uint32_t wds[4 * RV];
+#pragma unroll
+                    for (uint32_t q = 0; q < RV; ++q) {
+                        wds[4 * q] = row[u][q].x;
+                        wds[4 * q + 1] = row[u][q].y;
+                        wds[4 * q + 2] = row[u][q].z;
+                        wds[4 * q + 3] = row[u][q].w;
+                    }
 #pragma unroll
                     for (int q = 0; q < 16; ++q) {
-                        const uint32_t x = (wds[q >> 2] >> ((q & 3) * 8)) & 255u;
+                        const uint32_t x = W ? (wds[q >> 1] >> ((q & 1) * 16)) & 0xFFFFu
+                                             : (wds[q >> 2] >> ((q & 3) * 8)) & 255u;
                         if (x <= thr[u] && hh[u] < my_lo + q) prune |= 1u << q;
                     }
                 }
@@ -929,7 +976,7 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
         uint32_t pos = len + incl - cnt;
         for (uint32_t sb = surv; sb; sb &= sb - 1) {
             const uint32_t q = __ffs(sb) - 1;
-            S.pool[nb + pos++] = ((my_lo + q) << 8) | iter;  // insert_label_only, PARA:1285
+            S.pool[nb + pos++] = ((my_lo + q) << B) | iter;  // insert_label_only, PARA:1285
         }
         if (lane == 0) S.rec[v].len = len + total;
         // Roots of this batch also record the new label in the root table
@@ -937,8 +984,8 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
         if (v >= rs && v < rs + rsz) {
             for (uint32_t sb = surv; sb; sb &= sb - 1) {
                 const uint32_t h = my_lo + __ffs(sb) - 1;
-                dc[(uint64_t)h * kBatch + (v - rs)] = (uint8_t)iter;
-                if (iter < (hinfo[h] & 255u)) atomicMin(&hinfo[h], 0xFFFFFF00u | iter);
+                dc[(uint64_t)h * kBatch + (v - rs)] = (typename Hop<W>::Dist)iter;
+                if (iter < (hinfo[h] & INF)) atomicMin(&hinfo[h], ~INF | iter);
             }
         }
         // v becomes active with the new distance group as its last group.
@@ -1085,9 +1132,12 @@ static pado_gpu_status pool_repack(hipStream_t st, uint32_t n, DevBuf<uint32_t> 
     return PADO_GPU_OK;
 }
 
-pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevBuf<uint8_t> &d_used,
-                              const pado_gpu_build_cfg &cfg)
+template <bool W>
+static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, const DevBuf<uint8_t> &d_used,
+                                       const pado_gpu_build_cfg &cfg)
 {
+    using DT = typename Hop<W>::Dist;
+    constexpr uint32_t kIterEnd = Hop<W>::kInf;  // the first iteration that is never run
     hipStream_t st = g->ctx->stream;
     const uint32_t n = g->n;
     const uint32_t batch = cfg.batch_size ? cfg.batch_size : kBatch;
@@ -1095,13 +1145,20 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
         set_error("batch_size must be in [1, 1024]");
         return PADO_GPU_ERR_ARG;
     }
+    if (n > Hop<W>::kMaxN) {
+        set_error("graph too large for the wide hop domain (at most 2^20 vertices)");
+        return PADO_GPU_ERR_DOMAIN;
+    }
+    const std::string domain_msg =
+        W ? "batch iteration reached 4095: distances outside the wide (12-bit) hop domain"
+          : "batch iteration reached 255: distances outside the uint8 domain of the reference";
 
     DevBuf<unsigned long long> ind, cand, ag, chunks;
     DevBuf<uint32_t> tlist, clist, amask, aend, hinfo, pool;
     DevBuf<unsigned long long> aoff;  // per active slot: the vertex's first edge
     DevBuf<LRec> lrec;
     DevBuf<unsigned long long> vmark;  // ((batch << 16 | iteration) << 1 | candidate) of the last mark
-    DevBuf<uint8_t> dc;
+    DevBuf<DT> dc;
     DevBuf<ulonglong2> rset;  // batch roots' bit-parallel sets, slot-major [kBP][kBatch]
     DevBuf<unsigned long long> lmask;  // bit-parallel level masks, 8 words per vertex
     DevBuf<unsigned long long> rmask;  // batch roots' level masks, level-major [8][kBatch]
@@ -1140,7 +1197,7 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
 
     const uint32_t fg = 4096;
     hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, reinterpret_cast<uint32_t *>(dc.p),
-                       (uint64_t)dc_rows * kBatch / 4, ~0u);
+                       (uint64_t)dc_rows * kBatch * sizeof(DT) / 4, ~0u);
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, ind.p, (uint64_t)n * kWords);
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, cand.p, (uint64_t)n * kWords);
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, vmark.p, (uint64_t)n);
@@ -1184,25 +1241,26 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     auto enqueue_prologue = [&]() {
         // Previous batch's reset_at_end first (a no-op before the first batch).
         hipLaunchKernelGGL(k_batch_gate, dim3(1), dim3(1), 0, st, ctr.p);
-        hipLaunchKernelGGL(k_hub_table, dim3(root_blocks), dim3(256), 0, st, 0u, d_used.p, S, hinfo.p, dc.p, ctr.p);
+        hipLaunchKernelGGL(k_hub_table<W>, dim3(root_blocks), dim3(256), 0, st, 0u, d_used.p, S, hinfo.p, dc.p, ctr.p);
         hipLaunchKernelGGL(k_batch_end, dim3(1024), dim3(256), 0, st, ind.p, tlist.p, ctr.p);
         hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(batch), 0, st, d_used.p, ctr.p, batch, n);
-        hipLaunchKernelGGL(k_batch_init, dim3((batch + 15) / 16), dim3(1024), 0, st, d_used.p, g->off.p, g->adj.p, S, ind.p,
+        hipLaunchKernelGGL(k_batch_init<W>, dim3((batch + 15) / 16), dim3(1024), 0, st, d_used.p, g->off.p, g->adj.p, S, ind.p,
                            vmark.p, tlist.p, aoff.p, ag.p, amask.p, aend.p, chunks.p,
                            (const ulonglong2 *)idx->bps.p, rset.p, lmask.p, rmask.p, ctr.p);
-        hipLaunchKernelGGL(k_hub_table, dim3(root_blocks), dim3(256), 0, st, 1u, d_used.p, S, hinfo.p, dc.p, ctr.p);
+        hipLaunchKernelGGL(k_hub_table<W>, dim3(root_blocks), dim3(256), 0, st, 1u, d_used.p, S, hinfo.p, dc.p, ctr.p);
     };
     // Iterations first + 1 .. first + count of the current batch.
     auto enqueue_iterations = [&](uint32_t first, uint32_t count, bool timed) -> pado_gpu_status {
         for (uint32_t u = 0; u < count; ++u) {
             const uint32_t iter = first + 1 + u;
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u], st));
-            hipLaunchKernelGGL(k_push, dim3(push_grid), dim3(256), 0, st, iter, g->adj.p, aoff.p, ag.p,
+            hipLaunchKernelGGL(k_push<W>, dim3(push_grid), dim3(256), 0, st, iter, g->adj.p, aoff.p, ag.p,
                                amask.p, aend.p, chunks.p, ind.p, cand.p, vmark.p, tlist.p, clist.p,
-                               idx->bpd.p, lmask.p, rmask.p, (const ulonglong2 *)idx->bps.p, rset.p, ctr.p);
+                               idx->bpd.p, lmask.p, rmask.p, (const ulonglong2 *)idx->bps.p, rset.p, ctr.p,
+                               idx->bp_pass_through ? 0u : 1u);
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 1], st));
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 2], st));
-            hipLaunchKernelGGL(k_add, dim3(add_grid), dim3(256), 0, st, iter, d_used.p, g->off.p, g->adj.p, clist.p,
+            hipLaunchKernelGGL(k_add<W>, dim3(add_grid), dim3(256), 0, st, iter, d_used.p, g->off.p, g->adj.p, clist.p,
                                cand.p, S, hinfo.p, dc.p, aoff.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 3], st));
         }
@@ -1303,7 +1361,7 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
             dc_rows = std::min<uint32_t>(n, std::max<uint32_t>(batch_end, dc_rows * 2));
             PADO_TRY(dc.alloc((uint64_t)dc_rows * kBatch));
             hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, reinterpret_cast<uint32_t *>(dc.p),
-                               (uint64_t)dc_rows * kBatch / 4, ~0u);
+                               (uint64_t)dc_rows * kBatch * sizeof(DT) / 4, ~0u);
             drop_graphs();
         }
         const uint64_t batch_first_iter = iterations_seen;
@@ -1316,9 +1374,9 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
             cur_slot = ahead_slot;
             ahead = false;
         } else {
-            // Iteration 255 is the last one ever launched: k_push flags it if
-            // the frontier is still live there.
-            launched = cur_count = std::min<uint32_t>(first_block(prev_batch_iters), (uint32_t)kInf8);
+            // Iteration kIterEnd is the last one ever launched: k_push flags
+            // it if the frontier is still live there.
+            launched = cur_count = std::min<uint32_t>(first_block(prev_batch_iters), kIterEnd);
             PADO_TRY(launch_block(0, launched, true));
             PADO_TRY(snapshot(&cur_slot));
         }
@@ -1326,7 +1384,7 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
         bool spec = use_graph && b + 1 < nb && next_end <= dc_rows && growth != ~0ull &&
                     (top_prev + 4 * growth) * 5 < pool_size * 2;
         if (spec) {
-            ahead_count = std::min<uint32_t>(first_block(prev_batch_iters), (uint32_t)kInf8);
+            ahead_count = std::min<uint32_t>(first_block(prev_batch_iters), kIterEnd);
             PADO_TRY(launch_block(0, ahead_count, false));
             PADO_TRY(snapshot(&ahead_slot));
         }
@@ -1353,7 +1411,7 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
                 return PADO_GPU_ERR_NOMEM;
             }
             if (snap.domain) {
-                set_error("batch iteration reached 255: distances outside the uint8 domain of the reference");
+                set_error(domain_msg);
                 return PADO_GPU_ERR_DOMAIN;
             }
             // Keep at least 60% of the pool free for the next block of
@@ -1374,8 +1432,8 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
             }
             // The frontier of the next iteration is in slot (launched + 1) & 1.
             if (snap.n_act[(launched + 1) & 1u] == 0) break;
-            if (launched >= kInf8) {
-                set_error("batch iteration reached 255: distances outside the uint8 domain of the reference");
+            if (launched >= kIterEnd) {
+                set_error(domain_msg);
                 return PADO_GPU_ERR_DOMAIN;
             }
             // The batch continues: a block enqueued ahead was gated off.
@@ -1383,7 +1441,7 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
                 hipLaunchKernelGGL(k_unhold, dim3(1), dim3(1), 0, st, ctr.p);
                 spec = false;
             }
-            cur_count = std::min<uint32_t>(next_count, (uint32_t)kInf8 - launched);
+            cur_count = std::min<uint32_t>(next_count, kIterEnd - launched);
             next_count = std::min<uint32_t>(2 * next_count, U);  // the batch runs long: double the block
             PADO_TRY(launch_block(launched, cur_count, true));
             PADO_TRY(snapshot(&cur_slot));
@@ -1425,8 +1483,14 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     PADO_TRY(idx->lab.alloc(total));
     hipLaunchKernelGGL(k_gather_labels, dim3(grid_for((uint64_t)n * 64, 256, 8192)), dim3(256), 0, st, n, pool.p,
                        lrec.p, idx->loff.p, idx->lab.p);
-    PADO_TRY(segmented_sort_u32(idx->lab.p, total, idx->loff.p, n, st));
+    PADO_TRY(segmented_sort_u32(idx->lab.p, total, idx->loff.p, n, st, cfg.sort_max_items));
     return PADO_GPU_OK;
+}
+
+pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevBuf<uint8_t> &d_used,
+                              const pado_gpu_build_cfg &cfg)
+{
+    return idx->wide ? build_batches_t<true>(g, idx, d_used, cfg) : build_batches_t<false>(g, idx, d_used, cfg);
 }
 
 }  // namespace pado

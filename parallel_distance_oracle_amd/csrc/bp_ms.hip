@@ -272,6 +272,11 @@ pado_gpu_status build_bp_seeded(pado_gpu_graph *g, pado_gpu_index *idx, DevBuf<u
         if (next_count == 0) break;
         if (d + 1 > max_level) max_level = d + 1;
         if (d + 1 >= kInf8) {
+            // Wide hop domain: the sweep stops after level 254 (whose sibling
+            // sets are complete); vertices farther away keep INF (255) in
+            // that slot -- level 255 was written as 255 -- and the labeling
+            // and query kernels ignore such slots.
+            if (idx->wide) break;
             (void)hipEventDestroy(e0);
             (void)hipEventDestroy(e1);
             set_error("bit-parallel BFS reached level 255: graph eccentricity is outside the uint8 "

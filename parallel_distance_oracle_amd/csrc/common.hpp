@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "pado_gpu.h"
@@ -21,6 +22,25 @@ constexpr uint32_t kHubBits = 24;     // packed label = (hub << 8) | dist
 constexpr uint32_t kMaxVertices = 1u << kHubBits;
 constexpr uint32_t kEmpty = 0xFFFFFFFFu;
 constexpr uint32_t kChunk = 16;       // edges per push work descriptor
+
+// Distance domains of the labeling and query kernels (one templated path):
+//  * Hop<false> -- the reference's: u8 hop distances (smalli, GLB:29-40),
+//    255 = INF, packed label (hub << 8) | dist, hubs below 2^24, iterations
+//    below 255 (the reference's u8 counters wrap there);
+//  * Hop<true>  -- wide: 12-bit hop distances in (hub << 12) | dist, 4095 =
+//    INF, hubs below 2^20, for graphs whose BFS depth reaches 255 (large
+//    grids, road-like graphs).  Bit-parallel distances stay u8 with the BFS
+//    stopped at level 254 (farther vertices keep 255: no information in that
+//    slot), and the bit-parallel checks skip slots with 255 on either side.
+template <bool W>
+struct Hop {
+    static constexpr uint32_t kBits = W ? 12u : 8u;        // distance bits of a packed label
+    static constexpr uint32_t kInf = (1u << kBits) - 1u;   // INF distance; also the iteration limit
+    static constexpr uint32_t kMaxN = 1u << (32u - kBits); // vertex ids must stay below this
+    using Dist = typename std::conditional<W, uint16_t, uint8_t>::type;  // root-distance table entry
+    using Out = typename std::conditional<W, uint16_t, uint8_t>::type;   // query answer
+};
+constexpr uint32_t kWideMaxVertices = Hop<true>::kMaxN;
 // Default grids of the labeling kernels (256-thread blocks): one resident
 // round of waves at their occupancy, so a launch with little work ends fast.
 constexpr uint32_t kPushGrid = 1024;
@@ -161,6 +181,9 @@ struct pado_gpu_index {
     pado_gpu_ctx *ctx = nullptr;
     uint32_t n = 0;
     uint32_t weighted = 0;
+    uint32_t wide = 0;  // labels in the wide hop domain (Hop<true>)
+    uint32_t bp_pass_through = 0;  // wide build whose bit-parallel BFS was cut at level 254:
+                                   // its bit-parallel vertices were labeled as ordinary hubs
     // Bit-parallel labels, rank space: bpd[v*64 + i], bps[(v*50 + i)*2 + {0,1}].
     pado::DevBuf<uint8_t> bpd;
     pado::DevBuf<uint64_t> bps;

@@ -37,9 +37,12 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
 
 // Exclusive scan of n u32 values into n+1 u64 offsets (out[n] = total).
 pado_gpu_status scan_u32_to_u64(const uint32_t *in, uint64_t *out, uint32_t n, hipStream_t s);
-// Sort every CSR segment of u32 keys ascending in place.
+// Sort every CSR segment of u32 keys ascending in place, in vertex ranges of
+// at most max_items keys per hipCUB call (0 = kSortMaxItems; hipCUB counts
+// items in int).
+constexpr uint64_t kSortMaxItems = 0x7FFFFFFFull - 1024;
 pado_gpu_status segmented_sort_u32(uint32_t *keys, uint64_t total, const uint64_t *off, uint32_t n,
-                                   hipStream_t s);
+                                   hipStream_t s, uint64_t max_items = 0);
 
 // Launch geometry helpers.
 inline uint32_t grid_for(uint64_t threads, uint32_t block, uint32_t cap = 1u << 20)

@@ -302,24 +302,29 @@ pado_gpu_status pado_gpu_build_multi(pado_gpu_ctx *ctx, uint32_t num_graphs, con
         delete mi;
         return s;
     }
-    // Graphs outside the proven range: on their own (exact domain detection).
-    // Vertex 0 is the first bit-parallel root, and its BFS runs to its
-    // eccentricity, so an eccentricity >= 255 is the domain error that
-    // build would stop with, known without building.
+    // Graphs outside the proven range: on their own.  Vertex 0 is the first
+    // bit-parallel root and its BFS runs to its eccentricity, so an
+    // eccentricity >= 255 means the reference's u8 domain is left: such a
+    // graph is built in the wide hop domain (12-bit distances) at once.  The
+    // others try the u8 domain first and fall back to the wide one if a batch
+    // iteration reaches 255.  Only a graph the wide domain cannot hold (more
+    // than 2^20 vertices or distances past 4094) reports ERR_DOMAIN.
+    pado_gpu_build_cfg base{};
+    if (cfg) base = *cfg;
+    pado_gpu_build_cfg wide_cfg = base;
+    wide_cfg.flags |= PADO_GPU_BUILD_WIDE;
     for (uint32_t g = 0; g < num_graphs; ++g) {
         GraphSlot &G = mi->graphs[g];
         if (G.union_id >= 0) continue;
-        if (eccentricity(G.n, off + G.voff, adj + G.eoff, 0) >= kInf8) {
-            G.status = PADO_GPU_ERR_DOMAIN;
-            continue;
-        }
+        const bool deep = eccentricity(G.n, off + G.voff, adj + G.eoff, 0) >= kInf8;
         pado_gpu_graph *gg = nullptr;
         s = pado_gpu_graph_upload(ctx, G.n, off[G.voff + G.n], off + G.voff, adj + G.eoff, nullptr, &gg);
         if (s != PADO_GPU_OK) {
             delete mi;
             return s;
         }
-        G.status = pado_gpu_build(ctx, gg, cfg, &G.alone);
+        G.status = pado_gpu_build(ctx, gg, deep ? &wide_cfg : &base, &G.alone);
+        if (G.status == PADO_GPU_ERR_DOMAIN && !deep) G.status = pado_gpu_build(ctx, gg, &wide_cfg, &G.alone);
         pado_gpu_graph_free(gg);
         if (G.status != PADO_GPU_OK && G.status != PADO_GPU_ERR_DOMAIN) {
             s = G.status;
@@ -333,7 +338,7 @@ pado_gpu_status pado_gpu_build_multi(pado_gpu_ctx *ctx, uint32_t num_graphs, con
 }
 
 pado_gpu_status pado_gpu_multi_graph_info(const pado_gpu_multi *mi, uint32_t g, pado_gpu_status *status,
-                                          uint64_t *num_labels, int *in_union)
+                                          uint64_t *num_labels, int *in_union, uint32_t *dist_bits)
 {
     if (!mi || g >= mi->graphs.size()) {
         set_error("bad graph index");
@@ -342,6 +347,7 @@ pado_gpu_status pado_gpu_multi_graph_info(const pado_gpu_multi *mi, uint32_t g, 
     const GraphSlot &G = mi->graphs[g];
     if (status) *status = G.status;
     if (in_union) *in_union = G.union_id >= 0;
+    if (dist_bits) *dist_bits = (G.alone && G.alone->wide) ? Hop<true>::kBits : Hop<false>::kBits;
     if (num_labels) {
         if (G.union_id < 0) {
             *num_labels = G.alone ? G.alone->num_labels : 0;
@@ -356,10 +362,14 @@ pado_gpu_status pado_gpu_multi_graph_info(const pado_gpu_multi *mi, uint32_t g, 
     return PADO_GPU_OK;
 }
 
-pado_gpu_status pado_gpu_multi_export(pado_gpu_ctx *ctx, pado_gpu_multi *mi, uint32_t g, uint8_t *bp_dist,
-                                      uint64_t *bp_sets, uint64_t *label_off, uint32_t *hubs, uint8_t *dists)
+}  // extern "C"
+
+namespace {
+
+// Export of graph g with u8 (d8) or u16 (d16) label distances.
+pado_gpu_status multi_export(pado_gpu_ctx *ctx, pado_gpu_multi *mi, uint32_t g, uint8_t *bp_dist, uint64_t *bp_sets,
+                             uint64_t *label_off, uint32_t *hubs, uint8_t *d8, uint16_t *d16)
 {
-    clear_error();
     if (!ctx || !mi || g >= mi->graphs.size()) {
         set_error("bad argument");
         return PADO_GPU_ERR_ARG;
@@ -367,10 +377,11 @@ pado_gpu_status pado_gpu_multi_export(pado_gpu_ctx *ctx, pado_gpu_multi *mi, uin
     GraphSlot &G = mi->graphs[g];
     if (G.union_id < 0) {
         if (G.status != PADO_GPU_OK || !G.alone) {
-            set_error("graph " + std::to_string(g) + " has no index (outside the u8 distance domain)");
+            set_error("graph " + std::to_string(g) + " has no index (outside every hop domain)");
             return PADO_GPU_ERR_DOMAIN;
         }
-        return pado_gpu_index_export(ctx, G.alone, bp_dist, bp_sets, label_off, hubs, dists);
+        if (d16) return pado_gpu_index_export_wide(ctx, G.alone, bp_dist, bp_sets, label_off, hubs, d16);
+        return pado_gpu_index_export(ctx, G.alone, bp_dist, bp_sets, label_off, hubs, d8);
     }
     Union &U = mi->unions[G.union_id];
     if (!U.exported) {
@@ -392,10 +403,61 @@ pado_gpu_status pado_gpu_multi_export(pado_gpu_ctx *ctx, pado_gpu_multi *mi, uin
         if (bp_sets) std::memcpy(bp_sets + (uint64_t)v * kBP * 2, &U.bps[(uint64_t)x * kBP * 2], kBP * 16);
         for (uint64_t i = U.loff[x]; i < U.loff[x + 1]; ++i, ++o) {
             if (hubs) hubs[o] = U.local[U.hub[i]];
-            if (dists) dists[o] = U.dist[i];
+            if (d8) d8[o] = U.dist[i];
+            if (d16) d16[o] = U.dist[i];
         }
         if (label_off) label_off[v + 1] = o;
     }
+    return PADO_GPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pado_gpu_status pado_gpu_multi_export(pado_gpu_ctx *ctx, pado_gpu_multi *mi, uint32_t g, uint8_t *bp_dist,
+                                      uint64_t *bp_sets, uint64_t *label_off, uint32_t *hubs, uint8_t *dists)
+{
+    clear_error();
+    return multi_export(ctx, mi, g, bp_dist, bp_sets, label_off, hubs, dists, nullptr);
+}
+
+pado_gpu_status pado_gpu_multi_export_wide(pado_gpu_ctx *ctx, pado_gpu_multi *mi, uint32_t g, uint8_t *bp_dist,
+                                           uint64_t *bp_sets, uint64_t *label_off, uint32_t *hubs, uint16_t *dists)
+{
+    clear_error();
+    return multi_export(ctx, mi, g, bp_dist, bp_sets, label_off, hubs, nullptr, dists);
+}
+
+pado_gpu_status pado_gpu_multi_query(pado_gpu_ctx *ctx, pado_gpu_multi *mi, uint32_t g, const uint32_t *a,
+                                     const uint32_t *b, uint64_t k, uint16_t *out)
+{
+    clear_error();
+    if (!ctx || !mi || g >= mi->graphs.size() || (k && (!a || !b || !out))) {
+        set_error("bad argument");
+        return PADO_GPU_ERR_ARG;
+    }
+    if (k == 0) return PADO_GPU_OK;
+    GraphSlot &G = mi->graphs[g];
+    if (G.union_id < 0) {
+        if (G.status != PADO_GPU_OK || !G.alone) {
+            set_error("graph " + std::to_string(g) + " has no index (outside every hop domain)");
+            return PADO_GPU_ERR_DOMAIN;
+        }
+        return pado_gpu_query_wide(ctx, G.alone, a, b, k, out, PADO_GPU_QUERY_RANK_SPACE, nullptr);
+    }
+    // A union graph: local ids -> union ids; pairs with an id outside the
+    // graph are answered here (a == b ? 0 : 255, as the query kernel does).
+    std::vector<uint32_t> ua(k), ub(k);
+    for (uint64_t i = 0; i < k; ++i) {
+        const bool in = a[i] < G.n && b[i] < G.n;
+        ua[i] = in ? G.gid[a[i]] : 0u;
+        ub[i] = in ? G.gid[b[i]] : 0u;
+    }
+    PADO_TRY(pado_gpu_query_wide(ctx, mi->unions[G.union_id].idx, ua.data(), ub.data(), k, out,
+                                 PADO_GPU_QUERY_RANK_SPACE, nullptr));
+    for (uint64_t i = 0; i < k; ++i)
+        if (a[i] >= G.n || b[i] >= G.n) out[i] = a[i] == b[i] ? 0 : 255;
     return PADO_GPU_OK;
 }
 

@@ -18,6 +18,8 @@ pado_gpu_status launch_query(const pado_gpu_index *idx, bool rank_space, const u
                              uint64_t k, uint8_t *out, bool ref_compat, hipStream_t st);
 pado_gpu_status launch_wquery(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,
                               uint64_t k, uint32_t *out, bool ref_compat, hipStream_t st);
+pado_gpu_status launch_query_wide(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,
+                                  uint64_t k, uint16_t *out, bool ref_compat, hipStream_t st);
 
 namespace {
 
@@ -284,6 +286,12 @@ pado_gpu_status pado_gpu_build(pado_gpu_ctx *ctx, const pado_gpu_graph *graph, c
     auto *idx = new pado_gpu_index();
     idx->ctx = ctx;
     idx->n = graph->n;
+    idx->wide = (c.flags & PADO_GPU_BUILD_WIDE) && !graph->weighted ? 1u : 0u;
+    if (idx->wide && graph->n > kWideMaxVertices) {
+        delete idx;
+        set_error("graph too large for the wide hop domain (at most 2^20 vertices)");
+        return PADO_GPU_ERR_DOMAIN;
+    }
     pado_gpu_graph *g = const_cast<pado_gpu_graph *>(graph);
     pado_gpu_status s = PADO_GPU_OK;
     if (g->weighted) {
@@ -296,9 +304,20 @@ pado_gpu_status pado_gpu_build(pado_gpu_ctx *ctx, const pado_gpu_graph *graph, c
         DevBuf<uint8_t> used;
         std::vector<uint8_t> h_used;
         Timer t0;
-        s = (c.flags & PADO_GPU_BUILD_BP_PER_SLOT) ? build_bp(g, idx, used, h_used)
-                                                   : build_bp_multi(g, idx, used, h_used);
+        // The wide domain always uses the multi-source sweep (it stops at level 254).
+        s = (c.flags & PADO_GPU_BUILD_BP_PER_SLOT) && !idx->wide ? build_bp(g, idx, used, h_used)
+                                                                 : build_bp_multi(g, idx, used, h_used);
         idx->ms_bp = t0.ms();
+        if (s == PADO_GPU_OK && idx->wide && idx->bp_max_level >= kInf8) {
+            // The reference never pushes from a bit-parallel root or selected
+            // neighbour (PARA:1117-1119): every path through one is covered
+            // by its slot.  With the slot BFS stopped at level 254 that no
+            // longer holds, so here those vertices take part in the batch
+            // phase like any other (they become hubs and push); the slots
+            // still prune wherever both ends were reached.
+            if (hipMemsetAsync(used.p, 0, graph->n, ctx->stream) != hipSuccess) s = PADO_GPU_ERR_HIP;
+            idx->bp_pass_through = 1;
+        }
         if (s == PADO_GPU_OK) {
             Timer t1;
             s = build_batches(g, idx, used, c);
@@ -347,13 +366,19 @@ pado_gpu_status pado_gpu_index_get_info(const pado_gpu_index *idx, pado_gpu_inde
     info->n_push = idx->ktimes.n_push;
     info->n_add = idx->ktimes.n_add;
     info->n_bp_levels = idx->ktimes.n_bp;
+    info->dist_bits = idx->wide ? Hop<true>::kBits : Hop<false>::kBits;
     return PADO_GPU_OK;
 }
 
-pado_gpu_status pado_gpu_index_export(pado_gpu_ctx *ctx, const pado_gpu_index *idx, uint8_t *bp_dist,
-                                      uint64_t *bp_sets, uint64_t *label_off, uint32_t *hubs, uint8_t *dists)
+}  // extern "C"
+
+namespace {
+
+// Canonical rank-space export; exactly one of d8 / d16 (or neither) receives
+// the label distances.
+pado_gpu_status export_labels(pado_gpu_ctx *ctx, const pado_gpu_index *idx, uint8_t *bp_dist, uint64_t *bp_sets,
+                              uint64_t *label_off, uint32_t *hubs, uint8_t *d8, uint16_t *d16)
 {
-    clear_error();
     PADO_TRY(check_ctx(ctx));
     if (!idx) {
         set_error("null index");
@@ -363,6 +388,12 @@ pado_gpu_status pado_gpu_index_export(pado_gpu_ctx *ctx, const pado_gpu_index *i
         set_error("weighted index: use pado_gpu_windex_export");
         return PADO_GPU_ERR_STATE;
     }
+    if (idx->wide && d8) {
+        set_error("wide-domain index: its distances need pado_gpu_index_export_wide");
+        return PADO_GPU_ERR_STATE;
+    }
+    const uint32_t B = idx->wide ? Hop<true>::kBits : Hop<false>::kBits;
+    const uint32_t M = (1u << B) - 1u;
     if (!idx->loff.p) {
         set_error("index has no rank-space labels (loaded from an ordered file)");
         return PADO_GPU_ERR_STATE;
@@ -377,18 +408,38 @@ pado_gpu_status pado_gpu_index_export(pado_gpu_ctx *ctx, const pado_gpu_index *i
     }
     if (bp_sets) PADO_HIP(hipMemcpyAsync(bp_sets, idx->bps.p, sizeof(uint64_t) * n * kBP * 2, hipMemcpyDeviceToHost, st));
     if (label_off) PADO_HIP(hipMemcpyAsync(label_off, idx->loff.p, sizeof(uint64_t) * (n + 1), hipMemcpyDeviceToHost, st));
-    if (hubs || dists) {
+    if (hubs || d8 || d16) {
         std::vector<uint32_t> lab(idx->num_labels);
         if (idx->num_labels)
             PADO_HIP(hipMemcpyAsync(lab.data(), idx->lab.p, sizeof(uint32_t) * idx->num_labels, hipMemcpyDeviceToHost, st));
         PADO_HIP(hipStreamSynchronize(st));
         for (uint64_t i = 0; i < idx->num_labels; ++i) {
-            if (hubs) hubs[i] = lab[i] >> 8;
-            if (dists) dists[i] = (uint8_t)(lab[i] & 255u);
+            if (hubs) hubs[i] = lab[i] >> B;
+            if (d8) d8[i] = (uint8_t)(lab[i] & M);
+            if (d16) d16[i] = (uint16_t)(lab[i] & M);
         }
     }
     PADO_HIP(hipStreamSynchronize(st));
     return PADO_GPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+pado_gpu_status pado_gpu_index_export(pado_gpu_ctx *ctx, const pado_gpu_index *idx, uint8_t *bp_dist,
+                                      uint64_t *bp_sets, uint64_t *label_off, uint32_t *hubs, uint8_t *dists)
+{
+    clear_error();
+    // A wide index is refused only if its distances are asked for.
+    return export_labels(ctx, idx, bp_dist, bp_sets, label_off, hubs, dists, nullptr);
+}
+
+pado_gpu_status pado_gpu_index_export_wide(pado_gpu_ctx *ctx, const pado_gpu_index *idx, uint8_t *bp_dist,
+                                           uint64_t *bp_sets, uint64_t *label_off, uint32_t *hubs, uint16_t *dists)
+{
+    clear_error();
+    return export_labels(ctx, idx, bp_dist, bp_sets, label_off, hubs, nullptr, dists);
 }
 
 pado_gpu_status pado_gpu_index_import(pado_gpu_ctx *ctx, uint32_t n, const uint8_t *bp_dist, const uint64_t *bp_sets,
@@ -526,6 +577,10 @@ pado_gpu_status pado_gpu_query(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uin
         set_error("weighted index: use pado_gpu_wquery");
         return PADO_GPU_ERR_STATE;
     }
+    if (idx->wide) {
+        set_error("wide-domain index: its distances need pado_gpu_query_wide");
+        return PADO_GPU_ERR_STATE;
+    }
     const bool rank_space = (flags & PADO_GPU_QUERY_RANK_SPACE) != 0;
     if (!rank_space && !idx->ordered) {
         set_error("query before order_labels: call pado_gpu_index_order or pass PADO_GPU_QUERY_RANK_SPACE");
@@ -547,6 +602,49 @@ pado_gpu_status pado_gpu_query(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uin
     return host_query_chunked(ctx, st, a, b, k, out, 1u, idx->qa.p, idx->qb.p, idx->qout.p,
                               [&](const uint32_t *da, const uint32_t *db, uint64_t n, uint8_t *dout, hipStream_t s) {
                                   return launch_query(idx, rank_space, da, db, n, dout, ref, s);
+                              });
+}
+
+pado_gpu_status pado_gpu_query_wide(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uint32_t *a, const uint32_t *b,
+                                    uint64_t k, uint16_t *out, uint32_t flags, void *stream)
+{
+    clear_error();
+    PADO_TRY(check_ctx(ctx));
+    if (!idx || (k && (!a || !b || !out))) {
+        set_error("null argument");
+        return PADO_GPU_ERR_ARG;
+    }
+    if (idx->weighted) {
+        set_error("weighted index: use pado_gpu_wquery");
+        return PADO_GPU_ERR_STATE;
+    }
+    const bool ref = (flags & PADO_GPU_QUERY_REF_COMPAT) != 0;
+    if (idx->wide && ref) {
+        set_error("the reference has no wide hop domain: REF_COMPAT does not apply");
+        return PADO_GPU_ERR_ARG;
+    }
+    const bool rank_space = (flags & PADO_GPU_QUERY_RANK_SPACE) != 0;
+    if (!rank_space && !idx->ordered) {
+        set_error("query before order_labels: call pado_gpu_index_order or pass PADO_GPU_QUERY_RANK_SPACE");
+        return PADO_GPU_ERR_STATE;
+    }
+    if (rank_space && !idx->loff.p) {
+        set_error("index has no rank-space labels");
+        return PADO_GPU_ERR_STATE;
+    }
+    if (k == 0) return PADO_GPU_OK;
+    hipStream_t st = stream ? (hipStream_t)stream : ctx->stream;
+    if (flags & PADO_GPU_QUERY_DEVICE_PTRS) return launch_query_wide(idx, rank_space, a, b, k, out, ref, st);
+    if (idx->qa.n < k) {
+        PADO_TRY(idx->qa.alloc(k));
+        PADO_TRY(idx->qb.alloc(k));
+    }
+    if (idx->qout32.n * 2 < k) PADO_TRY(idx->qout32.alloc((k + 1) / 2));
+    return host_query_chunked(ctx, st, a, b, k, out, 2u, idx->qa.p, idx->qb.p,
+                              reinterpret_cast<uint8_t *>(idx->qout32.p),
+                              [&](const uint32_t *da, const uint32_t *db, uint64_t n, uint8_t *dout, hipStream_t s) {
+                                  return launch_query_wide(idx, rank_space, da, db, n,
+                                                           reinterpret_cast<uint16_t *>(dout), ref, s);
                               });
 }
 
@@ -621,6 +719,10 @@ pado_gpu_status pado_gpu_index_store(pado_gpu_ctx *ctx, const pado_gpu_index *id
     if (!idx || !filename || !rank || n != idx->n || idx->weighted) {
         set_error("bad argument");
         return PADO_GPU_ERR_ARG;
+    }
+    if (idx->wide) {
+        set_error("the reference's index file holds u8 distances: a wide-domain index cannot be stored in it");
+        return PADO_GPU_ERR_DOMAIN;
     }
     std::vector<uint8_t> bd((uint64_t)n * kBP);
     std::vector<uint64_t> bs((uint64_t)n * kBP * 2), off((uint64_t)n + 1);

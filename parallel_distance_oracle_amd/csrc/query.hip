@@ -60,13 +60,14 @@ constexpr uint32_t kWPre = 4;      // the same for weighted pairs (8 measured sl
 }  // namespace
 
 // Index of the first entry of the sorted list L[0..len) whose hub is >= key
-// (labels compared by hub, the upper 24 bits).
+// (labels compared by hub, the bits above the distance field).
+template <bool W>
 __device__ inline uint32_t lower_hub(const uint32_t *L, uint32_t len, uint32_t key)
 {
     uint32_t lo = 0, hi = len;
     while (lo < hi) {
         const uint32_t mid = (lo + hi) >> 1;
-        if ((L[mid] >> 8) < key)
+        if ((L[mid] >> Hop<W>::kBits) < key)
             lo = mid + 1;
         else
             hi = mid;
@@ -75,13 +76,18 @@ __device__ inline uint32_t lower_hub(const uint32_t *L, uint32_t len, uint32_t k
 }
 
 // One pair whose label offsets (oa, la), (ob, lb) are already loaded.
-// sbuf: the wave's kQLds-entry LDS buffer.
+// sbuf: the wave's kQLds-entry LDS buffer.  Hop<false>: the reference's
+// answers (255 for unreachable or >= 253); Hop<true>: exact hops, 0xFFFF for
+// unreachable, using only bit-parallel slots reached on both sides.
+template <bool W>
 __device__ inline uint32_t pair_distance_at(uint32_t n, const uint8_t *bpd, const ulonglong2 *bps,
                                             const uint32_t *lab, uint32_t a, uint32_t b, uint64_t oa, uint32_t la,
                                             uint64_t ob, uint32_t lb, bool ref_compat, uint32_t *sbuf)
 {
+    constexpr uint32_t B = Hop<W>::kBits, MASK = Hop<W>::kInf;
+    constexpr uint32_t UNREACHED = W ? 0xFFFFu : 255u;
     const uint32_t lane = threadIdx.x & 63;
-    if (a >= n || b >= n) return a == b ? 0u : 255u;
+    if (a >= n || b >= n) return a == b ? 0u : UNREACHED;
     // Every load that needs only the offsets goes out first: the slots'
     // distance bytes, the shorter list's first 64 labels and the longer
     // list's first kQPre * 64 labels (into registers; written to LDS after
@@ -112,7 +118,10 @@ __device__ inline uint32_t pair_distance_at(uint32_t n, const uint8_t *bpd, cons
     bool gate = false;
     if (lane < kBP) {
         td = (int)(da + db);
-        gate = !(ref_compat && td < 2) && td - 2 <= 255;
+        if (W)
+            gate = da != kInf8 && db != kInf8;  // 255: the slot's BFS did not reach this vertex
+        else
+            gate = !(ref_compat && td < 2) && td - 2 <= 255;
     }
     const uint32_t m = wave_min_u32(gate ? (uint32_t)td : 0xFFFFu);
     const bool use_sets = gate && (uint32_t)td <= m + 1u;
@@ -130,16 +139,16 @@ __device__ inline uint32_t pair_distance_at(uint32_t n, const uint8_t *bpd, cons
         for (uint32_t j = kQPre * 64 + lane; j < lt; j += 64) sbuf[j] = T[j];
         q_lds_sync();
     }
-    uint32_t d = 255u;
+    uint32_t d = UNREACHED;
     for (uint32_t j0 = 0; j0 < ls; j0 += 64) {
         const uint32_t j = j0 + lane;
         if (j < ls) {
             const uint32_t s = j0 == 0 ? s0 : S[j];
-            const uint32_t key = s >> 8;
+            const uint32_t key = s >> B;
             const uint32_t *L = staged ? sbuf : T;
-            const uint32_t lo = lower_hub(L, lt, key);
-            if (lo < lt && (L[lo] >> 8) == key) {
-                const uint32_t t = (s & 255u) + (L[lo] & 255u);
+            const uint32_t lo = lower_hub<W>(L, lt, key);
+            if (lo < lt && (L[lo] >> B) == key) {
+                const uint32_t t = (s & MASK) + (L[lo] & MASK);
                 if (t < d) d = t;
             }
         }
@@ -155,17 +164,19 @@ __device__ inline uint32_t pair_distance_at(uint32_t n, const uint8_t *bpd, cons
         if ((uint32_t)t < d) d = (uint32_t)t;
     }
     d = wave_min_u32(d);
+    if (W) return d;
     return d >= 253u ? 255u : d;
 }
 
+template <bool W>
 __device__ inline uint32_t pair_distance(uint32_t n, const uint8_t *bpd, const ulonglong2 *bps, const uint64_t *loff,
                                          const uint32_t *lab, uint32_t a, uint32_t b, bool ref_compat,
                                          uint32_t *sbuf)
 {
-    if (a >= n || b >= n) return a == b ? 0u : 255u;
+    if (a >= n || b >= n) return a == b ? 0u : (W ? 0xFFFFu : 255u);
     const uint64_t oa = loff[a], ob = loff[b];
-    return pair_distance_at(n, bpd, bps, lab, a, b, oa, (uint32_t)(loff[a + 1] - oa), ob,
-                            (uint32_t)(loff[b + 1] - ob), ref_compat, sbuf);
+    return pair_distance_at<W>(n, bpd, bps, lab, a, b, oa, (uint32_t)(loff[a + 1] - oa), ob,
+                               (uint32_t)(loff[b + 1] - ob), ref_compat, sbuf);
 }
 
 // A pair's ids and label offsets, loaded ahead of its turn.
@@ -195,9 +206,11 @@ __device__ inline void q_load_offsets(QPair &p, uint32_t n, const uint64_t *loff
     }
 }
 
+// OT: the answer type (u8, or u16 for a wide index or widened answers).
+template <bool W, class OT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_query(uint32_t n, const uint8_t *bpd, const ulonglong2 *bps,
                                                const uint64_t *loff, const uint32_t *lab, const uint32_t *qa,
-                                               const uint32_t *qb, uint64_t k, uint8_t *out, int ref_compat)
+                                               const uint32_t *qb, uint64_t k, OT *out, int ref_compat)
 {
     __shared__ uint32_t s_q[kQWaves][kQLds];
     uint32_t *sbuf = s_q[threadIdx.x >> 6];
@@ -214,9 +227,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     for (;;) {
         q_load_offsets(nxt, n, loff);
         q_load_ids(far, qa, qb, i + 2 * nw, k);
-        const uint32_t d = pair_distance_at(n, bpd, bps, lab, cur.a, cur.b, cur.oa, (uint32_t)(cur.ea - cur.oa),
-                                            cur.ob, (uint32_t)(cur.eb - cur.ob), ref_compat != 0, sbuf);
-        if ((threadIdx.x & 63) == 0) out[i] = (uint8_t)d;
+        const uint32_t d = pair_distance_at<W>(n, bpd, bps, lab, cur.a, cur.b, cur.oa, (uint32_t)(cur.ea - cur.oa),
+                                               cur.ob, (uint32_t)(cur.eb - cur.ob), ref_compat != 0, sbuf);
+        if ((threadIdx.x & 63) == 0) out[i] = (OT)d;
         i += nw;
         if (i >= k) break;
         cur = nxt;
@@ -342,7 +355,7 @@ __global__ __launch_bounds__(256) void k_query_ws(uint32_t n, const uint8_t *bpd
         if (c0 >= k) break;
         const uint64_t c1 = c0 + kGrab < k ? c0 + kGrab : k;
         for (uint64_t i = c0; i < c1; ++i) {
-            const uint32_t d = pair_distance(n, bpd, bps, loff, lab, qa[i], qb[i], ref_compat != 0, sbuf);
+            const uint32_t d = pair_distance<false>(n, bpd, bps, loff, lab, qa[i], qb[i], ref_compat != 0, sbuf);
             if (lane == 0) out[i] = (uint8_t)d;
         }
     }
@@ -386,6 +399,8 @@ pado_gpu_status launch_wquery(const pado_gpu_index *idx, bool rank_space, const 
                               uint64_t k, uint32_t *out, bool ref_compat, hipStream_t st);
 pado_gpu_status launch_query(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,
                              uint64_t k, uint8_t *out, bool ref_compat, hipStream_t st);
+pado_gpu_status launch_query_wide(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,
+                                  uint64_t k, uint16_t *out, bool ref_compat, hipStream_t st);
 
 // Query launch on device pairs for either index type: out holds u8
 // (unweighted) or u32 (weighted) answers.  work == nullptr selects the plain
@@ -397,6 +412,8 @@ pado_gpu_status launch_query_any(const pado_gpu_index *idx, bool rank_space, con
 {
     if (k == 0) return PADO_GPU_OK;
     const uint64_t *loff = rank_space ? idx->loff.p : idx->qoff.p;
+    // A wide index always takes the grid-stride kernel (u16 answers).
+    if (idx->wide) return launch_query_wide(idx, rank_space, qa, qb, k, (uint16_t *)out, false, st);
     if (!work) {
         if (idx->weighted) return launch_wquery(idx, rank_space, qa, qb, k, (uint32_t *)out, ref_compat, st);
         return launch_query(idx, rank_space, qa, qb, k, (uint8_t *)out, ref_compat, st);
@@ -443,8 +460,29 @@ pado_gpu_status launch_query(const pado_gpu_index *idx, bool rank_space, const u
     const uint64_t *loff = rank_space ? idx->loff.p : idx->qoff.p;
     const uint32_t *lab = rank_space ? idx->lab.p : idx->qlab.p;
     uint32_t grid = grid_for(k * 64, 256, 65536);
-    hipLaunchKernelGGL(k_query, dim3(grid), dim3(256), 0, st, idx->n, bpd, bps, loff, lab, qa, qb, k, out,
-                       ref_compat ? 1 : 0);
+    hipLaunchKernelGGL((k_query<false, uint8_t>), dim3(grid), dim3(256), 0, st, idx->n, bpd, bps, loff, lab, qa, qb,
+                       k, out, ref_compat ? 1 : 0);
+    PADO_HIP(hipGetLastError());
+    return PADO_GPU_OK;
+}
+
+// u16 answers on device pairs: exact hops for a wide index, the reference's
+// answers widened for one in the u8 domain.
+pado_gpu_status launch_query_wide(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,
+                                  uint64_t k, uint16_t *out, bool ref_compat, hipStream_t st)
+{
+    if (k == 0) return PADO_GPU_OK;
+    const uint8_t *bpd = rank_space ? idx->bpd.p : idx->qbpd.p;
+    const ulonglong2 *bps = (const ulonglong2 *)(rank_space ? idx->bps.p : idx->qbps.p);
+    const uint64_t *loff = rank_space ? idx->loff.p : idx->qoff.p;
+    const uint32_t *lab = rank_space ? idx->lab.p : idx->qlab.p;
+    uint32_t grid = grid_for(k * 64, 256, 65536);
+    if (idx->wide)
+        hipLaunchKernelGGL((k_query<true, uint16_t>), dim3(grid), dim3(256), 0, st, idx->n, bpd, bps, loff, lab, qa,
+                           qb, k, out, 0);
+    else
+        hipLaunchKernelGGL((k_query<false, uint16_t>), dim3(grid), dim3(256), 0, st, idx->n, bpd, bps, loff, lab,
+                           qa, qb, k, out, ref_compat ? 1 : 0);
     PADO_HIP(hipGetLastError());
     return PADO_GPU_OK;
 }

@@ -81,12 +81,16 @@ struct Request {
     uint64_t k = 0;
     int state = 0;       // 0 queued, 1 launched, 2 done, 3 consumed
     bool claimed = false;  // a waiter is copying the answers out
+    bool failed = false;   // its launch failed: no answers
+    bool cancelled = false;  // answers are not wanted: released once no launch reads its slot
     double t_submit = 0, t_done = 0;
 };
 
 struct Launch {
     std::vector<Request *> reqs;
     int stream = 0;
+    bool failed = false;        // a launch call returned an error
+    bool end_recorded = false;  // the completion event is on the stream
 };
 
 }  // namespace
@@ -136,6 +140,18 @@ uint64_t place(uint64_t tail, uint64_t k, uint64_t cap)
     return tail;
 }
 
+// With nothing live the ring is empty: the next request starts at physical
+// offset 0 (the next lap's start), so any request of up to `cap` pairs fits.
+// Restarting at the current physical offset would refuse a request larger
+// than that offset forever (it would be placed on the next lap, past head +
+// cap, and nothing live could move head).
+void rewind_if_empty(pado_gpu_server *s)
+{
+    if (!s->live.empty()) return;
+    const uint64_t lap = (s->tail + s->cap - 1) / s->cap * s->cap;
+    s->head = s->tail = lap;
+}
+
 void release_consumed(pado_gpu_server *s)
 {
     while (!s->live.empty() && s->live.front()->state == 3) {
@@ -145,7 +161,7 @@ void release_consumed(pado_gpu_server *s)
         s->head = r->start + r->k;
         delete r;
     }
-    if (s->live.empty()) s->head = s->tail;
+    rewind_if_empty(s);
 }
 
 // Continuous batching policy: with the device idle a launch starts at once
@@ -200,11 +216,9 @@ void dispatcher_loop(pado_gpu_server *s)
         const uint64_t phys = p0 % s->cap, k = p1 - p0;
         hipStream_t st = s->streams[si];
         const bool in_ring = phys + k <= s->cap;  // holds by the packing above; never launch past the ring
-        if (!in_ring) {
-            std::lock_guard<std::mutex> g(s->mu);
-            s->error = "serving launch crosses the staging ring's end";
-        }
-        (void)hipEventRecord(s->ev_start[si], st);
+        std::string why;
+        if (!in_ring) why = "serving launch crosses the staging ring's end";
+        if (hipEventRecord(s->ev_start[si], st) != hipSuccess) why = "event record failed";
         if (k && in_ring) {
             const uint32_t sgrid = (uint32_t)std::min<uint64_t>((k + 1023) / 1024, 1024);
             hipLaunchKernelGGL(k_stage_pairs, dim3(sgrid), dim3(256), 0, st, s->m_a + phys, s->m_b + phys,
@@ -214,14 +228,21 @@ void dispatcher_loop(pado_gpu_server *s)
                                                   s->d_out.p + phys * s->es,
                                                   (s->cfg.flags & PADO_GPU_QUERY_REF_COMPAT) != 0, st,
                                                   grid_stride ? nullptr : s->work.p + si, s->ws_blocks);
-            if (qs != PADO_GPU_OK) {
-                std::lock_guard<std::mutex> g(s->mu);
-                s->error = pado_gpu_last_error();
-            }
-            (void)hipMemcpyAsync(s->h_out + phys * s->es, s->d_out.p + phys * s->es, k * s->es,
-                                 hipMemcpyDeviceToHost, st);
+            if (qs != PADO_GPU_OK) why = pado_gpu_last_error();
+            if (hipMemcpyAsync(s->h_out + phys * s->es, s->d_out.p + phys * s->es, k * s->es,
+                               hipMemcpyDeviceToHost, st) != hipSuccess)
+                why = "answer copy failed";
         }
-        (void)hipEventRecord(s->ev_end[si], st);
+        // The end event is recorded whatever failed, so the completer retires
+        // this launch (and fails its requests) in order.
+        const hipError_t rec = hipEventRecord(s->ev_end[si], st);
+        if (rec != hipSuccess) why = std::string("event record failed: ") + hipGetErrorString(rec);
+        if (!why.empty()) {
+            L.failed = true;
+            std::lock_guard<std::mutex> g(s->mu);
+            if (s->error.empty()) s->error = why;
+        }
+        L.end_recorded = rec == hipSuccess;
         lk.lock();
         s->inflight.push_back(std::move(L));
         s->st.launches += 1;
@@ -247,22 +268,45 @@ void completer_loop(pado_gpu_server *s)
             continue;
         }
         const int si = s->inflight.front().stream;
+        const bool recorded = s->inflight.front().end_recorded;
         hipEvent_t ev0 = s->ev_start[si], ev1 = s->ev_end[si];
         lk.unlock();
         // Poll rather than block: a blocking event wait can resume
-        // milliseconds late, which would land in the latency tail.
-        while (hipEventQuery(ev1) == hipErrorNotReady) std::this_thread::yield();
+        // milliseconds late, which would land in the latency tail.  Any
+        // result other than not-ready / success fails the launch's requests;
+        // without a recorded end event the stream is drained instead, so no
+        // copy of this launch is still pending when its ring space is reused.
+        hipError_t q = hipSuccess;
+        if (recorded) {
+            while ((q = hipEventQuery(ev1)) == hipErrorNotReady) std::this_thread::yield();
+        } else {
+            q = hipStreamSynchronize(s->streams[si]);
+        }
         float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, ev0, ev1);
+        if (q == hipSuccess && recorded) (void)hipEventElapsedTime(&ms, ev0, ev1);
         const double t = now_ms();
         lk.lock();
         Launch L = std::move(s->inflight.front());
         s->inflight.pop_front();
+        if (q != hipSuccess) {
+            L.failed = true;
+            if (s->error.empty()) s->error = std::string("serving launch failed: ") + hipGetErrorString(q);
+        }
+        bool release = false;
         for (Request *r : L.reqs) {
             r->state = 2;
+            r->failed = L.failed;
             r->t_done = t;
             s->st.requests_done += 1;
             s->st.pairs_done += r->k;
+            if (r->cancelled) {
+                r->state = 3;  // nobody will wait for it
+                release = true;
+            }
+        }
+        if (release) {
+            release_consumed(s);
+            s->cv_space.notify_all();
         }
         s->st.busy_ms += ms;
         s->stream_busy[L.stream] = 0;
@@ -276,20 +320,36 @@ void completer_loop(pado_gpu_server *s)
 // The copy runs outside the lock (the slot is not reused before release): a
 // caller buffer that page-faults on first touch holds back only this caller,
 // not admission, dispatch or completion.
-pado_gpu_status wait_request(pado_gpu_server *s, uint64_t id, void *out, double *t_submit, double *t_done)
+pado_gpu_status wait_request(pado_gpu_server *s, uint64_t id, void *out, double *t_submit, double *t_done,
+                             int64_t timeout_us = -1)
 {
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->by_id.find(id);
-    if (it == s->by_id.end() || it->second->state == 3 || it->second->claimed) {
+    if (it == s->by_id.end() || it->second->state == 3 || it->second->claimed || it->second->cancelled) {
         set_error("unknown request id");
         return PADO_GPU_ERR_ARG;
     }
     Request *r = it->second;
     r->claimed = true;
-    s->cv_done.wait(lk, [&] { return r->state >= 2; });
-    if (!s->error.empty()) {
+    if (timeout_us < 0) {
+        s->cv_done.wait(lk, [&] { return r->state >= 2; });
+    } else if (!s->cv_done.wait_for(lk, std::chrono::microseconds(timeout_us), [&] { return r->state >= 2; })) {
+        // Not done in time: the request stays live; the caller may wait again
+        // or cancel it.
         r->claimed = false;
-        set_error(s->error);
+        set_error("request not done within the timeout");
+        return PADO_GPU_ERR_TIMEOUT;
+    }
+    if (r->failed) {
+        // No answers: the request is released (its ring space and slot are
+        // not leaked) and the server's first error is reported.
+        const std::string msg = s->error.empty() ? std::string("serving launch failed") : s->error;
+        r->state = 3;
+        release_consumed(s);
+        lk.unlock();
+        s->cv_work.notify_all();
+        s->cv_space.notify_all();
+        set_error(msg);
         return PADO_GPU_ERR_HIP;
     }
     lk.unlock();
@@ -330,7 +390,7 @@ pado_gpu_status pado_gpu_serve_start(pado_gpu_ctx *ctx, pado_gpu_index *idx, con
     if (!s->cfg.max_launch_pairs) s->cfg.max_launch_pairs = 4ull << 20;
     if (!s->cfg.max_requests) s->cfg.max_requests = 4096;
     if (!s->cfg.streams) s->cfg.streams = 3;
-    s->es = idx->weighted ? 4u : 1u;
+    s->es = idx->weighted ? 4u : (idx->wide ? 2u : 1u);
     // Persistent kernel: as many resident workgroups of 256 threads per
     // compute unit as the kernel's occupancy allows.
     s->ws_blocks = query_ws_blocks_per_cu(idx->weighted) * (ctx->num_cu > 0 ? (uint32_t)ctx->num_cu : 256u);
@@ -414,6 +474,12 @@ pado_gpu_status pado_gpu_serve_submit(pado_gpu_server *s, const uint32_t *a, con
         set_error("server stopping");
         return PADO_GPU_ERR_STATE;
     }
+    if (!s->error.empty()) {
+        // A launch failed: the device state is suspect, admit nothing more.
+        set_error("server failed: " + s->error);
+        return PADO_GPU_ERR_STATE;
+    }
+    rewind_if_empty(s);
     const uint64_t start = place(s->tail, k, s->cap);
     if (s->by_id.size() >= s->cfg.max_requests || start + k - s->head > s->cap) {
         s->st.rejected += 1;
@@ -438,10 +504,55 @@ pado_gpu_status pado_gpu_serve_submit(pado_gpu_server *s, const uint32_t *a, con
         std::memcpy(s->h_b + phys, b, k * 4);
     }
     lk.lock();
+    if (r->cancelled) {
+        // Cancelled while its pairs were copied in: never launched.
+        r->state = 3;
+        release_consumed(s);
+        lk.unlock();
+        s->cv_space.notify_all();
+        return PADO_GPU_OK;
+    }
     s->pending.push_back(r);
     s->pending_pairs += k;
     lk.unlock();
     s->cv_work.notify_one();
+    return PADO_GPU_OK;
+}
+
+pado_gpu_status pado_gpu_serve_cancel(pado_gpu_server *s, uint64_t req_id)
+{
+    clear_error();
+    if (!s) return PADO_GPU_ERR_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->by_id.find(req_id);
+    if (it == s->by_id.end() || it->second->state == 3 || it->second->cancelled) {
+        set_error("unknown request id");
+        return PADO_GPU_ERR_ARG;
+    }
+    Request *r = it->second;
+    if (r->claimed) {
+        set_error("a wait on this request is in progress");
+        return PADO_GPU_ERR_STATE;
+    }
+    r->cancelled = true;
+    if (r->state == 0) {
+        auto p = std::find(s->pending.begin(), s->pending.end(), r);
+        if (p != s->pending.end()) {
+            // Queued, not launched: dropped now.
+            s->pending.erase(p);
+            s->pending_pairs -= r->k;
+            r->state = 3;
+        }
+        // Otherwise submit is still copying its pairs in and drops it after.
+    } else if (r->state == 2) {
+        r->state = 3;  // done: its answers are discarded
+    }
+    // A launched request (state 1) is released by the completer when its
+    // launch finishes.
+    release_consumed(s);
+    lk.unlock();
+    s->cv_space.notify_all();
+    s->cv_work.notify_all();
     return PADO_GPU_OK;
 }
 
@@ -464,6 +575,19 @@ pado_gpu_status pado_gpu_serve_wait(pado_gpu_server *s, uint64_t req_id, void *o
     if (!s) return PADO_GPU_ERR_ARG;
     double t_submit = 0, t_done = 0;
     PADO_TRY(wait_request(s, req_id, out, &t_submit, &t_done));
+    if (latency_ms) *latency_ms = t_done - t_submit;
+    return PADO_GPU_OK;
+}
+
+pado_gpu_status pado_gpu_serve_wait_for(pado_gpu_server *s, uint64_t req_id, void *out, double *latency_ms,
+                                        uint64_t timeout_us)
+{
+    clear_error();
+    if (!s) return PADO_GPU_ERR_ARG;
+    double t_submit = 0, t_done = 0;
+    // Beyond about 11 days the wait is unbounded (and the clock arithmetic safe).
+    const int64_t t = timeout_us >= (1ull << 40) ? -1 : (int64_t)timeout_us;
+    PADO_TRY(wait_request(s, req_id, out, &t_submit, &t_done, t));
     if (latency_ms) *latency_ms = t_done - t_submit;
     return PADO_GPU_OK;
 }
@@ -493,11 +617,13 @@ pado_gpu_status pado_gpu_serve_replay(pado_gpu_server *s, const uint32_t *a, con
         for (uint32_t i = 0; i < num_req; ++i) {
             const double due = t0 + arrival_s[i] * 1e3;
             while (now_ms() < due) {
+                if (failed.load()) return;  // the waiter gave up: stop submitting
                 if (due - now_ms() > 0.2) std::this_thread::yield();
             }
             const uint64_t k = req_off[i + 1] - req_off[i];
             uint64_t id = 0;
             for (;;) {
+                if (failed.load()) return;
                 pado_gpu_status st = pado_gpu_serve_submit(s, a + req_off[i], b + req_off[i], k, &id);
                 if (st == PADO_GPU_OK) break;
                 if (st != PADO_GPU_ERR_BUSY) {

@@ -124,12 +124,34 @@ def weighted(tag, edges, weights, n_pairs=None):
     print("wrote", out, "n", n, "labels", hub.size)
 
 
+def unp_index(tag, edges, rank):
+    """An index file written by the reference's own writer (the sequential
+    VertexCentricPLL<1024>, UNP:1120-1180), stored as its bytes."""
+    data = np.frombuffer(oracle.ref_unp_file(edges, rank), np.uint8)
+    out = os.path.join(HERE, f"unp_{tag}.npz")
+    np.savez_compressed(out, edges=np.asarray(edges, np.uint32), rank=np.asarray(rank, np.uint32), index=data)
+    print("wrote", out, data.size, "bytes")
+
+
+def unp_fixtures():
+    for name in ("edgelist_in", "input_dat"):
+        g = np.load(os.path.join(HERE, f"tiny_{name}.npz"))
+        unp_index(name, g["edges"], g["rank"])
+    e = rmat_config(10, 8)
+    unp_index("rmat10_ef8", e, oracle.ref_rank(e))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--big", action="store_true", help="also R-MAT 16/18/20 digests (minutes)")
+    ap.add_argument("--unp-only", action="store_true", help="only the reference-written index files")
     args = ap.parse_args()
     if not oracle.have_ref():
         sys.exit("oracle/_ref/pado_ref is not built (run oracle/build_ref.sh)")
+    if args.unp_only:
+        unp_fixtures()
+        return
+    unp_fixtures()
     for name in ("edgelist.in", "input.dat", "deleted_graph.in"):
         tiny(name)
     rmat(10, 8, 20000, True)

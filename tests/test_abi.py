@@ -45,3 +45,29 @@ def test_cpp_drop_in_compiles(tmp_path, example):
            "-L", os.path.dirname(pado.LIB_PATH), "-lpado_gpu", "-Wl,-rpath," + os.path.dirname(pado.LIB_PATH)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.skipif(shutil.which("gcc") is None, reason="no host C compiler")
+def test_ctypes_structs_match_the_header(tmp_path):
+    """Every struct the Python mirror passes across the ABI has the header's
+    size and field offsets (a C compiler reads include/pado_gpu.h)."""
+    structs = {"pado_gpu_build_cfg": pado.BuildCfg, "pado_gpu_index_info": pado.IndexInfo,
+               "pado_gpu_serve_cfg": pado.ServeCfg, "pado_gpu_serve_stats": pado.ServeStats}
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "pado_gpu.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["return 0; }"]
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    r = subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], capture_output=True,
+                       text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l}
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)

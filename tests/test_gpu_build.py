@@ -160,3 +160,20 @@ def test_bad_graphs_are_rejected(gpu_ctx):
     G = pado.Graph(3, np.array([0, 2, 3, 4], np.uint64), np.array([1, 2, 0, 0], np.uint32))
     with pytest.raises(pado.PadoError):
         pado.DeviceGraph(gpu_ctx, G)  # adjacency of 0 not descending
+
+
+def test_label_sort_in_vertex_ranges(gpu_ctx):
+    """The final canonical sort split into many vertex ranges (the path an
+    index of more than 2^31 labels takes) gives the same labels."""
+    meta, arr = load_rmat(14, 8)
+    G, _, _ = rank_space(rmat_edges(14, 8))
+    cfg = pado.BuildCfg()
+    cfg.batch_size = 1024
+    cfg.sort_max_items = 3000  # ~10 ranges over the 27K labels
+    print("[gpu] rmat14 split sort", flush=True)
+    dg = pado.DeviceGraph(gpu_ctx, G)
+    idx = pado.Index.build(gpu_ctx, dg, cfg=cfg)
+    _, _, loff, hub, dist = idx.export()
+    assert np.array_equal(loff, arr["loff"]) and np.array_equal(hub, arr["hub"]) and np.array_equal(dist, arr["dist"])
+    idx.close()
+    dg.close()

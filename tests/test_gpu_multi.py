@@ -1,6 +1,7 @@
 """Many graphs labeled in one job: every graph's bit-parallel records and label
-sets equal the oracle's build of that graph alone; graphs outside the
-reference's u8 distance domain report PADO_GPU_ERR_DOMAIN."""
+sets equal the oracle's build of that graph alone; a graph outside the
+reference's u8 distance domain is labeled in the wide hop domain, with
+answers equal to BFS."""
 import sys
 
 import numpy as np
@@ -47,8 +48,22 @@ def test_multi_graph_job_matches_single_builds(gpu_ctx):
         X = oracle.Index(G.num_v, G.off, G.adj)
         assert np.array_equal(bp_d, X.bp_d) and np.array_equal(bp_s, X.bp_s), gi
         assert np.array_equal(loff, X.loff) and np.array_equal(hub, X.hub) and np.array_equal(dist, X.dist), gi
-    last = mi.graph_info(len(graphs) - 1)
-    assert last["status"] == 4 and not last["in_union"]
-    with pytest.raises(pado.PadoError):
-        mi.export(len(graphs) - 1)
+    # The deep grid is labeled on its own in the wide hop domain: u16 label
+    # distances, answers equal to BFS (beyond 255 too).
+    g = len(graphs) - 1
+    last = mi.graph_info(g)
+    assert last["status"] == 0 and not last["in_union"] and last["wide"]
+    _, _, loff, hub, dist = mi.export(g)
+    assert dist.dtype == np.uint16 and hub.size == last["num_labels"]
+    n = big.num_v
+    for s in (0, n // 2, n - 1):
+        d = oracle.bfs(n, big.off, big.adj, s).astype(np.int64)
+        got = mi.query(g, np.full(n, s, np.uint32), np.arange(n, dtype=np.uint32)).astype(np.int64)
+        assert np.array_equal(got, np.where(d == 0xFFFFFFFF, pado.WIDE_INF, d)), s
+        assert d.max() > 255
+    # Union graphs answer through the union's index, as their own build would.
+    G = graphs[0]
+    got = mi.query(0, np.zeros(G.num_v, np.uint32), np.arange(G.num_v, dtype=np.uint32))
+    assert np.array_equal(got, oracle.bfs_to_u8(oracle.bfs(G.num_v, G.off, G.adj, 0)))
+    assert mi.query(0, [G.num_v, G.num_v, 0], [G.num_v, G.num_v + 3, G.num_v]).tolist() == [0, 255, 255]
     mi.close()

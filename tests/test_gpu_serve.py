@@ -90,6 +90,13 @@ def test_admission_refuses_when_full(gpu_ctx, served):
             srv.submit(a, a)  # ring holds 4096 pairs
         out, lat = srv.wait(r1, a.size)
         assert np.all(out == 0) and lat >= 0
+        # The ring drained at physical offset 3000 of 4096: a request larger
+        # than that offset, and one of the whole ring, must still be admitted.
+        big = np.arange(4096, dtype=np.uint32) % G.num_v
+        r3 = srv.submit(big[:3001], big[:3001])
+        assert np.all(srv.wait(r3, 3001)[0] == 0)
+        r4 = srv.submit(big, big)
+        assert np.all(srv.wait(r4, 4096)[0] == 0)
         r2 = srv.submit(a, a)  # space released
         srv.wait(r2, a.size)
         ids = [srv.submit(a[:10], a[:10]) for _ in range(4)]
@@ -193,3 +200,53 @@ def test_requests_ending_at_the_ring_end(gpu_ctx, served):
         assert np.array_equal(ans, idx.query(A, B))
         assert st["requests_done"] == count and st["launches"] < count  # neighbours were packed
         assert np.all(lat >= 0)
+
+
+def test_cancel_and_wait_timeout(gpu_ctx, served):
+    """A cancelled request releases its slot and ring space whatever state it
+    was in, and its id becomes unknown; a timed wait either returns the
+    answers or leaves the request live for another wait."""
+    G, idx, _ = served
+    srv = pado.Server(gpu_ctx, idx, staging_pairs=4096, max_requests=8, streams=2)
+    try:
+        a = (np.arange(4096, dtype=np.uint32) * 7) % G.num_v
+        b = (np.arange(4096, dtype=np.uint32) * 13) % G.num_v
+        exp = idx.query(a, b)
+        # Cancel right after submit (queued, launched or done): the whole ring
+        # must become free again.
+        for _ in range(5):
+            r = srv.submit(a[:3000], b[:3000])
+            srv.cancel(r)
+            with pytest.raises(pado.PadoError):
+                srv.cancel(r)  # unknown id now
+            with pytest.raises(pado.PadoError):
+                srv.wait(r, 3000)
+        for _ in range(2000):  # a launched cancelled request frees its space when its launch ends
+            try:
+                r = srv.submit(a, b)
+                break
+            except pado.Busy:
+                import time
+                time.sleep(0.001)
+        else:
+            raise AssertionError("ring space of cancelled requests was not released")
+        assert np.array_equal(srv.wait(r, 4096)[0], exp)
+        # A done request that is cancelled discards its answers.
+        r = srv.submit(a[:100], b[:100])
+        while not srv.poll(r):
+            pass
+        srv.cancel(r)
+        # Timed waits: poll until done, then the answers are the plain ones.
+        r = srv.submit(a, b)
+        got = None
+        for _ in range(100000):
+            try:
+                got = srv.wait(r, 4096, timeout_us=0)[0]
+                break
+            except pado.Timeout:
+                pass
+        assert got is not None and np.array_equal(got, exp)
+        r = srv.submit(a[:10], b[:10])
+        assert np.array_equal(srv.wait(r, 10, timeout_us=10_000_000)[0], exp[:10])
+    finally:
+        srv.stop()

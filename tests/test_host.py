@@ -114,3 +114,28 @@ def test_weighted_text_loader(tmp_path):
     G = pado.WeightedGraph.from_file(str(p))
     H = pado.WeightedGraph.from_edges(np.array([[0, 1], [1, 2], [2, 0]], np.uint32), np.array([5, 44, 1]))
     assert np.array_equal(G.off, H.off) and np.array_equal(G.adj, H.adj) and np.array_equal(G.w, H.w)
+
+
+def test_sort_plan_ranges():
+    """The final label sort runs in vertex ranges that each hold at most the
+    cap (hipCUB counts items in int): the cuts partition [0, n), every range
+    fits, a greedy range cannot take one more vertex, and a vertex larger than
+    the cap is refused."""
+    rng = np.random.default_rng(2)
+    lens = rng.integers(0, 50, size=1000).astype(np.uint64)
+    off = np.concatenate([[0], np.cumsum(lens)]).astype(np.uint64)
+    for cap in (49, 50, 77, 1000, int(off[-1]), int(off[-1]) + 5):
+        cuts = pado.sort_plan(off, cap)
+        assert cuts is not None and cuts[0] == 0 and cuts[-1] == 1000
+        assert np.all(np.diff(cuts.astype(np.int64)) > 0)
+        for v0, v1 in zip(cuts[:-1], cuts[1:]):
+            assert off[v1] - off[v0] <= cap
+            if v1 < 1000:
+                assert off[v1 + 1] - off[v0] > cap  # greedy: the next vertex would not fit
+    assert pado.sort_plan(off, int(off[-1])).size == 2  # one range when everything fits
+    big = np.array([0, 3, 10, 11], np.uint64)
+    assert pado.sort_plan(big, 6) is None  # vertex 1 holds 7 labels
+    # 2^31 labels over 2^12 vertices: ranges below hipCUB's int limit.
+    off2 = (np.arange(4097, dtype=np.uint64) * np.uint64(1 << 19))
+    cuts = pado.sort_plan(off2, (1 << 31) - 1025)
+    assert cuts.size == 3 and all(off2[b] - off2[a] < 2 ** 31 for a, b in zip(cuts[:-1], cuts[1:]))

@@ -1,10 +1,12 @@
 """The CPU oracle against the reference's own outputs (golden fixtures written
 by tests/golden/make_golden.py from the reference build)."""
+import os
+
 import numpy as np
 import pytest
 
 import oracle
-from helpers import (RMAT_SMALL, TINY, bp_digest, digest, have_rmat, label_digest, load_rmat, load_tiny,
+from helpers import (GOLDEN, RMAT_SMALL, TINY, bp_digest, digest, have_rmat, label_digest, load_rmat, load_tiny,
                      oracle_rank_space, rmat_edges)
 
 
@@ -97,3 +99,49 @@ def test_rmat16_digests():
     X = oracle.Index(n, off, adj)
     assert label_digest(X.loff, X.hub, X.dist) == meta["labels_sha256"]
     assert bp_digest(X.bp_d, X.bp_s) == meta["bp_sha256"]
+
+
+def _parse_unp(buf):
+    """The reference's index file (UNP:1120-1180): u32 num_v, u32 50, then per
+    original id 50 x (u8 dist, u64 S^-1, u64 S^0), u32 size, size x (u32 hub,
+    u8 dist)."""
+    n, bp = np.frombuffer(buf[:8], np.uint32)
+    pos = 8
+    rec = np.dtype([("d", "u1"), ("s0", "<u8"), ("s1", "<u8")])
+    bp_d = np.zeros((n, bp), np.uint8)
+    bp_s = np.zeros((n, bp, 2), np.uint64)
+    off = [0]
+    hubs, dists = [], []
+    lab = np.dtype([("h", "<u4"), ("d", "u1")])
+    for x in range(n):
+        r = np.frombuffer(buf[pos:pos + 17 * bp], rec)
+        pos += 17 * bp
+        bp_d[x], bp_s[x, :, 0], bp_s[x, :, 1] = r["d"], r["s0"], r["s1"]
+        sz = int(np.frombuffer(buf[pos:pos + 4], np.uint32)[0])
+        pos += 4
+        L = np.frombuffer(buf[pos:pos + 5 * sz], lab)
+        pos += 5 * sz
+        hubs.append(L["h"])
+        dists.append(L["d"])
+        off.append(off[-1] + sz)
+    assert pos == len(buf)
+    return bp_d, bp_s, np.array(off, np.uint64), np.concatenate(hubs), np.concatenate(dists)
+
+
+@pytest.mark.parametrize("tag", ["edgelist_in", "input_dat", "rmat10_ef8"])
+def test_reference_written_index_file_equals_oracle(tag):
+    """Index files written by the reference's own writer (committed bytes)
+    hold, per original id, the oracle's bit-parallel records and canonical
+    labels of rank[id] (hubs in rank space)."""
+    g = np.load(os.path.join(GOLDEN, f"unp_{tag}.npz"))
+    bp_d, bp_s, off, hub, dist = _parse_unp(g["index"].tobytes())
+    n, off0, adj0 = oracle.graph_csr(g["edges"])
+    rank = g["rank"]
+    off2, adj2, _ = oracle.id_transfer(n, off0, adj0, rank)
+    X = oracle.Index(n, off2, adj2)
+    assert np.array_equal(bp_d, X.bp_d[rank]) and np.array_equal(bp_s, X.bp_s[rank])
+    for x in range(n):
+        v = int(rank[x])
+        s, e = int(off[x]), int(off[x + 1])
+        assert np.array_equal(hub[s:e], X.hub[int(X.loff[v]):int(X.loff[v + 1])]), x
+        assert np.array_equal(dist[s:e], X.dist[int(X.loff[v]):int(X.loff[v + 1])]), x

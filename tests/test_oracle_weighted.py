@@ -74,3 +74,21 @@ def test_weighted_labels_are_canonical(tag):
     for v in range(n):
         got = list(zip(X.hub[X.loff[v]:X.loff[v + 1]].tolist(), X.dist[X.loff[v]:X.loff[v + 1]].tolist()))
         assert got == C[v], v
+
+
+@pytest.mark.parametrize("tag", ["weighted_edgelist_in", "7_w_edgelist_in", "triangle_in", "rectangle_in",
+                                 "rmat10_ef8", "rmat12_ef8"])
+def test_canonical_of_one_vertex(tag):
+    """The one-Dijkstra canonical label set of a vertex (the checker of the
+    full-size weighted GPU test) equals the reference's labels of that vertex
+    on every vertex of the reference's fixtures, and its distances equal
+    Dijkstra's."""
+    g = _load(tag)
+    n, off, adj, w = _rank_space(g)
+    step = max(1, n // 60)
+    for v in range(0, n, step):
+        d, hub, hd = oracle.canonical_of(n, off, adj, w, v)
+        assert np.array_equal(d, oracle.dijkstra(n, off, adj, w, v))
+        s, e = int(g["loff"][v]), int(g["loff"][v + 1])
+        assert np.array_equal(hub, g["hub"][s:e]), v
+        assert np.array_equal(hd.astype(np.int64), g["dist"][s:e].astype(np.int64)), v
