@@ -260,6 +260,17 @@ pado_gpu_status pado_gpu_index_order(pado_gpu_ctx *ctx, pado_gpu_index *idx, con
 pado_gpu_status pado_gpu_query(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uint32_t *a, const uint32_t *b, uint64_t k,
                                uint8_t *out, uint32_t flags, void *stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): pado_gpu_query on HOST arrays a, b, out with the same
+ * answers and flags (REF_COMPAT, RANK_SPACE; DEVICE_PTRS is refused), staged
+ * through a fixed device array the library holds on every GPU it runs on
+ * (9 bytes per pair, 9 MiB) instead of allocated buffers.  At most
+ * PADO_GPU_NOALLOC_MAX_PAIRS pairs per call (PADO_GPU_ERR_ARG beyond).  It
+ * calls no hipMalloc / hipFree.  Runs on the context's stream and returns when
+ * the answers are in `out`.  All calls on one GPU share the array: calls that
+ * overlap on one GPU (from several threads or contexts) race and must be
+ * serialised by the caller. */
 /* query_distance for an index of either hop domain with u16 answers: a
  * wide-domain index answers exact hop counts, 0xFFFF if unreachable (its
  * bit-parallel part uses only slots reached on both sides; REF_COMPAT is
@@ -270,6 +281,9 @@ pado_gpu_status pado_gpu_query(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uin
 pado_gpu_status pado_gpu_query_wide(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uint32_t *a, const uint32_t *b,
                                     uint64_t k, uint16_t *out, uint32_t flags, void *stream);
 
+#define PADO_GPU_NOALLOC_MAX_PAIRS (1u << 20)
+pado_gpu_status pado_gpu_query_noalloc(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uint32_t *a, const uint32_t *b,
+                                       uint64_t k, uint8_t *out, uint32_t flags);
 
 void pado_gpu_index_free(pado_gpu_index *idx);
 

@@ -134,6 +134,7 @@ def lib():
         "pado_gpu_index_import": (ctypes.c_int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
         "pado_gpu_index_order": (ctypes.c_int, [_vp, _vp, _vp, _u32]),
         "pado_gpu_query": (ctypes.c_int, [_vp, _vp, _vp, _vp, _u64, _vp, _u32, _vp]),
+        "pado_gpu_query_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _u64, _vp, _u32]),
         "pado_gpu_query_wide": (ctypes.c_int, [_vp, _vp, _vp, _vp, _u64, _vp, _u32, _vp]),
         "pado_gpu_index_export_wide": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "pado_gpu_index_free": (None, [_vp]),

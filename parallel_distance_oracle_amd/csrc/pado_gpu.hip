@@ -5,10 +5,26 @@
 #include <chrono>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 
 #include "internal.hpp"
 
+// Staging memory of pado_gpu_query_noalloc: one call's pair ids (2 x u32) and
+// answers (u8), PADO_GPU_NOALLOC_MAX_PAIRS pairs at most (9 MiB).  Every GPU the library runs on
+// holds its own copy until the process ends.  In host code this name is only
+// a stand-in for the device symbol: kernels and copies use the address
+// hipGetSymbolAddress returns for the current device (g_scratch below).
+static __device__ __align__(256) uint8_t scratch[PADO_GPU_NOALLOC_MAX_PAIRS * 9ull];
+
 namespace pado {
+
+namespace {
+// Device address of `scratch` per device, resolved once per device by
+// pado_gpu_ctx_create with that device current.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};
+}  // namespace
 
 static thread_local std::string g_last_error;
 void set_error(const std::string &msg) { g_last_error = msg; }
@@ -152,6 +168,20 @@ pado_gpu_status pado_gpu_ctx_create(int device, pado_gpu_ctx **out)
         return PADO_GPU_ERR_ARG;
     }
     PADO_HIP(hipSetDevice(device));
+    if (device < kMaxDevices) {
+        // The allocation-free query's staging array: its address on this
+        // device, looked up once per device (the device is current here).
+        std::lock_guard<std::mutex> g(g_scratch_mu);
+        if (!g_scratch[device]) {
+            void *ptr = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !ptr) {
+                set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+                return PADO_GPU_ERR_HIP;
+            }
+            g_scratch[device] = ptr;
+        }
+    }
     auto *c = new pado_gpu_ctx();
     c->device = device;
     hipDeviceProp_t prop;
@@ -646,6 +676,57 @@ pado_gpu_status pado_gpu_query_wide(pado_gpu_ctx *ctx, pado_gpu_index *idx, cons
                                   return launch_query_wide(idx, rank_space, da, db, n,
                                                            reinterpret_cast<uint16_t *>(dout), ref, s);
                               });
+}
+
+pado_gpu_status pado_gpu_query_noalloc(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uint32_t *a, const uint32_t *b,
+                                       uint64_t k, uint8_t *out, uint32_t flags)
+{
+    clear_error();
+    PADO_TRY(check_ctx(ctx));
+    if (!idx || (k && (!a || !b || !out))) {
+        set_error("null argument");
+        return PADO_GPU_ERR_ARG;
+    }
+    if (flags & PADO_GPU_QUERY_DEVICE_PTRS) {
+        set_error("pado_gpu_query_noalloc takes host arrays (pado_gpu_query with device pointers allocates nothing)");
+        return PADO_GPU_ERR_ARG;
+    }
+    if (k > PADO_GPU_NOALLOC_MAX_PAIRS) {
+        set_error("pado_gpu_query_noalloc: " + std::to_string(k) + " pairs exceed the staging maximum of " +
+                  std::to_string(PADO_GPU_NOALLOC_MAX_PAIRS) + " per call");
+        return PADO_GPU_ERR_ARG;
+    }
+    if (idx->weighted || idx->wide) {
+        set_error("pado_gpu_query_noalloc answers u8 distances: not for weighted or wide-domain indexes");
+        return PADO_GPU_ERR_STATE;
+    }
+    const bool rank_space = (flags & PADO_GPU_QUERY_RANK_SPACE) != 0;
+    if (!rank_space && !idx->ordered) {
+        set_error("query before order_labels: call pado_gpu_index_order or pass PADO_GPU_QUERY_RANK_SPACE");
+        return PADO_GPU_ERR_STATE;
+    }
+    if (rank_space && !idx->loff.p) {
+        set_error("index has no rank-space labels");
+        return PADO_GPU_ERR_STATE;
+    }
+    if (k == 0) return PADO_GPU_OK;
+    void *base = ctx->device < kMaxDevices ? g_scratch[ctx->device] : nullptr;
+    if (!base) {
+        set_error("no staging array for this device (contexts on device ids >= 64 have none)");
+        return PADO_GPU_ERR_STATE;
+    }
+    // Layout: a ids [0, M), b ids [M, 2M), answers at byte 8M (M = maximum
+    // pairs).  Nothing needs clearing: each call writes what it reads.
+    uint32_t *da = static_cast<uint32_t *>(base);
+    uint32_t *db = da + PADO_GPU_NOALLOC_MAX_PAIRS;
+    uint8_t *dout = static_cast<uint8_t *>(base) + 8ull * PADO_GPU_NOALLOC_MAX_PAIRS;
+    hipStream_t st = ctx->stream;
+    PADO_HIP(hipMemcpyAsync(da, a, sizeof(uint32_t) * k, hipMemcpyHostToDevice, st));
+    PADO_HIP(hipMemcpyAsync(db, b, sizeof(uint32_t) * k, hipMemcpyHostToDevice, st));
+    PADO_TRY(launch_query(idx, rank_space, da, db, k, dout, (flags & PADO_GPU_QUERY_REF_COMPAT) != 0, st));
+    PADO_HIP(hipMemcpyAsync(out, dout, k, hipMemcpyDeviceToHost, st));
+    PADO_HIP(hipStreamSynchronize(st));
+    return PADO_GPU_OK;
 }
 
 pado_gpu_status pado_gpu_windex_export(pado_gpu_ctx *ctx, const pado_gpu_index *idx, uint64_t *label_off,

@@ -151,7 +151,7 @@ def test_wide_domain_limits_and_refusals(gpu_ctx):
         pado.Index.build(gpu_ctx, dg, wide=True)
     assert e.value.status == 4
     dg.close()
-    # A wide index refuses the u8 query and the file path.
+    # A wide index refuses the u8 query / export / file paths.
     G, rank, _ = _graph("grid", 30)
     dg = pado.DeviceGraph(gpu_ctx, G)
     W = pado.Index.build(gpu_ctx, dg, wide=True)
@@ -160,6 +160,7 @@ def test_wide_domain_limits_and_refusals(gpu_ctx):
     out = np.zeros(4, np.uint8)
     a = np.zeros(4, np.uint32)
     assert L.pado_gpu_query(gpu_ctx.h, W.h, pado._p(a), pado._p(a), 4, pado._p(out), 0, None) == 5
+    assert L.pado_gpu_query_noalloc(gpu_ctx.h, W.h, pado._p(a), pado._p(a), 4, pado._p(out), 0) == 5
     with pytest.raises(pado.PadoError):
         W.store("/tmp/pado_wide_refused.idx", rank)
     W.close()
