@@ -1119,7 +1119,7 @@ static pado_gpu_status pool_repack(hipStream_t st, uint32_t n, DevBuf<uint32_t> 
     hipLaunchKernelGGL(k_cap_for_len, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, rec.p, ncap.p, 0u);
     PADO_TRY(scan_u32_to_u64(ncap.p, nbase.p, n, st));
     uint64_t need = 0;
-    PADO_HIP(hipMemcpy(&need, nbase.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    PADO_HIP(copy_sync(&need, nbase.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     if (new_size < need * 2) new_size = need * 2;
     DevBuf<uint32_t> npool;
     PADO_TRY(npool.alloc(new_size));
@@ -1134,7 +1134,8 @@ static pado_gpu_status pool_repack(hipStream_t st, uint32_t n, DevBuf<uint32_t> 
 
 template <bool W>
 static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, const DevBuf<uint8_t> &d_used,
-                                       const pado_gpu_build_cfg &cfg)
+                                       const pado_gpu_build_cfg &cfg, unsigned long long pool_min,
+                                       unsigned long long *pool_exhausted)
 {
     using DT = typename Hop<W>::Dist;
     constexpr uint32_t kIterEnd = Hop<W>::kInf;  // the first iteration that is never run
@@ -1169,7 +1170,7 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
     // Per-wave reservations in k_add leave holes: the active list can exceed
     // the active count by kAlistGrab slots per wave, and padded descriptors
     // are fewer than the real ones plus kChunkGrab per wave.
-    const uint64_t add_waves = 4ull * (cfg.add_blocks ? cfg.add_blocks : kAddGrid);
+    const uint64_t add_waves = 4ull * (cfg.add_blocks ? cfg.add_blocks : default_grid(n));
     const uint64_t alist_cap = (uint64_t)n + kAlistGrab * add_waves;
     PADO_TRY(ag.alloc(alist_cap * kWords));
     PADO_TRY(chunks.alloc(2 * (g->m / kChunk + (uint64_t)n) + kChunkGrab * add_waves + 64));
@@ -1187,7 +1188,8 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
     // The label pool starts at 64 words per vertex and doubles whenever a
     // host check finds it over 40% full; the checks come every
     // iters_per_sync iterations, so the headroom covers several iterations.
-    unsigned long long pool_size = std::max<unsigned long long>(1ull << 22, 64ull * n);
+    unsigned long long pool_size = std::max<unsigned long long>(std::max<unsigned long long>(1ull << 22, 64ull * n),
+                                                                pool_min);
     PADO_TRY(pool.alloc(pool_size));
     // Root-distance table, one 1 KB row per hub id: the hubs of batch b's
     // roots are ids below the batch end, so that many rows suffice; grown
@@ -1212,13 +1214,17 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
     PADO_HIP(hipHostMalloc((void **)&h_ctr, 4 * sizeof(BuildCounters), hipHostMallocDefault));
     struct PinnedFree {
         BuildCounters *p;
-        ~PinnedFree() { (void)hipHostFree(p); }
+        ~PinnedFree()
+        {
+            std::lock_guard<std::recursive_mutex> g(capture_mutex());
+            (void)hipHostFree(p);
+        }
     } pin{h_ctr};
 
     const bool use_graph = (cfg.flags & PADO_GPU_BUILD_STREAM) == 0;
     const uint32_t U = cfg.iters_per_sync ? std::min<uint32_t>(cfg.iters_per_sync, 64u) : 8u;
-    const uint32_t push_grid = cfg.push_blocks ? cfg.push_blocks : kPushGrid;
-    const uint32_t add_grid = cfg.add_blocks ? cfg.add_blocks : kAddGrid;
+    const uint32_t push_grid = cfg.push_blocks ? cfg.push_blocks : default_grid(n);
+    const uint32_t add_grid = cfg.add_blocks ? cfg.add_blocks : default_grid(n);
     const uint32_t root_blocks = (batch * 64 + 255) / 256;
 
     // Stream mode brackets every push and add launch with HIP events (read
@@ -1286,6 +1292,9 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
         ~GraphFree() { f(); }
     } gxf{drop_graphs};
     auto capture = [&](uint32_t first, uint32_t count, hipGraphExec_t *out) -> pado_gpu_status {
+        // No other thread of this process allocates or frees while the
+        // stream captures (see capture_mutex).
+        std::lock_guard<std::recursive_mutex> cap_lock(capture_mutex());
         hipGraph_t gr = nullptr;
         PADO_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         if (first == 0) enqueue_prologue();
@@ -1351,6 +1360,12 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
     bool ahead = false;               // batch b's first block is already enqueued
     uint32_t ahead_count = 0, ahead_slot = 0;
     unsigned long long top_prev = 0, growth = ~0ull;  // pool use after the last batch, and its growth
+    // Pool words taken per iteration, from the last checked block (a recent
+    // maximum), and the pool top at that check: the headroom kept before the
+    // next blocks scales with it, so a fast-growing phase (labels-only
+    // pruning, early batches) cannot run the pool dry inside a block.
+    double rate = 0.0;
+    unsigned long long snap_top_prev = 0;
     for (uint32_t b = 0; b < nb; ++b) {
         const uint32_t batch_end = std::min<uint64_t>((uint64_t)(b + 1) * batch, n);
         if (batch_end > dc_rows) {
@@ -1406,20 +1421,36 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
                 }
             }
             iterations_seen = snap.iterations;
+            // Work enqueued after this block (a block ahead) still uses the
+            // buffers: it drains before an error return frees them.
             if (snap.overflow) {
+                (void)hipStreamSynchronize(st);
+                if (pool_exhausted) *pool_exhausted = pool_size;
                 set_error("label pool exhausted");
                 return PADO_GPU_ERR_NOMEM;
             }
             if (snap.domain) {
+                (void)hipStreamSynchronize(st);
                 set_error(domain_msg);
                 return PADO_GPU_ERR_DOMAIN;
             }
-            // Keep at least 60% of the pool free for the next block of
-            // iterations; with a block ahead, it runs to completion first.
-            if (snap.pool_top * 5 > pool_size * 2) {
+            if (snap.pool_top >= snap_top_prev && cur_count) {
+                const double r = (double)(snap.pool_top - snap_top_prev) / cur_count;
+                rate = std::max(r, rate * 0.75);
+            }
+            snap_top_prev = snap.pool_top;
+            // Keep at least 60% of the pool free, and room for four times the
+            // recent growth over the next blocks (this batch's next block and
+            // a block run ahead: U iterations each at most); a block ahead
+            // runs to completion first.
+            const double need = rate * 4.0 * (spec ? 2.0 * U : (double)U);
+            if (snap.pool_top * 5 > pool_size * 2 || (double)snap.pool_top + need > 0.8 * (double)pool_size) {
                 PADO_HIP(hipStreamSynchronize(st));
                 unsigned long long top = 0;
-                PADO_TRY(pool_repack(st, n, pool, lrec, pool_size * 2, &top));
+                const unsigned long long want =
+                    std::max<unsigned long long>(pool_size * 2, (unsigned long long)(2.0 * (snap.pool_top + need)));
+                PADO_TRY(pool_repack(st, n, pool, lrec, want, &top));
+                snap_top_prev = top;
                 pool_size = pool.n;
                 S.pool = pool.p;
                 S.pool_size = pool_size;
@@ -1433,6 +1464,7 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
             // The frontier of the next iteration is in slot (launched + 1) & 1.
             if (snap.n_act[(launched + 1) & 1u] == 0) break;
             if (launched >= kIterEnd) {
+                (void)hipStreamSynchronize(st);
                 set_error(domain_msg);
                 return PADO_GPU_ERR_DOMAIN;
             }
@@ -1478,7 +1510,7 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
     hipLaunchKernelGGL(k_cap_for_len, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, lrec.p, lens.p, 1u);
     PADO_TRY(scan_u32_to_u64(lens.p, idx->loff.p, n, st));
     uint64_t total = 0;
-    PADO_HIP(hipMemcpy(&total, idx->loff.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost));
+    PADO_HIP(copy_sync(&total, idx->loff.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
     idx->num_labels = total;
     PADO_TRY(idx->lab.alloc(total));
     hipLaunchKernelGGL(k_gather_labels, dim3(grid_for((uint64_t)n * 64, 256, 8192)), dim3(256), 0, st, n, pool.p,
@@ -1490,7 +1522,18 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
 pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevBuf<uint8_t> &d_used,
                               const pado_gpu_build_cfg &cfg)
 {
-    return idx->wide ? build_batches_t<true>(g, idx, d_used, cfg) : build_batches_t<false>(g, idx, d_used, cfg);
+    // A pool that still runs dry inside a block (growth far above the recent
+    // rate) costs one rerun of the batch phase from scratch with a larger
+    // pool; no label is ever dropped.
+    unsigned long long exhausted = 0, pool_min = 0;
+    for (int attempt = 0;; ++attempt) {
+        exhausted = 0;
+        const pado_gpu_status s = idx->wide ? build_batches_t<true>(g, idx, d_used, cfg, pool_min, &exhausted)
+                                            : build_batches_t<false>(g, idx, d_used, cfg, pool_min, &exhausted);
+        if (s != PADO_GPU_ERR_NOMEM || !exhausted || attempt == 2) return s;
+        pool_min = exhausted * 4;
+        (void)hipStreamSynchronize(g->ctx->stream);
+    }
 }
 
 }  // namespace pado

@@ -212,7 +212,11 @@ pado_gpu_status build_bp(pado_gpu_graph *g, pado_gpu_index *idx, DevBuf<uint8_t>
     PADO_HIP(hipHostMalloc((void **)&h_cnt, 2 * sizeof(uint32_t), hipHostMallocDefault));
     struct PinnedFree {
         uint32_t *p;
-        ~PinnedFree() { (void)hipHostFree(p); }
+        ~PinnedFree()
+        {
+            std::lock_guard<std::recursive_mutex> g(capture_mutex());
+            (void)hipHostFree(p);
+        }
     } pin{h_cnt};
 
     const uint32_t edge_grid = 2048;

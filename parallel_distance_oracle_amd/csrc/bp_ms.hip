@@ -238,7 +238,11 @@ pado_gpu_status build_bp_seeded(pado_gpu_graph *g, pado_gpu_index *idx, DevBuf<u
     PADO_HIP(hipHostMalloc((void **)&h_cnt, 4 * sizeof(uint32_t), hipHostMallocDefault));
     struct PinnedFree {
         uint32_t *p;
-        ~PinnedFree() { (void)hipHostFree(p); }
+        ~PinnedFree()
+        {
+            std::lock_guard<std::recursive_mutex> g(capture_mutex());
+            (void)hipHostFree(p);
+        }
     } pin{h_cnt};
 
     ulonglong2 *bps = (ulonglong2 *)idx->bps.p;

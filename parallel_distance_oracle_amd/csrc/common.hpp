@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -45,6 +46,10 @@ constexpr uint32_t kWideMaxVertices = Hop<true>::kMaxN;
 // round of waves at their occupancy, so a launch with little work ends fast.
 constexpr uint32_t kPushGrid = 1024;
 constexpr uint32_t kAddGrid = 1024;
+// Graphs up to 2^17 vertices rarely have more than a few thousand active
+// vertices or candidates per iteration: 256 blocks serve them faster (a 40K
+// grid: 0.18 s vs 0.22 s with 1024), and concurrent builds share the chip.
+inline uint32_t default_grid(uint32_t n) { return n <= (1u << 17) ? 256u : 1024u; }
 
 // Thread-local error text behind pado_gpu_last_error().
 void set_error(const std::string &msg);
@@ -66,6 +71,15 @@ void clear_error();
         if (s_ != PADO_GPU_OK) return s_;               \
     } while (0)
 
+// Serialises device allocation and release with hipGraph capture: builds of a
+// many-graph job run on several threads, and a release may synchronise the
+// device while another thread's stream is capturing.
+inline std::recursive_mutex &capture_mutex()
+{
+    static std::recursive_mutex m;
+    return m;
+}
+
 // Device buffer owned by a handle.
 template <class T>
 struct DevBuf {
@@ -75,6 +89,7 @@ struct DevBuf {
     {
         release();
         if (count == 0) count = 1;
+        std::lock_guard<std::recursive_mutex> g(capture_mutex());
         hipError_t e = hipMalloc((void **)&p, count * sizeof(T));
         if (e != hipSuccess) {
             p = nullptr;
@@ -86,7 +101,10 @@ struct DevBuf {
     }
     void release()
     {
-        if (p) (void)hipFree(p);
+        if (p) {
+            std::lock_guard<std::recursive_mutex> g(capture_mutex());
+            (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
     }

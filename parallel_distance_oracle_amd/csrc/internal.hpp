@@ -44,6 +44,15 @@ constexpr uint64_t kSortMaxItems = 0x7FFFFFFFull - 1024;
 pado_gpu_status segmented_sort_u32(uint32_t *keys, uint64_t total, const uint64_t *off, uint32_t n,
                                    hipStream_t s, uint64_t max_items = 0);
 
+// A blocking copy on `s` (never the legacy default stream, which would have
+// to wait for -- or fail against -- other threads' streams and captures).
+inline hipError_t copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind, hipStream_t s)
+{
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e;
+}
+
 // Launch geometry helpers.
 inline uint32_t grid_for(uint64_t threads, uint32_t block, uint32_t cap = 1u << 20)
 {

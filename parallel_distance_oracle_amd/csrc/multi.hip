@@ -12,15 +12,22 @@
 // union orders vertices by (local rank, graph), so batch 0 holds the top
 // root of every graph and every batch carries roots of many graphs.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
+#include <mutex>
 #include <numeric>
+#include <thread>
 
 #include "internal.hpp"
 
 using namespace pado;
 
 namespace {
+
+// Concurrent builds of one job: one stream each; the device exposes four
+// hardware queues per process by default, so more streams would share them.
+constexpr uint32_t kMultiWorkers = 4;
 
 struct GraphSlot {
     uint32_t n = 0;
@@ -146,6 +153,118 @@ struct pado_gpu_multi {
     }
 };
 
+namespace {
+
+// One union of graphs inside the u8 domain, labeled as one interleaved graph
+// on the worker's context: ids ordered by (local rank, graph), each graph with
+// its own 50 bit-parallel slots.
+pado_gpu_status build_union(pado_gpu_ctx *wctx, pado_gpu_multi *mi, const std::vector<uint32_t> &grp,
+                            const uint64_t *off, const uint32_t *adj, const pado_gpu_build_cfg *cfg, Union &U)
+{
+    uint32_t maxn = 0;
+    uint64_t total = 0, medges = 0;
+    for (uint32_t g : grp) {
+        maxn = std::max(maxn, mi->graphs[g].n);
+        total += mi->graphs[g].n;
+        medges += off[mi->graphs[g].voff + mi->graphs[g].n];
+    }
+    U.owner.resize(total);
+    U.local.resize(total);
+    uint32_t next = 0;
+    for (uint32_t g : grp) mi->graphs[g].gid.assign(mi->graphs[g].n, 0);
+    for (uint32_t r = 0; r < maxn; ++r)
+        for (uint32_t g : grp)
+            if (r < mi->graphs[g].n) {
+                mi->graphs[g].gid[r] = next;
+                U.owner[next] = g;
+                U.local[next] = r;
+                ++next;
+            }
+    // Union CSR: relabelled adjacency keeps each list descending because the
+    // relabelling preserves the order inside a graph.
+    std::vector<uint64_t> uoff(total + 1, 0);
+    std::vector<uint32_t> uadj(medges);
+    for (uint32_t x = 0; x < total; ++x) {
+        const GraphSlot &G = mi->graphs[U.owner[x]];
+        const uint64_t *o = off + G.voff;
+        const uint32_t v = U.local[x];
+        uoff[x + 1] = uoff[x] + (o[v + 1] - o[v]);
+    }
+    for (uint32_t x = 0; x < total; ++x) {
+        const GraphSlot &G = mi->graphs[U.owner[x]];
+        const uint64_t *o = off + G.voff;
+        const uint32_t *a = adj + G.eoff;
+        const uint32_t v = U.local[x];
+        uint64_t w = uoff[x];
+        for (uint64_t e = o[v]; e < o[v + 1]; ++e) uadj[w++] = G.gid[a[e]];
+    }
+    // Per-graph bit-parallel seeds, renamed into union ids.
+    BpSeeds seeds;
+    std::vector<uint8_t> used(total, 0), lused;
+    for (uint32_t g : grp) {
+        const GraphSlot &G = mi->graphs[g];
+        select_bp_roots(G.n, off + G.voff, adj + G.eoff, lused, nullptr, seeds, G.gid.data());
+        for (uint32_t v = 0; v < G.n; ++v)
+            if (lused[v]) used[G.gid[v]] = 1;
+    }
+    pado_gpu_graph *ug = nullptr;
+    PADO_TRY(pado_gpu_graph_upload(wctx, (uint32_t)total, medges, uoff.data(), uadj.data(), nullptr, &ug));
+    auto *idx = new pado_gpu_index();
+    idx->ctx = wctx;
+    idx->n = (uint32_t)total;
+    for (auto &r : idx->bp_roots) r = kEmpty;
+    DevBuf<uint8_t> d_used;
+    const auto tb = std::chrono::steady_clock::now();
+    pado_gpu_status s = build_bp_seeded(ug, idx, d_used, used, seeds);
+    idx->ms_bp = ms_since(tb);
+    if (s == PADO_GPU_OK) {
+        const auto tc = std::chrono::steady_clock::now();
+        pado_gpu_build_cfg c{};
+        if (cfg) c = *cfg;
+        s = build_batches(ug, idx, d_used, c);
+        idx->ms_batch = ms_since(tc);
+    }
+    pado_gpu_graph_free(ug);
+    if (s != PADO_GPU_OK) {
+        delete idx;
+        return s;
+    }
+    U.loff_host.resize(total + 1);
+    if (copy_sync(U.loff_host.data(), idx->loff.p, 8 * (total + 1), hipMemcpyDeviceToHost, wctx->stream) != hipSuccess) {
+        delete idx;
+        set_error("label offset copy failed");
+        return PADO_GPU_ERR_HIP;
+    }
+    U.idx = idx;
+    return PADO_GPU_OK;
+}
+
+// A graph outside the proven range, on its own.  Vertex 0 is the first
+// bit-parallel root and its BFS runs to its eccentricity, so an eccentricity
+// >= 255 means the reference's u8 domain is left: such a graph is built in
+// the wide hop domain (12-bit distances) at once.  The others try the u8
+// domain first and fall back to the wide one if a batch iteration reaches
+// 255.  Only a graph the wide domain cannot hold (more than 2^20 vertices or
+// distances past 4094) reports ERR_DOMAIN.
+pado_gpu_status build_alone(pado_gpu_ctx *wctx, GraphSlot &G, const uint64_t *off, const uint32_t *adj,
+                            const pado_gpu_build_cfg *cfg)
+{
+    pado_gpu_build_cfg base{};
+    if (cfg) base = *cfg;
+    pado_gpu_build_cfg wide_cfg = base;
+    wide_cfg.flags |= PADO_GPU_BUILD_WIDE;
+    const bool deep = eccentricity(G.n, off + G.voff, adj + G.eoff, 0) >= kInf8;
+    pado_gpu_graph *gg = nullptr;
+    PADO_TRY(pado_gpu_graph_upload(wctx, G.n, off[G.voff + G.n], off + G.voff, adj + G.eoff, nullptr, &gg));
+    G.status = pado_gpu_build(wctx, gg, deep ? &wide_cfg : &base, &G.alone);
+    if (G.status == PADO_GPU_ERR_DOMAIN && !deep) G.status = pado_gpu_build(wctx, gg, &wide_cfg, &G.alone);
+    pado_gpu_graph_free(gg);
+    if (G.status != PADO_GPU_OK && G.status != PADO_GPU_ERR_DOMAIN) return G.status;
+    return PADO_GPU_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 pado_gpu_status pado_gpu_build_multi(pado_gpu_ctx *ctx, uint32_t num_graphs, const uint32_t *n, const uint64_t *off,
@@ -185,153 +304,131 @@ pado_gpu_status pado_gpu_build_multi(pado_gpu_ctx *ctx, uint32_t num_graphs, con
         set_error("null adjacency");
         return PADO_GPU_ERR_ARG;
     }
-
-    // Partition: graphs inside the u8 domain go into unions of < 2^24
-    // vertices and < 2^32 directed edges; the others are built alone.
-    std::vector<std::vector<uint32_t>> groups(1);
-    uint64_t gv = 0, ge = 0;
     for (uint32_t g = 0; g < num_graphs; ++g) {
         const GraphSlot &G = mi->graphs[g];
         const uint64_t *o = off + G.voff;
-        const uint64_t m = o[G.n];
-        if (distance_bound(G.n, o, adj + G.eoff) == UINT32_MAX) continue;
-        if (gv + G.n >= kMaxVertices || ge + m >= (1ull << 32)) {
-            groups.emplace_back();
-            gv = ge = 0;
-        }
-        groups.back().push_back(g);
-        gv += G.n;
-        ge += m;
+        for (uint64_t e = 0; e < o[G.n]; ++e)
+            if (adj[G.eoff + e] >= G.n) {
+                set_error("adjacency id out of range in graph " + std::to_string(g));
+                delete mi;
+                return PADO_GPU_ERR_ARG;
+            }
     }
 
-    pado_gpu_status s = PADO_GPU_OK;
-    for (auto &grp : groups) {
-        if (grp.empty()) continue;
-        Union U;
-        // Interleave: order (local rank, graph).
-        uint32_t maxn = 0;
-        uint64_t total = 0, medges = 0;
-        for (uint32_t g : grp) {
-            maxn = std::max(maxn, mi->graphs[g].n);
-            total += mi->graphs[g].n;
-            medges += off[mi->graphs[g].voff + mi->graphs[g].n];
-        }
-        U.owner.resize(total);
-        U.local.resize(total);
-        uint32_t next = 0;
-        for (uint32_t g : grp) mi->graphs[g].gid.assign(mi->graphs[g].n, 0);
-        for (uint32_t r = 0; r < maxn; ++r)
-            for (uint32_t g : grp)
-                if (r < mi->graphs[g].n) {
-                    mi->graphs[g].gid[r] = next;
-                    U.owner[next] = g;
-                    U.local[next] = r;
-                    ++next;
+    // Workers: each task runs on its own context (stream), so the unions and
+    // the graphs built alone -- independent work, each a latency-bound chain
+    // of small launches -- overlap on the chip.
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const uint32_t workers = std::max(1u, std::min<uint32_t>(kMultiWorkers, hw));
+
+    // Graphs inside the u8 domain (2 x one vertex's eccentricity per component
+    // below 255) go into `workers` unions balanced by vertex count (each below
+    // 2^24 vertices and 2^32 directed edges); the others are built alone.
+    std::vector<uint8_t> in_domain(num_graphs, 0);
+    {
+        std::atomic<uint32_t> next{0};
+        std::vector<std::thread> th;
+        for (uint32_t w = 0; w < std::min<uint32_t>(hw, 16u); ++w)
+            th.emplace_back([&] {
+                for (uint32_t g; (g = next.fetch_add(1)) < num_graphs;) {
+                    const GraphSlot &G = mi->graphs[g];
+                    in_domain[g] = distance_bound(G.n, off + G.voff, adj + G.eoff) != UINT32_MAX;
                 }
-        // Union CSR: relabelled adjacency keeps each list descending because
-        // the relabelling preserves the order inside a graph.
-        std::vector<uint64_t> uoff(total + 1, 0);
-        std::vector<uint32_t> uadj(medges);
-        for (uint32_t x = 0; x < total; ++x) {
-            const GraphSlot &G = mi->graphs[U.owner[x]];
-            const uint64_t *o = off + G.voff;
-            const uint32_t v = U.local[x];
-            uoff[x + 1] = uoff[x] + (o[v + 1] - o[v]);
+            });
+        for (auto &t : th) t.join();
+    }
+    std::vector<uint32_t> order;
+    for (uint32_t g = 0; g < num_graphs; ++g)
+        if (in_domain[g]) order.push_back(g);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t x, uint32_t y) { return mi->graphs[x].n > mi->graphs[y].n; });
+    const size_t k0 = std::min<size_t>(workers, order.size());
+    std::vector<std::vector<uint32_t>> groups(k0);
+    std::vector<uint64_t> gv(k0, 0), ge(k0, 0);
+    for (uint32_t g : order) {
+        const uint64_t m = off[mi->graphs[g].voff + mi->graphs[g].n];
+        // The lightest union that can still take the graph; a new one if none can.
+        int best = -1;
+        for (size_t k = 0; k < groups.size(); ++k)
+            if (gv[k] + mi->graphs[g].n < kMaxVertices && ge[k] + m < (1ull << 32) &&
+                (best < 0 || gv[k] < gv[best]))
+                best = (int)k;
+        if (best < 0) {
+            groups.emplace_back();
+            gv.push_back(0);
+            ge.push_back(0);
+            best = (int)groups.size() - 1;
         }
-        for (uint32_t x = 0; x < total; ++x) {
-            const GraphSlot &G = mi->graphs[U.owner[x]];
-            const uint64_t *o = off + G.voff;
-            const uint32_t *a = adj + G.eoff;
-            const uint32_t v = U.local[x];
-            uint64_t w = uoff[x];
-            for (uint64_t e = o[v]; e < o[v + 1]; ++e) {
-                if (a[e] >= G.n) {
-                    set_error("adjacency id out of range in graph " + std::to_string(U.owner[x]));
-                    delete mi;
-                    return PADO_GPU_ERR_ARG;
+        groups[best].push_back(g);
+        gv[best] += mi->graphs[g].n;
+        ge[best] += m;
+    }
+    // Each union lists its graphs in index order (the interleaving order).
+    for (auto &grp : groups) std::sort(grp.begin(), grp.end());
+    mi->unions.resize(groups.size());
+
+    // Tasks: unions first (largest first), then the graphs built alone
+    // (largest first).
+    struct Task {
+        int union_id;   // >= 0: a union, else -1
+        uint32_t graph; // a graph built alone
+        uint64_t size;
+    };
+    std::vector<Task> tasks;
+    for (size_t k = 0; k < groups.size(); ++k) tasks.push_back({(int)k, 0u, gv[k]});
+    std::vector<Task> alone;
+    for (uint32_t g = 0; g < num_graphs; ++g)
+        if (!in_domain[g]) alone.push_back({-1, g, mi->graphs[g].n});
+    std::stable_sort(tasks.begin(), tasks.end(), [](const Task &a, const Task &b) { return a.size > b.size; });
+    std::stable_sort(alone.begin(), alone.end(), [](const Task &a, const Task &b) { return a.size > b.size; });
+    tasks.insert(tasks.end(), alone.begin(), alone.end());
+
+    std::atomic<size_t> next_task{0};
+    std::atomic<int> failed{0};
+    std::mutex err_mu;
+    pado_gpu_status first_status = PADO_GPU_OK;
+    std::string first_error;
+    auto worker = [&]() {
+        pado_gpu_ctx *wctx = nullptr;
+        pado_gpu_status s = pado_gpu_ctx_create(ctx->device, &wctx);
+        for (size_t t; s == PADO_GPU_OK && !failed.load() && (t = next_task.fetch_add(1)) < tasks.size();) {
+            const Task &T = tasks[t];
+            if (T.union_id >= 0) {
+                s = build_union(wctx, mi, groups[T.union_id], off, adj, cfg, mi->unions[T.union_id]);
+                if (s == PADO_GPU_OK) {
+                    mi->unions[T.union_id].idx->ctx = ctx;  // the job's context outlives the worker's
+                    for (uint32_t g : groups[T.union_id]) mi->graphs[g].union_id = T.union_id;
                 }
-                uadj[w++] = G.gid[a[e]];
+            } else {
+                GraphSlot &G = mi->graphs[T.graph];
+                s = build_alone(wctx, G, off, adj, cfg);
+                if (G.alone) G.alone->ctx = ctx;
             }
         }
-        // Per-graph bit-parallel seeds, renamed into union ids.
-        BpSeeds seeds;
-        std::vector<uint8_t> used(total, 0), lused;
-        for (uint32_t g : grp) {
-            const GraphSlot &G = mi->graphs[g];
-            select_bp_roots(G.n, off + G.voff, adj + G.eoff, lused, nullptr, seeds, G.gid.data());
-            for (uint32_t v = 0; v < G.n; ++v)
-                if (lused[v]) used[G.gid[v]] = 1;
-        }
-        pado_gpu_graph *ug = nullptr;
-        s = pado_gpu_graph_upload(ctx, (uint32_t)total, medges, uoff.data(), uadj.data(), nullptr, &ug);
-        if (s != PADO_GPU_OK) break;
-        auto *idx = new pado_gpu_index();
-        idx->ctx = ctx;
-        idx->n = (uint32_t)total;
-        for (auto &r : idx->bp_roots) r = kEmpty;
-        DevBuf<uint8_t> d_used;
-        const auto tb = std::chrono::steady_clock::now();
-        s = build_bp_seeded(ug, idx, d_used, used, seeds);
-        idx->ms_bp = ms_since(tb);
-        if (s == PADO_GPU_OK) {
-            const auto tc = std::chrono::steady_clock::now();
-            pado_gpu_build_cfg c{};
-            if (cfg) c = *cfg;
-            s = build_batches(ug, idx, d_used, c);
-            idx->ms_batch = ms_since(tc);
-        }
-        pado_gpu_graph_free(ug);
         if (s != PADO_GPU_OK) {
-            delete idx;
-            break;
+            std::lock_guard<std::mutex> g(err_mu);
+            if (first_status == PADO_GPU_OK) {
+                first_status = s;
+                first_error = pado_gpu_last_error();
+            }
+            failed.store(1);
         }
-        U.idx = idx;
-        U.loff_host.resize(total + 1);
-        if (hipMemcpy(U.loff_host.data(), idx->loff.p, 8 * (total + 1), hipMemcpyDeviceToHost) != hipSuccess) {
-            delete idx;
-            set_error("label offset copy failed");
-            s = PADO_GPU_ERR_HIP;
-            break;
+        if (wctx) {
+            (void)hipStreamSynchronize(wctx->stream);
+            pado_gpu_ctx_destroy(wctx);
         }
-        const int uid = (int)mi->unions.size();
-        for (uint32_t g : grp) mi->graphs[g].union_id = uid;
-        mi->union_vertices += total;
-        mi->unions.push_back(std::move(U));
-    }
-    if (s != PADO_GPU_OK) {
+    };
+    std::vector<std::thread> pool;
+    const uint32_t nthreads = std::max<uint32_t>(1, std::min<uint32_t>(workers, (uint32_t)tasks.size()));
+    for (uint32_t w = 0; w < nthreads; ++w) pool.emplace_back(worker);
+    for (auto &t : pool) t.join();
+    (void)hipSetDevice(ctx->device);
+    if (first_status != PADO_GPU_OK) {
         delete mi;
-        return s;
+        set_error(first_error);
+        return first_status;
     }
-    // Graphs outside the proven range: on their own.  Vertex 0 is the first
-    // bit-parallel root and its BFS runs to its eccentricity, so an
-    // eccentricity >= 255 means the reference's u8 domain is left: such a
-    // graph is built in the wide hop domain (12-bit distances) at once.  The
-    // others try the u8 domain first and fall back to the wide one if a batch
-    // iteration reaches 255.  Only a graph the wide domain cannot hold (more
-    // than 2^20 vertices or distances past 4094) reports ERR_DOMAIN.
-    pado_gpu_build_cfg base{};
-    if (cfg) base = *cfg;
-    pado_gpu_build_cfg wide_cfg = base;
-    wide_cfg.flags |= PADO_GPU_BUILD_WIDE;
-    for (uint32_t g = 0; g < num_graphs; ++g) {
-        GraphSlot &G = mi->graphs[g];
-        if (G.union_id >= 0) continue;
-        const bool deep = eccentricity(G.n, off + G.voff, adj + G.eoff, 0) >= kInf8;
-        pado_gpu_graph *gg = nullptr;
-        s = pado_gpu_graph_upload(ctx, G.n, off[G.voff + G.n], off + G.voff, adj + G.eoff, nullptr, &gg);
-        if (s != PADO_GPU_OK) {
-            delete mi;
-            return s;
-        }
-        G.status = pado_gpu_build(ctx, gg, deep ? &wide_cfg : &base, &G.alone);
-        if (G.status == PADO_GPU_ERR_DOMAIN && !deep) G.status = pado_gpu_build(ctx, gg, &wide_cfg, &G.alone);
-        pado_gpu_graph_free(gg);
-        if (G.status != PADO_GPU_OK && G.status != PADO_GPU_ERR_DOMAIN) {
-            s = G.status;
-            delete mi;
-            return s;
-        }
-    }
+    for (const Union &U : mi->unions) mi->union_vertices += U.owner.size();
     mi->ms_total = ms_since(t0);
     *out = mi;
     return PADO_GPU_OK;

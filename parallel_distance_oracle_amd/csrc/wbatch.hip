@@ -761,7 +761,11 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
     std::memset(h, 0, sizeof(WCounters));
     struct PinnedFree {
         WCounters *p;
-        ~PinnedFree() { (void)hipHostFree(p); }
+        ~PinnedFree()
+        {
+            std::lock_guard<std::recursive_mutex> g(capture_mutex());
+            (void)hipHostFree(p);
+        }
     } pin{h};
 
     WStore S{pool.p, base.p, len.p, cap.p, pool_size};
@@ -824,7 +828,7 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
             // most 2.5 * live + new + 8 * touched words.
             const u64 added = h->labels - labels_before;
             std::vector<uint32_t> hl(n);
-            PADO_HIP(hipMemcpy(hl.data(), len.p, 4ull * n, hipMemcpyDeviceToHost));
+            PADO_HIP(copy_sync(hl.data(), len.p, 4ull * n, hipMemcpyDeviceToHost, st));
             std::vector<uint64_t> no(n + 1, 0);
             std::vector<uint32_t> nc(n);
             for (uint32_t v = 0; v < n; ++v) {
@@ -836,20 +840,20 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
             DevBuf<uint64_t> dno;
             PADO_TRY(np.alloc(nsize));
             PADO_TRY(dno.alloc((uint64_t)n + 1));
-            PADO_HIP(hipMemcpy(dno.p, no.data(), 8ull * (n + 1), hipMemcpyHostToDevice));
+            PADO_HIP(copy_sync(dno.p, no.data(), 8ull * (n + 1), hipMemcpyHostToDevice, st));
             hipLaunchKernelGGL(k_w_move, dim3(grid_for((uint64_t)n * 64, 256, 8192)), dim3(256), 0, st, n, pool.p,
                                base.p, len.p, dno.p, np.p);
             PADO_HIP(hipStreamSynchronize(st));
             std::vector<u64> nb64(no.begin(), no.end() - 1);
-            PADO_HIP(hipMemcpy(base.p, nb64.data(), 8ull * n, hipMemcpyHostToDevice));
-            PADO_HIP(hipMemcpy(cap.p, nc.data(), 4ull * n, hipMemcpyHostToDevice));
+            PADO_HIP(copy_sync(base.p, nb64.data(), 8ull * n, hipMemcpyHostToDevice, st));
+            PADO_HIP(copy_sync(cap.p, nc.data(), 4ull * n, hipMemcpyHostToDevice, st));
             std::swap(pool.p, np.p);
             std::swap(pool.n, np.n);
             pool_size = nsize;
             S.pool = pool.p;
             S.pool_size = pool_size;
             const u64 top = no[n];
-            PADO_HIP(hipMemcpy(&ctr.p->pool_top, &top, 8, hipMemcpyHostToDevice));
+            PADO_HIP(copy_sync(&ctr.p->pool_top, &top, 8, hipMemcpyHostToDevice, st));
         }
         hipLaunchKernelGGL(k_w_commit, dim3(2048), dim3(256), 0, st, rs, tlist.p, T.p, I.p, S, ctr.p);
         hipLaunchKernelGGL(k_w_batch_end, dim3(2048), dim3(256), 0, st, tlist.p, T.p, P.p, hslot.p, hlist.p, ctr.p);
@@ -873,7 +877,7 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
     PADO_TRY(idx->loff.alloc((uint64_t)n + 1));
     PADO_TRY(scan_u32_to_u64(len.p, idx->loff.p, n, st));
     uint64_t total = 0;
-    PADO_HIP(hipMemcpy(&total, idx->loff.p + n, 8, hipMemcpyDeviceToHost));
+    PADO_HIP(copy_sync(&total, idx->loff.p + n, 8, hipMemcpyDeviceToHost, st));
     idx->num_labels = total;
     DevBuf<u64> packed;
     PADO_TRY(packed.alloc(total));

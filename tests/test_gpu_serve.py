@@ -212,32 +212,36 @@ def test_cancel_and_wait_timeout(gpu_ctx, served):
         a = (np.arange(4096, dtype=np.uint32) * 7) % G.num_v
         b = (np.arange(4096, dtype=np.uint32) * 13) % G.num_v
         exp = idx.query(a, b)
+        import time
+
+        def submit_retry(x, y):
+            # A cancelled request that was already launched frees its ring
+            # space only when its launch ends: retry briefly on Busy.
+            for _ in range(5000):
+                try:
+                    return srv.submit(x, y)
+                except pado.Busy:
+                    time.sleep(0.001)
+            raise AssertionError("ring space of cancelled requests was not released")
+
         # Cancel right after submit (queued, launched or done): the whole ring
         # must become free again.
         for _ in range(5):
-            r = srv.submit(a[:3000], b[:3000])
+            r = submit_retry(a[:3000], b[:3000])
             srv.cancel(r)
             with pytest.raises(pado.PadoError):
                 srv.cancel(r)  # unknown id now
             with pytest.raises(pado.PadoError):
                 srv.wait(r, 3000)
-        for _ in range(2000):  # a launched cancelled request frees its space when its launch ends
-            try:
-                r = srv.submit(a, b)
-                break
-            except pado.Busy:
-                import time
-                time.sleep(0.001)
-        else:
-            raise AssertionError("ring space of cancelled requests was not released")
+        r = submit_retry(a, b)
         assert np.array_equal(srv.wait(r, 4096)[0], exp)
         # A done request that is cancelled discards its answers.
-        r = srv.submit(a[:100], b[:100])
+        r = submit_retry(a[:100], b[:100])
         while not srv.poll(r):
             pass
         srv.cancel(r)
         # Timed waits: poll until done, then the answers are the plain ones.
-        r = srv.submit(a, b)
+        r = submit_retry(a, b)
         got = None
         for _ in range(100000):
             try:
@@ -246,7 +250,7 @@ def test_cancel_and_wait_timeout(gpu_ctx, served):
             except pado.Timeout:
                 pass
         assert got is not None and np.array_equal(got, exp)
-        r = srv.submit(a[:10], b[:10])
+        r = submit_retry(a[:10], b[:10])
         assert np.array_equal(srv.wait(r, 10, timeout_us=10_000_000)[0], exp[:10])
     finally:
         srv.stop()
