@@ -39,23 +39,18 @@ public:
     idi *out_degrees = nullptr;
 
     Graph() = default;
+    // The text edge list, parsed on all host cores with the reference's line
+    // rules (pado_read_edge_text).
     explicit Graph(const char *filename)
     {
-        std::ifstream ifin(filename);
-        if (!ifin.is_open()) {
+        uint32_t *flat = nullptr;
+        uint64_t m = 0;
+        if (pado_read_edge_text(filename, 2, &flat, &m) != 0) {
             fprintf(stderr, "Error: cannot open file %s\n", filename);
             exit(EXIT_FAILURE);
         }
-        std::string line;
-        idi head = 0, tail = 0;
-        std::vector<std::pair<idi, idi>> edge_list;
-        while (getline(ifin, line)) {
-            if (line[0] == '#' || line[0] == '%') continue;
-            std::istringstream lin(line);
-            lin >> head >> tail;
-            edge_list.push_back(std::make_pair(head, tail));
-        }
-        construct(edge_list);
+        construct_flat(flat, m);
+        pado_free(flat);
     }
     explicit Graph(std::vector<std::pair<idi, idi>> &edge_list) { construct(edge_list); }
     Graph(const Graph &) = delete;
@@ -78,13 +73,19 @@ public:
             flat[2 * i] = edge_list[i].first;
             flat[2 * i + 1] = edge_list[i].second;
         }
-        num_e = (idi)(2 * edge_list.size());
-        num_v = edge_list.empty() ? 0 : pado_graph_num_v(flat.data(), edge_list.size());
+        construct_flat(flat.data(), edge_list.size());
+    }
+
+    // Graph::construct (includes/graph.h:125-153) on m (head, tail) pairs.
+    void construct_flat(const uint32_t *flat, uint64_t m)
+    {
+        num_e = (idi)(2 * m);
+        num_v = m == 0 ? 0 : pado_graph_num_v(flat, m);
         std::vector<uint64_t> off((size_t)num_v + 1, 0);
         vertices = (idi *)malloc(sizeof(idi) * (num_v ? num_v : 1));
         out_edges = (idi *)malloc(sizeof(idi) * (num_e ? num_e : 1));
         out_degrees = (idi *)malloc(sizeof(idi) * (num_v ? num_v : 1));
-        if (num_v) pado_graph_csr(flat.data(), edge_list.size(), num_v, off.data(), out_edges);
+        if (num_v) pado_graph_csr(flat, m, num_v, off.data(), out_edges);
         for (idi v = 0; v < num_v; ++v) {
             vertices[v] = (idi)off[v];
             out_degrees[v] = (idi)(off[v + 1] - off[v]);
@@ -137,26 +138,23 @@ public:
     weighti *out_weights = nullptr;
 
     WeightedGraph() = default;
-    // "head tail weight" lines; '#' / '%' lines skipped (includes/graph.h:277-304).
+    // "head tail weight" lines (includes/graph.h:277-304), parsed on all host cores with the reference's line rules
+    // (pado_read_edge_text); weights cast to weighti as the reference does.
     explicit WeightedGraph(const char *filename)
     {
-        std::ifstream ifin(filename);
-        if (!ifin.is_open()) {
+        uint32_t *rows = nullptr;
+        uint64_t m = 0;
+        if (pado_read_edge_text(filename, 3, &rows, &m) != 0) {
             fprintf(stderr, "Error: cannot open file %s\n", filename);
             exit(EXIT_FAILURE);
         }
-        std::string line;
-        idi head = 0, tail = 0;
-        inti weight = 0;
-        std::vector<std::pair<idi, idi>> edge_list;
-        std::vector<weighti> weight_list;
-        while (getline(ifin, line)) {
-            if (line[0] == '#' || line[0] == '%') continue;
-            std::istringstream lin(line);
-            lin >> head >> tail >> weight;
-            edge_list.push_back(std::make_pair(head, tail));
-            weight_list.push_back(static_cast<weighti>(weight));
+        std::vector<std::pair<idi, idi>> edge_list(m);
+        std::vector<weighti> weight_list(m);
+        for (uint64_t i = 0; i < m; ++i) {
+            edge_list[i] = std::make_pair(rows[3 * i], rows[3 * i + 1]);
+            weight_list[i] = static_cast<weighti>(rows[3 * i + 2]);
         }
+        pado_free(rows);
         construct(edge_list, weight_list);
     }
     WeightedGraph(const std::vector<std::pair<idi, idi>> &edge_list, const std::vector<weighti> &weight_list)
@@ -238,6 +236,24 @@ public:
                 printf("%u %u %u\n", h, out_edges[e], (unsigned)out_weights[e]);
     }
 };
+
+// The reference's binary edge list (distributed_version/tools/graph-convert/
+// edgelist2binary.cpp:28-91: u32 num_v, u64 num_e, then the pairs) as the
+// edge list Graph(std::vector<pair>) takes; exits on a missing or malformed
+// file, as the reference's readers do.
+inline std::vector<std::pair<idi, idi>> read_binary_edge_list(const char *filename)
+{
+    uint32_t *flat = nullptr;
+    uint64_t m = 0;
+    if (pado_read_edge_binary(filename, &flat, &m, nullptr) != 0) {
+        fprintf(stderr, "Error: cannot read binary edge list %s\n", filename);
+        exit(EXIT_FAILURE);
+    }
+    std::vector<std::pair<idi, idi>> edge_list(m);
+    for (uint64_t i = 0; i < m; ++i) edge_list[i] = std::make_pair(flat[2 * i], flat[2 * i + 1]);
+    pado_free(flat);
+    return edge_list;
+}
 
 }  // namespace PADO
 

@@ -82,6 +82,23 @@ void pado_wid_transfer(uint32_t n, const uint64_t *off, const uint32_t *adj, con
                        const uint32_t *rank, uint64_t *off_new, uint32_t *adj_new, uint8_t *w_new,
                        uint32_t *rank2id);
 
+/* Edge-list files.  pado_read_edge_text reads a text edge list the way
+ * Graph(const char*) (ncols = 2, GR:95-118) or WeightedGraph(const char*)
+ * (ncols = 3, GR:277-304) does -- '#' / '%' lines skipped, one edge per other
+ * line, malformed lines decided by the reference's istringstream extraction
+ * (a missing field keeps the previous line's value) -- memory-mapped and
+ * parsed on all host cores.  *out receives m x ncols u32 values (malloc'd;
+ * release with pado_free; weights as read, before the u8 cast).
+ * pado_read_edge_binary reads the reference's binary edge list
+ * (distributed_version/tools/graph-convert/edgelist2binary.cpp:28-91: u32
+ * num_v, u64 num_e, num_e u32 pairs); pado_write_edge_binary writes one.
+ * Return 0, or -1 (cannot open), -2 (malformed binary), -3 (bad argument),
+ * -4 (out of memory). */
+int pado_read_edge_text(const char *path, uint32_t ncols, uint32_t **out, uint64_t *m);
+int pado_read_edge_binary(const char *path, uint32_t **out, uint64_t *m, uint32_t *num_v);
+int pado_write_edge_binary(const char *path, const uint32_t *edges, uint64_t m);
+void pado_free(void *p);
+
 /* Vertex ranges of the final canonical label sort (order_labels' per-vertex
  * sort, PARA:2240, done on the device in calls of bounded size): cuts[0] = 0
  * < ... < cuts[r] = n, each range [cuts[i], cuts[i+1]) holding at most

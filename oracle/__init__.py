@@ -454,3 +454,42 @@ def run_ref_weighted(edges, weights, rank=None, pairs=None, threads=1):
         if pairs is not None:
             res["answers"] = np.fromfile(pre + ".answers.bin", np.int16)
     return res
+
+
+def ref_text_csr(path, weighted=False):
+    """The reference's own text reader (Graph(const char*), GR:95-118, or
+    WeightedGraph(const char*), GR:277-304) on `path`: (num_v, off, adj[, w])
+    of the CSR it builds."""
+    if not have_ref():
+        raise FileNotFoundError(REF_BIN)
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "csr.bin")
+        subprocess.run([REF_BIN, "wtext" if weighted else "text", str(path), out], check=True,
+                       capture_output=True)
+        with open(out, "rb") as f:
+            n = int(np.frombuffer(f.read(4), np.uint32)[0])
+            m = int(np.frombuffer(f.read(8), np.uint64)[0])
+            off = np.frombuffer(f.read(8 * (n + 1)), np.uint64).copy()
+            adj = np.frombuffer(f.read(4 * m), np.uint32).copy()
+            if weighted:
+                w = np.frombuffer(f.read(m), np.uint8).copy()
+                return n, off, adj, w
+    return n, off, adj
+
+
+LINE_FIELDS = os.path.join(HERE, "line_fields")
+
+
+def line_fields(path, ncols=2):
+    """The reference's text-reader line loop (GR:95-118, 277-304) run by the
+    C++ standard library's own istringstream (line_fields.cpp): an (m, ncols)
+    u32 array of the values each line leaves in the loop's variables."""
+    if not os.path.exists(LINE_FIELDS):
+        subprocess.run(["make", "-s", "-C", HERE, "line_fields"], check=True)
+    with tempfile.TemporaryDirectory() as td:
+        out = os.path.join(td, "rows.bin")
+        subprocess.run([LINE_FIELDS, str(ncols), str(path), out], check=True, capture_output=True)
+        with open(out, "rb") as f:
+            m = int(np.frombuffer(f.read(8), np.uint64)[0])
+            rows = np.frombuffer(f.read(4 * m * ncols), np.uint32).copy()
+    return rows.reshape(-1, ncols)

@@ -146,7 +146,9 @@ static void usage()
             "usage: pado_ref unw <edges.bin> <rank.bin|-> <out_prefix> [pairs.bin] [repeat]\n"
             "       pado_ref rank <edges.bin> <rank_out.bin>\n"
             "       pado_ref csr <edges.bin> <rank.bin> <csr_out.bin>\n"
+            "       pado_ref text <edges.txt> <csr_out.bin>\n"
 #ifdef PADO_REF_WEIGHTED
+            "       pado_ref wtext <wedges.txt> <csr_out.bin>\n"
             "       pado_ref w <wedges.bin> <rank.bin|-> <out_prefix> [pairs.bin]\n"
 #endif
             );
@@ -163,6 +165,38 @@ int main(int argc, char **argv)
         auto e = read_edges(argv[2]);
         Graph G(e);
         write_rank(argv[3], G.make_rank());
+        return 0;
+    }
+    if (mode == "text" || mode == "wtext") {
+        // The reference's own text readers, Graph(const char*) (graph.h:95-118)
+        // and WeightedGraph(const char*) (graph.h:277-304), on a text file:
+        // u32 num_v, u64 num_e, u64 off[num_v + 1], u32 adj[num_e] (+ u8 w[num_e]).
+#ifdef PADO_REF_WEIGHTED
+        if (mode == "wtext") {
+            WeightedGraph G(argv[2]);
+            FILE *f = fopen(argv[3], "wb");
+            uint32_t n = G.num_v;
+            uint64_t m = G.num_e;
+            fwrite(&n, 4, 1, f);
+            fwrite(&m, 8, 1, f);
+            for (idi v = 0; v < n; ++v) { uint64_t o = G.vertices[v]; fwrite(&o, 8, 1, f); }
+            fwrite(&m, 8, 1, f);
+            fwrite(G.out_edges, 4, m, f);
+            fwrite(G.out_weights, 1, m, f);
+            fclose(f);
+            return 0;
+        }
+#endif
+        Graph G(argv[2]);
+        FILE *f = fopen(argv[3], "wb");
+        uint32_t n = G.num_v;
+        uint64_t m = G.num_e;
+        fwrite(&n, 4, 1, f);
+        fwrite(&m, 8, 1, f);
+        for (idi v = 0; v < n; ++v) { uint64_t o = G.vertices[v]; fwrite(&o, 8, 1, f); }
+        fwrite(&m, 8, 1, f);
+        fwrite(G.out_edges, 4, m, f);
+        fclose(f);
         return 0;
     }
     if (mode == "csr") {

@@ -108,6 +108,11 @@ def lib():
         "pado_wgraph_csr": (None, [_vp, _vp, _u64, _u32, _vp, _vp, _vp]),
         "pado_wid_transfer": (None, [_u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "pado_sort_plan": (_u32, [_vp, _u32, _u64, _vp, _u32]),
+        "pado_read_edge_text": (ctypes.c_int, [ctypes.c_char_p, _u32, _vp, ctypes.POINTER(_u64)]),
+        "pado_read_edge_binary": (ctypes.c_int, [ctypes.c_char_p, _vp, ctypes.POINTER(_u64),
+                                                 ctypes.POINTER(_u32)]),
+        "pado_write_edge_binary": (ctypes.c_int, [ctypes.c_char_p, _vp, _u64]),
+        "pado_free": (None, [_vp]),
         "pado_gpu_windex_export": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp]),
         "pado_gpu_wquery": (ctypes.c_int, [_vp, _vp, _vp, _vp, _u64, _vp, _u32, _vp]),
         "pado_rmat_edges": (_u64, [_u32, _u32, ctypes.c_double, ctypes.c_double, ctypes.c_double, _u64, _vp, _u64]),
@@ -226,6 +231,42 @@ def uniform_pairs(n, k, seed=1):
     return a[:k], b[:k]
 
 
+def read_edge_text(path, ncols=2):
+    """Edge-list text as the reference's readers take it (pado_read_edge_text):
+    an (m, ncols) u32 array."""
+    buf = ctypes.POINTER(ctypes.c_uint32)()
+    m = _u64()
+    st = lib().pado_read_edge_text(str(path).encode(), ncols, ctypes.byref(buf), ctypes.byref(m))
+    if st != 0:
+        raise OSError(f"cannot read edge list {path} ({st})")
+    try:
+        out = np.ctypeslib.as_array(buf, shape=(m.value * ncols,)).copy() if m.value else np.zeros(0, np.uint32)
+    finally:
+        lib().pado_free(buf)
+    return out.reshape(-1, ncols)
+
+
+def read_edge_binary(path):
+    """The reference's binary edge list (u32 num_v, u64 num_e, u32 pairs)."""
+    buf = ctypes.POINTER(ctypes.c_uint32)()
+    m = _u64()
+    nv = _u32()
+    st = lib().pado_read_edge_binary(str(path).encode(), ctypes.byref(buf), ctypes.byref(m), ctypes.byref(nv))
+    if st != 0:
+        raise OSError(f"cannot read binary edge list {path} ({st})")
+    try:
+        out = np.ctypeslib.as_array(buf, shape=(m.value * 2,)).copy() if m.value else np.zeros(0, np.uint32)
+    finally:
+        lib().pado_free(buf)
+    return out.reshape(-1, 2)
+
+
+def write_edge_binary(path, edges):
+    e = _c(edges, np.uint32).reshape(-1, 2)
+    if lib().pado_write_edge_binary(str(path).encode(), _p(e), e.shape[0]) != 0:
+        raise OSError(f"cannot write {path}")
+
+
 def mixed_graphs(count, seed=4):
     """The many-graph workload: `count` rank-space graphs with vertex counts
     log-uniform in [2^10, 2^16]; even-numbered ones R-MAT (edge factor 8,
@@ -299,16 +340,14 @@ class Graph:
 
     @classmethod
     def from_file(cls, path):
-        """Graph(const char*) (includes/graph.h:95-118): '#'/'%' lines skipped."""
-        rows = []
-        with open(path) as f:
-            for line in f:
-                if not line or line[0] in "#%":
-                    continue
-                t = line.split()
-                if len(t) >= 2:
-                    rows.append((int(t[0]), int(t[1])))
-        return cls.from_edges(np.array(rows, dtype=np.uint32).reshape(-1, 2))
+        """Graph(const char*) (includes/graph.h:95-118): '#'/'%' lines skipped,
+        the reference's rules for malformed lines; parsed on all host cores."""
+        return cls.from_edges(read_edge_text(path, 2))
+
+    @classmethod
+    def from_binary(cls, path):
+        """The reference's binary edge list (edgelist2binary.cpp:28-91)."""
+        return cls.from_edges(read_edge_binary(path))
 
     def make_rank(self, use_global_rand=False):
         rank = np.zeros(max(self.num_v, 1), np.uint32)
@@ -357,17 +396,9 @@ class WeightedGraph(Graph):
     @classmethod
     def from_file(cls, path):
         """WeightedGraph(const char*) (includes/graph.h:277-304): 'a b w' rows,
-        '#'/'%' lines skipped."""
-        rows = []
-        with open(path) as f:
-            for line in f:
-                if not line or line[0] in "#%":
-                    continue
-                t = line.split()
-                if len(t) >= 3:
-                    rows.append((int(t[0]), int(t[1]), int(t[2])))
-        a = np.array(rows, dtype=np.int64).reshape(-1, 3)
-        return cls.from_edges(a[:, :2].astype(np.uint32), (a[:, 2] & 0xFF).astype(np.uint32))
+        '#'/'%' lines skipped, weights cast to u8 as the reference does."""
+        a = read_edge_text(path, 3)
+        return cls.from_edges(a[:, :2].copy(), (a[:, 2] & 0xFF).astype(np.uint32))
 
     def id_transfer(self, rank):
         """Renumber in place into rank space (includes/graph.h:374-408);
