@@ -167,15 +167,17 @@ typedef struct {
     uint32_t batch_size;  /* roots per batch: 1..1024, 0 = 1024 (PARA:42); any value
                              gives the same labels, so it is a tuning knob */
     uint32_t flags;       /* PADO_GPU_BUILD_* */
-    uint32_t push_blocks; /* grid of the push kernel (256-thread blocks), 0 = 1024 */
-    uint32_t add_blocks;  /* grid of the add kernel, 0 = 1024 */
+    uint32_t push_blocks; /* grid of the push kernel (blocks of push_waves waves),
+                             0 = 256 up to 2^17 vertices, else 1024 */
+    uint32_t add_blocks;  /* grid of the add kernel (256-thread blocks), same default */
     uint32_t iters_per_sync; /* batch iterations enqueued per host check of the
                                 frontier (iterations past a batch's end are
                                 empty launches), 0 = 8 */
     uint32_t sort_max_items; /* labels per call of the final canonical sort
                                 (0 = 2^31 - 1025, hipCUB's int item count);
                                 larger indexes are sorted in vertex ranges */
-    uint32_t reserved[2];
+    uint32_t push_chunk;  /* edges per push work descriptor: 8, 16 or 32, 0 = 16 */
+    uint32_t push_waves;  /* waves per push block: 2, 4 or 8, 0 = 4 */
 } pado_gpu_build_cfg;
 
 /* Bit-parallel labeling one BFS per slot (as the reference runs it) instead of
@@ -346,11 +348,14 @@ pado_gpu_status pado_gpu_windex_load(pado_gpu_ctx *ctx, const char *filename, pa
  * Autotuner (no reference counterpart).  Every knob of pado_gpu_build_cfg
  * leaves the labels unchanged, so the tuner may pick any of them purely on
  * speed: it times full trial builds of `graph` in a coordinate search over
- * batch size {256, 512, 1024}, push grid {1024..8192}, add grid {1024..4096},
- * iterations per host check {2..16} and the flags (hipGraph batch step,
- * bit-parallel variant), starting from `start` (NULL = defaults), and returns
- * the fastest configuration.  trials_out (nullable) receives the number of
- * builds run (default cap 20); best_ms their best time.
+ * the push work-descriptor size {8, 16, 32}, push block shape {2, 4, 8}
+ * waves, push and add grids {256..4096}, iterations per host check {2..16},
+ * batch size {256, 512, 1024} and the flags (hipGraph or stream batch step,
+ * bit-parallel variant; PADO_GPU_BUILD_WIDE of `start` is kept), starting
+ * from `start` (NULL = defaults).  One untimed build first, then every
+ * configuration is timed as the median of three builds.  max_trials caps the
+ * configurations timed (0 = 24); trials_out (nullable) receives their number,
+ * best_ms the median of the returned configuration.
  * ------------------------------------------------------------------------ */
 pado_gpu_status pado_gpu_autotune(pado_gpu_ctx *ctx, const pado_gpu_graph *graph, const pado_gpu_build_cfg *start,
                                   uint32_t max_trials, pado_gpu_build_cfg *best, double *best_ms,
@@ -375,6 +380,16 @@ typedef struct pado_gpu_multi pado_gpu_multi;
 
 pado_gpu_status pado_gpu_build_multi(pado_gpu_ctx *ctx, uint32_t num_graphs, const uint32_t *n, const uint64_t *off,
                                      const uint32_t *adj, const pado_gpu_build_cfg *cfg, pado_gpu_multi **out);
+/* The autotuner for a many-graph job: times pado_gpu_build_multi on `sample`
+ * of the graphs (every num_graphs / sample-th one, so the mix of kinds and
+ * sizes is kept; 0 = all) over iterations per host check {4..32}, push
+ * work-descriptor size, push block shape and batch size (grids stay sized per
+ * graph), three builds per configuration as pado_gpu_autotune does
+ * (max_trials 0 = 12).  Same array layout as pado_gpu_build_multi. */
+pado_gpu_status pado_gpu_autotune_multi(pado_gpu_ctx *ctx, uint32_t num_graphs, const uint32_t *n,
+                                        const uint64_t *off, const uint32_t *adj, uint32_t sample,
+                                        const pado_gpu_build_cfg *start, uint32_t max_trials,
+                                        pado_gpu_build_cfg *best, double *best_ms, uint32_t *trials_out);
 /* Per-graph result: status (OK or ERR_DOMAIN), label count, whether it was
  * labeled inside the union (1) or on its own (0), and its label distance
  * bits (8: the reference's u8 domain, 12: wide).  Outputs are nullable. */

@@ -51,11 +51,13 @@ class PadoError(RuntimeError):
 class BuildCfg(ctypes.Structure):
     _fields_ = [("batch_size", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("push_blocks", ctypes.c_uint32),
                 ("add_blocks", ctypes.c_uint32), ("iters_per_sync", ctypes.c_uint32),
-                ("sort_max_items", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2)]
+                ("sort_max_items", ctypes.c_uint32), ("push_chunk", ctypes.c_uint32),
+                ("push_waves", ctypes.c_uint32)]
 
     def as_dict(self):
         return {"batch_size": self.batch_size, "flags": self.flags, "push_blocks": self.push_blocks,
-                "add_blocks": self.add_blocks, "iters_per_sync": self.iters_per_sync}
+                "add_blocks": self.add_blocks, "iters_per_sync": self.iters_per_sync,
+                "push_chunk": self.push_chunk, "push_waves": self.push_waves}
 
 
 class ServeCfg(ctypes.Structure):
@@ -149,6 +151,9 @@ def lib():
                                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u32)]),
         "pado_gpu_build_multi": (ctypes.c_int, [_vp, _u32, _vp, _vp, _vp, ctypes.POINTER(BuildCfg),
                                                 ctypes.POINTER(_vp)]),
+        "pado_gpu_autotune_multi": (ctypes.c_int, [_vp, _u32, _vp, _vp, _vp, _u32, ctypes.POINTER(BuildCfg), _u32,
+                                                   ctypes.POINTER(BuildCfg), ctypes.POINTER(ctypes.c_double),
+                                                   ctypes.POINTER(_u32)]),
         "pado_gpu_multi_graph_info": (ctypes.c_int, [_vp, _u32, ctypes.POINTER(ctypes.c_int),
                                                      ctypes.POINTER(_u64), ctypes.POINTER(ctypes.c_int),
                                                      ctypes.POINTER(_u32)]),
@@ -664,8 +669,9 @@ class Index:
             pass
 
 
-def autotune(ctx, dgraph, max_trials=16, start=None):
-    """Times trial builds; returns (best BuildCfg, best ms, trials)."""
+def autotune(ctx, dgraph, max_trials=0, start=None):
+    """Times trial builds (median of three each); returns (best BuildCfg,
+    its median ms, configurations timed)."""
     best = BuildCfg()
     ms = ctypes.c_double()
     tr = _u32()
@@ -674,16 +680,35 @@ def autotune(ctx, dgraph, max_trials=16, start=None):
     return best, ms.value, tr.value
 
 
+def _multi_arrays(graphs):
+    ns = np.array([G.num_v for G in graphs], np.uint32)
+    off = np.concatenate([G.off for G in graphs]).astype(np.uint64)
+    adj = np.concatenate([G.adj for G in graphs]).astype(np.uint32) if graphs else np.zeros(0, np.uint32)
+    return ns, off, adj
+
+
+def autotune_multi(ctx, graphs, sample=64, max_trials=0, start=None):
+    """The many-graph job's tuner on `sample` of the graphs; returns (best
+    BuildCfg, its median ms on the sample, configurations timed)."""
+    ns, off, adj = _multi_arrays(graphs)
+    best = BuildCfg()
+    ms = ctypes.c_double()
+    tr = _u32()
+    _check(lib().pado_gpu_autotune_multi(ctx.h, len(graphs), _p(ns), _p(off), _p(adj) if adj.size else _p(ns),
+                                         sample, ctypes.byref(start) if start is not None else None, max_trials,
+                                         ctypes.byref(best), ctypes.byref(ms), ctypes.byref(tr)))
+    return best, ms.value, tr.value
+
+
 class MultiIndex:
     """Many independent rank-space graphs labeled as one job."""
 
-    def __init__(self, ctx, graphs, batch_size=BATCH_SIZE):
+    def __init__(self, ctx, graphs, batch_size=BATCH_SIZE, cfg=None):
         self.ctx = ctx
-        ns = np.array([G.num_v for G in graphs], np.uint32)
-        off = np.concatenate([G.off for G in graphs]).astype(np.uint64)
-        adj = np.concatenate([G.adj for G in graphs]).astype(np.uint32) if graphs else np.zeros(0, np.uint32)
-        cfg = BuildCfg()
-        cfg.batch_size = batch_size
+        ns, off, adj = _multi_arrays(graphs)
+        if cfg is None:
+            cfg = BuildCfg()
+            cfg.batch_size = batch_size
         h = _vp()
         _check(lib().pado_gpu_build_multi(ctx.h, len(graphs), _p(ns), _p(off), _p(adj) if adj.size else None,
                                           ctypes.byref(cfg), ctypes.byref(h)))

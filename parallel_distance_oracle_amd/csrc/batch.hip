@@ -460,7 +460,7 @@ __global__ __launch_bounds__(256) void k_bp_masks(uint32_t n, const uint8_t *bpd
 // slot ro and touched-list slot ro (k_batch_begin sets both counts and the
 // frontier size), so the only shared counter left is the push-descriptor
 // count, updated once per block.
-template <bool W>
+template <bool W, uint32_t CH>
 __global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const uint64_t *off, const uint32_t *adj,
                                                      LabelStore S, unsigned long long *ind, unsigned long long *vmark,
                                                      uint32_t *tlist, unsigned long long *aoff, unsigned long long *ag,
@@ -503,7 +503,7 @@ __global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const 
             if (lane == 0) aend[ro] = pdeg;
         }
     }
-    const uint32_t nch = (pdeg + kChunk - 1) / kChunk;
+    const uint32_t nch = (pdeg + CH - 1) / CH;
     if (lane == 0) s_nch[wv] = nch;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -516,7 +516,7 @@ __global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const 
     if (nch == 0) return;
     uint32_t c0 = s_base;
     for (uint32_t w = 0; w < wv; ++w) c0 += s_nch[w];
-    for (uint32_t q = lane; q < nch; q += 64) chunks[c0 + q] = ((unsigned long long)(q * kChunk) << 32) | ro;
+    for (uint32_t q = lane; q < nch; q += 64) chunks[c0 + q] = ((unsigned long long)(q * CH) << 32) | ro;
 }
 
 // Root-distance table (dist_matrix, PARA:986-1013, transposed): row h holds
@@ -573,17 +573,19 @@ __global__ __launch_bounds__(256) void k_hub_table(uint32_t fill, const uint8_t 
     }
 }
 
-constexpr uint32_t kPushWaves = 4;  // waves per k_push block
+constexpr uint32_t kPushWaves = 4;  // default waves per k_push block
 
-// push (PARA:1486-1559 / push_labels PARA:1084-1211).  A wave takes four
-// 16-edge chunks (one edge per lane) and walks the root words of its edges'
+// push (PARA:1486-1559 / push_labels PARA:1084-1211).  A wave takes 64 / CH
+// work descriptors of CH edges each (one edge per lane; CH = kChunk and
+// PW = kPushWaves waves per block by default, other instantiations exist
+// for the autotuner) and walks the root words of its edges'
 // last distance groups in step, four words at a time: each lane
 // test-and-sets its indicator words, and the fresh (tail, root) pairs of all
 // 64 lanes are then checked against the bit-parallel labels 64 at a time,
 // one pair per lane, so a tail with many fresh roots does not hold up the
 // wave.
-template <bool W>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_push(uint32_t iter, const uint32_t *adj,
+template <bool W, uint32_t CH, uint32_t PW>
+__global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(4))) void k_push(uint32_t iter, const uint32_t *adj,
                                               const unsigned long long *aoff, const unsigned long long *ag,
                                               const uint32_t *amask, const uint32_t *aend,
                                               const unsigned long long *chunks, unsigned long long *ind,
@@ -593,17 +595,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                               const unsigned long long *rmask, const ulonglong2 *bps,
                                               const ulonglong2 *rset, BuildCounters *C, uint32_t bp_prune)
 {
-    __shared__ unsigned long long s_nb[kPushWaves][4][64]; // fresh roots of each lane's edge, per word of the step
-    __shared__ unsigned long long s_keep[kPushWaves][4][64];  // of those, the ones the BP check keeps
-    __shared__ uint32_t s_ex[kPushWaves][64];              // exclusive prefix of the lanes' fresh counts
-    __shared__ uint32_t s_cu[kPushWaves][64];              // a lane's running counts over its four words
-    __shared__ uint32_t s_t[kPushWaves][64];               // each lane's tail
+    static_assert(CH >= 4 && CH <= 64 && (CH & (CH - 1)) == 0, "chunk: a power of two in [4, 64]");
+    constexpr uint32_t DPS = 64 / CH;  // descriptors per wave step
+    __shared__ unsigned long long s_nb[PW][4][64]; // fresh roots of each lane's edge, per word of the step
+    __shared__ unsigned long long s_keep[PW][4][64];  // of those, the ones the BP check keeps
+    __shared__ uint32_t s_ex[PW][64];              // exclusive prefix of the lanes' fresh counts
+    __shared__ uint32_t s_cu[PW][64];              // a lane's running counts over its four words
+    __shared__ uint32_t s_t[PW][64];               // each lane's tail
     // set-phase scratch (bp_set_phase)
-    __shared__ unsigned long long s_srem[kPushWaves][64], s_sone[kPushWaves][64];
-    __shared__ uint32_t s_sro[kPushWaves][64], s_st[kPushWaves][64], s_sex[kPushWaves][64], s_shit[kPushWaves][64];
+    __shared__ unsigned long long s_srem[PW][64], s_sone[PW][64];
+    __shared__ uint32_t s_sro[PW][64], s_st[PW][64], s_sex[PW][64], s_shit[PW][64];
     const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
-    const uint32_t wave = blockIdx.x * kPushWaves + wv;
-    const uint32_t nwaves = gridDim.x * kPushWaves;
+    const uint32_t wave = blockIdx.x * PW + wv;
+    const uint32_t nwaves = gridDim.x * PW;
     const SetScratch sc{s_srem[wv], s_sone[wv], s_sro[wv], s_st[wv], s_sex[wv], s_shit[wv]};
     // The check reads level masks while they cover every td it acts on, the
     // distance bytes after that.
@@ -636,13 +640,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     }
     // Per-lane statistics of one launch fit 32 bits; summed in 64 at the end.
     uint32_t n_push = 0, n_pass = 0, n_bp = 0, n_cand = 0, n_edge = 0, n_tail = 0;
-    for (uint32_t c0 = wave * 4; c0 < nc; c0 += nwaves * 4) {
-        const uint32_t c = c0 + (lane >> 4);
+    for (uint32_t c0 = wave * DPS; c0 < nc; c0 += nwaves * DPS) {
+        const uint32_t c = c0 + lane / CH;
         const unsigned long long desc = c < nc ? chunks[c] : kChunkPad;
         uint32_t k = 0, t = 0, lim = 0, words = 0;
         if (desc != kChunkPad) {
             k = (uint32_t)desc;
-            const uint32_t e = (uint32_t)(desc >> 32) + (lane & (kChunk - 1));
+            const uint32_t e = (uint32_t)(desc >> 32) + (lane & (CH - 1));
             if (e < aend[k]) {
                 ++n_edge;
                 t = adj[aoff[k] + e];
@@ -843,7 +847,7 @@ __global__ __launch_bounds__(1024) void k_batch_begin(const uint8_t *used, Build
 // dist < iter: a hub h can prune only if d_v(h) + min_r d_r(h) <= iter, so a
 // per-hub minimum filters most hubs before the 1 KB root-distance row of the
 // survivors is read (16 B per lane).
-template <bool W>
+template <bool W, uint32_t CH>
 __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used, const uint64_t *off, const uint32_t *adj, const uint32_t *clist,
                                              unsigned long long *cand, LabelStore S, uint32_t *hinfo,
                                              typename Hop<W>::Dist *dc,
@@ -1013,7 +1017,7 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
             amask[k] = wmask;
             aend[k] = pdeg;
         }
-        const uint32_t nch = (pdeg + kChunk - 1) / kChunk;
+        const uint32_t nch = (pdeg + CH - 1) / CH;
         if (nch > c_left) {
             for (uint32_t q = lane; q < c_left; q += 64) chunks[c_next + q] = kChunkPad;
             const uint64_t guess = (uint64_t)nch * rem;  // chunks left if v is typical
@@ -1024,7 +1028,7 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
             c_next = __shfl(b0, 0);
             c_left = want;
         }
-        for (uint32_t q = lane; q < nch; q += 64) chunks[c_next + q] = ((unsigned long long)(q * kChunk) << 32) | k;
+        for (uint32_t q = lane; q < nch; q += 64) chunks[c_next + q] = ((unsigned long long)(q * CH) << 32) | k;
         c_next += nch;
         c_left -= nch;
     }
@@ -1132,6 +1136,33 @@ static pado_gpu_status pool_repack(hipStream_t st, uint32_t n, DevBuf<uint32_t> 
     return PADO_GPU_OK;
 }
 
+// The batch kernels of one (chunk, push waves) choice.
+template <bool W>
+struct BatchKernels {
+    decltype(&k_batch_init<W, kChunk>) init;
+    decltype(&k_push<W, kChunk, kPushWaves>) push;
+    decltype(&k_add<W, kChunk>) add;
+    uint32_t chunk, waves;
+};
+
+template <bool W, uint32_t CH>
+static BatchKernels<W> batch_kernels_ch(uint32_t waves)
+{
+    BatchKernels<W> K{k_batch_init<W, CH>, k_push<W, CH, kPushWaves>, k_add<W, CH>, CH, kPushWaves};
+    if (waves == 2) K.push = k_push<W, CH, 2>;
+    if (waves == 8) K.push = k_push<W, CH, 8>;
+    if (waves == 2 || waves == 8) K.waves = waves;
+    return K;
+}
+
+template <bool W>
+static BatchKernels<W> batch_kernels(uint32_t chunk, uint32_t waves)
+{
+    if (chunk == 8) return batch_kernels_ch<W, 8>(waves);
+    if (chunk == 32) return batch_kernels_ch<W, 32>(waves);
+    return batch_kernels_ch<W, kChunk>(waves);
+}
+
 template <bool W>
 static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, const DevBuf<uint8_t> &d_used,
                                        const pado_gpu_build_cfg &cfg, unsigned long long pool_min,
@@ -1150,6 +1181,13 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
         set_error("graph too large for the wide hop domain (at most 2^20 vertices)");
         return PADO_GPU_ERR_DOMAIN;
     }
+    const uint32_t chunk = cfg.push_chunk ? cfg.push_chunk : kChunk;
+    const uint32_t waves = cfg.push_waves ? cfg.push_waves : kPushWaves;
+    if ((chunk != 8 && chunk != 16 && chunk != 32) || (waves != 2 && waves != 4 && waves != 8)) {
+        set_error("push_chunk must be 8, 16 or 32 and push_waves 2, 4 or 8");
+        return PADO_GPU_ERR_ARG;
+    }
+    const BatchKernels<W> K = batch_kernels<W>(chunk, waves);
     const std::string domain_msg =
         W ? "batch iteration reached 4095: distances outside the wide (12-bit) hop domain"
           : "batch iteration reached 255: distances outside the uint8 domain of the reference";
@@ -1173,7 +1211,7 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
     const uint64_t add_waves = 4ull * (cfg.add_blocks ? cfg.add_blocks : default_grid(n));
     const uint64_t alist_cap = (uint64_t)n + kAlistGrab * add_waves;
     PADO_TRY(ag.alloc(alist_cap * kWords));
-    PADO_TRY(chunks.alloc(2 * (g->m / kChunk + (uint64_t)n) + kChunkGrab * add_waves + 64));
+    PADO_TRY(chunks.alloc(2 * (g->m / chunk + (uint64_t)n) + kChunkGrab * add_waves + 64));
     PADO_TRY(tlist.alloc(n));
     PADO_TRY(clist.alloc(n));
     PADO_TRY(aoff.alloc(alist_cap));
@@ -1250,7 +1288,7 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
         hipLaunchKernelGGL(k_hub_table<W>, dim3(root_blocks), dim3(256), 0, st, 0u, d_used.p, S, hinfo.p, dc.p, ctr.p);
         hipLaunchKernelGGL(k_batch_end, dim3(1024), dim3(256), 0, st, ind.p, tlist.p, ctr.p);
         hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(batch), 0, st, d_used.p, ctr.p, batch, n);
-        hipLaunchKernelGGL(k_batch_init<W>, dim3((batch + 15) / 16), dim3(1024), 0, st, d_used.p, g->off.p, g->adj.p, S, ind.p,
+        hipLaunchKernelGGL(K.init, dim3((batch + 15) / 16), dim3(1024), 0, st, d_used.p, g->off.p, g->adj.p, S, ind.p,
                            vmark.p, tlist.p, aoff.p, ag.p, amask.p, aend.p, chunks.p,
                            (const ulonglong2 *)idx->bps.p, rset.p, lmask.p, rmask.p, ctr.p);
         hipLaunchKernelGGL(k_hub_table<W>, dim3(root_blocks), dim3(256), 0, st, 1u, d_used.p, S, hinfo.p, dc.p, ctr.p);
@@ -1260,13 +1298,13 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
         for (uint32_t u = 0; u < count; ++u) {
             const uint32_t iter = first + 1 + u;
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u], st));
-            hipLaunchKernelGGL(k_push<W>, dim3(push_grid), dim3(256), 0, st, iter, g->adj.p, aoff.p, ag.p,
+            hipLaunchKernelGGL(K.push, dim3(push_grid), dim3(64 * K.waves), 0, st, iter, g->adj.p, aoff.p, ag.p,
                                amask.p, aend.p, chunks.p, ind.p, cand.p, vmark.p, tlist.p, clist.p,
                                idx->bpd.p, lmask.p, rmask.p, (const ulonglong2 *)idx->bps.p, rset.p, ctr.p,
                                idx->bp_pass_through ? 0u : 1u);
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 1], st));
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 2], st));
-            hipLaunchKernelGGL(k_add<W>, dim3(add_grid), dim3(256), 0, st, iter, d_used.p, g->off.p, g->adj.p, clist.p,
+            hipLaunchKernelGGL(K.add, dim3(add_grid), dim3(256), 0, st, iter, d_used.p, g->off.p, g->adj.p, clist.p,
                                cand.p, S, hinfo.p, dc.p, aoff.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 3], st));
         }
