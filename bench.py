@@ -46,6 +46,7 @@ import parallel_distance_oracle_amd as pado  # noqa: E402
 HBM_PEAK_GBS = 8000.0
 BP_RECORD_BYTES = 50 * (1 + 8 + 8)  # 850 B: bit-parallel record of one vertex
 SERVE_SLO_P99_MS = 1.0  # serving latency objective: p99 of completion - arrival
+SERVE_MODE = "launch"  # serving mode of the bench legs (--serve-mode)
 
 
 def log(*a):
@@ -332,7 +333,10 @@ def serve_measure(ctx, idx, n, rank2id, duration_s, seed, max_pairs_per_rate=600
         plain_s = min(plain_s, time.perf_counter() - t0)
     plain_qps = allA.size / plain_s
 
-    srv = pado.Server(ctx, idx, staging_pairs=16 << 20, max_launch_pairs=4 << 20, streams=3)
+    if SERVE_MODE == "resident":
+        srv = pado.Server(ctx, idx, staging_pairs=16 << 20, resident=True)
+    else:
+        srv = pado.Server(ctx, idx, staging_pairs=16 << 20, max_launch_pairs=4 << 20, streams=3)
     # Warm every stream and launch size with a burst of 256 requests, then with
     # a discarded open-loop window (0.25 s at the lowest offered rate): a
     # one-off host stall of several ms has landed in the first open-loop window
@@ -384,7 +388,7 @@ def serve_measure(ctx, idx, n, rank2id, duration_s, seed, max_pairs_per_rate=600
             "rates": rates, "saturation_served_queries_per_s": sat["served_queries_per_s"],
             "p99_ms_at_saturation": sat["p99_ms"], "results_equal_plain_batch": ok_all,
             "launches": st["launches"], "requests": st["requests_done"], "rejected": st["rejected"],
-            "busy_ms": round(st["busy_ms"], 2)}
+            "busy_ms": round(st["busy_ms"], 2), "mode": SERVE_MODE}
 
 
 def copy_calibrate(mib):
@@ -415,6 +419,7 @@ def load_traffic():
 
 
 def main():
+    global SERVE_MODE
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
@@ -444,10 +449,13 @@ def main():
     ap.add_argument("--multi-only", action="store_true", help="only run the multi-graph job and exit")
     ap.add_argument("--weighted-build-only", type=int, default=0, metavar="SCALE",
                     help="only build the weighted R-MAT of this scale (twice: warmup + timed) and exit (profiler runs)")
+    ap.add_argument("--serve-mode", choices=("resident", "launch"), default="launch",
+                    help="serving with the resident kernel or with one launch per coalesced batch")
     ap.add_argument("--serve-only", action="store_true",
                     help="only run the serving leg and exit (profiler runs; add --stream to build its index "
                          "with per-kernel launches)")
     args = ap.parse_args()
+    SERVE_MODE = args.serve_mode
     if args.copy_calibrate:
         copy_calibrate(args.copy_calibrate)
         return

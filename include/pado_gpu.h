@@ -437,13 +437,27 @@ typedef struct pado_gpu_server pado_gpu_server;
  * launch, taken 16 pairs at a time; its contended atomics make it slower on
  * a multi-XCD device, kept for comparison). */
 #define PADO_GPU_SERVE_PERSISTENT 0x20u
+/* Serve with one resident kernel instead of per-launch kernels: the kernel
+ * stays on the device while requests flow, reading each request's pairs from
+ * the mapped staging ring and writing its answers straight back into mapped
+ * host memory.  submit() cuts a request into descriptors of up to 64 pairs
+ * spread over eight queues and publishes them in mapped memory; one wave of
+ * the kernel mirrors the published counts into device memory, the others take
+ * descriptors by ticket, and the wave that finishes a request's last
+ * descriptor raises the request's completion flag, which a completer thread
+ * polls.  No dispatcher, no copies, no launch per batch.  The kernel leaves
+ * after 20 ms without work and at the latest after 100 ms (draining what it
+ * took in), and is relaunched while requests are outstanding, so device-wide
+ * synchronisation elsewhere in the process waits at most that long behind
+ * it.  Requests larger than 2^32 - 1 pairs are not served in this mode. */
+#define PADO_GPU_SERVE_RESIDENT 0x40u
 
 typedef struct {
     uint64_t staging_pairs;    /* pair capacity of the admission ring; 0 = derived from free memory */
     uint64_t max_launch_pairs; /* coalescing cap per launch (0 = 4M) */
     uint32_t max_requests;     /* request slots (0 = 4096) */
     uint32_t streams;          /* launches in flight (0 = 3) */
-    uint32_t flags;            /* PADO_GPU_QUERY_REF_COMPAT | PADO_GPU_SERVE_GRID_STRIDE */
+    uint32_t flags;            /* PADO_GPU_QUERY_REF_COMPAT | PADO_GPU_SERVE_* */
     uint32_t min_launch_pairs; /* while launches are in flight, pending pairs that start a
                                   launch at once (0 = 256Ki) */
     uint32_t max_wait_us;      /* ... or the oldest pending request's wait that does (0 = 200) */

@@ -36,6 +36,7 @@ BUILD_WIDE = 4  # wide hop domain: 12-bit label distances, u16 answers (graphs d
 WIDE_INF = 0xFFFF  # wide-domain query: unreachable
 SERVE_GRID_STRIDE = 0x10
 SERVE_PERSISTENT = 0x20
+SERVE_RESIDENT = 0x40
 WINF = 0xFFFFFFFF  # weighted query: no common hub (unreachable)
 
 _STATUS = {0: "OK", 1: "ERR_ARG", 2: "ERR_HIP", 3: "ERR_NOMEM", 4: "ERR_DOMAIN", 5: "ERR_STATE",
@@ -784,7 +785,7 @@ class Server:
     """Continuous-batching query service over an ordered, resident Index."""
 
     def __init__(self, ctx, index, staging_pairs=0, max_launch_pairs=0, max_requests=0, streams=0,
-                 ref_compat=False, persistent=False, min_launch_pairs=0, max_wait_us=0):
+                 ref_compat=False, persistent=False, min_launch_pairs=0, max_wait_us=0, resident=False):
         cfg = ServeCfg()
         cfg.min_launch_pairs = min_launch_pairs
         cfg.max_wait_us = max_wait_us
@@ -792,7 +793,8 @@ class Server:
         cfg.max_launch_pairs = max_launch_pairs
         cfg.max_requests = max_requests
         cfg.streams = streams
-        cfg.flags = (QUERY_REF_COMPAT if ref_compat else 0) | (SERVE_PERSISTENT if persistent else 0)
+        cfg.flags = ((QUERY_REF_COMPAT if ref_compat else 0) | (SERVE_PERSISTENT if persistent else 0)
+                     | (SERVE_RESIDENT if resident else 0))
         self.dtype = np.uint32 if index.weighted else (np.uint16 if index.wide else np.uint8)
         h = _vp()
         _check(lib().pado_gpu_serve_start(ctx.h, index.h, ctypes.byref(cfg), ctypes.byref(h)))

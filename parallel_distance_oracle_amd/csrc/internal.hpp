@@ -53,6 +53,56 @@ inline hipError_t copy_sync(void *dst, const void *src, size_t bytes, hipMemcpyK
     return e;
 }
 
+// Resident serving (serve.hip drives it, query.hip holds the kernel).
+// Work reaches the resident kernel as descriptors of at most kServeGroup
+// pairs each, spread over kServeQueues queues; the host publishes a queue's
+// descriptor count in mapped memory and the kernel reports each request's
+// completion by writing its sequence number into the request slot's flag.
+constexpr uint32_t kServeQueues = 8;
+constexpr uint32_t kServeGroup = 64;
+struct alignas(32) ServeDesc {
+    uint64_t pos;   // ring position of the first pair (monotonic)
+    uint32_t len;   // pairs, 1..kServeGroup
+    uint32_t slot;  // request slot
+    uint64_t seq;   // the request's sequence number (its id)
+    uint32_t k;     // the request's pairs
+    uint32_t pad;
+};
+// Host-written control words, mapped for the device.
+struct ServeCtl {
+    uint64_t pub[kServeQueues];  // descriptors published per queue
+    uint32_t stop;               // the kernel drains and exits
+    uint32_t pad[15];
+};
+// Device-side queue state, one 128-byte line per counter: claim (tickets
+// handed out), seen (published descriptors the kernel has taken in), done
+// (descriptors completed), then the exit flag.
+constexpr uint32_t kServeLine = 16;  // u64 per line
+inline constexpr uint32_t serve_claim(uint32_t q) { return q * kServeLine; }
+inline constexpr uint32_t serve_seen(uint32_t q) { return (kServeQueues + q) * kServeLine; }
+inline constexpr uint32_t serve_done(uint32_t q) { return (2 * kServeQueues + q) * kServeLine; }
+constexpr uint32_t kServeExit = 3 * kServeQueues * kServeLine;
+constexpr uint32_t kServeStateWords = kServeExit + kServeLine;
+
+struct ServeKernelArgs {
+    const uint32_t *ra, *rb;  // the pair ring (mapped host memory)
+    void *out;                // the answer ring (mapped host memory)
+    uint64_t cap;             // ring capacity in pairs
+    const ServeCtl *ctl;
+    const ServeDesc *desc;    // kServeQueues rings of dcap descriptors
+    uint32_t dcap;
+    uint64_t *flags;          // per request slot (mapped host memory)
+    unsigned long long *state;  // kServeStateWords (device memory)
+    uint32_t *rdone;          // pairs answered per request slot (device memory)
+    uint64_t idle_ticks;      // exit after this long without work (100 MHz ticks)
+    uint64_t max_ticks;       // drain and exit after this long in any case
+    int ref_compat;
+};
+// Enqueues the state reset and the resident kernel on `st`.
+pado_gpu_status launch_serve_resident(const pado_gpu_index *idx, const ServeKernelArgs &a, uint32_t blocks,
+                                      hipStream_t st);
+uint32_t serve_resident_blocks_per_cu(const pado_gpu_index *idx);
+
 // Launch geometry helpers.
 inline uint32_t grid_for(uint64_t threads, uint32_t block, uint32_t cap = 1u << 20)
 {

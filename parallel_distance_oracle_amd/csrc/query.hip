@@ -53,6 +53,7 @@ __device__ inline void q_lds_sync()
 }
 
 constexpr uint32_t kQLds = 1024;   // labels of the longer list staged in LDS per wave
+constexpr uint32_t kServeLds = 768;  // the same in the resident serving kernel
 constexpr uint32_t kQWaves = 4;    // waves per query block
 constexpr uint32_t kQPre = 4;      // longer-list loads per lane issued with the pair's first reads
 constexpr uint32_t kWPre = 4;      // the same for weighted pairs (8 measured slower)
@@ -79,7 +80,7 @@ __device__ inline uint32_t lower_hub(const uint32_t *L, uint32_t len, uint32_t k
 // sbuf: the wave's kQLds-entry LDS buffer.  Hop<false>: the reference's
 // answers (255 for unreachable or >= 253); Hop<true>: exact hops, 0xFFFF for
 // unreachable, using only bit-parallel slots reached on both sides.
-template <bool W>
+template <bool W, uint32_t LDS = kQLds>
 __device__ inline uint32_t pair_distance_at(uint32_t n, const uint8_t *bpd, const ulonglong2 *bps,
                                             const uint32_t *lab, uint32_t a, uint32_t b, uint64_t oa, uint32_t la,
                                             uint64_t ob, uint32_t lb, bool ref_compat, uint32_t *sbuf)
@@ -101,7 +102,7 @@ __device__ inline uint32_t pair_distance_at(uint32_t n, const uint8_t *bpd, cons
     const uint32_t *S = lab + (la <= lb ? oa : ob);
     const uint32_t *T = lab + (la <= lb ? ob : oa);
     const uint32_t ls = la <= lb ? la : lb, lt = la <= lb ? lb : la;
-    const bool staged = lt <= kQLds;
+    const bool staged = lt <= LDS;
     const uint32_t s0 = lane < ls ? S[lane] : 0u;
     uint32_t tv[kQPre];
     if (staged) {
@@ -242,6 +243,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
 // ref_compat the reference's int16 domain (>= 32765 -> 32767, ids out of
 // range -> a == b ? 0 : 32767).  One wave per pair, the shorter list's hubs
 // binary-searched in the longer one as for unweighted labels.
+template <uint32_t LDS = kQLds>
 __device__ inline uint32_t wpair_distance_at(uint32_t n, const uint32_t *hub, const uint32_t *dist, uint32_t a,
                                              uint32_t b, uint64_t oa, uint32_t la, uint64_t ob, uint32_t lb,
                                              bool ref_compat, uint32_t *sbuf)
@@ -253,7 +255,7 @@ __device__ inline uint32_t wpair_distance_at(uint32_t n, const uint32_t *hub, co
     // The longer list's hubs go to the wave's LDS (distances are read only
     // for matches); its first kWPre * 64 hubs and the shorter list's first 64
     // are requested together.
-    const bool staged = lt <= kQLds;
+    const bool staged = lt <= LDS;
     const uint32_t k0 = lane < ls ? hub[os + lane] : 0u;
     if (staged) {
         uint32_t tv[kWPre];
@@ -393,6 +395,208 @@ uint32_t query_ws_blocks_per_cu(bool weighted)
     const hipError_t e = weighted ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_wquery_ws, 256, 0)
                                   : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_query_ws, 256, 0);
     return (e == hipSuccess && nb > 0) ? (uint32_t)nb : 4u;
+}
+
+// ---------------------------------------------------------------------------
+// Resident serving kernel.  One launch stays on the device while requests
+// flow: the host writes a request's pairs into the mapped ring, cuts them
+// into descriptors of at most kServeGroup pairs spread over kServeQueues
+// queues, and raises each queue's published count in mapped memory.  Wave 0
+// of workgroup 0 polls those counts (one host read per queue per round) and
+// mirrors them into device memory; every other wave takes tickets from its
+// queue's device counter (queues are picked by workgroup, so the eight
+// counters and their traffic spread over the XCDs) and waits, sleeping, on
+// the mirrored count in L2.  A wave answers its descriptor's pairs with the
+// label merge of the batch kernel: one host read brings in the up-to-64
+// pairs' ids (lane j holds pair j), the lanes load their own pair's label
+// offsets together, and the pairs are then answered in turn from those
+// registers; the answers go back as one contiguous store into the mapped
+// answer ring.  The wave that completes a request's last descriptor writes
+// the request's sequence number into its slot's completion flag.  The poller
+// stops taking in new descriptors once asked to (host stop flag), after
+// max_ticks in any case, or after idle_ticks without work; when every
+// descriptor taken in is done it raises the exit flag, and every wave leaves.
+// The host relaunches the kernel when work arrives after an exit.
+namespace {
+
+__device__ inline unsigned long long ld_agent(unsigned long long *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ void serve_poll(const ServeKernelArgs &A)
+{
+    const uint32_t lane = threadIdx.x & 63;
+    unsigned long long *S = A.state;
+    uint64_t *pub = const_cast<uint64_t *>(A.ctl->pub);
+    uint32_t *stop_p = const_cast<uint32_t *>(&A.ctl->stop);
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    uint64_t last = t0;
+    bool frozen = false;
+    for (;;) {
+        const uint64_t now = __builtin_amdgcn_s_memrealtime();
+        const uint32_t stop = __hip_atomic_load(stop_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        bool moved = false, busy = false;
+        if (lane < kServeQueues) {
+            unsigned long long seen = ld_agent(&S[serve_seen(lane)]);
+            if (!frozen) {
+                const uint64_t p = __hip_atomic_load(pub + lane, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+                if (p > seen) {
+                    __hip_atomic_store(&S[serve_seen(lane)], (unsigned long long)p, __ATOMIC_RELEASE,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                    seen = p;
+                    moved = true;
+                }
+            }
+            busy = ld_agent(&S[serve_done(lane)]) != seen;
+        }
+        const bool any_busy = __ballot(busy) != 0ull;
+        if (__ballot(moved) != 0ull || any_busy) last = now;
+        if (stop || now - t0 > A.max_ticks || (!any_busy && now - last > A.idle_ticks)) frozen = true;
+        if (frozen && !any_busy) {
+            if (lane == 0)
+                __hip_atomic_store(&S[kServeExit], 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(16);
+    }
+}
+
+// K: 0 = u8 hop answers, 1 = wide index (u16 exact hops), 2 = weighted (u32).
+template <int K>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_serve(uint32_t n, const uint8_t *bpd, const ulonglong2 *bps,
+                                               const uint64_t *loff, const uint32_t *lab, const uint32_t *wdist,
+                                               ServeKernelArgs A)
+{
+    // A smaller label buffer than the batch kernel's leaves room for the
+    // descriptor's pairs within 20 KB of LDS per workgroup (eight per CU).
+    __shared__ uint32_t s_q[kQWaves][kServeLds];
+    // The descriptor's pairs: ids and label offsets, lane j's pair at [j].
+    __shared__ uint32_t s_ab[kQWaves][2][kServeGroup], s_len[kQWaves][2][kServeGroup];
+    __shared__ uint64_t s_off[kQWaves][2][kServeGroup];
+    const uint32_t lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (blockIdx.x == 0 && wv == 0) {
+        serve_poll(A);
+        return;
+    }
+    uint32_t *sbuf = s_q[wv];
+    unsigned long long *S = A.state;
+    const uint32_t q = blockIdx.x % kServeQueues;
+    for (;;) {
+        unsigned long long c = 0;
+        if (lane == 0) c = atomicAdd(&S[serve_claim(q)], 1ull);
+        c = __shfl(c, 0);
+        while (ld_agent(&S[serve_seen(q)]) <= c) {
+            if (ld_agent(&S[kServeExit])) return;
+            __builtin_amdgcn_s_sleep(8);
+        }
+        // The descriptor and the pairs live in mapped host memory that is
+        // not cached on the device.
+        ServeDesc *dp = const_cast<ServeDesc *>(A.desc) + (uint64_t)q * A.dcap + c % A.dcap;
+        const uint64_t pos = __hip_atomic_load(&dp->pos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        const uint32_t len = __hip_atomic_load(&dp->len, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        const uint32_t slot = __hip_atomic_load(&dp->slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        const uint64_t seq = __hip_atomic_load(&dp->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        const uint32_t rk = __hip_atomic_load(&dp->k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        const uint64_t phys = pos % A.cap;
+        {
+            uint32_t ia = 0, ib = 0;
+            if (lane < len) {
+                ia = A.ra[phys + lane];
+                ib = A.rb[phys + lane];
+            }
+            uint64_t oa = 0, ob = 0;
+            uint32_t la = 0, lb = 0;
+            if (lane < len && ia < n && ib < n) {
+                oa = loff[ia];
+                ob = loff[ib];
+                la = (uint32_t)(loff[ia + 1] - oa);
+                lb = (uint32_t)(loff[ib + 1] - ob);
+            }
+            s_ab[wv][0][lane] = ia;
+            s_ab[wv][1][lane] = ib;
+            s_off[wv][0][lane] = oa;
+            s_off[wv][1][lane] = ob;
+            s_len[wv][0][lane] = la;
+            s_len[wv][1][lane] = lb;
+            q_lds_sync();
+        }
+        uint32_t mine = 0;
+        for (uint32_t j = 0; j < len; ++j) {
+            const uint32_t a = s_ab[wv][0][j], b = s_ab[wv][1][j];
+            const uint64_t xa = s_off[wv][0][j], xb = s_off[wv][1][j];
+            const uint32_t ya = s_len[wv][0][j], yb = s_len[wv][1][j];
+            uint32_t d;
+            if (K == 2)
+                d = wpair_distance_at<kServeLds>(n, lab, wdist, a, b, xa, ya, xb, yb, A.ref_compat != 0, sbuf);
+            else
+                d = pair_distance_at<K == 1, kServeLds>(n, bpd, bps, lab, a, b, xa, ya, xb, yb, A.ref_compat != 0,
+                                                        sbuf);
+            if (lane == j) mine = d;
+        }
+        // The next descriptor's writes must not overtake these reads.
+        q_lds_sync();
+        if (lane < len) {
+            if (K == 0) static_cast<uint8_t *>(A.out)[phys + lane] = (uint8_t)mine;
+            if (K == 1) static_cast<uint16_t *>(A.out)[phys + lane] = (uint16_t)mine;
+            if (K == 2) static_cast<uint32_t *>(A.out)[phys + lane] = mine;
+        }
+        // The answers reach host memory before this wave's count does.
+        __threadfence_system();
+        if (lane == 0) {
+            const uint32_t prev = __hip_atomic_fetch_add(&A.rdone[slot], len, __ATOMIC_ACQ_REL,
+                                                         __HIP_MEMORY_SCOPE_AGENT);
+            if (prev + len == rk) {
+                // Last descriptor of the request: reset the slot's count
+                // (the host reuses the slot only after it sees the flag),
+                // then raise the flag.
+                __hip_atomic_store(&A.rdone[slot], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                __hip_atomic_store(&A.flags[slot], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            }
+            __hip_atomic_fetch_add(&S[serve_done(q)], 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+// Start of a launch: the previous launch exited only once every descriptor
+// it took in was done, so tickets restart at the done count.
+__global__ void k_serve_reset(unsigned long long *S)
+{
+    const uint32_t q = threadIdx.x;
+    if (q < kServeQueues) S[serve_claim(q)] = S[serve_done(q)];
+    if (q == 0) S[kServeExit] = 0ull;
+}
+
+}  // namespace
+
+uint32_t serve_resident_blocks_per_cu(const pado_gpu_index *idx)
+{
+    int nb = 0;
+    hipError_t e;
+    if (idx->weighted)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_serve<2>, 256, 0);
+    else if (idx->wide)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_serve<1>, 256, 0);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_serve<0>, 256, 0);
+    return (e == hipSuccess && nb > 0) ? (uint32_t)nb : 4u;
+}
+
+pado_gpu_status launch_serve_resident(const pado_gpu_index *idx, const ServeKernelArgs &a, uint32_t blocks,
+                                      hipStream_t st)
+{
+    hipLaunchKernelGGL(k_serve_reset, dim3(1), dim3(64), 0, st, a.state);
+    if (idx->weighted)
+        hipLaunchKernelGGL(k_serve<2>, dim3(blocks), dim3(256), 0, st, idx->n, nullptr, nullptr, idx->qoff.p,
+                           idx->qlab.p, idx->qwdist.p, a);
+    else if (idx->wide)
+        hipLaunchKernelGGL(k_serve<1>, dim3(blocks), dim3(256), 0, st, idx->n, idx->qbpd.p,
+                           (const ulonglong2 *)idx->qbps.p, idx->qoff.p, idx->qlab.p, nullptr, a);
+    else
+        hipLaunchKernelGGL(k_serve<0>, dim3(blocks), dim3(256), 0, st, idx->n, idx->qbpd.p,
+                           (const ulonglong2 *)idx->qbps.p, idx->qoff.p, idx->qlab.p, nullptr, a);
+    PADO_HIP(hipGetLastError());
+    return PADO_GPU_OK;
 }
 
 pado_gpu_status launch_wquery(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,

@@ -83,6 +83,7 @@ struct Request {
     bool claimed = false;  // a waiter is copying the answers out
     bool failed = false;   // its launch failed: no answers
     bool cancelled = false;  // answers are not wanted: released once no launch reads its slot
+    uint32_t slot = 0;       // resident mode: completion-flag slot
     double t_submit = 0, t_done = 0;
 };
 
@@ -127,6 +128,25 @@ struct pado_gpu_server {
     std::thread dispatcher, completer;
     pado_gpu_serve_stats st{};
     std::string error;
+    // Resident mode (PADO_GPU_SERVE_RESIDENT): no dispatcher; submit
+    // publishes descriptors to the resident kernel, the completer polls the
+    // completion flags and relaunches the kernel after it exits.
+    bool resident = false;
+    ServeCtl *h_ctl = nullptr;
+    ServeDesc *h_desc = nullptr;
+    uint64_t *h_flag = nullptr;
+    uint32_t dcap = 0;                       // descriptors per queue ring
+    uint64_t qpub[kServeQueues] = {};        // descriptors published per queue
+    uint32_t qnext = 0;                      // next queue (round robin over descriptors)
+    std::vector<uint32_t> free_slots;
+    std::deque<Request *> rinflight;         // published, not done
+    DevBuf<unsigned long long> rstate;
+    DevBuf<uint32_t> rdone;
+    ServeKernelArgs rargs{};
+    uint32_t rblocks = 0;
+    hipStream_t rstream = nullptr;
+    hipEvent_t rexit = nullptr;
+    bool rlive = false;                      // a resident launch may still run
 };
 
 namespace {
@@ -316,6 +336,124 @@ void completer_loop(pado_gpu_server *s)
     }
 }
 
+// ---- resident mode ---------------------------------------------------------
+
+// Enqueues a resident launch unless one may still run (mu held).
+void resident_launch(pado_gpu_server *s)
+{
+    if (s->rlive || !s->error.empty()) return;
+    pado_gpu_status e = launch_serve_resident(s->idx, s->rargs, s->rblocks, s->rstream);
+    hipError_t r = e == PADO_GPU_OK ? hipEventRecord(s->rexit, s->rstream) : hipErrorLaunchFailure;
+    if (e != PADO_GPU_OK || r != hipSuccess) {
+        s->error = e != PADO_GPU_OK ? std::string(pado_gpu_last_error())
+                                    : std::string("event record failed: ") + hipGetErrorString(r);
+        return;
+    }
+    s->rlive = true;
+    s->st.launches += 1;
+}
+
+// Cuts request r into descriptors, spreads them over the queues and raises
+// the queues' published counts (mu held; r's pairs are in the ring).
+void resident_publish(pado_gpu_server *s, Request *r)
+{
+    uint32_t touched = 0;
+    for (uint64_t o = 0; o < r->k; o += kServeGroup) {
+        const uint32_t q = s->qnext;
+        s->qnext = (s->qnext + 1) % kServeQueues;
+        ServeDesc &d = s->h_desc[(uint64_t)q * s->dcap + s->qpub[q] % s->dcap];
+        d.pos = r->start + o;
+        d.len = (uint32_t)std::min<uint64_t>(kServeGroup, r->k - o);
+        d.slot = r->slot;
+        d.seq = r->id;
+        d.k = (uint32_t)r->k;
+        s->qpub[q] += 1;
+        touched |= 1u << q;
+    }
+    // The descriptors (and the pairs copied in before) are visible before
+    // any count that covers them.
+    std::atomic_thread_fence(std::memory_order_release);
+    for (uint32_t q = 0; q < kServeQueues; ++q)
+        if (touched >> q & 1u) __atomic_store_n(&s->h_ctl->pub[q], s->qpub[q], __ATOMIC_RELEASE);
+    r->state = 1;
+    s->rinflight.push_back(r);
+    resident_launch(s);
+}
+
+void resident_finish(pado_gpu_server *s, Request *r, double t, bool failed, bool &release)
+{
+    r->state = 2;
+    r->failed = failed;
+    r->t_done = t;
+    s->free_slots.push_back(r->slot);
+    s->st.requests_done += 1;
+    s->st.pairs_done += r->k;
+    if (r->cancelled) {
+        r->state = 3;  // nobody will wait for it
+        release = true;
+    }
+}
+
+// Completion: polls the flags of published requests (oldest first; they
+// finish roughly in publication order) and keeps a resident launch running
+// while any is outstanding.
+void resident_completer_loop(pado_gpu_server *s)
+{
+    (void)hipSetDevice(s->ctx->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        if (s->rinflight.empty()) {
+            if (s->stopping) break;
+            s->cv_launch.wait(lk);
+            continue;
+        }
+        bool progressed = false, release = false;
+        const double t = now_ms();
+        size_t scanned = 0;
+        for (auto it = s->rinflight.begin(); it != s->rinflight.end() && scanned < 1024; ++scanned) {
+            Request *r = *it;
+            if (__atomic_load_n(&s->h_flag[r->slot], __ATOMIC_ACQUIRE) == r->id) {
+                resident_finish(s, r, t, false, release);
+                it = s->rinflight.erase(it);
+                progressed = true;
+            } else {
+                ++it;
+            }
+        }
+        if (s->rlive) {
+            const hipError_t q = hipEventQuery(s->rexit);
+            if (q == hipSuccess) {
+                s->rlive = false;
+            } else if (q != hipErrorNotReady) {
+                // The launch failed: no outstanding request gets answers.
+                if (s->error.empty()) s->error = std::string("resident serving launch failed: ") + hipGetErrorString(q);
+                s->rlive = false;
+                for (Request *r : s->rinflight) resident_finish(s, r, t, true, release);
+                s->rinflight.clear();
+                progressed = true;
+            }
+        }
+        if (!s->rlive && !s->rinflight.empty()) {
+            if (s->error.empty()) {
+                resident_launch(s);
+            } else {
+                for (Request *r : s->rinflight) resident_finish(s, r, t, true, release);
+                s->rinflight.clear();
+                progressed = true;
+            }
+        }
+        if (release) release_consumed(s);
+        lk.unlock();
+        if (progressed) {
+            s->cv_done.notify_all();
+            s->cv_space.notify_all();
+        } else {
+            std::this_thread::yield();
+        }
+        lk.lock();
+    }
+}
+
 // Blocks until request `id` is done, copies its answers, releases its slot.
 // The copy runs outside the lock (the slot is not reused before release): a
 // caller buffer that page-faults on first touch holds back only this caller,
@@ -365,6 +503,69 @@ pado_gpu_status wait_request(pado_gpu_server *s, uint64_t id, void *out, double 
     return PADO_GPU_OK;
 }
 
+// Resident-mode resources: control words, descriptor rings and completion
+// flags in mapped host memory, queue state and per-slot counts on the
+// device, the launch stream and its exit event, the kernel arguments.
+pado_gpu_status resident_setup(pado_gpu_server *s)
+{
+    const unsigned mf = hipHostMallocMapped | hipHostMallocCoherent;
+    const uint32_t nreq = s->cfg.max_requests;
+    // In-flight descriptors: at most cap / kServeGroup + max_requests, dealt
+    // round robin, so each queue holds at most an eighth of that plus one.
+    s->dcap = (uint32_t)((s->cap / kServeGroup + nreq) / kServeQueues + 64);
+    if (hipHostMalloc((void **)&s->h_ctl, sizeof(ServeCtl), mf) != hipSuccess ||
+        hipHostMalloc((void **)&s->h_desc, sizeof(ServeDesc) * kServeQueues * s->dcap, mf) != hipSuccess ||
+        hipHostMalloc((void **)&s->h_flag, sizeof(uint64_t) * nreq, mf) != hipSuccess) {
+        set_error("pinned control allocation failed");
+        return PADO_GPU_ERR_NOMEM;
+    }
+    std::memset(s->h_ctl, 0, sizeof(ServeCtl));
+    std::memset(s->h_flag, 0, sizeof(uint64_t) * nreq);
+    ServeKernelArgs &A = s->rargs;
+    void *m_out = nullptr, *m_ctl = nullptr, *m_desc = nullptr, *m_flag = nullptr;
+    if (hipHostGetDevicePointer(&m_out, s->h_out, 0) != hipSuccess ||
+        hipHostGetDevicePointer(&m_ctl, s->h_ctl, 0) != hipSuccess ||
+        hipHostGetDevicePointer(&m_desc, s->h_desc, 0) != hipSuccess ||
+        hipHostGetDevicePointer(&m_flag, s->h_flag, 0) != hipSuccess) {
+        set_error("control memory is not mapped for the device");
+        return PADO_GPU_ERR_HIP;
+    }
+    PADO_TRY(s->rstate.alloc(kServeStateWords));
+    PADO_TRY(s->rdone.alloc(nreq));
+    PADO_HIP(hipStreamCreateWithFlags(&s->rstream, hipStreamNonBlocking));
+    PADO_HIP(hipEventCreateWithFlags(&s->rexit, hipEventDisableTiming));
+    PADO_HIP(hipMemsetAsync(s->rstate.p, 0, sizeof(unsigned long long) * kServeStateWords, s->rstream));
+    PADO_HIP(hipMemsetAsync(s->rdone.p, 0, sizeof(uint32_t) * nreq, s->rstream));
+    PADO_HIP(hipStreamSynchronize(s->rstream));
+    s->free_slots.resize(nreq);
+    for (uint32_t i = 0; i < nreq; ++i) s->free_slots[i] = nreq - 1 - i;
+    int khz = 0;
+    if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s->ctx->device) != hipSuccess || khz <= 0)
+        khz = 100000;  // the 100 MHz constant clock of CDNA devices
+    A.ra = s->m_a;
+    A.rb = s->m_b;
+    A.out = m_out;
+    A.cap = s->cap;
+    A.ctl = static_cast<const ServeCtl *>(m_ctl);
+    A.desc = static_cast<const ServeDesc *>(m_desc);
+    A.dcap = s->dcap;
+    A.flags = static_cast<uint64_t *>(m_flag);
+    A.state = s->rstate.p;
+    A.rdone = s->rdone.p;
+    // A launch leaves after 20 ms without work, and drains and leaves after
+    // 100 ms in any case (the completer relaunches it while work is
+    // outstanding), so device-wide synchronisation elsewhere in the process
+    // never waits long behind it.
+    A.idle_ticks = (uint64_t)khz * 20;
+    A.max_ticks = (uint64_t)khz * 100;
+    A.ref_compat = (!s->idx->wide && (s->cfg.flags & PADO_GPU_QUERY_REF_COMPAT) != 0) ? 1 : 0;
+    // Workgroup b serves queue b % kServeQueues: every queue needs at least
+    // one workgroup of workers.
+    s->rblocks = serve_resident_blocks_per_cu(s->idx) * (s->ctx->num_cu > 0 ? (uint32_t)s->ctx->num_cu : 256u);
+    s->rblocks = std::max<uint32_t>(s->rblocks, 2 * kServeQueues);
+    return PADO_GPU_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -405,10 +606,16 @@ pado_gpu_status pado_gpu_serve_start(pado_gpu_ctx *ctx, pado_gpu_index *idx, con
     if (cap < 1024) cap = 1024;
     s->cap = cap;
     s->st.staging_pairs = cap;
+    s->resident = (s->cfg.flags & PADO_GPU_SERVE_RESIDENT) != 0;
     auto fail = [&](pado_gpu_status code) {
         if (s->h_a) (void)hipHostFree(s->h_a);
         if (s->h_b) (void)hipHostFree(s->h_b);
         if (s->h_out) (void)hipHostFree(s->h_out);
+        if (s->h_ctl) (void)hipHostFree(s->h_ctl);
+        if (s->h_desc) (void)hipHostFree(s->h_desc);
+        if (s->h_flag) (void)hipHostFree(s->h_flag);
+        if (s->rstream) (void)hipStreamDestroy(s->rstream);
+        if (s->rexit) (void)hipEventDestroy(s->rexit);
         for (auto q : s->streams) (void)hipStreamDestroy(q);
         for (auto e : s->ev_start) (void)hipEventDestroy(e);
         for (auto e : s->ev_end) (void)hipEventDestroy(e);
@@ -418,9 +625,12 @@ pado_gpu_status pado_gpu_serve_start(pado_gpu_ctx *ctx, pado_gpu_index *idx, con
     // The id rings are read by k_stage_pairs over the host link: coherent
     // (not cached on the device), since every lap rewrites the same addresses.
     const unsigned ring_flags = hipHostMallocMapped | hipHostMallocCoherent;
+    // In resident mode the kernel also writes the answers straight into the
+    // (then mapped) answer ring.
     if (hipHostMalloc((void **)&s->h_a, cap * 4, ring_flags) != hipSuccess ||
         hipHostMalloc((void **)&s->h_b, cap * 4, ring_flags) != hipSuccess ||
-        hipHostMalloc((void **)&s->h_out, cap * s->es, hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&s->h_out, cap * s->es, s->resident ? ring_flags : hipHostMallocDefault) !=
+            hipSuccess) {
         set_error("pinned staging allocation failed");
         return fail(PADO_GPU_ERR_NOMEM);
     }
@@ -428,6 +638,13 @@ pado_gpu_status pado_gpu_serve_start(pado_gpu_ctx *ctx, pado_gpu_index *idx, con
         hipHostGetDevicePointer((void **)&s->m_b, s->h_b, 0) != hipSuccess) {
         set_error("staging ring is not mapped for the device");
         return fail(PADO_GPU_ERR_HIP);
+    }
+    if (s->resident) {
+        pado_gpu_status rs = resident_setup(s);
+        if (rs != PADO_GPU_OK) return fail(rs);
+        s->completer = std::thread(resident_completer_loop, s);
+        *out = s;
+        return PADO_GPU_OK;
     }
     pado_gpu_status a1 = s->d_a.alloc(cap), a2 = s->d_b.alloc(cap), a3 = s->d_out.alloc(cap * s->es);
     pado_gpu_status a4 = s->work.alloc(s->cfg.streams);
@@ -465,7 +682,7 @@ pado_gpu_status pado_gpu_serve_submit(pado_gpu_server *s, const uint32_t *a, con
         set_error("null argument");
         return PADO_GPU_ERR_ARG;
     }
-    if (k > s->cap) {
+    if (k > s->cap || (s->resident && k > 0xFFFFFFFFull)) {
         set_error("request larger than the staging ring");
         return PADO_GPU_ERR_ARG;
     }
@@ -481,7 +698,8 @@ pado_gpu_status pado_gpu_serve_submit(pado_gpu_server *s, const uint32_t *a, con
     }
     rewind_if_empty(s);
     const uint64_t start = place(s->tail, k, s->cap);
-    if (s->by_id.size() >= s->cfg.max_requests || start + k - s->head > s->cap) {
+    if (s->by_id.size() >= s->cfg.max_requests || start + k - s->head > s->cap ||
+        (s->resident && s->free_slots.empty())) {
         s->st.rejected += 1;
         set_error("no free request slot or staging memory");
         return PADO_GPU_ERR_BUSY;
@@ -490,6 +708,10 @@ pado_gpu_status pado_gpu_serve_submit(pado_gpu_server *s, const uint32_t *a, con
     r->id = s->next_id++;
     r->start = start;
     r->k = k;
+    if (s->resident) {
+        r->slot = s->free_slots.back();
+        s->free_slots.pop_back();
+    }
     r->t_submit = now_ms();
     s->tail = start + k;
     s->live.push_back(r);
@@ -507,9 +729,24 @@ pado_gpu_status pado_gpu_serve_submit(pado_gpu_server *s, const uint32_t *a, con
     if (r->cancelled) {
         // Cancelled while its pairs were copied in: never launched.
         r->state = 3;
+        if (s->resident) s->free_slots.push_back(r->slot);
         release_consumed(s);
         lk.unlock();
         s->cv_space.notify_all();
+        return PADO_GPU_OK;
+    }
+    if (s->resident) {
+        if (k == 0) {
+            // Nothing to answer: done at once.
+            bool release = false;
+            resident_finish(s, r, now_ms(), false, release);
+            lk.unlock();
+            s->cv_done.notify_all();
+            return PADO_GPU_OK;
+        }
+        resident_publish(s, r);
+        lk.unlock();
+        s->cv_launch.notify_one();
         return PADO_GPU_OK;
     }
     s->pending.push_back(r);
@@ -686,6 +923,17 @@ void pado_gpu_serve_stop(pado_gpu_server *s)
     if (s->dispatcher.joinable()) s->dispatcher.join();
     s->cv_launch.notify_all();
     if (s->completer.joinable()) s->completer.join();
+    if (s->resident) {
+        // Every published request is done: the resident launch (if any)
+        // drains nothing more and leaves.
+        __atomic_store_n(&s->h_ctl->stop, 1u, __ATOMIC_RELEASE);
+        (void)hipStreamSynchronize(s->rstream);
+        (void)hipStreamDestroy(s->rstream);
+        (void)hipEventDestroy(s->rexit);
+        (void)hipHostFree(s->h_ctl);
+        (void)hipHostFree(s->h_desc);
+        (void)hipHostFree(s->h_flag);
+    }
     for (auto q : s->streams) {
         (void)hipStreamSynchronize(q);
         (void)hipStreamDestroy(q);
