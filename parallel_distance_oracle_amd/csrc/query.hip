@@ -424,6 +424,21 @@ __device__ inline unsigned long long ld_agent(unsigned long long *p)
     return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// The value of the wave's first active lane, as a wave-uniform (scalar)
+// value.  Every control decision of the serving loop goes through it: the
+// compiler must see the ticket, the published count, the exit flag and the
+// descriptor as uniform, or it structures the loops for lanes that could
+// part ways -- and a lane set that re-enters the wait without lane 0 would
+// broadcast a stale ticket.
+__device__ inline uint64_t uniform64(uint64_t x)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ inline uint32_t uniform32(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
 __device__ void serve_poll(const ServeKernelArgs &A)
 {
     const uint32_t lane = threadIdx.x & 63;
@@ -435,7 +450,7 @@ __device__ void serve_poll(const ServeKernelArgs &A)
     bool frozen = false;
     for (;;) {
         const uint64_t now = __builtin_amdgcn_s_memrealtime();
-        const uint32_t stop = __hip_atomic_load(stop_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        const uint32_t stop = uniform32(__hip_atomic_load(stop_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
         bool moved = false, busy = false;
         if (lane < kServeQueues) {
             unsigned long long seen = ld_agent(&S[serve_seen(lane)]);
@@ -454,8 +469,7 @@ __device__ void serve_poll(const ServeKernelArgs &A)
         if (__ballot(moved) != 0ull || any_busy) last = now;
         if (stop || now - t0 > A.max_ticks || (!any_busy && now - last > A.idle_ticks)) frozen = true;
         if (frozen && !any_busy) {
-            if (lane == 0)
-                __hip_atomic_store(&S[kServeExit], 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&S[kServeExit], 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             return;
         }
         __builtin_amdgcn_s_sleep(16);
@@ -483,21 +497,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
     unsigned long long *S = A.state;
     const uint32_t q = blockIdx.x % kServeQueues;
     for (;;) {
-        unsigned long long c = 0;
-        if (lane == 0) c = atomicAdd(&S[serve_claim(q)], 1ull);
-        c = __shfl(c, 0);
-        while (ld_agent(&S[serve_seen(q)]) <= c) {
-            if (ld_agent(&S[kServeExit])) return;
+        // Every lane takes part in the loop's atomics with lane 0 alone
+        // contributing (no lane-0-only branch for the loop structure to split
+        // on); lane 0 is the first active lane, so its result is the ticket.
+        const uint64_t c = uniform64(atomicAdd(&S[serve_claim(q)], lane == 0 ? 1ull : 0ull));
+        bool leave = false;
+        for (;;) {
+            if (uniform64(ld_agent(&S[serve_seen(q)])) > c) break;
+            if (uniform64(ld_agent(&S[kServeExit])) != 0) {
+                leave = true;
+                break;
+            }
             __builtin_amdgcn_s_sleep(8);
         }
+        if (leave) return;
         // The descriptor and the pairs live in mapped host memory that is
         // not cached on the device.
         ServeDesc *dp = const_cast<ServeDesc *>(A.desc) + (uint64_t)q * A.dcap + c % A.dcap;
-        const uint64_t pos = __hip_atomic_load(&dp->pos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        const uint32_t len = __hip_atomic_load(&dp->len, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        const uint32_t slot = __hip_atomic_load(&dp->slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        const uint64_t seq = __hip_atomic_load(&dp->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        const uint32_t rk = __hip_atomic_load(&dp->k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        const uint64_t pos = uniform64(__hip_atomic_load(&dp->pos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
+        const uint32_t len = uniform32(__hip_atomic_load(&dp->len, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
+        const uint32_t slot = uniform32(__hip_atomic_load(&dp->slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
+        const uint64_t seq = uniform64(__hip_atomic_load(&dp->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
+        const uint32_t rk = uniform32(__hip_atomic_load(&dp->k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
         const uint64_t phys = pos % A.cap;
         {
             uint32_t ia = 0, ib = 0;
@@ -543,18 +564,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
         }
         // The answers reach host memory before this wave's count does.
         __threadfence_system();
-        if (lane == 0) {
-            const uint32_t prev = __hip_atomic_fetch_add(&A.rdone[slot], len, __ATOMIC_ACQ_REL,
-                                                         __HIP_MEMORY_SCOPE_AGENT);
-            if (prev + len == rk) {
-                // Last descriptor of the request: reset the slot's count
-                // (the host reuses the slot only after it sees the flag),
-                // then raise the flag.
-                __hip_atomic_store(&A.rdone[slot], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(&A.flags[slot], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
-            }
-            __hip_atomic_fetch_add(&S[serve_done(q)], 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t prev = uniform32(__hip_atomic_fetch_add(&A.rdone[slot], lane == 0 ? len : 0u,
+                                                               __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT));
+        if (prev + len == rk) {
+            // Last descriptor of the request: reset the slot's count (the
+            // host reuses the slot only after it sees the flag), then raise
+            // the flag.  Every lane stores the same value.
+            __hip_atomic_store(&A.rdone[slot], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&A.flags[slot], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
         }
+        __hip_atomic_fetch_add(&S[serve_done(q)], lane == 0 ? 1ull : 0ull, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
