@@ -419,10 +419,31 @@ uint32_t query_ws_blocks_per_cu(bool weighted)
 // The host relaunches the kernel when work arrives after an exit.
 namespace {
 
+// Relaxed device-coherent load.  No acquire anywhere in the serving loop: an
+// acquire at device scope invalidates the L2, and thousands of polling waves
+// doing that would strip the cache from the waves that answer queries.
+// Nothing a wave reads after a poll can be stale without it: the descriptors,
+// pairs and answers are in uncached host memory, read and written with
+// system-scope accesses, and the index does not change while it is served.
 __device__ inline unsigned long long ld_agent(unsigned long long *p)
 {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+
+template <class T>
+__device__ inline T ld_sys(const T *p)
+{
+    return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+template <class T>
+__device__ inline void st_sys(T *p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+// Waits until this wave's earlier memory operations (stores included) are done.
+__device__ inline void wait_memory() { __builtin_amdgcn_s_waitcnt(0); }
 
 // The value of the wave's first active lane, as a wave-uniform (scalar)
 // value.  Every control decision of the serving loop goes through it: the
@@ -450,14 +471,14 @@ __device__ void serve_poll(const ServeKernelArgs &A)
     bool frozen = false;
     for (;;) {
         const uint64_t now = __builtin_amdgcn_s_memrealtime();
-        const uint32_t stop = uniform32(__hip_atomic_load(stop_p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
+        const uint32_t stop = uniform32(ld_sys(stop_p));
         bool moved = false, busy = false;
         if (lane < kServeQueues) {
             unsigned long long seen = ld_agent(&S[serve_seen(lane)]);
             if (!frozen) {
-                const uint64_t p = __hip_atomic_load(pub + lane, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_SYSTEM);
+                const uint64_t p = ld_sys(pub + lane);
                 if (p > seen) {
-                    __hip_atomic_store(&S[serve_seen(lane)], (unsigned long long)p, __ATOMIC_RELEASE,
+                    __hip_atomic_store(&S[serve_seen(lane)], (unsigned long long)p, __ATOMIC_RELAXED,
                                        __HIP_MEMORY_SCOPE_AGENT);
                     seen = p;
                     moved = true;
@@ -469,7 +490,7 @@ __device__ void serve_poll(const ServeKernelArgs &A)
         if (__ballot(moved) != 0ull || any_busy) last = now;
         if (stop || now - t0 > A.max_ticks || (!any_busy && now - last > A.idle_ticks)) frozen = true;
         if (frozen && !any_busy) {
-            __hip_atomic_store(&S[kServeExit], 1ull, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&S[kServeExit], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             return;
         }
         __builtin_amdgcn_s_sleep(16);
@@ -508,23 +529,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
                 leave = true;
                 break;
             }
-            __builtin_amdgcn_s_sleep(8);
+            // About a microsecond between polls: waiting waves stay off the
+            // memory system the working ones use.
+            __builtin_amdgcn_s_sleep(40);
         }
         if (leave) return;
         // The descriptor and the pairs live in mapped host memory that is
         // not cached on the device.
         ServeDesc *dp = const_cast<ServeDesc *>(A.desc) + (uint64_t)q * A.dcap + c % A.dcap;
-        const uint64_t pos = uniform64(__hip_atomic_load(&dp->pos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
-        const uint32_t len = uniform32(__hip_atomic_load(&dp->len, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
-        const uint32_t slot = uniform32(__hip_atomic_load(&dp->slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
-        const uint64_t seq = uniform64(__hip_atomic_load(&dp->seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
-        const uint32_t rk = uniform32(__hip_atomic_load(&dp->k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM));
+        const uint64_t pos = uniform64(ld_sys(&dp->pos));
+        const uint32_t len = uniform32(ld_sys(&dp->len));
+        const uint32_t slot = uniform32(ld_sys(&dp->slot));
+        const uint64_t seq = uniform64(ld_sys(&dp->seq));
+        const uint32_t rk = uniform32(ld_sys(&dp->k));
         const uint64_t phys = pos % A.cap;
         {
             uint32_t ia = 0, ib = 0;
             if (lane < len) {
-                ia = A.ra[phys + lane];
-                ib = A.rb[phys + lane];
+                ia = ld_sys(A.ra + phys + lane);
+                ib = ld_sys(A.rb + phys + lane);
             }
             uint64_t oa = 0, ob = 0;
             uint32_t la = 0, lb = 0;
@@ -558,22 +581,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
         // The next descriptor's writes must not overtake these reads.
         q_lds_sync();
         if (lane < len) {
-            if (K == 0) static_cast<uint8_t *>(A.out)[phys + lane] = (uint8_t)mine;
-            if (K == 1) static_cast<uint16_t *>(A.out)[phys + lane] = (uint16_t)mine;
-            if (K == 2) static_cast<uint32_t *>(A.out)[phys + lane] = mine;
+            if (K == 0) st_sys(static_cast<uint8_t *>(A.out) + phys + lane, (uint8_t)mine);
+            if (K == 1) st_sys(static_cast<uint16_t *>(A.out) + phys + lane, (uint16_t)mine);
+            if (K == 2) st_sys(static_cast<uint32_t *>(A.out) + phys + lane, mine);
         }
-        // The answers reach host memory before this wave's count does.
-        __threadfence_system();
+        // The answers have reached host memory before this wave's count
+        // moves (the stores write through to the host; waiting for them is
+        // all the ordering they need).
+        wait_memory();
         const uint32_t prev = uniform32(__hip_atomic_fetch_add(&A.rdone[slot], lane == 0 ? len : 0u,
-                                                               __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT));
+                                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
         if (prev + len == rk) {
             // Last descriptor of the request: reset the slot's count (the
             // host reuses the slot only after it sees the flag), then raise
             // the flag.  Every lane stores the same value.
             __hip_atomic_store(&A.rdone[slot], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&A.flags[slot], seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+            wait_memory();
+            st_sys(A.flags + slot, seq);
+            wait_memory();
         }
-        __hip_atomic_fetch_add(&S[serve_done(q)], lane == 0 ? 1ull : 0ull, __ATOMIC_RELEASE,
+        __hip_atomic_fetch_add(&S[serve_done(q)], lane == 0 ? 1ull : 0ull, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
     }
 }
