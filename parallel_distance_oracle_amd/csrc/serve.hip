@@ -394,12 +394,25 @@ void resident_finish(pado_gpu_server *s, Request *r, double t, bool failed, bool
     }
 }
 
-// Completion: polls the flags of published requests (oldest first; they
-// finish roughly in publication order) and keeps a resident launch running
-// while any is outstanding.
+// Completion: polls the flags of published requests and keeps a resident
+// launch running while any is outstanding.  The flags are read without the
+// lock (a snapshot of the oldest outstanding requests -- they finish roughly
+// in publication order), so submitters and waiters are not held up by the
+// polling; the lock is taken only to retire finished requests and to check
+// the launch.
 void resident_completer_loop(pado_gpu_server *s)
 {
     (void)hipSetDevice(s->ctx->device);
+    constexpr size_t kScan = 256;
+    struct Snap {
+        Request *r;
+        uint32_t slot;
+        uint64_t id;
+    };
+    std::vector<Snap> snap;
+    std::vector<Request *> finished;
+    snap.reserve(kScan);
+    uint32_t idle_rounds = 0;
     std::unique_lock<std::mutex> lk(s->mu);
     for (;;) {
         if (s->rinflight.empty()) {
@@ -407,20 +420,33 @@ void resident_completer_loop(pado_gpu_server *s)
             s->cv_launch.wait(lk);
             continue;
         }
-        bool progressed = false, release = false;
-        const double t = now_ms();
-        size_t scanned = 0;
-        for (auto it = s->rinflight.begin(); it != s->rinflight.end() && scanned < 1024; ++scanned) {
-            Request *r = *it;
-            if (__atomic_load_n(&s->h_flag[r->slot], __ATOMIC_ACQUIRE) == r->id) {
-                resident_finish(s, r, t, false, release);
-                it = s->rinflight.erase(it);
-                progressed = true;
-            } else {
-                ++it;
-            }
+        snap.clear();
+        for (size_t i = 0; i < s->rinflight.size() && i < kScan; ++i) {
+            Request *r = s->rinflight[i];
+            snap.push_back({r, r->slot, r->id});
         }
-        if (s->rlive) {
+        lk.unlock();
+        finished.clear();
+        for (const Snap &x : snap)
+            if (__atomic_load_n(&s->h_flag[x.slot], __ATOMIC_ACQUIRE) == x.id) finished.push_back(x.r);
+        const double t = now_ms();
+        lk.lock();
+        bool progressed = false, release = false;
+        if (!finished.empty()) {
+            // Only this thread removes entries, so every finished request is
+            // still outstanding; erase them in one pass.
+            for (Request *r : finished) resident_finish(s, r, t, false, release);
+            size_t w = 0;
+            for (size_t i = 0; i < s->rinflight.size(); ++i)
+                if (s->rinflight[i]->state == 1) s->rinflight[w++] = s->rinflight[i];
+            s->rinflight.resize(w);
+            progressed = true;
+            idle_rounds = 0;
+        }
+        // The launch's state: every round while nothing finishes for a while
+        // (it may have left), otherwise every 64th.
+        if (s->rlive && (++idle_rounds > 64 || finished.empty())) {
+            idle_rounds = 0;
             const hipError_t q = hipEventQuery(s->rexit);
             if (q == hipSuccess) {
                 s->rlive = false;
@@ -443,14 +469,16 @@ void resident_completer_loop(pado_gpu_server *s)
             }
         }
         if (release) release_consumed(s);
-        lk.unlock();
         if (progressed) {
+            lk.unlock();
             s->cv_done.notify_all();
             s->cv_space.notify_all();
+            lk.lock();
         } else {
+            lk.unlock();
             std::this_thread::yield();
+            lk.lock();
         }
-        lk.lock();
     }
 }
 
@@ -564,6 +592,93 @@ pado_gpu_status resident_setup(pado_gpu_server *s)
     s->rblocks = serve_resident_blocks_per_cu(s->idx) * (s->ctx->num_cu > 0 ? (uint32_t)s->ctx->num_cu : 256u);
     s->rblocks = std::max<uint32_t>(s->rblocks, 2 * kServeQueues);
     return PADO_GPU_OK;
+}
+
+// Admission: ring space and a request slot (plus a completion-flag slot in
+// resident mode) for k pairs.  The request is not yet visible to the
+// dispatcher or the kernel.
+pado_gpu_status submit_reserve(pado_gpu_server *s, uint64_t k, Request **out)
+{
+    if (k > s->cap || (s->resident && k > 0xFFFFFFFFull)) {
+        set_error("request larger than the staging ring");
+        return PADO_GPU_ERR_ARG;
+    }
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (s->stopping) {
+        set_error("server stopping");
+        return PADO_GPU_ERR_STATE;
+    }
+    if (!s->error.empty()) {
+        // A launch failed: the device state is suspect, admit nothing more.
+        set_error("server failed: " + s->error);
+        return PADO_GPU_ERR_STATE;
+    }
+    rewind_if_empty(s);
+    const uint64_t start = place(s->tail, k, s->cap);
+    if (s->by_id.size() >= s->cfg.max_requests || start + k - s->head > s->cap ||
+        (s->resident && s->free_slots.empty())) {
+        s->st.rejected += 1;
+        set_error("no free request slot or staging memory");
+        return PADO_GPU_ERR_BUSY;
+    }
+    auto *r = new Request();
+    r->id = s->next_id++;
+    r->start = start;
+    r->k = k;
+    if (s->resident) {
+        r->slot = s->free_slots.back();
+        s->free_slots.pop_back();
+    }
+    r->t_submit = now_ms();
+    s->tail = start + k;
+    s->live.push_back(r);
+    s->by_id[r->id] = r;
+    *out = r;
+    return PADO_GPU_OK;
+}
+
+// The request's pairs into its ring space (no lock: the space is its own).
+void submit_copy(pado_gpu_server *s, const Request *r, const uint32_t *a, const uint32_t *b)
+{
+    const uint64_t phys = r->start % s->cap;
+    if (r->k) {
+        std::memcpy(s->h_a + phys, a, r->k * 4);
+        std::memcpy(s->h_b + phys, b, r->k * 4);
+    }
+}
+
+// Makes a reserved, filled request visible: queued for the dispatcher, or
+// published to the resident kernel.
+void submit_commit(pado_gpu_server *s, Request *r)
+{
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (r->cancelled) {
+        // Cancelled while its pairs were copied in: never launched.
+        r->state = 3;
+        if (s->resident) s->free_slots.push_back(r->slot);
+        release_consumed(s);
+        lk.unlock();
+        s->cv_space.notify_all();
+        return;
+    }
+    if (s->resident) {
+        if (r->k == 0) {
+            // Nothing to answer: done at once.
+            bool release = false;
+            resident_finish(s, r, now_ms(), false, release);
+            lk.unlock();
+            s->cv_done.notify_all();
+            return;
+        }
+        resident_publish(s, r);
+        lk.unlock();
+        s->cv_launch.notify_one();
+        return;
+    }
+    s->pending.push_back(r);
+    s->pending_pairs += r->k;
+    lk.unlock();
+    s->cv_work.notify_one();
 }
 
 }  // namespace
@@ -682,77 +797,14 @@ pado_gpu_status pado_gpu_serve_submit(pado_gpu_server *s, const uint32_t *a, con
         set_error("null argument");
         return PADO_GPU_ERR_ARG;
     }
-    if (k > s->cap || (s->resident && k > 0xFFFFFFFFull)) {
-        set_error("request larger than the staging ring");
-        return PADO_GPU_ERR_ARG;
-    }
-    std::unique_lock<std::mutex> lk(s->mu);
-    if (s->stopping) {
-        set_error("server stopping");
-        return PADO_GPU_ERR_STATE;
-    }
-    if (!s->error.empty()) {
-        // A launch failed: the device state is suspect, admit nothing more.
-        set_error("server failed: " + s->error);
-        return PADO_GPU_ERR_STATE;
-    }
-    rewind_if_empty(s);
-    const uint64_t start = place(s->tail, k, s->cap);
-    if (s->by_id.size() >= s->cfg.max_requests || start + k - s->head > s->cap ||
-        (s->resident && s->free_slots.empty())) {
-        s->st.rejected += 1;
-        set_error("no free request slot or staging memory");
-        return PADO_GPU_ERR_BUSY;
-    }
-    auto *r = new Request();
-    r->id = s->next_id++;
-    r->start = start;
-    r->k = k;
-    if (s->resident) {
-        r->slot = s->free_slots.back();
-        s->free_slots.pop_back();
-    }
-    r->t_submit = now_ms();
-    s->tail = start + k;
-    s->live.push_back(r);
-    s->by_id[r->id] = r;
+    Request *r = nullptr;
+    PADO_TRY(submit_reserve(s, k, &r));
     *req_id = r->id;
     // The slot is reserved; the pairs are copied in without the lock and the
-    // request becomes visible to the dispatcher only after the copy.
-    lk.unlock();
-    const uint64_t phys = start % s->cap;
-    if (k) {
-        std::memcpy(s->h_a + phys, a, k * 4);
-        std::memcpy(s->h_b + phys, b, k * 4);
-    }
-    lk.lock();
-    if (r->cancelled) {
-        // Cancelled while its pairs were copied in: never launched.
-        r->state = 3;
-        if (s->resident) s->free_slots.push_back(r->slot);
-        release_consumed(s);
-        lk.unlock();
-        s->cv_space.notify_all();
-        return PADO_GPU_OK;
-    }
-    if (s->resident) {
-        if (k == 0) {
-            // Nothing to answer: done at once.
-            bool release = false;
-            resident_finish(s, r, now_ms(), false, release);
-            lk.unlock();
-            s->cv_done.notify_all();
-            return PADO_GPU_OK;
-        }
-        resident_publish(s, r);
-        lk.unlock();
-        s->cv_launch.notify_one();
-        return PADO_GPU_OK;
-    }
-    s->pending.push_back(r);
-    s->pending_pairs += k;
-    lk.unlock();
-    s->cv_work.notify_one();
+    // request becomes visible to the dispatcher (or the resident kernel)
+    // only after the copy.
+    submit_copy(s, r, a, b);
+    submit_commit(s, r);
     return PADO_GPU_OK;
 }
 
@@ -848,35 +900,53 @@ pado_gpu_status pado_gpu_serve_replay(pado_gpu_server *s, const uint32_t *a, con
     std::atomic<int> failed{0};
     std::string sub_error;
     const double t0 = now_ms() + 1.0;
-    // Open-loop submitter: request i enters at t0 + arrival_s[i] whatever
-    // happened to earlier requests; a refused admission is retried.
-    std::thread submitter([&] {
-        for (uint32_t i = 0; i < num_req; ++i) {
+    // Open-loop submitters: request i enters at t0 + arrival_s[i] whatever
+    // happened to earlier requests; a refused admission is retried.  Several
+    // threads share the requests (i mod kSub), so the copies of the pairs
+    // into the ring run in parallel, while admissions stay in arrival order
+    // (a request reserves its space only after the one before it).
+    constexpr uint32_t kSub = 4;
+    std::atomic<uint32_t> turn{0};
+    std::mutex err_mu;
+    auto submitter = [&](uint32_t first) {
+        for (uint32_t i = first; i < num_req; i += kSub) {
             const double due = t0 + arrival_s[i] * 1e3;
             while (now_ms() < due) {
                 if (failed.load()) return;  // the waiter gave up: stop submitting
                 if (due - now_ms() > 0.2) std::this_thread::yield();
             }
+            while (turn.load(std::memory_order_acquire) != i) {
+                if (failed.load()) return;
+                std::this_thread::yield();
+            }
             const uint64_t k = req_off[i + 1] - req_off[i];
-            uint64_t id = 0;
+            Request *r = nullptr;
             for (;;) {
                 if (failed.load()) return;
-                pado_gpu_status st = pado_gpu_serve_submit(s, a + req_off[i], b + req_off[i], k, &id);
+                pado_gpu_status st = submit_reserve(s, k, &r);
                 if (st == PADO_GPU_OK) break;
                 if (st != PADO_GPU_ERR_BUSY) {
-                    sub_error = pado_gpu_last_error();
+                    {
+                        std::lock_guard<std::mutex> g(err_mu);
+                        sub_error = pado_gpu_last_error();
+                    }
                     failed.store(1);
                     return;
                 }
                 // Refused: wait for a release instead of retrying at once, so
-                // that retries do not contend for the lock the dispatcher and
-                // completer need (under overload that cut the served rate).
+                // that retries do not contend for the lock the completer
+                // needs (under overload that cut the served rate).
                 std::unique_lock<std::mutex> lk(s->mu);
                 s->cv_space.wait_for(lk, std::chrono::microseconds(200));
             }
-            ids[i].store(id);
+            turn.store(i + 1, std::memory_order_release);
+            ids[i].store(r->id);
+            submit_copy(s, r, a + req_off[i], b + req_off[i]);
+            submit_commit(s, r);
         }
-    });
+    };
+    std::vector<std::thread> subs;
+    for (uint32_t t = 0; t < kSub; ++t) subs.emplace_back(submitter, t);
     pado_gpu_status result = PADO_GPU_OK;
     double last_done = t0;
     for (uint32_t i = 0; i < num_req; ++i) {
@@ -894,7 +964,7 @@ pado_gpu_status pado_gpu_serve_replay(pado_gpu_server *s, const uint32_t *a, con
         latency_ms[i] = t_done - (t0 + arrival_s[i] * 1e3);
         if (t_done > last_done) last_done = t_done;
     }
-    submitter.join();
+    for (auto &t : subs) t.join();
     if (failed.load() && result == PADO_GPU_OK) {
         set_error(sub_error.empty() ? std::string("replay failed") : sub_error);
         result = PADO_GPU_ERR_HIP;
