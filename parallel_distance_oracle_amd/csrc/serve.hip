@@ -901,11 +901,13 @@ pado_gpu_status pado_gpu_serve_replay(pado_gpu_server *s, const uint32_t *a, con
     std::string sub_error;
     const double t0 = now_ms() + 1.0;
     // Open-loop submitters: request i enters at t0 + arrival_s[i] whatever
-    // happened to earlier requests; a refused admission is retried.  Several
-    // threads share the requests (i mod kSub), so the copies of the pairs
-    // into the ring run in parallel, while admissions stay in arrival order
-    // (a request reserves its space only after the one before it).
-    constexpr uint32_t kSub = 4;
+    // happened to earlier requests; a refused admission is retried.  In
+    // resident mode several threads share the requests (i mod kSub), so the
+    // copies of the pairs into the ring run in parallel, while admissions
+    // stay in arrival order (a request reserves its space only after the one
+    // before it).  The dispatcher of the launch mode packs requests that
+    // become pending in ring order, so there one thread submits.
+    const uint32_t kSub = s->resident ? 4u : 1u;
     std::atomic<uint32_t> turn{0};
     std::mutex err_mu;
     auto submitter = [&](uint32_t first) {
