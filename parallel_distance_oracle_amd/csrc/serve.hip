@@ -79,7 +79,9 @@ struct Request {
     uint64_t id = 0;
     uint64_t start = 0;  // monotonic ring position of the first pair
     uint64_t k = 0;
-    int state = 0;       // 0 queued, 1 launched, 2 done, 3 consumed
+    // 0 queued, 1 launched, 2 done, 3 consumed; written under the server
+    // lock, read without it by a waiter that spins before it blocks.
+    std::atomic<int> state{0};
     bool claimed = false;  // a waiter is copying the answers out
     bool failed = false;   // its launch failed: no answers
     bool cancelled = false;  // answers are not wanted: released once no launch reads its slot
@@ -497,9 +499,24 @@ pado_gpu_status wait_request(pado_gpu_server *s, uint64_t id, void *out, double 
     }
     Request *r = it->second;
     r->claimed = true;
+    const double t_wait0 = now_ms();
+    if (r->state < 2 && timeout_us != 0) {
+        // Spin (without the lock) for up to 200 us or the timeout before
+        // blocking: a condition-variable wake-up costs tens of microseconds,
+        // which at hundreds of thousands of requests per second would bound
+        // the service.  The request stays live: only this waiter releases it.
+        lk.unlock();
+        const double spin_ms = timeout_us < 0 ? 0.2 : std::min(0.2, timeout_us * 1e-3);
+        const double t_end = now_ms() + spin_ms;
+        while (r->state.load(std::memory_order_acquire) < 2 && now_ms() < t_end) std::this_thread::yield();
+        lk.lock();
+    }
     if (timeout_us < 0) {
         s->cv_done.wait(lk, [&] { return r->state >= 2; });
-    } else if (!s->cv_done.wait_for(lk, std::chrono::microseconds(timeout_us), [&] { return r->state >= 2; })) {
+    } else if (!s->cv_done.wait_for(lk,
+                                    std::chrono::microseconds(std::max<int64_t>(
+                                        0, timeout_us - (int64_t)((now_ms() - t_wait0) * 1e3))),
+                                    [&] { return r->state >= 2; })) {
         // Not done in time: the request stays live; the caller may wait again
         // or cancel it.
         r->claimed = false;
