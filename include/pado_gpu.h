@@ -205,6 +205,15 @@ typedef struct {
  * default build's.  Query with pado_gpu_query_wide; export with
  * pado_gpu_index_export_wide.  Ignored for weighted graphs. */
 #define PADO_GPU_BUILD_WIDE 4u
+/* Run the whole batch phase inside one workgroup (no reference counterpart):
+ * the same kernels' work back to back with workgroup barriers in place of
+ * kernel launches and host checks.  For graphs whose batches take many small
+ * iterations (grids, road-like graphs: hundreds of BFS levels each), where the
+ * launch schedule is bound by launch and host-check latency; slower for
+ * graphs with large frontiers.  Same labels.  Its label pool is sized up
+ * front; if it runs dry the build falls back to the launch schedule.
+ * Ignored with PADO_GPU_BUILD_STREAM. */
+#define PADO_GPU_BUILD_DEVICE_LOOP 8u
 
 /* ParaVertexCentricPLL<BATCH_SIZE>(const Graph&) -> construct (PARA:309-313,
  * 1740-1895): bit-parallel labeling (PARA:544-840) then batch_process over all

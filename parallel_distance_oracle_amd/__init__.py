@@ -33,6 +33,7 @@ QUERY_RANK_SPACE = 4
 BUILD_BP_PER_SLOT = 1
 BUILD_STREAM = 2  # per-kernel launches with HIP-event timing (default: hipGraph batch step)
 BUILD_WIDE = 4  # wide hop domain: 12-bit label distances, u16 answers (graphs deeper than 254)
+BUILD_DEVICE_LOOP = 8  # whole batch phase in one workgroup (deep graphs: many small iterations)
 WIDE_INF = 0xFFFF  # wide-domain query: unreachable
 SERVE_GRID_STRIDE = 0x10
 SERVE_PERSISTENT = 0x20

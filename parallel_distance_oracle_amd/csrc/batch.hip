@@ -461,7 +461,7 @@ __global__ __launch_bounds__(256) void k_bp_masks(uint32_t n, const uint8_t *bpd
 // frontier size), so the only shared counter left is the push-descriptor
 // count, updated once per block.
 template <bool W, uint32_t CH>
-__global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const uint64_t *off, const uint32_t *adj,
+__device__ __forceinline__ void batch_init_body(uint32_t bid, uint32_t nblk, const uint8_t *used, const uint64_t *off, const uint32_t *adj,
                                                      LabelStore S, unsigned long long *ind, unsigned long long *vmark,
                                                      uint32_t *tlist, unsigned long long *aoff, unsigned long long *ag,
                                                      uint32_t *amask, uint32_t *aend, unsigned long long *chunks,
@@ -473,7 +473,7 @@ __global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const 
     __shared__ uint32_t s_base;
     if (C->hold) return;
     const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
-    const uint32_t ro = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t ro = (bid * blockDim.x + threadIdx.x) >> 6;
     const uint32_t rs = C->rs, rsz = C->rsz, gen_batch = C->gen_batch;
     const uint32_t r = rs + ro;
     // Every root of the batch (bit-parallel roots included) is on the
@@ -519,6 +519,18 @@ __global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const 
     for (uint32_t q = lane; q < nch; q += 64) chunks[c0 + q] = ((unsigned long long)(q * CH) << 32) | ro;
 }
 
+template <bool W, uint32_t CH>
+__global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const uint64_t *off, const uint32_t *adj,
+                                                     LabelStore S, unsigned long long *ind, unsigned long long *vmark,
+                                                     uint32_t *tlist, unsigned long long *aoff, unsigned long long *ag,
+                                                     uint32_t *amask, uint32_t *aend, unsigned long long *chunks,
+                                                     const ulonglong2 *bps, ulonglong2 *rset,
+                                                     const unsigned long long *lmask, unsigned long long *rmask,
+                                                     BuildCounters *C)
+{
+    batch_init_body<W, CH>(blockIdx.x, gridDim.x, used, off, adj, S, ind, vmark, tlist, aoff, ag, amask, aend, chunks, bps, rset, lmask, rmask, C);
+}
+
 // Root-distance table (dist_matrix, PARA:986-1013, transposed): row h holds
 // dist(root r, hub h) at column r - rs for the hubs of the batch roots'
 // labels, 255 elsewhere; hinfo[h] holds 0xFFFFFF00 | min over the batch
@@ -528,13 +540,13 @@ __global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const 
 // previous batch's roots (the labels they held at its end, which include
 // every entry written during it) are reset.
 template <bool W>
-__global__ __launch_bounds__(256) void k_hub_table(uint32_t fill, const uint8_t *used, LabelStore S, uint32_t *hinfo,
+__device__ __forceinline__ void hub_table_body(uint32_t bid, uint32_t nblk, uint32_t fill, const uint8_t *used, LabelStore S, uint32_t *hinfo,
                                                    typename Hop<W>::Dist *dc, const BuildCounters *C)
 {
     constexpr uint32_t B = Hop<W>::kBits, INF = Hop<W>::kInf;
     if (C->hold) return;
     const uint32_t lane = lane_id();
-    const uint32_t ro = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t ro = (bid * blockDim.x + threadIdx.x) >> 6;
     const uint32_t rs = C->rs, rsz = C->rsz;
     if (ro >= rsz) return;
     const uint32_t r = rs + ro;
@@ -573,6 +585,13 @@ __global__ __launch_bounds__(256) void k_hub_table(uint32_t fill, const uint8_t 
     }
 }
 
+template <bool W>
+__global__ __launch_bounds__(256) void k_hub_table(uint32_t fill, const uint8_t *used, LabelStore S, uint32_t *hinfo,
+                                                   typename Hop<W>::Dist *dc, const BuildCounters *C)
+{
+    hub_table_body<W>(blockIdx.x, gridDim.x, fill, used, S, hinfo, dc, C);
+}
+
 constexpr uint32_t kPushWaves = 4;  // default waves per k_push block
 
 // push (PARA:1486-1559 / push_labels PARA:1084-1211).  A wave takes 64 / CH
@@ -585,7 +604,7 @@ constexpr uint32_t kPushWaves = 4;  // default waves per k_push block
 // one pair per lane, so a tail with many fresh roots does not hold up the
 // wave.
 template <bool W, uint32_t CH, uint32_t PW>
-__global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(4))) void k_push(uint32_t iter, const uint32_t *adj,
+__device__ __forceinline__ void push_body(uint32_t bid, uint32_t nblk, uint32_t iter, const uint32_t *adj,
                                               const unsigned long long *aoff, const unsigned long long *ag,
                                               const uint32_t *amask, const uint32_t *aend,
                                               const unsigned long long *chunks, unsigned long long *ind,
@@ -606,8 +625,8 @@ __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
     __shared__ unsigned long long s_srem[PW][64], s_sone[PW][64];
     __shared__ uint32_t s_sro[PW][64], s_st[PW][64], s_sex[PW][64], s_shit[PW][64];
     const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
-    const uint32_t wave = blockIdx.x * PW + wv;
-    const uint32_t nwaves = gridDim.x * PW;
+    const uint32_t wave = bid * PW + wv;
+    const uint32_t nwaves = nblk * PW;
     const SetScratch sc{s_srem[wv], s_sone[wv], s_sro[wv], s_st[wv], s_sex[wv], s_shit[wv]};
     // The check reads level masks while they cover every td it acts on, the
     // distance bytes after that.
@@ -624,7 +643,7 @@ __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
     const uint32_t nc = live ? C->n_chunk[p] : 0u;
     const uint32_t rs = C->rs, gen_batch = C->gen_batch;
     const unsigned long long gen_iter = ((unsigned long long)gen_batch << 16) | iter;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (bid == 0 && threadIdx.x == 0) {
         C->iter_last = iter;
         C->n_act[p ^ 1u] = 0;
         C->n_chunk[p ^ 1u] = 0;
@@ -812,22 +831,24 @@ __global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(4))) vo
     }
 }
 
-// Start of a batch (construct's loop over batches, PARA:1791-1812): roots
-// [b * batch, ...) with b taken from the device-side batch counter.  One
-// thread per root offset: the block also counts the batch's
-// non-bit-parallel roots, which are the first frontier (active slots and
-// touched-list slots [0, rsz) belong to the root offsets, filled by
-// k_batch_init) and get one self label each.
-__global__ __launch_bounds__(1024) void k_batch_begin(const uint8_t *used, BuildCounters *C, uint32_t batch,
-                                                      uint32_t n)
+template <bool W, uint32_t CH, uint32_t PW>
+__global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(4))) void k_push(uint32_t iter, const uint32_t *adj,
+                                              const unsigned long long *aoff, const unsigned long long *ag,
+                                              const uint32_t *amask, const uint32_t *aend,
+                                              const unsigned long long *chunks, unsigned long long *ind,
+                                              unsigned long long *cand, unsigned long long *vmark, uint32_t *tlist,
+                                              uint32_t *clist,
+                                              const uint8_t *bpd, const unsigned long long *lmask,
+                                              const unsigned long long *rmask, const ulonglong2 *bps,
+                                              const ulonglong2 *rset, BuildCounters *C, uint32_t bp_prune)
 {
-    if (C->hold) return;
-    const uint32_t b = C->next_batch;
-    const uint32_t rs = b * batch;
-    const uint32_t rsz = rs < n ? min(batch, n - rs) : 0u;
-    const uint32_t o = threadIdx.x;
-    const uint32_t cnt = (uint32_t)__syncthreads_count(o < rsz && !used[rs + o]);
-    if (o != 0) return;
+    push_body<W, CH, PW>(blockIdx.x, gridDim.x, iter, adj, aoff, ag, amask, aend, chunks, ind, cand, vmark, tlist, clist, bpd, lmask, rmask, bps, rset, C, bp_prune);
+}
+
+// Batch b's schedule into the counters: roots [rs, rs + rsz), cnt of them
+// not bit-parallel (the first frontier, one self label each).
+__device__ inline void batch_begin_write(BuildCounters *C, uint32_t b, uint32_t rs, uint32_t rsz, uint32_t cnt)
+{
     C->next_batch = b + 1;
     C->rs = rs;
     C->rsz = rsz;
@@ -842,21 +863,46 @@ __global__ __launch_bounds__(1024) void k_batch_begin(const uint8_t *used, Build
     C->labels += cnt;
 }
 
+// Start of a batch (construct's loop over batches, PARA:1791-1812): roots
+// [b * batch, ...) with b taken from the device-side batch counter.  One
+// thread per root offset: the block also counts the batch's
+// non-bit-parallel roots, which are the first frontier (active slots and
+// touched-list slots [0, rsz) belong to the root offsets, filled by
+// k_batch_init) and get one self label each.
+__device__ __forceinline__ void batch_begin_body(uint32_t bid, uint32_t nblk, const uint8_t *used, BuildCounters *C, uint32_t batch,
+                                                      uint32_t n)
+{
+    if (C->hold) return;
+    const uint32_t b = C->next_batch;
+    const uint32_t rs = b * batch;
+    const uint32_t rsz = rs < n ? min(batch, n - rs) : 0u;
+    const uint32_t o = threadIdx.x;
+    const uint32_t cnt = (uint32_t)__syncthreads_count(o < rsz && !used[rs + o]);
+    if (o != 0) return;
+    batch_begin_write(C, b, rs, rsz, cnt);
+}
+
+__global__ __launch_bounds__(1024) void k_batch_begin(const uint8_t *used, BuildCounters *C, uint32_t batch,
+                                                      uint32_t n)
+{
+    batch_begin_body(blockIdx.x, gridDim.x, used, C, batch, n);
+}
+
 // add (PARA:1568-1665): one wave per candidate vertex v.  Lane l owns roots
 // [16l, 16l+16).  distance_query (PARA:1218-1270) runs over v's labels with
 // dist < iter: a hub h can prune only if d_v(h) + min_r d_r(h) <= iter, so a
 // per-hub minimum filters most hubs before the 1 KB root-distance row of the
 // survivors is read (16 B per lane).
 template <bool W, uint32_t CH>
-__global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used, const uint64_t *off, const uint32_t *adj, const uint32_t *clist,
+__device__ __forceinline__ void add_body(uint32_t bid, uint32_t nblk, uint32_t iter, const uint8_t *used, const uint64_t *off, const uint32_t *adj, const uint32_t *clist,
                                              unsigned long long *cand, LabelStore S, uint32_t *hinfo,
                                              typename Hop<W>::Dist *dc,
                                              unsigned long long *aoff, unsigned long long *ag, uint32_t *amask,
                                              uint32_t *aend, unsigned long long *chunks, BuildCounters *C)
 {
     const uint32_t lane = lane_id();
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t wave = (bid * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (nblk * blockDim.x) >> 6;
     // Candidates of iteration `iter` (slot iter & 1); the next frontier goes
     // to the other slot, and block 0 empties the other candidate slot for
     // the next iteration's push.
@@ -865,7 +911,7 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
     constexpr uint32_t B = Hop<W>::kBits, INF = Hop<W>::kInf;
     const uint32_t ncand = iter < INF ? C->n_cand[p] : 0u;  // 0 past the batch's end
     const uint32_t rs = C->rs, rsz = C->rsz;
-    if (blockIdx.x == 0 && threadIdx.x == 0) C->n_cand[p ^ 1u] = 0;
+    if (bid == 0 && threadIdx.x == 0) C->n_cand[p ^ 1u] = 0;
     unsigned long long n_dq = 0, n_lab = 0, n_rows = 0, n_read = 0;
     // Next-frontier slots (active entries and push descriptors) are reserved
     // kAlistGrab / kChunkGrab at a time per wave, so the shared counters see
@@ -1044,6 +1090,16 @@ __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used,
     }
 }
 
+template <bool W, uint32_t CH>
+__global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used, const uint64_t *off, const uint32_t *adj, const uint32_t *clist,
+                                             unsigned long long *cand, LabelStore S, uint32_t *hinfo,
+                                             typename Hop<W>::Dist *dc,
+                                             unsigned long long *aoff, unsigned long long *ag, uint32_t *amask,
+                                             uint32_t *aend, unsigned long long *chunks, BuildCounters *C)
+{
+    add_body<W, CH>(blockIdx.x, gridDim.x, iter, used, off, adj, clist, cand, S, hinfo, dc, aoff, ag, amask, aend, chunks, C);
+}
+
 // First kernel of a batch's prologue: if the previous batch still has a
 // frontier (its next iteration's count in slot (iter_last + 1) & 1), this
 // batch was enqueued too early and everything up to the host's k_unhold is
@@ -1058,15 +1114,110 @@ __global__ void k_unhold(BuildCounters *C) { C->hold = 0u; }
 // reset_at_end (PARA:1328-1410) and the next batch's indicator reset
 // (PARA:877-901): clear the indicator rows of the touched vertices (the
 // root-distance table is reset by k_hub_table).
-__global__ __launch_bounds__(256) void k_batch_end(unsigned long long *ind, const uint32_t *tlist,
+__device__ __forceinline__ void batch_end_body(uint32_t bid, uint32_t nblk, unsigned long long *ind, const uint32_t *tlist,
                                                    const BuildCounters *C)
 {
     if (C->hold) return;
     const uint32_t nt = C->n_touched;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t stride = (uint64_t)nblk * blockDim.x;
     const uint64_t tot = (uint64_t)nt * kWords;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += stride)
+    for (uint64_t i = (uint64_t)bid * blockDim.x + threadIdx.x; i < tot; i += stride)
         ind[(uint64_t)tlist[i / kWords] * kWords + (i % kWords)] = 0ull;
+}
+
+__global__ __launch_bounds__(256) void k_batch_end(unsigned long long *ind, const uint32_t *tlist,
+                                                   const BuildCounters *C)
+{
+    batch_end_body(blockIdx.x, gridDim.x, ind, tlist, C);
+}
+
+// One graph's whole batch phase in one workgroup (no reference
+// counterpart: the reference runs batch_process from the host, PARA:1413-1736
+// and construct's loop over batches, PARA:1791-1812).  For graphs whose
+// iterations are many and small -- road-like graphs and grids, hundreds of
+// BFS levels per batch -- the launch-per-iteration schedule is bound by
+// launch and host-check latency; here the same bodies run back to back with
+// workgroup barriers in place of kernel boundaries, and a launch carries
+// one workgroup per graph, so a set of such graphs is labeled by one launch.
+template <bool W>
+struct BatchPlan {
+    uint32_t n, batch, bp_prune, pad;
+    const uint8_t *used;
+    const uint64_t *off;
+    const uint32_t *adj;
+    LabelStore S;
+    unsigned long long *ind, *cand, *vmark;
+    uint32_t *tlist, *clist;
+    unsigned long long *aoff, *ag;
+    uint32_t *amask, *aend;
+    unsigned long long *chunks;
+    const uint8_t *bpd;
+    const unsigned long long *lmask;
+    unsigned long long *rmask;
+    const ulonglong2 *bps;
+    ulonglong2 *rset;
+    uint32_t *hinfo;
+    typename Hop<W>::Dist *dc;
+    BuildCounters *C;
+};
+
+constexpr uint32_t kLoopWaves = 8;  // waves of a k_batch_loop workgroup
+
+// Every phase ends at a workgroup barrier; the bodies' virtual blocks are
+// looped over (bid) with nblk = 1, so their grid-stride loops cover the work.
+// Leaves early (every thread, after a barrier) on a full pool or when an
+// iteration would leave the hop domain; the counters say which.
+template <bool W>
+__global__ __launch_bounds__(kLoopWaves * 64) void k_batch_loop(const BatchPlan<W> *plans)
+{
+    constexpr uint32_t kEnd = Hop<W>::kInf;  // the last iteration ever run
+    __shared__ uint32_t s_cnt;
+    const BatchPlan<W> &P = plans[blockIdx.x];
+    BuildCounters *C = P.C;
+    const uint32_t nb = (P.n + P.batch - 1) / P.batch;
+    for (uint32_t b = 0; b < nb; ++b) {
+        // The previous batch's root-distance entries and indicator rows.
+        const uint32_t prsz = C->rsz;
+        for (uint32_t vb = 0; vb * kLoopWaves < prsz; ++vb)
+            hub_table_body<W>(vb, 1, 0u, P.used, P.S, P.hinfo, P.dc, C);
+        batch_end_body(0, 1, P.ind, P.tlist, C);
+        // Batch b: its non-bit-parallel roots are the first frontier.
+        const uint32_t rs = b * P.batch, rsz = min(P.batch, P.n - rs);
+        uint32_t c = 0;
+        for (uint32_t o = threadIdx.x; o < rsz; o += blockDim.x) c += P.used[rs + o] ? 0u : 1u;
+        c = wave_sum_u32(c);
+        if (threadIdx.x == 0) s_cnt = 0;
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) atomicAdd(&s_cnt, c);
+        __syncthreads();
+        if (threadIdx.x == 0) batch_begin_write(C, b, rs, rsz, s_cnt);
+        __syncthreads();
+        for (uint32_t vb = 0; vb * kLoopWaves < rsz; ++vb)
+            batch_init_body<W, kChunk>(vb, 1, P.used, P.off, P.adj, P.S, P.ind, P.vmark, P.tlist, P.aoff, P.ag,
+                                       P.amask, P.aend, P.chunks, P.bps, P.rset, P.lmask, P.rmask, C);
+        __syncthreads();
+        for (uint32_t vb = 0; vb * kLoopWaves < rsz; ++vb)
+            hub_table_body<W>(vb, 1, 1u, P.used, P.S, P.hinfo, P.dc, C);
+        __syncthreads();
+        for (uint32_t iter = 1;; ++iter) {
+            push_body<W, kChunk, kLoopWaves>(0, 1, iter, P.adj, P.aoff, P.ag, P.amask, P.aend, P.chunks, P.ind,
+                                             P.cand, P.vmark, P.tlist, P.clist, P.bpd, P.lmask, P.rmask, P.bps,
+                                             P.rset, C, P.bp_prune);
+            __syncthreads();
+            add_body<W, kChunk>(0, 1, iter, P.used, P.off, P.adj, P.clist, P.cand, P.S, P.hinfo, P.dc, P.aoff,
+                                P.ag, P.amask, P.aend, P.chunks, C);
+            __syncthreads();
+            const bool stop = C->overflow != 0u || C->domain != 0u;
+            const bool done = C->n_act[(iter + 1) & 1u] == 0u;
+            __syncthreads();  // every thread has read the counters before any writes them
+            if (stop) return;
+            if (done) break;
+            if (iter >= kEnd) {
+                if (threadIdx.x == 0) C->domain = 1u;
+                return;
+            }
+        }
+    }
 }
 
 // Pool compaction: copy each vertex's labels into a fresh pool.
@@ -1136,6 +1287,112 @@ static pado_gpu_status pool_repack(hipStream_t st, uint32_t n, DevBuf<uint32_t> 
     return PADO_GPU_OK;
 }
 
+// The device buffers of one batch phase (layout at the top of this file).
+template <bool W>
+struct BatchBuffers {
+    DevBuf<unsigned long long> ind, cand, ag, chunks;
+    DevBuf<unsigned long long> aoff;   // per active slot: the vertex's first edge
+    DevBuf<unsigned long long> vmark;  // ((batch << 16 | iteration) << 1 | candidate) of the last mark
+    DevBuf<unsigned long long> lmask;  // bit-parallel level masks, 8 words per vertex
+    DevBuf<unsigned long long> rmask;  // batch roots' level masks, level-major [8][kBatch]
+    DevBuf<uint32_t> tlist, clist, amask, aend, hinfo, pool;
+    DevBuf<LRec> lrec;
+    DevBuf<typename Hop<W>::Dist> dc;
+    DevBuf<ulonglong2> rset;  // batch roots' bit-parallel sets, slot-major [kBP][kBatch]
+    DevBuf<BuildCounters> ctr;
+    unsigned long long pool_size = 0;
+    uint32_t dc_rows = 0;
+    void release()
+    {
+        for (auto *b : {&ind, &cand, &ag, &chunks, &aoff, &vmark, &lmask, &rmask}) b->release();
+        for (auto *b : {&tlist, &clist, &amask, &aend, &hinfo, &pool}) b->release();
+        lrec.release();
+        dc.release();
+        rset.release();
+        ctr.release();
+    }
+};
+
+// Allocates and initialises them on the graph's stream.  add_waves bounds
+// the holes of k_add's per-wave reservations: the active list can exceed the
+// active count by kAlistGrab slots per wave, and padded descriptors are
+// fewer than the real ones plus kChunkGrab per wave.  The pool holds
+// pool_size label words, the root-distance table dc_rows 1 KB rows (255).
+template <bool W>
+static pado_gpu_status batch_alloc(const pado_gpu_graph *g, const pado_gpu_index *idx, uint64_t add_waves,
+                                   uint32_t chunk, unsigned long long pool_size, uint32_t dc_rows,
+                                   BatchBuffers<W> &B)
+{
+    using DT = typename Hop<W>::Dist;
+    hipStream_t st = g->ctx->stream;
+    const uint32_t n = g->n;
+    const uint64_t alist_cap = (uint64_t)n + kAlistGrab * add_waves;
+    PADO_TRY(B.vmark.alloc(n));
+    PADO_TRY(B.ind.alloc((uint64_t)n * kWords));
+    PADO_TRY(B.cand.alloc((uint64_t)n * kWords));
+    PADO_TRY(B.ag.alloc(alist_cap * kWords));
+    PADO_TRY(B.chunks.alloc(2 * (g->m / chunk + (uint64_t)n) + kChunkGrab * add_waves + 64));
+    PADO_TRY(B.tlist.alloc(n));
+    PADO_TRY(B.clist.alloc(n));
+    PADO_TRY(B.aoff.alloc(alist_cap));
+    PADO_TRY(B.amask.alloc(alist_cap));
+    PADO_TRY(B.aend.alloc(alist_cap));
+    PADO_TRY(B.hinfo.alloc(n));
+    PADO_TRY(B.lrec.alloc(n));
+    PADO_TRY(B.rset.alloc((uint64_t)kBP * kBatch));
+    PADO_TRY(B.lmask.alloc((uint64_t)n * 8));
+    PADO_TRY(B.rmask.alloc(8ull * kBatch));
+    PADO_TRY(B.ctr.alloc(1));
+    PADO_TRY(B.pool.alloc(pool_size));
+    PADO_TRY(B.dc.alloc((uint64_t)dc_rows * kBatch));
+    B.pool_size = pool_size;
+    B.dc_rows = dc_rows;
+    const uint32_t fg = 4096;
+    hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, reinterpret_cast<uint32_t *>(B.dc.p),
+                       (uint64_t)dc_rows * kBatch * sizeof(DT) / 4, ~0u);
+    hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, B.ind.p, (uint64_t)n * kWords);
+    hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, B.cand.p, (uint64_t)n * kWords);
+    hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, B.vmark.p, (uint64_t)n);
+    hipLaunchKernelGGL(k_bp_masks, dim3(fg), dim3(256), 0, st, n, idx->bpd.p, B.lmask.p);
+    hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, B.hinfo.p, (uint64_t)n, kEmpty);
+    hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, reinterpret_cast<unsigned long long *>(B.lrec.p),
+                       2ull * n);
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t *>(B.ctr.p),
+                       (uint64_t)(sizeof(BuildCounters) / 4), 0u);
+    PADO_HIP(hipGetLastError());
+    return PADO_GPU_OK;
+}
+
+// End of the batch phase: clear the indicator rows still marked, keep the
+// counters, and build the canonical rank-space CSR -- each vertex's labels
+// gathered and sorted by hub (order_labels' per-vertex std::sort, PARA:2240).
+template <bool W>
+static pado_gpu_status batch_finish(pado_gpu_graph *g, pado_gpu_index *idx, const pado_gpu_build_cfg &cfg,
+                                    BatchBuffers<W> &B, const KernelTimes &kt)
+{
+    hipStream_t st = g->ctx->stream;
+    const uint32_t n = g->n;
+    hipLaunchKernelGGL(k_batch_end, dim3(1024), dim3(256), 0, st, B.ind.p, B.tlist.p, B.ctr.p);
+    BuildCounters c{};
+    PADO_HIP(copy_sync(&c, B.ctr.p, sizeof(BuildCounters), hipMemcpyDeviceToHost, st));
+    idx->counters = c;
+    idx->iterations = c.iterations;
+    idx->ktimes = kt;
+    PADO_TRY(idx->loff.alloc((uint64_t)n + 1));
+    DevBuf<uint32_t> lens;  // label counts (lives until the end: the scan is asynchronous)
+    PADO_TRY(lens.alloc(n));
+    hipLaunchKernelGGL(k_cap_for_len, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, B.lrec.p, lens.p, 1u);
+    PADO_TRY(scan_u32_to_u64(lens.p, idx->loff.p, n, st));
+    uint64_t total = 0;
+    PADO_HIP(copy_sync(&total, idx->loff.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+    idx->num_labels = total;
+    PADO_TRY(idx->lab.alloc(total));
+    hipLaunchKernelGGL(k_gather_labels, dim3(grid_for((uint64_t)n * 64, 256, 8192)), dim3(256), 0, st, n, B.pool.p,
+                       B.lrec.p, idx->loff.p, idx->lab.p);
+    PADO_TRY(segmented_sort_u32(idx->lab.p, total, idx->loff.p, n, st, cfg.sort_max_items));
+    return PADO_GPU_OK;
+}
+
 // The batch kernels of one (chunk, push waves) choice.
 template <bool W>
 struct BatchKernels {
@@ -1192,61 +1449,28 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
         W ? "batch iteration reached 4095: distances outside the wide (12-bit) hop domain"
           : "batch iteration reached 255: distances outside the uint8 domain of the reference";
 
-    DevBuf<unsigned long long> ind, cand, ag, chunks;
-    DevBuf<uint32_t> tlist, clist, amask, aend, hinfo, pool;
-    DevBuf<unsigned long long> aoff;  // per active slot: the vertex's first edge
-    DevBuf<LRec> lrec;
-    DevBuf<unsigned long long> vmark;  // ((batch << 16 | iteration) << 1 | candidate) of the last mark
-    DevBuf<DT> dc;
-    DevBuf<ulonglong2> rset;  // batch roots' bit-parallel sets, slot-major [kBP][kBatch]
-    DevBuf<unsigned long long> lmask;  // bit-parallel level masks, 8 words per vertex
-    DevBuf<unsigned long long> rmask;  // batch roots' level masks, level-major [8][kBatch]
-    DevBuf<BuildCounters> ctr;
-    PADO_TRY(vmark.alloc(n));
-    PADO_TRY(ind.alloc((uint64_t)n * kWords));
-    PADO_TRY(cand.alloc((uint64_t)n * kWords));
-    // Per-wave reservations in k_add leave holes: the active list can exceed
-    // the active count by kAlistGrab slots per wave, and padded descriptors
-    // are fewer than the real ones plus kChunkGrab per wave.
+    // Per-wave reservations in k_add leave holes in the active list and the
+    // descriptor array (see batch_alloc).
     const uint64_t add_waves = 4ull * (cfg.add_blocks ? cfg.add_blocks : default_grid(n));
-    const uint64_t alist_cap = (uint64_t)n + kAlistGrab * add_waves;
-    PADO_TRY(ag.alloc(alist_cap * kWords));
-    PADO_TRY(chunks.alloc(2 * (g->m / chunk + (uint64_t)n) + kChunkGrab * add_waves + 64));
-    PADO_TRY(tlist.alloc(n));
-    PADO_TRY(clist.alloc(n));
-    PADO_TRY(aoff.alloc(alist_cap));
-    PADO_TRY(amask.alloc(alist_cap));
-    PADO_TRY(aend.alloc(alist_cap));
-    PADO_TRY(hinfo.alloc(n));
-    PADO_TRY(lrec.alloc(n));
-    PADO_TRY(rset.alloc((uint64_t)kBP * kBatch));
-    PADO_TRY(lmask.alloc((uint64_t)n * 8));
-    PADO_TRY(rmask.alloc(8ull * kBatch));
-    PADO_TRY(ctr.alloc(1));
     // The label pool starts at 64 words per vertex and doubles whenever a
     // host check finds it over 40% full; the checks come every
     // iters_per_sync iterations, so the headroom covers several iterations.
-    unsigned long long pool_size = std::max<unsigned long long>(std::max<unsigned long long>(1ull << 22, 64ull * n),
-                                                                pool_min);
-    PADO_TRY(pool.alloc(pool_size));
-    // Root-distance table, one 1 KB row per hub id: the hubs of batch b's
-    // roots are ids below the batch end, so that many rows suffice; grown
-    // (all 255) on the host before a batch that could exceed it.
-    uint32_t dc_rows = std::min<uint32_t>(n, 1u << 16);
-    PADO_TRY(dc.alloc((uint64_t)dc_rows * kBatch));
-
+    // The root-distance table covers the hubs of the first 2^16 ids and
+    // grows on the host before a batch that could exceed it.
+    BatchBuffers<W> B;
+    PADO_TRY(batch_alloc<W>(g, idx, add_waves, chunk,
+                            std::max<unsigned long long>(std::max<unsigned long long>(1ull << 22, 64ull * n), pool_min),
+                            std::min<uint32_t>(n, 1u << 16), B));
+    auto &ind = B.ind, &cand = B.cand, &ag = B.ag, &chunks = B.chunks, &aoff = B.aoff, &vmark = B.vmark;
+    auto &lmask = B.lmask, &rmask = B.rmask;
+    auto &tlist = B.tlist, &clist = B.clist, &amask = B.amask, &aend = B.aend, &hinfo = B.hinfo, &pool = B.pool;
+    auto &lrec = B.lrec;
+    auto &dc = B.dc;
+    auto &rset = B.rset;
+    auto &ctr = B.ctr;
+    unsigned long long pool_size = B.pool_size;
+    uint32_t dc_rows = B.dc_rows;
     const uint32_t fg = 4096;
-    hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, reinterpret_cast<uint32_t *>(dc.p),
-                       (uint64_t)dc_rows * kBatch * sizeof(DT) / 4, ~0u);
-    hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, ind.p, (uint64_t)n * kWords);
-    hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, cand.p, (uint64_t)n * kWords);
-    hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, vmark.p, (uint64_t)n);
-    hipLaunchKernelGGL(k_bp_masks, dim3(fg), dim3(256), 0, st, n, idx->bpd.p, lmask.p);
-    hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, hinfo.p, (uint64_t)n, kEmpty);
-    hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, reinterpret_cast<unsigned long long *>(lrec.p),
-                       2ull * n);
-    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t *>(ctr.p),
-                       (uint64_t)(sizeof(BuildCounters) / 4), 0u);
 
     BuildCounters *h_ctr = nullptr;
     PADO_HIP(hipHostMalloc((void **)&h_ctr, 4 * sizeof(BuildCounters), hipHostMallocDefault));
@@ -1530,31 +1754,99 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
         }
         ahead = spec;
     }
-    hipLaunchKernelGGL(k_batch_end, dim3(1024), dim3(256), 0, st, ind.p, tlist.p, ctr.p);
-    PADO_HIP(hipMemcpyAsync(h_ctr, ctr.p, sizeof(BuildCounters), hipMemcpyDeviceToHost, st));
-    PADO_HIP(hipStreamSynchronize(st));
-    idx->counters = *h_ctr;
-    idx->iterations = h_ctr->iterations;
-    idx->ktimes.ms_push = kt.ms_push;
-    idx->ktimes.ms_add = kt.ms_add;
-    idx->ktimes.n_push = kt.n_push;
-    idx->ktimes.n_add = kt.n_add;
+    return batch_finish<W>(g, idx, cfg, B, kt);
+}
 
-    // Canonical CSR: gather each vertex's labels and sort them by hub
-    // (order_labels' per-vertex std::sort, PARA:2240).
-    PADO_TRY(idx->loff.alloc((uint64_t)n + 1));
-    DevBuf<uint32_t> lens;  // label counts (lives until the end: the scan is asynchronous)
-    PADO_TRY(lens.alloc(n));
-    hipLaunchKernelGGL(k_cap_for_len, dim3(grid_for(n, 256, 4096)), dim3(256), 0, st, n, lrec.p, lens.p, 1u);
-    PADO_TRY(scan_u32_to_u64(lens.p, idx->loff.p, n, st));
-    uint64_t total = 0;
-    PADO_HIP(copy_sync(&total, idx->loff.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-    idx->num_labels = total;
-    PADO_TRY(idx->lab.alloc(total));
-    hipLaunchKernelGGL(k_gather_labels, dim3(grid_for((uint64_t)n * 64, 256, 8192)), dim3(256), 0, st, n, pool.p,
-                       lrec.p, idx->loff.p, idx->lab.p);
-    PADO_TRY(segmented_sort_u32(idx->lab.p, total, idx->loff.p, n, st, cfg.sort_max_items));
+template <bool W>
+static BatchPlan<W> batch_plan(const pado_gpu_graph *g, const pado_gpu_index *idx, const uint8_t *used, uint32_t batch,
+                               BatchBuffers<W> &B)
+{
+    BatchPlan<W> P{};
+    P.n = g->n;
+    P.batch = batch;
+    P.bp_prune = idx->bp_pass_through ? 0u : 1u;
+    P.used = used;
+    P.off = g->off.p;
+    P.adj = g->adj.p;
+    P.S = LabelStore{B.pool.p, B.lrec.p, B.pool_size};
+    P.ind = B.ind.p;
+    P.cand = B.cand.p;
+    P.vmark = B.vmark.p;
+    P.tlist = B.tlist.p;
+    P.clist = B.clist.p;
+    P.aoff = B.aoff.p;
+    P.ag = B.ag.p;
+    P.amask = B.amask.p;
+    P.aend = B.aend.p;
+    P.chunks = B.chunks.p;
+    P.bpd = idx->bpd.p;
+    P.lmask = B.lmask.p;
+    P.rmask = B.rmask.p;
+    P.bps = (const ulonglong2 *)idx->bps.p;
+    P.rset = B.rset.p;
+    P.hinfo = B.hinfo.p;
+    P.dc = B.dc.p;
+    P.C = B.ctr.p;
+    return P;
+}
+
+// The batch phase of jobs[i] for the i with jobs[i].idx->wide == W, all in
+// one k_batch_loop launch on st.  The pool cannot grow inside the loop: a
+// job whose pool runs dry ends with PADO_GPU_ERR_NOMEM and pool_exhausted.
+template <bool W>
+static pado_gpu_status batch_loop_t(std::vector<LoopJob> &jobs, hipStream_t st)
+{
+    std::vector<size_t> mine;
+    for (size_t i = 0; i < jobs.size(); ++i)
+        if ((jobs[i].idx->wide != 0) == W && jobs[i].status == PADO_GPU_OK) mine.push_back(i);
+    if (mine.empty()) return PADO_GPU_OK;
+    std::vector<BatchBuffers<W>> bufs(mine.size());
+    std::vector<BatchPlan<W>> plans(mine.size());
+    for (size_t j = 0; j < mine.size(); ++j) {
+        LoopJob &J = jobs[mine[j]];
+        const uint32_t n = J.g->n;
+        if (J.batch == 0 || J.batch > kBatch || n > Hop<W>::kMaxN || n == 0) {
+            set_error("device-loop batch phase: batch size or vertex count out of range");
+            return PADO_GPU_ERR_ARG;
+        }
+        // Labels per vertex of deep graphs stay in the low hundreds; the
+        // growable per-vertex arrays take up to four times their length.
+        const unsigned long long pool =
+            std::max<unsigned long long>(std::max<unsigned long long>(1ull << 22, 384ull * n), J.pool_min);
+        PADO_TRY(batch_alloc<W>(J.g, J.idx, kLoopWaves, kChunk, pool, n, bufs[j]));
+        plans[j] = batch_plan<W>(J.g, J.idx, J.used, J.batch, bufs[j]);
+    }
+    DevBuf<BatchPlan<W>> dplans;
+    PADO_TRY(dplans.alloc(plans.size()));
+    PADO_HIP(hipMemcpyAsync(dplans.p, plans.data(), sizeof(BatchPlan<W>) * plans.size(), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_batch_loop<W>, dim3((uint32_t)plans.size()), dim3(kLoopWaves * 64), 0, st, dplans.p);
+    PADO_HIP(hipGetLastError());
+    PADO_HIP(hipStreamSynchronize(st));
+    for (size_t j = 0; j < mine.size(); ++j) {
+        LoopJob &J = jobs[mine[j]];
+        BuildCounters c{};
+        PADO_HIP(copy_sync(&c, bufs[j].ctr.p, sizeof(BuildCounters), hipMemcpyDeviceToHost, st));
+        if (c.overflow) {
+            J.status = PADO_GPU_ERR_NOMEM;
+            J.pool_exhausted = bufs[j].pool_size;
+            continue;
+        }
+        if (c.domain) {
+            J.status = PADO_GPU_ERR_DOMAIN;
+            continue;
+        }
+        pado_gpu_build_cfg cfg{};
+        cfg.batch_size = J.batch;
+        J.status = batch_finish<W>(J.g, J.idx, cfg, bufs[j], KernelTimes{});
+        bufs[j].release();  // this graph's buffers go before the next finish allocates
+    }
     return PADO_GPU_OK;
+}
+
+pado_gpu_status build_batches_loop(std::vector<LoopJob> &jobs, hipStream_t st)
+{
+    PADO_TRY(batch_loop_t<false>(jobs, st));
+    return batch_loop_t<true>(jobs, st);
 }
 
 pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevBuf<uint8_t> &d_used,
@@ -1564,6 +1856,18 @@ pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevB
     // rate) costs one rerun of the batch phase from scratch with a larger
     // pool; no label is ever dropped.
     unsigned long long exhausted = 0, pool_min = 0;
+    if ((cfg.flags & PADO_GPU_BUILD_DEVICE_LOOP) && !(cfg.flags & PADO_GPU_BUILD_STREAM)) {
+        // One workgroup runs the whole batch phase; a pool that runs dry
+        // there falls back to the launch schedule, which grows its pool.
+        std::vector<LoopJob> job(1);
+        job[0].g = g;
+        job[0].idx = idx;
+        job[0].used = d_used.p;
+        job[0].batch = cfg.batch_size ? cfg.batch_size : kBatch;
+        PADO_TRY(build_batches_loop(job, g->ctx->stream));
+        if (job[0].status != PADO_GPU_ERR_NOMEM) return job[0].status;
+        pool_min = job[0].pool_exhausted * 2;
+    }
     for (int attempt = 0;; ++attempt) {
         exhausted = 0;
         const pado_gpu_status s = idx->wide ? build_batches_t<true>(g, idx, d_used, cfg, pool_min, &exhausted)

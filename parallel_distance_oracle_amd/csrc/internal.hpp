@@ -32,6 +32,21 @@ pado_gpu_status build_bp_seeded(pado_gpu_graph *g, pado_gpu_index *idx, DevBuf<u
 pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevBuf<uint8_t> &d_used,
                               const pado_gpu_build_cfg &cfg);
 
+// The same batch phase run by one workgroup per graph (k_batch_loop), for
+// several graphs in one launch on st: jobs with status PADO_GPU_OK on entry
+// take part; on return each has its own status (PADO_GPU_ERR_NOMEM with
+// pool_exhausted when its fixed pool ran dry -- rerun it with build_batches).
+struct LoopJob {
+    pado_gpu_graph *g = nullptr;
+    pado_gpu_index *idx = nullptr;
+    const uint8_t *used = nullptr;   // device bit-parallel flags of the graph
+    uint32_t batch = kBatch;
+    unsigned long long pool_min = 0;
+    pado_gpu_status status = PADO_GPU_OK;
+    unsigned long long pool_exhausted = 0;
+};
+pado_gpu_status build_batches_loop(std::vector<LoopJob> &jobs, hipStream_t st);
+
 // Weighted labeling (WPARA).
 pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t batch_size);
 
