@@ -27,6 +27,12 @@ void select_bp_roots(uint32_t n, const uint64_t *off, const uint32_t *adj, std::
 pado_gpu_status build_bp_seeded(pado_gpu_graph *g, pado_gpu_index *idx, DevBuf<uint8_t> &d_used,
                                 const std::vector<uint8_t> &h_used, const BpSeeds &seeds);
 
+// An unweighted index up to its batch phase: the bit-parallel labels
+// (idx->wide chooses the sweep) and the used flags -- all zero when the
+// wide domain's slot BFS was cut at level 254 (pass-through).
+pado_gpu_status build_bp_phase(pado_gpu_graph *g, pado_gpu_index *idx, const pado_gpu_build_cfg &c,
+                               DevBuf<uint8_t> &used);
+
 // Batched pruned labeling (PARA:848-1836).  Produces the canonical rank-space
 // label CSR idx->loff / idx->lab.
 pado_gpu_status build_batches(pado_gpu_graph *g, pado_gpu_index *idx, const DevBuf<uint8_t> &d_used,

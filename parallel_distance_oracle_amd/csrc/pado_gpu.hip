@@ -142,6 +142,32 @@ pado_gpu_status order_index(pado_gpu_index *idx, const uint32_t *h_rank)
 
 using namespace pado;
 
+namespace pado {
+
+pado_gpu_status build_bp_phase(pado_gpu_graph *g, pado_gpu_index *idx, const pado_gpu_build_cfg &c,
+                               DevBuf<uint8_t> &used)
+{
+    std::vector<uint8_t> h_used;
+    Timer t0;
+    // The wide domain always uses the multi-source sweep (it stops at level 254).
+    pado_gpu_status s = (c.flags & PADO_GPU_BUILD_BP_PER_SLOT) && !idx->wide ? build_bp(g, idx, used, h_used)
+                                                                            : build_bp_multi(g, idx, used, h_used);
+    idx->ms_bp = t0.ms();
+    if (s == PADO_GPU_OK && idx->wide && idx->bp_max_level >= kInf8) {
+        // The reference never pushes from a bit-parallel root or selected
+        // neighbour (PARA:1117-1119): every path through one is covered by
+        // its slot.  With the slot BFS stopped at level 254 that no longer
+        // holds, so here those vertices take part in the batch phase like
+        // any other (they become hubs and push); the slots still prune
+        // wherever both ends were reached.
+        if (hipMemsetAsync(used.p, 0, g->n, g->ctx->stream) != hipSuccess) s = PADO_GPU_ERR_HIP;
+        idx->bp_pass_through = 1;
+    }
+    return s;
+}
+
+}  // namespace pado
+
 extern "C" {
 
 const char *pado_gpu_last_error(void) { return g_last_error.c_str(); }
@@ -332,22 +358,7 @@ pado_gpu_status pado_gpu_build(pado_gpu_ctx *ctx, const pado_gpu_graph *graph, c
         idx->ms_batch = t1.ms();
     } else {
         DevBuf<uint8_t> used;
-        std::vector<uint8_t> h_used;
-        Timer t0;
-        // The wide domain always uses the multi-source sweep (it stops at level 254).
-        s = (c.flags & PADO_GPU_BUILD_BP_PER_SLOT) && !idx->wide ? build_bp(g, idx, used, h_used)
-                                                                 : build_bp_multi(g, idx, used, h_used);
-        idx->ms_bp = t0.ms();
-        if (s == PADO_GPU_OK && idx->wide && idx->bp_max_level >= kInf8) {
-            // The reference never pushes from a bit-parallel root or selected
-            // neighbour (PARA:1117-1119): every path through one is covered
-            // by its slot.  With the slot BFS stopped at level 254 that no
-            // longer holds, so here those vertices take part in the batch
-            // phase like any other (they become hubs and push); the slots
-            // still prune wherever both ends were reached.
-            if (hipMemsetAsync(used.p, 0, graph->n, ctx->stream) != hipSuccess) s = PADO_GPU_ERR_HIP;
-            idx->bp_pass_through = 1;
-        }
+        s = build_bp_phase(g, idx, c, used);
         if (s == PADO_GPU_OK) {
             Timer t1;
             s = build_batches(g, idx, used, c);
