@@ -207,12 +207,13 @@ typedef struct {
 #define PADO_GPU_BUILD_WIDE 4u
 /* Run the whole batch phase inside one workgroup (no reference counterpart):
  * the same kernels' work back to back with workgroup barriers in place of
- * kernel launches and host checks.  For graphs whose batches take many small
- * iterations (grids, road-like graphs: hundreds of BFS levels each), where the
- * launch schedule is bound by launch and host-check latency; slower for
- * graphs with large frontiers.  Same labels.  Its label pool is sized up
- * front; if it runs dry the build falls back to the launch schedule.
- * Ignored with PADO_GPU_BUILD_STREAM. */
+ * kernel launches and host checks.  Same labels.  It removes the launch and
+ * host-check latency of every iteration but runs on one compute unit, so it
+ * pays off only on small graphs (measured on MI355X: 7x faster on a
+ * 1K-vertex R-MAT, 1.7x slower on a 1.6K-vertex grid, 20-40x slower on
+ * 20K-60K-vertex grids); the autotuner tries it below 4K vertices.  Its label
+ * pool is sized up front; if it runs dry the build falls back to the launch
+ * schedule.  Ignored with PADO_GPU_BUILD_STREAM. */
 #define PADO_GPU_BUILD_DEVICE_LOOP 8u
 
 /* ParaVertexCentricPLL<BATCH_SIZE>(const Graph&) -> construct (PARA:309-313,
@@ -360,7 +361,8 @@ pado_gpu_status pado_gpu_windex_load(pado_gpu_ctx *ctx, const char *filename, pa
  * the push work-descriptor size {8, 16, 32}, push block shape {2, 4, 8}
  * waves, push and add grids {256..4096}, iterations per host check {2..16},
  * batch size {256, 512, 1024} and the flags (hipGraph or stream batch step,
- * bit-parallel variant; PADO_GPU_BUILD_WIDE of `start` is kept), starting
+ * bit-parallel variant, and below 4K vertices the one-workgroup batch loop;
+ * PADO_GPU_BUILD_WIDE of `start` is kept), starting
  * from `start` (NULL = defaults).  One untimed build first, then every
  * configuration is timed as the median of three builds.  max_trials caps the
  * configurations timed (0 = 24); trials_out (nullable) receives their number,

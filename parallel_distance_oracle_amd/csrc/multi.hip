@@ -17,7 +17,6 @@
 #include <cstring>
 #include <mutex>
 #include <numeric>
-#include <memory>
 #include <thread>
 
 #include "internal.hpp"
@@ -264,68 +263,6 @@ pado_gpu_status build_alone(pado_gpu_ctx *wctx, GraphSlot &G, const uint64_t *of
     return PADO_GPU_OK;
 }
 
-// Graphs built alone, a group at a time on one worker: each graph's
-// bit-parallel phase, then one k_batch_loop launch for the whole group's
-// batch phases (one workgroup per graph: these are the deep graphs, whose
-// batches take hundreds of small iterations), then each graph's canonical
-// sort.  A graph whose fixed loop pool ran dry, or that left the u8 domain,
-// is rebuilt by build_alone.
-pado_gpu_status build_alone_group(pado_gpu_ctx *wctx, std::vector<GraphSlot *> &grp, const uint64_t *off,
-                                  const uint32_t *adj, const pado_gpu_build_cfg *cfg)
-{
-    pado_gpu_build_cfg base{};
-    if (cfg) base = *cfg;
-    if (base.batch_size == 0) base.batch_size = kBatch;
-    struct One {
-        GraphSlot *G = nullptr;
-        pado_gpu_graph *gg = nullptr;
-        pado_gpu_index *idx = nullptr;
-        DevBuf<uint8_t> used;
-        bool deep = false;
-        ~One()
-        {
-            delete idx;
-            if (gg) pado_gpu_graph_free(gg);
-        }
-    };
-    std::vector<std::unique_ptr<One>> v;
-    std::vector<LoopJob> jobs;
-    for (GraphSlot *G : grp) {
-        v.emplace_back(new One());
-        One &o = *v.back();
-        o.G = G;
-        o.deep = eccentricity(G->n, off + G->voff, adj + G->eoff, 0) >= kInf8;
-        PADO_TRY(pado_gpu_graph_upload(wctx, G->n, off[G->voff + G->n], off + G->voff, adj + G->eoff, nullptr, &o.gg));
-        o.idx = new pado_gpu_index();
-        o.idx->ctx = wctx;
-        o.idx->n = G->n;
-        o.idx->wide = o.deep ? 1u : 0u;
-        pado_gpu_build_cfg c = base;
-        if (o.deep) c.flags |= PADO_GPU_BUILD_WIDE;
-        PADO_TRY(build_bp_phase(o.gg, o.idx, c, o.used));
-        LoopJob J;
-        J.g = o.gg;
-        J.idx = o.idx;
-        J.used = o.used.p;
-        J.batch = base.batch_size;
-        jobs.push_back(J);
-    }
-    PADO_TRY(build_batches_loop(jobs, wctx->stream));
-    for (size_t i = 0; i < v.size(); ++i) {
-        One &o = *v[i];
-        if (jobs[i].status == PADO_GPU_OK) {
-            o.G->alone = o.idx;
-            o.G->status = PADO_GPU_OK;
-            o.idx = nullptr;
-        } else if (jobs[i].status == PADO_GPU_ERR_NOMEM || jobs[i].status == PADO_GPU_ERR_DOMAIN) {
-            PADO_TRY(build_alone(wctx, *o.G, off, adj, cfg));
-        } else {
-            return jobs[i].status;
-        }
-    }
-    return PADO_GPU_OK;
-}
-
 }  // namespace
 
 extern "C" {
@@ -430,27 +367,21 @@ pado_gpu_status pado_gpu_build_multi(pado_gpu_ctx *ctx, uint32_t num_graphs, con
     for (auto &grp : groups) std::sort(grp.begin(), grp.end());
     mi->unions.resize(groups.size());
 
-    // Tasks: the graphs built alone first, in `workers` groups dealt round
-    // robin by size (each group's batch phases are one long single-launch
-    // job, so they start first), then the unions (largest first).
+    // Tasks: unions first (largest first), then the graphs built alone
+    // (largest first).
     struct Task {
-        int union_id;   // >= 0: a union
-        int group_id;   // >= 0: a group of graphs built alone
+        int union_id;   // >= 0: a union, else -1
+        uint32_t graph; // a graph built alone
         uint64_t size;
     };
-    std::vector<uint32_t> alone;
-    for (uint32_t g = 0; g < num_graphs; ++g)
-        if (!in_domain[g]) alone.push_back(g);
-    std::stable_sort(alone.begin(), alone.end(),
-                     [&](uint32_t x, uint32_t y) { return mi->graphs[x].n > mi->graphs[y].n; });
-    std::vector<std::vector<GraphSlot *>> agroups(std::min<size_t>(workers, alone.size()));
-    for (size_t i = 0; i < alone.size(); ++i) agroups[i % agroups.size()].push_back(&mi->graphs[alone[i]]);
     std::vector<Task> tasks;
-    for (size_t k = 0; k < agroups.size(); ++k) tasks.push_back({-1, (int)k, 0});
-    std::vector<Task> utasks;
-    for (size_t k = 0; k < groups.size(); ++k) utasks.push_back({(int)k, -1, gv[k]});
-    std::stable_sort(utasks.begin(), utasks.end(), [](const Task &a, const Task &b) { return a.size > b.size; });
-    tasks.insert(tasks.end(), utasks.begin(), utasks.end());
+    for (size_t k = 0; k < groups.size(); ++k) tasks.push_back({(int)k, 0u, gv[k]});
+    std::vector<Task> alone;
+    for (uint32_t g = 0; g < num_graphs; ++g)
+        if (!in_domain[g]) alone.push_back({-1, g, mi->graphs[g].n});
+    std::stable_sort(tasks.begin(), tasks.end(), [](const Task &a, const Task &b) { return a.size > b.size; });
+    std::stable_sort(alone.begin(), alone.end(), [](const Task &a, const Task &b) { return a.size > b.size; });
+    tasks.insert(tasks.end(), alone.begin(), alone.end());
 
     std::atomic<size_t> next_task{0};
     std::atomic<int> failed{0};
@@ -469,9 +400,9 @@ pado_gpu_status pado_gpu_build_multi(pado_gpu_ctx *ctx, uint32_t num_graphs, con
                     for (uint32_t g : groups[T.union_id]) mi->graphs[g].union_id = T.union_id;
                 }
             } else {
-                s = build_alone_group(wctx, agroups[T.group_id], off, adj, cfg);
-                for (GraphSlot *G : agroups[T.group_id])
-                    if (G->alone) G->alone->ctx = ctx;  // the job's context outlives the worker's
+                GraphSlot &G = mi->graphs[T.graph];
+                s = build_alone(wctx, G, off, adj, cfg);
+                if (G.alone) G.alone->ctx = ctx;
             }
         }
         if (s != PADO_GPU_OK) {
