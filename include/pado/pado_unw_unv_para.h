@@ -116,18 +116,20 @@ public:
     // unsigned bit-parallel gate over-estimates some distances to its
     // bit-parallel roots; this returns the exact value (see
     // query_distance_reference for the reference's answer).
+    // One launch per call, no copies (pado_gpu_query_pair); a loop over many
+    // pairs is faster through the batched form below.
     weighti query_distance(idi a, idi b)
     {
-        uint8_t out = 0;
-        detail::die_on(pado_gpu_query(detail::shared_ctx(), idx_, &a, &b, 1, &out, 0, nullptr), "pado_gpu_query");
-        return out;
+        uint32_t out = 0;
+        detail::die_on(pado_gpu_query_pair(detail::shared_ctx(), idx_, a, b, 0, &out), "pado_gpu_query_pair");
+        return (weighti)out;
     }
     weighti query_distance_reference(idi a, idi b)
     {
-        uint8_t out = 0;
-        detail::die_on(pado_gpu_query(detail::shared_ctx(), idx_, &a, &b, 1, &out, PADO_GPU_QUERY_REF_COMPAT, nullptr),
-                       "pado_gpu_query");
-        return out;
+        uint32_t out = 0;
+        detail::die_on(pado_gpu_query_pair(detail::shared_ctx(), idx_, a, b, PADO_GPU_QUERY_REF_COMPAT, &out),
+                       "pado_gpu_query_pair");
+        return (weighti)out;
     }
     // Batched form (one GPU launch for all pairs).
     void query_distance(const std::vector<idi> &a, const std::vector<idi> &b, std::vector<weighti> &out)

@@ -84,16 +84,15 @@ public:
     weightiLarge query_distance(idi a, idi b)
     {
         uint32_t out = 0;
-        detail::die_on(pado_gpu_wquery(detail::shared_ctx(), idx_, &a, &b, 1, &out, PADO_GPU_QUERY_REF_COMPAT,
-                                       nullptr),
-                       "pado_gpu_wquery");
+        detail::die_on(pado_gpu_query_pair(detail::shared_ctx(), idx_, a, b, PADO_GPU_QUERY_REF_COMPAT, &out),
+                       "pado_gpu_query_pair");
         return (weightiLarge)out;
     }
     // Exact u32 distance (PADO_GPU_WINF if unreachable).
     uint32_t query_distance_exact(idi a, idi b)
     {
         uint32_t out = 0;
-        detail::die_on(pado_gpu_wquery(detail::shared_ctx(), idx_, &a, &b, 1, &out, 0, nullptr), "pado_gpu_wquery");
+        detail::die_on(pado_gpu_query_pair(detail::shared_ctx(), idx_, a, b, 0, &out), "pado_gpu_query_pair");
         return out;
     }
     // Batched exact form (one GPU launch for all pairs).

@@ -300,6 +300,18 @@ pado_gpu_status pado_gpu_query(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uin
  * the answers are in `out`.  All calls on one GPU share the array: calls that
  * overlap on one GPU (from several threads or contexts) race and must be
  * serialised by the caller. */
+/* query_distance(a, b) (PARA:2283-2416, WPARA:2125-2243) for one pair, as the
+ * reference's drivers call it in a loop (pado.cpp:114-119): the ids go to
+ * the kernel as arguments and the answer comes back through mapped host
+ * memory, so a pair costs one launch and one stream synchronisation (no
+ * copies).  *out: u8 hops (255 = unreachable or >= 253) for an index in the
+ * reference's domain, u16 exact hops (0xFFFF = unreachable) for a wide one,
+ * u32 distances for a weighted one (flags as for pado_gpu_query /
+ * pado_gpu_wquery: PADO_GPU_QUERY_REF_COMPAT, PADO_GPU_QUERY_RANK_SPACE).
+ * Calls on one index are serialised. */
+pado_gpu_status pado_gpu_query_pair(pado_gpu_ctx *ctx, pado_gpu_index *idx, uint32_t a, uint32_t b, uint32_t flags,
+                                    uint32_t *out);
+
 /* query_distance for an index of either hop domain with u16 answers: a
  * wide-domain index answers exact hop counts, 0xFFFF if unreachable (its
  * bit-parallel part uses only slots reached on both sides; REF_COMPAT is

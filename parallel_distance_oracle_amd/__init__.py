@@ -145,6 +145,7 @@ def lib():
         "pado_gpu_query": (ctypes.c_int, [_vp, _vp, _vp, _vp, _u64, _vp, _u32, _vp]),
         "pado_gpu_query_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _u64, _vp, _u32]),
         "pado_gpu_query_wide": (ctypes.c_int, [_vp, _vp, _vp, _vp, _u64, _vp, _u32, _vp]),
+        "pado_gpu_query_pair": (ctypes.c_int, [_vp, _vp, _u32, _u32, _u32, ctypes.POINTER(_u32)]),
         "pado_gpu_index_export_wide": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
         "pado_gpu_index_free": (None, [_vp]),
         "pado_gpu_index_store": (ctypes.c_int, [_vp, _vp, ctypes.c_char_p, _vp, _u32]),
@@ -644,6 +645,13 @@ class Index:
             out = np.zeros(max(a.size, 1), np.uint8)
             _check(lib().pado_gpu_query(self.ctx.h, self.h, _p(a), _p(b), a.size, _p(out), flags, None))
         return out[: a.size]
+
+    def query_pair(self, a, b, ref_compat=False, rank_space=False):
+        """One pair through pado_gpu_query_pair (one launch, no copies)."""
+        out = _u32()
+        flags = (QUERY_REF_COMPAT if ref_compat else 0) | (QUERY_RANK_SPACE if rank_space else 0)
+        _check(lib().pado_gpu_query_pair(self.ctx.h, self.h, int(a), int(b), flags, ctypes.byref(out)))
+        return out.value
 
     def query_device(self, a_ptr, b_ptr, k, out_ptr, ref_compat=False, rank_space=False, stream=None):
         """Asynchronous query on device buffers (device pointers of the library's

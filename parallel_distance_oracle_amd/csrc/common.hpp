@@ -224,6 +224,18 @@ struct pado_gpu_index {
     pado::DevBuf<uint32_t> qa, qb;
     pado::DevBuf<uint8_t> qout;
     pado::DevBuf<uint32_t> qout32;
+    // One-pair queries: the answer word in mapped host memory, the kernel's
+    // result slot (pado_gpu_query_pair); its lock serialises callers.
+    uint32_t *pair_out = nullptr;
+    uint32_t *pair_out_dev = nullptr;
+    std::mutex pair_mu;
+    ~pado_gpu_index()
+    {
+        if (pair_out) {
+            std::lock_guard<std::recursive_mutex> g(pado::capture_mutex());
+            (void)hipHostFree(pair_out);
+        }
+    }
     // Build statistics.
     double ms_bp = 0, ms_batch = 0, ms_final = 0;
     uint32_t bp_roots[pado::kBP] = {0};

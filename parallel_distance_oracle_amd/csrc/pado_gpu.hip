@@ -36,6 +36,8 @@ pado_gpu_status launch_wquery(const pado_gpu_index *idx, bool rank_space, const 
                               uint64_t k, uint32_t *out, bool ref_compat, hipStream_t st);
 pado_gpu_status launch_query_wide(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,
                                   uint64_t k, uint16_t *out, bool ref_compat, hipStream_t st);
+pado_gpu_status launch_query_pair(const pado_gpu_index *idx, bool rank_space, uint32_t a, uint32_t b,
+                                  bool ref_compat, uint32_t *out_dev, hipStream_t st);
 
 namespace {
 
@@ -644,6 +646,48 @@ pado_gpu_status pado_gpu_query(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uin
                               [&](const uint32_t *da, const uint32_t *db, uint64_t n, uint8_t *dout, hipStream_t s) {
                                   return launch_query(idx, rank_space, da, db, n, dout, ref, s);
                               });
+}
+
+pado_gpu_status pado_gpu_query_pair(pado_gpu_ctx *ctx, pado_gpu_index *idx, uint32_t a, uint32_t b, uint32_t flags,
+                                    uint32_t *out)
+{
+    clear_error();
+    PADO_TRY(check_ctx(ctx));
+    if (!idx || !out) {
+        set_error("null argument");
+        return PADO_GPU_ERR_ARG;
+    }
+    const bool rank_space = (flags & PADO_GPU_QUERY_RANK_SPACE) != 0;
+    if (!rank_space && !idx->ordered) {
+        set_error("query before order_labels: call pado_gpu_index_order or pass PADO_GPU_QUERY_RANK_SPACE");
+        return PADO_GPU_ERR_STATE;
+    }
+    if (rank_space && !idx->loff.p) {
+        set_error("index has no rank-space labels");
+        return PADO_GPU_ERR_STATE;
+    }
+    std::lock_guard<std::mutex> g(idx->pair_mu);
+    if (!idx->pair_out) {
+        std::lock_guard<std::recursive_mutex> cg(capture_mutex());
+        uint32_t *h = nullptr;
+        void *d = nullptr;
+        if (hipHostMalloc((void **)&h, sizeof(uint32_t), hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+            set_error("pinned answer slot allocation failed");
+            return PADO_GPU_ERR_NOMEM;
+        }
+        if (hipHostGetDevicePointer(&d, h, 0) != hipSuccess) {
+            (void)hipHostFree(h);
+            set_error("answer slot is not mapped for the device");
+            return PADO_GPU_ERR_HIP;
+        }
+        idx->pair_out = h;
+        idx->pair_out_dev = static_cast<uint32_t *>(d);
+    }
+    PADO_TRY(launch_query_pair(idx, rank_space, a, b, (flags & PADO_GPU_QUERY_REF_COMPAT) != 0, idx->pair_out_dev,
+                               ctx->stream));
+    PADO_HIP(hipStreamSynchronize(ctx->stream));
+    *out = __atomic_load_n(idx->pair_out, __ATOMIC_ACQUIRE);
+    return PADO_GPU_OK;
 }
 
 pado_gpu_status pado_gpu_query_wide(pado_gpu_ctx *ctx, pado_gpu_index *idx, const uint32_t *a, const uint32_t *b,

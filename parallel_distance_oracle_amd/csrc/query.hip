@@ -646,6 +646,43 @@ pado_gpu_status launch_serve_resident(const pado_gpu_index *idx, const ServeKern
     return PADO_GPU_OK;
 }
 
+// One pair, ids passed by value, the answer written into mapped host memory
+// with a system-scope store: a lone query costs one launch and one stream
+// synchronisation, no copies (K: 0 u8 hops, 1 wide, 2 weighted).
+template <int K>
+__global__ __launch_bounds__(64) void k_query_pair(uint32_t n, const uint8_t *bpd, const ulonglong2 *bps,
+                                                   const uint64_t *loff, const uint32_t *lab, const uint32_t *wdist,
+                                                   uint32_t a, uint32_t b, int ref_compat, uint32_t *out)
+{
+    __shared__ uint32_t sbuf[kQLds];
+    uint32_t d;
+    if (K == 2)
+        d = wpair_distance(n, loff, lab, wdist, a, b, ref_compat != 0, sbuf);
+    else
+        d = pair_distance<K == 1>(n, bpd, bps, loff, lab, a, b, ref_compat != 0, sbuf);
+    if (threadIdx.x == 0) __hip_atomic_store(out, d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+pado_gpu_status launch_query_pair(const pado_gpu_index *idx, bool rank_space, uint32_t a, uint32_t b,
+                                  bool ref_compat, uint32_t *out_dev, hipStream_t st)
+{
+    const uint64_t *loff = rank_space ? idx->loff.p : idx->qoff.p;
+    if (idx->weighted)
+        hipLaunchKernelGGL(k_query_pair<2>, dim3(1), dim3(64), 0, st, idx->n, nullptr, nullptr, loff,
+                           rank_space ? idx->whub.p : idx->qlab.p, rank_space ? idx->wdist.p : idx->qwdist.p, a, b,
+                           ref_compat ? 1 : 0, out_dev);
+    else if (idx->wide)
+        hipLaunchKernelGGL(k_query_pair<1>, dim3(1), dim3(64), 0, st, idx->n, rank_space ? idx->bpd.p : idx->qbpd.p,
+                           (const ulonglong2 *)(rank_space ? idx->bps.p : idx->qbps.p), loff,
+                           rank_space ? idx->lab.p : idx->qlab.p, nullptr, a, b, 0, out_dev);
+    else
+        hipLaunchKernelGGL(k_query_pair<0>, dim3(1), dim3(64), 0, st, idx->n, rank_space ? idx->bpd.p : idx->qbpd.p,
+                           (const ulonglong2 *)(rank_space ? idx->bps.p : idx->qbps.p), loff,
+                           rank_space ? idx->lab.p : idx->qlab.p, nullptr, a, b, ref_compat ? 1 : 0, out_dev);
+    PADO_HIP(hipGetLastError());
+    return PADO_GPU_OK;
+}
+
 pado_gpu_status launch_wquery(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,
                               uint64_t k, uint32_t *out, bool ref_compat, hipStream_t st);
 pado_gpu_status launch_query(const pado_gpu_index *idx, bool rank_space, const uint32_t *qa, const uint32_t *qb,
