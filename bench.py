@@ -604,6 +604,20 @@ def main():
     qbytes = 8 + 1 + 2 * BP_RECORD_BYTES + 5 * (lab_per_v[rank[qa]] + lab_per_v[rank[qb]])
     query["alg_bytes"] = int(qbytes.sum())
     query["achieved_GBps"] = round(query["alg_bytes"] / (q_ms * 1e-3) / 1e9, 1)
+    # The drop-in's query_distance(a, b) loop (pado.cpp:114-119 calls it once
+    # per pair): one-pair calls (pado_gpu_query_pair), and the same pairs as
+    # one-element batch calls, each checked against the batch answers.
+    npair = 2000
+    t0 = time.perf_counter()
+    one = [idx.query_pair(int(x), int(y)) for x, y in zip(qa[:npair], qb[:npair])]
+    t_pair = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    batch1 = [int(idx.query(qa[i:i + 1], qb[i:i + 1])[0]) for i in range(npair)]
+    t_batch1 = time.perf_counter() - t0
+    query["pair_calls_us"] = round(t_pair / npair * 1e6, 2)
+    query["one_pair_batch_calls_us"] = round(t_batch1 / npair * 1e6, 2)
+    query["pair_calls_equal_batch"] = bool(np.array_equal(np.array(one, np.uint8), res[:npair]) and
+                                           np.array_equal(np.array(batch1, np.uint8), res[:npair]))
     log(f"query: {query}")
 
     serving = None
