@@ -34,7 +34,9 @@
 //   lmask[n][8] u64  bit-parallel level masks (bit i of word k: slot i at
 //                    distance k); rmask / rset: the batch roots' masks and
 //                    set pairs, level- / slot-major
-//   hinfo[n] u32     per-hub 0xFFFFFF00 | min distance from any batch root
+//   hinfo[n] 32 B    per hub: lane masks of the batch roots at distance 1, 2,
+//                    3 (bit l: roots [16l, 16l+16)) and 0xFFFFFF00 | min
+//                    distance from any batch root
 //   dc[h][1024] u8   root-distance table: dc[h][r - rs] = dist(root r, hub h),
 //                    the transposed form of dist_matrix, rows by hub id
 // Kernels per iteration: k_push (edge-parallel, one edge per lane, 16-edge
@@ -386,6 +388,45 @@ struct LabelStore {
     unsigned long long pool_size;
 };
 
+// Each vertex's labels at distance 1, kept apart from the label arrays for
+// the distance query of iteration 2 (the only one that needs them alone).  A
+// distance-1 label comes from a root pushing along an edge in iteration 1, so
+// its hub is a neighbour and a vertex has at most deg(v) of them: vertex v's
+// list lives in the adjacency slots [off[v], off[v + 1]) of lab, packed like
+// the pool, with len[v] entries.
+struct HubList1 {
+    uint32_t *lab;
+    uint32_t *len;
+};
+
+// Summary of one row of the root-distance table (hub h): near[k] has bit l
+// set iff some batch root in [16l, 16l+16) -- the roots lane l of k_add owns --
+// is at distance k + 1 from h; info is 0xFFFFFF00 | the minimum distance of
+// any batch root (0xFFFFFFFF when h is no hub of any of them).  A hub can
+// prune a candidate root r of v only if d_r(h) <= iter - d_v(h), so for
+// thresholds up to kNear the masks name the lanes whose row slices can
+// matter, and the minimum rules out the rest.
+constexpr uint32_t kNear = 3;
+struct alignas(32) HubInfo {
+    unsigned long long near[kNear];
+    uint32_t info, pad;
+};
+
+__device__ inline void hub_info_clear(HubInfo &x)
+{
+    for (uint32_t k = 0; k < kNear; ++k) x.near[k] = 0ull;
+    x.info = kEmpty;
+}
+
+__global__ void k_hub_info_init(HubInfo *p, uint64_t count)
+{
+    uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) {
+        hub_info_clear(p[i]);
+        p[i].pad = 0u;
+    }
+}
+
 // Reserve room for `add` more labels of v (wave-uniform call).  Returns the
 // (possibly new) base; on pool exhaustion sets the overflow flag and returns
 // ~0ull.
@@ -533,14 +574,13 @@ __global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const 
 
 // Root-distance table (dist_matrix, PARA:986-1013, transposed): row h holds
 // dist(root r, hub h) at column r - rs for the hubs of the batch roots'
-// labels, 255 elsewhere; hinfo[h] holds 0xFFFFFF00 | min over the batch
-// roots (0xFFFFFFFF when h is no hub of any of them).  One wave per root; each
-// lane handles four labels per step so the dependent loads of a step overlap.
-// With `fill` the root's labels are entered; without, the entries of the
-// previous batch's roots (the labels they held at its end, which include
-// every entry written during it) are reset.
+// labels, 255 elsewhere; hinfo[h] summarises the row (HubInfo).  One wave per
+// root; each lane handles four labels per step so the dependent loads of a
+// step overlap.  With `fill` the root's labels are entered; without, the
+// entries of the previous batch's roots (the labels they held at its end,
+// which include every entry written during it) are reset.
 template <bool W>
-__device__ __forceinline__ void hub_table_body(uint32_t bid, uint32_t nblk, uint32_t fill, const uint8_t *used, LabelStore S, uint32_t *hinfo,
+__device__ __forceinline__ void hub_table_body(uint32_t bid, uint32_t nblk, uint32_t fill, const uint8_t *used, LabelStore S, HubInfo *hinfo,
                                                    typename Hop<W>::Dist *dc, const BuildCounters *C)
 {
     constexpr uint32_t B = Hop<W>::kBits, INF = Hop<W>::kInf;
@@ -555,7 +595,7 @@ __device__ __forceinline__ void hub_table_body(uint32_t bid, uint32_t nblk, uint
     const uint32_t len = rec.len;
     const unsigned long long base = rec.base;
     for (uint32_t j0 = 0; j0 < len; j0 += 256) {
-        uint32_t lab[4], info[4];
+        uint32_t lab[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             const uint32_t j = j0 + 64 * u + lane;
@@ -567,26 +607,31 @@ __device__ __forceinline__ void hub_table_body(uint32_t bid, uint32_t nblk, uint
                 if (lab[u] == kEmpty) continue;
                 const uint32_t h = lab[u] >> B;
                 dc[(uint64_t)h * kBatch + ro] = (typename Hop<W>::Dist)~0u;
-                hinfo[h] = kEmpty;
+                hub_info_clear(hinfo[h]);
             }
             continue;
         }
+        HubInfo hi[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) info[u] = lab[u] != kEmpty ? hinfo[lab[u] >> B] : 0u;
+        for (int u = 0; u < 4; ++u)
+            if (lab[u] != kEmpty) hi[u] = hinfo[lab[u] >> B];
+        const unsigned long long bit = 1ull << (ro >> 4);
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (lab[u] == kEmpty) continue;
             const uint32_t h = lab[u] >> B, d = lab[u] & INF;
             dc[(uint64_t)h * kBatch + ro] = (typename Hop<W>::Dist)d;
-            // The minimum only falls during the kernel, so a value read as
-            // <= d (even a stale one) makes the atomic a no-op.
-            if (d < (info[u] & INF)) atomicMin(&hinfo[h], ~INF | d);
+            // The minimum only falls and mask bits are only set during the
+            // kernel, so a value read as already there (even a stale one)
+            // makes the atomic a no-op.
+            if (d < (hi[u].info & INF)) atomicMin(&hinfo[h].info, ~INF | d);
+            if (d >= 1u && d <= kNear && !(hi[u].near[d - 1] & bit)) atomicOr(&hinfo[h].near[d - 1], bit);
         }
     }
 }
 
 template <bool W>
-__global__ __launch_bounds__(256) void k_hub_table(uint32_t fill, const uint8_t *used, LabelStore S, uint32_t *hinfo,
+__global__ __launch_bounds__(256) void k_hub_table(uint32_t fill, const uint8_t *used, LabelStore S, HubInfo *hinfo,
                                                    typename Hop<W>::Dist *dc, const BuildCounters *C)
 {
     hub_table_body<W>(blockIdx.x, gridDim.x, fill, used, S, hinfo, dc, C);
@@ -892,10 +937,14 @@ __global__ __launch_bounds__(1024) void k_batch_begin(const uint8_t *used, Build
 // [16l, 16l+16).  distance_query (PARA:1218-1270) runs over v's labels with
 // dist < iter: a hub h can prune only if d_v(h) + min_r d_r(h) <= iter, so a
 // per-hub minimum filters most hubs before the 1 KB root-distance row of the
-// survivors is read (16 B per lane).
+// survivors is read (16 B per lane).  Only hubs below the candidate root
+// count (PARA:1250-1254), so v's own label (hub v, above every root that
+// pushes to v) never prunes: iteration 1, whose only labels with dist < 1 are
+// such self labels, scans nothing, and iteration 2 scans v's distance-1 list
+// instead of the whole label array.
 template <bool W, uint32_t CH>
 __device__ __forceinline__ void add_body(uint32_t bid, uint32_t nblk, uint32_t iter, const uint8_t *used, const uint64_t *off, const uint32_t *adj, const uint32_t *clist,
-                                             unsigned long long *cand, LabelStore S, uint32_t *hinfo,
+                                             unsigned long long *cand, LabelStore S, HubList1 H1, HubInfo *hinfo,
                                              typename Hop<W>::Dist *dc,
                                              unsigned long long *aoff, unsigned long long *ag, uint32_t *amask,
                                              uint32_t *aend, unsigned long long *chunks, BuildCounters *C)
@@ -932,8 +981,14 @@ __device__ __forceinline__ void add_body(uint32_t bid, uint32_t nblk, uint32_t i
         const uint32_t cap = rec.cap;
         const bool v_used = used[v] != 0;
         const uint64_t e0 = off[v], e1 = off[v + 1];
+        const uint32_t n1 = iter <= 2u ? H1.len[v] : 0u;  // distance-1 labels held
         if (i + nwaves < ncand) v_next = clist[i + nwaves];
-        const uint32_t lab0 = lane < len ? S.pool[base + lane] : 0u;
+        // The labels the distance query scans (see above); the first 64 of
+        // the distance-1 list are read within v's slots before n1 is known.
+        const uint32_t slen = iter == 1u ? 0u : (iter == 2u ? n1 : len);
+        const uint32_t *lsrc = iter == 2u ? H1.lab + e0 : S.pool + base;
+        const uint32_t lim0 = iter == 1u ? 0u : (iter == 2u ? (uint32_t)(e1 - e0) : len);
+        const uint32_t lab0 = lane < lim0 ? lsrc[lane] : 0u;
         const uint32_t pdeg = v_used ? 0u : push_degree_range(adj, e0, e1, rs);
         const uint32_t cb = (uint32_t)(word >> (16 * (lane & 3))) & 0xFFFFu;
         // The clear below is control-dependent on every lane's loaded word
@@ -942,23 +997,35 @@ __device__ __forceinline__ void add_body(uint32_t bid, uint32_t nblk, uint32_t i
         if (__ballot(cb != 0) != 0ull && lane < kWords) cand[(uint64_t)v * kWords + lane] = 0ull;
         uint32_t prune = 0;
         const uint32_t my_lo = rs + 16 * lane;  // real id of this lane's first root
-        for (uint32_t j0 = 0; j0 < len; j0 += 64) {
+        for (uint32_t j0 = 0; j0 < slen; j0 += 64) {
             const uint32_t j = j0 + lane;
-            uint32_t h = 0, d = INF, info = kEmpty;
-            if (j < len) {
-                const uint32_t lab = j0 == 0 ? lab0 : S.pool[base + j];
+            uint32_t h = 0, d = INF;
+            unsigned long long lm = 0ull;  // lanes whose roots h can reach in time
+            if (j < slen) {
+                const uint32_t lab = j0 == 0 ? lab0 : lsrc[j];
                 h = lab >> B;
                 d = lab & INF;
                 if (d < iter) {
-                    info = hinfo[h];
+                    const HubInfo hi = hinfo[h];
+                    const uint32_t t = iter - d;
+                    if (t <= kNear) {
+                        lm = hi.near[0];
+                        if (t >= 2u) lm |= hi.near[1];
+                        if (t >= 3u) lm |= hi.near[2];
+                    } else if (d + (hi.info & INF) <= iter) {
+                        lm = ~0ull;
+                    }
                     ++n_read;
                 }
             }
-            const bool pass = d + (info & INF) <= iter;
-            unsigned long long m = __ballot(pass);
+            // A hub is worth a row only if some lane that still holds an
+            // unpruned candidate has a root close enough to it.
+            const unsigned long long live = __ballot((cb & ~prune) != 0u);
+            unsigned long long m = __ballot((lm & live) != 0ull);
             // Up to four passing hubs per step: their row loads are issued
-            // together, and only lanes that still hold an unpruned candidate
-            // load their slice of a row (16 entries: 16 bytes, 32 wide).
+            // together, and only lanes of the hub's mask that still hold an
+            // unpruned candidate load their slice of a row (16 entries: 16
+            // bytes, 32 wide).
             while (m) {
                 int src[4];
                 bool ok[4];
@@ -980,9 +1047,11 @@ __device__ __forceinline__ void add_body(uint32_t bid, uint32_t nblk, uint32_t i
                     // all-INF row never meets
                     const uint32_t du = __shfl(d, src[u]);
                     thr[u] = ok[u] ? iter - du : 0u;
+                    const uint32_t mlo = __shfl((uint32_t)lm, src[u]), mhi = __shfl((uint32_t)(lm >> 32), src[u]);
+                    const bool mine = ((lane < 32 ? mlo : mhi) >> (lane & 31)) & 1u;
 #pragma unroll
                     for (uint32_t q = 0; q < RV; ++q) row[u][q] = make_uint4(~0u, ~0u, ~0u, ~0u);
-                    if (ok[u] && need) {
+                    if (ok[u] && need && mine) {
                         const uint4 *rp = reinterpret_cast<const uint4 *>(dc + (uint64_t)hh[u] * kBatch) + RV * lane;
 #pragma unroll
                         for (uint32_t q = 0; q < RV; ++q) row[u][q] = rp[q];
@@ -1029,13 +1098,20 @@ __device__ __forceinline__ void add_body(uint32_t bid, uint32_t nblk, uint32_t i
             S.pool[nb + pos++] = ((my_lo + q) << B) | iter;  // insert_label_only, PARA:1285
         }
         if (lane == 0) S.rec[v].len = len + total;
+        if (iter == 1u) {
+            // The new labels are at distance 1: v's list gets them too.
+            uint32_t p1 = n1 + incl - cnt;
+            for (uint32_t sb = surv; sb; sb &= sb - 1) H1.lab[e0 + p1++] = ((my_lo + __ffs(sb) - 1) << B) | 1u;
+            if (lane == 0) H1.len[v] = n1 + total;
+        }
         // Roots of this batch also record the new label in the root table
         // (dist_matrix[v_root][cand_real] = iter, PARA:1287-1290).
         if (v >= rs && v < rs + rsz) {
             for (uint32_t sb = surv; sb; sb &= sb - 1) {
                 const uint32_t h = my_lo + __ffs(sb) - 1;
                 dc[(uint64_t)h * kBatch + (v - rs)] = (typename Hop<W>::Dist)iter;
-                if (iter < (hinfo[h] & INF)) atomicMin(&hinfo[h], ~INF | iter);
+                if (iter < (hinfo[h].info & INF)) atomicMin(&hinfo[h].info, ~INF | iter);
+                if (iter <= kNear) atomicOr(&hinfo[h].near[iter - 1], 1ull << ((v - rs) >> 4));
             }
         }
         // v becomes active with the new distance group as its last group.
@@ -1092,12 +1168,12 @@ __device__ __forceinline__ void add_body(uint32_t bid, uint32_t nblk, uint32_t i
 
 template <bool W, uint32_t CH>
 __global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used, const uint64_t *off, const uint32_t *adj, const uint32_t *clist,
-                                             unsigned long long *cand, LabelStore S, uint32_t *hinfo,
+                                             unsigned long long *cand, LabelStore S, HubList1 H1, HubInfo *hinfo,
                                              typename Hop<W>::Dist *dc,
                                              unsigned long long *aoff, unsigned long long *ag, uint32_t *amask,
                                              uint32_t *aend, unsigned long long *chunks, BuildCounters *C)
 {
-    add_body<W, CH>(blockIdx.x, gridDim.x, iter, used, off, adj, clist, cand, S, hinfo, dc, aoff, ag, amask, aend, chunks, C);
+    add_body<W, CH>(blockIdx.x, gridDim.x, iter, used, off, adj, clist, cand, S, H1, hinfo, dc, aoff, ag, amask, aend, chunks, C);
 }
 
 // First kernel of a batch's prologue: if the previous batch still has a
@@ -1146,6 +1222,7 @@ struct BatchPlan {
     const uint64_t *off;
     const uint32_t *adj;
     LabelStore S;
+    HubList1 H1;
     unsigned long long *ind, *cand, *vmark;
     uint32_t *tlist, *clist;
     unsigned long long *aoff, *ag;
@@ -1156,7 +1233,7 @@ struct BatchPlan {
     unsigned long long *rmask;
     const ulonglong2 *bps;
     ulonglong2 *rset;
-    uint32_t *hinfo;
+    HubInfo *hinfo;
     typename Hop<W>::Dist *dc;
     BuildCounters *C;
 };
@@ -1204,7 +1281,7 @@ __global__ __launch_bounds__(kLoopWaves * 64) void k_batch_loop(const BatchPlan<
                                              P.cand, P.vmark, P.tlist, P.clist, P.bpd, P.lmask, P.rmask, P.bps,
                                              P.rset, C, P.bp_prune);
             __syncthreads();
-            add_body<W, kChunk>(0, 1, iter, P.used, P.off, P.adj, P.clist, P.cand, P.S, P.hinfo, P.dc, P.aoff,
+            add_body<W, kChunk>(0, 1, iter, P.used, P.off, P.adj, P.clist, P.cand, P.S, P.H1, P.hinfo, P.dc, P.aoff,
                                 P.ag, P.amask, P.aend, P.chunks, C);
             __syncthreads();
             const bool stop = C->overflow != 0u || C->domain != 0u;
@@ -1295,7 +1372,9 @@ struct BatchBuffers {
     DevBuf<unsigned long long> vmark;  // ((batch << 16 | iteration) << 1 | candidate) of the last mark
     DevBuf<unsigned long long> lmask;  // bit-parallel level masks, 8 words per vertex
     DevBuf<unsigned long long> rmask;  // batch roots' level masks, level-major [8][kBatch]
-    DevBuf<uint32_t> tlist, clist, amask, aend, hinfo, pool;
+    DevBuf<uint32_t> tlist, clist, amask, aend, pool;
+    DevBuf<HubInfo> hinfo;
+    DevBuf<uint32_t> d1, d1len;  // distance-1 lists (HubList1): m slots, n counts
     DevBuf<LRec> lrec;
     DevBuf<typename Hop<W>::Dist> dc;
     DevBuf<ulonglong2> rset;  // batch roots' bit-parallel sets, slot-major [kBP][kBatch]
@@ -1305,7 +1384,8 @@ struct BatchBuffers {
     void release()
     {
         for (auto *b : {&ind, &cand, &ag, &chunks, &aoff, &vmark, &lmask, &rmask}) b->release();
-        for (auto *b : {&tlist, &clist, &amask, &aend, &hinfo, &pool}) b->release();
+        for (auto *b : {&tlist, &clist, &amask, &aend, &pool, &d1, &d1len}) b->release();
+        hinfo.release();
         lrec.release();
         dc.release();
         rset.release();
@@ -1338,6 +1418,8 @@ static pado_gpu_status batch_alloc(const pado_gpu_graph *g, const pado_gpu_index
     PADO_TRY(B.amask.alloc(alist_cap));
     PADO_TRY(B.aend.alloc(alist_cap));
     PADO_TRY(B.hinfo.alloc(n));
+    PADO_TRY(B.d1.alloc(g->m ? g->m : 1));
+    PADO_TRY(B.d1len.alloc(n));
     PADO_TRY(B.lrec.alloc(n));
     PADO_TRY(B.rset.alloc((uint64_t)kBP * kBatch));
     PADO_TRY(B.lmask.alloc((uint64_t)n * 8));
@@ -1354,7 +1436,8 @@ static pado_gpu_status batch_alloc(const pado_gpu_graph *g, const pado_gpu_index
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, B.cand.p, (uint64_t)n * kWords);
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, B.vmark.p, (uint64_t)n);
     hipLaunchKernelGGL(k_bp_masks, dim3(fg), dim3(256), 0, st, n, idx->bpd.p, B.lmask.p);
-    hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, B.hinfo.p, (uint64_t)n, kEmpty);
+    hipLaunchKernelGGL(k_hub_info_init, dim3(fg), dim3(256), 0, st, B.hinfo.p, (uint64_t)n);
+    hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, B.d1len.p, (uint64_t)n, 0u);
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, reinterpret_cast<unsigned long long *>(B.lrec.p),
                        2ull * n);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t *>(B.ctr.p),
@@ -1463,7 +1546,8 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
                             std::min<uint32_t>(n, 1u << 16), B));
     auto &ind = B.ind, &cand = B.cand, &ag = B.ag, &chunks = B.chunks, &aoff = B.aoff, &vmark = B.vmark;
     auto &lmask = B.lmask, &rmask = B.rmask;
-    auto &tlist = B.tlist, &clist = B.clist, &amask = B.amask, &aend = B.aend, &hinfo = B.hinfo, &pool = B.pool;
+    auto &tlist = B.tlist, &clist = B.clist, &amask = B.amask, &aend = B.aend, &pool = B.pool;
+    auto &hinfo = B.hinfo;
     auto &lrec = B.lrec;
     auto &dc = B.dc;
     auto &rset = B.rset;
@@ -1506,6 +1590,7 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
     KernelTimes kt{};
 
     LabelStore S{pool.p, lrec.p, pool_size};
+    const HubList1 H1{B.d1.p, B.d1len.p};
     auto enqueue_prologue = [&]() {
         // Previous batch's reset_at_end first (a no-op before the first batch).
         hipLaunchKernelGGL(k_batch_gate, dim3(1), dim3(1), 0, st, ctr.p);
@@ -1529,7 +1614,7 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 1], st));
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 2], st));
             hipLaunchKernelGGL(K.add, dim3(add_grid), dim3(256), 0, st, iter, d_used.p, g->off.p, g->adj.p, clist.p,
-                               cand.p, S, hinfo.p, dc.p, aoff.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
+                               cand.p, S, H1, hinfo.p, dc.p, aoff.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 3], st));
         }
         return PADO_GPU_OK;
@@ -1769,6 +1854,7 @@ static BatchPlan<W> batch_plan(const pado_gpu_graph *g, const pado_gpu_index *id
     P.off = g->off.p;
     P.adj = g->adj.p;
     P.S = LabelStore{B.pool.p, B.lrec.p, B.pool_size};
+    P.H1 = HubList1{B.d1.p, B.d1len.p};
     P.ind = B.ind.p;
     P.cand = B.cand.p;
     P.vmark = B.vmark.p;
