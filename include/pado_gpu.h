@@ -178,6 +178,13 @@ typedef struct {
                                 larger indexes are sorted in vertex ranges */
     uint32_t push_chunk;  /* edges per push work descriptor: 8, 16 or 32, 0 = 16 */
     uint32_t push_waves;  /* waves per push block: 2, 4 or 8, 0 = 4 */
+    uint32_t pull_ratio;  /* direction of the push step: an iteration whose frontier
+                             has more push edges than pull_ratio / 1000 of the
+                             adjacency entries of the vertices >= roots_start runs
+                             in pull form (each tail gathers the roots offered to
+                             it; same labels and counters).  0 = 0xFFFFFFFF =
+                             always push: on R-MAT 20 every threshold measured
+                             slower (400: +0.6%, 100: +11%) */
 } pado_gpu_build_cfg;
 
 /* Bit-parallel labeling one BFS per slot (as the reference runs it) instead of
@@ -252,7 +259,7 @@ typedef struct {
     uint64_t n_push, n_add, n_bp_levels;  /* launches */
     uint32_t dist_bits;        /* label distance bits: 8 (the reference's u8
                                   domain) or 12 (PADO_GPU_BUILD_WIDE) */
-    uint32_t reserved_info;
+    uint32_t pull_iters;       /* batch iterations run in pull form */
 } pado_gpu_index_info;
 
 pado_gpu_status pado_gpu_index_get_info(const pado_gpu_index *idx, pado_gpu_index_info *info);

@@ -54,12 +54,12 @@ class BuildCfg(ctypes.Structure):
     _fields_ = [("batch_size", ctypes.c_uint32), ("flags", ctypes.c_uint32), ("push_blocks", ctypes.c_uint32),
                 ("add_blocks", ctypes.c_uint32), ("iters_per_sync", ctypes.c_uint32),
                 ("sort_max_items", ctypes.c_uint32), ("push_chunk", ctypes.c_uint32),
-                ("push_waves", ctypes.c_uint32)]
+                ("push_waves", ctypes.c_uint32), ("pull_ratio", ctypes.c_uint32)]
 
     def as_dict(self):
         return {"batch_size": self.batch_size, "flags": self.flags, "push_blocks": self.push_blocks,
                 "add_blocks": self.add_blocks, "iters_per_sync": self.iters_per_sync,
-                "push_chunk": self.push_chunk, "push_waves": self.push_waves}
+                "push_chunk": self.push_chunk, "push_waves": self.push_waves, "pull_ratio": self.pull_ratio}
 
 
 class ServeCfg(ctypes.Structure):
@@ -85,7 +85,7 @@ class IndexInfo(ctypes.Structure):
         ("dq_labels", ctypes.c_uint64), ("labels_appended", ctypes.c_uint64),
         ("ms_push", ctypes.c_double), ("ms_add", ctypes.c_double), ("ms_bp_levels", ctypes.c_double),
         ("n_push", ctypes.c_uint64), ("n_add", ctypes.c_uint64), ("n_bp_levels", ctypes.c_uint64),
-        ("dist_bits", ctypes.c_uint32), ("reserved_info", ctypes.c_uint32),
+        ("dist_bits", ctypes.c_uint32), ("pull_iters", ctypes.c_uint32),
     ]
 
 

@@ -31,8 +31,9 @@
 //                    vertex's array); rec[n] = (base, len, cap), 16 B each
 //   vmark[n] u64     ((batch << 16 | iteration) << 1 | candidate) of the last
 //                    push into the vertex, raised by atomicMax
-//   lmask[n][8] u64  bit-parallel level masks (bit i of word k: slot i at
-//                    distance k); rmask / rset: the batch roots' masks and
+//   lmask[n][16] u64 bit-parallel level masks (bit i of word k < 8: slot i at
+//                    distance k) and nonempty-set masks (words 8, 9: slots
+//                    with S^-1 / S^0 nonempty); rmask / rset: the batch roots' masks and
 //                    set pairs, level- / slot-major
 //   hinfo[n] 32 B    per hub: lane masks of the batch roots at distance 1, 2,
 //                    3 (bit l: roots [16l, 16l+16)) and 0xFFFFFF00 | min
@@ -70,10 +71,10 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long x)
 // edge offset is past every adjacency list, so k_push skips it.
 constexpr unsigned long long kChunkPad = 0xFFFFFFF0ull << 32;
 
-// k_add reserves next-frontier slots per wave: kAlistGrab active-list
-// entries and up to kChunkGrab push descriptors per atomic on the shared
-// counters (the unused rest of a reservation is a hole / padding).
-constexpr uint32_t kAlistGrab = 64, kChunkGrab = 64;
+// k_add reserves next-frontier push descriptors per wave, up to kChunkGrab
+// per atomic on the shared counter (the unused rest of a reservation is
+// padding).
+constexpr uint32_t kChunkGrab = 64;
 
 // Number of neighbours of v with id >= rs: the adjacency is sorted
 // descending, so they form a prefix (PARA:1121-1123 returns at the first tail
@@ -172,6 +173,22 @@ __device__ inline bool bp_dist_scan(const uint32_t (&dr)[13], const uint32_t (&d
 // the slots in L_r[a] & L_t[b].
 constexpr uint32_t kMaskIters = 5;
 
+// Words of a vertex's mask row: 8 level masks, then the slots whose S^-1 /
+// S^0 set is nonempty, then padding to one 128-byte line.
+constexpr uint32_t kLmWords = 16, kLmNz1 = 8, kLmNz0 = 9;
+
+// The candidate slots of bp_dist_scan / bp_mask_scan whose set test can
+// prune at all (bp_set_hit): at td == iter + 2 both S^-1 sets must be
+// nonempty, at td == iter + 1 the S^-1 set of one side and the S^-1 or S^0
+// set of the other.  The rest are dropped before any set pair is loaded.
+__device__ inline unsigned long long bp_nz_filter(unsigned long long cs, unsigned long long one,
+                                                  unsigned long long nz1r, unsigned long long nz0r,
+                                                  unsigned long long nz1t, unsigned long long nz0t)
+{
+    const unsigned long long two = cs & ~one, o1 = cs & one;
+    return (two & nz1r & nz1t) | (o1 & ((nz1r & (nz1t | nz0t)) | (nz0r & nz1t)));
+}
+
 template <int IT>
 __device__ inline bool bp_mask_scan_it(const unsigned long long *rm, uint32_t ro, const unsigned long long *lt,
                                        unsigned long long &cand, unsigned long long &one)
@@ -204,9 +221,9 @@ __device__ inline bool bp_mask_scan_it(const unsigned long long *rm, uint32_t ro
 }
 
 // Same contract as bp_dist_scan for 1 <= iter <= kMaskIters (wave-uniform):
-// root offset ro's masks come from the batch's level-major root table rm
+// root offset ro's masks come from the batch's word-major root table rm
 // (rm[k * kBatch + ro]: lanes holding consecutive roots read consecutive
-// words), the tail's from its 8-word line lt.
+// words), the tail's from its mask row lt.
 __device__ inline bool bp_mask_scan(uint32_t iter, const unsigned long long *rm, uint32_t ro,
                                     const unsigned long long *lt, unsigned long long &cand,
                                     unsigned long long &one)
@@ -308,13 +325,13 @@ struct SetScratch {
 // first tests its two lowest candidate slots, which settles most pruned
 // pairs; the (pair, slot) tests left over, mostly those of pairs that pass,
 // are then spread over the wave 64 at a time, so a pair with dozens of
-// candidate slots costs a few wave steps instead of dozens.  In the first
-// step lanes hold different roots, so root set pairs come from the batch's
-// slot-major table (consecutive roots are adjacent); in the spread steps
-// lanes hold slots of the same few pairs, so both sides come from the
-// vertex-major table.  Returns whether the lane's pair is pruned.
+// candidate slots costs a few wave steps instead of dozens.  Root set pairs
+// always come from the batch's slot-major table (800 KB, which stays in the
+// L2 cache), tail set pairs from the vertex-major table (in the spread steps
+// lanes hold slots of the same few tails, whose records they share).
+// Returns whether the lane's pair is pruned.
 __device__ inline bool bp_set_phase(unsigned long long rem, unsigned long long one, const ulonglong2 *rset,
-                                    uint32_t ro, uint32_t rs, const ulonglong2 *bps, uint32_t t, const SetScratch &sc)
+                                    uint32_t ro, const ulonglong2 *bps, uint32_t t, const SetScratch &sc)
 {
     const uint32_t lane = lane_id();
     bool pr = false;
@@ -341,7 +358,7 @@ __device__ inline bool bp_set_phase(unsigned long long rem, unsigned long long o
     if (total == 0) return pr;
     sc.rem[lane] = rem;
     sc.one[lane] = one;
-    sc.r[lane] = rs + ro;
+    sc.r[lane] = ro;
     sc.t[lane] = t;
     sc.ex[lane] = incl - cnt;
     sc.hit[lane] = 0u;
@@ -352,9 +369,7 @@ __device__ inline bool bp_set_phase(unsigned long long rem, unsigned long long o
         const uint32_t o = item_owner(sc.ex, x);
         const unsigned long long ro_rem = sc.rem[o];
         const uint32_t i = nth_set_bit(ro_rem, x - sc.ex[o]);
-        // Items of one pair are slots of one root: its vertex-major record
-        // keeps their loads together.
-        const ulonglong2 rr = bps[(uint64_t)sc.r[o] * kBP + i], tt = bps[(uint64_t)sc.t[o] * kBP + i];
+        const ulonglong2 rr = rset[i * kBatch + sc.r[o]], tt = bps[(uint64_t)sc.t[o] * kBP + i];
         if (bp_set_hit(rr, tt, (sc.one[o] >> i) & 1ull)) atomicOr(&sc.hit[o], 1u);
     }
     wave_lds_sync();
@@ -472,24 +487,29 @@ __global__ void k_zero_u64(unsigned long long *p, uint64_t count)
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += stride) p[i] = 0ull;
 }
 
-// Level masks (see bp_mask_scan) of every vertex, 64 bytes per vertex.
-__global__ __launch_bounds__(256) void k_bp_masks(uint32_t n, const uint8_t *bpd, unsigned long long *lmask)
+// Level masks (see bp_mask_scan) and nonempty-set masks (bp_nz_filter) of
+// every vertex: one 128-byte row per vertex.
+__global__ __launch_bounds__(256) void k_bp_masks(uint32_t n, const uint8_t *bpd, const ulonglong2 *bps,
+                                                  unsigned long long *lmask)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < n; v += stride) {
         uint32_t d[13];
         load_dline(bpd, (uint32_t)v, d);
-        unsigned long long m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        unsigned long long m[kLmWords] = {};
 #pragma unroll
         for (int i = 0; i < (int)kBP; ++i) {
             const uint32_t x = (d[i >> 2] >> (8 * (i & 3))) & 255u;
 #pragma unroll
             for (int k = 0; k < 8; ++k)
                 if (x == (uint32_t)k) m[k] |= 1ull << i;
+            const ulonglong2 st = bps[v * kBP + i];
+            if (st.x) m[8] |= 1ull << i;
+            if (st.y) m[9] |= 1ull << i;
         }
-        uint4 *dst = reinterpret_cast<uint4 *>(lmask + v * 8);
+        uint4 *dst = reinterpret_cast<uint4 *>(lmask + v * kLmWords);
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
+        for (int q = 0; q < (int)kLmWords / 2; ++q)
             dst[q] = make_uint4((uint32_t)m[2 * q], (uint32_t)(m[2 * q] >> 32), (uint32_t)m[2 * q + 1],
                                 (uint32_t)(m[2 * q + 1] >> 32));
     }
@@ -497,14 +517,14 @@ __global__ __launch_bounds__(256) void k_bp_masks(uint32_t n, const uint8_t *bpd
 
 // initialize (PARA:848-1080): non-BP roots become active with their own bit as
 // last group, get the self label (r, 0), and set their own indicator bit.
-// One wave per root, sixteen roots per block.  Root offset ro owns active
-// slot ro and touched-list slot ro (k_batch_begin sets both counts and the
-// frontier size), so the only shared counter left is the push-descriptor
-// count, updated once per block.
+// One wave per root, sixteen roots per block.  Root offset ro owns
+// touched-list slot ro (k_batch_begin sets the count and the frontier size),
+// so the shared counters left are the push-descriptor count, updated once
+// per block, and the frontier's push-edge count.
 template <bool W, uint32_t CH>
 __device__ __forceinline__ void batch_init_body(uint32_t bid, uint32_t nblk, const uint8_t *used, const uint64_t *off, const uint32_t *adj,
                                                      LabelStore S, unsigned long long *ind, unsigned long long *vmark,
-                                                     uint32_t *tlist, unsigned long long *aoff, unsigned long long *ag,
+                                                     uint32_t *tlist, unsigned long long *abits, unsigned long long *ag,
                                                      uint32_t *amask, uint32_t *aend, unsigned long long *chunks,
                                                      const ulonglong2 *bps, ulonglong2 *rset,
                                                      const unsigned long long *lmask, unsigned long long *rmask,
@@ -523,10 +543,10 @@ __device__ __forceinline__ void batch_init_body(uint32_t bid, uint32_t nblk, con
         vmark[r] = (unsigned long long)gen_batch << 17;
         tlist[ro] = r;
     }
-    // Slot-major copy of the batch roots' bit-parallel sets and level-major
-    // copy of their level masks (push's bit-parallel check).
+    // Slot-major copy of the batch roots' bit-parallel sets and word-major
+    // copy of their mask rows (push's bit-parallel check).
     if (ro < rsz && lane < kBP) rset[lane * kBatch + ro] = bps[(uint64_t)r * kBP + lane];
-    if (ro < rsz && lane < 8) rmask[lane * kBatch + ro] = lmask[(uint64_t)r * 8 + lane];
+    if (ro < rsz && lane < kLmNz0 + 1) rmask[lane * kBatch + ro] = lmask[(uint64_t)r * kLmWords + lane];
     uint32_t pdeg = 0;
     if (ro < rsz && !used[r]) {
         const uint32_t len = S.rec[r].len;
@@ -536,12 +556,16 @@ __device__ __forceinline__ void batch_init_body(uint32_t bid, uint32_t nblk, con
                 S.pool[base + len] = r << Hop<W>::kBits;  // (hub r, dist 0), PARA:932-949
                 S.rec[r].len = len + 1;
                 ind[(uint64_t)r * kWords + (ro >> 6)] |= 1ull << (ro & 63);
-                aoff[ro] = off[r];
-                amask[ro] = 1u << (ro >> 6);
+                amask[r] = 1u << (ro >> 6);
+                // frontier of iteration 1: slot 1 of the bitmap
+                atomicOr(&abits[C->abits_words + (r >> 6)], 1ull << (r & 63));
             }
-            if (lane < kWords) ag[(uint64_t)ro * kWords + lane] = (lane == (ro >> 6)) ? (1ull << (ro & 63)) : 0ull;
+            if (lane < kWords) ag[(uint64_t)r * kWords + lane] = (lane == (ro >> 6)) ? (1ull << (ro & 63)) : 0ull;
             pdeg = push_degree(off, adj, r, rs);
-            if (lane == 0) aend[ro] = pdeg;
+            if (lane == 0) {
+                aend[r] = pdeg;
+                if (pdeg) atomicAdd(&C->n_pedge[1], (unsigned long long)pdeg);
+            }
         }
     }
     const uint32_t nch = (pdeg + CH - 1) / CH;
@@ -557,19 +581,19 @@ __device__ __forceinline__ void batch_init_body(uint32_t bid, uint32_t nblk, con
     if (nch == 0) return;
     uint32_t c0 = s_base;
     for (uint32_t w = 0; w < wv; ++w) c0 += s_nch[w];
-    for (uint32_t q = lane; q < nch; q += 64) chunks[c0 + q] = ((unsigned long long)(q * CH) << 32) | ro;
+    for (uint32_t q = lane; q < nch; q += 64) chunks[c0 + q] = ((unsigned long long)(q * CH) << 32) | r;
 }
 
 template <bool W, uint32_t CH>
 __global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const uint64_t *off, const uint32_t *adj,
                                                      LabelStore S, unsigned long long *ind, unsigned long long *vmark,
-                                                     uint32_t *tlist, unsigned long long *aoff, unsigned long long *ag,
+                                                     uint32_t *tlist, unsigned long long *abits, unsigned long long *ag,
                                                      uint32_t *amask, uint32_t *aend, unsigned long long *chunks,
                                                      const ulonglong2 *bps, ulonglong2 *rset,
                                                      const unsigned long long *lmask, unsigned long long *rmask,
                                                      BuildCounters *C)
 {
-    batch_init_body<W, CH>(blockIdx.x, gridDim.x, used, off, adj, S, ind, vmark, tlist, aoff, ag, amask, aend, chunks, bps, rset, lmask, rmask, C);
+    batch_init_body<W, CH>(blockIdx.x, gridDim.x, used, off, adj, S, ind, vmark, tlist, abits, ag, amask, aend, chunks, bps, rset, lmask, rmask, C);
 }
 
 // Root-distance table (dist_matrix, PARA:986-1013, transposed): row h holds
@@ -625,7 +649,11 @@ __device__ __forceinline__ void hub_table_body(uint32_t bid, uint32_t nblk, uint
             // kernel, so a value read as already there (even a stale one)
             // makes the atomic a no-op.
             if (d < (hi[u].info & INF)) atomicMin(&hinfo[h].info, ~INF | d);
-            if (d >= 1u && d <= kNear && !(hi[u].near[d - 1] & bit)) atomicOr(&hinfo[h].near[d - 1], bit);
+            if (d >= 1u && d <= kNear) {
+                // (no runtime index into hi: it would live in scratch memory)
+                const unsigned long long had = d == 1u ? hi[u].near[0] : (d == 2u ? hi[u].near[1] : hi[u].near[2]);
+                if (!(had & bit)) atomicOr(&hinfo[h].near[d - 1], bit);
+            }
         }
     }
 }
@@ -639,6 +667,171 @@ __global__ __launch_bounds__(256) void k_hub_table(uint32_t fill, const uint8_t 
 
 constexpr uint32_t kPushWaves = 4;  // default waves per k_push block
 
+// Default of pado_gpu_build_cfg::pull_ratio: an iteration runs in pull form
+// when its frontier's push edges exceed this many thousandths of the
+// adjacency entries of the vertices >= roots_start (what the pull form scans).
+constexpr uint32_t kPullRatio = ~0u;
+
+// The pull form of one push step (no reference counterpart: the reference
+// always pushes, PARA:1486-1559).  Heavy iterations offer most tails many
+// roots over many edges; pushing, every (edge, root word) pair is a random
+// test-and-set of the tail's indicator row.  Pulling, each tail t >= rs
+// gathers its own offers: the lanes that own t scan t's adjacency prefix
+// (ids >= rs: exactly the push edges into t, adjacency being symmetric), OR
+// the last-group rows of the neighbours on the frontier bitmap into a 1024-bit
+// offer, and test-and-set t's indicator row once.  A (tail, root) pair passes
+// iff the push form's edges would pass it, and the bit-parallel check, the
+// candidate bits, the touched mark and the lists follow the same rules, so
+// the iteration's outcome (a set) is the same.  Tails go four per wave, 16
+// lanes each (lane 16 g + w: tail g, root word w); the fresh pairs of the four
+// tails are then checked 64 at a time as in the push form, lane o of the
+// wave owning word o & 15 of tail o >> 4.
+template <bool W>
+__device__ __forceinline__ void pull_step(uint32_t wave, uint32_t nwaves, uint32_t iter, uint32_t p, uint32_t rs,
+                                          uint32_t n, unsigned long long gen_iter, uint32_t gen_batch,
+                                          const uint64_t *off, const uint32_t *adj, const unsigned long long *bits,
+                                          const unsigned long long *ag, unsigned long long *ind,
+                                          unsigned long long *cand, unsigned long long *vmark, uint32_t *tlist,
+                                          uint32_t *clist, const uint8_t *bpd, const unsigned long long *lmask,
+                                          const unsigned long long *rmask, const ulonglong2 *bps,
+                                          const ulonglong2 *rset, BuildCounters *C, uint32_t bp_prune,
+                                          unsigned long long *s_nb, unsigned long long *s_keep, uint32_t *s_ex,
+                                          uint32_t *s_t, const SetScratch &sc, uint32_t &n_push, uint32_t &n_pass,
+                                          uint32_t &n_bp, uint32_t &n_cand, uint32_t &n_edge, uint32_t &n_tail)
+{
+    const uint32_t lane = lane_id(), g = lane >> 4, w = lane & 15;
+    const bool use_masks = iter <= kMaskIters;
+    for (uint32_t t0 = rs + 4 * wave; t0 < n; t0 += 4 * nwaves) {
+        const uint32_t t = t0 + g;
+        const bool tv = t < n;
+        uint64_t e = tv ? off[t] : 0, e1 = tv ? off[t + 1] : 0;
+        // roots below t (PARA:1134): offsets < lim; wm = those of word w
+        const uint32_t lim = t - rs;
+        const unsigned long long wm =
+            lim >= (w + 1) * 64 ? ~0ull : (lim <= w * 64 ? 0ull : (1ull << (lim - w * 64)) - 1ull);
+        unsigned long long acc = 0ull;
+        bool more = tv && e < e1;
+        while (__ballot(more) != 0ull) {
+            // sixteen entries of the group's prefix per step
+            const uint64_t j = e + w;
+            uint32_t k = 0;
+            bool inp = false, act = false;
+            if (more && j < e1) {
+                k = adj[j];
+                inp = k >= rs;
+            }
+            if (inp) act = (bits[k >> 6] >> (k & 63)) & 1ull;
+            n_edge += act ? 1u : 0u;
+            uint32_t gm = (uint32_t)(__ballot(act) >> (16 * g)) & 0xFFFFu;
+            // the last-group rows of the group's active neighbours, up to
+            // four loads in flight per lane
+            while (__ballot(gm != 0u) != 0ull) {
+                unsigned long long x[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t sl = gm ? (uint32_t)__ffs(gm) - 1u : 0u;
+                    const uint32_t kk = __shfl(k, (int)(16 * g + sl));
+                    x[u] = 0ull;
+                    if (gm) {
+                        x[u] = ag[(uint64_t)kk * kWords + w] & wm;
+                        gm &= gm - 1u;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    acc |= x[u];
+                    n_push += (uint32_t)__popcll(x[u]);
+                }
+            }
+            // the prefix goes on past this step iff the group's last lane was in it
+            const unsigned long long inm = __ballot(inp);
+            more = more && ((inm >> (16 * g + 15)) & 1ull) && e + 16 < e1;
+            e += 16;
+        }
+        // t's indicator row: one test-and-set per word (its only writer in
+        // this iteration)
+        unsigned long long nb = 0ull;
+        if (acc) {
+            const unsigned long long seen = ind[(uint64_t)t * kWords + w];
+            nb = acc & ~seen;
+            if (nb) ind[(uint64_t)t * kWords + w] = seen | acc;
+        }
+        const uint32_t cnt = (uint32_t)__popcll(nb);
+        n_pass += cnt;
+        const uint32_t incl = wave_incl_scan(cnt);
+        const uint32_t total = __shfl(incl, 63);
+        unsigned long long keep = 0ull;
+        if (total) {
+            s_nb[lane] = nb;
+            s_keep[lane] = 0ull;
+            s_ex[lane] = incl - cnt;
+            s_t[lane] = t;
+            wave_lds_sync();
+            for (uint32_t x0 = 0; x0 < total; x0 += 64) {
+                const uint32_t x = x0 + lane;
+                const bool xa = x < total;
+                uint32_t o = 0, b = 0, ro = 0, to = 0;
+                unsigned long long cs = 0ull, one = 0ull;
+                bool dist_pr = true;
+                if (xa) {
+                    o = item_owner(s_ex, x);
+                    b = nth_set_bit(s_nb[o], x - s_ex[o]);
+                    ro = (o & 15u) * 64 + b;
+                    to = s_t[o];
+                    const unsigned long long *lt = lmask + (uint64_t)to * kLmWords;
+                    if (use_masks) {
+                        dist_pr = bp_mask_scan(iter, rmask, ro, lt, cs, one);
+                    } else {
+                        uint32_t dr[13], dt[13];
+                        load_dline(bpd, rs + ro, dr);
+                        load_dline(bpd, to, dt);
+                        dist_pr = bp_dist_scan<W>(dr, dt, iter, cs, one);
+                    }
+                    if (!bp_prune) {
+                        dist_pr = false;
+                        cs = 0ull;
+                    }
+                    if (!dist_pr && cs)
+                        cs = bp_nz_filter(cs, one, rmask[kLmNz1 * kBatch + ro], rmask[kLmNz0 * kBatch + ro], lt[kLmNz1],
+                                          lt[kLmNz0]);
+                }
+                const bool set_pr = bp_set_phase(dist_pr ? 0ull : cs, one, rset, ro, bps, to, sc);
+                if (xa) {
+                    if (dist_pr || set_pr)
+                        ++n_bp;
+                    else
+                        atomicOr(&s_keep[o], 1ull << b);
+                }
+            }
+            wave_lds_sync();
+            keep = s_keep[lane];
+            // the scratch is rewritten by the next step
+            wave_lds_sync();
+        }
+        if (keep) {
+            cand[(uint64_t)t * kWords + w] = keep;  // the row was empty (k_add clears what it takes)
+            n_cand += (uint32_t)__popcll(keep);
+        }
+        const bool touched = ((__ballot(nb != 0ull) >> (16 * g)) & 0xFFFFull) != 0ull;
+        const bool got = ((__ballot(keep != 0ull) >> (16 * g)) & 0xFFFFull) != 0ull;
+        bool new_t = false, new_c = false;
+        if (w == 0 && touched) {
+            const unsigned long long want = (gen_iter << 1) | (got ? 1ull : 0ull);
+            const unsigned long long prev = vmark[t];
+            if (prev < want) {
+                vmark[t] = want;
+                new_t = (prev >> 17) != gen_batch;
+                if ((prev >> 1) != gen_iter) ++n_tail;
+                new_c = got;
+            }
+        }
+        const uint32_t ti = wave_append(&C->n_touched, new_t);
+        if (new_t) tlist[ti] = t;
+        const uint32_t ci = wave_append(&C->n_cand[p], new_c);
+        if (new_c) clist[ci] = t;
+    }
+}
+
 // push (PARA:1486-1559 / push_labels PARA:1084-1211).  A wave takes 64 / CH
 // work descriptors of CH edges each (one edge per lane; CH = kChunk and
 // PW = kPushWaves waves per block by default, other instantiations exist
@@ -647,17 +840,22 @@ constexpr uint32_t kPushWaves = 4;  // default waves per k_push block
 // test-and-sets its indicator words, and the fresh (tail, root) pairs of all
 // 64 lanes are then checked against the bit-parallel labels 64 at a time,
 // one pair per lane, so a tail with many fresh roots does not hold up the
-// wave.
-template <bool W, uint32_t CH, uint32_t PW>
-__device__ __forceinline__ void push_body(uint32_t bid, uint32_t nblk, uint32_t iter, const uint32_t *adj,
-                                              const unsigned long long *aoff, const unsigned long long *ag,
+// wave.  Descriptors name the active vertex; its last group is row v of ag.
+// An iteration whose frontier has more push edges than pull_ratio / 1000 of
+// the adjacency entries at or above roots_start runs in pull form instead
+// (pull_step; ~0u: never; PL = false compiles the pull form out).
+template <bool W, uint32_t CH, uint32_t PW, bool PL = true>
+__device__ __forceinline__ void push_body(uint32_t bid, uint32_t nblk, uint32_t iter, const uint64_t *off,
+                                              const uint32_t *adj, unsigned long long *abits,
+                                              const unsigned long long *ag,
                                               const uint32_t *amask, const uint32_t *aend,
                                               const unsigned long long *chunks, unsigned long long *ind,
                                               unsigned long long *cand, unsigned long long *vmark, uint32_t *tlist,
                                               uint32_t *clist,
                                               const uint8_t *bpd, const unsigned long long *lmask,
                                               const unsigned long long *rmask, const ulonglong2 *bps,
-                                              const ulonglong2 *rset, BuildCounters *C, uint32_t bp_prune)
+                                              const ulonglong2 *rset, BuildCounters *C, uint32_t bp_prune,
+                                              uint32_t pull_ratio)
 {
     static_assert(CH >= 4 && CH <= 64 && (CH & (CH - 1)) == 0, "chunk: a power of two in [4, 64]");
     constexpr uint32_t DPS = 64 / CH;  // descriptors per wave step
@@ -688,23 +886,39 @@ __device__ __forceinline__ void push_body(uint32_t bid, uint32_t nblk, uint32_t 
     const uint32_t nc = live ? C->n_chunk[p] : 0u;
     const uint32_t rs = C->rs, gen_batch = C->gen_batch;
     const unsigned long long gen_iter = ((unsigned long long)gen_batch << 16) | iter;
+    const uint32_t n = C->n_vert, nw = C->abits_words;
+    // Pull or push: the same decision in every wave (none of its inputs
+    // changes during the launch).
+    const bool pull = PL && live && pull_ratio != ~0u &&
+                      C->n_pedge[p] * 1000ull > (unsigned long long)(off[n] - off[rs]) * pull_ratio;
     if (bid == 0 && threadIdx.x == 0) {
         C->iter_last = iter;
         C->n_act[p ^ 1u] = 0;
         C->n_chunk[p ^ 1u] = 0;
-        C->a_top[p ^ 1u] = 0;
+        C->n_pedge[p ^ 1u] = 0;
         if (n_act0 != 0) {
             if (iter >= Hop<W>::kInf) {
                 C->domain = 1;
             } else {
                 C->iterations += 1;
                 C->actives += n_act0;
+                if (pull) C->pulls += 1;
             }
         }
     }
+    // The other bitmap slot is emptied for k_add to mark the next frontier.
+    if (live) {
+        unsigned long long *nbits = abits + (uint64_t)(p ^ 1u) * nw;
+        for (uint32_t i = bid * blockDim.x + threadIdx.x; i < nw; i += nblk * blockDim.x) nbits[i] = 0ull;
+    }
     // Per-lane statistics of one launch fit 32 bits; summed in 64 at the end.
     uint32_t n_push = 0, n_pass = 0, n_bp = 0, n_cand = 0, n_edge = 0, n_tail = 0;
-    for (uint32_t c0 = wave * DPS; c0 < nc; c0 += nwaves * DPS) {
+    if constexpr (PL)
+        if (pull)
+            pull_step<W>(wave, nwaves, iter, p, rs, n, gen_iter, gen_batch, off, adj, abits + (uint64_t)p * nw, ag, ind,
+                     cand, vmark, tlist, clist, bpd, lmask, rmask, bps, rset, C, bp_prune, s_nb[wv][0], s_keep[wv][0],
+                     s_ex[wv], s_t[wv], sc, n_push, n_pass, n_bp, n_cand, n_edge, n_tail);
+    for (uint32_t c0 = wave * DPS; !pull && c0 < nc; c0 += nwaves * DPS) {
         const uint32_t c = c0 + lane / CH;
         const unsigned long long desc = c < nc ? chunks[c] : kChunkPad;
         uint32_t k = 0, t = 0, lim = 0, words = 0;
@@ -713,7 +927,7 @@ __device__ __forceinline__ void push_body(uint32_t bid, uint32_t nblk, uint32_t 
             const uint32_t e = (uint32_t)(desc >> 32) + (lane & (CH - 1));
             if (e < aend[k]) {
                 ++n_edge;
-                t = adj[aoff[k] + e];
+                t = adj[off[k] + e];
                 lim = t - rs;  // roots with offset < lim satisfy root < tail (PARA:1134)
                 // words w with w * 64 < lim
                 words = amask[k] & (lim >= kBatch ? 0xFFFFu : ((1u << ((lim + 63) / 64)) - 1u));
@@ -722,6 +936,7 @@ __device__ __forceinline__ void push_body(uint32_t bid, uint32_t nblk, uint32_t 
         s_t[wv][lane] = t;
         uint32_t wall = words;
         for (int o = 32; o > 0; o >>= 1) wall |= __shfl_xor(wall, o);
+        wall = uniform32(wall);  // the same in every lane: the step's word choice lives in scalar registers
         bool touched = false, got = false;
         while (wall) {
             // Up to four root words per step (wave-uniform): their indicator
@@ -799,7 +1014,7 @@ __device__ __forceinline__ void push_body(uint32_t bid, uint32_t nblk, uint32_t 
                     w = u == 0 ? wsel[0] : (u == 1 ? wsel[1] : (u == 2 ? wsel[2] : wsel[3]));
                     to = s_t[wv][o];
                     if (use_masks) {
-                        dist_pr = bp_mask_scan(iter, rmask, w * 64 + b, lmask + (uint64_t)to * 8, cs, one);
+                        dist_pr = bp_mask_scan(iter, rmask, w * 64 + b, lmask + (uint64_t)to * kLmWords, cs, one);
                     } else {
                         const uint4 *rl = reinterpret_cast<const uint4 *>(bpd + (uint64_t)(rs + w * 64 + b) * kBpdStride);
                         const uint4 *tl = reinterpret_cast<const uint4 *>(bpd + (uint64_t)to * kBpdStride);
@@ -818,8 +1033,14 @@ __device__ __forceinline__ void push_body(uint32_t bid, uint32_t nblk, uint32_t 
                         dist_pr = false;
                         cs = 0ull;
                     }
+                    if (!dist_pr && cs) {
+                        const uint32_t ro = w * 64 + b;
+                        const unsigned long long *lt = lmask + (uint64_t)to * kLmWords;
+                        cs = bp_nz_filter(cs, one, rmask[kLmNz1 * kBatch + ro], rmask[kLmNz0 * kBatch + ro], lt[kLmNz1],
+                                          lt[kLmNz0]);
+                    }
                 }
-                const bool set_pr = bp_set_phase(dist_pr ? 0ull : cs, one, rset, w * 64 + b, rs, bps, to, sc);
+                const bool set_pr = bp_set_phase(dist_pr ? 0ull : cs, one, rset, w * 64 + b, bps, to, sc);
                 if (act) {
                     if (dist_pr || set_pr)
                         ++n_bp;
@@ -877,17 +1098,20 @@ __device__ __forceinline__ void push_body(uint32_t bid, uint32_t nblk, uint32_t 
 }
 
 template <bool W, uint32_t CH, uint32_t PW>
-__global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(4))) void k_push(uint32_t iter, const uint32_t *adj,
-                                              const unsigned long long *aoff, const unsigned long long *ag,
+__global__ __launch_bounds__(PW * 64) __attribute__((amdgpu_waves_per_eu(4))) void k_push(uint32_t iter, const uint64_t *off,
+                                              const uint32_t *adj, unsigned long long *abits,
+                                              const unsigned long long *ag,
                                               const uint32_t *amask, const uint32_t *aend,
                                               const unsigned long long *chunks, unsigned long long *ind,
                                               unsigned long long *cand, unsigned long long *vmark, uint32_t *tlist,
                                               uint32_t *clist,
                                               const uint8_t *bpd, const unsigned long long *lmask,
                                               const unsigned long long *rmask, const ulonglong2 *bps,
-                                              const ulonglong2 *rset, BuildCounters *C, uint32_t bp_prune)
+                                              const ulonglong2 *rset, BuildCounters *C, uint32_t bp_prune,
+                                              uint32_t pull_ratio)
 {
-    push_body<W, CH, PW>(blockIdx.x, gridDim.x, iter, adj, aoff, ag, amask, aend, chunks, ind, cand, vmark, tlist, clist, bpd, lmask, rmask, bps, rset, C, bp_prune);
+    push_body<W, CH, PW>(blockIdx.x, gridDim.x, iter, off, adj, abits, ag, amask, aend, chunks, ind, cand, vmark, tlist,
+                         clist, bpd, lmask, rmask, bps, rset, C, bp_prune, pull_ratio);
 }
 
 // Batch b's schedule into the counters: roots [rs, rs + rsz), cnt of them
@@ -900,8 +1124,7 @@ __device__ inline void batch_begin_write(BuildCounters *C, uint32_t b, uint32_t 
     C->gen_batch = b + 1;
     C->n_act[0] = 0;
     C->n_act[1] = cnt;
-    C->a_top[0] = 0;
-    C->a_top[1] = rsz;
+    C->n_pedge[0] = C->n_pedge[1] = 0;
     C->n_chunk[0] = C->n_chunk[1] = 0;
     C->n_cand[0] = C->n_cand[1] = 0;
     C->n_touched = rsz;
@@ -942,11 +1165,13 @@ __global__ __launch_bounds__(1024) void k_batch_begin(const uint8_t *used, Build
 // pushes to v) never prunes: iteration 1, whose only labels with dist < 1 are
 // such self labels, scans nothing, and iteration 2 scans v's distance-1 list
 // instead of the whole label array.
+constexpr uint32_t kAddHubs = 2;  // root-distance rows k_add loads per step
+
 template <bool W, uint32_t CH>
 __device__ __forceinline__ void add_body(uint32_t bid, uint32_t nblk, uint32_t iter, const uint8_t *used, const uint64_t *off, const uint32_t *adj, const uint32_t *clist,
                                              unsigned long long *cand, LabelStore S, HubList1 H1, HubInfo *hinfo,
                                              typename Hop<W>::Dist *dc,
-                                             unsigned long long *aoff, unsigned long long *ag, uint32_t *amask,
+                                             unsigned long long *abits, unsigned long long *ag, uint32_t *amask,
                                              uint32_t *aend, unsigned long long *chunks, BuildCounters *C)
 {
     const uint32_t lane = lane_id();
@@ -961,27 +1186,31 @@ __device__ __forceinline__ void add_body(uint32_t bid, uint32_t nblk, uint32_t i
     const uint32_t ncand = iter < INF ? C->n_cand[p] : 0u;  // 0 past the batch's end
     const uint32_t rs = C->rs, rsz = C->rsz;
     if (bid == 0 && threadIdx.x == 0) C->n_cand[p ^ 1u] = 0;
-    unsigned long long n_dq = 0, n_lab = 0, n_rows = 0, n_read = 0;
-    // Next-frontier slots (active entries and push descriptors) are reserved
-    // kAlistGrab / kChunkGrab at a time per wave, so the shared counters see
-    // one atomic per reservation, not per vertex; unused descriptor slots are
-    // padded.  The exact active count is added once per wave at the end.
-    uint32_t a_next = 0, a_left = 0, c_next = 0, c_left = 0, my_act = 0;
+    // Per-lane (n_dq, n_read) and per-wave (n_lab, n_rows) counts of one launch.
+    uint32_t n_dq = 0, n_lab = 0, n_rows = 0, n_read = 0;
+    // Next-frontier push descriptors are reserved up to kChunkGrab at a time
+    // per wave, so the shared counter sees one atomic per reservation, not per
+    // vertex; unused descriptor slots are padded.  The active count and the
+    // frontier's push edges are added once per wave at the end.
+    uint32_t c_next = 0, c_left = 0, my_act = 0;
+    uint32_t my_pedge = 0;  // push edges of this wave's new frontier entries (flushed below 2^31)
+    unsigned long long *nbits = abits + (uint64_t)(p ^ 1u) * C->abits_words;
     // The candidate list is read one vertex ahead, and everything that
     // depends only on the vertex (candidate word, label-store entry, the
     // first 64 labels, the push degree of its next-frontier entry) is loaded
     // before the first store of the step, so those loads overlap.
     uint32_t v_next = wave < ncand ? clist[wave] : 0u;
     for (uint32_t i = wave; i < ncand; i += nwaves) {
-        const uint32_t v = v_next;
+        // Per-vertex values are the same in every lane: kept in scalar registers.
+        const uint32_t v = uniform32(v_next);
         const unsigned long long word = cand[(uint64_t)v * kWords + (lane >> 2)];
         const LRec rec = S.rec[v];
-        const uint32_t len = rec.len;
-        const unsigned long long base = rec.base;
-        const uint32_t cap = rec.cap;
-        const bool v_used = used[v] != 0;
-        const uint64_t e0 = off[v], e1 = off[v + 1];
-        const uint32_t n1 = iter <= 2u ? H1.len[v] : 0u;  // distance-1 labels held
+        const uint32_t len = uniform32(rec.len);
+        const unsigned long long base = uniform64(rec.base);
+        const uint32_t cap = uniform32(rec.cap);
+        const bool v_used = uniform32(used[v]) != 0u;
+        const uint64_t e0 = uniform64(off[v]), e1 = uniform64(off[v + 1]);
+        const uint32_t n1 = iter <= 2u ? uniform32(H1.len[v]) : 0u;  // distance-1 labels held
         if (i + nwaves < ncand) v_next = clist[i + nwaves];
         // The labels the distance query scans (see above); the first 64 of
         // the distance-1 list are read within v's slots before n1 is known.
@@ -1022,28 +1251,28 @@ __device__ __forceinline__ void add_body(uint32_t bid, uint32_t nblk, uint32_t i
             // unpruned candidate has a root close enough to it.
             const unsigned long long live = __ballot((cb & ~prune) != 0u);
             unsigned long long m = __ballot((lm & live) != 0ull);
-            // Up to four passing hubs per step: their row loads are issued
-            // together, and only lanes of the hub's mask that still hold an
-            // unpruned candidate load their slice of a row (16 entries: 16
-            // bytes, 32 wide).
+            // Up to kAddHubs passing hubs per step: their row loads are
+            // issued together, and only lanes of the hub's mask that still
+            // hold an unpruned candidate load their slice of a row (16
+            // entries: 16 bytes, 32 wide).
             while (m) {
-                int src[4];
-                bool ok[4];
+                int src[kAddHubs];
+                bool ok[kAddHubs];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < (int)kAddHubs; ++u) {
                     ok[u] = m != 0ull;
                     src[u] = ok[u] ? __ffsll(m) - 1 : 0;
                     if (ok[u]) m &= m - 1;
                 }
                 const bool need = (cb & ~prune) != 0u;
                 constexpr uint32_t RV = W ? 2 : 1;  // 16-byte vectors of a lane's slice
-                uint32_t hh[4], thr[4];
-                uint4 row[4][RV];
+                uint32_t hh[kAddHubs], thr[kAddHubs];
+                uint4 row[kAddHubs][RV];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < (int)kAddHubs; ++u) {
                     hh[u] = __shfl(h, src[u]);
                     // prune root r if d_r(h) <= iter - d_v(h); an unused slot u
-                    // (fewer than four hubs left) gets threshold 0, which the
+                    // (fewer than kAddHubs hubs left) gets threshold 0, which the
                     // all-INF row never meets
                     const uint32_t du = __shfl(d, src[u]);
                     thr[u] = ok[u] ? iter - du : 0u;
@@ -1059,7 +1288,7 @@ __device__ __forceinline__ void add_body(uint32_t bid, uint32_t nblk, uint32_t i
                     n_rows += ok[u] ? 1 : 0;
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < (int)kAddHubs; ++u) {
                     uint32_t wds[4 * RV];
 #pragma unroll
                     for (uint32_t q = 0; q < RV; ++q) {
@@ -1114,30 +1343,29 @@ __device__ __forceinline__ void add_body(uint32_t bid, uint32_t nblk, uint32_t i
                 if (iter <= kNear) atomicOr(&hinfo[h].near[iter - 1], 1ull << ((v - rs) >> 4));
             }
         }
-        // v becomes active with the new distance group as its last group.
-        // Reservations are capped by the vertices this wave has left in the
-        // launch (grid-stride), so light launches reserve exactly.
+        // v becomes active with the new distance group as its last group
+        // (row v of ag) and, unless it is a bit-parallel vertex (whose
+        // pushes the reference skips, PARA:1117-1119), joins the next
+        // frontier's bitmap.  Descriptor reservations are capped by the
+        // vertices this wave has left in the launch (grid-stride), so light
+        // launches reserve exactly.
         const uint32_t rem = (ncand - 1 - i) / nwaves + 1;
-        if (a_left == 0) {
-            const uint32_t grab = rem < kAlistGrab ? rem : kAlistGrab;
-            uint32_t b0 = 0;
-            if (lane == 0) b0 = atomicAdd(&C->a_top[p ^ 1u], grab);
-            a_next = __shfl(b0, 0);
-            a_left = grab;
-        }
-        const uint32_t k = a_next++;
-        --a_left;
         ++my_act;
-        reinterpret_cast<uint16_t *>(ag + (uint64_t)k * kWords)[lane] = (uint16_t)surv;
+        my_pedge += __builtin_amdgcn_readfirstlane(pdeg);
+        if (my_pedge >= 0x80000000u) {
+            if (lane == 0) atomicAdd(&C->n_pedge[p ^ 1u], (unsigned long long)my_pedge);
+            my_pedge = 0;
+        }
+        reinterpret_cast<uint16_t *>(ag + (uint64_t)v * kWords)[lane] = (uint16_t)surv;
         const unsigned long long nzm = __ballot(surv != 0);
         uint32_t wmask = 0;
 #pragma unroll
         for (int w = 0; w < (int)kWords; ++w)
             if ((nzm >> (4 * w)) & 0xFull) wmask |= 1u << w;
         if (lane == 0) {
-            aoff[k] = e0;
-            amask[k] = wmask;
-            aend[k] = pdeg;
+            amask[v] = wmask;
+            aend[v] = pdeg;
+            if (!v_used) atomicOr(&nbits[v >> 6], 1ull << (v & 63));
         }
         const uint32_t nch = (pdeg + CH - 1) / CH;
         if (nch > c_left) {
@@ -1150,36 +1378,42 @@ __device__ __forceinline__ void add_body(uint32_t bid, uint32_t nblk, uint32_t i
             c_next = __shfl(b0, 0);
             c_left = want;
         }
-        for (uint32_t q = lane; q < nch; q += 64) chunks[c_next + q] = ((unsigned long long)(q * CH) << 32) | k;
+        for (uint32_t q = lane; q < nch; q += 64) chunks[c_next + q] = ((unsigned long long)(q * CH) << 32) | v;
         c_next += nch;
         c_left -= nch;
     }
     for (uint32_t q = lane; q < c_left; q += 64) chunks[c_next + q] = kChunkPad;
     if (lane == 0 && my_act) atomicAdd(&C->n_act[p ^ 1u], my_act);
-    n_dq = wave_sum_u64(n_dq);
-    n_read = wave_sum_u64(n_read);
+    if (lane == 0 && my_pedge) atomicAdd(&C->n_pedge[p ^ 1u], (unsigned long long)my_pedge);
+    const unsigned long long t_dq = wave_sum_u64(n_dq), t_read = wave_sum_u64(n_read);
     if (lane == 0) {
-        if (n_read) atomicAdd(&C->dq_labels, n_read);
-        if (n_dq) atomicAdd(&C->dq_pruned, n_dq);
-        if (n_lab) atomicAdd(&C->labels, n_lab);
-        if (n_rows) atomicAdd(&C->dq_rows, (uint32_t)n_rows);
+        if (t_read) atomicAdd(&C->dq_labels, t_read);
+        if (t_dq) atomicAdd(&C->dq_pruned, t_dq);
+        if (n_lab) atomicAdd(&C->labels, (unsigned long long)n_lab);
+        if (n_rows) atomicAdd(&C->dq_rows, n_rows);
     }
 }
 
 template <bool W, uint32_t CH>
-__global__ __launch_bounds__(256) void k_add(uint32_t iter, const uint8_t *used, const uint64_t *off, const uint32_t *adj, const uint32_t *clist,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_add(uint32_t iter, const uint8_t *used, const uint64_t *off, const uint32_t *adj, const uint32_t *clist,
                                              unsigned long long *cand, LabelStore S, HubList1 H1, HubInfo *hinfo,
                                              typename Hop<W>::Dist *dc,
-                                             unsigned long long *aoff, unsigned long long *ag, uint32_t *amask,
+                                             unsigned long long *abits, unsigned long long *ag, uint32_t *amask,
                                              uint32_t *aend, unsigned long long *chunks, BuildCounters *C)
 {
-    add_body<W, CH>(blockIdx.x, gridDim.x, iter, used, off, adj, clist, cand, S, H1, hinfo, dc, aoff, ag, amask, aend, chunks, C);
+    add_body<W, CH>(blockIdx.x, gridDim.x, iter, used, off, adj, clist, cand, S, H1, hinfo, dc, abits, ag, amask, aend, chunks, C);
 }
 
 // First kernel of a batch's prologue: if the previous batch still has a
 // frontier (its next iteration's count in slot (iter_last + 1) & 1), this
 // batch was enqueued too early and everything up to the host's k_unhold is
 // a no-op.  A no-op before the first batch (all counts zero).
+__global__ void k_ctr_shape(BuildCounters *C, uint32_t n, uint32_t nw)
+{
+    C->n_vert = n;
+    C->abits_words = nw;
+}
+
 __global__ void k_batch_gate(BuildCounters *C)
 {
     if (!C->hold && C->n_act[(C->iter_last + 1u) & 1u] != 0u) C->hold = 1u;
@@ -1189,22 +1423,24 @@ __global__ void k_unhold(BuildCounters *C) { C->hold = 0u; }
 
 // reset_at_end (PARA:1328-1410) and the next batch's indicator reset
 // (PARA:877-901): clear the indicator rows of the touched vertices (the
-// root-distance table is reset by k_hub_table).
+// root-distance table is reset by k_hub_table), and frontier bitmap slot 1,
+// which the next batch's first frontier goes into.
 __device__ __forceinline__ void batch_end_body(uint32_t bid, uint32_t nblk, unsigned long long *ind, const uint32_t *tlist,
-                                                   const BuildCounters *C)
+                                                   unsigned long long *abits, const BuildCounters *C)
 {
     if (C->hold) return;
-    const uint32_t nt = C->n_touched;
+    const uint32_t nt = C->n_touched, nw = C->abits_words;
     const uint64_t stride = (uint64_t)nblk * blockDim.x;
     const uint64_t tot = (uint64_t)nt * kWords;
     for (uint64_t i = (uint64_t)bid * blockDim.x + threadIdx.x; i < tot; i += stride)
         ind[(uint64_t)tlist[i / kWords] * kWords + (i % kWords)] = 0ull;
+    for (uint64_t i = (uint64_t)bid * blockDim.x + threadIdx.x; i < nw; i += stride) abits[nw + i] = 0ull;
 }
 
 __global__ __launch_bounds__(256) void k_batch_end(unsigned long long *ind, const uint32_t *tlist,
-                                                   const BuildCounters *C)
+                                                   unsigned long long *abits, const BuildCounters *C)
 {
-    batch_end_body(blockIdx.x, gridDim.x, ind, tlist, C);
+    batch_end_body(blockIdx.x, gridDim.x, ind, tlist, abits, C);
 }
 
 // One graph's whole batch phase in one workgroup (no reference
@@ -1217,7 +1453,7 @@ __global__ __launch_bounds__(256) void k_batch_end(unsigned long long *ind, cons
 // one workgroup per graph, so a set of such graphs is labeled by one launch.
 template <bool W>
 struct BatchPlan {
-    uint32_t n, batch, bp_prune, pad;
+    uint32_t n, batch, bp_prune, pull_ratio;
     const uint8_t *used;
     const uint64_t *off;
     const uint32_t *adj;
@@ -1225,7 +1461,7 @@ struct BatchPlan {
     HubList1 H1;
     unsigned long long *ind, *cand, *vmark;
     uint32_t *tlist, *clist;
-    unsigned long long *aoff, *ag;
+    unsigned long long *abits, *ag;
     uint32_t *amask, *aend;
     unsigned long long *chunks;
     const uint8_t *bpd;
@@ -1257,7 +1493,7 @@ __global__ __launch_bounds__(kLoopWaves * 64) void k_batch_loop(const BatchPlan<
         const uint32_t prsz = C->rsz;
         for (uint32_t vb = 0; vb * kLoopWaves < prsz; ++vb)
             hub_table_body<W>(vb, 1, 0u, P.used, P.S, P.hinfo, P.dc, C);
-        batch_end_body(0, 1, P.ind, P.tlist, C);
+        batch_end_body(0, 1, P.ind, P.tlist, P.abits, C);
         // Batch b: its non-bit-parallel roots are the first frontier.
         const uint32_t rs = b * P.batch, rsz = min(P.batch, P.n - rs);
         uint32_t c = 0;
@@ -1270,18 +1506,18 @@ __global__ __launch_bounds__(kLoopWaves * 64) void k_batch_loop(const BatchPlan<
         if (threadIdx.x == 0) batch_begin_write(C, b, rs, rsz, s_cnt);
         __syncthreads();
         for (uint32_t vb = 0; vb * kLoopWaves < rsz; ++vb)
-            batch_init_body<W, kChunk>(vb, 1, P.used, P.off, P.adj, P.S, P.ind, P.vmark, P.tlist, P.aoff, P.ag,
+            batch_init_body<W, kChunk>(vb, 1, P.used, P.off, P.adj, P.S, P.ind, P.vmark, P.tlist, P.abits, P.ag,
                                        P.amask, P.aend, P.chunks, P.bps, P.rset, P.lmask, P.rmask, C);
         __syncthreads();
         for (uint32_t vb = 0; vb * kLoopWaves < rsz; ++vb)
             hub_table_body<W>(vb, 1, 1u, P.used, P.S, P.hinfo, P.dc, C);
         __syncthreads();
         for (uint32_t iter = 1;; ++iter) {
-            push_body<W, kChunk, kLoopWaves>(0, 1, iter, P.adj, P.aoff, P.ag, P.amask, P.aend, P.chunks, P.ind,
-                                             P.cand, P.vmark, P.tlist, P.clist, P.bpd, P.lmask, P.rmask, P.bps,
-                                             P.rset, C, P.bp_prune);
+            push_body<W, kChunk, kLoopWaves, false>(0, 1, iter, P.off, P.adj, P.abits, P.ag, P.amask, P.aend, P.chunks,
+                                             P.ind, P.cand, P.vmark, P.tlist, P.clist, P.bpd, P.lmask, P.rmask,
+                                             P.bps, P.rset, C, P.bp_prune, P.pull_ratio);
             __syncthreads();
-            add_body<W, kChunk>(0, 1, iter, P.used, P.off, P.adj, P.clist, P.cand, P.S, P.H1, P.hinfo, P.dc, P.aoff,
+            add_body<W, kChunk>(0, 1, iter, P.used, P.off, P.adj, P.clist, P.cand, P.S, P.H1, P.hinfo, P.dc, P.abits,
                                 P.ag, P.amask, P.aend, P.chunks, C);
             __syncthreads();
             const bool stop = C->overflow != 0u || C->domain != 0u;
@@ -1368,10 +1604,10 @@ static pado_gpu_status pool_repack(hipStream_t st, uint32_t n, DevBuf<uint32_t> 
 template <bool W>
 struct BatchBuffers {
     DevBuf<unsigned long long> ind, cand, ag, chunks;
-    DevBuf<unsigned long long> aoff;   // per active slot: the vertex's first edge
+    DevBuf<unsigned long long> abits;  // frontier bitmaps, two slots of ceil(n / 64) words
     DevBuf<unsigned long long> vmark;  // ((batch << 16 | iteration) << 1 | candidate) of the last mark
-    DevBuf<unsigned long long> lmask;  // bit-parallel level masks, 8 words per vertex
-    DevBuf<unsigned long long> rmask;  // batch roots' level masks, level-major [8][kBatch]
+    DevBuf<unsigned long long> lmask;  // bit-parallel mask rows, kLmWords words per vertex
+    DevBuf<unsigned long long> rmask;  // batch roots' mask words 0..kLmNz0, word-major [10][kBatch]
     DevBuf<uint32_t> tlist, clist, amask, aend, pool;
     DevBuf<HubInfo> hinfo;
     DevBuf<uint32_t> d1, d1len;  // distance-1 lists (HubList1): m slots, n counts
@@ -1383,7 +1619,7 @@ struct BatchBuffers {
     uint32_t dc_rows = 0;
     void release()
     {
-        for (auto *b : {&ind, &cand, &ag, &chunks, &aoff, &vmark, &lmask, &rmask}) b->release();
+        for (auto *b : {&ind, &cand, &ag, &chunks, &abits, &vmark, &lmask, &rmask}) b->release();
         for (auto *b : {&tlist, &clist, &amask, &aend, &pool, &d1, &d1len}) b->release();
         hinfo.release();
         lrec.release();
@@ -1394,10 +1630,10 @@ struct BatchBuffers {
 };
 
 // Allocates and initialises them on the graph's stream.  add_waves bounds
-// the holes of k_add's per-wave reservations: the active list can exceed the
-// active count by kAlistGrab slots per wave, and padded descriptors are
-// fewer than the real ones plus kChunkGrab per wave.  The pool holds
-// pool_size label words, the root-distance table dc_rows 1 KB rows (255).
+// the padding of k_add's per-wave descriptor reservations: padded
+// descriptors are fewer than the real ones plus kChunkGrab per wave.  The
+// pool holds pool_size label words, the root-distance table dc_rows 1 KB
+// rows (255).
 template <bool W>
 static pado_gpu_status batch_alloc(const pado_gpu_graph *g, const pado_gpu_index *idx, uint64_t add_waves,
                                    uint32_t chunk, unsigned long long pool_size, uint32_t dc_rows,
@@ -1406,24 +1642,24 @@ static pado_gpu_status batch_alloc(const pado_gpu_graph *g, const pado_gpu_index
     using DT = typename Hop<W>::Dist;
     hipStream_t st = g->ctx->stream;
     const uint32_t n = g->n;
-    const uint64_t alist_cap = (uint64_t)n + kAlistGrab * add_waves;
+    const uint32_t nw = (n + 63) / 64;  // words of one frontier bitmap slot
     PADO_TRY(B.vmark.alloc(n));
     PADO_TRY(B.ind.alloc((uint64_t)n * kWords));
     PADO_TRY(B.cand.alloc((uint64_t)n * kWords));
-    PADO_TRY(B.ag.alloc(alist_cap * kWords));
+    PADO_TRY(B.ag.alloc((uint64_t)n * kWords));
+    PADO_TRY(B.abits.alloc(2ull * nw));
     PADO_TRY(B.chunks.alloc(2 * (g->m / chunk + (uint64_t)n) + kChunkGrab * add_waves + 64));
     PADO_TRY(B.tlist.alloc(n));
     PADO_TRY(B.clist.alloc(n));
-    PADO_TRY(B.aoff.alloc(alist_cap));
-    PADO_TRY(B.amask.alloc(alist_cap));
-    PADO_TRY(B.aend.alloc(alist_cap));
+    PADO_TRY(B.amask.alloc(n));
+    PADO_TRY(B.aend.alloc(n));
     PADO_TRY(B.hinfo.alloc(n));
     PADO_TRY(B.d1.alloc(g->m ? g->m : 1));
     PADO_TRY(B.d1len.alloc(n));
     PADO_TRY(B.lrec.alloc(n));
     PADO_TRY(B.rset.alloc((uint64_t)kBP * kBatch));
-    PADO_TRY(B.lmask.alloc((uint64_t)n * 8));
-    PADO_TRY(B.rmask.alloc(8ull * kBatch));
+    PADO_TRY(B.lmask.alloc((uint64_t)n * kLmWords));
+    PADO_TRY(B.rmask.alloc((kLmNz0 + 1ull) * kBatch));
     PADO_TRY(B.ctr.alloc(1));
     PADO_TRY(B.pool.alloc(pool_size));
     PADO_TRY(B.dc.alloc((uint64_t)dc_rows * kBatch));
@@ -1435,13 +1671,16 @@ static pado_gpu_status batch_alloc(const pado_gpu_graph *g, const pado_gpu_index
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, B.ind.p, (uint64_t)n * kWords);
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, B.cand.p, (uint64_t)n * kWords);
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, B.vmark.p, (uint64_t)n);
-    hipLaunchKernelGGL(k_bp_masks, dim3(fg), dim3(256), 0, st, n, idx->bpd.p, B.lmask.p);
+    hipLaunchKernelGGL(k_bp_masks, dim3(fg), dim3(256), 0, st, n, idx->bpd.p, (const ulonglong2 *)idx->bps.p,
+                       B.lmask.p);
     hipLaunchKernelGGL(k_hub_info_init, dim3(fg), dim3(256), 0, st, B.hinfo.p, (uint64_t)n);
     hipLaunchKernelGGL(k_fill_u32, dim3(fg), dim3(256), 0, st, B.d1len.p, (uint64_t)n, 0u);
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, reinterpret_cast<unsigned long long *>(B.lrec.p),
                        2ull * n);
+    hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, B.abits.p, 2ull * nw);
     hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, reinterpret_cast<uint32_t *>(B.ctr.p),
                        (uint64_t)(sizeof(BuildCounters) / 4), 0u);
+    hipLaunchKernelGGL(k_ctr_shape, dim3(1), dim3(1), 0, st, B.ctr.p, n, nw);
     PADO_HIP(hipGetLastError());
     return PADO_GPU_OK;
 }
@@ -1455,7 +1694,7 @@ static pado_gpu_status batch_finish(pado_gpu_graph *g, pado_gpu_index *idx, cons
 {
     hipStream_t st = g->ctx->stream;
     const uint32_t n = g->n;
-    hipLaunchKernelGGL(k_batch_end, dim3(1024), dim3(256), 0, st, B.ind.p, B.tlist.p, B.ctr.p);
+    hipLaunchKernelGGL(k_batch_end, dim3(1024), dim3(256), 0, st, B.ind.p, B.tlist.p, B.abits.p, B.ctr.p);
     BuildCounters c{};
     PADO_HIP(copy_sync(&c, B.ctr.p, sizeof(BuildCounters), hipMemcpyDeviceToHost, st));
     idx->counters = c;
@@ -1544,7 +1783,7 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
     PADO_TRY(batch_alloc<W>(g, idx, add_waves, chunk,
                             std::max<unsigned long long>(std::max<unsigned long long>(1ull << 22, 64ull * n), pool_min),
                             std::min<uint32_t>(n, 1u << 16), B));
-    auto &ind = B.ind, &cand = B.cand, &ag = B.ag, &chunks = B.chunks, &aoff = B.aoff, &vmark = B.vmark;
+    auto &ind = B.ind, &cand = B.cand, &ag = B.ag, &chunks = B.chunks, &abits = B.abits, &vmark = B.vmark;
     auto &lmask = B.lmask, &rmask = B.rmask;
     auto &tlist = B.tlist, &clist = B.clist, &amask = B.amask, &aend = B.aend, &pool = B.pool;
     auto &hinfo = B.hinfo;
@@ -1572,6 +1811,7 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
     const uint32_t push_grid = cfg.push_blocks ? cfg.push_blocks : default_grid(n);
     const uint32_t add_grid = cfg.add_blocks ? cfg.add_blocks : default_grid(n);
     const uint32_t root_blocks = (batch * 64 + 255) / 256;
+    const uint32_t pull_ratio = cfg.pull_ratio ? cfg.pull_ratio : kPullRatio;
 
     // Stream mode brackets every push and add launch with HIP events (read
     // after the host check that follows each block of U iterations).
@@ -1595,10 +1835,10 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
         // Previous batch's reset_at_end first (a no-op before the first batch).
         hipLaunchKernelGGL(k_batch_gate, dim3(1), dim3(1), 0, st, ctr.p);
         hipLaunchKernelGGL(k_hub_table<W>, dim3(root_blocks), dim3(256), 0, st, 0u, d_used.p, S, hinfo.p, dc.p, ctr.p);
-        hipLaunchKernelGGL(k_batch_end, dim3(1024), dim3(256), 0, st, ind.p, tlist.p, ctr.p);
+        hipLaunchKernelGGL(k_batch_end, dim3(1024), dim3(256), 0, st, ind.p, tlist.p, abits.p, ctr.p);
         hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(batch), 0, st, d_used.p, ctr.p, batch, n);
         hipLaunchKernelGGL(K.init, dim3((batch + 15) / 16), dim3(1024), 0, st, d_used.p, g->off.p, g->adj.p, S, ind.p,
-                           vmark.p, tlist.p, aoff.p, ag.p, amask.p, aend.p, chunks.p,
+                           vmark.p, tlist.p, abits.p, ag.p, amask.p, aend.p, chunks.p,
                            (const ulonglong2 *)idx->bps.p, rset.p, lmask.p, rmask.p, ctr.p);
         hipLaunchKernelGGL(k_hub_table<W>, dim3(root_blocks), dim3(256), 0, st, 1u, d_used.p, S, hinfo.p, dc.p, ctr.p);
     };
@@ -1607,14 +1847,14 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
         for (uint32_t u = 0; u < count; ++u) {
             const uint32_t iter = first + 1 + u;
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u], st));
-            hipLaunchKernelGGL(K.push, dim3(push_grid), dim3(64 * K.waves), 0, st, iter, g->adj.p, aoff.p, ag.p,
-                               amask.p, aend.p, chunks.p, ind.p, cand.p, vmark.p, tlist.p, clist.p,
+            hipLaunchKernelGGL(K.push, dim3(push_grid), dim3(64 * K.waves), 0, st, iter, g->off.p, g->adj.p, abits.p,
+                               ag.p, amask.p, aend.p, chunks.p, ind.p, cand.p, vmark.p, tlist.p, clist.p,
                                idx->bpd.p, lmask.p, rmask.p, (const ulonglong2 *)idx->bps.p, rset.p, ctr.p,
-                               idx->bp_pass_through ? 0u : 1u);
+                               idx->bp_pass_through ? 0u : 1u, pull_ratio);
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 1], st));
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 2], st));
             hipLaunchKernelGGL(K.add, dim3(add_grid), dim3(256), 0, st, iter, d_used.p, g->off.p, g->adj.p, clist.p,
-                               cand.p, S, H1, hinfo.p, dc.p, aoff.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
+                               cand.p, S, H1, hinfo.p, dc.p, abits.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 3], st));
         }
         return PADO_GPU_OK;
@@ -1860,7 +2100,8 @@ static BatchPlan<W> batch_plan(const pado_gpu_graph *g, const pado_gpu_index *id
     P.vmark = B.vmark.p;
     P.tlist = B.tlist.p;
     P.clist = B.clist.p;
-    P.aoff = B.aoff.p;
+    P.abits = B.abits.p;
+    P.pull_ratio = ~0u;  // the one-workgroup loop always pushes (its push_body has no pull form)
     P.ag = B.ag.p;
     P.amask = B.amask.p;
     P.aend = B.aend.p;

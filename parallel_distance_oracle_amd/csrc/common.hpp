@@ -114,6 +114,18 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
+// The first active lane's value, in scalar registers: for values that are the
+// same in every lane (loaded from one address, or broadcast), so that the
+// compiler treats them as wave-uniform.
+__device__ inline uint64_t uniform64(uint64_t x)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ inline uint32_t uniform32(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
 // Append one item per lane with `pred` to a list counted by *counter: the
 // active lanes of the wave take consecutive slots with a single atomic, so a
 // shared counter sees one update per wave instead of one per item.  Returns
@@ -136,7 +148,8 @@ struct BuildCounters {
     // each kernel reads its own slot and empties the other one for the next
     // kernel to fill, so no control launch runs between iterations.
     uint32_t n_act[2];     // active vertices of the iteration
-    uint32_t a_top[2];     // active-list slots handed out (>= n_act: per-wave reservations)
+    uint32_t n_vert;       // vertices of the graph
+    uint32_t abits_words;  // 64-bit words of one frontier bitmap slot
     uint32_t n_chunk[2];   // push work descriptors of the iteration
     uint32_t n_cand[2];    // candidate vertices of the iteration
     uint32_t n_touched;    // vertices whose indicator row was touched this batch
@@ -147,6 +160,8 @@ struct BuildCounters {
     uint32_t overflow;     // label pool exhausted
     uint32_t dq_rows;      // root-distance rows scanned (statistic)
     unsigned long long pool_top;  // next free label slot in the pool
+    unsigned long long n_pedge[2];  // push edges of the iteration's frontier (sum of push degrees)
+    unsigned long long pulls;     // iterations run in pull form (statistic)
     unsigned long long pushes;    // (tail, root) pairs offered by push
     unsigned long long ind_pass;  // pairs that set a fresh indicator bit
     unsigned long long bp_pruned; // pairs pruned by the bit-parallel check

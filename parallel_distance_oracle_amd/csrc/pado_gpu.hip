@@ -410,6 +410,7 @@ pado_gpu_status pado_gpu_index_get_info(const pado_gpu_index *idx, pado_gpu_inde
     info->n_add = idx->ktimes.n_add;
     info->n_bp_levels = idx->ktimes.n_bp;
     info->dist_bits = idx->wide ? Hop<true>::kBits : Hop<false>::kBits;
+    info->pull_iters = (uint32_t)idx->counters.pulls;
     return PADO_GPU_OK;
 }
 

@@ -451,14 +451,7 @@ __device__ inline void wait_memory() { __builtin_amdgcn_s_waitcnt(0); }
 // descriptor as uniform, or it structures the loops for lanes that could
 // part ways -- and a lane set that re-enters the wait without lane 0 would
 // broadcast a stale ticket.
-__device__ inline uint64_t uniform64(uint64_t x)
-{
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)x);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(x >> 32));
-    return ((uint64_t)hi << 32) | lo;
-}
-
-__device__ inline uint32_t uniform32(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+// (uniform64 / uniform32: common.hpp)
 
 __device__ void serve_poll(const ServeKernelArgs &A)
 {

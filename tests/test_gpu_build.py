@@ -177,3 +177,44 @@ def test_label_sort_in_vertex_ranges(gpu_ctx):
     assert np.array_equal(loff, arr["loff"]) and np.array_equal(hub, arr["hub"]) and np.array_equal(dist, arr["dist"])
     idx.close()
     dg.close()
+
+
+@pytest.mark.parametrize("case", ["edgelist_in", "rmat12", "rmat14"])
+@pytest.mark.parametrize("graph_mode", [False, True])
+def test_pull_form_same_labels_and_counters(gpu_ctx, case, graph_mode):
+    """The pull form of the push step (each tail gathers its offers) against
+    the push form: pull_ratio 1 runs nearly every iteration pulled,
+    0xFFFFFFFF none; labels equal the reference's and every counter -- the
+    edge and tail counts included -- is the same either way."""
+    if case.startswith("rmat"):
+        s = int(case[4:])
+        meta, arr = load_rmat(s, 8)
+        G, _, _ = rank_space(rmat_edges(s, 8))
+        ref = (arr["loff"], arr["hub"], arr["dist"])
+    else:
+        g = load_tiny(case)
+        G, _, _ = rank_space(g["edges"], g["rank"])
+        ref = (g["loff"], g["hub"], g["dist"])
+    dg = pado.DeviceGraph(gpu_ctx, G)
+    stats = {}
+    for ratio in (1, 0xFFFFFFFF):
+        cfg = pado.BuildCfg()
+        cfg.flags = 0 if graph_mode else pado.BUILD_STREAM
+        cfg.pull_ratio = ratio
+        print(f"[gpu] pull form {case} graph={graph_mode} pull_ratio={ratio}", flush=True)
+        idx = pado.Index.build(gpu_ctx, dg, cfg=cfg)
+        _, _, loff, hub, dist = idx.export()
+        assert np.array_equal(loff, ref[0]) and np.array_equal(hub, ref[1]) and np.array_equal(dist, ref[2])
+        stats[ratio] = idx.stats()
+        idx.close()
+    dg.close()
+    pulled, pushed = stats[1], stats[0xFFFFFFFF]
+    assert pushed["pull_iters"] == 0
+    if case.startswith("rmat"):  # (the tiny graph is covered by its bit-parallel labels)
+        assert pulled["pull_iters"] > 0
+    for k in ("iterations", "pushes", "ind_pass", "bp_pruned", "cands", "dq_pruned", "edges", "tails", "actives",
+              "labels_appended"):
+        assert pulled[k] == pushed[k], k
+    X = oracle.Index(G.num_v, G.off, G.adj)
+    for k in ("pushes", "ind_pass", "bp_pruned", "cands", "dq_pruned"):
+        assert pulled[k] == X.stats[k], k
