@@ -515,109 +515,18 @@ __global__ __launch_bounds__(256) void k_bp_masks(uint32_t n, const uint8_t *bpd
     }
 }
 
-// initialize (PARA:848-1080): non-BP roots become active with their own bit as
-// last group, get the self label (r, 0), and set their own indicator bit.
-// One wave per root, sixteen roots per block.  Root offset ro owns
-// touched-list slot ro (k_batch_begin sets the count and the frontier size),
-// so the shared counters left are the push-descriptor count, updated once
-// per block, and the frontier's push-edge count.
-template <bool W, uint32_t CH>
-__device__ __forceinline__ void batch_init_body(uint32_t bid, uint32_t nblk, const uint8_t *used, const uint64_t *off, const uint32_t *adj,
-                                                     LabelStore S, unsigned long long *ind, unsigned long long *vmark,
-                                                     uint32_t *tlist, unsigned long long *abits, unsigned long long *ag,
-                                                     uint32_t *amask, uint32_t *aend, unsigned long long *chunks,
-                                                     const ulonglong2 *bps, ulonglong2 *rset,
-                                                     const unsigned long long *lmask, unsigned long long *rmask,
-                                                     BuildCounters *C)
-{
-    __shared__ uint32_t s_nch[16];
-    __shared__ uint32_t s_base;
-    if (C->hold) return;
-    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
-    const uint32_t ro = (bid * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t rs = C->rs, rsz = C->rsz, gen_batch = C->gen_batch;
-    const uint32_t r = rs + ro;
-    // Every root of the batch (bit-parallel roots included) is on the
-    // touched list; a row that stays empty is cleared at no harm.
-    if (ro < rsz && lane == 0) {
-        vmark[r] = (unsigned long long)gen_batch << 17;
-        tlist[ro] = r;
-    }
-    // Slot-major copy of the batch roots' bit-parallel sets and word-major
-    // copy of their mask rows (push's bit-parallel check).
-    if (ro < rsz && lane < kBP) rset[lane * kBatch + ro] = bps[(uint64_t)r * kBP + lane];
-    if (ro < rsz && lane < kLmNz0 + 1) rmask[lane * kBatch + ro] = lmask[(uint64_t)r * kLmWords + lane];
-    uint32_t pdeg = 0;
-    if (ro < rsz && !used[r]) {
-        const uint32_t len = S.rec[r].len;
-        const unsigned long long base = store_reserve(S, C, r, len, 1);
-        if (base != ~0ull) {
-            if (lane == 0) {
-                S.pool[base + len] = r << Hop<W>::kBits;  // (hub r, dist 0), PARA:932-949
-                S.rec[r].len = len + 1;
-                ind[(uint64_t)r * kWords + (ro >> 6)] |= 1ull << (ro & 63);
-                amask[r] = 1u << (ro >> 6);
-                // frontier of iteration 1: slot 1 of the bitmap
-                atomicOr(&abits[C->abits_words + (r >> 6)], 1ull << (r & 63));
-            }
-            if (lane < kWords) ag[(uint64_t)r * kWords + lane] = (lane == (ro >> 6)) ? (1ull << (ro & 63)) : 0ull;
-            pdeg = push_degree(off, adj, r, rs);
-            if (lane == 0) {
-                aend[r] = pdeg;
-                if (pdeg) atomicAdd(&C->n_pedge[1], (unsigned long long)pdeg);
-            }
-        }
-    }
-    const uint32_t nch = (pdeg + CH - 1) / CH;
-    if (lane == 0) s_nch[wv] = nch;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t tot = 0;
-        for (uint32_t w = 0; w < blockDim.x / 64; ++w) tot += s_nch[w];
-        // The frontier of iteration i is counted in slot i & 1: iteration 1 -> 1.
-        s_base = tot ? atomicAdd(&C->n_chunk[1], tot) : 0u;
-    }
-    __syncthreads();
-    if (nch == 0) return;
-    uint32_t c0 = s_base;
-    for (uint32_t w = 0; w < wv; ++w) c0 += s_nch[w];
-    for (uint32_t q = lane; q < nch; q += 64) chunks[c0 + q] = ((unsigned long long)(q * CH) << 32) | r;
-}
-
-template <bool W, uint32_t CH>
-__global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const uint64_t *off, const uint32_t *adj,
-                                                     LabelStore S, unsigned long long *ind, unsigned long long *vmark,
-                                                     uint32_t *tlist, unsigned long long *abits, unsigned long long *ag,
-                                                     uint32_t *amask, uint32_t *aend, unsigned long long *chunks,
-                                                     const ulonglong2 *bps, ulonglong2 *rset,
-                                                     const unsigned long long *lmask, unsigned long long *rmask,
-                                                     BuildCounters *C)
-{
-    batch_init_body<W, CH>(blockIdx.x, gridDim.x, used, off, adj, S, ind, vmark, tlist, abits, ag, amask, aend, chunks, bps, rset, lmask, rmask, C);
-}
-
 // Root-distance table (dist_matrix, PARA:986-1013, transposed): row h holds
 // dist(root r, hub h) at column r - rs for the hubs of the batch roots'
-// labels, 255 elsewhere; hinfo[h] summarises the row (HubInfo).  One wave per
-// root; each lane handles four labels per step so the dependent loads of a
-// step overlap.  With `fill` the root's labels are entered; without, the
-// entries of the previous batch's roots (the labels they held at its end,
-// which include every entry written during it) are reset.
+// labels, 255 elsewhere; hinfo[h] summarises the row (HubInfo).  Root
+// offset ro's labels pool[base, base + len) are entered (fill) or their
+// entries reset, by one wave; each lane handles four labels per step so the
+// dependent loads of a step overlap.
 template <bool W>
-__device__ __forceinline__ void hub_table_body(uint32_t bid, uint32_t nblk, uint32_t fill, const uint8_t *used, LabelStore S, HubInfo *hinfo,
-                                                   typename Hop<W>::Dist *dc, const BuildCounters *C)
+__device__ __forceinline__ void hub_rows(uint32_t fill, uint32_t ro, unsigned long long base, uint32_t len, LabelStore S,
+                                         HubInfo *hinfo, typename Hop<W>::Dist *dc)
 {
     constexpr uint32_t B = Hop<W>::kBits, INF = Hop<W>::kInf;
-    if (C->hold) return;
     const uint32_t lane = lane_id();
-    const uint32_t ro = (bid * blockDim.x + threadIdx.x) >> 6;
-    const uint32_t rs = C->rs, rsz = C->rsz;
-    if (ro >= rsz) return;
-    const uint32_t r = rs + ro;
-    if (used[r]) return;
-    const LRec rec = S.rec[r];
-    const uint32_t len = rec.len;
-    const unsigned long long base = rec.base;
     for (uint32_t j0 = 0; j0 < len; j0 += 256) {
         uint32_t lab[4];
 #pragma unroll
@@ -658,12 +567,126 @@ __device__ __forceinline__ void hub_table_body(uint32_t bid, uint32_t nblk, uint
     }
 }
 
+// initialize (PARA:848-1080): non-BP roots become active with their own bit as
+// last group, get the self label (r, 0), and set their own indicator bit;
+// then their labels go into the root-distance table (PARA:986-1013).  One
+// wave per root, sixteen roots per block.  Root offset ro owns touched-list
+// slot ro (k_batch_begin sets the count and the frontier size), so the shared
+// counters left are the push-descriptor count, updated once per block, and
+// the frontier's push-edge count.
+template <bool W, uint32_t CH>
+__device__ __forceinline__ void batch_init_body(uint32_t bid, uint32_t nblk, const uint8_t *used, const uint64_t *off, const uint32_t *adj,
+                                                     LabelStore S, HubInfo *hinfo, typename Hop<W>::Dist *dc,
+                                                     unsigned long long *ind, unsigned long long *vmark,
+                                                     uint32_t *tlist, unsigned long long *abits, unsigned long long *ag,
+                                                     uint32_t *amask, uint32_t *aend, unsigned long long *chunks,
+                                                     const ulonglong2 *bps, ulonglong2 *rset,
+                                                     const unsigned long long *lmask, unsigned long long *rmask,
+                                                     BuildCounters *C)
+{
+    __shared__ uint32_t s_nch[16];
+    __shared__ uint32_t s_base;
+    if (C->hold) return;
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+    const uint32_t ro = (bid * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t rs = C->rs, rsz = C->rsz, gen_batch = C->gen_batch;
+    const uint32_t r = rs + ro;
+    // Every root of the batch (bit-parallel roots included) is on the
+    // touched list; a row that stays empty is cleared at no harm.
+    if (ro < rsz && lane == 0) {
+        vmark[r] = (unsigned long long)gen_batch << 17;
+        tlist[ro] = r;
+    }
+    // Slot-major copy of the batch roots' bit-parallel sets and word-major
+    // copy of their mask rows (push's bit-parallel check).
+    if (ro < rsz && lane < kBP) rset[lane * kBatch + ro] = bps[(uint64_t)r * kBP + lane];
+    if (ro < rsz && lane < kLmNz0 + 1) rmask[lane * kBatch + ro] = lmask[(uint64_t)r * kLmWords + lane];
+    uint32_t pdeg = 0;
+    // the root's labels before its self label (their array, even if it moves)
+    uint32_t len0 = 0;
+    unsigned long long base0 = 0;
+    bool table = false;
+    if (ro < rsz && !used[r]) {
+        const LRec rec0 = S.rec[r];
+        len0 = rec0.len;
+        base0 = rec0.base;
+        const uint32_t len = len0;
+        const unsigned long long base = store_reserve_at(S, C, r, len, rec0.base, rec0.cap, 1);
+        if (base != ~0ull) {
+            table = true;
+            if (lane == 0) {
+                S.pool[base + len] = r << Hop<W>::kBits;  // (hub r, dist 0), PARA:932-949
+                S.rec[r].len = len + 1;
+                ind[(uint64_t)r * kWords + (ro >> 6)] |= 1ull << (ro & 63);
+                amask[r] = 1u << (ro >> 6);
+                // frontier of iteration 1: slot 1 of the bitmap
+                atomicOr(&abits[C->abits_words + (r >> 6)], 1ull << (r & 63));
+            }
+            if (lane < kWords) ag[(uint64_t)r * kWords + lane] = (lane == (ro >> 6)) ? (1ull << (ro & 63)) : 0ull;
+            pdeg = push_degree(off, adj, r, rs);
+            if (lane == 0) {
+                aend[r] = pdeg;
+                if (pdeg) atomicAdd(&C->n_pedge[1], (unsigned long long)pdeg);
+            }
+        }
+    }
+    const uint32_t nch = (pdeg + CH - 1) / CH;
+    if (lane == 0) s_nch[wv] = nch;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tot = 0;
+        for (uint32_t w = 0; w < blockDim.x / 64; ++w) tot += s_nch[w];
+        // The frontier of iteration i is counted in slot i & 1: iteration 1 -> 1.
+        s_base = tot ? atomicAdd(&C->n_chunk[1], tot) : 0u;
+    }
+    __syncthreads();
+    if (nch) {
+        uint32_t c0 = s_base;
+        for (uint32_t w = 0; w < wv; ++w) c0 += s_nch[w];
+        for (uint32_t q = lane; q < nch; q += 64) chunks[c0 + q] = ((unsigned long long)(q * CH) << 32) | r;
+    }
+    if (table) {
+        // root-distance entries: the earlier labels, then the self label
+        hub_rows<W>(1u, ro, base0, len0, S, hinfo, dc);
+        if (lane == 0) {
+            dc[(uint64_t)r * kBatch + ro] = 0;
+            atomicMin(&hinfo[r].info, ~Hop<W>::kInf);
+        }
+    }
+}
+
+template <bool W, uint32_t CH>
+__global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const uint64_t *off, const uint32_t *adj,
+                                                     LabelStore S, HubInfo *hinfo, typename Hop<W>::Dist *dc,
+                                                     unsigned long long *ind, unsigned long long *vmark,
+                                                     uint32_t *tlist, unsigned long long *abits, unsigned long long *ag,
+                                                     uint32_t *amask, uint32_t *aend, unsigned long long *chunks,
+                                                     const ulonglong2 *bps, ulonglong2 *rset,
+                                                     const unsigned long long *lmask, unsigned long long *rmask,
+                                                     BuildCounters *C)
+{
+    batch_init_body<W, CH>(blockIdx.x, gridDim.x, used, off, adj, S, hinfo, dc, ind, vmark, tlist, abits, ag, amask, aend,
+                           chunks, bps, rset, lmask, rmask, C);
+}
+
+// One root's entries of the root-distance table from the batch roots'
+// offsets (one wave per root): with `fill` its labels are entered, without,
+// the entries of the previous batch's roots (the labels they held at its end,
+// which include every entry written during it) are reset.
 template <bool W>
-__global__ __launch_bounds__(256) void k_hub_table(uint32_t fill, const uint8_t *used, LabelStore S, HubInfo *hinfo,
+__device__ __forceinline__ void hub_table_body(uint32_t bid, uint32_t nblk, uint32_t fill, const uint8_t *used, LabelStore S, HubInfo *hinfo,
                                                    typename Hop<W>::Dist *dc, const BuildCounters *C)
 {
-    hub_table_body<W>(blockIdx.x, gridDim.x, fill, used, S, hinfo, dc, C);
+    if (C->hold) return;
+    const uint32_t ro = (bid * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t rs = C->rs, rsz = C->rsz;
+    if (ro >= rsz) return;
+    const uint32_t r = rs + ro;
+    if (used[r]) return;
+    const LRec rec = S.rec[r];
+    hub_rows<W>(fill, ro, rec.base, rec.len, S, hinfo, dc);
 }
+
 
 constexpr uint32_t kPushWaves = 4;  // default waves per k_push block
 
@@ -1404,26 +1427,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     add_body<W, CH>(blockIdx.x, gridDim.x, iter, used, off, adj, clist, cand, S, H1, hinfo, dc, abits, ag, amask, aend, chunks, C);
 }
 
-// First kernel of a batch's prologue: if the previous batch still has a
-// frontier (its next iteration's count in slot (iter_last + 1) & 1), this
-// batch was enqueued too early and everything up to the host's k_unhold is
-// a no-op.  A no-op before the first batch (all counts zero).
+// The graph's shape in the counters (once, at allocation).
 __global__ void k_ctr_shape(BuildCounters *C, uint32_t n, uint32_t nw)
 {
     C->n_vert = n;
     C->abits_words = nw;
 }
 
-__global__ void k_batch_gate(BuildCounters *C)
-{
-    if (!C->hold && C->n_act[(C->iter_last + 1u) & 1u] != 0u) C->hold = 1u;
-}
-
 __global__ void k_unhold(BuildCounters *C) { C->hold = 0u; }
 
 // reset_at_end (PARA:1328-1410) and the next batch's indicator reset
 // (PARA:877-901): clear the indicator rows of the touched vertices (the
-// root-distance table is reset by k_hub_table), and frontier bitmap slot 1,
+// root-distance table is reset by hub_table_body), and frontier bitmap slot 1,
 // which the next batch's first frontier goes into.
 __device__ __forceinline__ void batch_end_body(uint32_t bid, uint32_t nblk, unsigned long long *ind, const uint32_t *tlist,
                                                    unsigned long long *abits, const BuildCounters *C)
@@ -1440,6 +1455,27 @@ __device__ __forceinline__ void batch_end_body(uint32_t bid, uint32_t nblk, unsi
 __global__ __launch_bounds__(256) void k_batch_end(unsigned long long *ind, const uint32_t *tlist,
                                                    unsigned long long *abits, const BuildCounters *C)
 {
+    batch_end_body(blockIdx.x, gridDim.x, ind, tlist, abits, C);
+}
+
+// First kernel of a batch's prologue: the gate -- if the previous batch
+// still has a frontier (its next iteration's count in slot (iter_last + 1) &
+// 1), this batch was enqueued too early and everything up to the host's
+// k_unhold is a no-op (a no-op before the first batch: all counts zero) --
+// then the previous batch's reset -- its roots' root-distance entries (one wave per
+// root in the first blocks) and batch_end_body -- which are independent of
+// each other.  Every block takes the gate's decision itself from counters
+// that nothing in this kernel changes.
+template <bool W>
+__global__ __launch_bounds__(256) void k_batch_close(const uint8_t *used, LabelStore S, HubInfo *hinfo,
+                                                     typename Hop<W>::Dist *dc, unsigned long long *ind,
+                                                     const uint32_t *tlist, unsigned long long *abits, BuildCounters *C)
+{
+    if (C->hold || C->n_act[(C->iter_last + 1u) & 1u] != 0u) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) C->hold = 1u;
+        return;
+    }
+    hub_table_body<W>(blockIdx.x, gridDim.x, 0u, used, S, hinfo, dc, C);
     batch_end_body(blockIdx.x, gridDim.x, ind, tlist, abits, C);
 }
 
@@ -1506,11 +1542,8 @@ __global__ __launch_bounds__(kLoopWaves * 64) void k_batch_loop(const BatchPlan<
         if (threadIdx.x == 0) batch_begin_write(C, b, rs, rsz, s_cnt);
         __syncthreads();
         for (uint32_t vb = 0; vb * kLoopWaves < rsz; ++vb)
-            batch_init_body<W, kChunk>(vb, 1, P.used, P.off, P.adj, P.S, P.ind, P.vmark, P.tlist, P.abits, P.ag,
-                                       P.amask, P.aend, P.chunks, P.bps, P.rset, P.lmask, P.rmask, C);
-        __syncthreads();
-        for (uint32_t vb = 0; vb * kLoopWaves < rsz; ++vb)
-            hub_table_body<W>(vb, 1, 1u, P.used, P.S, P.hinfo, P.dc, C);
+            batch_init_body<W, kChunk>(vb, 1, P.used, P.off, P.adj, P.S, P.hinfo, P.dc, P.ind, P.vmark, P.tlist,
+                                       P.abits, P.ag, P.amask, P.aend, P.chunks, P.bps, P.rset, P.lmask, P.rmask, C);
         __syncthreads();
         for (uint32_t iter = 1;; ++iter) {
             push_body<W, kChunk, kLoopWaves, false>(0, 1, iter, P.off, P.adj, P.abits, P.ag, P.amask, P.aend, P.chunks,
@@ -1833,14 +1866,12 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
     const HubList1 H1{B.d1.p, B.d1len.p};
     auto enqueue_prologue = [&]() {
         // Previous batch's reset_at_end first (a no-op before the first batch).
-        hipLaunchKernelGGL(k_batch_gate, dim3(1), dim3(1), 0, st, ctr.p);
-        hipLaunchKernelGGL(k_hub_table<W>, dim3(root_blocks), dim3(256), 0, st, 0u, d_used.p, S, hinfo.p, dc.p, ctr.p);
-        hipLaunchKernelGGL(k_batch_end, dim3(1024), dim3(256), 0, st, ind.p, tlist.p, abits.p, ctr.p);
+        hipLaunchKernelGGL(k_batch_close<W>, dim3(std::max<uint32_t>(1024u, root_blocks)), dim3(256), 0, st, d_used.p,
+                           S, hinfo.p, dc.p, ind.p, tlist.p, abits.p, ctr.p);
         hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(batch), 0, st, d_used.p, ctr.p, batch, n);
-        hipLaunchKernelGGL(K.init, dim3((batch + 15) / 16), dim3(1024), 0, st, d_used.p, g->off.p, g->adj.p, S, ind.p,
-                           vmark.p, tlist.p, abits.p, ag.p, amask.p, aend.p, chunks.p,
+        hipLaunchKernelGGL(K.init, dim3((batch + 15) / 16), dim3(1024), 0, st, d_used.p, g->off.p, g->adj.p, S,
+                           hinfo.p, dc.p, ind.p, vmark.p, tlist.p, abits.p, ag.p, amask.p, aend.p, chunks.p,
                            (const ulonglong2 *)idx->bps.p, rset.p, lmask.p, rmask.p, ctr.p);
-        hipLaunchKernelGGL(k_hub_table<W>, dim3(root_blocks), dim3(256), 0, st, 1u, d_used.p, S, hinfo.p, dc.p, ctr.p);
     };
     // Iterations first + 1 .. first + count of the current batch.
     auto enqueue_iterations = [&](uint32_t first, uint32_t count, bool timed) -> pado_gpu_status {

@@ -108,13 +108,16 @@ __global__ __launch_bounds__(256) void k_ms_child(uint32_t d, const uint64_t *of
         const uint32_t u = adj[e];
         const u64 m0 = fm[v] & ~vis[u];
         if (!m0) continue;
-        if (atomicOr(&nm[u], m0) == 0ull)
-            ms_emit(u, off[u + 1] - off[u], nxt_chunks, cnt_nxt_chunks, nxt_list, cnt_nxt_list);
+        const u64 prev = atomicOr(&nm[u], m0);
+        if (prev == 0ull) ms_emit(u, off[u + 1] - off[u], nxt_chunks, cnt_nxt_chunks, nxt_list, cnt_nxt_list);
+        // The lane that sets u's level bit of a slot writes its distance
+        // (every parent would write the same d + 1).
+        const u64 fresh = m0 & ~prev;
         u64 m = m0;
         while (m) {
             const uint32_t i = __ffsll(m) - 1;
             m &= m - 1;
-            bpd[(uint64_t)u * kBpdStride + i] = (uint8_t)(d + 1);
+            if ((fresh >> i) & 1ull) bpd[(uint64_t)u * kBpdStride + i] = (uint8_t)(d + 1);
             const ulonglong2 sv = bps[(uint64_t)v * kBP + i];
             if (sv.x) atomicOr(&bps[(uint64_t)u * kBP + i].x, sv.x);
             if (sv.y) atomicOr(&bps[(uint64_t)u * kBP + i].y, sv.y);
