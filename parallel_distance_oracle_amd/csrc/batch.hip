@@ -1218,30 +1218,96 @@ __device__ __forceinline__ void add_body(uint32_t bid, uint32_t nblk, uint32_t i
     uint32_t c_next = 0, c_left = 0, my_act = 0;
     uint32_t my_pedge = 0;  // push edges of this wave's new frontier entries (flushed below 2^31)
     unsigned long long *nbits = abits + (uint64_t)(p ^ 1u) * C->abits_words;
-    // The candidate list is read one vertex ahead, and everything that
-    // depends only on the vertex (candidate word, label-store entry, the
-    // first 64 labels, the push degree of its next-frontier entry) is loaded
-    // before the first store of the step, so those loads overlap.
-    uint32_t v_next = wave < ncand ? clist[wave] : 0u;
+    // The loop over this wave's candidates (grid-stride: i, i + nwaves, ...)
+    // is software-pipelined: while candidate i is processed, the loads of
+    // the next two are in flight -- for candidate i + nwaves its first labels
+    // and first adjacency entries (stage B), which need its stage A, and for
+    // candidate i + 2 nwaves its candidate word, label-store entry, flag,
+    // offsets and distance-1 count (stage A); the candidate list itself is
+    // read three ahead.  The vertices differ, so none of these loads can see
+    // a store of an earlier candidate.
+    struct StageA {
+        unsigned long long word;  // candidate bits of this lane's four words' quarter
+        LRec rec;
+        uint64_t e0, e1;
+        uint32_t v, used, n1;
+    };
+    struct StageB {
+        uint32_t lab0, lab1, probe;
+    };
+    auto load_a = [&](uint32_t v, bool ok) {
+        StageA a{};
+        a.v = v;
+        if (ok) {
+            a.word = cand[(uint64_t)v * kWords + (lane >> 2)];
+            a.rec = S.rec[v];
+            a.used = used[v];
+            a.e0 = off[v];
+            a.e1 = off[v + 1];
+            a.n1 = iter <= 2u ? H1.len[v] : 0u;
+        }
+        return a;
+    };
+    // Per-vertex values are the same in every lane: kept in scalar registers.
+    auto uniform_a = [&](const StageA &a) {
+        StageA u = a;
+        u.rec.len = uniform32(a.rec.len);
+        u.rec.base = uniform64(a.rec.base);
+        u.rec.cap = uniform32(a.rec.cap);
+        u.e0 = uniform64(a.e0);
+        u.e1 = uniform64(a.e1);
+        u.v = uniform32(a.v);
+        u.used = uniform32(a.used);
+        u.n1 = uniform32(a.n1);
+        return u;
+    };
+    // The labels the distance query scans (see above): iteration 2 reads the
+    // distance-1 list, whose first 128 entries are read within v's slots
+    // before n1 is known.
+    auto load_b = [&](const StageA &a, bool ok) {
+        StageB b{0u, 0u, 0u};
+        if (ok) {
+            const uint32_t lim0 = iter == 1u ? 0u : (iter == 2u ? (uint32_t)(a.e1 - a.e0) : a.rec.len);
+            const uint32_t *lsrc = iter == 2u ? H1.lab + a.e0 : S.pool + a.rec.base;
+            if (lane < lim0) b.lab0 = lsrc[lane];
+            if (lane + 64 < lim0) b.lab1 = lsrc[lane + 64];
+            if (!a.used && lane < a.e1 - a.e0) b.probe = adj[a.e0 + lane];
+        }
+        return b;
+    };
+    const uint32_t c0 = wave < ncand ? clist[wave] : 0u;
+    const uint32_t c1 = wave + nwaves < ncand ? clist[wave + nwaves] : 0u;
+    uint32_t vq = wave + 2 * nwaves < ncand ? clist[wave + 2 * nwaves] : 0u;  // candidate i + 2 nwaves
+    StageA A0 = uniform_a(load_a(c0, wave < ncand));                           // candidate i
+    StageB B0 = load_b(A0, wave < ncand);
+    StageA A1 = load_a(c1, wave + nwaves < ncand);                             // candidate i + nwaves
     for (uint32_t i = wave; i < ncand; i += nwaves) {
-        // Per-vertex values are the same in every lane: kept in scalar registers.
-        const uint32_t v = uniform32(v_next);
-        const unsigned long long word = cand[(uint64_t)v * kWords + (lane >> 2)];
-        const LRec rec = S.rec[v];
-        const uint32_t len = uniform32(rec.len);
-        const unsigned long long base = uniform64(rec.base);
-        const uint32_t cap = uniform32(rec.cap);
-        const bool v_used = uniform32(used[v]) != 0u;
-        const uint64_t e0 = uniform64(off[v]), e1 = uniform64(off[v + 1]);
-        const uint32_t n1 = iter <= 2u ? uniform32(H1.len[v]) : 0u;  // distance-1 labels held
-        if (i + nwaves < ncand) v_next = clist[i + nwaves];
-        // The labels the distance query scans (see above); the first 64 of
-        // the distance-1 list are read within v's slots before n1 is known.
+        const StageA cu = A0;
+        const StageB cbl = B0;
+        {
+            const bool ok1 = i + nwaves < ncand, ok2 = i + 2 * nwaves < ncand;
+            A0 = uniform_a(A1);
+            B0 = load_b(A0, ok1);
+            A1 = load_a(uniform32(vq), ok2);
+            vq = i + 3 * nwaves < ncand ? clist[i + 3 * nwaves] : 0u;
+        }
+        const uint32_t v = cu.v;
+        const unsigned long long word = cu.word;
+        const uint32_t len = cu.rec.len;
+        const unsigned long long base = cu.rec.base;
+        const uint32_t cap = cu.rec.cap;
+        const bool v_used = cu.used != 0u;
+        const uint64_t e0 = cu.e0, e1 = cu.e1;
+        const uint32_t n1 = cu.n1;  // distance-1 labels held
         const uint32_t slen = iter == 1u ? 0u : (iter == 2u ? n1 : len);
         const uint32_t *lsrc = iter == 2u ? H1.lab + e0 : S.pool + base;
-        const uint32_t lim0 = iter == 1u ? 0u : (iter == 2u ? (uint32_t)(e1 - e0) : len);
-        const uint32_t lab0 = lane < lim0 ? lsrc[lane] : 0u;
-        const uint32_t pdeg = v_used ? 0u : push_degree_range(adj, e0, e1, rs);
+        const uint32_t lab0 = cbl.lab0, lab1 = cbl.lab1;
+        // Neighbours >= rs: a prefix of the descending adjacency; counted
+        // from the loaded entries when they cover it.
+        const uint32_t pdeg = v_used ? 0u
+                              : e1 - e0 <= 64u
+                                  ? (uint32_t)__popcll(__ballot(lane < e1 - e0 && cbl.probe >= rs))
+                                  : push_degree_range(adj, e0, e1, rs);
         const uint32_t cb = (uint32_t)(word >> (16 * (lane & 3))) & 0xFFFFu;
         // The clear below is control-dependent on every lane's loaded word
         // (through the ballot), so no lane can clear a word before all lanes
@@ -1249,35 +1315,15 @@ __device__ __forceinline__ void add_body(uint32_t bid, uint32_t nblk, uint32_t i
         if (__ballot(cb != 0) != 0ull && lane < kWords) cand[(uint64_t)v * kWords + lane] = 0ull;
         uint32_t prune = 0;
         const uint32_t my_lo = rs + 16 * lane;  // real id of this lane's first root
-        for (uint32_t j0 = 0; j0 < slen; j0 += 64) {
-            const uint32_t j = j0 + lane;
-            uint32_t h = 0, d = INF;
-            unsigned long long lm = 0ull;  // lanes whose roots h can reach in time
-            if (j < slen) {
-                const uint32_t lab = j0 == 0 ? lab0 : lsrc[j];
-                h = lab >> B;
-                d = lab & INF;
-                if (d < iter) {
-                    const HubInfo hi = hinfo[h];
-                    const uint32_t t = iter - d;
-                    if (t <= kNear) {
-                        lm = hi.near[0];
-                        if (t >= 2u) lm |= hi.near[1];
-                        if (t >= 3u) lm |= hi.near[2];
-                    } else if (d + (hi.info & INF) <= iter) {
-                        lm = ~0ull;
-                    }
-                    ++n_read;
-                }
-            }
+        // The passing hubs of one set of 64 labels (h, d, lm per lane): up to
+        // kAddHubs per step, their row loads issued together; only lanes of
+        // the hub's mask that still hold an unpruned candidate load their
+        // slice of a row (16 entries: 16 bytes, 32 wide).
+        auto hub_pass = [&](uint32_t h, uint32_t d, unsigned long long lm) {
             // A hub is worth a row only if some lane that still holds an
             // unpruned candidate has a root close enough to it.
             const unsigned long long live = __ballot((cb & ~prune) != 0u);
             unsigned long long m = __ballot((lm & live) != 0ull);
-            // Up to kAddHubs passing hubs per step: their row loads are
-            // issued together, and only lanes of the hub's mask that still
-            // hold an unpruned candidate load their slice of a row (16
-            // entries: 16 bytes, 32 wide).
             while (m) {
                 int src[kAddHubs];
                 bool ok[kAddHubs];
@@ -1328,7 +1374,52 @@ __device__ __forceinline__ void add_body(uint32_t bid, uint32_t nblk, uint32_t i
                     }
                 }
             }
+        };
+        // What hub h of a label at distance d can do (lanes whose roots it
+        // can reach in time; 0 if none).
+        auto hub_lanes = [&](uint32_t h, uint32_t d) -> unsigned long long {
+            const HubInfo hi = hinfo[h];
+            const uint32_t t = iter - d;
+            unsigned long long lm = 0ull;
+            if (t <= kNear) {
+                lm = hi.near[0];
+                if (t >= 2u) lm |= hi.near[1];
+                if (t >= 3u) lm |= hi.near[2];
+            } else if (d + (hi.info & INF) <= iter) {
+                lm = ~0ull;
+            }
+            return lm;
+        };
+        // 128 labels per step (two per lane, their hub summaries loaded
+        // together); the next step's labels are loaded while this one runs.
+        uint32_t la = lab0, lb = lab1;
+        for (uint32_t j0 = 0; j0 < slen; j0 += 128) {
+            const uint32_t ja = j0 + lane, jb = j0 + 64 + lane;
+            const uint32_t ca = la, cbl = lb;
+            la = ja + 128 < slen ? lsrc[ja + 128] : 0u;
+            lb = jb + 128 < slen ? lsrc[jb + 128] : 0u;
+            uint32_t ha = 0, da = INF, hb = 0, db = INF;
+            if (ja < slen) {
+                ha = ca >> B;
+                da = ca & INF;
+            }
+            if (jb < slen) {
+                hb = cbl >> B;
+                db = cbl & INF;
+            }
+            unsigned long long lma = 0ull, lmb = 0ull;
+            if (da < iter) {
+                lma = hub_lanes(ha, da);
+                ++n_read;
+            }
+            if (db < iter) {
+                lmb = hub_lanes(hb, db);
+                ++n_read;
+            }
+            hub_pass(ha, da, lma);
             // Every candidate of v pruned: the rest of its labels cannot matter.
+            if (__ballot((cb & ~prune) != 0u) == 0ull) break;
+            hub_pass(hb, db, lmb);
             if (__ballot((cb & ~prune) != 0u) == 0ull) break;
         }
         const uint32_t surv = cb & ~prune;
