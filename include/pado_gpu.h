@@ -168,15 +168,16 @@ typedef struct {
                              gives the same labels, so it is a tuning knob */
     uint32_t flags;       /* PADO_GPU_BUILD_* */
     uint32_t push_blocks; /* grid of the push kernel (blocks of push_waves waves),
-                             0 = 256 up to 2^17 vertices, else 1024 */
-    uint32_t add_blocks;  /* grid of the add kernel (256-thread blocks), same default */
+                             0 = 256 up to 2^17 vertices, else 768 */
+    uint32_t add_blocks;  /* grid of the add kernel (256-thread blocks),
+                             0 = 256 up to 2^17 vertices, else 512 */
     uint32_t iters_per_sync; /* batch iterations enqueued per host check of the
                                 frontier (iterations past a batch's end are
                                 empty launches), 0 = 8 */
     uint32_t sort_max_items; /* labels per call of the final canonical sort
                                 (0 = 2^31 - 1025, hipCUB's int item count);
                                 larger indexes are sorted in vertex ranges */
-    uint32_t push_chunk;  /* edges per push work descriptor: 8, 16 or 32, 0 = 16 */
+    uint32_t push_chunk;  /* edges per push work descriptor: 4, 8, 16 or 32, 0 = 8 */
     uint32_t push_waves;  /* waves per push block: 2, 4 or 8, 0 = 4 */
     uint32_t pull_ratio;  /* direction of the push step: an iteration whose frontier
                              has more push edges than pull_ratio / 1000 of the
@@ -377,8 +378,9 @@ pado_gpu_status pado_gpu_windex_load(pado_gpu_ctx *ctx, const char *filename, pa
  * Autotuner (no reference counterpart).  Every knob of pado_gpu_build_cfg
  * leaves the labels unchanged, so the tuner may pick any of them purely on
  * speed: it times full trial builds of `graph` in a coordinate search over
- * the push work-descriptor size {8, 16, 32}, push block shape {2, 4, 8}
- * waves, push and add grids {256..4096}, iterations per host check {2..16},
+ * the push work-descriptor size {4, 8, 16}, push block shape {2, 4, 8}
+ * waves, push grid {256..2048} and add grid {256..1024} blocks, iterations
+ * per host check {2..16},
  * batch size {256, 512, 1024} and the flags (hipGraph or stream batch step,
  * bit-parallel variant, and below 4K vertices the one-workgroup batch loop;
  * PADO_GPU_BUILD_WIDE of `start` is kept), starting

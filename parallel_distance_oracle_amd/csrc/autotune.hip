@@ -88,10 +88,10 @@ extern "C" pado_gpu_status pado_gpu_autotune(pado_gpu_ctx *ctx, const pado_gpu_g
     pado_gpu_build_cfg cur{};
     if (start) cur = *start;
     if (!cur.batch_size) cur.batch_size = pado::kBatch;
-    if (!cur.push_blocks) cur.push_blocks = pado::default_grid(n);
-    if (!cur.add_blocks) cur.add_blocks = pado::default_grid(n);
+    if (!cur.push_blocks) cur.push_blocks = pado::default_push_grid(n);
+    if (!cur.add_blocks) cur.add_blocks = pado::default_add_grid(n);
     if (!cur.iters_per_sync) cur.iters_per_sync = 8;
-    if (!cur.push_chunk) cur.push_chunk = pado::kChunk;
+    if (!cur.push_chunk) cur.push_chunk = pado::kPushChunk;
     if (!cur.push_waves) cur.push_waves = 4;
     if (!max_trials) max_trials = 24;
     const TrialFn once = [&](const pado_gpu_build_cfg &c, double *ms) -> pado_gpu_status {
@@ -102,10 +102,10 @@ extern "C" pado_gpu_status pado_gpu_autotune(pado_gpu_ctx *ctx, const pado_gpu_g
     };
     const uint32_t keep = cur.flags & PADO_GPU_BUILD_WIDE;
     std::vector<Knob> knobs = {
-        {&pado_gpu_build_cfg::push_chunk, {8, 16, 32}},
+        {&pado_gpu_build_cfg::push_chunk, {4, 8, 16}},
         {&pado_gpu_build_cfg::push_waves, {2, 4, 8}},
-        {&pado_gpu_build_cfg::push_blocks, {256, 512, 1024, 2048, 4096}},
-        {&pado_gpu_build_cfg::add_blocks, {256, 512, 1024, 2048, 4096}},
+        {&pado_gpu_build_cfg::push_blocks, {256, 512, 768, 1024, 2048}},
+        {&pado_gpu_build_cfg::add_blocks, {256, 384, 512, 768, 1024}},
         {&pado_gpu_build_cfg::iters_per_sync, {2, 4, 8, 16}},
         {&pado_gpu_build_cfg::batch_size, {256, 512, 1024}},
         {&pado_gpu_build_cfg::flags, {keep, keep | PADO_GPU_BUILD_STREAM, keep | PADO_GPU_BUILD_BP_PER_SLOT}},
@@ -155,7 +155,7 @@ extern "C" pado_gpu_status pado_gpu_autotune_multi(pado_gpu_ctx *ctx, uint32_t n
     pado_gpu_build_cfg cur{};
     if (start) cur = *start;
     if (!cur.iters_per_sync) cur.iters_per_sync = 8;
-    if (!cur.push_chunk) cur.push_chunk = pado::kChunk;
+    if (!cur.push_chunk) cur.push_chunk = pado::kPushChunk;
     if (!cur.push_waves) cur.push_waves = 4;
     if (!max_trials) max_trials = 12;
     const TrialFn once = [&](const pado_gpu_build_cfg &c, double *ms) -> pado_gpu_status {
@@ -174,7 +174,7 @@ extern "C" pado_gpu_status pado_gpu_autotune_multi(pado_gpu_ctx *ctx, uint32_t n
     // the push shape.
     const std::vector<Knob> knobs = {
         {&pado_gpu_build_cfg::iters_per_sync, {4, 8, 16, 32}},
-        {&pado_gpu_build_cfg::push_chunk, {8, 16, 32}},
+        {&pado_gpu_build_cfg::push_chunk, {4, 8, 16}},
         {&pado_gpu_build_cfg::push_waves, {2, 4, 8}},
         {&pado_gpu_build_cfg::batch_size, {256, 512, 1024}},
     };

@@ -1861,6 +1861,7 @@ static BatchKernels<W> batch_kernels_ch(uint32_t waves)
 template <bool W>
 static BatchKernels<W> batch_kernels(uint32_t chunk, uint32_t waves)
 {
+    if (chunk == 4) return batch_kernels_ch<W, 4>(waves);
     if (chunk == 8) return batch_kernels_ch<W, 8>(waves);
     if (chunk == 32) return batch_kernels_ch<W, 32>(waves);
     return batch_kernels_ch<W, kChunk>(waves);
@@ -1884,10 +1885,10 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
         set_error("graph too large for the wide hop domain (at most 2^20 vertices)");
         return PADO_GPU_ERR_DOMAIN;
     }
-    const uint32_t chunk = cfg.push_chunk ? cfg.push_chunk : kChunk;
+    const uint32_t chunk = cfg.push_chunk ? cfg.push_chunk : kPushChunk;
     const uint32_t waves = cfg.push_waves ? cfg.push_waves : kPushWaves;
-    if ((chunk != 8 && chunk != 16 && chunk != 32) || (waves != 2 && waves != 4 && waves != 8)) {
-        set_error("push_chunk must be 8, 16 or 32 and push_waves 2, 4 or 8");
+    if ((chunk != 4 && chunk != 8 && chunk != 16 && chunk != 32) || (waves != 2 && waves != 4 && waves != 8)) {
+        set_error("push_chunk must be 4, 8, 16 or 32 and push_waves 2, 4 or 8");
         return PADO_GPU_ERR_ARG;
     }
     const BatchKernels<W> K = batch_kernels<W>(chunk, waves);
@@ -1897,7 +1898,7 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
 
     // Per-wave reservations in k_add leave holes in the active list and the
     // descriptor array (see batch_alloc).
-    const uint64_t add_waves = 4ull * (cfg.add_blocks ? cfg.add_blocks : default_grid(n));
+    const uint64_t add_waves = 4ull * (cfg.add_blocks ? cfg.add_blocks : default_add_grid(n));
     // The label pool starts at 64 words per vertex and doubles whenever a
     // host check finds it over 40% full; the checks come every
     // iters_per_sync iterations, so the headroom covers several iterations.
@@ -1932,8 +1933,8 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
 
     const bool use_graph = (cfg.flags & PADO_GPU_BUILD_STREAM) == 0;
     const uint32_t U = cfg.iters_per_sync ? std::min<uint32_t>(cfg.iters_per_sync, 64u) : 8u;
-    const uint32_t push_grid = cfg.push_blocks ? cfg.push_blocks : default_grid(n);
-    const uint32_t add_grid = cfg.add_blocks ? cfg.add_blocks : default_grid(n);
+    const uint32_t push_grid = cfg.push_blocks ? cfg.push_blocks : default_push_grid(n);
+    const uint32_t add_grid = cfg.add_blocks ? cfg.add_blocks : default_add_grid(n);
     const uint32_t root_blocks = (batch * 64 + 255) / 256;
     const uint32_t pull_ratio = cfg.pull_ratio ? cfg.pull_ratio : kPullRatio;
 

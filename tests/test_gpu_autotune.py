@@ -36,7 +36,8 @@ def test_push_shapes_give_identical_labels(gpu_ctx, graph):
     dg = pado.DeviceGraph(gpu_ctx, G)
     flags = pado.BUILD_WIDE if wide else 0
     ref, ref_st = _build(gpu_ctx, dg, flags=flags)
-    for chunk, waves, grid in ((8, 2, 64), (8, 8, 1024), (16, 2, 4096), (32, 4, 256), (32, 8, 512), (16, 8, 128)):
+    for chunk, waves, grid in ((4, 4, 768), (4, 2, 64), (8, 2, 64), (8, 8, 1024), (16, 2, 4096), (32, 4, 256),
+                               (32, 8, 512), (16, 8, 128)):
         print(f"[gpu] {graph} chunk={chunk} waves={waves} push_blocks={grid}", flush=True)
         got, st = _build(gpu_ctx, dg, flags=flags, push_chunk=chunk, push_waves=waves, push_blocks=grid,
                          add_blocks=grid)
@@ -63,7 +64,7 @@ def test_autotune_result_builds_identical_labels(gpu_ctx):
     print("[gpu] autotune R-MAT 11 (8 configurations)", flush=True)
     best, ms, trials = pado.autotune(gpu_ctx, dg, max_trials=8)
     assert 1 <= trials <= 8 and ms > 0
-    assert best.push_chunk in (8, 16, 32) and best.push_waves in (2, 4, 8)
+    assert best.push_chunk in (4, 8, 16, 32) and best.push_waves in (2, 4, 8)
     assert best.batch_size in (256, 512, 1024)
     ref, ref_st = _build(gpu_ctx, dg)
     idx = pado.Index.build(gpu_ctx, dg, cfg=best)
