@@ -1189,8 +1189,13 @@ __global__ __launch_bounds__(1024) void k_batch_begin(const uint8_t *used, Build
 // such self labels, scans nothing, and iteration 2 scans v's distance-1 list
 // instead of the whole label array.
 constexpr uint32_t kAddHubs = 2;  // root-distance rows k_add loads per step
+constexpr uint32_t kGather = 8;   // candidate roots up to which k_add reads single entries
 
-template <bool W, uint32_t CH>
+// DEEP compiles in the hub-parallel distance query (see below), which only
+// iterations past kNear + 1 use; the launch schedule runs the other
+// instantiation for the iterations before, whose code it leaves as lean as
+// it was.
+template <bool W, uint32_t CH, bool DEEP = true>
 __device__ __forceinline__ void add_body(uint32_t bid, uint32_t nblk, uint32_t iter, const uint8_t *used, const uint64_t *off, const uint32_t *adj, const uint32_t *clist,
                                              unsigned long long *cand, LabelStore S, HubList1 H1, HubInfo *hinfo,
                                              typename Hop<W>::Dist *dc,
@@ -1390,37 +1395,102 @@ __device__ __forceinline__ void add_body(uint32_t bid, uint32_t nblk, uint32_t i
             }
             return lm;
         };
+        // Past iteration kNear + 1 the hub summaries stop naming lanes (a
+        // threshold above kNear only has the minimum), so most hubs pass and
+        // reading a row slice per passing hub costs a round trip per two
+        // hubs.  There, a vertex with few candidate roots takes the distance
+        // query hub-parallel: lanes hold labels, and each lane reads the one
+        // root-distance entry per candidate root it needs, all of a step's
+        // reads in flight together.  Otherwise lanes hold roots (16 each) and
+        // read row slices of the passing hubs (hub_pass).  croot: the
+        // candidate root offsets (wave-uniform); cpr: which of them are
+        // pruned.
+        uint32_t croot[kGather];
+        uint32_t ncand_v = kGather + 1, cpr = 0;
+        if (DEEP && iter > kNear + 1 && slen != 0) {
+            ncand_v = 0;
+            uint32_t rest = cb;
+#pragma unroll
+            for (uint32_t k = 0; k < kGather; ++k) {
+                const unsigned long long m = __ballot(rest != 0u);
+                croot[k] = ~0u;
+                if (m) {
+                    const uint32_t l0 = (uint32_t)__ffsll(m) - 1;
+                    const uint32_t bits = uniform32(__shfl(rest, (int)l0));
+                    croot[k] = 16 * l0 + (uint32_t)__ffs(bits) - 1;
+                    if (lane == l0) rest &= rest - 1;
+                    ++ncand_v;
+                }
+            }
+            if (__ballot(rest != 0u) != 0ull) ncand_v = kGather + 1;  // too many: row slices
+        }
+        const bool gather = ncand_v <= kGather;
+        // One label per lane against the listed candidates: prune candidate k
+        // if d + d_r(h) <= iter with h below it (PARA:1250-1262).
+        auto gather_pass = [&](uint32_t h, uint32_t d, unsigned long long lm) {
+            if (__ballot(lm != 0ull) == 0ull) return;
+            uint32_t x[kGather];
+#pragma unroll
+            for (uint32_t k = 0; k < kGather; ++k) {
+                const uint32_t ro = croot[k];
+                x[k] = ~0u;
+                if (k < ncand_v && !((cpr >> k) & 1u) && ((lm >> (ro >> 4)) & 1ull) && h < rs + ro)
+                    x[k] = dc[(uint64_t)h * kBatch + ro];
+            }
+            bool any = false;
+#pragma unroll
+            for (uint32_t k = 0; k < kGather; ++k) {
+                any = any || x[k] != ~0u;
+                if (__ballot(x[k] != ~0u && d + x[k] <= iter) != 0ull) cpr |= 1u << k;
+            }
+            n_rows += __ballot(any) != 0ull ? 1u : 0u;
+        };
         // 128 labels per step (two per lane, their hub summaries loaded
         // together); the next step's labels are loaded while this one runs.
-        uint32_t la = lab0, lb = lab1;
-        for (uint32_t j0 = 0; j0 < slen; j0 += 128) {
-            const uint32_t ja = j0 + lane, jb = j0 + 64 + lane;
-            const uint32_t ca = la, cbl = lb;
-            la = ja + 128 < slen ? lsrc[ja + 128] : 0u;
-            lb = jb + 128 < slen ? lsrc[jb + 128] : 0u;
-            uint32_t ha = 0, da = INF, hb = 0, db = INF;
-            if (ja < slen) {
-                ha = ca >> B;
-                da = ca & INF;
+        // visit(h, d, lm) per half step; it returns true when every candidate
+        // is pruned (the rest of the labels cannot matter).
+        auto scan_labels = [&](auto &&visit) {
+            uint32_t la = lab0, lb = lab1;
+            for (uint32_t j0 = 0; j0 < slen; j0 += 128) {
+                const uint32_t ja = j0 + lane, jb = j0 + 64 + lane;
+                const uint32_t ca = la, cbl = lb;
+                la = ja + 128 < slen ? lsrc[ja + 128] : 0u;
+                lb = jb + 128 < slen ? lsrc[jb + 128] : 0u;
+                uint32_t ha = 0, da = INF, hb = 0, db = INF;
+                if (ja < slen) {
+                    ha = ca >> B;
+                    da = ca & INF;
+                }
+                if (jb < slen) {
+                    hb = cbl >> B;
+                    db = cbl & INF;
+                }
+                unsigned long long lma = 0ull, lmb = 0ull;
+                if (da < iter) {
+                    lma = hub_lanes(ha, da);
+                    ++n_read;
+                }
+                if (db < iter) {
+                    lmb = hub_lanes(hb, db);
+                    ++n_read;
+                }
+                if (visit(ha, da, lma)) break;
+                if (visit(hb, db, lmb)) break;
             }
-            if (jb < slen) {
-                hb = cbl >> B;
-                db = cbl & INF;
-            }
-            unsigned long long lma = 0ull, lmb = 0ull;
-            if (da < iter) {
-                lma = hub_lanes(ha, da);
-                ++n_read;
-            }
-            if (db < iter) {
-                lmb = hub_lanes(hb, db);
-                ++n_read;
-            }
-            hub_pass(ha, da, lma);
-            // Every candidate of v pruned: the rest of its labels cannot matter.
-            if (__ballot((cb & ~prune) != 0u) == 0ull) break;
-            hub_pass(hb, db, lmb);
-            if (__ballot((cb & ~prune) != 0u) == 0ull) break;
+        };
+        if (DEEP && gather) {
+            scan_labels([&](uint32_t h, uint32_t d, unsigned long long lm) {
+                gather_pass(h, d, lm);
+                return cpr == (1u << ncand_v) - 1u;
+            });
+#pragma unroll
+            for (uint32_t k = 0; k < kGather; ++k)
+                if (((cpr >> k) & 1u) && lane == (croot[k] >> 4)) prune |= 1u << (croot[k] & 15u);
+        } else {
+            scan_labels([&](uint32_t h, uint32_t d, unsigned long long lm) {
+                hub_pass(h, d, lm);
+                return __ballot((cb & ~prune) != 0u) == 0ull;
+            });
         }
         const uint32_t surv = cb & ~prune;
         n_dq += __popc(cb & prune);
@@ -1508,14 +1578,15 @@ __device__ __forceinline__ void add_body(uint32_t bid, uint32_t nblk, uint32_t i
     }
 }
 
-template <bool W, uint32_t CH>
+template <bool W, uint32_t CH, bool DEEP>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_add(uint32_t iter, const uint8_t *used, const uint64_t *off, const uint32_t *adj, const uint32_t *clist,
                                              unsigned long long *cand, LabelStore S, HubList1 H1, HubInfo *hinfo,
                                              typename Hop<W>::Dist *dc,
                                              unsigned long long *abits, unsigned long long *ag, uint32_t *amask,
                                              uint32_t *aend, unsigned long long *chunks, BuildCounters *C)
 {
-    add_body<W, CH>(blockIdx.x, gridDim.x, iter, used, off, adj, clist, cand, S, H1, hinfo, dc, abits, ag, amask, aend, chunks, C);
+    add_body<W, CH, DEEP>(blockIdx.x, gridDim.x, iter, used, off, adj, clist, cand, S, H1, hinfo, dc, abits, ag, amask,
+                          aend, chunks, C);
 }
 
 // The graph's shape in the counters (once, at allocation).
@@ -1844,14 +1915,16 @@ template <bool W>
 struct BatchKernels {
     decltype(&k_batch_init<W, kChunk>) init;
     decltype(&k_push<W, kChunk, kPushWaves>) push;
-    decltype(&k_add<W, kChunk>) add;
+    decltype(&k_add<W, kChunk, false>) add;       // iterations up to kNear + 1
+    decltype(&k_add<W, kChunk, true>) add_deep;   // the later ones
     uint32_t chunk, waves;
 };
 
 template <bool W, uint32_t CH>
 static BatchKernels<W> batch_kernels_ch(uint32_t waves)
 {
-    BatchKernels<W> K{k_batch_init<W, CH>, k_push<W, CH, kPushWaves>, k_add<W, CH>, CH, kPushWaves};
+    BatchKernels<W> K{k_batch_init<W, CH>, k_push<W, CH, kPushWaves>, k_add<W, CH, false>, k_add<W, CH, true>, CH,
+                      kPushWaves};
     if (waves == 2) K.push = k_push<W, CH, 2>;
     if (waves == 8) K.push = k_push<W, CH, 8>;
     if (waves == 2 || waves == 8) K.waves = waves;
@@ -1976,7 +2049,8 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
                                idx->bp_pass_through ? 0u : 1u, pull_ratio);
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 1], st));
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 2], st));
-            hipLaunchKernelGGL(K.add, dim3(add_grid), dim3(256), 0, st, iter, d_used.p, g->off.p, g->adj.p, clist.p,
+            hipLaunchKernelGGL(iter > kNear + 1 ? K.add_deep : K.add, dim3(add_grid), dim3(256), 0, st, iter, d_used.p,
+                               g->off.p, g->adj.p, clist.p,
                                cand.p, S, H1, hinfo.p, dc.p, abits.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 3], st));
         }
