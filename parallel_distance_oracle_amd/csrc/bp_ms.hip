@@ -256,17 +256,25 @@ pado_gpu_status build_bp_seeded(pado_gpu_graph *g, pado_gpu_index *idx, DevBuf<u
         hipLaunchKernelGGL(k_ms_seed, dim3(grid_for(ns, 256, 4096)), dim3(256), 0, st, ns, dseeds.p, dseeds.p + ns, dseeds.p + 2 * ns, g->off.p,
                            idx->bpd.p, bps, fm, nm, vis.p, cur_c, cnt_cur_c, cur_l, cnt_cur_l, nxt_c, cnt_nxt_c,
                            nxt_l, cnt_nxt_l);
-    const uint32_t egrid = 4096;
+    // Grids follow the frontier: a level's edge kernels get enough threads
+    // for its descriptors in one pass (16 lanes each), up to kMsGrid blocks;
+    // level 0's size is not read back, so it gets the most.
+    constexpr uint32_t kMsGrid = 4096;
+    uint32_t cur_chunks = ~0u;
     uint32_t max_level = 0;
     hipEvent_t e0, e1;
     PADO_HIP(hipEventCreate(&e0));
     PADO_HIP(hipEventCreate(&e1));
     for (uint32_t d = 0; ns; ++d) {
+        const uint32_t egrid = cur_chunks == ~0u
+                                   ? kMsGrid
+                                   : (uint32_t)std::min<uint64_t>(kMsGrid, std::max<uint64_t>(1, ((uint64_t)cur_chunks * kChunk + 255) / 256));
         PADO_HIP(hipEventRecord(e0, st));
         hipLaunchKernelGGL(k_ms_sib, dim3(egrid), dim3(256), 0, st, g->off.p, g->adj.p, fm, bps, cur_c, cnt_cur_c);
         hipLaunchKernelGGL(k_ms_child, dim3(egrid), dim3(256), 0, st, d, g->off.p, g->adj.p, fm, nm, vis.p,
                            idx->bpd.p, bps, cur_c, cnt_cur_c, nxt_c, cnt_nxt_c, nxt_l, cnt_nxt_l);
-        hipLaunchKernelGGL(k_ms_update, dim3(1024), dim3(256), 0, st, cur_l, cnt_cur_l, fm, nxt_l, cnt_nxt_l, nm,
+        // (the new frontier has at most one vertex per edge of this one)
+        hipLaunchKernelGGL(k_ms_update, dim3(egrid), dim3(256), 0, st, cur_l, cnt_cur_l, fm, nxt_l, cnt_nxt_l, nm,
                            vis.p);
         PADO_HIP(hipEventRecord(e1, st));
         PADO_HIP(hipMemcpyAsync(h_cnt, cnt.p, 4 * sizeof(uint32_t), hipMemcpyDeviceToHost, st));
@@ -275,7 +283,9 @@ pado_gpu_status build_bp_seeded(pado_gpu_graph *g, pado_gpu_index *idx, DevBuf<u
         PADO_HIP(hipEventElapsedTime(&ms, e0, e1));
         idx->ktimes.ms_bp += ms;
         idx->ktimes.n_bp += 1;
-        const uint32_t next_count = h_cnt[3];
+        // the counter slots swap roles every level: read them through the pointers
+        const uint32_t next_count = h_cnt[cnt_nxt_l - cnt.p];
+        cur_chunks = h_cnt[cnt_nxt_c - cnt.p];
         if (next_count == 0) break;
         if (d + 1 > max_level) max_level = d + 1;
         if (d + 1 >= kInf8) {
