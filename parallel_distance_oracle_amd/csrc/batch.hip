@@ -29,12 +29,12 @@
 //                    (hub << 8 | dist) labels, grown fourfold (at least 32)
 //                    on overflow by a bump allocator (one wave owns one
 //                    vertex's array); rec[n] = (base, len, cap), 16 B each
-//   vmark[n] u64     ((batch << 16 | iteration) << 1 | candidate) of the last
-//                    push into the vertex, raised by atomicMax
 //   lmask[n][16] u64 bit-parallel level masks (bit i of word k < 8: slot i at
-//                    distance k) and nonempty-set masks (words 8, 9: slots
-//                    with S^-1 / S^0 nonempty); rmask / rset: the batch roots' masks and
-//                    set pairs, level- / slot-major
+//                    distance k), nonempty-set masks (words 8, 9: slots with
+//                    S^-1 / S^0 nonempty) and vmark (word 10: ((batch << 16 |
+//                    iteration) << 1 | candidate) of the last push into the
+//                    vertex, raised by atomicMax); rmask / rset: the batch
+//                    roots' mask words and set pairs, word- / slot-major
 //   hinfo[n] 32 B    per hub: lane masks of the batch roots at distance 1, 2,
 //                    3 (bit l: roots [16l, 16l+16)) and 0xFFFFFF00 | min
 //                    distance from any batch root
@@ -174,8 +174,12 @@ __device__ inline bool bp_dist_scan(const uint32_t (&dr)[13], const uint32_t (&d
 constexpr uint32_t kMaskIters = 5;
 
 // Words of a vertex's mask row: 8 level masks, then the slots whose S^-1 /
-// S^0 set is nonempty, then padding to one 128-byte line.
+// S^0 set is nonempty, the touched mark, then padding to one 128-byte line.
 constexpr uint32_t kLmWords = 16, kLmNz1 = 8, kLmNz0 = 9;
+// Word 10 of the row is the vertex's touched mark (vmark, see push_body):
+// kept in the row whose other words the bit-parallel check of a fresh pair
+// reads anyway, so marking a tail touches no further cache line.
+constexpr uint32_t kLmMark = 10;
 
 // The candidate slots of bp_dist_scan / bp_mask_scan whose set test can
 // prune at all (bp_set_hit): at td == iter + 2 both S^-1 sets must be
@@ -594,7 +598,7 @@ __device__ __forceinline__ void batch_init_body(uint32_t bid, uint32_t nblk, con
     // Every root of the batch (bit-parallel roots included) is on the
     // touched list; a row that stays empty is cleared at no harm.
     if (ro < rsz && lane == 0) {
-        vmark[r] = (unsigned long long)gen_batch << 17;
+        vmark[(uint64_t)r * kLmWords] = (unsigned long long)gen_batch << 17;
         tlist[ro] = r;
     }
     // Slot-major copy of the batch roots' bit-parallel sets and word-major
@@ -840,9 +844,9 @@ __device__ __forceinline__ void pull_step(uint32_t wave, uint32_t nwaves, uint32
         bool new_t = false, new_c = false;
         if (w == 0 && touched) {
             const unsigned long long want = (gen_iter << 1) | (got ? 1ull : 0ull);
-            const unsigned long long prev = vmark[t];
+            const unsigned long long prev = vmark[(uint64_t)t * kLmWords];
             if (prev < want) {
-                vmark[t] = want;
+                vmark[(uint64_t)t * kLmWords] = want;
                 new_t = (prev >> 17) != gen_batch;
                 if ((prev >> 1) != gen_iter) ++n_tail;
                 new_c = got;
@@ -1092,9 +1096,9 @@ __device__ __forceinline__ void push_body(uint32_t bid, uint32_t nblk, uint32_t 
         bool new_t = false, new_c = false;
         if (touched) {
             const unsigned long long want = (gen_iter << 1) | (got ? 1ull : 0ull);
-            unsigned long long prev = vmark[t];
+            unsigned long long prev = vmark[(uint64_t)t * kLmWords];
             if (prev < want) {
-                prev = atomicMax(&vmark[t], want);
+                prev = atomicMax(&vmark[(uint64_t)t * kLmWords], want);
                 new_t = (prev >> 17) != gen_batch;
                 if ((prev >> 1) != gen_iter) ++n_tail;
                 new_c = got && prev < want;
@@ -1800,7 +1804,6 @@ template <bool W>
 struct BatchBuffers {
     DevBuf<unsigned long long> ind, cand, ag, chunks;
     DevBuf<unsigned long long> abits;  // frontier bitmaps, two slots of ceil(n / 64) words
-    DevBuf<unsigned long long> vmark;  // ((batch << 16 | iteration) << 1 | candidate) of the last mark
     DevBuf<unsigned long long> lmask;  // bit-parallel mask rows, kLmWords words per vertex
     DevBuf<unsigned long long> rmask;  // batch roots' mask words 0..kLmNz0, word-major [10][kBatch]
     DevBuf<uint32_t> tlist, clist, amask, aend, pool;
@@ -1814,7 +1817,7 @@ struct BatchBuffers {
     uint32_t dc_rows = 0;
     void release()
     {
-        for (auto *b : {&ind, &cand, &ag, &chunks, &abits, &vmark, &lmask, &rmask}) b->release();
+        for (auto *b : {&ind, &cand, &ag, &chunks, &abits, &lmask, &rmask}) b->release();
         for (auto *b : {&tlist, &clist, &amask, &aend, &pool, &d1, &d1len}) b->release();
         hinfo.release();
         lrec.release();
@@ -1838,7 +1841,6 @@ static pado_gpu_status batch_alloc(const pado_gpu_graph *g, const pado_gpu_index
     hipStream_t st = g->ctx->stream;
     const uint32_t n = g->n;
     const uint32_t nw = (n + 63) / 64;  // words of one frontier bitmap slot
-    PADO_TRY(B.vmark.alloc(n));
     PADO_TRY(B.ind.alloc((uint64_t)n * kWords));
     PADO_TRY(B.cand.alloc((uint64_t)n * kWords));
     PADO_TRY(B.ag.alloc((uint64_t)n * kWords));
@@ -1865,7 +1867,6 @@ static pado_gpu_status batch_alloc(const pado_gpu_graph *g, const pado_gpu_index
                        (uint64_t)dc_rows * kBatch * sizeof(DT) / 4, ~0u);
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, B.ind.p, (uint64_t)n * kWords);
     hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, B.cand.p, (uint64_t)n * kWords);
-    hipLaunchKernelGGL(k_zero_u64, dim3(fg), dim3(256), 0, st, B.vmark.p, (uint64_t)n);
     hipLaunchKernelGGL(k_bp_masks, dim3(fg), dim3(256), 0, st, n, idx->bpd.p, (const ulonglong2 *)idx->bps.p,
                        B.lmask.p);
     hipLaunchKernelGGL(k_hub_info_init, dim3(fg), dim3(256), 0, st, B.hinfo.p, (uint64_t)n);
@@ -1981,7 +1982,8 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
     PADO_TRY(batch_alloc<W>(g, idx, add_waves, chunk,
                             std::max<unsigned long long>(std::max<unsigned long long>(1ull << 22, 64ull * n), pool_min),
                             std::min<uint32_t>(n, 1u << 16), B));
-    auto &ind = B.ind, &cand = B.cand, &ag = B.ag, &chunks = B.chunks, &abits = B.abits, &vmark = B.vmark;
+    auto &ind = B.ind, &cand = B.cand, &ag = B.ag, &chunks = B.chunks, &abits = B.abits;
+    unsigned long long *vmark = B.lmask.p + kLmMark;  // word kLmMark of each mask row (stride kLmWords)
     auto &lmask = B.lmask, &rmask = B.rmask;
     auto &tlist = B.tlist, &clist = B.clist, &amask = B.amask, &aend = B.aend, &pool = B.pool;
     auto &hinfo = B.hinfo;
@@ -2035,7 +2037,7 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
                            S, hinfo.p, dc.p, ind.p, tlist.p, abits.p, ctr.p);
         hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(batch), 0, st, d_used.p, ctr.p, batch, n);
         hipLaunchKernelGGL(K.init, dim3((batch + 15) / 16), dim3(1024), 0, st, d_used.p, g->off.p, g->adj.p, S,
-                           hinfo.p, dc.p, ind.p, vmark.p, tlist.p, abits.p, ag.p, amask.p, aend.p, chunks.p,
+                           hinfo.p, dc.p, ind.p, vmark, tlist.p, abits.p, ag.p, amask.p, aend.p, chunks.p,
                            (const ulonglong2 *)idx->bps.p, rset.p, lmask.p, rmask.p, ctr.p);
     };
     // Iterations first + 1 .. first + count of the current batch.
@@ -2044,7 +2046,7 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
             const uint32_t iter = first + 1 + u;
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u], st));
             hipLaunchKernelGGL(K.push, dim3(push_grid), dim3(64 * K.waves), 0, st, iter, g->off.p, g->adj.p, abits.p,
-                               ag.p, amask.p, aend.p, chunks.p, ind.p, cand.p, vmark.p, tlist.p, clist.p,
+                               ag.p, amask.p, aend.p, chunks.p, ind.p, cand.p, vmark, tlist.p, clist.p,
                                idx->bpd.p, lmask.p, rmask.p, (const ulonglong2 *)idx->bps.p, rset.p, ctr.p,
                                idx->bp_pass_through ? 0u : 1u, pull_ratio);
             if (timed) PADO_HIP(hipEventRecord(ev[4 * u + 1], st));
@@ -2294,7 +2296,7 @@ static BatchPlan<W> batch_plan(const pado_gpu_graph *g, const pado_gpu_index *id
     P.H1 = HubList1{B.d1.p, B.d1len.p};
     P.ind = B.ind.p;
     P.cand = B.cand.p;
-    P.vmark = B.vmark.p;
+    P.vmark = B.lmask.p + kLmMark;
     P.tlist = B.tlist.p;
     P.clist = B.clist.p;
     P.abits = B.abits.p;
