@@ -574,7 +574,7 @@ __device__ __forceinline__ void hub_rows(uint32_t fill, uint32_t ro, unsigned lo
 // initialize (PARA:848-1080): non-BP roots become active with their own bit as
 // last group, get the self label (r, 0), and set their own indicator bit;
 // then their labels go into the root-distance table (PARA:986-1013).  One
-// wave per root, sixteen roots per block.  Root offset ro owns touched-list
+// wave per root, four roots per block.  Root offset ro owns touched-list
 // slot ro (k_batch_begin sets the count and the frontier size), so the shared
 // counters left are the push-descriptor count, updated once per block, and
 // the frontier's push-edge count.
@@ -660,7 +660,7 @@ __device__ __forceinline__ void batch_init_body(uint32_t bid, uint32_t nblk, con
 }
 
 template <bool W, uint32_t CH>
-__global__ __launch_bounds__(1024) void k_batch_init(const uint8_t *used, const uint64_t *off, const uint32_t *adj,
+__global__ __launch_bounds__(256) void k_batch_init(const uint8_t *used, const uint64_t *off, const uint32_t *adj,
                                                      LabelStore S, HubInfo *hinfo, typename Hop<W>::Dist *dc,
                                                      unsigned long long *ind, unsigned long long *vmark,
                                                      uint32_t *tlist, unsigned long long *abits, unsigned long long *ag,
@@ -2036,7 +2036,7 @@ static pado_gpu_status build_batches_t(pado_gpu_graph *g, pado_gpu_index *idx, c
         hipLaunchKernelGGL(k_batch_close<W>, dim3(std::max<uint32_t>(1024u, root_blocks)), dim3(256), 0, st, d_used.p,
                            S, hinfo.p, dc.p, ind.p, tlist.p, abits.p, ctr.p);
         hipLaunchKernelGGL(k_batch_begin, dim3(1), dim3(batch), 0, st, d_used.p, ctr.p, batch, n);
-        hipLaunchKernelGGL(K.init, dim3((batch + 15) / 16), dim3(1024), 0, st, d_used.p, g->off.p, g->adj.p, S,
+        hipLaunchKernelGGL(K.init, dim3((batch + 3) / 4), dim3(256), 0, st, d_used.p, g->off.p, g->adj.p, S,
                            hinfo.p, dc.p, ind.p, vmark, tlist.p, abits.p, ag.p, amask.p, aend.p, chunks.p,
                            (const ulonglong2 *)idx->bps.p, rset.p, lmask.p, rmask.p, ctr.p);
     };
