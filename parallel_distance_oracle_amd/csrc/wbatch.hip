@@ -249,6 +249,9 @@ __device__ inline void w_lds_sync()
 
 constexpr uint32_t kWPushWaves = 4;  // waves per k_w_push block
 
+// Improved pairs of a tail up to which k_w_check reads single table entries.
+constexpr uint32_t kWGather = 8;
+
 // k_w_check reserves next-frontier slots per wave: kWAlistGrab active-list
 // entries and up to kWChunkGrab push descriptors per atomic on the shared
 // counters; a descriptor slot reserved but not used holds kWChunkPad (edge
@@ -408,55 +411,129 @@ __global__ __launch_bounds__(256) void k_w_check(uint32_t round, uint32_t rs, co
             wmax = y > wmax ? y : wmax;
         }
         uint32_t prune = 0;
-        const uint32_t len = S.len[t];
-        const u64 base = S.base[t];
-        for (uint32_t j0 = 0; j0 < len; j0 += 64) {
-            const uint32_t j = j0 + lane;
-            uint32_t h = 0, d = kWInf, slot = kEmpty;
-            if (j < len) {
-                const u64 lab = S.pool[base + j];
-                h = (uint32_t)(lab >> 32);
-                d = (uint32_t)lab;
-                slot = hslot[h];
-            }
-            const bool pass = slot < kEmpty - 1 && (uint64_t)d + hmin[h] <= wmax;
-            u64 m = __ballot(pass);
-            // Up to four passing hubs per step: their row loads go out
-            // together (32 B per lane each), and only lanes that still hold an
-            // unpruned improved pair load.
-            while (m) {
-                int src[4];
-                bool ok[4];
+        const uint32_t len = uniform32(S.len[t]);
+        const u64 base = uniform64(S.base[t]);
+        // A tail with few improved pairs (most of them) takes the check
+        // hub-parallel: lanes hold labels and read one table entry per
+        // improved pair, all of a step's reads in flight together; with more,
+        // lanes hold roots (8 each) and read row slices of the passing hubs.
+        // croot / cdv: the improved root offsets and their tentative
+        // distances (wave-uniform); cpr: which are pruned.
+        uint32_t croot[kWGather], cdv[kWGather];
+        uint32_t ncand_t = 0, cpr = 0;
+        {
+            uint32_t rest = imp;
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    ok[u] = m != 0ull;
-                    src[u] = ok[u] ? __ffsll(m) - 1 : 0;
-                    if (ok[u]) m &= m - 1;
+            for (uint32_t k = 0; k < kWGather; ++k) {
+                const unsigned long long m = __ballot(rest != 0u);
+                croot[k] = 0u;
+                cdv[k] = 0u;
+                if (m) {
+                    const uint32_t l0 = (uint32_t)__ffsll(m) - 1;
+                    const uint32_t bits = uniform32(__shfl(rest, (int)l0));
+                    const uint32_t q = (uint32_t)__ffs(bits) - 1;
+                    // dv[q] of lane l0 (unrolled select: no runtime index into dv)
+                    uint32_t myd = 0;
+#pragma unroll
+                    for (uint32_t qq = 0; qq < 8; ++qq)
+                        if (qq == q) myd = dv[qq];
+                    croot[k] = 8 * l0 + q;
+                    cdv[k] = uniform32(__shfl(myd, (int)l0));
+                    if (lane == l0) rest &= rest - 1;
+                    ++ncand_t;
                 }
-                const bool need = (imp & ~prune) != 0u;
-                uint32_t dd[4];
-                uint4 r0[4], r1[4];
+            }
+            if (__ballot(rest != 0u) != 0ull) ncand_t = kWGather + 1;
+        }
+        if (ncand_t <= kWGather) {
+            auto gather_pass = [&](uint32_t d, uint32_t slot, uint32_t hm) {
+                uint32_t x[kWGather];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    dd[u] = __shfl(d, src[u]);
-                    const uint32_t sl = __shfl(slot, src[u]);
-                    r0[u] = r1[u] = make_uint4(kWInf, kWInf, kWInf, kWInf);
-                    if (ok[u] && need) {
-                        const uint4 *row = reinterpret_cast<const uint4 *>(dcw + (uint64_t)sl * kWB + lane * 8);
-                        r0[u] = row[0];
-                        r1[u] = row[1];
+                for (uint32_t k = 0; k < kWGather; ++k) {
+                    x[k] = kWInf;
+                    if (k < ncand_t && !((cpr >> k) & 1u) && slot < kEmpty - 1 && (uint64_t)d + hm <= cdv[k])
+                        x[k] = dcw[(uint64_t)slot * kWB + croot[k]];
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < kWGather; ++k)
+                    if (__ballot(x[k] != kWInf && (uint64_t)d + x[k] <= cdv[k]) != 0ull) cpr |= 1u << k;
+            };
+            // 128 labels per step, the next step's labels loaded while this one runs
+            u64 la = lane < len ? S.pool[base + lane] : 0ull, lb = lane + 64 < len ? S.pool[base + lane + 64] : 0ull;
+            for (uint32_t j0 = 0; j0 < len; j0 += 128) {
+                const uint32_t ja = j0 + lane, jb = j0 + 64 + lane;
+                const u64 ca = la, cbl = lb;
+                la = ja + 128 < len ? S.pool[base + ja + 128] : 0ull;
+                lb = jb + 128 < len ? S.pool[base + jb + 128] : 0ull;
+                uint32_t da = kWInf, db = kWInf, sa = kEmpty, sb = kEmpty, ma = kWInf, mb = kWInf;
+                if (ja < len) {
+                    const uint32_t h = (uint32_t)(ca >> 32);
+                    da = (uint32_t)ca;
+                    sa = hslot[h];
+                    ma = hmin[h];
+                }
+                if (jb < len) {
+                    const uint32_t h = (uint32_t)(cbl >> 32);
+                    db = (uint32_t)cbl;
+                    sb = hslot[h];
+                    mb = hmin[h];
+                }
+                gather_pass(da, sa, ma);
+                gather_pass(db, sb, mb);
+                if (cpr == (1u << ncand_t) - 1u) break;
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < kWGather; ++k)
+                if (((cpr >> k) & 1u) && lane == (croot[k] >> 3)) prune |= 1u << (croot[k] & 7u);
+        } else {
+            for (uint32_t j0 = 0; j0 < len; j0 += 64) {
+                const uint32_t j = j0 + lane;
+                uint32_t h = 0, d = kWInf, slot = kEmpty;
+                if (j < len) {
+                    const u64 lab = S.pool[base + j];
+                    h = (uint32_t)(lab >> 32);
+                    d = (uint32_t)lab;
+                    slot = hslot[h];
+                }
+                const bool pass = slot < kEmpty - 1 && (uint64_t)d + hmin[h] <= wmax;
+                u64 m = __ballot(pass);
+                // Up to four passing hubs per step: their row loads go out
+                // together (32 B per lane each), and only lanes that still hold an
+                // unpruned improved pair load.
+                while (m) {
+                    int src[4];
+                    bool ok[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        ok[u] = m != 0ull;
+                        src[u] = ok[u] ? __ffsll(m) - 1 : 0;
+                        if (ok[u]) m &= m - 1;
+                    }
+                    const bool need = (imp & ~prune) != 0u;
+                    uint32_t dd[4];
+                    uint4 r0[4], r1[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        dd[u] = __shfl(d, src[u]);
+                        const uint32_t sl = __shfl(slot, src[u]);
+                        r0[u] = r1[u] = make_uint4(kWInf, kWInf, kWInf, kWInf);
+                        if (ok[u] && need) {
+                            const uint4 *row = reinterpret_cast<const uint4 *>(dcw + (uint64_t)sl * kWB + lane * 8);
+                            r0[u] = row[0];
+                            r1[u] = row[1];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const uint32_t x[8] = {r0[u].x, r0[u].y, r0[u].z, r0[u].w, r1[u].x, r1[u].y, r1[u].z, r1[u].w};
+#pragma unroll
+                        for (int q = 0; q < 8; ++q)
+                            if (((imp >> q) & 1u) && x[q] != kWInf && (uint64_t)dd[u] + x[q] <= dv[q]) prune |= 1u << q;
                     }
                 }
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const uint32_t x[8] = {r0[u].x, r0[u].y, r0[u].z, r0[u].w, r1[u].x, r1[u].y, r1[u].z, r1[u].w};
-#pragma unroll
-                    for (int q = 0; q < 8; ++q)
-                        if (((imp >> q) & 1u) && x[q] != kWInf && (uint64_t)dd[u] + x[q] <= dv[q]) prune |= 1u << q;
-                }
+                // Every improved pair pruned: the rest of the labels cannot matter.
+                if (__ballot((imp & ~prune) != 0u) == 0ull) break;
             }
-            // Every improved pair pruned: the rest of the labels cannot matter.
-            if (__ballot((imp & ~prune) != 0u) == 0ull) break;
         }
         const uint32_t live = imp & ~prune;
         n_pruned += __popc(imp & prune);
