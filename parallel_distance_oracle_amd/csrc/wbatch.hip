@@ -343,8 +343,13 @@ __global__ __launch_bounds__(256) void k_w_push(uint32_t round, uint32_t rs, uin
                 if (du == kWInf) continue;
                 const uint32_t cand = du + s_w[wv][o];
                 ++n_relax;
-                const uint32_t old = atomicMin(&T[(uint64_t)s_t[wv][o] * kWB + ro], cand);
-                if (cand < old) atomicOr(&s_imp[wv][o], 1ull << b);
+                // A plain read first: values only fall during the kernel, so
+                // one read as <= cand (even a stale one) makes the atomic a no-op.
+                uint32_t *tt = &T[(uint64_t)s_t[wv][o] * kWB + ro];
+                if (cand < *tt) {
+                    const uint32_t old = atomicMin(tt, cand);
+                    if (cand < old) atomicOr(&s_imp[wv][o], 1ull << b);
+                }
             }
             w_lds_sync();
             const u64 imp = s_imp[wv][lane];
