@@ -50,6 +50,9 @@ struct WCounters {
     uint32_t n_act[2], n_chunk[2], n_imp[2], n_touched, n_hub, overflow, pad1;
     uint32_t a_top[2];  // active-list slots handed out (>= n_act: per-wave reservations)
     u64 pool_top, relax, improved, pruned, labels, filtered, rounds, need;
+    // Pool words the batch's commit can reserve at most: per touched vertex
+    // the reservation its list would take if every batch root were kept.
+    u64 need_bound;
 };
 
 struct WStore {
@@ -116,52 +119,66 @@ __device__ inline u64 store_reserve(WStore S, WCounters *C, uint32_t v, uint32_t
     return nb;
 }
 
-__device__ inline void emit_chunks(uint32_t k, uint32_t pdeg, uint32_t *n_chunk, u64 *chunks)
+// Pool words a commit of up to `add` new labels may reserve for a list of
+// length len and capacity cap (store_reserve's doubling rule).
+__device__ inline u64 reserve_words(uint32_t len, uint32_t cap, uint32_t add)
 {
-    const uint32_t lane = lane_id();
-    uint32_t nch = (pdeg + kChunk - 1) / kChunk;
-    if (!nch) return;
-    uint32_t c0 = 0;
-    if (lane == 0) c0 = atomicAdd(n_chunk, nch);
-    c0 = __shfl(c0, 0);
-    for (uint32_t q = lane; q < nch; q += 64) chunks[c0 + q] = ((u64)(q * kChunk) << 32) | k;
+    if (len + add <= cap) return 0;
+    uint32_t ncap = cap * 2;
+    if (ncap < len + add) ncap = len + add;
+    if (ncap < 8) ncap = 8;
+    return ncap;
 }
 
-__global__ void k_w_reset_counters(WCounters *C)
+// Round 1's frontier is the batch's roots in order: root offset ro holds
+// active-list slot ro and touched-list entry ro (k_w_init).
+__global__ void k_w_reset_counters(WCounters *C, uint32_t rsz)
 {
-    C->n_act[0] = C->n_act[1] = C->n_chunk[0] = C->n_chunk[1] = C->n_imp[0] = C->n_imp[1] = 0;
-    C->a_top[0] = C->a_top[1] = 0;
-    C->n_touched = C->n_hub = 0;
-    C->need = 0;
+    C->n_act[0] = C->n_chunk[0] = C->n_chunk[1] = C->n_imp[0] = C->n_imp[1] = 0;
+    C->a_top[0] = 0;
+    C->n_act[1] = C->a_top[1] = rsz;
+    C->n_touched = rsz;
+    C->n_hub = 0;
+    C->need = C->need_bound = 0;
 }
 
 // initialize_tables (WPARA:305-383): every root at distance 0 from itself,
-// active with its own bit.
+// active with its own bit.  One wave per root, one chunk reservation per block.
 __global__ __launch_bounds__(256) void k_w_init(uint32_t rs, uint32_t rsz, uint32_t gen_batch, const uint64_t *off,
                                                 const uint32_t *adj, uint32_t *T, uint32_t *tmark, uint32_t *tlist,
                                                 uint32_t *alist, u64 *ag, uint32_t *amask, uint32_t *aend, u64 *chunks,
-                                                WCounters *C)
+                                                WStore S, WCounters *C)
 {
-    const uint32_t lane = lane_id();
-    const uint32_t ro = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (ro >= rsz) return;
-    const uint32_t r = rs + ro;
-    uint32_t k = 0;
-    if (lane == 0) {
-        T[(uint64_t)r * kWB + ro] = 0;
-        if (atomicExch(&tmark[r], gen_batch) != gen_batch) tlist[atomicAdd(&C->n_touched, 1u)] = r;
-        k = atomicAdd(&C->a_top[1], 1u);  // round 1 -> slot 1
-        atomicAdd(&C->n_act[1], 1u);
+    __shared__ uint32_t s_nch[4], s_c0;
+    const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
+    const uint32_t ro = blockIdx.x * 4 + wv;
+    uint32_t nch = 0;
+    if (ro < rsz) {
+        const uint32_t r = rs + ro;
+        if (lane == 0) {
+            T[(uint64_t)r * kWB + ro] = 0;
+            tmark[r] = gen_batch;
+            tlist[ro] = r;
+            alist[ro] = r;
+            amask[ro] = 1u << (ro >> 6);
+            const u64 nb = reserve_words(S.len[r], S.cap[r], rsz);
+            if (nb) atomicAdd(&C->need_bound, nb);
+        }
+        if (lane < kWW) ag[(uint64_t)ro * kWW + lane] = (lane == (ro >> 6)) ? (1ull << (ro & 63)) : 0ull;
+        const uint32_t pdeg = prefix_above(off, adj, r, rs);
+        if (lane == 0) aend[ro] = pdeg;
+        nch = (pdeg + kChunk - 1) / kChunk;
     }
-    k = __shfl(k, 0);
-    if (lane < kWW) ag[(uint64_t)k * kWW + lane] = (lane == (ro >> 6)) ? (1ull << (ro & 63)) : 0ull;
-    const uint32_t pdeg = prefix_above(off, adj, r, rs);
-    if (lane == 0) {
-        alist[k] = r;
-        amask[k] = 1u << (ro >> 6);
-        aend[k] = pdeg;
+    if (lane == 0) s_nch[wv] = nch;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t tot = s_nch[0] + s_nch[1] + s_nch[2] + s_nch[3];
+        s_c0 = tot ? atomicAdd(&C->n_chunk[1], tot) : 0u;
     }
-    emit_chunks(k, pdeg, &C->n_chunk[1], chunks);
+    __syncthreads();
+    uint32_t c0 = s_c0;
+    for (uint32_t w = 0; w < wv; ++w) c0 += s_nch[w];
+    for (uint32_t q = lane; q < nch; q += 64) chunks[c0 + q] = ((u64)(q * kChunk) << 32) | ro;
 }
 
 // Earlier-batch labels of the batch roots -> hub table (roots_labels_buffer,
@@ -177,7 +194,8 @@ __global__ __launch_bounds__(256) void k_w_hub_mark(uint32_t rs, uint32_t rsz, W
     const u64 base = S.base[r];
     for (uint32_t j = lane; j < len; j += 64) {
         const uint32_t h = (uint32_t)(S.pool[base + j] >> 32);
-        const bool fresh = atomicCAS(&hslot[h], kEmpty, kEmpty - 1) == kEmpty;
+        // A plain read first: once marked, a hub needs no atomic.
+        const bool fresh = hslot[h] == kEmpty && atomicCAS(&hslot[h], kEmpty, kEmpty - 1) == kEmpty;
         const uint32_t hi = wave_append(&C->n_hub, fresh);
         if (fresh) hlist[hi] = h;
     }
@@ -212,7 +230,7 @@ __global__ __launch_bounds__(256) void k_w_hub_fill(uint32_t rs, uint32_t rsz, W
         const u64 lab = S.pool[base + j];
         const uint32_t h = (uint32_t)(lab >> 32), d = (uint32_t)lab;
         dcw[(uint64_t)hslot[h] * kWB + ro] = d;
-        atomicMin(&hmin[h], d);
+        if (d < hmin[h]) atomicMin(&hmin[h], d);
     }
 }
 
@@ -264,12 +282,12 @@ constexpr u64 kWChunkPad = 0xFFFFFFF0ull << 32;
 // lane) and walks the root words of its edges' groups in step; per word the
 // (edge, root) relaxations of all 64 lanes are spread over the wave, 64 per
 // step, so an edge with many roots does not hold up the others.
-__global__ __launch_bounds__(256) void k_w_push(uint32_t round, uint32_t rs, uint32_t gen_batch,
+__global__ __launch_bounds__(256) void k_w_push(uint32_t round, uint32_t rs, uint32_t rsz, uint32_t gen_batch,
                                                 const uint64_t *off, const uint32_t *adj, const uint8_t *wgt,
                                                 const uint32_t *alist, const u64 *ag, const uint32_t *amask,
                                                 const uint32_t *aend, const u64 *chunks, uint32_t *T, u64 *I,
                                                 uint32_t *tmark, uint32_t *tlist, u64 *imark, uint32_t *ilist,
-                                                WCounters *C)
+                                                WStore S, WCounters *C)
 {
     __shared__ u64 s_g[kWPushWaves][64], s_imp[kWPushWaves][64];
     __shared__ uint32_t s_ex[kWPushWaves][64], s_u[kWPushWaves][64], s_t[kWPushWaves][64], s_w[kWPushWaves][64];
@@ -288,6 +306,7 @@ __global__ __launch_bounds__(256) void k_w_push(uint32_t round, uint32_t rs, uin
         C->n_chunk[p ^ 1u] = 0;
     }
     uint32_t n_relax = 0, n_imp = 0;
+    u64 n_bound = 0;
     for (uint32_t c0 = wave * 4; c0 < nc; c0 += nwaves * 4) {
         const uint32_t c = c0 + (lane >> 4);
         uint32_t k = 0, u = 0, t = 0, w = 0, lim = 0, words = 0;
@@ -363,7 +382,10 @@ __global__ __launch_bounds__(256) void k_w_push(uint32_t round, uint32_t rs, uin
         const bool new_t = got && tmark[t] != gen_batch && atomicExch(&tmark[t], gen_batch) != gen_batch;
         const bool new_i = got && imark[t] != gen_round && atomicExch(&imark[t], gen_round) != gen_round;
         const uint32_t ti = wave_append(&C->n_touched, new_t);
-        if (new_t) tlist[ti] = t;
+        if (new_t) {
+            tlist[ti] = t;
+            n_bound += reserve_words(S.len[t], S.cap[t], rsz);
+        }
         const uint32_t ii = wave_append(&C->n_imp[p], new_i);
         if (new_i) {
             ilist[ii] = t;
@@ -372,10 +394,11 @@ __global__ __launch_bounds__(256) void k_w_push(uint32_t round, uint32_t rs, uin
         // s_u / s_t / s_w are rewritten by the next step.
         w_lds_sync();
     }
-    const u64 t_relax = wave_sum(n_relax), t_imp = wave_sum(n_imp);
+    const u64 t_relax = wave_sum(n_relax), t_imp = wave_sum(n_imp), t_bound = wave_sum(n_bound);
     if (lane == 0) {
         if (t_relax) atomicAdd(&C->relax, t_relax);
         if (t_imp) atomicAdd(&C->improved, t_imp);
+        if (t_bound) atomicAdd(&C->need_bound, t_bound);
     }
 }
 
@@ -682,16 +705,7 @@ __global__ __launch_bounds__(256) void k_w_filter(uint32_t rs, const uint32_t *t
         const uint32_t total = (uint32_t)wave_sum((u64)__popc(keep));
         if (!total) continue;
         n_lab += total;
-        if (lane == 0) {
-            // Pool words a doubling reservation may take (store_reserve).
-            const uint32_t len = S.len[v], cap = S.cap[v];
-            if (len + total > cap) {
-                uint32_t ncap = cap * 2;
-                if (ncap < len + total) ncap = len + total;
-                if (ncap < 8) ncap = 8;
-                n_need += ncap;
-            }
-        }
+        if (lane == 0) n_need += reserve_words(S.len[v], S.cap[v], total);
     }
     n_filt = wave_sum(n_filt);
     if (lane == 0) {
@@ -856,9 +870,9 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
     for (uint32_t b = 0; b < nb; ++b) {
         const uint32_t rs = b * batch, rsz = std::min(batch, n - rs), gen_batch = b + 1;
         const uint32_t rb = (rsz * 64 + 255) / 256;
-        hipLaunchKernelGGL(k_w_reset_counters, dim3(1), dim3(1), 0, st, ctr.p);
+        hipLaunchKernelGGL(k_w_reset_counters, dim3(1), dim3(1), 0, st, ctr.p, rsz);
         hipLaunchKernelGGL(k_w_init, dim3(rb), dim3(256), 0, st, rs, rsz, gen_batch, g->off.p, g->adj.p, T.p, tmark.p,
-                           tlist.p, alist.p, ag.p, amask.p, aend.p, chunks.p, ctr.p);
+                           tlist.p, alist.p, ag.p, amask.p, aend.p, chunks.p, S, ctr.p);
         hipLaunchKernelGGL(k_w_hub_mark, dim3(rb), dim3(256), 0, st, rs, rsz, S, hslot.p, hlist.p, ctr.p);
         // The hubs of the batch roots' labels are ids below the batch end, so
         // that many table rows always suffice (no device round trip).
@@ -884,9 +898,9 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
             }
             for (uint32_t u = 0; u < (1u << j); ++u) {
                 const uint32_t round = launched + 1 + u;
-                hipLaunchKernelGGL(k_w_push, dim3(kPushGrid), dim3(256), 0, st, round, rs, gen_batch, g->off.p, g->adj.p,
-                                   g->w.p, alist.p, ag.p, amask.p, aend.p, chunks.p, T.p, I.p, tmark.p, tlist.p,
-                                   imark.p, ilist.p, ctr.p);
+                hipLaunchKernelGGL(k_w_push, dim3(kPushGrid), dim3(256), 0, st, round, rs, rsz, gen_batch, g->off.p,
+                                   g->adj.p, g->w.p, alist.p, ag.p, amask.p, aend.p, chunks.p, T.p, I.p, tmark.p,
+                                   tlist.p, imark.p, ilist.p, S, ctr.p);
                 hipLaunchKernelGGL(k_w_check, dim3(kAddGrid), dim3(256), 0, st, round, rs, ilist.p, g->off.p, g->adj.p,
                                    T.p, I.p, P.p, S, hslot.p, hmin.p, dcw.p, alist.p, ag.p, amask.p, aend.p,
                                    chunks.p, ctr.p);
@@ -902,9 +916,15 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
         hipLaunchKernelGGL(k_w_roots_t, dim3((kWB / 64) * (kWB / 64)), dim3(256), 0, st, rs, rsz, T.p, P.p, RT.p, PT.p);
         hipLaunchKernelGGL(k_w_filter, dim3(2048), dim3(256), 0, st, rs, tlist.p, T.p, P.p, RT.p, PT.p, I.p, S,
                            ctr.p);
-        PADO_HIP(hipMemcpyAsync(h, ctr.p, sizeof(WCounters), hipMemcpyDeviceToHost, st));
-        PADO_HIP(hipStreamSynchronize(st));
-        if (h->pool_top + h->need > pool_size) {
+        // The touched vertices are known once the rounds end: when even the
+        // reservation bound fits the pool, the commit needs no host check.
+        bool grow = h->pool_top + h->need_bound > pool_size;
+        if (grow) {
+            PADO_HIP(hipMemcpyAsync(h, ctr.p, sizeof(WCounters), hipMemcpyDeviceToHost, st));
+            PADO_HIP(hipStreamSynchronize(st));
+            grow = h->pool_top + h->need > pool_size;
+        }
+        if (grow) {
             // Compact every list into a larger pool.  After compaction a list of
             // length L has capacity L + L/4 + 4, so the commit can reserve at
             // most 2.5 * live + new + 8 * touched words.
