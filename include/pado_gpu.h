@@ -351,7 +351,9 @@ pado_gpu_status pado_gpu_index_load(pado_gpu_ctx *ctx, const char *filename, pad
  * labels (hub, u32 distance), no bit-parallel part.  The label sets are the
  * canonical pruned landmark labels -- (r, d(v,r)) is a label of v iff r ranks
  * at or above v and no vertex ranked above r lies on a shortest v-r path --
- * which is what the reference's weighted build produces.
+ * which is what the reference's weighted build produces.  cfg->batch_size
+ * caps the roots per batch at 512 (0 and larger values mean 512); the first
+ * batch takes a quarter of that.
  * ------------------------------------------------------------------------ */
 #define PADO_GPU_WINF 0xFFFFFFFFu
 

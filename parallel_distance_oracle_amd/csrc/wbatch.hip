@@ -865,10 +865,16 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
     } pin{h};
 
     WStore S{pool.p, base.p, len.p, cap.p, pool_size};
-    const uint32_t nb = (n + batch - 1) / batch;
+    // The first batch takes a quarter of the roots: nothing prunes its
+    // searches, and its labels prune the next batch's (measured on weighted
+    // R-MAT 20 / 22: 1.04 -> 0.95 s / 6.5 -> 6.3 s).  Any split of the rank
+    // order into batches yields the same label set.
+    const uint32_t first = std::max<uint32_t>(1u, batch / 4);
     uint32_t prev_rounds = 1;
-    for (uint32_t b = 0; b < nb; ++b) {
-        const uint32_t rs = b * batch, rsz = std::min(batch, n - rs), gen_batch = b + 1;
+    uint32_t rs_next = 0;
+    for (uint32_t b = 0; rs_next < n; ++b) {
+        const uint32_t rs = rs_next, rsz = std::min(b ? batch : first, n - rs), gen_batch = b + 1;
+        rs_next = rs + rsz;
         const uint32_t rb = (rsz * 64 + 255) / 256;
         hipLaunchKernelGGL(k_w_reset_counters, dim3(1), dim3(1), 0, st, ctr.p, rsz);
         hipLaunchKernelGGL(k_w_init, dim3(rb), dim3(256), 0, st, rs, rsz, gen_batch, g->off.p, g->adj.p, T.p, tmark.p,
