@@ -441,79 +441,13 @@ __global__ __launch_bounds__(256) void k_w_check(uint32_t round, uint32_t rs, co
         uint32_t prune = 0;
         const uint32_t len = uniform32(S.len[t]);
         const u64 base = uniform64(S.base[t]);
-        // A tail with few improved pairs (most of them) takes the check
-        // hub-parallel: lanes hold labels and read one table entry per
-        // improved pair, all of a step's reads in flight together; with more,
-        // lanes hold roots (8 each) and read row slices of the passing hubs.
-        // croot / cdv: the improved root offsets and their tentative
-        // distances (wave-uniform); cpr: which are pruned.
-        uint32_t croot[kWGather], cdv[kWGather];
-        uint32_t ncand_t = 0, cpr = 0;
-        {
-            uint32_t rest = imp;
-#pragma unroll
-            for (uint32_t k = 0; k < kWGather; ++k) {
-                const unsigned long long m = __ballot(rest != 0u);
-                croot[k] = 0u;
-                cdv[k] = 0u;
-                if (m) {
-                    const uint32_t l0 = (uint32_t)__ffsll(m) - 1;
-                    const uint32_t bits = uniform32(__shfl(rest, (int)l0));
-                    const uint32_t q = (uint32_t)__ffs(bits) - 1;
-                    // dv[q] of lane l0 (unrolled select: no runtime index into dv)
-                    uint32_t myd = 0;
-#pragma unroll
-                    for (uint32_t qq = 0; qq < 8; ++qq)
-                        if (qq == q) myd = dv[qq];
-                    croot[k] = 8 * l0 + q;
-                    cdv[k] = uniform32(__shfl(myd, (int)l0));
-                    if (lane == l0) rest &= rest - 1;
-                    ++ncand_t;
-                }
-            }
-            if (__ballot(rest != 0u) != 0ull) ncand_t = kWGather + 1;
-        }
-        if (ncand_t <= kWGather) {
-            auto gather_pass = [&](uint32_t d, uint32_t slot, uint32_t hm) {
-                uint32_t x[kWGather];
-#pragma unroll
-                for (uint32_t k = 0; k < kWGather; ++k) {
-                    x[k] = kWInf;
-                    if (k < ncand_t && !((cpr >> k) & 1u) && slot < kEmpty - 1 && (uint64_t)d + hm <= cdv[k])
-                        x[k] = dcw[(uint64_t)slot * kWB + croot[k]];
-                }
-#pragma unroll
-                for (uint32_t k = 0; k < kWGather; ++k)
-                    if (__ballot(x[k] != kWInf && (uint64_t)d + x[k] <= cdv[k]) != 0ull) cpr |= 1u << k;
-            };
-            // 128 labels per step, the next step's labels loaded while this one runs
-            u64 la = lane < len ? S.pool[base + lane] : 0ull, lb = lane + 64 < len ? S.pool[base + lane + 64] : 0ull;
-            for (uint32_t j0 = 0; j0 < len; j0 += 128) {
-                const uint32_t ja = j0 + lane, jb = j0 + 64 + lane;
-                const u64 ca = la, cbl = lb;
-                la = ja + 128 < len ? S.pool[base + ja + 128] : 0ull;
-                lb = jb + 128 < len ? S.pool[base + jb + 128] : 0ull;
-                uint32_t da = kWInf, db = kWInf, sa = kEmpty, sb = kEmpty, ma = kWInf, mb = kWInf;
-                if (ja < len) {
-                    const uint32_t h = (uint32_t)(ca >> 32);
-                    da = (uint32_t)ca;
-                    sa = hslot[h];
-                    ma = hmin[h];
-                }
-                if (jb < len) {
-                    const uint32_t h = (uint32_t)(cbl >> 32);
-                    db = (uint32_t)cbl;
-                    sb = hslot[h];
-                    mb = hmin[h];
-                }
-                gather_pass(da, sa, ma);
-                gather_pass(db, sb, mb);
-                if (cpr == (1u << ncand_t) - 1u) break;
-            }
-#pragma unroll
-            for (uint32_t k = 0; k < kWGather; ++k)
-                if (((cpr >> k) & 1u) && lane == (croot[k] >> 3)) prune |= 1u << (croot[k] & 7u);
-        } else {
+        // Row phase: while more than kWGather improved pairs are unpruned,
+        // lanes hold roots (8 each) and read the passing hubs' row slices.
+        // The first hubs usually prune most pairs; the labels after the step
+        // where at most kWGather remain go to the gather phase.
+        uint32_t j_gather = 0;
+        if (wave_sum((u64)__popc(imp)) > kWGather) {
+            j_gather = len;
             for (uint32_t j0 = 0; j0 < len; j0 += 64) {
                 const uint32_t j = j0 + lane;
                 uint32_t h = 0, d = kWInf, slot = kEmpty;
@@ -559,9 +493,84 @@ __global__ __launch_bounds__(256) void k_w_check(uint32_t round, uint32_t rs, co
                             if (((imp >> q) & 1u) && x[q] != kWInf && (uint64_t)dd[u] + x[q] <= dv[q]) prune |= 1u << q;
                     }
                 }
-                // Every improved pair pruned: the rest of the labels cannot matter.
-                if (__ballot((imp & ~prune) != 0u) == 0ull) break;
+                if (wave_sum((u64)__popc(imp & ~prune)) <= kWGather) {
+                    j_gather = j0 + 64 < len ? j0 + 64 : len;
+                    break;
+                }
             }
+        }
+        // Gather phase: the (at most kWGather) improved pairs still unpruned
+        // are checked hub-parallel over labels j_gather.. : lanes hold labels
+        // and read one table entry per pair, 128 labels per step with the
+        // next step's labels loaded while this one runs.  croot / cdv: the
+        // pairs' root offsets and tentative distances (wave-uniform); cpr:
+        // which are pruned.
+        uint32_t croot[kWGather], cdv[kWGather];
+        uint32_t ncand_t = 0, cpr = 0;
+        {
+            uint32_t rest = imp & ~prune;
+#pragma unroll
+            for (uint32_t k = 0; k < kWGather; ++k) {
+                const unsigned long long m = __ballot(rest != 0u);
+                croot[k] = 0u;
+                cdv[k] = 0u;
+                if (m) {
+                    const uint32_t l0 = (uint32_t)__ffsll(m) - 1;
+                    const uint32_t bits = uniform32(__shfl(rest, (int)l0));
+                    const uint32_t q = (uint32_t)__ffs(bits) - 1;
+                    // dv[q] of lane l0 (unrolled select: no runtime index into dv)
+                    uint32_t myd = 0;
+#pragma unroll
+                    for (uint32_t qq = 0; qq < 8; ++qq)
+                        if (qq == q) myd = dv[qq];
+                    croot[k] = 8 * l0 + q;
+                    cdv[k] = uniform32(__shfl(myd, (int)l0));
+                    if (lane == l0) rest &= rest - 1;
+                    ++ncand_t;
+                }
+            }
+        }
+        if (ncand_t && j_gather < len) {
+            auto gather_pass = [&](uint32_t d, uint32_t slot, uint32_t hm) {
+                uint32_t x[kWGather];
+#pragma unroll
+                for (uint32_t k = 0; k < kWGather; ++k) {
+                    x[k] = kWInf;
+                    if (k < ncand_t && !((cpr >> k) & 1u) && slot < kEmpty - 1 && (uint64_t)d + hm <= cdv[k])
+                        x[k] = dcw[(uint64_t)slot * kWB + croot[k]];
+                }
+#pragma unroll
+                for (uint32_t k = 0; k < kWGather; ++k)
+                    if (__ballot(x[k] != kWInf && (uint64_t)d + x[k] <= cdv[k]) != 0ull) cpr |= 1u << k;
+            };
+            const u64 gb = base + j_gather;
+            const uint32_t glen = len - j_gather;
+            u64 la = lane < glen ? S.pool[gb + lane] : 0ull, lb = lane + 64 < glen ? S.pool[gb + lane + 64] : 0ull;
+            for (uint32_t j0 = 0; j0 < glen; j0 += 128) {
+                const uint32_t ja = j0 + lane, jb = j0 + 64 + lane;
+                const u64 ca = la, cbl = lb;
+                la = ja + 128 < glen ? S.pool[gb + ja + 128] : 0ull;
+                lb = jb + 128 < glen ? S.pool[gb + jb + 128] : 0ull;
+                uint32_t da = kWInf, db = kWInf, sa = kEmpty, sb = kEmpty, ma = kWInf, mb = kWInf;
+                if (ja < glen) {
+                    const uint32_t h = (uint32_t)(ca >> 32);
+                    da = (uint32_t)ca;
+                    sa = hslot[h];
+                    ma = hmin[h];
+                }
+                if (jb < glen) {
+                    const uint32_t h = (uint32_t)(cbl >> 32);
+                    db = (uint32_t)cbl;
+                    sb = hslot[h];
+                    mb = hmin[h];
+                }
+                gather_pass(da, sa, ma);
+                gather_pass(db, sb, mb);
+                if (cpr == (1u << ncand_t) - 1u) break;
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < kWGather; ++k)
+                if (((cpr >> k) & 1u) && lane == (croot[k] >> 3)) prune |= 1u << (croot[k] & 7u);
         }
         const uint32_t live = imp & ~prune;
         n_pruned += __popc(imp & prune);
