@@ -462,6 +462,7 @@ __global__ __launch_bounds__(256) void k_w_check(uint32_t round, uint32_t rs, co
                 // Up to four passing hubs per step: their row loads go out
                 // together (32 B per lane each), and only lanes that still hold an
                 // unpruned improved pair load.
+                bool few = false;
                 while (m) {
                     int src[4];
                     bool ok[4];
@@ -492,9 +493,15 @@ __global__ __launch_bounds__(256) void k_w_check(uint32_t round, uint32_t rs, co
                         for (int q = 0; q < 8; ++q)
                             if (((imp >> q) & 1u) && x[q] != kWInf && (uint64_t)dd[u] + x[q] <= dv[q]) prune |= 1u << q;
                     }
+                    if (wave_sum((u64)__popc(imp & ~prune)) <= kWGather) {
+                        few = true;
+                        break;
+                    }
                 }
-                if (wave_sum((u64)__popc(imp & ~prune)) <= kWGather) {
-                    j_gather = j0 + 64 < len ? j0 + 64 : len;
+                if (few) {
+                    // the step's passing hubs not yet read start the gather phase
+                    const uint32_t nxt = m ? j0 + (uint32_t)__ffsll(m) - 1 : j0 + 64;
+                    j_gather = nxt < len ? nxt : len;
                     break;
                 }
             }
