@@ -192,10 +192,15 @@ __global__ __launch_bounds__(256) void k_w_hub_mark(uint32_t rs, uint32_t rsz, W
     const uint32_t r = rs + ro;
     const uint32_t len = S.len[r];
     const u64 base = S.base[r];
-    for (uint32_t j = lane; j < len; j += 64) {
-        const uint32_t h = (uint32_t)(S.pool[base + j] >> 32);
-        // A plain read first: once marked, a hub needs no atomic.
-        const bool fresh = hslot[h] == kEmpty && atomicCAS(&hslot[h], kEmpty, kEmpty - 1) == kEmpty;
+    // Lists are hub-sorted and every root holds the top-ranked hubs: walking
+    // them from the end staggers the roots' arrival at those hubs, so the
+    // plain read finds most of them marked (a plain read first: once marked,
+    // a hub needs no atomic).
+    for (uint32_t j0 = (len + 63) & ~63u; j0 > 0; j0 -= 64) {
+        const uint32_t j = j0 - 64 + lane;
+        uint32_t h = 0;
+        if (j < len) h = (uint32_t)(S.pool[base + j] >> 32);
+        const bool fresh = j < len && hslot[h] == kEmpty && atomicCAS(&hslot[h], kEmpty, kEmpty - 1) == kEmpty;
         const uint32_t hi = wave_append(&C->n_hub, fresh);
         if (fresh) hlist[hi] = h;
     }
@@ -226,7 +231,10 @@ __global__ __launch_bounds__(256) void k_w_hub_fill(uint32_t rs, uint32_t rsz, W
     const uint32_t r = rs + ro;
     const uint32_t len = S.len[r];
     const u64 base = S.base[r];
-    for (uint32_t j = lane; j < len; j += 64) {
+    // From the end of the list, as in k_w_hub_mark.
+    for (uint32_t j0 = (len + 63) & ~63u; j0 > 0; j0 -= 64) {
+        const uint32_t j = j0 - 64 + lane;
+        if (j >= len) continue;
         const u64 lab = S.pool[base + j];
         const uint32_t h = (uint32_t)(lab >> 32), d = (uint32_t)lab;
         dcw[(uint64_t)hslot[h] * kWB + ro] = d;
