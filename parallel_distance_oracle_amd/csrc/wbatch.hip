@@ -202,22 +202,9 @@ __global__ __launch_bounds__(256) void k_w_hub_mark(uint32_t rs, uint32_t rsz, W
         if (j < len) h = (uint32_t)(S.pool[base + j] >> 32);
         const bool fresh = j < len && hslot[h] == kEmpty && atomicCAS(&hslot[h], kEmpty, kEmpty - 1) == kEmpty;
         const uint32_t hi = wave_append(&C->n_hub, fresh);
-        if (fresh) hlist[hi] = h;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_w_hub_slots(const uint32_t *hlist, uint32_t *hslot, uint32_t *hmin,
-                                                     uint32_t *dcw, const WCounters *C)
-{
-    const uint32_t nh = C->n_hub;
-    const uint64_t total = (uint64_t)nh * (kWB / 4);
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
-        const uint32_t slot = (uint32_t)(i / (kWB / 4)), part = (uint32_t)(i % (kWB / 4));
-        reinterpret_cast<uint4 *>(dcw + (uint64_t)slot * kWB)[part] = make_uint4(kWInf, kWInf, kWInf, kWInf);
-        if (part == 0) {
-            hslot[hlist[slot]] = slot;
-            hmin[hlist[slot]] = kWInf;
+        if (fresh) {
+            hlist[hi] = h;
+            hslot[h] = hi;  // table row hi: all INF (k_w_batch_end), as is hmin[h]
         }
     }
 }
@@ -274,6 +261,9 @@ __device__ inline void w_lds_sync()
 }
 
 constexpr uint32_t kWPushWaves = 4;  // waves per k_w_push block
+
+// Touched vertices up to which k_w_filter reads the roots' entries in place.
+constexpr uint32_t kWDirectFilter = 4096;
 
 // Improved pairs of a tail up to which k_w_check reads single table entries.
 constexpr uint32_t kWGather = 8;
@@ -671,10 +661,13 @@ __global__ __launch_bounds__(256) void k_w_roots_t(uint32_t rs, uint32_t rsz, co
 // pairs (finite, not pruned) are dropped if an earlier batch root h witnesses
 // S(v,h) + S(r,h) <= S(v,r).  Reads T/P only, so every wave sees the same
 // pre-filter state; the kept bitmap goes to I and the pool words the commit
-// may reserve are summed into C->need.
-__global__ __launch_bounds__(256) void k_w_filter(uint32_t rs, const uint32_t *tlist, const uint32_t *T,
-                                                  const u64 *P, const uint32_t *RT, const u64 *PT, u64 *I, WStore S,
-                                                  WCounters *C)
+// may reserve are summed into C->need.  kDirect: the roots' entries come from
+// T/P rows directly (8 scattered reads per witness and lane, no transpose
+// kernel; for batches that touch few vertices), else from RT/PT.
+template <bool kDirect>
+__global__ __launch_bounds__(256) void k_w_filter(uint32_t rs, uint32_t rsz, const uint32_t *tlist,
+                                                  const uint32_t *T, const u64 *P, const uint32_t *RT, const u64 *PT,
+                                                  u64 *I, WStore S, WCounters *C)
 {
     __shared__ uint8_t sv[4][64];
     const uint32_t lane = lane_id();
@@ -707,10 +700,25 @@ __global__ __launch_bounds__(256) void k_w_filter(uint32_t rs, const uint32_t *t
                 if (h >= lane * 8 + 7) break;  // only roots r > h of this lane can use h
                 const uint32_t dvh = T[(uint64_t)v * kWB + h];
                 // Column h of the roots' entries, 8 consecutive roots per lane.
-                const uint4 *rt = reinterpret_cast<const uint4 *>(RT + (uint64_t)h * kWB + lane * 8);
-                const uint4 ra = rt[0], rb = rt[1];
-                const uint32_t drh[8] = {ra.x, ra.y, ra.z, ra.w, rb.x, rb.y, rb.z, rb.w};
-                const uint32_t prh = reinterpret_cast<const uint8_t *>(PT + (uint64_t)h * kWW)[lane];
+                uint32_t drh[8];
+                uint32_t prh = 0;
+                if constexpr (kDirect) {
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        const uint32_t ro = lane * 8 + q;
+                        drh[q] = kWInf;
+                        if (((surv >> q) & 1u) && ro > h && ro < rsz) {
+                            drh[q] = T[(uint64_t)(rs + ro) * kWB + h];
+                            prh |= (uint32_t)((P[(uint64_t)(rs + ro) * kWW + h / 64] >> (h % 64)) & 1ull) << q;
+                        }
+                    }
+                } else {
+                    const uint4 *rt = reinterpret_cast<const uint4 *>(RT + (uint64_t)h * kWB + lane * 8);
+                    const uint4 ra = rt[0], rb = rt[1];
+                    drh[0] = ra.x, drh[1] = ra.y, drh[2] = ra.z, drh[3] = ra.w;
+                    drh[4] = rb.x, drh[5] = rb.y, drh[6] = rb.z, drh[7] = rb.w;
+                    prh = reinterpret_cast<const uint8_t *>(PT + (uint64_t)h * kWW)[lane];
+                }
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
                     const uint32_t ro = lane * 8 + q;
@@ -776,7 +784,8 @@ __global__ __launch_bounds__(256) void k_w_commit(uint32_t rs, const uint32_t *t
 
 // reset_tables (WPARA:918-956) for touched rows and the hub table.
 __global__ __launch_bounds__(256) void k_w_batch_end(const uint32_t *tlist, uint32_t *T, u64 *P, uint32_t *hslot,
-                                                     const uint32_t *hlist, const WCounters *C)
+                                                     uint32_t *hmin, uint32_t *dcw, const uint32_t *hlist,
+                                                     const WCounters *C)
 {
     const uint32_t nt = C->n_touched, nh = C->n_hub;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -786,7 +795,14 @@ __global__ __launch_bounds__(256) void k_w_batch_end(const uint32_t *tlist, uint
         reinterpret_cast<uint4 *>(T + (uint64_t)v * kWB)[part] = make_uint4(kWInf, kWInf, kWInf, kWInf);
         if (part < kWW) P[(uint64_t)v * kWW + part] = 0ull;
     }
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh; i += stride) hslot[hlist[i]] = kEmpty;
+    // The hub table back to empty: slots, minima and the used rows.
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh; i += stride) {
+        hslot[hlist[i]] = kEmpty;
+        hmin[hlist[i]] = kWInf;
+    }
+    const uint64_t rows = (uint64_t)nh * (kWB / 4);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += stride)
+        reinterpret_cast<uint4 *>(dcw)[i] = make_uint4(kWInf, kWInf, kWInf, kWInf);
 }
 
 __global__ void k_w_fill(uint32_t *p, uint64_t count, uint32_t val)
@@ -866,12 +882,14 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
     uint32_t dc_slots = std::min<uint32_t>(n, 1u << 14);
     PADO_TRY(dcw.alloc((uint64_t)dc_slots * kWB));
     const uint32_t fg = 4096;
+    hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, dcw.p, (uint64_t)dc_slots * kWB, kWInf);
     hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, T.p, (uint64_t)n * kWB, kWInf);
     hipLaunchKernelGGL(k_w_zero64, dim3(fg), dim3(256), 0, st, P.p, (uint64_t)n * kWW);
     hipLaunchKernelGGL(k_w_zero64, dim3(fg), dim3(256), 0, st, I.p, (uint64_t)n * kWW);
     hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, tmark.p, (uint64_t)n, 0u);
     hipLaunchKernelGGL(k_w_zero64, dim3(fg), dim3(256), 0, st, imark.p, (uint64_t)n);
     hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, hslot.p, (uint64_t)n, kEmpty);
+    hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, hmin.p, (uint64_t)n, kWInf);
     hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, len.p, (uint64_t)n, 0u);
     hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, cap.p, (uint64_t)n, 0u);
     hipLaunchKernelGGL(k_w_zero64, dim3(fg), dim3(256), 0, st, base.p, (uint64_t)n);
@@ -909,8 +927,8 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
         if (rs + rsz > dc_slots) {
             dc_slots = std::min<uint32_t>(n, std::max<uint32_t>(rs + rsz, dc_slots * 2));
             PADO_TRY(dcw.alloc((uint64_t)dc_slots * kWB));
+            hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, dcw.p, (uint64_t)dc_slots * kWB, kWInf);
         }
-        hipLaunchKernelGGL(k_w_hub_slots, dim3(2048), dim3(256), 0, st, hlist.p, hslot.p, hmin.p, dcw.p, ctr.p);
         hipLaunchKernelGGL(k_w_hub_fill, dim3(rb), dim3(256), 0, st, rs, rsz, S, hslot.p, hmin.p, dcw.p);
         // Blocks of 2^j rounds per host check; j follows the previous batch's
         // round count and doubles while a batch runs long.
@@ -943,9 +961,17 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
         }
         prev_rounds = (uint32_t)(h->rounds - rounds_before);
         const u64 labels_before = h->labels;
-        hipLaunchKernelGGL(k_w_roots_t, dim3((kWB / 64) * (kWB / 64)), dim3(256), 0, st, rs, rsz, T.p, P.p, RT.p, PT.p);
-        hipLaunchKernelGGL(k_w_filter, dim3(2048), dim3(256), 0, st, rs, tlist.p, T.p, P.p, RT.p, PT.p, I.p, S,
-                           ctr.p);
+        // Few touched vertices: the filter reads the roots' entries in place
+        // instead of transposing them first.
+        if (h->n_touched <= kWDirectFilter) {
+            hipLaunchKernelGGL(k_w_filter<true>, dim3(2048), dim3(256), 0, st, rs, rsz, tlist.p, T.p, P.p, RT.p, PT.p,
+                               I.p, S, ctr.p);
+        } else {
+            hipLaunchKernelGGL(k_w_roots_t, dim3((kWB / 64) * (kWB / 64)), dim3(256), 0, st, rs, rsz, T.p, P.p, RT.p,
+                               PT.p);
+            hipLaunchKernelGGL(k_w_filter<false>, dim3(2048), dim3(256), 0, st, rs, rsz, tlist.p, T.p, P.p, RT.p,
+                               PT.p, I.p, S, ctr.p);
+        }
         // The touched vertices are known once the rounds end: when even the
         // reservation bound fits the pool, the commit needs no host check.
         bool grow = h->pool_top + h->need_bound > pool_size;
@@ -988,7 +1014,8 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
             PADO_HIP(copy_sync(&ctr.p->pool_top, &top, 8, hipMemcpyHostToDevice, st));
         }
         hipLaunchKernelGGL(k_w_commit, dim3(2048), dim3(256), 0, st, rs, tlist.p, T.p, I.p, S, ctr.p);
-        hipLaunchKernelGGL(k_w_batch_end, dim3(2048), dim3(256), 0, st, tlist.p, T.p, P.p, hslot.p, hlist.p, ctr.p);
+        hipLaunchKernelGGL(k_w_batch_end, dim3(2048), dim3(256), 0, st, tlist.p, T.p, P.p, hslot.p, hmin.p, dcw.p,
+                           hlist.p, ctr.p);
     }
     PADO_HIP(hipMemcpyAsync(h, ctr.p, sizeof(WCounters), hipMemcpyDeviceToHost, st));
     PADO_HIP(hipStreamSynchronize(st));
