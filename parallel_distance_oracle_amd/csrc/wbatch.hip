@@ -930,8 +930,9 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
             hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, dcw.p, (uint64_t)dc_slots * kWB, kWInf);
         }
         hipLaunchKernelGGL(k_w_hub_fill, dim3(rb), dim3(256), 0, st, rs, rsz, S, hslot.p, hmin.p, dcw.p);
-        // Blocks of 2^j rounds per host check; j follows the previous batch's
-        // round count and doubles while a batch runs long.
+        // Rounds per host check: the previous batch's round count + 1 first
+        // (enough when this batch takes no more), then blocks of 2^j that
+        // double while a batch runs long.
         // Round 1's frontier is the batch's roots (never empty); the counters
         // seen last (after the previous batch's filter) hold every round so far.
         uint32_t live = rsz;
@@ -940,11 +941,12 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
         int j = 0;
         while ((1u << j) < prev_rounds + 1 && j < 6) ++j;
         while (live) {
-            if (launched + (1u << j) >= (1u << 24)) {
+            const uint32_t blk = launched ? 1u << j : std::min<uint32_t>(prev_rounds + 1, 64u);
+            if (launched + blk >= (1u << 24)) {
                 set_error("a weighted batch needs more than 2^24 Bellman-Ford rounds");
                 return PADO_GPU_ERR_DOMAIN;
             }
-            for (uint32_t u = 0; u < (1u << j); ++u) {
+            for (uint32_t u = 0; u < blk; ++u) {
                 const uint32_t round = launched + 1 + u;
                 hipLaunchKernelGGL(k_w_push, dim3(kPushGrid), dim3(256), 0, st, round, rs, rsz, gen_batch, g->off.p,
                                    g->adj.p, g->w.p, alist.p, ag.p, amask.p, aend.p, chunks.p, T.p, I.p, tmark.p,
@@ -953,7 +955,7 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
                                    T.p, I.p, P.p, S, hslot.p, hmin.p, dcw.p, alist.p, ag.p, amask.p, aend.p,
                                    chunks.p, ctr.p);
             }
-            launched += 1u << j;
+            launched += blk;
             PADO_HIP(hipMemcpyAsync(h, ctr.p, sizeof(WCounters), hipMemcpyDeviceToHost, st));
             PADO_HIP(hipStreamSynchronize(st));
             live = h->n_act[(launched + 1) & 1u];  // frontier of the next round
