@@ -589,7 +589,7 @@ __device__ __forceinline__ void batch_init_body(uint32_t bid, uint32_t nblk, con
                                                      const unsigned long long *lmask, unsigned long long *rmask,
                                                      BuildCounters *C)
 {
-    __shared__ uint32_t s_nch[16];
+    __shared__ uint32_t s_nch[16], s_pdeg[16];
     __shared__ uint32_t s_base;
     if (C->hold) return;
     const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
@@ -629,20 +629,27 @@ __device__ __forceinline__ void batch_init_body(uint32_t bid, uint32_t nblk, con
             }
             if (lane < kWords) ag[(uint64_t)r * kWords + lane] = (lane == (ro >> 6)) ? (1ull << (ro & 63)) : 0ull;
             pdeg = push_degree(off, adj, r, rs);
-            if (lane == 0) {
-                aend[r] = pdeg;
-                if (pdeg) atomicAdd(&C->n_pedge[1], (unsigned long long)pdeg);
-            }
+            if (lane == 0) aend[r] = pdeg;
         }
     }
     const uint32_t nch = (pdeg + CH - 1) / CH;
-    if (lane == 0) s_nch[wv] = nch;
+    if (lane == 0) {
+        s_nch[wv] = nch;
+        s_pdeg[wv] = pdeg;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         uint32_t tot = 0;
-        for (uint32_t w = 0; w < blockDim.x / 64; ++w) tot += s_nch[w];
+        unsigned long long tot_e = 0;
+        for (uint32_t w = 0; w < blockDim.x / 64; ++w) {
+            tot += s_nch[w];
+            tot_e += s_pdeg[w];
+        }
         // The frontier of iteration i is counted in slot i & 1: iteration 1 -> 1.
+        // One update per block of each shared counter (every root would
+        // otherwise queue on the same two addresses).
         s_base = tot ? atomicAdd(&C->n_chunk[1], tot) : 0u;
+        if (tot_e) atomicAdd(&C->n_pedge[1], tot_e);
     }
     __syncthreads();
     if (nch) {
