@@ -29,7 +29,7 @@
 //
 // Layout per batch (B = 512 roots): T[n][B] u32 tentative distances, P and I
 // bitmaps (pruned at the current value / improved this round) [n][B/64] u64,
-// the earlier-batch hub table (hslot, hmin, dcw[slot][B] u32), and the
+// the earlier-batch root-distance table (hmin[n], dcw[hub][B] u32), and the
 // frontier as (vertex, root bitmap) entries with 16-edge push chunks.  Labels
 // are packed u64 (hub << 32 | dist) in per-vertex growable arrays.
 #include <cstring>
@@ -181,10 +181,11 @@ __global__ __launch_bounds__(256) void k_w_init(uint32_t rs, uint32_t rsz, uint3
     for (uint32_t q = lane; q < nch; q += 64) chunks[c0 + q] = ((u64)(q * kChunk) << 32) | ro;
 }
 
-// Earlier-batch labels of the batch roots -> hub table (roots_labels_buffer,
-// WPARA:322-348, transposed and compacted to the hubs that occur).
-__global__ __launch_bounds__(256) void k_w_hub_mark(uint32_t rs, uint32_t rsz, WStore S, uint32_t *hslot,
-                                                    uint32_t *hlist, WCounters *C)
+// Earlier-batch labels of the batch roots -> root-distance table
+// (roots_labels_buffer, WPARA:322-348, transposed): dcw[h][ro] = d(root
+// rs + ro, hub h), indexed by hub id, and hmin[h] = the least of them.  Both
+// are INF outside the batch roots' hubs (k_w_commit resets what was written).
+__global__ __launch_bounds__(256) void k_w_hub_fill(uint32_t rs, uint32_t rsz, WStore S, uint32_t *hmin, uint32_t *dcw)
 {
     const uint32_t lane = lane_id();
     const uint32_t ro = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -194,37 +195,13 @@ __global__ __launch_bounds__(256) void k_w_hub_mark(uint32_t rs, uint32_t rsz, W
     const u64 base = S.base[r];
     // Lists are hub-sorted and every root holds the top-ranked hubs: walking
     // them from the end staggers the roots' arrival at those hubs, so the
-    // plain read finds most of them marked (a plain read first: once marked,
-    // a hub needs no atomic).
-    for (uint32_t j0 = (len + 63) & ~63u; j0 > 0; j0 -= 64) {
-        const uint32_t j = j0 - 64 + lane;
-        uint32_t h = 0;
-        if (j < len) h = (uint32_t)(S.pool[base + j] >> 32);
-        const bool fresh = j < len && hslot[h] == kEmpty && atomicCAS(&hslot[h], kEmpty, kEmpty - 1) == kEmpty;
-        const uint32_t hi = wave_append(&C->n_hub, fresh);
-        if (fresh) {
-            hlist[hi] = h;
-            hslot[h] = hi;  // table row hi: all INF (k_w_batch_end), as is hmin[h]
-        }
-    }
-}
-
-__global__ __launch_bounds__(256) void k_w_hub_fill(uint32_t rs, uint32_t rsz, WStore S, const uint32_t *hslot,
-                                                    uint32_t *hmin, uint32_t *dcw)
-{
-    const uint32_t lane = lane_id();
-    const uint32_t ro = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (ro >= rsz) return;
-    const uint32_t r = rs + ro;
-    const uint32_t len = S.len[r];
-    const u64 base = S.base[r];
-    // From the end of the list, as in k_w_hub_mark.
+    // plain read of hmin skips most atomics.
     for (uint32_t j0 = (len + 63) & ~63u; j0 > 0; j0 -= 64) {
         const uint32_t j = j0 - 64 + lane;
         if (j >= len) continue;
         const u64 lab = S.pool[base + j];
         const uint32_t h = (uint32_t)(lab >> 32), d = (uint32_t)lab;
-        dcw[(uint64_t)hslot[h] * kWB + ro] = d;
+        dcw[(uint64_t)h * kWB + ro] = d;
         if (d < hmin[h]) atomicMin(&hmin[h], d);
     }
 }
@@ -405,7 +382,7 @@ __global__ __launch_bounds__(256) void k_w_push(uint32_t round, uint32_t rs, uin
 // stop propagating; the others become the tail's next frontier bitmap.
 __global__ __launch_bounds__(256) void k_w_check(uint32_t round, uint32_t rs, const uint32_t *ilist, const uint64_t *off,
                                                  const uint32_t *adj, uint32_t *T, u64 *I, u64 *P, WStore S,
-                                                 const uint32_t *hslot, const uint32_t *hmin, const uint32_t *dcw,
+                                                 const uint32_t *hmin, const uint32_t *dcw,
                                                  uint32_t *alist, u64 *ag, uint32_t *amask, uint32_t *aend,
                                                  u64 *chunks, WCounters *C)
 {
@@ -448,14 +425,14 @@ __global__ __launch_bounds__(256) void k_w_check(uint32_t round, uint32_t rs, co
             j_gather = len;
             for (uint32_t j0 = 0; j0 < len; j0 += 64) {
                 const uint32_t j = j0 + lane;
-                uint32_t h = 0, d = kWInf, slot = kEmpty;
+                uint32_t h = 0, d = kWInf;
                 if (j < len) {
                     const u64 lab = S.pool[base + j];
                     h = (uint32_t)(lab >> 32);
                     d = (uint32_t)lab;
-                    slot = hslot[h];
                 }
-                const bool pass = slot < kEmpty - 1 && (uint64_t)d + hmin[h] <= wmax;
+                // hmin is INF for hubs no batch root holds
+                const bool pass = j < len && (uint64_t)d + hmin[h] <= wmax;
                 u64 m = __ballot(pass);
                 // Up to four passing hubs per step: their row loads go out
                 // together (32 B per lane each), and only lanes that still hold an
@@ -476,7 +453,7 @@ __global__ __launch_bounds__(256) void k_w_check(uint32_t round, uint32_t rs, co
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         dd[u] = __shfl(d, src[u]);
-                        const uint32_t sl = __shfl(slot, src[u]);
+                        const uint32_t sl = __shfl(h, src[u]);
                         r0[u] = r1[u] = make_uint4(kWInf, kWInf, kWInf, kWInf);
                         if (ok[u] && need) {
                             const uint4 *row = reinterpret_cast<const uint4 *>(dcw + (uint64_t)sl * kWB + lane * 8);
@@ -536,13 +513,13 @@ __global__ __launch_bounds__(256) void k_w_check(uint32_t round, uint32_t rs, co
             }
         }
         if (ncand_t && j_gather < len) {
-            auto gather_pass = [&](uint32_t d, uint32_t slot, uint32_t hm) {
+            auto gather_pass = [&](uint32_t d, uint32_t h, uint32_t hm) {
                 uint32_t x[kWGather];
 #pragma unroll
                 for (uint32_t k = 0; k < kWGather; ++k) {
                     x[k] = kWInf;
-                    if (k < ncand_t && !((cpr >> k) & 1u) && slot < kEmpty - 1 && (uint64_t)d + hm <= cdv[k])
-                        x[k] = dcw[(uint64_t)slot * kWB + croot[k]];
+                    if (k < ncand_t && !((cpr >> k) & 1u) && (uint64_t)d + hm <= cdv[k])
+                        x[k] = dcw[(uint64_t)h * kWB + croot[k]];
                 }
 #pragma unroll
                 for (uint32_t k = 0; k < kWGather; ++k)
@@ -556,18 +533,17 @@ __global__ __launch_bounds__(256) void k_w_check(uint32_t round, uint32_t rs, co
                 const u64 ca = la, cbl = lb;
                 la = ja + 128 < glen ? S.pool[gb + ja + 128] : 0ull;
                 lb = jb + 128 < glen ? S.pool[gb + jb + 128] : 0ull;
-                uint32_t da = kWInf, db = kWInf, sa = kEmpty, sb = kEmpty, ma = kWInf, mb = kWInf;
+                // (past the list: d = hmin = INF, so nothing is read)
+                uint32_t da = kWInf, db = kWInf, sa = 0, sb = 0, ma = kWInf, mb = kWInf;
                 if (ja < glen) {
-                    const uint32_t h = (uint32_t)(ca >> 32);
+                    sa = (uint32_t)(ca >> 32);
                     da = (uint32_t)ca;
-                    sa = hslot[h];
-                    ma = hmin[h];
+                    ma = hmin[sa];
                 }
                 if (jb < glen) {
-                    const uint32_t h = (uint32_t)(cbl >> 32);
+                    sb = (uint32_t)(cbl >> 32);
                     db = (uint32_t)cbl;
-                    sb = hslot[h];
-                    mb = hmin[h];
+                    mb = hmin[sb];
                 }
                 gather_pass(da, sa, ma);
                 gather_pass(db, sb, mb);
@@ -750,8 +726,12 @@ __global__ __launch_bounds__(256) void k_w_filter(uint32_t rs, uint32_t rsz, con
 // update_index (WPARA:961-990): append the kept pairs of each touched vertex,
 // hubs ascending (batches are committed in root order and each lane's roots
 // follow the lower lanes'), then clear the carried bitmap.
-__global__ __launch_bounds__(256) void k_w_commit(uint32_t rs, const uint32_t *tlist, const uint32_t *T, u64 *I,
-                                                  WStore S, WCounters *C)
+// update_index (WPARA:961-990) and reset_tables (WPARA:918-956) in one pass
+// over the touched vertices: each vertex's wave appends its kept pairs and
+// then empties its T / P rows; the batch roots (touched-list entries
+// 0..rsz-1) first clear the root-distance entries their labels wrote.
+__global__ __launch_bounds__(256) void k_w_commit(uint32_t rs, uint32_t rsz, const uint32_t *tlist, uint32_t *T, u64 *P,
+                                                  u64 *I, uint32_t *hmin, uint32_t *dcw, WStore S, WCounters *C)
 {
     const uint32_t lane = lane_id();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -759,6 +739,16 @@ __global__ __launch_bounds__(256) void k_w_commit(uint32_t rs, const uint32_t *t
     const uint32_t nt = C->n_touched;
     for (uint32_t i = wave; i < nt; i += nwaves) {
         const uint32_t v = tlist[i];
+        if (i < rsz) {
+            // root offset i: its labels are those k_w_hub_fill entered
+            const uint32_t len = S.len[v];
+            const u64 base = S.base[v];
+            for (uint32_t j = lane; j < len; j += 64) {
+                const uint32_t h = (uint32_t)(S.pool[base + j] >> 32);
+                dcw[(uint64_t)h * kWB + i] = kWInf;
+                hmin[h] = kWInf;
+            }
+        }
         uint8_t *Kb = reinterpret_cast<uint8_t *>(I + (uint64_t)v * kWW);
         const uint32_t keep = Kb[lane];
         const uint32_t cnt = __popc(keep);
@@ -768,41 +758,24 @@ __global__ __launch_bounds__(256) void k_w_commit(uint32_t rs, const uint32_t *t
             if (lane >= (uint32_t)o) incl += y;
         }
         const uint32_t total = __shfl(incl, 63);
-        if (!total) continue;
-        if (keep) Kb[lane] = 0;
-        const uint32_t len = S.len[v];
-        const u64 nb = store_reserve(S, C, v, len, total);
-        if (nb == ~0ull) continue;
-        uint32_t pos = len + incl - cnt;
-        for (uint32_t kb = keep; kb; kb &= kb - 1) {
-            const uint32_t q = __ffs(kb) - 1;
-            S.pool[nb + pos++] = ((u64)(rs + lane * 8 + q) << 32) | T[(uint64_t)v * kWB + lane * 8 + q];
+        if (total) {
+            if (keep) Kb[lane] = 0;
+            const uint32_t len = S.len[v];
+            const u64 nb = store_reserve(S, C, v, len, total);
+            if (nb != ~0ull) {
+                uint32_t pos = len + incl - cnt;
+                for (uint32_t kb = keep; kb; kb &= kb - 1) {
+                    const uint32_t q = __ffs(kb) - 1;
+                    S.pool[nb + pos++] = ((u64)(rs + lane * 8 + q) << 32) | T[(uint64_t)v * kWB + lane * 8 + q];
+                }
+                if (lane == 0) S.len[v] = len + total;
+            }
         }
-        if (lane == 0) S.len[v] = len + total;
+        // This lane's 8 entries were read above; the row goes back to INF.
+        uint4 *row = reinterpret_cast<uint4 *>(T + (uint64_t)v * kWB + lane * 8);
+        row[0] = row[1] = make_uint4(kWInf, kWInf, kWInf, kWInf);
+        if (lane < kWW) P[(uint64_t)v * kWW + lane] = 0ull;
     }
-}
-
-// reset_tables (WPARA:918-956) for touched rows and the hub table.
-__global__ __launch_bounds__(256) void k_w_batch_end(const uint32_t *tlist, uint32_t *T, u64 *P, uint32_t *hslot,
-                                                     uint32_t *hmin, uint32_t *dcw, const uint32_t *hlist,
-                                                     const WCounters *C)
-{
-    const uint32_t nt = C->n_touched, nh = C->n_hub;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t tot = (uint64_t)nt * (kWB / 4);
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += stride) {
-        const uint32_t v = tlist[i / (kWB / 4)], part = (uint32_t)(i % (kWB / 4));
-        reinterpret_cast<uint4 *>(T + (uint64_t)v * kWB)[part] = make_uint4(kWInf, kWInf, kWInf, kWInf);
-        if (part < kWW) P[(uint64_t)v * kWW + part] = 0ull;
-    }
-    // The hub table back to empty: slots, minima and the used rows.
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nh; i += stride) {
-        hslot[hlist[i]] = kEmpty;
-        hmin[hlist[i]] = kWInf;
-    }
-    const uint64_t rows = (uint64_t)nh * (kWB / 4);
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += stride)
-        reinterpret_cast<uint4 *>(dcw)[i] = make_uint4(kWInf, kWInf, kWInf, kWInf);
 }
 
 __global__ void k_w_fill(uint32_t *p, uint64_t count, uint32_t val)
@@ -846,7 +819,7 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
     hipStream_t st = g->ctx->stream;
     const uint32_t n = g->n;
     const uint32_t batch = (batch_req == 0 || batch_req > kWB) ? kWB : batch_req;
-    DevBuf<uint32_t> T, tmark, tlist, ilist, alist, amask, aend, hslot, hmin, hlist, dcw, len, cap;
+    DevBuf<uint32_t> T, tmark, tlist, ilist, alist, amask, aend, hmin, dcw, len, cap;
     DevBuf<u64> P, I, ag, chunks, pool, base, imark;
     DevBuf<uint32_t> RT;  // batch roots' entries, transposed for the filter
     DevBuf<u64> PT;
@@ -870,9 +843,7 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
     PADO_TRY(alist.alloc(alist_cap));
     PADO_TRY(amask.alloc(alist_cap));
     PADO_TRY(aend.alloc(alist_cap));
-    PADO_TRY(hslot.alloc(n));
     PADO_TRY(hmin.alloc(n));
-    PADO_TRY(hlist.alloc(n));
     PADO_TRY(len.alloc(n));
     PADO_TRY(cap.alloc(n));
     PADO_TRY(base.alloc(n));
@@ -888,7 +859,6 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
     hipLaunchKernelGGL(k_w_zero64, dim3(fg), dim3(256), 0, st, I.p, (uint64_t)n * kWW);
     hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, tmark.p, (uint64_t)n, 0u);
     hipLaunchKernelGGL(k_w_zero64, dim3(fg), dim3(256), 0, st, imark.p, (uint64_t)n);
-    hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, hslot.p, (uint64_t)n, kEmpty);
     hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, hmin.p, (uint64_t)n, kWInf);
     hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, len.p, (uint64_t)n, 0u);
     hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, cap.p, (uint64_t)n, 0u);
@@ -921,7 +891,6 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
         hipLaunchKernelGGL(k_w_reset_counters, dim3(1), dim3(1), 0, st, ctr.p, rsz);
         hipLaunchKernelGGL(k_w_init, dim3(rb), dim3(256), 0, st, rs, rsz, gen_batch, g->off.p, g->adj.p, T.p, tmark.p,
                            tlist.p, alist.p, ag.p, amask.p, aend.p, chunks.p, S, ctr.p);
-        hipLaunchKernelGGL(k_w_hub_mark, dim3(rb), dim3(256), 0, st, rs, rsz, S, hslot.p, hlist.p, ctr.p);
         // The hubs of the batch roots' labels are ids below the batch end, so
         // that many table rows always suffice (no device round trip).
         if (rs + rsz > dc_slots) {
@@ -929,7 +898,7 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
             PADO_TRY(dcw.alloc((uint64_t)dc_slots * kWB));
             hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, dcw.p, (uint64_t)dc_slots * kWB, kWInf);
         }
-        hipLaunchKernelGGL(k_w_hub_fill, dim3(rb), dim3(256), 0, st, rs, rsz, S, hslot.p, hmin.p, dcw.p);
+        hipLaunchKernelGGL(k_w_hub_fill, dim3(rb), dim3(256), 0, st, rs, rsz, S, hmin.p, dcw.p);
         // Rounds per host check: the previous batch's round count + 1 first
         // (enough when this batch takes no more), then blocks of 2^j that
         // double while a batch runs long.
@@ -952,7 +921,7 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
                                    g->adj.p, g->w.p, alist.p, ag.p, amask.p, aend.p, chunks.p, T.p, I.p, tmark.p,
                                    tlist.p, imark.p, ilist.p, S, ctr.p);
                 hipLaunchKernelGGL(k_w_check, dim3(kAddGrid), dim3(256), 0, st, round, rs, ilist.p, g->off.p, g->adj.p,
-                                   T.p, I.p, P.p, S, hslot.p, hmin.p, dcw.p, alist.p, ag.p, amask.p, aend.p,
+                                   T.p, I.p, P.p, S, hmin.p, dcw.p, alist.p, ag.p, amask.p, aend.p,
                                    chunks.p, ctr.p);
             }
             launched += blk;
@@ -1015,9 +984,8 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
             const u64 top = no[n];
             PADO_HIP(copy_sync(&ctr.p->pool_top, &top, 8, hipMemcpyHostToDevice, st));
         }
-        hipLaunchKernelGGL(k_w_commit, dim3(2048), dim3(256), 0, st, rs, tlist.p, T.p, I.p, S, ctr.p);
-        hipLaunchKernelGGL(k_w_batch_end, dim3(2048), dim3(256), 0, st, tlist.p, T.p, P.p, hslot.p, hmin.p, dcw.p,
-                           hlist.p, ctr.p);
+        hipLaunchKernelGGL(k_w_commit, dim3(2048), dim3(256), 0, st, rs, rsz, tlist.p, T.p, P.p, I.p, hmin.p, dcw.p,
+                           S, ctr.p);
     }
     PADO_HIP(hipMemcpyAsync(h, ctr.p, sizeof(WCounters), hipMemcpyDeviceToHost, st));
     PADO_HIP(hipStreamSynchronize(st));
