@@ -142,12 +142,35 @@ __global__ void k_w_reset_counters(WCounters *C, uint32_t rsz)
     C->need = C->need_bound = 0;
 }
 
+// Earlier-batch labels of the batch roots -> root-distance table
+// (roots_labels_buffer, WPARA:322-348, transposed): dcw[h][ro] = d(root
+// rs + ro, hub h), indexed by hub id, and hmin[h] = the least of them.  Both
+// are INF outside the batch roots' hubs (k_w_commit resets what was written).
+__device__ inline void w_hub_fill(uint32_t ro, uint32_t r, WStore S, uint32_t *hmin, uint32_t *dcw)
+{
+    const uint32_t lane = lane_id();
+    const uint32_t len = S.len[r];
+    const u64 base = S.base[r];
+    // Lists are hub-sorted and every root holds the top-ranked hubs: walking
+    // them from the end staggers the roots' arrival at those hubs, so the
+    // plain read of hmin skips most atomics.
+    for (uint32_t j0 = (len + 63) & ~63u; j0 > 0; j0 -= 64) {
+        const uint32_t j = j0 - 64 + lane;
+        if (j >= len) continue;
+        const u64 lab = S.pool[base + j];
+        const uint32_t h = (uint32_t)(lab >> 32), d = (uint32_t)lab;
+        dcw[(uint64_t)h * kWB + ro] = d;
+        if (d < hmin[h]) atomicMin(&hmin[h], d);
+    }
+}
+
 // initialize_tables (WPARA:305-383): every root at distance 0 from itself,
-// active with its own bit.  One wave per root, one chunk reservation per block.
+// active with its own bit, and its earlier labels in the root-distance table.
+// One wave per root, one chunk reservation per block.
 __global__ __launch_bounds__(256) void k_w_init(uint32_t rs, uint32_t rsz, uint32_t gen_batch, const uint64_t *off,
                                                 const uint32_t *adj, uint32_t *T, uint32_t *tmark, uint32_t *tlist,
                                                 uint32_t *alist, u64 *ag, uint32_t *amask, uint32_t *aend, u64 *chunks,
-                                                WStore S, WCounters *C)
+                                                uint32_t *hmin, uint32_t *dcw, WStore S, WCounters *C)
 {
     __shared__ uint32_t s_nch[4], s_c0;
     const uint32_t lane = lane_id(), wv = threadIdx.x >> 6;
@@ -179,32 +202,9 @@ __global__ __launch_bounds__(256) void k_w_init(uint32_t rs, uint32_t rsz, uint3
     uint32_t c0 = s_c0;
     for (uint32_t w = 0; w < wv; ++w) c0 += s_nch[w];
     for (uint32_t q = lane; q < nch; q += 64) chunks[c0 + q] = ((u64)(q * kChunk) << 32) | ro;
+    if (ro < rsz) w_hub_fill(ro, rs + ro, S, hmin, dcw);
 }
 
-// Earlier-batch labels of the batch roots -> root-distance table
-// (roots_labels_buffer, WPARA:322-348, transposed): dcw[h][ro] = d(root
-// rs + ro, hub h), indexed by hub id, and hmin[h] = the least of them.  Both
-// are INF outside the batch roots' hubs (k_w_commit resets what was written).
-__global__ __launch_bounds__(256) void k_w_hub_fill(uint32_t rs, uint32_t rsz, WStore S, uint32_t *hmin, uint32_t *dcw)
-{
-    const uint32_t lane = lane_id();
-    const uint32_t ro = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    if (ro >= rsz) return;
-    const uint32_t r = rs + ro;
-    const uint32_t len = S.len[r];
-    const u64 base = S.base[r];
-    // Lists are hub-sorted and every root holds the top-ranked hubs: walking
-    // them from the end staggers the roots' arrival at those hubs, so the
-    // plain read of hmin skips most atomics.
-    for (uint32_t j0 = (len + 63) & ~63u; j0 > 0; j0 -= 64) {
-        const uint32_t j = j0 - 64 + lane;
-        if (j >= len) continue;
-        const u64 lab = S.pool[base + j];
-        const uint32_t h = (uint32_t)(lab >> 32), d = (uint32_t)lab;
-        dcw[(uint64_t)h * kWB + ro] = d;
-        if (d < hmin[h]) atomicMin(&hmin[h], d);
-    }
-}
 
 // Position of the rank-th set bit of m (rank < popcount(m)).
 __device__ inline uint32_t w_nth_set_bit(u64 m, uint32_t rank)
@@ -740,7 +740,7 @@ __global__ __launch_bounds__(256) void k_w_commit(uint32_t rs, uint32_t rsz, con
     for (uint32_t i = wave; i < nt; i += nwaves) {
         const uint32_t v = tlist[i];
         if (i < rsz) {
-            // root offset i: its labels are those k_w_hub_fill entered
+            // root offset i: its labels are those k_w_init entered
             const uint32_t len = S.len[v];
             const u64 base = S.base[v];
             for (uint32_t j = lane; j < len; j += 64) {
@@ -889,8 +889,6 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
         rs_next = rs + rsz;
         const uint32_t rb = (rsz * 64 + 255) / 256;
         hipLaunchKernelGGL(k_w_reset_counters, dim3(1), dim3(1), 0, st, ctr.p, rsz);
-        hipLaunchKernelGGL(k_w_init, dim3(rb), dim3(256), 0, st, rs, rsz, gen_batch, g->off.p, g->adj.p, T.p, tmark.p,
-                           tlist.p, alist.p, ag.p, amask.p, aend.p, chunks.p, S, ctr.p);
         // The hubs of the batch roots' labels are ids below the batch end, so
         // that many table rows always suffice (no device round trip).
         if (rs + rsz > dc_slots) {
@@ -898,7 +896,8 @@ pado_gpu_status build_weighted(pado_gpu_graph *g, pado_gpu_index *idx, uint32_t 
             PADO_TRY(dcw.alloc((uint64_t)dc_slots * kWB));
             hipLaunchKernelGGL(k_w_fill, dim3(fg), dim3(256), 0, st, dcw.p, (uint64_t)dc_slots * kWB, kWInf);
         }
-        hipLaunchKernelGGL(k_w_hub_fill, dim3(rb), dim3(256), 0, st, rs, rsz, S, hmin.p, dcw.p);
+        hipLaunchKernelGGL(k_w_init, dim3(rb), dim3(256), 0, st, rs, rsz, gen_batch, g->off.p, g->adj.p, T.p, tmark.p,
+                           tlist.p, alist.p, ag.p, amask.p, aend.p, chunks.p, hmin.p, dcw.p, S, ctr.p);
         // Rounds per host check: the previous batch's round count + 1 first
         // (enough when this batch takes no more), then blocks of 2^j that
         // double while a batch runs long.
