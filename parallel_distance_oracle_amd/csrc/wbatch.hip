@@ -47,7 +47,7 @@ using u64 = unsigned long long;
 // Frontier / improved-tail counts of round i live in slot i & 1: each round's
 // kernels read their slot and empty the other one for the next kernel.
 struct WCounters {
-    uint32_t n_act[2], n_chunk[2], n_imp[2], n_touched, n_hub, overflow, pad1;
+    uint32_t n_act[2], n_chunk[2], n_imp[2], n_touched, overflow;
     uint32_t a_top[2];  // active-list slots handed out (>= n_act: per-wave reservations)
     u64 pool_top, relax, improved, pruned, labels, filtered, rounds, need;
     // Pool words the batch's commit can reserve at most: per touched vertex
@@ -138,7 +138,6 @@ __global__ void k_w_reset_counters(WCounters *C, uint32_t rsz)
     C->a_top[0] = 0;
     C->n_act[1] = C->a_top[1] = rsz;
     C->n_touched = rsz;
-    C->n_hub = 0;
     C->need = C->need_bound = 0;
 }
 
