@@ -417,8 +417,9 @@ __global__ __launch_bounds__(256) void k_w_check(uint32_t round, uint32_t rs, co
         const u64 base = uniform64(S.base[t]);
         // Row phase: while more than kWGather improved pairs are unpruned,
         // lanes hold roots (8 each) and read the passing hubs' row slices.
-        // The first hubs usually prune most pairs; the labels after the step
-        // where at most kWGather remain go to the gather phase.
+        // The first hubs usually prune most pairs; once a 4-hub group leaves
+        // at most kWGather unpruned, the labels not yet read go to the gather
+        // phase.
         uint32_t j_gather = 0;
         if (wave_sum((u64)__popc(imp)) > kWGather) {
             j_gather = len;
